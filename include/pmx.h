@@ -1,0 +1,244 @@
+/*
+ * pmx — C ABI of the MI355X-native solve path for ParMooN-style
+ * convection-diffusion problems (assembly into the reference CSR pattern,
+ * FE-matrix operator, Krylov / geometric-multigrid solvers).
+ *
+ * Conventions
+ *  - every function returns PMX_OK (0) or a PMX_E_* code; pmx_last_error()
+ *    gives the message of the last failure of the calling thread;
+ *  - "dev" pointers are device (HBM) addresses, "host" pointers host memory;
+ *    sizes are element counts;
+ *  - all device work of a context runs on the context's HIP stream and is
+ *    asynchronous unless stated otherwise; calls that return host results
+ *    synchronise that stream;
+ *  - objects are not thread-safe; use one pmx_ctx per host thread.
+ *
+ * Each entry point names the reference interface (nahmedwias/MyMooNMD) it
+ * replaces.
+ */
+#ifndef PMX_H
+#define PMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ errors */
+#define PMX_OK 0
+#define PMX_E_ARG 1         /* invalid argument */
+#define PMX_E_HIP 2         /* HIP runtime error */
+#define PMX_E_ZERO_DIAG 3   /* zero diagonal entry (Iteration_jacobi.C:8-15) */
+#define PMX_E_DIVERGED 4    /* IterativeMethod::converged divergence (IterativeMethod.h:105-131) */
+#define PMX_E_BREAKDOWN 5   /* FGMRES unhappy breakdown (Iteration_gmres.C:470-477) */
+#define PMX_E_STATE 6       /* object used in a wrong state */
+#define PMX_E_NOMEM 7       /* allocation failed */
+#define PMX_E_INTERNAL 8
+
+/* Message of the last error on the calling thread (never NULL). */
+const char* pmx_last_error(void);
+/* ABI version (major*100 + minor). */
+int pmx_version(void);
+
+/* ----------------------------------------------------------------- context */
+typedef struct pmx_ctx pmx_ctx;
+/* Binds HIP device `device` and creates a stream. */
+int pmx_ctx_create(int device, pmx_ctx** out);
+int pmx_ctx_destroy(pmx_ctx* ctx);
+int pmx_ctx_synchronize(pmx_ctx* ctx);
+
+/* ------------------------------------------------------------ domain/mesh */
+/* Replaces TDomain(db) + TDomain::Init(PRM, GEO) + refine_and_get_hierarchy_
+ * of_collections (src/Geometry/Domain.C:270-461, 3065-3173) for the default
+ * geometries; level k holds the collection after k regular refinements in
+ * It_Finest order. */
+#define PMX_GEO_UNIT_SQUARE_QUAD 0   /* "UnitSquare" */
+#define PMX_GEO_UNIT_SQUARE_TRI 1    /* "TwoTriangles" */
+#define PMX_GEO_UNIT_CUBE_HEXA 2     /* "Default_UnitCube_Hexa" */
+typedef struct pmx_domain pmx_domain;
+int pmx_domain_create(int geometry, int n_refine, pmx_domain** out);
+int pmx_domain_destroy(pmx_domain* d);
+/* number of cells of a level; vertex coordinates [n_cells][nv][dim] */
+int pmx_domain_level_info(const pmx_domain* d, int level, int* n_cells, int* n_vertices_per_cell,
+                          int* dim);
+int pmx_domain_get_vertices(const pmx_domain* d, int level, double* host_xyz);
+
+/* ------------------------------------------------------------------ spaces */
+/* Boundary-condition callbacks, replacing BoundCondFunct2D / BoundCondFunct3D
+ * (include/General/Constants.h); return 0 DIRICHLET, 1 NEUMANN, ... */
+typedef int (*pmx_bc2d_fn)(int component, double t, void* user);
+typedef int (*pmx_bc3d_fn)(double x, double y, double z, void* user);
+/* Built-in boundary conditions (bc callbacks may then be NULL) */
+#define PMX_BC_ALL_DIRICHLET 0
+#define PMX_BC_SINE_LAPLACE 1        /* component 1 Neumann, others Dirichlet */
+
+typedef struct pmx_space pmx_space;
+/* Replaces TFESpace2D/TFESpace3D(collection, ..., BoundaryCondition, order)
+ * + ConstructSpace (src/FE/FESpace3D.C:58-229, 506-2080).  order 1 or 2
+ * (Q1/Q2 on quads and hexahedra, P1 on triangles).  When bc2d/bc3d are NULL
+ * the built-in `builtin_bc` is used.  ctx may be NULL for a host-only space
+ * (numbering and pattern queries, no assembly). */
+int pmx_space_create(pmx_ctx* ctx, const pmx_domain* d, int level, int order, pmx_bc2d_fn bc2d,
+                     pmx_bc3d_fn bc3d, void* user, int builtin_bc, pmx_space** out);
+int pmx_space_destroy(pmx_space* s);
+typedef struct {
+  int n_cells, ndof_cell, n_dof, active_bound, n_inner, n_dirichlet;
+  int n_bound[9];   /* NEUMANN, ROBIN, SLIP, ... DIRICHLET_WEAK */
+  int dim;
+} pmx_space_info;
+int pmx_space_get_info(const pmx_space* s, pmx_space_info* out);
+/* GlobalNumbers [n_cells][ndof_cell] (TFESpace::GetGlobalNumbers) */
+int pmx_space_get_global_numbers(const pmx_space* s, int* host_out);
+/* nodal points of all DOFs [n_dof][3] */
+int pmx_space_get_dof_coords(const pmx_space* s, double* host_out);
+/* TStructure(FESpace) pattern on the host: *nnz always set; row_ptr [n_dof+1]
+ * and col [nnz] filled when not NULL.  Works without a device (ctx NULL). */
+int pmx_space_get_structure(const pmx_space* s, int* nnz, int* host_row_ptr, int* host_col);
+/* Grid-transfer matrix P (fine x coarse) between consecutive levels:
+ * prolongation = P*u (GridTransfer::Prolongate, src/Multigrid/GridTransfer.C:665-941),
+ * defect restriction = P^T*d (GridTransfer::DefectRestriction, :943-1230). */
+int pmx_space_get_prolongation(const pmx_space* coarse, const pmx_space* fine, int* nnz,
+                               int* host_row_ptr, int* host_col, double* host_val);
+
+/* ---------------------------------------------------------------- matrices */
+/* Square FE matrix on device: TStructure(FESpace) pattern
+ * (src/Matrix/Structure.C:829-1277) + FEMatrix values. */
+typedef struct pmx_matrix pmx_matrix;
+int pmx_matrix_create(pmx_ctx* ctx, const pmx_space* s, pmx_matrix** out);
+int pmx_matrix_destroy(pmx_matrix* A);
+int pmx_matrix_get_info(const pmx_matrix* A, int* n_rows, int* nnz, int* active_bound);
+int pmx_matrix_get_structure(const pmx_matrix* A, int* host_row_ptr, int* host_col);
+int pmx_matrix_get_entries(pmx_ctx* ctx, const pmx_matrix* A, double* host_values);
+int pmx_matrix_set_entries(pmx_ctx* ctx, pmx_matrix* A, const double* host_values);
+/* device address of the value array (nnz doubles) */
+int pmx_matrix_entries_device(pmx_matrix* A, double** dev_values);
+/* TMatrix::reset: all entries to zero */
+int pmx_matrix_reset(pmx_ctx* ctx, pmx_matrix* A);
+/* Lanes per row of the SpMV kernels (4, 8, 16, 32, 64); 0 = automatic */
+int pmx_matrix_set_spmv_lanes(pmx_matrix* A, int lanes);
+
+/* BlockFEMatrix::apply_scaled_add (src/Matrix/BlockFEMatrix.C:863-924):
+ * y += a*A*x on active rows, y += a*x on the non-active (Dirichlet) rows. */
+int pmx_matrix_apply_scaled_add(pmx_ctx* ctx, const pmx_matrix* A, const double* dev_x,
+                                double* dev_y, double a);
+/* BlockFEMatrix::apply (src/Matrix/BlockFEMatrix.C:855-860): y = A*x with
+ * identity on non-active rows. */
+int pmx_matrix_apply(pmx_ctx* ctx, const pmx_matrix* A, const double* dev_x, double* dev_y);
+
+/* ---------------------------------------------------------------- assembly */
+#define PMX_DISC_GALERKIN 0
+#define PMX_DISC_SUPG 1
+/* built-in examples (coefficients + exact solution + boundary values) */
+#define PMX_EX_CONST 0
+#define PMX_EX_SINE_LAPLACE_2D 1   /* include/Examples/CD_2D/SineLaplace.h */
+#define PMX_EX_P0_3D 2             /* include/Examples/CD_3D/test_p0.h */
+#define PMX_EX_P1_3D 3             /* include/Examples/CD_3D/test_p1.h */
+#define PMX_EX_P2_3D 4             /* include/Examples/CD_3D/test_p2.h */
+#define PMX_EX_EXP_SIN_3D 5        /* u = exp(-x) sin(pi y) sin(pi z) */
+#define PMX_EX_SINE_2D 6           /* u = sin(pi x) sin(pi y), Dirichlet u */
+typedef struct {
+  int disc;            /* space_discretization_type: galerkin / supg */
+  int example;         /* PMX_EX_* : right-hand side from the exact solution */
+  int hexa_quad_rule;  /* 0: degree table (Q2 hexa -> Gauss4), 2: Gauss3 (INTERNAL_QUAD_RULE=2) */
+  double eps;          /* diffusion coefficient (coeff[0]) */
+  double b[3];         /* convection (coeff[1..d]) */
+  double c;            /* reaction */
+  double f0;           /* constant rhs for PMX_EX_CONST */
+  double delta0;       /* DELTA0 (default 1) */
+  double poly_degree;  /* INTERNAL_POLYNOMIAL_DEGREE used by SDFEM delta (reference default 1) */
+  const double* dev_coeff_table; /* optional [n_cells][n_quad][8] coefficients (eps,b1,b2,b3,c,f,-,-)
+                                    replacing the CoeffFct callback; NULL: constants + example */
+  const double* host_dirichlet;  /* optional Dirichlet values [n_dirichlet]; NULL: example values */
+} pmx_cd_params;
+/* number of quadrature points per cell the assembly uses for this space */
+int pmx_assembly_n_quad(const pmx_space* s, int hexa_quad_rule, int* n_quad);
+/* physical quadrature points [n_cells][n_quad][3] (to evaluate user CoeffFct on host) */
+int pmx_assembly_quad_points(const pmx_space* s, int hexa_quad_rule, double* host_xyz);
+
+/* Assemble3D / Assemble2D with LocalAssembling<d>(ConvDiff)
+ * (src/AssembleRoutines/Assemble3D.C:37-1132, Assemble2D.C:6940-7620):
+ * ADDS the Galerkin (+SUPG) matrix into A and the load vector into dev_rhs,
+ * sets Dirichlet diagonals to 1 and Dirichlet rhs entries to the boundary
+ * values, and adds Neumann boundary integrals.  As in the reference the
+ * caller zeroes A and rhs first (src/System/ConvectionDiffusion.C:329-340). */
+int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s, const pmx_cd_params* p, pmx_matrix* A,
+                    double* dev_rhs);
+
+/* TFEFunction::GetErrors with conv_diff_l2_h1_linf_error
+ * (src/AssembleRoutines/ConvDiff.C:2134-2185): errors[0..3] = L2, H1-semi,
+ * SD, L_inf (at quadrature points) against the example's exact solution. */
+int pmx_compute_errors(pmx_ctx* ctx, const pmx_space* s, const pmx_cd_params* p,
+                       const double* dev_u, double* host_errors4);
+
+/* ---------------------------------------------------------- vector (BLAS-1) */
+/* BlockVector::dot / norm (src/Matrix/BlockVector.C:293-386, 491-530) */
+int pmx_vector_dot(pmx_ctx* ctx, int n, const double* dev_x, const double* dev_y, double* host_out);
+int pmx_vector_norm(pmx_ctx* ctx, int n, const double* dev_x, double* host_out);
+/* BlockVector::add_scaled: y += a*x */
+int pmx_vector_add_scaled(pmx_ctx* ctx, int n, double a, const double* dev_x, double* dev_y);
+
+/* ------------------------------------------------------------------ solver */
+#define PMX_IT_FGMRES 0
+#define PMX_IT_CG 1
+#define PMX_IT_RICHARDSON 2
+#define PMX_IT_JACOBI 3
+#define PMX_PC_NONE 0
+#define PMX_PC_JACOBI 1
+#define PMX_PC_MULTIGRID 2
+#define PMX_SMOOTHER_JACOBI 0
+#define PMX_SMOOTHER_DIRECT 1
+#define PMX_CYCLE_V 0
+#define PMX_CYCLE_W 1
+#define PMX_CYCLE_F 2
+#define PMX_PRECISION_FP64 0
+#define PMX_PRECISION_MIXED 1   /* FP32 V-cycle on bf16 (row-scaled) matrices */
+typedef struct {
+  int iterative_solver_type;   /* iterative_solver_type */
+  int preconditioner;          /* preconditioner */
+  int max_n_iterations;        /* default 100 */
+  int min_n_iterations;        /* default 0 */
+  double residual_tolerance;   /* default 1e-8 */
+  double residual_reduction;   /* default 0 */
+  double divergence_factor;    /* default 1e10 */
+  int gmres_restart;           /* default 20 */
+  double damping;              /* Richardson damping (default 1) */
+  int mg_cycle;                /* multigrid_cycle_type */
+  int mg_smoother;             /* multigrid_smoother */
+  int mg_smoother_coarse;      /* multigrid_smoother_coarse */
+  double mg_correction_damp;   /* multigrid_correction_damp_factor */
+  int mg_n_pre_smooth, mg_n_post_smooth;
+  double mg_coarse_residual;   /* multigrid_coarse_residual */
+  int mg_coarse_max_n_iterations;
+  double jacobi_damp;          /* jacobi_damp */
+  int precision;               /* PMX_PRECISION_* (multigrid preconditioner only) */
+} pmx_solver_params;
+/* Defaults of Solver<>::default_solver_database (src/Solver/Solver.C:20-117)
+ * merged with Multigrid::default_multigrid_database (src/Multigrid/Multigrid.C:422-514). */
+int pmx_solver_default_params(pmx_solver_params* p);
+
+typedef struct pmx_solver pmx_solver;
+int pmx_solver_create(pmx_ctx* ctx, const pmx_solver_params* p, pmx_solver** out);
+int pmx_solver_destroy(pmx_solver* s);
+/* Solver::update_matrix (src/Solver/Solver.C:260-326) + Multigrid::initialize/update:
+ * `levels` are the matrices of all multigrid levels, COARSEST FIRST (the
+ * order of Multigrid::initialize); `spaces` their spaces (consecutive mesh
+ * levels).  Without multigrid pass n_levels = 1. */
+int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space* const* spaces,
+                      int n_levels);
+typedef struct {
+  int iterations;
+  double initial_residual;
+  double final_residual;
+  int converged;
+} pmx_solve_report;
+/* Solver::solve(rhs, solution) (src/Solver/Solver.C:414-488): iterates from
+ * the given dev_x.  Blocks until done. */
+int pmx_solver_solve(pmx_solver* s, const double* dev_b, double* dev_x, pmx_solve_report* rep);
+/* residual history of the last solve (residuals as printed by LoopInfo) */
+int pmx_solver_get_history(const pmx_solver* s, double* host_out, int max, int* n);
+/* One multigrid cycle as preconditioner: dev_z = MG(dev_r) (Iteration_multigrid::apply) */
+int pmx_solver_apply_preconditioner(pmx_solver* s, const double* dev_r, double* dev_z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMX_H */

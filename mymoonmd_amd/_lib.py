@@ -1,0 +1,131 @@
+"""ctypes binding of libpmx.so (the C ABI declared in include/pmx.h).
+
+The library is the product: this module only loads it and declares the
+signatures.  Loading fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpmx.so")
+
+c_int = ctypes.c_int
+c_double = ctypes.c_double
+c_void_p = ctypes.c_void_p
+P = ctypes.POINTER
+
+PMX_OK = 0
+ERRORS = {1: "PMX_E_ARG", 2: "PMX_E_HIP", 3: "PMX_E_ZERO_DIAG", 4: "PMX_E_DIVERGED",
+          5: "PMX_E_BREAKDOWN", 6: "PMX_E_STATE", 7: "PMX_E_NOMEM", 8: "PMX_E_INTERNAL"}
+
+BC2D_FN = ctypes.CFUNCTYPE(c_int, c_int, c_double, c_void_p)
+BC3D_FN = ctypes.CFUNCTYPE(c_int, c_double, c_double, c_double, c_void_p)
+
+
+class SpaceInfo(ctypes.Structure):
+    _fields_ = [("n_cells", c_int), ("ndof_cell", c_int), ("n_dof", c_int),
+                ("active_bound", c_int), ("n_inner", c_int), ("n_dirichlet", c_int),
+                ("n_bound", c_int * 9), ("dim", c_int)]
+
+
+class CDParams(ctypes.Structure):
+    _fields_ = [("disc", c_int), ("example", c_int), ("hexa_quad_rule", c_int),
+                ("eps", c_double), ("b", c_double * 3), ("c", c_double), ("f0", c_double),
+                ("delta0", c_double), ("poly_degree", c_double),
+                ("dev_coeff_table", c_void_p), ("host_dirichlet", c_void_p)]
+
+
+class SolverParams(ctypes.Structure):
+    _fields_ = [("iterative_solver_type", c_int), ("preconditioner", c_int),
+                ("max_n_iterations", c_int), ("min_n_iterations", c_int),
+                ("residual_tolerance", c_double), ("residual_reduction", c_double),
+                ("divergence_factor", c_double), ("gmres_restart", c_int),
+                ("damping", c_double), ("mg_cycle", c_int), ("mg_smoother", c_int),
+                ("mg_smoother_coarse", c_int), ("mg_correction_damp", c_double),
+                ("mg_n_pre_smooth", c_int), ("mg_n_post_smooth", c_int),
+                ("mg_coarse_residual", c_double), ("mg_coarse_max_n_iterations", c_int),
+                ("jacobi_damp", c_double), ("precision", c_int)]
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [("iterations", c_int), ("initial_residual", c_double),
+                ("final_residual", c_double), ("converged", c_int)]
+
+
+# name -> argtypes (all functions return int unless listed in _RESTYPES)
+SIGNATURES = {
+    "pmx_last_error": [],
+    "pmx_version": [],
+    "pmx_ctx_create": [c_int, P(c_void_p)],
+    "pmx_ctx_destroy": [c_void_p],
+    "pmx_ctx_synchronize": [c_void_p],
+    "pmx_domain_create": [c_int, c_int, P(c_void_p)],
+    "pmx_domain_destroy": [c_void_p],
+    "pmx_domain_level_info": [c_void_p, c_int, P(c_int), P(c_int), P(c_int)],
+    "pmx_domain_get_vertices": [c_void_p, c_int, P(c_double)],
+    "pmx_space_create": [c_void_p, c_void_p, c_int, c_int, BC2D_FN, BC3D_FN, c_void_p, c_int,
+                         P(c_void_p)],
+    "pmx_space_destroy": [c_void_p],
+    "pmx_space_get_info": [c_void_p, P(SpaceInfo)],
+    "pmx_space_get_global_numbers": [c_void_p, P(c_int)],
+    "pmx_space_get_dof_coords": [c_void_p, P(c_double)],
+    "pmx_space_get_structure": [c_void_p, P(c_int), P(c_int), P(c_int)],
+    "pmx_space_get_prolongation": [c_void_p, c_void_p, P(c_int), P(c_int), P(c_int), P(c_double)],
+    "pmx_matrix_create": [c_void_p, c_void_p, P(c_void_p)],
+    "pmx_matrix_destroy": [c_void_p],
+    "pmx_matrix_get_info": [c_void_p, P(c_int), P(c_int), P(c_int)],
+    "pmx_matrix_get_structure": [c_void_p, P(c_int), P(c_int)],
+    "pmx_matrix_get_entries": [c_void_p, c_void_p, P(c_double)],
+    "pmx_matrix_set_entries": [c_void_p, c_void_p, P(c_double)],
+    "pmx_matrix_entries_device": [c_void_p, P(c_void_p)],
+    "pmx_matrix_reset": [c_void_p, c_void_p],
+    "pmx_matrix_set_spmv_lanes": [c_void_p, c_int],
+    "pmx_matrix_apply_scaled_add": [c_void_p, c_void_p, c_void_p, c_void_p, c_double],
+    "pmx_matrix_apply": [c_void_p, c_void_p, c_void_p, c_void_p],
+    "pmx_assembly_n_quad": [c_void_p, c_int, P(c_int)],
+    "pmx_assembly_quad_points": [c_void_p, c_int, P(c_double)],
+    "pmx_assemble_cd": [c_void_p, c_void_p, P(CDParams), c_void_p, c_void_p],
+    "pmx_compute_errors": [c_void_p, c_void_p, P(CDParams), c_void_p, P(c_double)],
+    "pmx_vector_dot": [c_void_p, c_int, c_void_p, c_void_p, P(c_double)],
+    "pmx_vector_norm": [c_void_p, c_int, c_void_p, P(c_double)],
+    "pmx_vector_add_scaled": [c_void_p, c_int, c_double, c_void_p, c_void_p],
+    "pmx_solver_default_params": [P(SolverParams)],
+    "pmx_solver_create": [c_void_p, P(SolverParams), P(c_void_p)],
+    "pmx_solver_destroy": [c_void_p],
+    "pmx_solver_update": [c_void_p, P(c_void_p), P(c_void_p), c_int],
+    "pmx_solver_solve": [c_void_p, c_void_p, c_void_p, P(SolveReport)],
+    "pmx_solver_get_history": [c_void_p, P(c_double), c_int, P(c_int)],
+    "pmx_solver_apply_preconditioner": [c_void_p, c_void_p, c_void_p],
+}
+_RESTYPES = {"pmx_last_error": ctypes.c_char_p}
+
+_lib = None
+
+
+class PmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERRORS.get(code, code), msg))
+        self.code = code
+
+
+def lib():
+    """Load libpmx.so once; raise if it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libpmx.so is not built: run python -m mymoonmd_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, argtypes in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = argtypes
+            f.restype = _RESTYPES.get(name, c_int)
+        _lib = L
+    return _lib
+
+
+def call(name, *args):
+    L = lib()
+    rc = getattr(L, name)(*args)
+    if rc != PMX_OK:
+        raise PmxError(rc, L.pmx_last_error().decode())
+    return rc

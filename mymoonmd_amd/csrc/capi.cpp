@@ -1,0 +1,527 @@
+// C ABI of the solve path (include/pmx.h): host-side set-up objects, device
+// data and the host-driven Krylov / multigrid loops that launch the HIP
+// kernels.  Every numerical operation on vectors and matrices runs on the
+// GPU; the host only keeps scalars (Hessenberg matrix, Givens rotations,
+// stopping decisions), as the reference keeps them in its solver objects.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hip/examples.hpp"
+#include "internal.hpp"
+
+using namespace pmx_internal;
+thread_local std::string pmx_internal::g_last_error = "";
+
+namespace {
+
+int auto_lanes(int nnz, int n) {
+  const double avg = n ? (double)nnz / n : 1.0;
+  if (avg > 40) return 64;
+  if (avg > 20) return 32;
+  if (avg > 10) return 16;
+  if (avg > 5) return 8;
+  return 4;
+}
+
+int builtin_cond2d(int builtin, int comp, double) {
+  if (builtin == PMX_BC_SINE_LAPLACE) return comp == 1 ? pmx::BC_NEUMANN : pmx::BC_DIRICHLET;
+  return pmx::BC_DIRICHLET;
+}
+
+const RefTable& ref_table(pmx_space* s, int hexa_rule) {
+  const int key = (s->fs.el.shape == pmx::SHAPE_HEXA) ? hexa_rule : 0;
+  auto it = s->ref.find(key);
+  if (it != s->ref.end()) return it->second;
+  RefTable t;
+  const pmx::Element& e = s->fs.el;
+  t.rule = pmx::assembly_rule(e, key);
+  t.nq = t.rule.n;
+  const int dim = e.dim, nd = e.ndof, nc = (dim == 3) ? 6 : 3;
+  const int rs = 1 + dim + nd * (1 + dim + nc);
+  t.h_ref.assign((size_t)rs * t.nq, 0.0);
+  for (int q = 0; q < t.nq; ++q) {
+    double* r = &t.h_ref[(size_t)q * rs];
+    r[0] = t.rule.w[q];
+    for (int d = 0; d < dim; ++d) r[1 + d] = t.rule.xi[q * dim + d];
+    e.eval(&t.rule.xi[q * dim], r + 1 + dim, r + 1 + dim + nd, r + 1 + dim + nd * (1 + dim));
+  }
+  t.d_ref = upload(t.h_ref.data(), t.h_ref.size(), s->ctx->stream);
+  return s->ref.emplace(key, std::move(t)).first->second;
+}
+
+pmx::AssemblyData assembly_data(pmx_space* s, const RefTable& t) {
+  pmx::AssemblyData ad;
+  ad.dim = s->fs.el.dim;
+  ad.ndof = s->fs.ndof_cell;
+  ad.nq = t.nq;
+  ad.n_cells = s->fs.n_cells;
+  ad.ref = t.d_ref;
+  ad.geo = s->d_geo;
+  ad.dofs = s->d_dofs;
+  return ad;
+}
+
+pmx::CDCoeffs cd_coeffs(const pmx_cd_params* p) {
+  pmx::CDCoeffs co;
+  co.example = p->example;
+  co.supg = (p->disc == PMX_DISC_SUPG);
+  co.eps = p->eps;
+  for (int d = 0; d < 3; ++d) co.b[d] = p->b[d];
+  co.c = p->c;
+  co.f0 = p->f0;
+  co.delta0 = p->delta0;
+  co.poly_degree = p->poly_degree;
+  co.table = p->dev_coeff_table;
+  return co;
+}
+
+// Neumann boundary integrals of Assemble2D (src/AssembleRoutines/Assemble2D.C:7529-7570):
+// line Gauss rule of degree 2*p, value g(comp, t) * hE * w times the test values.
+void neumann_2d(const pmx_space* s, const pmx_cd_params* p, std::vector<double>& out) {
+  const pmx::FESpace& fs = s->fs;
+  const pmx::Mesh& m = *fs.mesh;
+  out.assign(fs.n_dof, 0.0);
+  if (m.dim != 2 || p->example != PMX_EX_SINE_LAPLACE_2D) return;
+  const int npts = (fs.el.order == 1) ? 2 : 3;   // QFLineFromDegree[2p]
+  double zx[5], zw[5];
+  pmx::gauss_legendre(npts, zx, zw);
+  std::vector<double> v(fs.ndof_cell);
+  for (int c = 0; c < fs.n_cells; ++c) {
+    for (int j = 0; j < m.n_faces; ++j) {
+      const int comp = pmx::joint_boundary_component(m, fs.level, c, j);
+      if (comp < 0) continue;
+      int vv[2];
+      pmx::joint_vertices(m.shape, j, vv);
+      const double* a = m.vertex(fs.level, c, vv[0]);
+      const double* b = m.vertex(fs.level, c, vv[1]);
+      const double t0 = pmx::boundary_param_2d(comp, a[0], a[1]);
+      const double t1 = pmx::boundary_param_2d(comp, b[0], b[1]);
+      const int cond = s->bc2d ? s->bc2d(comp, 0.5 * (t0 + t1), s->user)
+                               : builtin_cond2d(s->builtin_bc, comp, 0.5 * (t0 + t1));
+      if (cond != pmx::BC_NEUMANN) continue;
+      const pmx::AffineMap am = pmx::affine_map(m, fs.level, c);
+      const double hE = std::sqrt((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1])) / 2;
+      for (int l = 0; l < npts; ++l) {
+        const double t = t0 + 0.5 * (t1 - t0) * (zx[l] + 1);
+        double x[2] = {a[0] + 0.5 * (zx[l] + 1) * (b[0] - a[0]), a[1] + 0.5 * (zx[l] + 1) * (b[1] - a[1])};
+        // SineLaplace BoundValue on component 1: eps * du/dx (1, t)
+        double ex[5];
+        pmx::example_exact(p->example, 1.0, t, 0.0, ex);
+        double g = (comp == 1) ? p->eps * ex[1] : 0.0;
+        g *= hE * zw[l];
+        double xi[2];
+        pmx::map_to_ref(am, x, xi);
+        fs.el.eval(xi, v.data(), nullptr, nullptr);
+        for (int k = 0; k < fs.ndof_cell; ++k) {
+          const int dof = fs.global_numbers[(size_t)c * fs.ndof_cell + k];
+          if (dof < fs.active_bound) out[dof] += g * v[k];
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// ========================================================================
+extern "C" {
+
+const char* pmx_last_error(void) { return g_last_error.c_str(); }
+int pmx_version(void) { return 100; }
+
+int pmx_ctx_create(int device, pmx_ctx** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    int n = 0;
+    HIP_CHECK(hipGetDeviceCount(&n));
+    require(device >= 0 && device < n, "device index out of range");
+    HIP_CHECK(hipSetDevice(device));
+    auto c = std::make_unique<pmx_ctx>();
+    c->device = device;
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->partial = dmalloc<double>(4096);
+    c->scal = dmalloc<double>(4096);
+    *out = c.release();
+  });
+}
+
+int pmx_ctx_destroy(pmx_ctx* ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    dfree(ctx->partial);
+    dfree(ctx->scal);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int pmx_ctx_synchronize(pmx_ctx* ctx) {
+  return guarded([&] {
+    require(ctx != nullptr, "ctx is NULL");
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int pmx_domain_create(int geometry, int n_refine, pmx_domain** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    require(geometry >= 0 && geometry <= 2, "unknown geometry");
+    auto d = std::make_unique<pmx_domain>();
+    d->mesh = pmx::build_mesh(geometry, n_refine);
+    *out = d.release();
+  });
+}
+
+int pmx_domain_destroy(pmx_domain* d) {
+  return guarded([&] { delete d; });
+}
+
+int pmx_domain_level_info(const pmx_domain* d, int level, int* n_cells, int* nv, int* dim) {
+  return guarded([&] {
+    require(d != nullptr, "domain is NULL");
+    require(level >= 0 && level < (int)d->mesh.levels.size(), "level out of range");
+    if (n_cells) *n_cells = d->mesh.levels[level].n_cells;
+    if (nv) *nv = d->mesh.nv;
+    if (dim) *dim = d->mesh.dim;
+  });
+}
+
+int pmx_domain_get_vertices(const pmx_domain* d, int level, double* host_xyz) {
+  return guarded([&] {
+    require(d != nullptr && host_xyz != nullptr, "NULL argument");
+    require(level >= 0 && level < (int)d->mesh.levels.size(), "level out of range");
+    const auto& x = d->mesh.levels[level].xyz;
+    std::memcpy(host_xyz, x.data(), x.size() * sizeof(double));
+  });
+}
+
+int pmx_space_create(pmx_ctx* ctx, const pmx_domain* d, int level, int order, pmx_bc2d_fn bc2d,
+                     pmx_bc3d_fn bc3d, void* user, int builtin_bc, pmx_space** out) {
+  return guarded([&] {
+    require(d && out, "NULL argument");
+    require(builtin_bc == PMX_BC_ALL_DIRICHLET || builtin_bc == PMX_BC_SINE_LAPLACE, "unknown builtin bc");
+    auto s = std::make_unique<pmx_space>();
+    s->ctx = ctx;
+    s->dom = d;
+    s->builtin_bc = builtin_bc;
+    s->bc2d = bc2d;
+    s->bc3d = bc3d;
+    s->user = user;
+    pmx::BoundaryConditions bc;
+    pmx_space* sp = s.get();
+    bc.cond2d = [sp](int comp, double t) {
+      return sp->bc2d ? sp->bc2d(comp, t, sp->user) : builtin_cond2d(sp->builtin_bc, comp, t);
+    };
+    bc.cond3d = [sp](double x, double y, double z) {
+      return sp->bc3d ? sp->bc3d(x, y, z, sp->user) : (int)pmx::BC_DIRICHLET;
+    };
+    s->fs = pmx::build_space(d->mesh, level, order, bc);
+    const pmx::FESpace& fs = s->fs;
+    const pmx::Mesh& m = d->mesh;
+    s->pattern = std::make_shared<pmx::CSRPattern>(pmx::build_structure(fs));
+    if (!ctx) { *out = s.release(); return; }
+    // device geometry
+    std::vector<double> geo((size_t)fs.n_cells * pmx::kGeoStride, 0.0);
+    for (int c = 0; c < fs.n_cells; ++c) {
+      const pmx::AffineMap a = pmx::affine_map(m, level, c);
+      double* g = &geo[(size_t)c * pmx::kGeoStride];
+      for (int r = 0; r < 3; ++r) g[r] = (r < m.dim) ? a.x0[r] : 0.0;
+      for (int k = 0; k < m.dim * m.dim; ++k) { g[3 + k] = a.J[k]; g[12 + k] = a.Jinv[k]; }
+      g[21] = a.detJ;
+      g[22] = pmx::cell_diameter(m, level, c);
+    }
+    hipStream_t st = ctx->stream;
+    s->d_geo = upload(geo.data(), geo.size(), st);
+    s->d_dofs = upload(fs.global_numbers.data(), fs.global_numbers.size(), st);
+    std::vector<int> order_cells;
+    pmx::color_cells(fs, order_cells, s->color_ptr);
+    s->d_cells = upload(order_cells.data(), order_cells.size(), st);
+    const int ndir = fs.n_dof - fs.active_bound;
+    s->d_dir_xyz = upload(fs.dof_xyz.data() + (size_t)fs.active_bound * 3, (size_t)ndir * 3, st);
+    *out = s.release();
+  });
+}
+
+int pmx_space_destroy(pmx_space* s) {
+  return guarded([&] { delete s; });
+}
+
+int pmx_space_get_info(const pmx_space* s, pmx_space_info* out) {
+  return guarded([&] {
+    require(s && out, "NULL argument");
+    const pmx::FESpace& fs = s->fs;
+    out->n_cells = fs.n_cells;
+    out->ndof_cell = fs.ndof_cell;
+    out->n_dof = fs.n_dof;
+    out->active_bound = fs.active_bound;
+    out->n_inner = fs.n_inner;
+    out->n_dirichlet = fs.n_dirichlet;
+    for (int k = 0; k < 9; ++k) out->n_bound[k] = fs.n_bound[k];
+    out->dim = fs.el.dim;
+  });
+}
+
+int pmx_space_get_global_numbers(const pmx_space* s, int* host_out) {
+  return guarded([&] {
+    require(s && host_out, "NULL argument");
+    std::memcpy(host_out, s->fs.global_numbers.data(), s->fs.global_numbers.size() * sizeof(int));
+  });
+}
+
+int pmx_space_get_dof_coords(const pmx_space* s, double* host_out) {
+  return guarded([&] {
+    require(s && host_out, "NULL argument");
+    std::memcpy(host_out, s->fs.dof_xyz.data(), s->fs.dof_xyz.size() * sizeof(double));
+  });
+}
+
+int pmx_space_get_structure(const pmx_space* s, int* nnz, int* rp, int* col) {
+  return guarded([&] {
+    require(s && nnz, "NULL argument");
+    *nnz = (int)s->pattern->col.size();
+    if (rp) std::memcpy(rp, s->pattern->row_ptr.data(), s->pattern->row_ptr.size() * sizeof(int));
+    if (col) std::memcpy(col, s->pattern->col.data(), s->pattern->col.size() * sizeof(int));
+  });
+}
+
+int pmx_space_get_prolongation(const pmx_space* coarse, const pmx_space* fine, int* nnz, int* rp,
+                               int* col, double* val) {
+  return guarded([&] {
+    require(coarse && fine && nnz, "NULL argument");
+    require(coarse->dom == fine->dom && fine->fs.level == coarse->fs.level + 1,
+            "spaces must be consecutive levels of one domain");
+    pmx::CSRPattern P;
+    std::vector<double> v;
+    pmx::build_prolongation(coarse->fs, fine->fs, P, v);
+    *nnz = (int)P.col.size();
+    if (rp) std::memcpy(rp, P.row_ptr.data(), P.row_ptr.size() * sizeof(int));
+    if (col) std::memcpy(col, P.col.data(), P.col.size() * sizeof(int));
+    if (val) std::memcpy(val, v.data(), v.size() * sizeof(double));
+  });
+}
+
+int pmx_matrix_create(pmx_ctx* ctx, const pmx_space* s, pmx_matrix** out) {
+  return guarded([&] {
+    require(ctx && s && out, "NULL argument");
+    require(s->ctx == ctx, "space was created without this device context");
+    auto M = std::make_unique<pmx_matrix>();
+    M->ctx = ctx;
+    M->space = s;
+    M->pattern = s->pattern;
+    const pmx::CSRPattern& p = *M->pattern;
+    hipStream_t st = ctx->stream;
+    M->A.n = p.n_rows;
+    M->A.m = p.n_cols;
+    M->A.nnz = (int)p.col.size();
+    M->A.active = p.active_bound;
+    M->A.lpr = auto_lanes(M->A.nnz, M->A.n);
+    M->A.row_ptr = upload(p.row_ptr.data(), p.row_ptr.size(), st);
+    M->A.col = upload(p.col.data(), p.col.size(), st);
+    M->A.val = dmalloc<double>(p.col.size());
+    HIP_CHECK(hipMemsetAsync(M->A.val, 0, p.col.size() * sizeof(double), st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    *out = M.release();
+  });
+}
+
+int pmx_matrix_destroy(pmx_matrix* A) {
+  return guarded([&] { delete A; });
+}
+
+int pmx_matrix_get_info(const pmx_matrix* A, int* n_rows, int* nnz, int* active_bound) {
+  return guarded([&] {
+    require(A != nullptr, "NULL matrix");
+    if (n_rows) *n_rows = A->A.n;
+    if (nnz) *nnz = A->A.nnz;
+    if (active_bound) *active_bound = A->A.active;
+  });
+}
+
+int pmx_matrix_get_structure(const pmx_matrix* A, int* rp, int* col) {
+  return guarded([&] {
+    require(A != nullptr, "NULL matrix");
+    if (rp) std::memcpy(rp, A->pattern->row_ptr.data(), A->pattern->row_ptr.size() * sizeof(int));
+    if (col) std::memcpy(col, A->pattern->col.data(), A->pattern->col.size() * sizeof(int));
+  });
+}
+
+int pmx_matrix_get_entries(pmx_ctx* ctx, const pmx_matrix* A, double* host_values) {
+  return guarded([&] {
+    require(ctx && A && host_values, "NULL argument");
+    HIP_CHECK(hipMemcpyAsync(host_values, A->A.val, (size_t)A->A.nnz * sizeof(double),
+                             hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int pmx_matrix_set_entries(pmx_ctx* ctx, pmx_matrix* A, const double* host_values) {
+  return guarded([&] {
+    require(ctx && A && host_values, "NULL argument");
+    HIP_CHECK(hipMemcpyAsync(A->A.val, host_values, (size_t)A->A.nnz * sizeof(double),
+                             hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int pmx_matrix_entries_device(pmx_matrix* A, double** dev_values) {
+  return guarded([&] {
+    require(A && dev_values, "NULL argument");
+    *dev_values = A->A.val;
+  });
+}
+
+int pmx_matrix_reset(pmx_ctx* ctx, pmx_matrix* A) {
+  return guarded([&] {
+    require(ctx && A, "NULL argument");
+    pmx::vec_fill(A->A.nnz, 0.0, A->A.val, ctx->stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_matrix_set_spmv_lanes(pmx_matrix* A, int lanes) {
+  return guarded([&] {
+    require(A != nullptr, "NULL matrix");
+    require(lanes == 0 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64,
+            "lanes must be 0, 4, 8, 16, 32 or 64");
+    A->A.lpr = lanes ? lanes : auto_lanes(A->A.nnz, A->A.n);
+  });
+}
+
+int pmx_matrix_apply_scaled_add(pmx_ctx* ctx, const pmx_matrix* A, const double* x, double* y,
+                                double a) {
+  return guarded([&] {
+    require(ctx && A && x && y, "NULL argument");
+    pmx::fe_apply_scaled_add(A->A, x, y, a, ctx->stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_matrix_apply(pmx_ctx* ctx, const pmx_matrix* A, const double* x, double* y) {
+  return guarded([&] {
+    require(ctx && A && x && y, "NULL argument");
+    require(x != y, "x and y must not alias");
+    pmx::fe_apply(A->A, x, y, ctx->stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_assembly_n_quad(const pmx_space* s, int hexa_quad_rule, int* n_quad) {
+  return guarded([&] {
+    require(s && n_quad, "NULL argument");
+    *n_quad = pmx::assembly_rule(s->fs.el, s->fs.el.shape == pmx::SHAPE_HEXA ? hexa_quad_rule : 0).n;
+  });
+}
+
+int pmx_assembly_quad_points(const pmx_space* s, int hexa_quad_rule, double* host_xyz) {
+  return guarded([&] {
+    require(s && host_xyz, "NULL argument");
+    const pmx::FESpace& fs = s->fs;
+    const pmx::QuadRule q = pmx::assembly_rule(fs.el, fs.el.shape == pmx::SHAPE_HEXA ? hexa_quad_rule : 0);
+    for (int c = 0; c < fs.n_cells; ++c) {
+      const pmx::AffineMap a = pmx::affine_map(*fs.mesh, fs.level, c);
+      for (int k = 0; k < q.n; ++k) {
+        double x[3] = {0, 0, 0};
+        pmx::map_to_phys(a, &q.xi[k * q.dim], x);
+        for (int d = 0; d < 3; ++d) host_xyz[((size_t)c * q.n + k) * 3 + d] = x[d];
+      }
+    }
+  });
+}
+
+int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p, pmx_matrix* A,
+                    double* dev_rhs) {
+  return guarded([&] {
+    require(ctx && s_ && p && A && dev_rhs, "NULL argument");
+    require(A->space == s_, "matrix was not created on this space");
+    require(s_->ctx == ctx, "space has no device data for this context");
+    require(p->disc == PMX_DISC_GALERKIN || p->disc == PMX_DISC_SUPG, "unknown discretisation");
+    require(p->example >= 0 && p->example <= 6, "unknown example");
+    pmx_space* s = const_cast<pmx_space*>(s_);
+    hipStream_t st = ctx->stream;
+    const RefTable& t = ref_table(s, p->hexa_quad_rule);
+    const pmx::AssemblyData ad = assembly_data(s, t);
+    const pmx::CDCoeffs co = cd_coeffs(p);
+    for (size_t k = 0; k + 1 < s->color_ptr.size(); ++k) {
+      const int n = s->color_ptr[k + 1] - s->color_ptr[k];
+      pmx::assemble_cd(ad, co, s->d_cells + s->color_ptr[k], n, A->A, dev_rhs, st);
+    }
+    HIP_CHECK(hipGetLastError());
+    const pmx::FESpace& fs = s->fs;
+    const int ndir = fs.n_dof - fs.active_bound;
+    if (ndir > 0) {
+      if (p->host_dirichlet) {
+        HIP_CHECK(hipMemcpyAsync(dev_rhs + fs.active_bound, p->host_dirichlet, ndir * sizeof(double),
+                                 hipMemcpyHostToDevice, st));
+      } else {
+        pmx::dirichlet_rhs_example(p->example, ndir, s->d_dir_xyz, dev_rhs + fs.active_bound, co, st);
+      }
+    }
+    if (fs.el.dim == 2 && p->example == PMX_EX_SINE_LAPLACE_2D) {
+      std::vector<double> neu;
+      neumann_2d(s, p, neu);
+      double* d = upload(neu.data(), neu.size(), st);
+      pmx::vec_axpy(fs.n_dof, 1.0, nullptr, 1.0, d, dev_rhs, st);
+      HIP_CHECK(hipStreamSynchronize(st));
+      dfree(d);
+    }
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_compute_errors(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p,
+                       const double* dev_u, double* host_errors4) {
+  return guarded([&] {
+    require(ctx && s_ && p && dev_u && host_errors4, "NULL argument");
+    require(s_->ctx == ctx, "space has no device data for this context");
+    pmx_space* s = const_cast<pmx_space*>(s_);
+    hipStream_t st = ctx->stream;
+    const RefTable& t = ref_table(s, p->hexa_quad_rule);
+    const pmx::AssemblyData ad = assembly_data(s, t);
+    const pmx::CDCoeffs co = cd_coeffs(p);
+    const int nb = (ad.n_cells + 255) / 256;
+    double* partial = dmalloc<double>((size_t)nb * 4);
+    pmx::cd_errors(ad, co, dev_u, partial, ctx->scal, st);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(host_errors4, ctx->scal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    dfree(partial);
+  });
+}
+
+int pmx_vector_dot(pmx_ctx* ctx, int n, const double* x, const double* y, double* out) {
+  return guarded([&] {
+    require(ctx && x && y && out && n >= 0, "invalid argument");
+    pmx::vec_dot(n, x, y, ctx->partial, ctx->scal, ctx->stream);
+    HIP_CHECK(hipMemcpyAsync(out, ctx->scal, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int pmx_vector_norm(pmx_ctx* ctx, int n, const double* x, double* out) {
+  return guarded([&] {
+    require(ctx && x && out && n >= 0, "invalid argument");
+    pmx::vec_norm(n, x, ctx->partial, ctx->scal, ctx->stream);
+    HIP_CHECK(hipMemcpyAsync(out, ctx->scal, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int pmx_vector_add_scaled(pmx_ctx* ctx, int n, double a, const double* x, double* y) {
+  return guarded([&] {
+    require(ctx && x && y && n >= 0, "invalid argument");
+    pmx::vec_axpy(n, a, nullptr, 1.0, x, y, ctx->stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+}  // extern "C"

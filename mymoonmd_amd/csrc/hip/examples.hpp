@@ -1,0 +1,96 @@
+// Built-in convection-diffusion examples (exact solution u, right-hand side
+// f = -eps*lap(u) + b.grad(u) + c*u and boundary values), usable on host and
+// device.  They restate the examples the reference tests use:
+//   EX_SINE_LAPLACE_2D  include/Examples/CD_2D/SineLaplace.h (Dirichlet value 0,
+//                       Neumann value eps*du/dx on component 1)
+//   EX_P0_3D / EX_P1_3D / EX_P2_3D  include/Examples/CD_3D/test_p0.h, test_p1.h,
+//                       test_p2.h (solutions 1, 1+2x+3y+4z, x^2+yz)
+//   EX_EXP_SIN_3D       u = exp(-x) sin(pi y) sin(pi z) (CD3D benchmark problem)
+//   EX_SINE_2D          u = sin(pi x) sin(pi y) with Dirichlet value u everywhere
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+namespace pmx {
+
+enum Example : int {
+  EX_CONST = 0,
+  EX_SINE_LAPLACE_2D = 1,
+  EX_P0_3D = 2,
+  EX_P1_3D = 3,
+  EX_P2_3D = 4,
+  EX_EXP_SIN_3D = 5,
+  EX_SINE_2D = 6,
+};
+
+constexpr double kPi = 3.14159265358979323846;
+
+// v[0] = u, v[1..3] = grad u, v[4] = laplace u
+__host__ __device__ inline void example_exact(int ex, double x, double y, double z, double* v) {
+  v[0] = v[1] = v[2] = v[3] = v[4] = 0.0;
+  switch (ex) {
+    case EX_SINE_LAPLACE_2D:
+    case EX_SINE_2D: {
+      const double sx = sin(kPi * x), sy = sin(kPi * y);
+      v[0] = sx * sy;
+      v[1] = kPi * cos(kPi * x) * sy;
+      v[2] = kPi * sx * cos(kPi * y);
+      v[4] = -2 * kPi * kPi * sx * sy;
+      break;
+    }
+    case EX_P0_3D: v[0] = 1; break;
+    case EX_P1_3D: v[0] = 1 + 2 * x + 3 * y + 4 * z; v[1] = 2; v[2] = 3; v[3] = 4; break;
+    case EX_P2_3D: v[0] = x * x + y * z; v[1] = 2 * x; v[2] = z; v[3] = y; v[4] = 2; break;
+    case EX_EXP_SIN_3D: {
+      const double e = exp(-x), sy = sin(kPi * y), sz = sin(kPi * z);
+      v[0] = e * sy * sz;
+      v[1] = -v[0];
+      v[2] = kPi * e * cos(kPi * y) * sz;
+      v[3] = kPi * e * sy * cos(kPi * z);
+      v[4] = (1.0 - 2.0 * kPi * kPi) * v[0];
+      break;
+    }
+    default: break;
+  }
+}
+
+__host__ __device__ inline double example_rhs(int ex, double x, double y, double z, double eps,
+                                              const double* b, double c, double f0) {
+  if (ex == EX_CONST) return f0;
+  double v[5];
+  example_exact(ex, x, y, z, v);
+  double f = -eps * v[4];
+  f += b[0] * v[1] + b[1] * v[2] + b[2] * v[3];
+  f += c * v[0];
+  return f;
+}
+
+// Dirichlet boundary value at a boundary point
+__host__ __device__ inline double example_dirichlet(int ex, double x, double y, double z) {
+  if (ex == EX_SINE_LAPLACE_2D || ex == EX_CONST) return 0.0;
+  double v[5];
+  example_exact(ex, x, y, z, v);
+  return v[0];
+}
+
+// SUPG parameter of Compute_SDFEM_delta with SDFEM_TYPE 2
+// (src/AssembleRoutines/ConvDiff.C:273-345): 1D Green's-function formula with
+// nu = 1 / INTERNAL_POLYNOMIAL_DEGREE and h_K = cell diameter.
+__host__ __device__ inline double sdfem_delta(double hK, double eps, const double* b, int dim,
+                                              double c, double delta0, double poly_degree) {
+  (void)c;
+  if (fabs(eps) < 1e-20) eps = 1e-20;
+  double nb = b[0] * b[0] + b[1] * b[1];
+  if (dim == 3) nb += b[2] * b[2];
+  nb = sqrt(nb);
+  const double nu = 1.0 / poly_degree;
+  double delta = 0.0;
+  if (nb > 0) {
+    const double alpha = nu * nb * hK / (2 * eps);
+    delta = nu * hK * (1 / tanh(alpha) - 1 / alpha) / (2 * nb);
+  }
+  return delta * delta0;
+}
+
+}  // namespace pmx

@@ -1,0 +1,307 @@
+// Element-batched quadrature assembly of the convection-diffusion operator
+// into the reference CSR pattern (gfx950).
+//
+// Restates, per cell: LocalAssembling<d>::GetLocalForms
+// (src/AssembleRoutines/LocalAssembling.C:668-764) with the ConvDiff forms
+// BilinearAssembleGalerkin<d> and BilinearAssemble_SD<d>
+// (src/AssembleRoutines/ConvDiff.C:1794-2011) and the scatter of Assemble3D /
+// Assemble2D (src/AssembleRoutines/Assemble3D.C:319-409, 467-507):
+//   A_ij += w|J| (eps grad phi_j . grad phi_i + (b.grad phi_j) phi_i + c phi_j phi_i)
+//   A_ij += w|J| delta (-eps lap phi_j + b.grad phi_j + c phi_j)(b.grad phi_i)  [SUPG]
+//   f_i  += w|J| f phi_i (+ w|J| delta (b.grad phi_i) f)                         [SUPG]
+// rows of Dirichlet DOFs get their diagonal set to 1.
+//
+// Layout: a 256-thread block holds 256/SLOT cells; the SLOT lanes of a cell
+// own one local row (test function) each.  Per quadrature point the lanes
+// stage the physical basis data of their cell in LDS, then every lane
+// accumulates its 27-entry row in registers.  Cells of one colour share no
+// DOF, so the scatter into the CSR row (binary search of the sorted columns)
+// is race free without atomics.
+#include "common.hpp"
+#include "examples.hpp"
+#include "launch.hpp"
+
+namespace pmx {
+
+namespace {
+
+template <int DIM>
+struct Dims {
+  static constexpr int NC = (DIM == 3) ? 6 : 3;   // second-derivative components
+};
+
+__device__ __forceinline__ int find_col(const int* __restrict__ ci, int s, int e, int c) {
+  int lo = s, hi = e - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (ci[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// per-(cell, q) coefficients: eps, b1, b2, b3, c, f, delta, mult
+template <int DIM>
+__device__ void coeffs_at(const CDCoeffs& co, const double* geo, const double* xi, double w,
+                          int cell, int q, int nq, double* out) {
+  double x[3] = {0, 0, 0};
+  for (int r = 0; r < DIM; ++r) {
+    double s = geo[r];
+    for (int c = 0; c < DIM; ++c) s += geo[3 + r * DIM + c] * xi[c];
+    x[r] = s;
+  }
+  double eps = co.eps, b[3] = {co.b[0], co.b[1], co.b[2]}, c = co.c, f;
+  if (co.table) {
+    const double* t = co.table + ((size_t)cell * nq + q) * 8;
+    eps = t[0]; b[0] = t[1]; b[1] = t[2]; b[2] = (DIM == 3) ? t[3] : 0.0; c = t[4]; f = t[5];
+  } else {
+    f = example_rhs(co.example, x[0], x[1], x[2], eps, b, c, co.f0);
+  }
+  const double detJ = geo[21], hK = geo[22];
+  out[0] = eps; out[1] = b[0]; out[2] = b[1]; out[3] = b[2]; out[4] = c; out[5] = f;
+  out[6] = co.supg ? sdfem_delta(hK, eps, b, DIM, c, co.delta0, co.poly_degree) : 0.0;
+  out[7] = w * fabs(detJ);
+}
+
+template <int ND, int DIM, int SLOT>
+__global__ __launch_bounds__(256) void k_assemble_cd(AssemblyData ad, CDCoeffs co,
+                                                     const int* __restrict__ cells, int n,
+                                                     DevCSR A, double* __restrict__ rhs) {
+  constexpr int NC = Dims<DIM>::NC;
+  constexpr int CPB = 256 / SLOT;
+  constexpr int RS = 1 + DIM + ND * (1 + DIM + NC);   // ref table stride per q
+  __shared__ double s_u[CPB][ND];
+  __shared__ double s_g[CPB][DIM][ND];
+  __shared__ double s_l[CPB][ND];
+  __shared__ double s_c[CPB][8];
+  const int slot = threadIdx.x / SLOT;
+  const int i = threadIdx.x % SLOT;
+  const int ci = blockIdx.x * CPB + slot;
+  const bool valid = ci < n;
+  const int cell = valid ? cells[ci] : 0;
+  const double* geo = ad.geo + (size_t)cell * kGeoStride;
+  double Jinv[DIM * DIM];
+#pragma unroll
+  for (int k = 0; k < DIM * DIM; ++k) Jinv[k] = geo[12 + k];
+  // G = Jinv Jinv^T for the physical Laplacian on affine cells
+  double G[DIM * DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k)
+#pragma unroll
+    for (int l = 0; l < DIM; ++l) {
+      double s = 0;
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) s += Jinv[k * DIM + a] * Jinv[l * DIM + a];
+      G[k * DIM + l] = s;
+    }
+  double arow[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) arow[j] = 0.0;
+  double frow = 0.0;
+
+  for (int q = 0; q < ad.nq; ++q) {
+    const double* rq = ad.ref + (size_t)q * RS;
+    if (valid && i < ND) {
+      const double* dphi = rq + 1 + DIM + ND;
+      s_u[slot][i] = rq[1 + DIM + i];
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) s += dphi[k * ND + i] * Jinv[k * DIM + a];
+        s_g[slot][a][i] = s;
+      }
+      if (co.supg) {
+        const double* d2 = rq + 1 + DIM + ND * (1 + DIM);
+        double lap;
+        if (DIM == 3) {
+          lap = d2[0 * ND + i] * G[0] + d2[3 * ND + i] * G[4] + d2[5 * ND + i] * G[8] +
+                2.0 * (d2[1 * ND + i] * G[1] + d2[2 * ND + i] * G[2] + d2[4 * ND + i] * G[5]);
+        } else {
+          lap = d2[0 * ND + i] * G[0] + d2[2 * ND + i] * G[3] + 2.0 * d2[1 * ND + i] * G[1];
+        }
+        s_l[slot][i] = lap;
+      }
+    }
+    if (valid && i == 0) coeffs_at<DIM>(co, geo, rq + 1, rq[0], cell, q, ad.nq, s_c[slot]);
+    __syncthreads();
+    if (valid && i < ND) {
+      const double eps = s_c[slot][0], b1 = s_c[slot][1], b2 = s_c[slot][2];
+      const double b3 = (DIM == 3) ? s_c[slot][3] : 0.0;
+      const double c = s_c[slot][4], f = s_c[slot][5], delta = s_c[slot][6], mult = s_c[slot][7];
+      const double ti = s_u[slot][i];
+      const double tx = s_g[slot][0][i], ty = s_g[slot][1][i];
+      const double tz = (DIM == 3) ? s_g[slot][DIM - 1][i] : 0.0;
+      const double bgradv = b1 * tx + b2 * ty + b3 * tz;
+      frow += mult * ti * f;
+      if (co.supg) frow += mult * (delta * bgradv) * f;
+#pragma unroll
+      for (int j = 0; j < ND; ++j) {
+        const double ax = s_g[slot][0][j], ay = s_g[slot][1][j];
+        const double az = (DIM == 3) ? s_g[slot][DIM - 1][j] : 0.0;
+        const double aj = s_u[slot][j];
+        const double bgu = b1 * ax + b2 * ay + b3 * az;
+        double val = eps * (tx * ax + ty * ay + tz * az);
+        val += bgu * ti;
+        val += c * aj * ti;
+        arow[j] += val * mult;
+        if (co.supg) {
+          double v2 = -eps * s_l[slot][j];
+          v2 += bgu + c * aj;
+          arow[j] += v2 * delta * bgradv * mult;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!valid || i >= ND) return;
+  const int* dofs = ad.dofs + (size_t)cell * ND;
+  const int r = dofs[i];
+  const int rs = A.row_ptr[r], re = A.row_ptr[r + 1];
+  if (r < A.active) {
+#pragma unroll
+    for (int j = 0; j < ND; ++j) {
+      const int pos = find_col(A.col, rs, re, dofs[j]);
+      A.val[pos] += arow[j];
+    }
+    rhs[r] += frow;
+  } else {
+    A.val[find_col(A.col, rs, re, r)] = 1.0;
+  }
+}
+
+template <int ND, int DIM, int SLOT>
+void launch_asm(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n,
+                const DevCSR& A, double* rhs, hipStream_t st) {
+  constexpr int CPB = 256 / SLOT;
+  const int grid = (n + CPB - 1) / CPB;
+  if (grid > 0)
+    hipLaunchKernelGGL((k_assemble_cd<ND, DIM, SLOT>), dim3(grid), dim3(256), 0, st, ad, co, cells, n,
+                       A, rhs);
+}
+
+__global__ void k_dirichlet_example(int ex, int n, const double* __restrict__ xyz,
+                                    double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = example_dirichlet(ex, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]);
+}
+
+// one thread per cell; partial sums per block
+template <int ND, int DIM>
+__global__ __launch_bounds__(256) void k_cd_errors(AssemblyData ad, CDCoeffs co,
+                                                   const double* __restrict__ u,
+                                                   double* __restrict__ partial) {
+  constexpr int NC = Dims<DIM>::NC;
+  constexpr int RS = 1 + DIM + ND * (1 + DIM + NC);
+  __shared__ double red[16];
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  double e[4] = {0, 0, 0, 0};
+  if (cell < ad.n_cells) {
+    const double* geo = ad.geo + (size_t)cell * kGeoStride;
+    double uloc[ND];
+    for (int j = 0; j < ND; ++j) uloc[j] = u[ad.dofs[(size_t)cell * ND + j]];
+    for (int q = 0; q < ad.nq; ++q) {
+      const double* rq = ad.ref + (size_t)q * RS;
+      double cf[8];
+      coeffs_at<DIM>(co, geo, rq + 1, rq[0], cell, q, ad.nq, cf);
+      double val = 0, g[3] = {0, 0, 0};
+      const double* dphi = rq + 1 + DIM + ND;
+      for (int j = 0; j < ND; ++j) {
+        val += uloc[j] * rq[1 + DIM + j];
+        for (int a = 0; a < DIM; ++a) {
+          double s = 0;
+          for (int k = 0; k < DIM; ++k) s += dphi[k * ND + j] * geo[12 + k * DIM + a];
+          g[a] += uloc[j] * s;
+        }
+      }
+      double x[3] = {0, 0, 0};
+      for (int r = 0; r < DIM; ++r) {
+        double s = geo[r];
+        for (int c = 0; c < DIM; ++c) s += geo[3 + r * DIM + c] * rq[1 + c];
+        x[r] = s;
+      }
+      double ex[5];
+      example_exact(co.example, x[0], x[1], x[2], ex);
+      const double w = cf[7];
+      const double e0 = val - ex[0], e1 = g[0] - ex[1], e2 = g[1] - ex[2];
+      const double e3 = (DIM == 3) ? g[2] - ex[3] : 0.0;
+      e[0] += w * e0 * e0;
+      e[1] += w * (e1 * e1 + e2 * e2 + e3 * e3);
+      const double e4 = cf[1] * e1 + cf[2] * e2 + cf[3] * e3;
+      e[2] += w * (cf[0] * (e1 * e1 + e2 * e2 + e3 * e3) + cf[4] * e0 * e0 + cf[6] * e4 * e4);
+      e[3] = fmax(e[3], fabs(e0));
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    const double s = block_sum(e[k], red);
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.x * 4 + k] = s;
+  }
+  // max via the same reduction tree on squared-free values
+  double m = e[3];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = 0;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) mm = fmax(mm, red[w]);
+    partial[(size_t)blockIdx.x * 4 + 3] = mm;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_errors(int nb, const double* __restrict__ partial, double* __restrict__ out) {
+  __shared__ double red[16];
+  double s[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    for (int k = 0; k < 3; ++k) s[k] += partial[(size_t)b * 4 + k];
+    s[3] = fmax(s[3], partial[(size_t)b * 4 + 3]);
+  }
+  for (int k = 0; k < 3; ++k) {
+    const double t = block_sum(s[k], red);
+    if (threadIdx.x == 0) out[k] = sqrt(t);
+  }
+  double m = s[3];
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = 0;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) mm = fmax(mm, red[w]);
+    out[3] = mm;
+  }
+}
+
+}  // namespace
+
+void assemble_cd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n,
+                 const DevCSR& A, double* rhs, hipStream_t st) {
+  if (ad.dim == 3 && ad.ndof == 27) launch_asm<27, 3, 32>(ad, co, cells, n, A, rhs, st);
+  else if (ad.dim == 3 && ad.ndof == 8) launch_asm<8, 3, 8>(ad, co, cells, n, A, rhs, st);
+  else if (ad.dim == 2 && ad.ndof == 4) launch_asm<4, 2, 4>(ad, co, cells, n, A, rhs, st);
+  else if (ad.dim == 2 && ad.ndof == 9) launch_asm<9, 2, 16>(ad, co, cells, n, A, rhs, st);
+  else if (ad.dim == 2 && ad.ndof == 3) launch_asm<3, 2, 4>(ad, co, cells, n, A, rhs, st);
+}
+
+void dirichlet_rhs_example(int example, int n_dir, const double* dir_xyz, double* rhs_dir,
+                           const CDCoeffs&, hipStream_t st) {
+  if (n_dir > 0)
+    hipLaunchKernelGGL(k_dirichlet_example, dim3((n_dir + 255) / 256), dim3(256), 0, st, example, n_dir,
+                       dir_xyz, rhs_dir);
+}
+
+void cd_errors(const AssemblyData& ad, const CDCoeffs& co, const double* u, double* partial,
+               double* out4, hipStream_t st) {
+  const int nb = (ad.n_cells + 255) / 256;
+  if (ad.dim == 3 && ad.ndof == 27)
+    hipLaunchKernelGGL((k_cd_errors<27, 3>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
+  else if (ad.dim == 3 && ad.ndof == 8)
+    hipLaunchKernelGGL((k_cd_errors<8, 3>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
+  else if (ad.dim == 2 && ad.ndof == 4)
+    hipLaunchKernelGGL((k_cd_errors<4, 2>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
+  else if (ad.dim == 2 && ad.ndof == 9)
+    hipLaunchKernelGGL((k_cd_errors<9, 2>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
+  else if (ad.dim == 2 && ad.ndof == 3)
+    hipLaunchKernelGGL((k_cd_errors<3, 2>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
+  hipLaunchKernelGGL(k_reduce_errors, dim3(1), dim3(1024), 0, st, nb, partial, out4);
+}
+
+}  // namespace pmx

@@ -1,0 +1,374 @@
+// Sparse/dense linear-algebra kernels of the solve path (gfx950).
+//
+//  * FE-matrix SpMV with the reference's Dirichlet semantics:
+//    BlockFEMatrix::apply_scaled_add (src/Matrix/BlockFEMatrix.C:863-924) =
+//    FEMatrix::multiplyActive on rows < ActiveBound (src/Matrix/FEMatrix.C:186-199)
+//    plus y += a*x on the non-active rows (BlockVector::addScaledNonActive,
+//    src/Matrix/BlockVector.C:212-228).
+//  * fused defect d = b - A x with ||d||^2 (MultigridLevel::calculate_defect,
+//    src/Multigrid/MultigridLevel.C:131-150).
+//  * fused damped Jacobi step x' = x + w D^-1 (b - A x)
+//    (JacobiSmoother::smooth, src/Multigrid/JacobiSmoother.C:22-30).
+//  * plain CSR SpMV for the grid-transfer matrices.
+//  * BLAS-1 with deterministic two-stage reductions (Ddot/Dnorm,
+//    src/FE/Blas1.C:14-117 sum sequentially; here a fixed tree is used).
+//
+// Every kernel is HBM-bound: one row is handled by a group of LPR lanes of a
+// 64-wide wavefront so that the row's values/columns are read coalesced.
+#include <type_traits>
+
+#include "common.hpp"
+#include "launch.hpp"
+
+namespace pmx {
+
+namespace {
+
+enum SpmvMode { MODE_SCALED_ADD = 0, MODE_DEFECT = 1, MODE_JACOBI = 2, MODE_APPLY = 3 };
+
+template <int LPR, int MODE, typename VT, typename XT>
+__global__ __launch_bounds__(256) void k_fe_spmv(
+    int n, int active, const int* __restrict__ rp, const int* __restrict__ ci,
+    const VT* __restrict__ va, const float* __restrict__ rscale,
+    const XT* __restrict__ x, const XT* __restrict__ b, const XT* __restrict__ dinv,
+    XT* __restrict__ y, double a, double* __restrict__ partial) {
+  __shared__ double red[16];
+  const int lane = threadIdx.x & (LPR - 1);
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
+  AT acc = 0;
+  double d2 = 0;
+  if (row < n) {
+    if (row < active) {
+      const int s = rp[row], e = rp[row + 1];
+      for (int k = s + lane; k < e; k += LPR) {
+        AT v;
+        if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(va[k]);
+        else v = (AT)va[k];
+        acc += v * (AT)x[ci[k]];
+      }
+      acc = group_sum<LPR>(acc);
+      if (rscale) acc *= (AT)rscale[row];
+    }
+    if (lane == 0) {
+      if (MODE == MODE_SCALED_ADD) {
+        if (row < active) y[row] += (XT)(a * (double)acc);
+        else y[row] += (XT)(a * (double)x[row]);
+      } else if (MODE == MODE_APPLY) {
+        y[row] = (row < active) ? (XT)acc : x[row];
+      } else if (MODE == MODE_DEFECT) {
+        const XT d = (row < active) ? (XT)((AT)b[row] - acc) : (XT)(b[row] - x[row]);
+        y[row] = d;
+        d2 = (double)d * (double)d;
+      } else {  // MODE_JACOBI: y = x + a * dinv * (b - A x)
+        const AT r = (row < active) ? ((AT)b[row] - acc) : (AT)(b[row] - x[row]);
+        y[row] = (XT)((AT)x[row] + (AT)a * (AT)dinv[row] * r);
+      }
+    }
+  }
+  if (MODE == MODE_DEFECT) {
+    const double s = block_sum(d2, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+template <int LPR, int MODE, typename VT, typename XT>
+void launch_fe_spmv(const DevCSR& A, const VT* va, const float* rscale, const XT* x, const XT* b,
+                    const XT* dinv, XT* y, double a, double* partial, hipStream_t st) {
+  const int rows_per_block = 256 / LPR;
+  const int grid = (A.n + rows_per_block - 1) / rows_per_block;
+  hipLaunchKernelGGL((k_fe_spmv<LPR, MODE, VT, XT>), dim3(grid), dim3(256), 0, st, A.n, A.active,
+                     A.row_ptr, A.col, va, rscale, x, b, dinv, y, a, partial);
+}
+
+template <int MODE, typename VT, typename XT>
+void dispatch_fe_spmv(const DevCSR& A, const VT* va, const float* rscale, const XT* x, const XT* b,
+                      const XT* dinv, XT* y, double a, double* partial, hipStream_t st) {
+  switch (A.lpr) {
+    case 4: launch_fe_spmv<4, MODE>(A, va, rscale, x, b, dinv, y, a, partial, st); break;
+    case 8: launch_fe_spmv<8, MODE>(A, va, rscale, x, b, dinv, y, a, partial, st); break;
+    case 16: launch_fe_spmv<16, MODE>(A, va, rscale, x, b, dinv, y, a, partial, st); break;
+    case 32: launch_fe_spmv<32, MODE>(A, va, rscale, x, b, dinv, y, a, partial, st); break;
+    default: launch_fe_spmv<64, MODE>(A, va, rscale, x, b, dinv, y, a, partial, st); break;
+  }
+}
+
+// y = alpha * P x + beta * y over all rows (grid transfer)
+template <int LPR, typename T>
+__global__ __launch_bounds__(256) void k_csr_spmv(int n, const int* __restrict__ rp,
+                                                   const int* __restrict__ ci,
+                                                   const double* __restrict__ va,
+                                                   const T* __restrict__ x, T* __restrict__ y,
+                                                   double alpha, double beta, int row_end,
+                                                   int zero_rest) {
+  const int lane = threadIdx.x & (LPR - 1);
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  if (row >= n) return;
+  if (row >= row_end) {
+    if (zero_rest && lane == 0) y[row] = (T)0;
+    return;
+  }
+  double acc = 0;
+  const int s = rp[row], e = rp[row + 1];
+  for (int k = s + lane; k < e; k += LPR) acc += va[k] * (double)x[ci[k]];
+  acc = group_sum<LPR>(acc);
+  if (lane == 0) {
+    double r = alpha * acc;
+    if (beta != 0.0) r += beta * (double)y[row];
+    y[row] = (T)r;
+  }
+}
+
+// ---------------------------------------------------------------- BLAS-1
+constexpr int kRedBlocks = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_dot_partial(int n, const T* __restrict__ x,
+                                                      const T* __restrict__ y,
+                                                      double* __restrict__ partial) {
+  __shared__ double red[16];
+  double s = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    s += (double)x[i] * (double)y[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out[0] = sum(partial[0..m)) (op 0), sqrt of it (op 1); optional scale out
+__global__ __launch_bounds__(1024) void k_reduce_partials(int m, const double* __restrict__ partial,
+                                                          double* __restrict__ out, int op) {
+  __shared__ double red[16];
+  double s = 0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) s += partial[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[0] = (op == 1) ? sqrt(s) : s;
+}
+
+template <typename T>
+__global__ void k_axpy(int n, double a, const double* __restrict__ a_dev, double a_dev_sign,
+                       const T* __restrict__ x, T* __restrict__ y) {
+  const double aa = a_dev ? a_dev_sign * a_dev[0] : a;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = (T)((double)y[i] + aa * (double)x[i]);
+}
+
+// r -= h[k] * v ; partial dot of the updated r with w (pipelined modified
+// Gram-Schmidt step of Iteration_gmres::flexible_gmres, src/Solver/Iteration_gmres.C:452-466)
+__global__ __launch_bounds__(256) void k_axpy_dot(int n, const double* __restrict__ h,
+                                                  const double* __restrict__ v,
+                                                  double* __restrict__ r,
+                                                  const double* __restrict__ w,
+                                                  double* __restrict__ partial) {
+  __shared__ double red[16];
+  const double hk = h ? h[0] : 0.0;
+  double s = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double ri = r[i];
+    if (h) { ri -= hk * v[i]; r[i] = ri; }
+    s += ri * (w ? w[i] : ri);
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void k_scale(int n, double a, const double* __restrict__ a_dev, int a_dev_inv,
+                        const T* __restrict__ x, T* __restrict__ y) {
+  double aa = a;
+  if (a_dev) aa = a_dev_inv ? 1.0 / a_dev[0] : a_dev[0];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = (T)(aa * (double)x[i]);
+}
+
+template <typename TO, typename TI>
+__global__ void k_convert(int n, const TI* __restrict__ x, TO* __restrict__ y) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = (TO)x[i];
+}
+
+__global__ void k_mul(int n, const double* __restrict__ a, const double* __restrict__ x,
+                      double* __restrict__ y) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = a[i] * x[i];
+}
+
+template <typename T>
+__global__ void k_fill(int n, T v, T* __restrict__ y) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = v;
+}
+
+// y[i] += a*x[i] for i in [from, n)
+template <typename TY, typename TX>
+__global__ void k_add_scaled_range(int from, int n, double a, const TX* __restrict__ x,
+                                   TY* __restrict__ y) {
+  for (int i = from + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = (TY)((double)y[i] + a * (double)x[i]);
+}
+
+// dinv[i] = 1 / A_ii ; flags zero diagonals
+__global__ void k_inv_diag(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                           const double* __restrict__ va, double* __restrict__ dinv,
+                           int* __restrict__ zero_flag) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    double d = 0;
+    for (int k = rp[r]; k < rp[r + 1]; ++k)
+      if (ci[k] == r) { d = va[k]; break; }
+    if (d == 0.0) { atomicOr(zero_flag, 1); dinv[r] = 0.0; }
+    else dinv[r] = 1.0 / d;
+  }
+}
+
+// FP64 values -> bf16 values with an FP32 per-row scale (row max-abs
+// normalised to 1), for the mixed-precision V-cycle
+__global__ void k_to_bf16_rows(int n, const int* __restrict__ rp, const double* __restrict__ va,
+                               uint16_t* __restrict__ vb, float* __restrict__ rscale) {
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    double m = 0;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) m = fmax(m, fabs(va[k]));
+    const double sc = (m > 0) ? m : 1.0;
+    rscale[r] = (float)sc;
+    const double inv = 1.0 / sc;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      const float f = (float)(va[k] * inv);
+      uint32_t u = __float_as_uint(f);
+      u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+      vb[k] = (uint16_t)(u >> 16);
+    }
+  }
+}
+
+int grid_for(int n, int block = 256, int cap = 4096) {
+  int g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  return g > cap ? cap : g;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- launchers
+void fe_apply_scaled_add(const DevCSR& A, const double* x, double* y, double a, hipStream_t st) {
+  dispatch_fe_spmv<MODE_SCALED_ADD, double, double>(A, A.val, nullptr, x, nullptr, nullptr, y, a,
+                                                    nullptr, st);
+}
+void fe_apply(const DevCSR& A, const double* x, double* y, hipStream_t st) {
+  dispatch_fe_spmv<MODE_APPLY, double, double>(A, A.val, nullptr, x, nullptr, nullptr, y, 1.0,
+                                               nullptr, st);
+}
+void fe_defect(const DevCSR& A, const double* x, const double* b, double* d, double* partial,
+               double* norm_out, hipStream_t st) {
+  dispatch_fe_spmv<MODE_DEFECT, double, double>(A, A.val, nullptr, x, b, nullptr, d, 1.0, partial,
+                                                st);
+  const int nb = (A.n + (256 / A.lpr) - 1) / (256 / A.lpr);
+  if (norm_out) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, nb, partial, norm_out, 1);
+}
+int fe_defect_partials(const DevCSR& A) { return (A.n + (256 / A.lpr) - 1) / (256 / A.lpr); }
+void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* dinv, double* xn,
+               double omega, hipStream_t st) {
+  dispatch_fe_spmv<MODE_JACOBI, double, double>(A, A.val, nullptr, x, b, dinv, xn, omega, nullptr,
+                                                st);
+}
+
+// mixed precision: bf16 values with FP32 row scale, FP32 vectors
+void fe_apply_scaled_add_bf16(const DevCSR& A, const float* x, float* y, double a, hipStream_t st) {
+  dispatch_fe_spmv<MODE_SCALED_ADD, uint16_t, float>(A, A.val_bf16, A.rscale, x, nullptr, nullptr,
+                                                     y, a, nullptr, st);
+}
+void fe_defect_bf16(const DevCSR& A, const float* x, const float* b, float* d, double* partial,
+                    double* norm_out, hipStream_t st) {
+  dispatch_fe_spmv<MODE_DEFECT, uint16_t, float>(A, A.val_bf16, A.rscale, x, b, nullptr, d, 1.0,
+                                                 partial, st);
+  const int nb = fe_defect_partials(A);
+  if (norm_out) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, nb, partial, norm_out, 1);
+}
+void fe_jacobi_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv, float* xn,
+                    double omega, hipStream_t st) {
+  dispatch_fe_spmv<MODE_JACOBI, uint16_t, float>(A, A.val_bf16, A.rscale, x, b, dinv, xn, omega,
+                                                 nullptr, st);
+}
+
+template <typename T>
+void csr_spmv_t(const DevCSR& P, const T* x, T* y, double alpha, double beta, int row_end,
+                int zero_rest, hipStream_t st) {
+  const int lpr = 8;
+  const int grid = (P.n + (256 / lpr) - 1) / (256 / lpr);
+  hipLaunchKernelGGL((k_csr_spmv<8, T>), dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, P.val, x,
+                     y, alpha, beta, row_end, zero_rest);
+}
+void csr_spmv(const DevCSR& P, const double* x, double* y, double alpha, double beta, int row_end,
+              int zero_rest, hipStream_t st) {
+  csr_spmv_t<double>(P, x, y, alpha, beta, row_end, zero_rest, st);
+}
+void csr_spmv_f32(const DevCSR& P, const float* x, float* y, double alpha, double beta,
+                  int row_end, int zero_rest, hipStream_t st) {
+  csr_spmv_t<float>(P, x, y, alpha, beta, row_end, zero_rest, st);
+}
+
+void vec_dot(int n, const double* x, const double* y, double* partial, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_partial<double>, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, kRedBlocks, partial, out, 0);
+}
+void vec_norm(int n, const double* x, double* partial, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_partial<double>, dim3(kRedBlocks), dim3(256), 0, st, n, x, x, partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, kRedBlocks, partial, out, 1);
+}
+void vec_norm_f32(int n, const float* x, double* partial, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_partial<float>, dim3(kRedBlocks), dim3(256), 0, st, n, x, x, partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, kRedBlocks, partial, out, 1);
+}
+void vec_axpy_dot(int n, const double* h, const double* v, double* r, const double* w,
+                  double* partial, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_dot, dim3(kRedBlocks), dim3(256), 0, st, n, h, v, r, w, partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, kRedBlocks, partial, out,
+                     w ? 0 : 1);
+}
+void vec_axpy(int n, double a, const double* a_dev, double sign, const double* x, double* y,
+              hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy<double>, dim3(grid_for(n)), dim3(256), 0, st, n, a, a_dev, sign, x, y);
+}
+void vec_axpy_f32(int n, double a, const float* x, float* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy<float>, dim3(grid_for(n)), dim3(256), 0, st, n, a, (const double*)nullptr,
+                     1.0, x, y);
+}
+void vec_scale(int n, double a, const double* a_dev, int a_dev_inv, const double* x, double* y,
+               hipStream_t st) {
+  hipLaunchKernelGGL(k_scale<double>, dim3(grid_for(n)), dim3(256), 0, st, n, a, a_dev, a_dev_inv, x, y);
+}
+void vec_scale_f32(int n, double a, const double* a_dev, int a_dev_inv, const float* x, float* y,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_scale<float>, dim3(grid_for(n)), dim3(256), 0, st, n, a, a_dev, a_dev_inv, x, y);
+}
+void vec_mul(int n, const double* a, const double* x, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(256), 0, st, n, a, x, y);
+}
+void vec_fill(int n, double v, double* y, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_fill<double>, dim3(grid_for(n)), dim3(256), 0, st, n, v, y);
+}
+void vec_fill_f32(int n, float v, float* y, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_fill<float>, dim3(grid_for(n)), dim3(256), 0, st, n, v, y);
+}
+void vec_add_scaled_range(int from, int n, double a, const double* x, double* y, hipStream_t st) {
+  if (n > from)
+    hipLaunchKernelGGL((k_add_scaled_range<double, double>), dim3(grid_for(n - from)), dim3(256), 0, st,
+                       from, n, a, x, y);
+}
+void vec_add_scaled_range_f32(int from, int n, double a, const float* x, float* y, hipStream_t st) {
+  if (n > from)
+    hipLaunchKernelGGL((k_add_scaled_range<float, float>), dim3(grid_for(n - from)), dim3(256), 0, st,
+                       from, n, a, x, y);
+}
+void vec_f64_to_f32(int n, const double* x, float* y, hipStream_t st) {
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+}
+void vec_f32_to_f64(int n, const float* x, double* y, hipStream_t st) {
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
+}
+void csr_inv_diag(const DevCSR& A, double* dinv, int* zero_flag, hipStream_t st) {
+  hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(A.n)), dim3(256), 0, st, A.n, A.row_ptr, A.col, A.val,
+                     dinv, zero_flag);
+}
+void csr_to_bf16(const DevCSR& A, uint16_t* vb, float* rscale, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_bf16_rows, dim3(grid_for(A.n)), dim3(256), 0, st, A.n, A.row_ptr, A.val, vb,
+                     rscale);
+}
+
+}  // namespace pmx

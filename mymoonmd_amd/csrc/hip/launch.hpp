@@ -1,0 +1,102 @@
+// Host-callable launchers of the device kernels (all asynchronous on `st`).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace pmx {
+
+// Device CSR matrix (0-based int32 row pointer / columns, sorted columns).
+struct DevCSR {
+  int n = 0;            // rows
+  int m = 0;            // columns
+  int nnz = 0;
+  int active = 0;       // ActiveBound: rows >= active are identity rows
+  int lpr = 64;         // lanes per row used by the SpMV kernels
+  int* row_ptr = nullptr;
+  int* col = nullptr;
+  double* val = nullptr;
+  uint16_t* val_bf16 = nullptr;   // mixed precision copy (bf16, row-scaled)
+  float* rscale = nullptr;        // per-row scale of the bf16 copy
+};
+
+// ---- FE-matrix operators (FP64)
+void fe_apply_scaled_add(const DevCSR& A, const double* x, double* y, double a, hipStream_t st);
+void fe_apply(const DevCSR& A, const double* x, double* y, hipStream_t st);
+int fe_defect_partials(const DevCSR& A);
+void fe_defect(const DevCSR& A, const double* x, const double* b, double* d, double* partial,
+               double* norm_out, hipStream_t st);
+void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* dinv, double* xn,
+               double omega, hipStream_t st);
+// ---- FE-matrix operators (bf16 matrix, FP32 vectors)
+void fe_apply_scaled_add_bf16(const DevCSR& A, const float* x, float* y, double a, hipStream_t st);
+void fe_defect_bf16(const DevCSR& A, const float* x, const float* b, float* d, double* partial,
+                    double* norm_out, hipStream_t st);
+void fe_jacobi_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv, float* xn,
+                    double omega, hipStream_t st);
+// ---- plain CSR (transfer matrices): y = alpha*P x + beta*y on rows < row_end;
+//      rows >= row_end are set to 0 when zero_rest, else left unchanged
+void csr_spmv(const DevCSR& P, const double* x, double* y, double alpha, double beta, int row_end,
+              int zero_rest, hipStream_t st);
+void csr_spmv_f32(const DevCSR& P, const float* x, float* y, double alpha, double beta,
+                  int row_end, int zero_rest, hipStream_t st);
+void csr_inv_diag(const DevCSR& A, double* dinv, int* zero_flag, hipStream_t st);
+void csr_to_bf16(const DevCSR& A, uint16_t* vb, float* rscale, hipStream_t st);
+
+// ---- BLAS-1 (partial buffers hold >= 1024 doubles)
+void vec_dot(int n, const double* x, const double* y, double* partial, double* out, hipStream_t st);
+void vec_norm(int n, const double* x, double* partial, double* out, hipStream_t st);
+void vec_norm_f32(int n, const float* x, double* partial, double* out, hipStream_t st);
+// r -= h[0]*v (skipped when h == nullptr); out = <r, w> (or ||r|| when w == nullptr)
+void vec_axpy_dot(int n, const double* h, const double* v, double* r, const double* w,
+                  double* partial, double* out, hipStream_t st);
+// y += (a_dev ? sign*a_dev[0] : a) * x
+void vec_axpy(int n, double a, const double* a_dev, double sign, const double* x, double* y,
+              hipStream_t st);
+void vec_axpy_f32(int n, double a, const float* x, float* y, hipStream_t st);
+// y = s * x with s = a, a_dev[0] or 1/a_dev[0]
+void vec_scale(int n, double a, const double* a_dev, int a_dev_inv, const double* x, double* y,
+               hipStream_t st);
+void vec_scale_f32(int n, double a, const double* a_dev, int a_dev_inv, const float* x, float* y,
+                   hipStream_t st);
+void vec_fill(int n, double v, double* y, hipStream_t st);
+// y = a .* x (elementwise)
+void vec_mul(int n, const double* a, const double* x, double* y, hipStream_t st);
+void vec_fill_f32(int n, float v, float* y, hipStream_t st);
+void vec_add_scaled_range(int from, int n, double a, const double* x, double* y, hipStream_t st);
+void vec_add_scaled_range_f32(int from, int n, double a, const float* x, float* y, hipStream_t st);
+void vec_f64_to_f32(int n, const double* x, float* y, hipStream_t st);
+void vec_f32_to_f64(int n, const float* x, double* y, hipStream_t st);
+
+// ---- assembly
+struct AssemblyData {
+  int dim = 0;
+  int ndof = 0;           // local DOFs per cell
+  int nq = 0;             // quadrature points
+  int n_cells = 0;
+  const double* ref = nullptr;     // per q: w, phi[nd], dphi[dim][nd], d2phi[ncomb][nd]
+  const double* geo = nullptr;     // per cell: x0[3], J[9], Jinv[9], detJ, hK, pad
+  const int* dofs = nullptr;       // [n_cells][ndof] global numbers
+};
+constexpr int kGeoStride = 24;
+
+struct CDCoeffs {
+  int example = 0;        // built-in right-hand side / exact solution
+  int supg = 0;           // add BilinearAssemble_SD
+  double eps = 1, b[3] = {0, 0, 0}, c = 0, f0 = 0;
+  double delta0 = 1, poly_degree = 1;
+  const double* table = nullptr;   // optional per-(cell,q) coefficients [8]
+};
+
+// Adds the cell contributions of the cells cells[0..n) (one colour) into A and rhs.
+void assemble_cd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n,
+                 const DevCSR& A, double* rhs, hipStream_t st);
+// rhs[i] = g(x_i) for Dirichlet rows from the built-in example (i >= active)
+void dirichlet_rhs_example(int example, int n_dir, const double* dir_xyz, double* rhs_dir,
+                           const CDCoeffs& co, hipStream_t st);
+// L2, H1-semi, SD errors (squared partials) and Linf over quadrature points
+void cd_errors(const AssemblyData& ad, const CDCoeffs& co, const double* u, double* partial4,
+               double* out4, hipStream_t st);
+
+}  // namespace pmx

@@ -1,0 +1,303 @@
+#include "space.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace pmx {
+
+namespace {
+
+// Lattice resolution: nodal points of order-k elements on a mesh refined L
+// times from the unit square/cube lie on the grid (1/(k 2^L)) Z^d.
+int64_t lattice_size(const Mesh& m, int level, int order) {
+  return (int64_t)order << level;
+}
+
+int64_t lattice_key(const double* x, int dim, int64_t S) {
+  int64_t k = 0;
+  for (int d = dim - 1; d >= 0; --d) {
+    const int64_t i = std::llround(x[d] * (double)S);
+    if (i < 0 || i > S) throw std::runtime_error("nodal point outside the unit domain");
+    k = k * (S + 1) + i;
+  }
+  return k;
+}
+
+int class_of_cond(int cond) {
+  if (cond == BC_DIRICHLET) return 10;
+  if (cond < 1 || cond > kNBoundTypes) throw std::runtime_error("invalid boundary condition code");
+  return 10 - cond;   // NEUMANN (1) -> 9 ... DIRICHLET_WEAK (9) -> 1
+}
+
+}  // namespace
+
+FESpace build_space(const Mesh& m, int level, int order, const BoundaryConditions& bc) {
+  if (level < 0 || level >= (int)m.levels.size()) throw std::runtime_error("mesh level out of range");
+  FESpace s;
+  s.mesh = &m;
+  s.level = level;
+  s.el = make_element(m.shape, order);
+  s.n_cells = m.levels[level].n_cells;
+  s.ndof_cell = s.el.ndof;
+  const int nd = s.ndof_cell, dim = m.dim;
+  const int64_t S = lattice_size(m, level, order);
+  const int64_t n_keys = (dim == 3) ? (S + 1) * (S + 1) * (S + 1) : (S + 1) * (S + 1);
+
+  // nodal point key of every (cell, local dof)
+  std::vector<int64_t> key((size_t)s.n_cells * nd);
+  std::vector<double> xloc((size_t)s.n_cells * nd * 3, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < s.n_cells; ++c) {
+    const AffineMap a = affine_map(m, level, c);
+    for (int j = 0; j < nd; ++j) {
+      double x[3] = {0, 0, 0};
+      map_to_phys(a, &s.el.ref_nodes[j * dim], x);
+      key[(size_t)c * nd + j] = lattice_key(x, dim, S);
+      for (int d = 0; d < 3; ++d) xloc[((size_t)c * nd + j) * 3 + d] = x[d];
+    }
+  }
+
+  // class of each nodal point: highest class of the boundary joints it lies on
+  std::vector<uint8_t> cls(n_keys, 0);
+  std::vector<int> last_comp(dim == 2 ? n_keys : 0, -1);
+  std::vector<std::vector<int>> jdofs(m.n_faces);
+  for (int j = 0; j < m.n_faces; ++j) jdofs[j] = s.el.joint_dofs(j);
+  for (int c = 0; c < s.n_cells; ++c) {
+    for (int j = 0; j < m.n_faces; ++j) {
+      const int comp = joint_boundary_component(m, level, c, j);
+      if (comp < 0) continue;
+      int vv[4];
+      const int nvj = joint_vertices(m.shape, j, vv);
+      int cond;
+      if (dim == 2) {
+        if (!bc.cond2d) throw std::runtime_error("2D space needs a BoundCondFunct2D");
+        const double* a = m.vertex(level, c, vv[0]);
+        const double* b = m.vertex(level, c, vv[1]);
+        double t0 = boundary_param_2d(comp, a[0], a[1]);
+        double t1 = boundary_param_2d(comp, b[0], b[1]);
+        const double eps = 1e-6;   // FESpace2D.C:1284
+        int c0, c1;
+        if (t0 < t1) { c0 = bc.cond2d(comp, t0 + eps); c1 = bc.cond2d(comp, t1 - eps); }
+        else { c0 = bc.cond2d(comp, t0 - eps); c1 = bc.cond2d(comp, t1 + eps); }
+        if (c0 != c1) throw std::runtime_error("different boundary condition on one edge are not allowed");
+        cond = c0;
+      } else {
+        if (!bc.cond3d) throw std::runtime_error("3D space needs a BoundCondFunct3D");
+        double X = 0, Y = 0, Z = 0;
+        for (int k = 0; k < nvj; ++k) {
+          const double* p = m.vertex(level, c, vv[k]);
+          X += p[0]; Y += p[1]; Z += p[2];
+        }
+        X /= nvj; Y /= nvj; Z /= nvj;
+        cond = bc.cond3d(X, Y, Z);
+      }
+      const uint8_t pcls = (uint8_t)class_of_cond(cond);
+      for (int ld : jdofs[j]) {
+        const int64_t k = key[(size_t)c * nd + ld];
+        if (pcls > cls[k]) cls[k] = pcls;
+        if (dim == 2 && cond == BC_DIRICHLET) last_comp[k] = comp;
+      }
+    }
+  }
+
+  // first-occurrence numbering per class
+  std::vector<int> tmp(n_keys, -1);
+  int count[11] = {0};
+  std::vector<int> first_occ;   // (cell*nd + j) of first occurrence, in order of temporary id
+  s.global_numbers.resize((size_t)s.n_cells * nd);
+  for (size_t i = 0; i < key.size(); ++i) {
+    const int64_t k = key[i];
+    if (tmp[k] < 0) tmp[k] = count[cls[k]]++;
+    s.global_numbers[i] = tmp[k];
+  }
+  s.n_inner = count[0];
+  int offset[11];
+  offset[0] = 0;
+  int acc = count[0];
+  for (int t = 1; t <= kNBoundTypes; ++t) {
+    // boundary type t (NEUMANN = 1) has class 10 - t
+    s.n_bound[t - 1] = count[10 - t];
+    offset[10 - t] = acc;
+    acc += count[10 - t];
+  }
+  s.active_bound = acc;
+  s.n_dirichlet = count[10];
+  offset[10] = acc;
+  s.n_dof = acc + count[10];
+
+  s.dof_xyz.assign((size_t)s.n_dof * 3, 0.0);
+  s.dof_class.assign(s.n_dof, 0);
+  if (dim == 2) { s.dof_comp.assign(s.n_dof, -1); s.dof_param.assign(s.n_dof, 0.0); }
+  for (size_t i = 0; i < key.size(); ++i) {
+    const int64_t k = key[i];
+    const int g = s.global_numbers[i] + offset[cls[k]];
+    s.global_numbers[i] = g;
+    s.dof_class[g] = cls[k];
+    for (int d = 0; d < 3; ++d) s.dof_xyz[(size_t)g * 3 + d] = xloc[i * 3 + d];
+    if (dim == 2 && last_comp[k] >= 0) {
+      s.dof_comp[g] = last_comp[k];
+      s.dof_param[g] = boundary_param_2d(last_comp[k], xloc[i * 3], xloc[i * 3 + 1]);
+    }
+  }
+  return s;
+}
+
+CSRPattern build_structure(const FESpace& s) {
+  const int nd = s.ndof_cell;
+  CSRPattern p;
+  p.n_rows = p.n_cols = s.n_dof;
+  p.active_bound = s.active_bound;
+  // row -> incident cells
+  std::vector<int> inc_ptr(s.n_dof + 1, 0);
+  for (size_t i = 0; i < s.global_numbers.size(); ++i) inc_ptr[s.global_numbers[i] + 1]++;
+  for (int r = 0; r < s.n_dof; ++r) inc_ptr[r + 1] += inc_ptr[r];
+  std::vector<int> inc(inc_ptr[s.n_dof]);
+  {
+    std::vector<int> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+    for (int c = 0; c < s.n_cells; ++c)
+      for (int j = 0; j < nd; ++j) inc[fill[s.global_numbers[(size_t)c * nd + j]]++] = c;
+  }
+  std::vector<int> len(s.n_dof, 1);
+  std::vector<std::vector<int>> rows(s.n_dof);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int r = 0; r < s.active_bound; ++r) {
+    std::vector<int>& cols = rows[r];
+    cols.reserve((inc_ptr[r + 1] - inc_ptr[r]) * nd);
+    for (int q = inc_ptr[r]; q < inc_ptr[r + 1]; ++q) {
+      const int* dof = &s.global_numbers[(size_t)inc[q] * nd];
+      cols.insert(cols.end(), dof, dof + nd);
+    }
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    len[r] = (int)cols.size();
+  }
+  p.row_ptr.assign(s.n_dof + 1, 0);
+  for (int r = 0; r < s.n_dof; ++r) p.row_ptr[r + 1] = p.row_ptr[r] + len[r];
+  p.col.resize(p.row_ptr[s.n_dof]);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int r = 0; r < s.n_dof; ++r) {
+    if (r < s.active_bound) {
+      std::copy(rows[r].begin(), rows[r].end(), p.col.begin() + p.row_ptr[r]);
+      std::vector<int>().swap(rows[r]);
+    } else {
+      p.col[p.row_ptr[r]] = r;
+    }
+  }
+  return p;
+}
+
+void build_prolongation(const FESpace& coarse, const FESpace& fine, CSRPattern& P,
+                        std::vector<double>& vals) {
+  const Mesh& m = *fine.mesh;
+  if (coarse.mesh != fine.mesh || fine.level != coarse.level + 1)
+    throw std::runtime_error("prolongation needs consecutive levels of one mesh");
+  const int ndf = fine.ndof_cell, ndc = coarse.ndof_cell, dim = m.dim;
+  std::vector<std::vector<std::pair<int, double>>> rows(fine.n_dof);
+  std::vector<uint8_t> done(fine.n_dof, 0);
+  std::vector<double> v(ndc);
+  const MeshLevel& fl = m.levels[fine.level];
+  for (int fc = 0; fc < fine.n_cells; ++fc) {
+    const int pc = fl.parent[fc];
+    const AffineMap af = affine_map(m, fine.level, fc);
+    const AffineMap ac = affine_map(m, coarse.level, pc);
+    for (int k = 0; k < ndf; ++k) {
+      const int g = fine.global_numbers[(size_t)fc * ndf + k];
+      if (done[g]) continue;
+      done[g] = 1;
+      double x[3], xi[3];
+      map_to_phys(af, &fine.el.ref_nodes[k * dim], x);
+      map_to_ref(ac, x, xi);
+      coarse.el.eval(xi, v.data(), nullptr, nullptr);
+      auto& row = rows[g];
+      for (int l = 0; l < ndc; ++l) {
+        // basis values at refined nodal points are exact binary fractions;
+        // snap round-off of the inverse map
+        double val = std::nearbyint(v[l] * 1024.0) / 1024.0;
+        if (std::fabs(v[l] - val) > 1e-12) val = v[l];
+        if (val != 0.0) row.emplace_back(coarse.global_numbers[(size_t)pc * ndc + l], val);
+      }
+      std::sort(row.begin(), row.end());
+    }
+  }
+  P.n_rows = fine.n_dof;
+  P.n_cols = coarse.n_dof;
+  P.active_bound = fine.n_dof;
+  P.row_ptr.assign(fine.n_dof + 1, 0);
+  for (int r = 0; r < fine.n_dof; ++r) P.row_ptr[r + 1] = P.row_ptr[r] + (int)rows[r].size();
+  P.col.resize(P.row_ptr.back());
+  vals.resize(P.row_ptr.back());
+  for (int r = 0; r < fine.n_dof; ++r) {
+    int o = P.row_ptr[r];
+    for (auto& e : rows[r]) { P.col[o] = e.first; vals[o] = e.second; ++o; }
+  }
+}
+
+void transpose_csr(const CSRPattern& A, const std::vector<double>& av, CSRPattern& T,
+                   std::vector<double>& tv) {
+  T.n_rows = A.n_cols;
+  T.n_cols = A.n_rows;
+  T.active_bound = T.n_rows;
+  T.row_ptr.assign(T.n_rows + 1, 0);
+  for (int c : A.col) T.row_ptr[c + 1]++;
+  for (int r = 0; r < T.n_rows; ++r) T.row_ptr[r + 1] += T.row_ptr[r];
+  T.col.resize(A.col.size());
+  tv.resize(A.col.size());
+  std::vector<int> fill(T.row_ptr.begin(), T.row_ptr.end() - 1);
+  for (int r = 0; r < A.n_rows; ++r)
+    for (int q = A.row_ptr[r]; q < A.row_ptr[r + 1]; ++q) {
+      const int o = fill[A.col[q]]++;
+      T.col[o] = r;
+      tv[o] = av[q];
+    }
+}
+
+void color_cells(const FESpace& s, std::vector<int>& order, std::vector<int>& color_ptr) {
+  const Mesh& m = *s.mesh;
+  const int nc = s.n_cells, dim = m.dim;
+  std::vector<int> color(nc, -1);
+  int ncolors = 0;
+  if (m.shape == SHAPE_QUAD || m.shape == SHAPE_HEXA) {
+    // structured lattice: cells sharing a vertex differ by <= 1 in every index,
+    // so the parity of the cell index vector is a valid colouring
+    const double n = (double)(1 << s.level);
+    for (int c = 0; c < nc; ++c) {
+      double ctr[3] = {0, 0, 0};
+      for (int v = 0; v < m.nv; ++v)
+        for (int d = 0; d < dim; ++d) ctr[d] += m.vertex(s.level, c, v)[d] / m.nv;
+      int col = 0;
+      for (int d = 0; d < dim; ++d) col |= ((int)std::floor(ctr[d] * n) & 1) << d;
+      color[c] = col;
+    }
+    ncolors = 1 << dim;
+  } else {
+    // greedy colouring over the DOF -> cell incidence
+    std::vector<int> inc_ptr(s.n_dof + 1, 0);
+    for (int g : s.global_numbers) inc_ptr[g + 1]++;
+    for (int r = 0; r < s.n_dof; ++r) inc_ptr[r + 1] += inc_ptr[r];
+    std::vector<int> inc(inc_ptr.back());
+    std::vector<int> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+    for (int c = 0; c < nc; ++c)
+      for (int j = 0; j < s.ndof_cell; ++j) inc[fill[s.global_numbers[(size_t)c * s.ndof_cell + j]]++] = c;
+    std::vector<int> mark(64, -1);
+    for (int c = 0; c < nc; ++c) {
+      for (int j = 0; j < s.ndof_cell; ++j) {
+        const int g = s.global_numbers[(size_t)c * s.ndof_cell + j];
+        for (int q = inc_ptr[g]; q < inc_ptr[g + 1]; ++q)
+          if (color[inc[q]] >= 0) mark[color[inc[q]]] = c;
+      }
+      int col = 0;
+      while (mark[col] == c) ++col;
+      color[c] = col;
+      ncolors = std::max(ncolors, col + 1);
+    }
+  }
+  color_ptr.assign(ncolors + 1, 0);
+  for (int c = 0; c < nc; ++c) color_ptr[color[c] + 1]++;
+  for (int k = 0; k < ncolors; ++k) color_ptr[k + 1] += color_ptr[k];
+  order.resize(nc);
+  std::vector<int> fill(color_ptr.begin(), color_ptr.end() - 1);
+  for (int c = 0; c < nc; ++c) order[fill[color[c]]++] = c;
+}
+
+}  // namespace pmx

@@ -1,0 +1,85 @@
+// FE space (DOF numbering), CSR sparsity pattern and grid-transfer matrices.
+//
+// DOF numbering restates TFESpace3D::ConstructSpace (src/FE/FESpace3D.C:506-2080)
+// and TFESpace2D::ConstructSpace (src/FE/FESpace2D.C:1284-2000).  The
+// reference links coinciding local DOFs of neighbouring cells by a
+// union-find whose root is always the lowest (cell, local dof) index
+// (TFE3DMapper::MapDOF, src/FE/FE3DMapper.C:143-220) and marks each root with a
+// class counter; the highest class wins (Dirichlet > Neumann > Robin > ... >
+// inner).  Its renumbering pass (src/FE/FESpace3D.C:1830-1905) then walks the
+// cells in collection order and local DOFs in order, and hands out the next
+// number of a DOF's class at the DOF's first occurrence; the classes are laid
+// out inner, Neumann, Robin, ..., hanging, Dirichlet (src/FE/FESpace3D.C:1923-1990).
+// This file computes the same result directly: coinciding DOFs are
+// identified by their nodal point, each DOF's class is the highest class of
+// the boundary joints it lies on, and numbers are issued at first occurrence.
+//
+// The CSR pattern restates TStructure(FESpace) (src/Matrix/Structure.C:829-1277):
+// active rows hold every DOF that shares a cell with the row DOF, ascending;
+// Dirichlet rows (the last rows) hold only their diagonal.
+#pragma once
+#include <cstdint>
+#include <functional>
+#include <vector>
+
+#include "fe.hpp"
+#include "mesh.hpp"
+
+namespace pmx {
+
+enum BoundCond : int {
+  BC_DIRICHLET = 0, BC_NEUMANN = 1, BC_ROBIN = 2, BC_SLIP = 3, BC_FREESURF = 4,
+  BC_SLIP_FRICTION = 5, BC_INTERFACE = 6, BC_SUBDOMAIN_INTERFACE = 7,
+  BC_SUBDOMAIN_HALOBOUND = 8, BC_DIRICHLET_WEAK = 9
+};
+constexpr int kNBoundTypes = 9;   // non-Dirichlet boundary classes
+
+struct BoundaryConditions {
+  std::function<int(int comp, double t)> cond2d;                 // BoundCondFunct2D
+  std::function<int(double x, double y, double z)> cond3d;       // BoundCondFunct3D
+};
+
+struct FESpace {
+  const Mesh* mesh = nullptr;
+  int level = 0;
+  Element el;
+  int n_cells = 0;
+  int ndof_cell = 0;
+  std::vector<int> global_numbers;   // [n_cells][ndof_cell]
+  int n_dof = 0;
+  int n_inner = 0;
+  int n_bound[kNBoundTypes] = {0};
+  int n_dirichlet = 0;
+  int active_bound = 0;
+  std::vector<double> dof_xyz;       // [n_dof][3] nodal points
+  std::vector<int> dof_comp;         // 2D: boundary component of the last Dirichlet joint (or -1)
+  std::vector<double> dof_param;     // 2D: parameter on that component
+  std::vector<uint8_t> dof_class;    // 0 inner, 1..9 boundary types, 10 Dirichlet
+};
+
+FESpace build_space(const Mesh& m, int level, int order, const BoundaryConditions& bc);
+
+struct CSRPattern {
+  int n_rows = 0, n_cols = 0;
+  int active_bound = 0;
+  std::vector<int> row_ptr, col;
+};
+
+// Square FE matrix pattern of a space.
+CSRPattern build_structure(const FESpace& s);
+
+// Interpolation matrix from the coarse space (mesh level l) to the fine space
+// (level l+1): P[f][c] = phi_c(x_f).  Prolongation of GridTransfer::Prolongate
+// (src/Multigrid/GridTransfer.C:665-941) equals P*u (the reference averages
+// identical per-child values by multiplicity), and DefectRestriction
+// (src/Multigrid/GridTransfer.C:943-1230) equals P^T*d.
+void build_prolongation(const FESpace& coarse, const FESpace& fine, CSRPattern& P,
+                        std::vector<double>& vals);
+void transpose_csr(const CSRPattern& A, const std::vector<double>& av, CSRPattern& T,
+                   std::vector<double>& tv);
+
+// Greedy colouring of the cells such that no two cells of one colour share a
+// DOF (used by the device scatter).  Returns colour offsets into `order`.
+void color_cells(const FESpace& s, std::vector<int>& order, std::vector<int>& color_ptr);
+
+}  // namespace pmx

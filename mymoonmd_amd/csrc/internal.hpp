@@ -1,0 +1,126 @@
+// Internal definitions shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pmx.h"
+#include "hip/launch.hpp"
+#include "host/fe.hpp"
+#include "host/mesh.hpp"
+#include "host/space.hpp"
+
+namespace pmx_internal {
+
+extern thread_local std::string g_last_error;
+
+struct PmxError : std::runtime_error {
+  int code;
+  PmxError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw pmx_internal::PmxError(PMX_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return PMX_OK;
+  } catch (const PmxError& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "out of host memory";
+    return PMX_E_NOMEM;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return PMX_E_ARG;
+  }
+}
+
+inline void require(bool cond, const char* msg) {
+  if (!cond) throw PmxError(PMX_E_ARG, msg);
+}
+
+template <typename T>
+T* dmalloc(size_t n) {
+  if (n == 0) n = 1;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, n * sizeof(T));
+  if (e != hipSuccess) throw PmxError(PMX_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return (T*)p;
+}
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+template <typename T>
+T* upload(const T* h, size_t n, hipStream_t st) {
+  T* d = dmalloc<T>(n);
+  if (n) HIP_CHECK(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  return d;
+}
+
+}  // namespace pmx_internal
+
+struct pmx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double* partial = nullptr;   // reduction scratch (>= 1024 doubles)
+  double* scal = nullptr;      // device scalars
+};
+
+struct pmx_domain {
+  pmx::Mesh mesh;
+};
+
+struct RefTable {
+  int nq = 0;
+  double* d_ref = nullptr;
+  std::vector<double> h_ref;
+  pmx::QuadRule rule;
+};
+
+struct pmx_space {
+  pmx_ctx* ctx = nullptr;
+  const pmx_domain* dom = nullptr;
+  pmx::FESpace fs;
+  int builtin_bc = 0;
+  pmx_bc2d_fn bc2d = nullptr;
+  pmx_bc3d_fn bc3d = nullptr;
+  void* user = nullptr;
+  std::shared_ptr<pmx::CSRPattern> pattern;
+  // device data
+  int* d_dofs = nullptr;
+  double* d_geo = nullptr;
+  int* d_cells = nullptr;          // cells ordered by colour
+  std::vector<int> color_ptr;
+  double* d_dir_xyz = nullptr;     // nodal points of the Dirichlet DOFs
+  std::map<int, RefTable> ref;     // keyed by hexa quad rule
+  ~pmx_space() {
+    pmx_internal::dfree(d_dofs); pmx_internal::dfree(d_geo); pmx_internal::dfree(d_cells); pmx_internal::dfree(d_dir_xyz);
+    for (auto& kv : ref) pmx_internal::dfree(kv.second.d_ref);
+  }
+};
+
+struct pmx_matrix {
+  pmx_ctx* ctx = nullptr;
+  const pmx_space* space = nullptr;
+  std::shared_ptr<pmx::CSRPattern> pattern;
+  pmx::DevCSR A;
+  ~pmx_matrix() {
+    pmx_internal::dfree(A.row_ptr); pmx_internal::dfree(A.col); pmx_internal::dfree(A.val); pmx_internal::dfree(A.val_bf16); pmx_internal::dfree(A.rscale);
+  }
+};
+

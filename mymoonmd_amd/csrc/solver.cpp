@@ -1,0 +1,647 @@
+// Krylov solvers and geometric multigrid of the solve path (C ABI part).
+//
+// Algorithms restated from the reference:
+//  * IterativeMethod::converged (include/Solver/IterativeMethod.h:81-139)
+//  * Iteration_gmres::flexible_gmres (src/Solver/Iteration_gmres.C:374-532) with
+//    modified Gram-Schmidt, Givens rotations (:123-150) and the back solve (:151-168)
+//  * Iteration_cg::iterate (src/Solver/Iteration_cg.C:22-79)
+//  * Iteration_richardson::iterate (src/Solver/Iteration_richardson.C:28-117)
+//  * Iteration_jacobi::iterate/apply (src/Solver/Iteration_jacobi.C:31-108)
+//  * Multigrid::cycle / cycle_step / update_rhs_in_coarser_grid /
+//    update_solution_in_finer_grid (src/Multigrid/Multigrid.C:157-415) driven by
+//    the CycleControl direction sequence (src/Multigrid/CycleControl.C:17-111)
+//  * JacobiSmoother::smooth (src/Multigrid/JacobiSmoother.C:22-30)
+//  * DirectSmoother (src/Multigrid/DirectSmoother.C:25-42): the coarsest
+//    system is solved exactly, here through its dense inverse.
+// The vectors live on the GPU; the loops run on the host and read back only
+// scalars (residual norms and one Hessenberg column per FGMRES iteration).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace pmx_internal;
+
+namespace {
+
+enum Dir { DIR_DUMMY = 0, DIR_DOWN = 1, DIR_UP = 2, DIR_END = 3 };
+
+void fill_recursively(std::vector<int>& rec, int level, int n_levels, int cycle, std::vector<Dir>& cc) {
+  if (level == 0) { cc.push_back(DIR_UP); return; }
+  for (int j = 0; j < rec[level]; ++j) {
+    cc.push_back(DIR_DOWN);
+    fill_recursively(rec, level - 1, n_levels, cycle, cc);
+  }
+  if (cycle == PMX_CYCLE_F) rec[level] = 1;
+  cc.push_back(level == n_levels - 1 ? DIR_END : DIR_UP);
+}
+
+std::vector<Dir> cycle_control(int cycle, int n_levels) {
+  std::vector<int> rec(n_levels, cycle == PMX_CYCLE_V ? 1 : 2);
+  rec[n_levels - 1] = 1;
+  std::vector<Dir> cc{DIR_DUMMY};
+  fill_recursively(rec, n_levels - 1, n_levels, cycle, cc);
+  return cc;
+}
+
+void free_csr(pmx::DevCSR& M) {
+  dfree(M.row_ptr); dfree(M.col); dfree(M.val); dfree(M.val_bf16); dfree(M.rscale);
+}
+
+pmx::DevCSR upload_csr(const pmx::CSRPattern& p, const std::vector<double>& v, hipStream_t st) {
+  pmx::DevCSR M;
+  M.n = p.n_rows;
+  M.m = p.n_cols;
+  M.nnz = (int)p.col.size();
+  M.active = p.active_bound;
+  M.lpr = 8;
+  M.row_ptr = upload(p.row_ptr.data(), p.row_ptr.size(), st);
+  M.col = upload(p.col.data(), p.col.size(), st);
+  M.val = upload(v.data(), v.size(), st);
+  return M;
+}
+
+struct Level {
+  pmx::DevCSR A;             // matrix view: fp64 arrays owned by pmx_matrix, bf16 copy owned here
+  int n = 0, active = 0;
+  double* dinv = nullptr;
+  float* dinv32 = nullptr;
+  pmx::DevCSR P;             // prolongation to the next finer level (rows: finer DOFs)
+  pmx::DevCSR R;             // restriction from the next finer level (rows: this level)
+  pmx::DevCSR Dinv;          // dense inverse (direct coarse solve), stored as full CSR rows
+  double *rhs = nullptr, *sol = nullptr, *def = nullptr, *tmp = nullptr;
+  float *rhs32 = nullptr, *sol32 = nullptr, *def32 = nullptr, *tmp32 = nullptr;
+  double* partial = nullptr;
+  void release() {
+    dfree(dinv); dfree(dinv32);
+    free_csr(P); free_csr(R); free_csr(Dinv);
+    dfree(rhs); dfree(sol); dfree(def); dfree(tmp);
+    dfree(rhs32); dfree(sol32); dfree(def32); dfree(tmp32);
+    dfree(partial);
+    dfree(A.val_bf16); dfree(A.rscale);
+  }
+};
+
+}  // namespace
+
+struct pmx_solver {
+  pmx_ctx* ctx = nullptr;
+  pmx_solver_params p{};
+  std::vector<Level> lv;     // coarsest first
+  std::vector<Dir> cc;
+  pmx::DevCSR A;             // finest operator (view)
+  int n = 0;
+  std::vector<double*> V, Z;
+  double *r = nullptr, *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *dinv = nullptr;
+  double* d_H = nullptr;
+  double* d_scal = nullptr;
+  double* partial = nullptr;
+  std::vector<double> history;
+  double initial_res = 0, previous_res = 0;
+  bool updated = false;
+  void release() {
+    for (auto& l : lv) l.release();
+    lv.clear();
+    for (auto*& v : V) dfree(v);
+    for (auto*& z : Z) dfree(z);
+    V.clear(); Z.clear();
+    dfree(r); dfree(w1); dfree(w2); dfree(w3); dfree(dinv); dfree(d_H); dfree(d_scal); dfree(partial);
+    updated = false;
+  }
+  ~pmx_solver() { release(); }
+};
+
+namespace {
+
+hipStream_t S(pmx_solver* s) { return s->ctx->stream; }
+
+double read_scalar(pmx_solver* s, const double* d) {
+  HIP_CHECK(hipGetLastError());
+  double h = 0;
+  HIP_CHECK(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, S(s)));
+  HIP_CHECK(hipStreamSynchronize(S(s)));
+  return h;
+}
+
+// IterativeMethod::converged
+bool converged(pmx_solver* s, double res, int it) {
+  const pmx_solver_params& p = s->p;
+  if (it == 0) { s->initial_res = res; s->previous_res = res; }
+  s->history.push_back(res);
+  const bool small = res < p.residual_tolerance;
+  const bool reduced = res < p.residual_reduction * s->initial_res;
+  const bool enough = it > p.min_n_iterations || p.min_n_iterations == 0;
+  const bool stagnation = res > 1e10 * s->previous_res;
+  const bool diverged = res > p.divergence_factor * s->initial_res;
+  bool conv = (small || reduced || stagnation || diverged) && enough;
+  if (it == p.max_n_iterations) conv = true;
+  s->previous_res = res;
+  if (diverged) throw PmxError(PMX_E_DIVERGED, "Execution of the code stopped because of solver divergence.");
+  return conv;
+}
+
+// ---------------------------------------------------------------- multigrid
+void mg_smooth(pmx_solver* s, int l, bool mixed) {
+  Level& L = s->lv[l];
+  if (l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) {
+    if (mixed) pmx::csr_spmv_f32(L.Dinv, L.rhs32, L.sol32, 1.0, 0.0, L.n, 0, S(s));
+    else pmx::csr_spmv(L.Dinv, L.rhs, L.sol, 1.0, 0.0, L.n, 0, S(s));
+    return;
+  }
+  if (mixed) {
+    pmx::fe_jacobi_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp, S(s));
+    std::swap(L.sol32, L.tmp32);
+  } else {
+    pmx::fe_jacobi(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s));
+    std::swap(L.sol, L.tmp);
+  }
+}
+
+void mg_defect(pmx_solver* s, int l, bool mixed, double* norm_out) {
+  Level& L = s->lv[l];
+  if (mixed) pmx::fe_defect_bf16(L.A, L.sol32, L.rhs32, L.def32, L.partial, norm_out, S(s));
+  else pmx::fe_defect(L.A, L.sol, L.rhs, L.def, L.partial, norm_out, S(s));
+}
+
+void mg_update_solution_in_finer_grid(pmx_solver* s, int l, bool mixed) {
+  Level& C = s->lv[l];
+  Level& F = s->lv[l + 1];
+  const double damp = s->p.mg_correction_damp;
+  if (mixed) pmx::csr_spmv_f32(C.P, C.sol32, F.sol32, damp, 1.0, F.active, 0, S(s));
+  else pmx::csr_spmv(C.P, C.sol, F.sol, damp, 1.0, F.active, 0, S(s));
+}
+
+void mg_update_rhs_in_coarser_grid(pmx_solver* s, int l, bool mixed) {
+  Level& F = s->lv[l];
+  Level& C = s->lv[l - 1];
+  if (mixed) pmx::csr_spmv_f32(C.R, F.def32, C.rhs32, 1.0, 0.0, C.active, 1, S(s));
+  else pmx::csr_spmv(C.R, F.def, C.rhs, 1.0, 0.0, C.active, 1, S(s));
+}
+
+int mg_cycle_step(pmx_solver* s, int step, int level, bool mixed) {
+  const int nl = (int)s->lv.size();
+  if (level == 0) {
+    // the initial defect only feeds verbose output in the reference
+    double res = 1e10;
+    for (int i = 0; i < s->p.mg_coarse_max_n_iterations && res > s->p.mg_coarse_residual; ++i) {
+      mg_smooth(s, 0, mixed);
+      mg_defect(s, 0, mixed, s->d_scal);
+      res = read_scalar(s, s->d_scal);
+    }
+    if (nl > 1) mg_update_solution_in_finer_grid(s, 0, mixed);
+    return 1;
+  }
+  const bool from_below = s->cc[step - 1] == DIR_UP;
+  const bool up = s->cc[step] == DIR_UP;
+  const bool down = s->cc[step] == DIR_DOWN;
+  if (from_below)
+    for (int i = 0; i < s->p.mg_n_post_smooth; ++i) mg_smooth(s, level, mixed);
+  if (down) {
+    for (int i = 0; i < s->p.mg_n_pre_smooth; ++i) mg_smooth(s, level, mixed);
+    mg_defect(s, level, mixed, nullptr);
+    Level& C = s->lv[level - 1];
+    if (mixed) pmx::vec_fill_f32(C.n, 0.f, C.sol32, S(s));
+    else pmx::vec_fill(C.n, 0.0, C.sol, S(s));
+    mg_update_rhs_in_coarser_grid(s, level, mixed);
+    return level - 1;
+  } else if (up) {
+    mg_update_solution_in_finer_grid(s, level, mixed);
+    return level + 1;
+  }
+  return -1;
+}
+
+// Multigrid::cycle: finest solution = 0 with the rhs' non-active entries
+void mg_cycle(pmx_solver* s, bool mixed) {
+  Level& F = s->lv.back();
+  if (mixed) {
+    pmx::vec_fill_f32(F.active, 0.f, F.sol32, S(s));
+    if (F.n > F.active)
+      HIP_CHECK(hipMemcpyAsync(F.sol32 + F.active, F.rhs32 + F.active, (F.n - F.active) * sizeof(float),
+                               hipMemcpyDeviceToDevice, S(s)));
+  } else {
+    pmx::vec_fill(F.active, 0.0, F.sol, S(s));
+    if (F.n > F.active)
+      HIP_CHECK(hipMemcpyAsync(F.sol + F.active, F.rhs + F.active, (F.n - F.active) * sizeof(double),
+                               hipMemcpyDeviceToDevice, S(s)));
+  }
+  int level = (int)s->lv.size() - 1;
+  const int n_steps = (int)s->cc.size() - 1;
+  for (int step = 1; step <= n_steps; ++step) level = mg_cycle_step(s, step, level, mixed);
+}
+
+// Iteration_multigrid::apply: z = MG(v)
+void mg_apply(pmx_solver* s, const double* v, double* z) {
+  Level& F = s->lv.back();
+  if (s->p.precision == PMX_PRECISION_MIXED) {
+    pmx::vec_f64_to_f32(F.n, v, F.rhs32, S(s));
+    mg_cycle(s, true);
+    pmx::vec_f32_to_f64(F.n, F.sol32, z, S(s));
+  } else {
+    HIP_CHECK(hipMemcpyAsync(F.rhs, v, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+    mg_cycle(s, false);
+    HIP_CHECK(hipMemcpyAsync(z, F.sol, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+  }
+}
+
+// Dense inverse of the coarsest matrix by Gauss-Jordan elimination with
+// partial pivoting (set-up only; the coarse solve is then one dense matvec).
+void build_coarse_inverse(pmx_solver* s, const pmx_matrix* M) {
+  Level& L = s->lv[0];
+  const int n = L.n;
+  if (n > 6000) throw PmxError(PMX_E_ARG, "direct coarse solve supports at most 6000 DOFs");
+  std::vector<double> vals(M->A.nnz);
+  HIP_CHECK(hipMemcpyAsync(vals.data(), M->A.val, vals.size() * sizeof(double), hipMemcpyDeviceToHost, S(s)));
+  HIP_CHECK(hipStreamSynchronize(S(s)));
+  const auto& pat = *M->pattern;
+  std::vector<double> a((size_t)n * n, 0.0), inv((size_t)n * n, 0.0);
+  for (int r = 0; r < n; ++r) {
+    for (int k = pat.row_ptr[r]; k < pat.row_ptr[r + 1]; ++k) a[(size_t)r * n + pat.col[k]] = vals[k];
+    inv[(size_t)r * n + r] = 1.0;
+  }
+  for (int k = 0; k < n; ++k) {
+    int piv = k;
+    double best = std::fabs(a[(size_t)k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (std::fabs(a[(size_t)i * n + k]) > best) { best = std::fabs(a[(size_t)i * n + k]); piv = i; }
+    if (best == 0.0) throw PmxError(PMX_E_ZERO_DIAG, "singular coarse matrix in direct coarse solve");
+    if (piv != k)
+      for (int j = 0; j < n; ++j) {
+        std::swap(a[(size_t)k * n + j], a[(size_t)piv * n + j]);
+        std::swap(inv[(size_t)k * n + j], inv[(size_t)piv * n + j]);
+      }
+    const double d = 1.0 / a[(size_t)k * n + k];
+    for (int j = 0; j < n; ++j) { a[(size_t)k * n + j] *= d; inv[(size_t)k * n + j] *= d; }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      if (i == k) continue;
+      const double f = a[(size_t)i * n + k];
+      if (f == 0.0) continue;
+      double* ai = &a[(size_t)i * n];
+      double* ii = &inv[(size_t)i * n];
+      const double* ak = &a[(size_t)k * n];
+      const double* ik = &inv[(size_t)k * n];
+      for (int j = 0; j < n; ++j) { ai[j] -= f * ak[j]; ii[j] -= f * ik[j]; }
+    }
+  }
+  pmx::CSRPattern dp;
+  dp.n_rows = dp.n_cols = dp.active_bound = n;
+  dp.row_ptr.resize(n + 1);
+  dp.col.resize((size_t)n * n);
+  for (int r = 0; r <= n; ++r) dp.row_ptr[r] = r * n;
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) dp.col[(size_t)r * n + c] = c;
+  L.Dinv = upload_csr(dp, inv, S(s));
+  L.Dinv.lpr = 64;
+}
+
+template <typename T>
+T* dzeros(size_t n, hipStream_t st) {
+  T* p = dmalloc<T>(n);
+  HIP_CHECK(hipMemsetAsync(p, 0, n * sizeof(T), st));
+  return p;
+}
+
+}  // namespace
+
+namespace {
+
+void precondition(pmx_solver* s, const double* v, double* z) {
+  if (s->p.preconditioner == PMX_PC_MULTIGRID) mg_apply(s, v, z);
+  else if (s->p.preconditioner == PMX_PC_JACOBI) pmx::vec_mul(s->n, s->dinv, v, z, S(s));
+  else HIP_CHECK(hipMemcpyAsync(z, v, s->n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+}
+
+void generate_rotation(double dx, double dy, double& cs, double& sn) {
+  if (dy == 0.0) { cs = 1.0; sn = 0.0; }
+  else if (std::fabs(dy) > std::fabs(dx)) {
+    const double t = dx / dy;
+    sn = 1.0 / std::sqrt(1.0 + t * t);
+    cs = t * sn;
+  } else {
+    const double t = dy / dx;
+    cs = 1.0 / std::sqrt(1.0 + t * t);
+    sn = t * cs;
+  }
+}
+void apply_rotation(double& dx, double& dy, double cs, double sn) {
+  const double t = cs * dx + sn * dy;
+  dy = -sn * dx + cs * dy;
+  dx = t;
+}
+
+// x += sum_k y_k z_k after the back solve with H (Update, Iteration_gmres.C:151-168)
+void gmres_update(pmx_solver* s, double* x, int k, const std::vector<double>& H, int ld,
+                  std::vector<double> y) {
+  for (int i = k; i >= 0; --i) {
+    y[i] /= H[i + (size_t)i * ld];
+    for (int j = i - 1; j >= 0; --j) y[j] -= H[j + (size_t)i * ld] * y[i];
+  }
+  for (int j = 0; j <= k; ++j) pmx::vec_axpy(s->n, y[j], nullptr, 1.0, s->Z[j], x, S(s));
+}
+
+void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep) {
+  const int n = s->n;
+  const int m = std::min(s->p.gmres_restart, s->p.max_n_iterations);
+  const int ld = m + 1;
+  std::vector<double> H((size_t)ld * m, 0.0), sv(m + 1), cs(m + 1), sn(m + 1);
+  // r = b - A x
+  pmx::fe_defect(s->A, x, b, s->r, s->partial, s->d_scal, S(s));
+  double beta = read_scalar(s, s->d_scal);
+  double resid = beta;
+  rep->initial_residual = beta;
+  if (converged(s, resid, 0)) { rep->iterations = 0; rep->final_residual = resid; rep->converged = 1; return; }
+  int j = 1;
+  while (j <= s->p.max_n_iterations) {
+    pmx::vec_scale(n, 1.0 / beta, nullptr, 0, s->r, s->V[0], S(s));
+    std::fill(sv.begin(), sv.end(), 0.0);
+    sv[0] = beta;
+    int i = 0;
+    for (i = 0; i < m && j <= s->p.max_n_iterations; ++i, ++j) {
+      precondition(s, s->V[i], s->Z[i]);
+      pmx::fe_apply(s->A, s->Z[i], s->r, S(s));
+      double* Hcol = s->d_H + (size_t)i * ld;
+      // modified Gram-Schmidt, pipelined: H(k,i) = <r, v_k>; r -= H(k,i) v_k
+      pmx::vec_dot(n, s->r, s->V[0], s->partial, Hcol, S(s));
+      for (int k = 0; k < i; ++k)
+        pmx::vec_axpy_dot(n, Hcol + k, s->V[k], s->r, s->V[k + 1], s->partial, Hcol + k + 1, S(s));
+      pmx::vec_axpy_dot(n, Hcol + i, s->V[i], s->r, nullptr, s->partial, Hcol + i + 1, S(s));
+      HIP_CHECK(hipMemcpyAsync(&H[(size_t)i * ld], Hcol, (i + 2) * sizeof(double), hipMemcpyDeviceToHost, S(s)));
+      HIP_CHECK(hipStreamSynchronize(S(s)));
+      const double hn = H[(i + 1) + (size_t)i * ld];
+      if (hn == 0.0) throw PmxError(PMX_E_BREAKDOWN, "Unhappy breakdown in flexible gmres");
+      pmx::vec_scale(n, 0.0, Hcol + i + 1, 1, s->r, s->V[i + 1], S(s));
+      for (int k = 0; k < i; ++k)
+        apply_rotation(H[k + (size_t)i * ld], H[k + 1 + (size_t)i * ld], cs[k], sn[k]);
+      generate_rotation(H[i + (size_t)i * ld], H[i + 1 + (size_t)i * ld], cs[i], sn[i]);
+      apply_rotation(H[i + (size_t)i * ld], H[i + 1 + (size_t)i * ld], cs[i], sn[i]);
+      apply_rotation(sv[i], sv[i + 1], cs[i], sn[i]);
+      resid = std::fabs(sv[i + 1]);
+      if (converged(s, resid, j)) {
+        gmres_update(s, x, i, H, ld, sv);
+        HIP_CHECK(hipStreamSynchronize(S(s)));
+        rep->iterations = j; rep->final_residual = resid; rep->converged = 1;
+        return;
+      }
+    }
+    gmres_update(s, x, m - 1, H, ld, sv);
+    pmx::fe_defect(s->A, x, b, s->r, s->partial, s->d_scal, S(s));
+    beta = read_scalar(s, s->d_scal);
+    if (converged(s, resid, j)) {
+      rep->iterations = j; rep->final_residual = resid; rep->converged = 1;
+      return;
+    }
+  }
+  rep->iterations = s->p.max_n_iterations;
+  rep->final_residual = resid;
+  rep->converged = 0;
+}
+
+void solve_cg(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep) {
+  const int n = s->n;
+  double* r = s->r;
+  double* z = s->w1;
+  double* pv = s->w2;
+  double* q = s->w3;
+  pmx::fe_defect(s->A, x, b, r, s->partial, s->d_scal, S(s));
+  double rn = read_scalar(s, s->d_scal);
+  pmx::vec_norm(n, b, s->partial, s->d_scal, S(s));
+  double normb = read_scalar(s, s->d_scal);
+  if (normb == 0.0) normb = 1;
+  double resid = rn / normb;
+  rep->initial_residual = resid;
+  if (converged(s, resid, 0)) { rep->iterations = 0; rep->final_residual = resid; rep->converged = 1; return; }
+  double rho_1 = 0;
+  for (int i = 1; i <= s->p.max_n_iterations; ++i) {
+    precondition(s, r, z);
+    pmx::vec_dot(n, r, z, s->partial, s->d_scal, S(s));
+    const double rho = read_scalar(s, s->d_scal);
+    if (i == 1) HIP_CHECK(hipMemcpyAsync(pv, z, n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+    else {
+      const double beta = rho / rho_1;
+      pmx::vec_scale(n, beta, nullptr, 0, pv, pv, S(s));
+      pmx::vec_axpy(n, 1.0, nullptr, 1.0, z, pv, S(s));
+    }
+    pmx::fe_apply(s->A, pv, q, S(s));
+    pmx::vec_dot(n, pv, q, s->partial, s->d_scal, S(s));
+    const double alpha = rho / read_scalar(s, s->d_scal);
+    pmx::vec_axpy(n, alpha, nullptr, 1.0, pv, x, S(s));
+    pmx::vec_axpy(n, -alpha, nullptr, 1.0, q, r, S(s));
+    pmx::vec_norm(n, r, s->partial, s->d_scal, S(s));
+    resid = read_scalar(s, s->d_scal) / normb;
+    if (converged(s, resid, i)) { rep->iterations = i; rep->final_residual = resid; rep->converged = 1; return; }
+    rho_1 = rho;
+  }
+  rep->iterations = s->p.max_n_iterations; rep->final_residual = resid; rep->converged = 0;
+}
+
+// Richardson (preconditioned) and the plain Jacobi method
+void solve_richardson(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep, bool jacobi) {
+  const int n = s->n;
+  double* r = s->r;
+  double* z = s->w1;
+  pmx::vec_norm(n, b, s->partial, s->d_scal, S(s));
+  double normb = read_scalar(s, s->d_scal);
+  if (normb == 0.0) normb = 1;
+  pmx::fe_defect(s->A, x, b, r, s->partial, s->d_scal, S(s));
+  double resid = read_scalar(s, s->d_scal) / normb;
+  rep->initial_residual = resid;
+  if (converged(s, resid, 0)) { rep->iterations = 0; rep->final_residual = resid; rep->converged = 1; return; }
+  const double damp = s->p.damping;
+  for (int i = 1; i <= s->p.max_n_iterations; ++i) {
+    if (jacobi) pmx::vec_mul(n, s->dinv, r, z, S(s));
+    else precondition(s, r, z);
+    pmx::vec_axpy(n, damp, nullptr, 1.0, z, x, S(s));
+    pmx::fe_defect(s->A, x, b, r, s->partial, s->d_scal, S(s));
+    resid = read_scalar(s, s->d_scal) / normb;
+    if (converged(s, resid, i)) { rep->iterations = i; rep->final_residual = resid; rep->converged = 1; return; }
+  }
+  rep->iterations = s->p.max_n_iterations; rep->final_residual = resid; rep->converged = 0;
+}
+
+void check_zero_diag(pmx_solver* s, const pmx::DevCSR& A, double* dinv) {
+  int* flag = dmalloc<int>(1);
+  HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), S(s)));
+  pmx::csr_inv_diag(A, dinv, flag, S(s));
+  int h = 0;
+  HIP_CHECK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, S(s)));
+  HIP_CHECK(hipStreamSynchronize(S(s)));
+  dfree(flag);
+  if (h) throw PmxError(PMX_E_ZERO_DIAG, "zero entry on the diagonal using Jacobi solver/preconditioner");
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmx_solver_default_params(pmx_solver_params* p) {
+  return guarded([&] {
+    require(p != nullptr, "NULL argument");
+    pmx_solver_params d{};
+    d.iterative_solver_type = PMX_IT_FGMRES;
+    d.preconditioner = PMX_PC_NONE;
+    d.max_n_iterations = 100;
+    d.min_n_iterations = 0;
+    d.residual_tolerance = 1e-8;
+    d.residual_reduction = 0.0;
+    d.divergence_factor = 1e10;
+    d.gmres_restart = 20;
+    d.damping = 1.0;
+    d.mg_cycle = PMX_CYCLE_V;
+    d.mg_smoother = PMX_SMOOTHER_JACOBI;
+    d.mg_smoother_coarse = PMX_SMOOTHER_DIRECT;
+    d.mg_correction_damp = 1.0;
+    d.mg_n_pre_smooth = 1;
+    d.mg_n_post_smooth = 1;
+    d.mg_coarse_residual = 1e-1;
+    d.mg_coarse_max_n_iterations = 10;
+    d.jacobi_damp = 1.0;
+    d.precision = PMX_PRECISION_FP64;
+    *p = d;
+  });
+}
+
+int pmx_solver_create(pmx_ctx* ctx, const pmx_solver_params* p, pmx_solver** out) {
+  return guarded([&] {
+    require(ctx && p && out, "NULL argument");
+    require(p->iterative_solver_type >= 0 && p->iterative_solver_type <= 3, "unknown iterative solver");
+    require(p->preconditioner >= 0 && p->preconditioner <= 2, "unknown preconditioner");
+    require(p->max_n_iterations >= 0 && p->gmres_restart >= 1, "invalid iteration counts");
+    require(p->mg_smoother == PMX_SMOOTHER_JACOBI, "only the jacobi multigrid smoother is available");
+    require(p->mg_smoother_coarse == PMX_SMOOTHER_JACOBI || p->mg_smoother_coarse == PMX_SMOOTHER_DIRECT,
+            "unknown coarse smoother");
+    require(p->precision == PMX_PRECISION_FP64 || p->precision == PMX_PRECISION_MIXED, "unknown precision");
+    auto s = std::make_unique<pmx_solver>();
+    s->ctx = ctx;
+    s->p = *p;
+    if (s->p.min_n_iterations > s->p.max_n_iterations) s->p.min_n_iterations = s->p.max_n_iterations;
+    *out = s.release();
+  });
+}
+
+int pmx_solver_destroy(pmx_solver* s) {
+  return guarded([&] {
+    if (!s) return;
+    (void)hipStreamSynchronize(s->ctx->stream);
+    delete s;
+  });
+}
+
+int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space* const* spaces,
+                      int n_levels) {
+  return guarded([&] {
+    require(s && levels && n_levels >= 1, "invalid argument");
+    const bool mg = s->p.preconditioner == PMX_PC_MULTIGRID;
+    require(!mg || (spaces && n_levels >= 2), "multigrid needs >= 2 levels and their spaces");
+    require(mg || n_levels == 1, "without multigrid pass exactly one matrix");
+    s->release();
+    hipStream_t st = S(s);
+    const pmx_matrix* fine = levels[n_levels - 1];
+    require(fine != nullptr, "NULL matrix");
+    s->A = fine->A;
+    s->n = fine->A.n;
+    const int n = s->n;
+    s->partial = dmalloc<double>(std::max(4096, pmx::fe_defect_partials(s->A)));
+    s->d_scal = dmalloc<double>(64);
+    s->r = dzeros<double>(n, st);
+    s->w1 = dzeros<double>(n, st);
+    s->w2 = dzeros<double>(n, st);
+    s->w3 = dzeros<double>(n, st);
+    s->dinv = dmalloc<double>(n);
+    if (s->p.preconditioner == PMX_PC_JACOBI || s->p.iterative_solver_type == PMX_IT_JACOBI)
+      check_zero_diag(s, s->A, s->dinv);
+    if (s->p.iterative_solver_type == PMX_IT_FGMRES) {
+      const int m = std::min(s->p.gmres_restart, std::max(1, s->p.max_n_iterations));
+      for (int k = 0; k <= m; ++k) s->V.push_back(dzeros<double>(n, st));
+      for (int k = 0; k < m; ++k) s->Z.push_back(dzeros<double>(n, st));
+      s->d_H = dzeros<double>((size_t)(m + 1) * m + 16, st);
+    }
+    if (mg) {
+      const bool mixed = s->p.precision == PMX_PRECISION_MIXED;
+      s->lv.resize(n_levels);
+      for (int l = 0; l < n_levels; ++l) {
+        Level& L = s->lv[l];
+        const pmx_matrix* M = levels[l];
+        require(M != nullptr && spaces[l] != nullptr, "NULL level");
+        require(M->space == spaces[l], "matrix/space mismatch");
+        L.A = M->A;
+        L.A.val_bf16 = nullptr;
+        L.A.rscale = nullptr;
+        L.n = M->A.n;
+        L.active = M->A.active;
+        L.partial = dmalloc<double>(std::max(4096, pmx::fe_defect_partials(L.A)));
+        L.dinv = dmalloc<double>(L.n);
+        check_zero_diag(s, L.A, L.dinv);
+        if (mixed) {
+          L.rhs32 = dzeros<float>(L.n, st); L.sol32 = dzeros<float>(L.n, st);
+          L.def32 = dzeros<float>(L.n, st); L.tmp32 = dzeros<float>(L.n, st);
+          L.A.val_bf16 = dmalloc<uint16_t>(L.A.nnz);
+          L.A.rscale = dmalloc<float>(L.n);
+          pmx::csr_to_bf16(L.A, L.A.val_bf16, L.A.rscale, st);
+          L.dinv32 = dmalloc<float>(L.n);
+          pmx::vec_f64_to_f32(L.n, L.dinv, L.dinv32, st);
+        } else {
+          L.rhs = dzeros<double>(L.n, st); L.sol = dzeros<double>(L.n, st);
+          L.def = dzeros<double>(L.n, st); L.tmp = dzeros<double>(L.n, st);
+        }
+        if (l + 1 < n_levels) {
+          const pmx_space* cs = spaces[l];
+          const pmx_space* fsp = spaces[l + 1];
+          require(cs->dom == fsp->dom && fsp->fs.level == cs->fs.level + 1,
+                  "multigrid spaces must be consecutive levels of one domain");
+          pmx::CSRPattern P, R;
+          std::vector<double> pv, rv;
+          pmx::build_prolongation(cs->fs, fsp->fs, P, pv);
+          pmx::transpose_csr(P, pv, R, rv);
+          L.P = upload_csr(P, pv, st);
+          L.R = upload_csr(R, rv, st);
+        }
+      }
+      if (s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) build_coarse_inverse(s, levels[0]);
+      s->cc = cycle_control(s->p.mg_cycle, n_levels);
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    s->updated = true;
+  });
+}
+
+int pmx_solver_solve(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep) {
+  return guarded([&] {
+    require(s && b && x && rep, "NULL argument");
+    if (!s->updated) throw PmxError(PMX_E_STATE, "pmx_solver_update must be called before solve");
+    s->history.clear();
+    *rep = pmx_solve_report{};
+    switch (s->p.iterative_solver_type) {
+      case PMX_IT_FGMRES: solve_fgmres(s, b, x, rep); break;
+      case PMX_IT_CG: solve_cg(s, b, x, rep); break;
+      case PMX_IT_RICHARDSON: solve_richardson(s, b, x, rep, false); break;
+      case PMX_IT_JACOBI: solve_richardson(s, b, x, rep, true); break;
+    }
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(S(s)));
+  });
+}
+
+int pmx_solver_get_history(const pmx_solver* s, double* out, int max, int* n) {
+  return guarded([&] {
+    require(s && n, "NULL argument");
+    const int k = std::min<int>(max, (int)s->history.size());
+    if (out) for (int i = 0; i < k; ++i) out[i] = s->history[i];
+    *n = (int)s->history.size();
+  });
+}
+
+int pmx_solver_apply_preconditioner(pmx_solver* s, const double* r, double* z) {
+  return guarded([&] {
+    require(s && r && z, "NULL argument");
+    if (!s->updated) throw PmxError(PMX_E_STATE, "pmx_solver_update must be called first");
+    precondition(s, r, z);
+    HIP_CHECK(hipStreamSynchronize(S(s)));
+  });
+}
+
+}  // extern "C"

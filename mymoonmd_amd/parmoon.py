@@ -1,0 +1,355 @@
+"""Host-side mirror of the reference's interface for the solve path.
+
+Names and argument meaning follow ParMooN (nahmedwias/MyMooNMD):
+TDomain + refine_and_get_hierarchy_of_collections, TFESpace, FEMatrix /
+BlockFEMatrix::apply(_scaled_add), Solver (default_solver_database,
+update_matrix, solve) and ConvectionDiffusion (assemble, solve, output,
+get_L2_error ...).  Every operation calls libpmx.so through the C ABI of
+include/pmx.h; device vectors are torch tensors on the HIP device (torch is
+only used as the device-memory allocator).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (BC2D_FN, BC3D_FN, CDParams, SolveReport, SolverParams, SpaceInfo, call,
+                   c_double, c_int, c_void_p)
+
+GEOMETRY = {"UnitSquare": 0, "TwoTriangles": 1, "Default_UnitCube_Hexa": 2}
+EXAMPLES = {"const": 0, "sine_laplace_2d": 1, "p0_3d": 2, "p1_3d": 3, "p2_3d": 4,
+            "exp_sin_3d": 5, "sine_2d": 6}
+IT_SOLVER = {"fgmres": 0, "cg": 1, "richardson": 2, "jacobi": 3}
+PRECOND = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2}
+SMOOTHER = {"jacobi": 0, "direct_solve": 1}
+CYCLE = {"V": 0, "W": 1, "F": 2}
+BC_BUILTIN = {"all_dirichlet": 0, "sine_laplace": 1}
+
+
+def _destroy(name, h):
+    try:
+        getattr(_lib.lib(), name)(h)
+    except Exception:  # interpreter shutdown
+        pass
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """One HIP device + stream (pmx_ctx)."""
+
+    def __init__(self, device=0):
+        self.h = c_void_p()
+        call("pmx_ctx_create", device, ctypes.byref(self.h))
+        self.device = device
+
+    def synchronize(self):
+        call("pmx_ctx_synchronize", self.h)
+
+    def __del__(self, _d=_destroy):
+        if getattr(self, "h", None) and self.h.value:
+            _d("pmx_ctx_destroy", self.h)
+            self.h = None
+
+
+class TDomain:
+    """Default geometry refined `n_refine` times; level k = collection after k refinements."""
+
+    def __init__(self, geo_file="UnitSquare", n_refine=0):
+        self.h = c_void_p()
+        call("pmx_domain_create", GEOMETRY[geo_file], n_refine, ctypes.byref(self.h))
+        self.n_refine = n_refine
+
+    def level_info(self, level):
+        n, nv, dim = c_int(), c_int(), c_int()
+        call("pmx_domain_level_info", self.h, level, ctypes.byref(n), ctypes.byref(nv), ctypes.byref(dim))
+        return n.value, nv.value, dim.value
+
+    def vertices(self, level):
+        n, nv, dim = self.level_info(level)
+        out = np.zeros(n * nv * dim)
+        call("pmx_domain_get_vertices", self.h, level, out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out.reshape(n, nv, dim)
+
+    def __del__(self, _d=_destroy):
+        if getattr(self, "h", None) and self.h.value:
+            _d("pmx_domain_destroy", self.h)
+            self.h = None
+
+
+class TFESpace:
+    """Conforming Lagrange space (ConstructSpace numbering)."""
+
+    def __init__(self, domain, level, order, ctx=None, bc="all_dirichlet", bc2d=None, bc3d=None):
+        self.domain = domain
+        self.ctx = ctx
+        self.level = level
+        self._bc2d = BC2D_FN(bc2d) if bc2d else BC2D_FN()
+        self._bc3d = BC3D_FN(bc3d) if bc3d else BC3D_FN()
+        self.h = c_void_p()
+        call("pmx_space_create", ctx.h if ctx else None, domain.h, level, order, self._bc2d,
+             self._bc3d, None, BC_BUILTIN[bc], ctypes.byref(self.h))
+        inf = SpaceInfo()
+        call("pmx_space_get_info", self.h, ctypes.byref(inf))
+        self.info = inf
+        self.n_dof = inf.n_dof
+        self.active_bound = inf.active_bound
+        self.n_cells = inf.n_cells
+        self.ndof_cell = inf.ndof_cell
+
+    def global_numbers(self):
+        out = np.zeros(self.n_cells * self.ndof_cell, dtype=np.int32)
+        call("pmx_space_get_global_numbers", self.h, out.ctypes.data_as(ctypes.POINTER(c_int)))
+        return out.reshape(self.n_cells, self.ndof_cell)
+
+    def dof_coords(self):
+        out = np.zeros(self.n_dof * 3)
+        call("pmx_space_get_dof_coords", self.h, out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out.reshape(self.n_dof, 3)
+
+    def structure(self):
+        nnz = c_int()
+        call("pmx_space_get_structure", self.h, ctypes.byref(nnz), None, None)
+        rp = np.zeros(self.n_dof + 1, dtype=np.int32)
+        col = np.zeros(nnz.value, dtype=np.int32)
+        call("pmx_space_get_structure", self.h, ctypes.byref(nnz),
+             rp.ctypes.data_as(ctypes.POINTER(c_int)), col.ctypes.data_as(ctypes.POINTER(c_int)))
+        return rp, col
+
+    def prolongation_from(self, coarse):
+        nnz = c_int()
+        call("pmx_space_get_prolongation", coarse.h, self.h, ctypes.byref(nnz), None, None, None)
+        rp = np.zeros(self.n_dof + 1, dtype=np.int32)
+        col = np.zeros(nnz.value, dtype=np.int32)
+        val = np.zeros(nnz.value)
+        call("pmx_space_get_prolongation", coarse.h, self.h, ctypes.byref(nnz),
+             rp.ctypes.data_as(ctypes.POINTER(c_int)), col.ctypes.data_as(ctypes.POINTER(c_int)),
+             val.ctypes.data_as(ctypes.POINTER(c_double)))
+        return rp, col, val
+
+    def __del__(self, _d=_destroy):
+        if getattr(self, "h", None) and self.h.value:
+            _d("pmx_space_destroy", self.h)
+            self.h = None
+
+
+class FEMatrix:
+    """Square FE matrix on the device (TStructure pattern + values)."""
+
+    def __init__(self, ctx, space):
+        self.ctx = ctx
+        self.space = space
+        self.h = c_void_p()
+        call("pmx_matrix_create", ctx.h, space.h, ctypes.byref(self.h))
+        n, nnz, ab = c_int(), c_int(), c_int()
+        call("pmx_matrix_get_info", self.h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(ab))
+        self.n_rows, self.nnz, self.active_bound = n.value, nnz.value, ab.value
+
+    def reset(self):
+        call("pmx_matrix_reset", self.ctx.h, self.h)
+
+    def get_entries(self):
+        out = np.zeros(self.nnz)
+        call("pmx_matrix_get_entries", self.ctx.h, self.h, out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out
+
+    def set_entries(self, vals):
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        call("pmx_matrix_set_entries", self.ctx.h, self.h, vals.ctypes.data_as(ctypes.POINTER(c_double)))
+
+    def get_structure(self):
+        rp = np.zeros(self.n_rows + 1, dtype=np.int32)
+        col = np.zeros(self.nnz, dtype=np.int32)
+        call("pmx_matrix_get_structure", self.h, rp.ctypes.data_as(ctypes.POINTER(c_int)),
+             col.ctypes.data_as(ctypes.POINTER(c_int)))
+        return rp, col
+
+    def set_spmv_lanes(self, lanes):
+        call("pmx_matrix_set_spmv_lanes", self.h, lanes)
+
+    def apply(self, x, y):
+        """BlockFEMatrix::apply: y = A x (identity on non-active rows)."""
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_matrix_apply", self.ctx.h, self.h, _ptr(x), _ptr(y))
+        self.ctx.synchronize()
+
+    def apply_scaled_add(self, x, y, a=1.0):
+        """BlockFEMatrix::apply_scaled_add: y += a A x (y += a x on non-active rows)."""
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_matrix_apply_scaled_add", self.ctx.h, self.h, _ptr(x), _ptr(y), a)
+        self.ctx.synchronize()
+
+    def __del__(self, _d=_destroy):
+        if getattr(self, "h", None) and self.h.value:
+            _d("pmx_matrix_destroy", self.h)
+            self.h = None
+
+
+def cd_params(example="const", disc="galerkin", eps=1.0, b=(0.0, 0.0, 0.0), c=0.0, f0=0.0,
+              hexa_quad_rule=0, delta0=1.0, poly_degree=1.0):
+    p = CDParams()
+    p.disc = 1 if disc == "supg" else 0
+    p.example = EXAMPLES[example]
+    p.hexa_quad_rule = hexa_quad_rule
+    p.eps = eps
+    for k in range(3):
+        p.b[k] = b[k] if k < len(b) else 0.0
+    p.c = c
+    p.f0 = f0
+    p.delta0 = delta0
+    p.poly_degree = poly_degree
+    return p
+
+
+def assemble_cd(ctx, space, params, matrix, rhs):
+    """Assemble3D/Assemble2D + LocalAssembling(ConvDiff): adds into matrix and rhs."""
+    import torch
+    torch.cuda.synchronize()
+    call("pmx_assemble_cd", ctx.h, space.h, ctypes.byref(params), matrix.h, _ptr(rhs))
+    ctx.synchronize()
+
+
+def compute_errors(ctx, space, params, u):
+    import torch
+    torch.cuda.synchronize()
+    out = (c_double * 4)()
+    call("pmx_compute_errors", ctx.h, space.h, ctypes.byref(params), _ptr(u), out)
+    return list(out)
+
+
+def default_solver_database():
+    p = SolverParams()
+    call("pmx_solver_default_params", ctypes.byref(p))
+    return p
+
+
+def solver_params(**kw):
+    """Solver parameters by the reference's database keys."""
+    p = default_solver_database()
+    keymap = {"max_n_iterations": "max_n_iterations", "min_n_iterations": "min_n_iterations",
+              "residual_tolerance": "residual_tolerance", "residual_reduction": "residual_reduction",
+              "divergence_factor": "divergence_factor", "gmres_restart": "gmres_restart",
+              "damping": "damping", "multigrid_correction_damp_factor": "mg_correction_damp",
+              "multigrid_n_pre_smooth": "mg_n_pre_smooth", "multigrid_n_post_smooth": "mg_n_post_smooth",
+              "multigrid_coarse_residual": "mg_coarse_residual",
+              "multigrid_coarse_max_n_iterations": "mg_coarse_max_n_iterations",
+              "jacobi_damp": "jacobi_damp"}
+    for k, v in kw.items():
+        if k == "iterative_solver_type":
+            p.iterative_solver_type = IT_SOLVER[v]
+        elif k == "preconditioner":
+            p.preconditioner = PRECOND[v]
+        elif k == "multigrid_smoother":
+            p.mg_smoother = SMOOTHER[v]
+        elif k == "multigrid_smoother_coarse":
+            p.mg_smoother_coarse = SMOOTHER[v]
+        elif k == "multigrid_cycle_type":
+            p.mg_cycle = CYCLE[v]
+        elif k == "precision":
+            p.precision = {"fp64": 0, "mixed": 1}[v]
+        else:
+            setattr(p, keymap[k], v)
+    return p
+
+
+class Solver:
+    """Solver<BlockFEMatrix, BlockVector> with an iterative method (+ multigrid)."""
+
+    def __init__(self, ctx, params):
+        self.ctx = ctx
+        self.params = params
+        self.h = c_void_p()
+        call("pmx_solver_create", ctx.h, ctypes.byref(params), ctypes.byref(self.h))
+
+    def update_matrix(self, matrices, spaces=None):
+        """matrices/spaces of all multigrid levels, coarsest first (or one matrix)."""
+        if not isinstance(matrices, (list, tuple)):
+            matrices = [matrices]
+        mats = (c_void_p * len(matrices))(*[m.h.value for m in matrices])
+        sps = (c_void_p * len(matrices))(*[s.h.value for s in spaces]) if spaces else None
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_solver_update", self.h, mats, sps, len(matrices))
+
+    def solve(self, rhs, sol):
+        import torch
+        torch.cuda.synchronize()
+        rep = SolveReport()
+        call("pmx_solver_solve", self.h, _ptr(rhs), _ptr(sol), ctypes.byref(rep))
+        return rep
+
+    def history(self):
+        n = c_int()
+        call("pmx_solver_get_history", self.h, None, 0, ctypes.byref(n))
+        out = np.zeros(max(n.value, 1))
+        call("pmx_solver_get_history", self.h, out.ctypes.data_as(ctypes.POINTER(c_double)), n.value,
+             ctypes.byref(n))
+        return out[: n.value]
+
+    def apply_preconditioner(self, r, z):
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_solver_apply_preconditioner", self.h, _ptr(r), _ptr(z))
+
+    def __del__(self, _d=_destroy):
+        if getattr(self, "h", None) and self.h.value:
+            _d("pmx_solver_destroy", self.h)
+            self.h = None
+
+
+class ConvectionDiffusion:
+    """ConvectionDiffusion<d> (src/System/ConvectionDiffusion.C): one system per
+    multigrid level, assemble() / solve() / output() and the error getters."""
+
+    def __init__(self, ctx, domain, order, params, solver, n_mg_levels=1, bc="all_dirichlet",
+                 bc2d=None, bc3d=None):
+        import torch
+        self.ctx = ctx
+        self.params = params
+        nref = domain.n_refine
+        levels = list(range(nref - n_mg_levels + 1, nref + 1))
+        self.spaces = [TFESpace(domain, l, order, ctx, bc=bc, bc2d=bc2d, bc3d=bc3d) for l in levels]
+        self.matrices = [FEMatrix(ctx, s) for s in self.spaces]
+        dev = torch.device("cuda", ctx.device)
+        self.rhs = [torch.zeros(s.n_dof, dtype=torch.float64, device=dev) for s in self.spaces]
+        self.solution = torch.zeros(self.spaces[-1].n_dof, dtype=torch.float64, device=dev)
+        self.solver = Solver(ctx, solver)
+        self.errors = None
+        self.report = None
+
+    def assemble(self):
+        for s, A, f in zip(self.spaces, self.matrices, self.rhs):
+            A.reset()
+            f.zero_()
+            assemble_cd(self.ctx, s, self.params, A, f)
+        # solution.copy_nonactive(rhs)
+        ab = self.spaces[-1].active_bound
+        self.solution[ab:] = self.rhs[-1][ab:]
+
+    def solve(self):
+        if self.solver.params.preconditioner == PRECOND["multigrid"]:
+            self.solver.update_matrix(self.matrices, self.spaces)
+        else:
+            self.solver.update_matrix(self.matrices[-1])
+        self.report = self.solver.solve(self.rhs[-1], self.solution)
+        return self.report
+
+    def output(self):
+        self.errors = compute_errors(self.ctx, self.spaces[-1], self.params, self.solution)
+        return self.errors
+
+    def get_L2_error(self):
+        return self.errors[0]
+
+    def get_H1_semi_error(self):
+        return self.errors[1]
+
+    def get_SD_error(self):
+        return self.errors[2]
+
+    def get_L_inf_error(self):
+        return self.errors[3]
