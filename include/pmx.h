@@ -104,6 +104,11 @@ int pmx_space_get_prolongation(const pmx_space* coarse, const pmx_space* fine, i
  * (src/Matrix/Structure.C:829-1277) + FEMatrix values. */
 typedef struct pmx_matrix pmx_matrix;
 int pmx_matrix_create(pmx_ctx* ctx, const pmx_space* s, pmx_matrix** out);
+/* Square matrix from a host CSR array set (BlockMatrix::get_combined_matrix
+ * style use, src/Matrix/BlockMatrix.C); rows >= active_bound act as identity
+ * rows in the operator. */
+int pmx_matrix_create_csr(pmx_ctx* ctx, int n, int nnz, const int* host_row_ptr, const int* host_col,
+                          const double* host_val, int active_bound, pmx_matrix** out);
 int pmx_matrix_destroy(pmx_matrix* A);
 int pmx_matrix_get_info(const pmx_matrix* A, int* n_rows, int* nnz, int* active_bound);
 int pmx_matrix_get_structure(const pmx_matrix* A, int* host_row_ptr, int* host_col);

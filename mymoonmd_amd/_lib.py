@@ -72,6 +72,8 @@ SIGNATURES = {
     "pmx_space_get_structure": [c_void_p, P(c_int), P(c_int), P(c_int)],
     "pmx_space_get_prolongation": [c_void_p, c_void_p, P(c_int), P(c_int), P(c_int), P(c_double)],
     "pmx_matrix_create": [c_void_p, c_void_p, P(c_void_p)],
+    "pmx_matrix_create_csr": [c_void_p, c_int, c_int, P(c_int), P(c_int), P(c_double), c_int,
+                              P(c_void_p)],
     "pmx_matrix_destroy": [c_void_p],
     "pmx_matrix_get_info": [c_void_p, P(c_int), P(c_int), P(c_int)],
     "pmx_matrix_get_structure": [c_void_p, P(c_int), P(c_int)],

@@ -334,6 +334,40 @@ int pmx_matrix_create(pmx_ctx* ctx, const pmx_space* s, pmx_matrix** out) {
   });
 }
 
+int pmx_matrix_create_csr(pmx_ctx* ctx, int n, int nnz, const int* rp, const int* col,
+                          const double* val, int active_bound, pmx_matrix** out) {
+  return guarded([&] {
+    require(ctx && rp && col && val && out, "NULL argument");
+    require(n >= 1 && nnz >= 0 && active_bound >= 0 && active_bound <= n, "invalid sizes");
+    require(rp[0] == 0 && rp[n] == nnz, "row pointer inconsistent with nnz");
+    for (int r = 0; r < n; ++r) {
+      require(rp[r + 1] >= rp[r], "row pointer not monotone");
+      for (int k = rp[r]; k < rp[r + 1]; ++k) {
+        require(col[k] >= 0 && col[k] < n, "column index out of range");
+        require(k == rp[r] || col[k] > col[k - 1], "columns must be sorted and unique per row");
+      }
+    }
+    auto pat = std::make_shared<pmx::CSRPattern>();
+    pat->n_rows = pat->n_cols = n;
+    pat->active_bound = active_bound;
+    pat->row_ptr.assign(rp, rp + n + 1);
+    pat->col.assign(col, col + nnz);
+    auto M = std::make_unique<pmx_matrix>();
+    M->ctx = ctx;
+    M->space = nullptr;
+    M->pattern = pat;
+    hipStream_t st = ctx->stream;
+    M->A.n = M->A.m = n;
+    M->A.nnz = nnz;
+    M->A.active = active_bound;
+    M->A.lpr = auto_lanes(nnz, n);
+    M->A.row_ptr = upload(pat->row_ptr.data(), pat->row_ptr.size(), st);
+    M->A.col = upload(pat->col.data(), pat->col.size(), st);
+    M->A.val = upload(val, (size_t)nnz, st);
+    *out = M.release();
+  });
+}
+
 int pmx_matrix_destroy(pmx_matrix* A) {
   return guarded([&] { delete A; });
 }

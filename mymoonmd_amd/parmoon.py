@@ -136,13 +136,23 @@ class TFESpace:
 
 
 class FEMatrix:
-    """Square FE matrix on the device (TStructure pattern + values)."""
+    """Square FE matrix on the device (TStructure pattern + values).  With
+    `csr=(row_ptr, col, val, active_bound)` a plain CSR matrix is uploaded."""
 
-    def __init__(self, ctx, space):
+    def __init__(self, ctx, space=None, csr=None):
         self.ctx = ctx
         self.space = space
         self.h = c_void_p()
-        call("pmx_matrix_create", ctx.h, space.h, ctypes.byref(self.h))
+        if csr is not None:
+            rp, col, val, ab = csr
+            rp = np.ascontiguousarray(rp, np.int32)
+            col = np.ascontiguousarray(col, np.int32)
+            val = np.ascontiguousarray(val, np.float64)
+            call("pmx_matrix_create_csr", ctx.h, len(rp) - 1, len(col),
+                 rp.ctypes.data_as(ctypes.POINTER(c_int)), col.ctypes.data_as(ctypes.POINTER(c_int)),
+                 val.ctypes.data_as(ctypes.POINTER(c_double)), ab, ctypes.byref(self.h))
+        else:
+            call("pmx_matrix_create", ctx.h, space.h, ctypes.byref(self.h))
         n, nnz, ab = c_int(), c_int(), c_int()
         call("pmx_matrix_get_info", self.h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(ab))
         self.n_rows, self.nnz, self.active_bound = n.value, nnz.value, ab.value

@@ -1,0 +1,69 @@
+/* ORACLE — test infrastructure only (see cd_oracle.c). */
+#ifndef CD_ORACLE_H
+#define CD_ORACLE_H
+
+#define OR_TRI 0
+#define OR_QUAD 1
+#define OR_HEXA 2
+#define OR_GALERKIN 0
+#define OR_SUPG 1
+#define OR_DIRICHLET 0
+#define OR_NEUMANN 1
+#define OR_EX_CONST 0
+#define OR_EX_SINE_LAPLACE_2D 1
+#define OR_EX_P0_3D 2
+#define OR_EX_P1_3D 3
+#define OR_EX_P2_3D 4
+#define OR_EX_EXP_SIN_3D 5
+#define OR_EX_SINE_2D 6
+
+typedef struct {
+  int shape, order, nv, n_cells, ndof_cell, n_dof, active;
+  const double* verts;   /* [n_cells][nv][dim] */
+  const int* gnum;       /* [n_cells][ndof_cell] */
+  const int* rp;         /* CSR pattern */
+  const int* col;
+  const int* bd_comp;    /* [n_cells][n_joints] boundary component or -1 */
+} or_space;
+
+typedef struct {
+  int disc, example, hexa_rule, bc_sine_laplace;
+  double eps, b[3], c, f0, delta0, poly_degree;
+} or_cd;
+
+typedef struct {
+  int iterative_solver_type;   /* 0 fgmres, 1 cg, 2 richardson, 3 jacobi */
+  int preconditioner;          /* 0 none, 1 jacobi, 2 multigrid */
+  int max_n_iterations, min_n_iterations;
+  double residual_tolerance, residual_reduction, divergence_factor;
+  int gmres_restart;
+  double damping;
+  int mg_cycle;                /* 0 V, 1 W, 2 F */
+  int mg_smoother_coarse;      /* 0 jacobi, 1 direct */
+  double mg_correction_damp;
+  int mg_n_pre_smooth, mg_n_post_smooth;
+  double mg_coarse_residual;
+  int mg_coarse_max_n_iterations;
+  double jacobi_damp;
+} or_solver_params;
+
+typedef struct {
+  or_space s;
+  const double* val;     /* matrix values on the pattern */
+  const int* parent;     /* fine cell -> coarse cell (NULL on the coarsest level) */
+} or_level;
+
+int or_ndof(int shape, int order);
+void or_basis(int shape, int order, const double* xi, double* v, double* g, double* h);
+int or_quad_rule(int shape, int order, int hexa_rule, double* xi, double* w);
+void or_assemble_cd(const or_space* s, const or_cd* p, double* val, double* rhs);
+void or_errors(const or_space* s, const or_cd* p, const double* sol, double* err4);
+void or_apply_scaled_add(int n, int active, const int* rp, const int* col, const double* val,
+                         const double* x, double* y, double a);
+void or_prolongate(const or_space* cs, const or_space* fs, const int* parent, const double* cu, double* fu);
+void or_restrict(const or_space* cs, const or_space* fs, const int* parent, double* cu, const double* fd);
+int or_solve(const or_solver_params* p, int nl, const or_level* lv, const double* b, double* x,
+             int* iterations, double* hist, int max_hist, int* n_hist);
+void or_mg_apply(const or_solver_params* p, int nl, const or_level* lv, const double* v, double* z);
+
+#endif

@@ -1,0 +1,559 @@
+"""ORACLE — test infrastructure only (tests/, smoke(), bench.py's cpu_baseline).
+
+Python/numpy restatement of the reference's set-up path plus ctypes access to
+the C restatement of its assembly/solve path (cd_oracle.c):
+
+* mesh(): macro meshes of TDomain::UnitSquare / TwoTriangles /
+  initialize_cube_hexa_mesh (src/Geometry/Tests.C:95-170, 449-482, 1815-1856)
+  refined with the child-vertex tables of RefQuadRegDesc.C:20-21,
+  RefTriRegDesc.C:20-21, RefHexaRegDesc.C:31-35 in It_Finest depth-first order
+  (src/Geometry/It_LE.C:17-56).
+* number_dofs(): TFESpace3D/2D::ConstructSpace (src/FE/FESpace3D.C:506-2080,
+  FESpace2D.C:1284-2000) with the mark-based union-find of TFE3DMapper::MapBound
+  / MapDOF (src/FE/FE3DMapper.C:67-220): boundary joints are marked with the
+  class counters, joints shared with a later cell are paired (lowest index
+  becomes the root, the highest mark wins), inner DOFs get a fresh mark; the
+  renumbering pass and the final class offsets follow FESpace3D.C:1786-1990.
+  Coinciding local DOFs of neighbouring cells are paired by their nodal
+  points (the reference pairs them through its face tables and twists).
+* structure(): TStructure(FESpace) (src/Matrix/Structure.C:829-1277).
+
+This module never runs in the product path.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+TRI, QUAD, HEXA = 0, 1, 2
+FIRSTMARK = -10
+
+# ------------------------------------------------------------------ meshes
+_QUAD_MEAN = [[0], [1], [2], [3], [0, 1], [1, 2], [2, 3], [3, 0], [0, 1, 2, 3]]
+_QUAD_CHILD = [[0, 4, 8, 7], [1, 5, 8, 4], [2, 6, 8, 5], [3, 7, 8, 6]]
+_TRI_MEAN = [[0], [1], [2], [0, 1], [1, 2], [2, 0]]
+_TRI_CHILD = [[0, 3, 5], [1, 4, 3], [2, 5, 4], [3, 4, 5]]
+_HEXA_MEAN = [[0], [1], [2], [3], [4], [5], [6], [7], [0, 1], [1, 2], [2, 3], [0, 3],
+              [0, 1, 2, 3], [1, 5], [4, 5], [0, 4], [0, 1, 5, 4], [2, 6], [5, 6],
+              [1, 2, 6, 5], [3, 7], [6, 7], [2, 3, 7, 6], [4, 7], [0, 3, 7, 4],
+              [4, 5, 6, 7], [0, 1, 2, 3, 4, 5, 6, 7]]
+_HEXA_CHILD = [[0, 8, 12, 11, 15, 16, 26, 24], [1, 9, 12, 8, 13, 19, 26, 16],
+               [2, 10, 12, 9, 17, 22, 26, 19], [3, 11, 12, 10, 20, 24, 26, 22],
+               [4, 23, 25, 14, 15, 24, 26, 16], [5, 14, 25, 18, 13, 16, 26, 19],
+               [6, 18, 25, 21, 17, 19, 26, 22], [7, 21, 25, 23, 20, 22, 26, 24]]
+
+JOINT_VERTS = {
+    QUAD: [[0, 1], [1, 2], [2, 3], [3, 0]],
+    TRI: [[0, 1], [1, 2], [2, 0]],
+    HEXA: [[0, 1, 2, 3], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7], [4, 7, 6, 5]],
+}
+
+
+def mesh(geometry, n_refine):
+    """Return (shape, levels) with levels[k] = dict(verts [nc,nv,dim], parent [nc])."""
+    if geometry == 0:
+        shape, v0 = QUAD, np.array([[[0, 0], [1, 0], [1, 1], [0, 1]]], float)
+        mean, child = _QUAD_MEAN, _QUAD_CHILD
+    elif geometry == 1:
+        shape, v0 = TRI, np.array([[[0, 0], [1, 0], [1, 1]], [[0, 0], [1, 1], [0, 1]]], float)
+        mean, child = _TRI_MEAN, _TRI_CHILD
+    else:
+        shape = HEXA
+        v0 = np.array([[[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                        [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]]], float)
+        mean, child = _HEXA_MEAN, _HEXA_CHILD
+    levels = [dict(verts=v0, parent=np.full(len(v0), -1, np.int32))]
+    for _ in range(n_refine):
+        pv = levels[-1]["verts"]
+        new = np.stack([pv[:, m, :].mean(axis=1) for m in mean], axis=1)   # [nc, n_new, dim]
+        ch = np.stack([new[:, c, :] for c in child], axis=1)                 # [nc, nch, nv, dim]
+        nc, nch = ch.shape[0], ch.shape[1]
+        levels.append(dict(verts=ch.reshape(nc * nch, ch.shape[2], ch.shape[3]),
+                           parent=np.repeat(np.arange(nc, dtype=np.int32), nch)))
+    return shape, levels
+
+
+def ndof_cell(shape, order):
+    if shape == TRI:
+        return 3
+    return (order + 1) ** (3 if shape == HEXA else 2)
+
+
+def ref_nodes(shape, order):
+    if shape == TRI:
+        return np.array([[0, 0], [1, 0], [0, 1]], float)
+    n1, dim = order + 1, (3 if shape == HEXA else 2)
+    out = []
+    for k in range(ndof_cell(shape, order)):
+        idx = [k % n1, (k // n1) % n1, k // (n1 * n1)]
+        out.append([-1.0 + 2.0 * idx[d] / order for d in range(dim)])
+    return np.array(out)
+
+
+def node_phys(shape, verts, xi):
+    """nodal point of a parallelogram/parallelepiped/triangle cell (vertex-weighted map)"""
+    if shape == TRI:
+        return verts[0] + (verts[1] - verts[0]) * xi[0] + (verts[2] - verts[0]) * xi[1]
+    if shape == QUAD:
+        s = [(-1, -1), (1, -1), (1, 1), (-1, 1)]
+        return sum(0.25 * (1 + a * xi[0]) * (1 + b * xi[1]) * verts[k] for k, (a, b) in enumerate(s))
+    s = [(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)]
+    return sum(0.125 * (1 + a * xi[0]) * (1 + b * xi[1]) * (1 + c * xi[2]) * verts[k]
+               for k, (a, b, c) in enumerate(s))
+
+
+def boundary_component(shape, fv):
+    """boundary component of a joint with vertex coordinates fv, or -1"""
+    if shape in (QUAD, TRI):
+        a, b = fv
+        if a[1] == 0 and b[1] == 0:
+            return 0
+        if a[0] == 1 and b[0] == 1:
+            return 1
+        if a[1] == 1 and b[1] == 1:
+            return 2
+        if a[0] == 0 and b[0] == 0:
+            return 3
+        return -1
+    for f, (ax, val) in enumerate([(2, 0), (1, 0), (0, 1), (1, 1), (0, 0), (2, 1)]):
+        if all(p[ax] == val for p in fv):
+            return f
+    return -1
+
+
+def bparam(comp, x, y):
+    return [x, y, 1 - x, 1 - y][comp]
+
+
+def joint_dofs(shape, order, j):
+    xi = ref_nodes(shape, order)
+    out = []
+    for k, p in enumerate(xi):
+        if shape == TRI:
+            on = [p[1] == 0, p[0] + p[1] == 1, p[0] == 0][j]
+        elif shape == QUAD:
+            on = p[[1, 0, 1, 0][j]] == [-1, 1, 1, -1][j]
+        else:
+            on = p[[2, 1, 0, 1, 0, 2][j]] == [-1, -1, 1, 1, -1, 1][j]
+        if on:
+            out.append(k)
+    return out
+
+
+def number_dofs(shape, verts, order, cond2d=None, cond3d=None):
+    """ConstructSpace: returns (global_numbers [nc, nd], info dict, bd_comp [nc, nj])."""
+    nc, nd = len(verts), ndof_cell(shape, order)
+    nj = len(JOINT_VERTS[shape])
+    xi = ref_nodes(shape, order)
+    key = np.zeros((nc, nd), dtype=np.int64)
+    coords = {}
+    keymap = {}
+    for c in range(nc):
+        for k in range(nd):
+            x = tuple(np.round(node_phys(shape, verts[c], xi[k]), 12))
+            if x not in keymap:
+                keymap[x] = len(keymap)
+                coords[keymap[x]] = x
+            key[c, k] = keymap[x]
+    # joints: boundary component and neighbour (cell, local joint)
+    bd = np.full((nc, nj), -1, np.int32)
+    face_owner = {}
+    neigh = {}
+    for c in range(nc):
+        for j, fvi in enumerate(JOINT_VERTS[shape]):
+            fv = [tuple(verts[c][v]) for v in fvi]
+            comp = boundary_component(shape, fv)
+            if comp >= 0:
+                bd[c, j] = comp
+                continue
+            fk = tuple(sorted(fv))
+            if fk in face_owner:
+                c2, j2 = face_owner[fk]
+                neigh[(c, j)] = (c2, j2)
+                neigh[(c2, j2)] = (c, j)
+            else:
+                face_owner[fk] = (c, j)
+    jd = [joint_dofs(shape, order, j) for j in range(nj)]
+
+    def cond_of(c, j):
+        comp = bd[c, j]
+        fvi = JOINT_VERTS[shape][j]
+        if shape == HEXA:
+            X = np.mean([verts[c][v] for v in fvi], axis=0)
+            return cond3d(X[0], X[1], X[2])
+        a, b = verts[c][fvi[0]], verts[c][fvi[1]]
+        t0, t1 = bparam(comp, a[0], a[1]), bparam(comp, b[0], b[1])
+        eps = 1e-6
+        c0, c1 = (cond2d(comp, t0 + eps), cond2d(comp, t1 - eps)) if t0 < t1 else \
+                 (cond2d(comp, t0 - eps), cond2d(comp, t1 + eps))
+        assert c0 == c1
+        return c0
+
+    # upper bounds of the boundary classes (first pass of ConstructSpace)
+    NT = 9
+    dir_ub, bnd_ub = 0, [0] * NT
+    for c in range(nc):
+        for j in range(nj):
+            if bd[c, j] >= 0:
+                cond = cond_of(c, j)
+                if cond == 0:
+                    dir_ub += len(jd[0])
+                else:
+                    bnd_ub[cond - 1] += len(jd[0])
+    G = np.full(nc * nd, -1, dtype=np.int64)
+    sum_loc = nc * nd
+    counters = {"dir": FIRSTMARK + 1}
+    l = -dir_ub + FIRSTMARK + 1
+    bound_counter = []
+    for t in range(NT):
+        bound_counter.append(l)
+        l -= bnd_ub[t]
+    counters["inner"] = l
+
+    def root(w):
+        while G[w] > -1:
+            w = G[w]
+        return w
+
+    def map_bound(c, dofs, which):
+        for k in dofs:
+            dof = c * nd + k
+            w = root(dof)
+            cur = counters["dir"] if which == "dir" else bound_counter[which]
+            if G[w] == -1:
+                cur -= 1
+                G[dof] = cur
+            elif G[w] < cur:
+                cur -= 1
+                G[w] = cur
+                if dof != w:
+                    G[dof] = w
+            if which == "dir":
+                counters["dir"] = cur
+            else:
+                bound_counter[which] = cur
+
+    def map_dof(d0, d1):
+        w0, w1 = root(d0), root(d1)
+        if G[d0] == -1 and G[d1] == -1:
+            counters["inner"] -= 1
+            if d0 < d1:
+                G[d0] = counters["inner"]; G[d1] = d0
+            else:
+                G[d1] = counters["inner"]; G[d0] = d1
+        elif G[d0] == -1:
+            if w1 > d0:
+                G[d0] = G[w1]; G[d1] = d0; G[w1] = d0
+            else:
+                G[d0] = w1
+        elif G[d1] == -1:
+            if w0 > d1:
+                G[d1] = G[w0]; G[d0] = d1; G[w0] = d1
+            else:
+                G[d1] = w0
+        else:
+            if G[w0] != G[w1]:
+                e = max(G[w0], G[w1])
+                w = min(w0, w1)
+                G[w0] = w; G[w1] = w; G[w] = e
+
+    for c in range(nc):
+        for j in range(nj):
+            if bd[c, j] >= 0:
+                cond = cond_of(c, j)
+                map_bound(c, jd[j], "dir" if cond == 0 else cond - 1)
+            else:
+                c2, j2 = neigh[(c, j)]
+                if c2 > c:
+                    for k in jd[j]:
+                        for k2 in jd[j2]:
+                            if key[c, k] == key[c2, k2]:
+                                map_dof(c * nd + k, c2 * nd + k2)
+        # inner dofs: nodes on no joint
+        on_joint = set(k for j in range(nj) for k in jd[j])
+        for k in range(nd):
+            if k not in on_joint:
+                counters["inner"] -= 1
+                G[c * nd + k] = counters["inner"]
+    # renumbering pass (FESpace3D.C:1804-1905)
+    bound_mark = []
+    l = FIRSTMARK - dir_ub
+    for t in range(NT):
+        bound_mark.append(l)
+        l -= bnd_ub[t]
+    inner_mark = l
+    dcount = 0
+    bcount = []
+    l = dir_ub
+    for t in range(NT):
+        bcount.append(l)
+        l += bnd_ub[t]
+    count = l
+    n_dir, n_inner, n_bnd = 0, 0, [0] * NT
+    for i in range(sum_loc):
+        v = G[i]
+        if v < -1:
+            if v <= inner_mark:
+                G[i] = count; count += 1; n_inner += 1
+                continue
+            m = NT - 1
+            matched = False
+            while m >= 0:
+                if v <= bound_mark[m]:
+                    G[i] = bcount[m]; bcount[m] += 1; n_bnd[m] += 1
+                    matched = True
+                    break
+                m -= 1
+            if not matched:
+                G[i] = dcount; dcount += 1; n_dir += 1
+        else:
+            G[i] = G[v]
+    # class offsets (FESpace3D.C:1923-1975)
+    m_ub = sum(bnd_ub)
+    dir_off = n_inner + sum(n_bnd)
+    inner_off = -dir_ub - m_ub
+    bound_off, l, m = [], n_inner, dir_ub
+    for t in range(NT):
+        bound_off.append(l - m)
+        l += n_bnd[t]
+        m += bnd_ub[t]
+    bmark, l = [], dir_ub
+    for t in range(NT):
+        l += bnd_ub[t]
+        bmark.append(l)
+    out = np.empty(sum_loc, dtype=np.int64)
+    for i in range(sum_loc):
+        n = G[i]
+        if n < dir_ub:
+            out[i] = n + dir_off
+        elif n < bmark[NT - 1]:
+            for t in range(NT):
+                if n < bmark[t]:
+                    out[i] = n + bound_off[t]
+                    break
+        else:
+            out[i] = n + inner_off
+    active = n_inner + sum(n_bnd)
+    info = dict(n_dof=active + n_dir, active_bound=active, n_inner=n_inner, n_bound=n_bnd,
+                n_dirichlet=n_dir)
+    return out.reshape(nc, nd).astype(np.int32), info, bd
+
+
+def structure(gnum, n_dof, active):
+    """TStructure(FESpace): active rows = sorted union of cell DOFs; Dirichlet rows = diagonal."""
+    rows = [set() for _ in range(n_dof)]
+    for cell in gnum:
+        for a in cell:
+            if a < active:
+                rows[a].update(int(b) for b in cell)
+    rp = [0]
+    col = []
+    for r in range(n_dof):
+        cols = sorted(rows[r]) if r < active else [r]
+        col.extend(cols)
+        rp.append(len(col))
+    return np.array(rp, np.int32), np.array(col, np.int32)
+
+
+# ------------------------------------------------------------- C oracle
+class Space(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int), ("order", ctypes.c_int), ("nv", ctypes.c_int),
+                ("n_cells", ctypes.c_int), ("ndof_cell", ctypes.c_int), ("n_dof", ctypes.c_int),
+                ("active", ctypes.c_int), ("verts", ctypes.c_void_p), ("gnum", ctypes.c_void_p),
+                ("rp", ctypes.c_void_p), ("col", ctypes.c_void_p), ("bd_comp", ctypes.c_void_p)]
+
+
+class CD(ctypes.Structure):
+    _fields_ = [("disc", ctypes.c_int), ("example", ctypes.c_int), ("hexa_rule", ctypes.c_int),
+                ("bc_sine_laplace", ctypes.c_int), ("eps", ctypes.c_double),
+                ("b", ctypes.c_double * 3), ("c", ctypes.c_double), ("f0", ctypes.c_double),
+                ("delta0", ctypes.c_double), ("poly_degree", ctypes.c_double)]
+
+
+class SolverParams(ctypes.Structure):
+    _fields_ = [("iterative_solver_type", ctypes.c_int), ("preconditioner", ctypes.c_int),
+                ("max_n_iterations", ctypes.c_int), ("min_n_iterations", ctypes.c_int),
+                ("residual_tolerance", ctypes.c_double), ("residual_reduction", ctypes.c_double),
+                ("divergence_factor", ctypes.c_double), ("gmres_restart", ctypes.c_int),
+                ("damping", ctypes.c_double), ("mg_cycle", ctypes.c_int),
+                ("mg_smoother_coarse", ctypes.c_int), ("mg_correction_damp", ctypes.c_double),
+                ("mg_n_pre_smooth", ctypes.c_int), ("mg_n_post_smooth", ctypes.c_int),
+                ("mg_coarse_residual", ctypes.c_double), ("mg_coarse_max_n_iterations", ctypes.c_int),
+                ("jacobi_damp", ctypes.c_double)]
+
+
+class Level(ctypes.Structure):
+    _fields_ = [("s", Space), ("val", ctypes.c_void_p), ("parent", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-s", "-C", HERE], check=True)
+        L = ctypes.CDLL(LIB)
+        L.or_assemble_cd.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p, ctypes.c_void_p]
+        L.or_errors.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p, ctypes.c_void_p]
+        L.or_apply_scaled_add.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
+        L.or_prolongate.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(Space), ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p]
+        L.or_restrict.argtypes = L.or_prolongate.argtypes
+        L.or_solve.argtypes = [ctypes.POINTER(SolverParams), ctypes.c_int, ctypes.POINTER(Level),
+                               ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int),
+                               ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.or_mg_apply.argtypes = [ctypes.POINTER(SolverParams), ctypes.c_int, ctypes.POINTER(Level),
+                                  ctypes.c_void_p, ctypes.c_void_p]
+        L.or_basis.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                               ctypes.c_void_p, ctypes.c_void_p]
+        L.or_quad_rule.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleSpace:
+    """Arrays of one FE space kept alive for the C structs."""
+
+    def __init__(self, shape, order, verts, gnum, n_dof, active, rp, col, bd, parent=None):
+        self.shape, self.order = shape, order
+        self.verts = np.ascontiguousarray(verts, dtype=np.float64)
+        self.gnum = np.ascontiguousarray(gnum, dtype=np.int32)
+        self.rp = np.ascontiguousarray(rp, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.bd = np.ascontiguousarray(bd, dtype=np.int32)
+        self.parent = None if parent is None else np.ascontiguousarray(parent, dtype=np.int32)
+        self.n_dof, self.active = int(n_dof), int(active)
+        self.n_cells, self.ndof_cell = self.gnum.shape
+        self.nnz = len(self.col)
+
+    def cstruct(self):
+        return Space(self.shape, self.order, self.verts.shape[1], self.n_cells, self.ndof_cell,
+                     self.n_dof, self.active, _p(self.verts), _p(self.gnum), _p(self.rp),
+                     _p(self.col), _p(self.bd))
+
+
+def build_space(geometry, n_refine, level, order, cond2d=None, cond3d=None):
+    shape, levels = mesh(geometry, n_refine)
+    v = levels[level]["verts"]
+    if cond2d is None:
+        cond2d = lambda comp, t: 0   # noqa: E731
+    if cond3d is None:
+        cond3d = lambda x, y, z: 0   # noqa: E731
+    g, info, bd = number_dofs(shape, v, order, cond2d, cond3d)
+    rp, col = structure(g, info["n_dof"], info["active_bound"])
+    return OracleSpace(shape, order, v, g, info["n_dof"], info["active_bound"], rp, col, bd,
+                       levels[level]["parent"]), info
+
+
+def cd_params(example=0, disc=0, eps=1.0, b=(0, 0, 0), c=0.0, f0=0.0, hexa_rule=0,
+              sine_laplace_bc=False, delta0=1.0, poly_degree=1.0):
+    p = CD()
+    p.disc, p.example, p.hexa_rule, p.bc_sine_laplace = disc, example, hexa_rule, int(sine_laplace_bc)
+    p.eps, p.c, p.f0, p.delta0, p.poly_degree = eps, c, f0, delta0, poly_degree
+    for k in range(3):
+        p.b[k] = b[k] if k < len(b) else 0.0
+    return p
+
+
+def assemble(space, params):
+    val = np.zeros(space.nnz)
+    rhs = np.zeros(space.n_dof)
+    s = space.cstruct()
+    lib().or_assemble_cd(ctypes.byref(s), ctypes.byref(params), _p(val), _p(rhs))
+    return val, rhs
+
+
+def errors(space, params, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(4)
+    s = space.cstruct()
+    lib().or_errors(ctypes.byref(s), ctypes.byref(params), _p(u), _p(out))
+    return out
+
+
+def apply_scaled_add(n, active, rp, col, val, x, y, a):
+    rp = np.ascontiguousarray(rp, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    assert y.dtype == np.float64 and y.flags["C_CONTIGUOUS"]
+    lib().or_apply_scaled_add(n, active, _p(rp), _p(col), _p(val), _p(x), _p(y), a)
+    return y
+
+
+def prolongate(coarse, fine, u):
+    u = np.ascontiguousarray(u, np.float64)
+    out = np.zeros(fine.n_dof)
+    cs, fs = coarse.cstruct(), fine.cstruct()
+    lib().or_prolongate(ctypes.byref(cs), ctypes.byref(fs), _p(fine.parent), _p(u), _p(out))
+    return out
+
+
+def restrict(coarse, fine, d):
+    d = np.ascontiguousarray(d, np.float64)
+    out = np.zeros(coarse.n_dof)
+    cs, fs = coarse.cstruct(), fine.cstruct()
+    lib().or_restrict(ctypes.byref(cs), ctypes.byref(fs), _p(fine.parent), _p(out), _p(d))
+    return out
+
+
+def solver_params(iterative_solver_type=0, preconditioner=0, max_n_iterations=100,
+                  min_n_iterations=0, residual_tolerance=1e-8, residual_reduction=0.0,
+                  divergence_factor=1e10, gmres_restart=20, damping=1.0, mg_cycle=0,
+                  mg_smoother_coarse=1, mg_correction_damp=1.0, mg_n_pre_smooth=1,
+                  mg_n_post_smooth=1, mg_coarse_residual=0.1, mg_coarse_max_n_iterations=10,
+                  jacobi_damp=1.0):
+    return SolverParams(iterative_solver_type, preconditioner, max_n_iterations, min_n_iterations,
+                        residual_tolerance, residual_reduction, divergence_factor, gmres_restart,
+                        damping, mg_cycle, mg_smoother_coarse, mg_correction_damp, mg_n_pre_smooth,
+                        mg_n_post_smooth, mg_coarse_residual, mg_coarse_max_n_iterations, jacobi_damp)
+
+
+def _levels(spaces, vals):
+    keep = []
+    arr = (Level * len(spaces))()
+    for k, (s, v) in enumerate(zip(spaces, vals)):
+        v = np.ascontiguousarray(v, np.float64)
+        keep.append(v)
+        arr[k].s = s.cstruct()
+        arr[k].val = _p(v)
+        arr[k].parent = _p(s.parent) if (k > 0 and s.parent is not None) else None
+    return arr, keep
+
+
+def solve(params, spaces, vals, b, x0):
+    """spaces/vals of the multigrid levels, coarsest first (or a single level)."""
+    arr, keep = _levels(spaces, vals)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.array(x0, dtype=np.float64, copy=True)
+    it = ctypes.c_int()
+    hist = np.zeros(100000)
+    nh = ctypes.c_int()
+    st = lib().or_solve(ctypes.byref(params), len(spaces), arr, _p(b), _p(x), ctypes.byref(it),
+                        _p(hist), len(hist), ctypes.byref(nh))
+    return x, it.value, hist[: min(nh.value, len(hist))], st
+
+
+def mg_apply(params, spaces, vals, v):
+    arr, keep = _levels(spaces, vals)
+    v = np.ascontiguousarray(v, np.float64)
+    z = np.zeros_like(v)
+    lib().or_mg_apply(ctypes.byref(params), len(spaces), arr, _p(v), _p(z))
+    return z
+
+
+def csr_general(n, active, rp, col):
+    """OracleSpace-like holder for a plain CSR matrix (no mesh)."""
+    return OracleSpace(HEXA, 1, np.zeros((1, 8, 3)), np.zeros((1, 8), np.int32), n, active, rp, col,
+                       np.full((1, 6), -1, np.int32))

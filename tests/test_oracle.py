@@ -1,0 +1,180 @@
+"""Pins the CPU oracle (oracle/) against the reference's own golden values and
+against closed forms, before it is used as the checker of the HIP path."""
+import numpy as np
+import pytest
+
+from conftest import iterative_7x7
+from oracle import oracle as O
+
+GEO = {"UnitSquare": 0, "TwoTriangles": 1, "Default_UnitCube_Hexa": 2}
+EX = {"const": 0, "sine_laplace_2d": 1, "p0_3d": 2, "p1_3d": 3, "p2_3d": 4, "exp_sin_3d": 5, "sine_2d": 6}
+
+
+def solve_direct_like(space, val, rhs, tol=1e-14):
+    """FGMRES without preconditioner run to round-off (stands in for the
+    reference's direct solver in the golden tests)."""
+    x0 = np.zeros(space.n_dof)
+    x0[space.active:] = rhs[space.active:]
+    sp = O.solver_params(iterative_solver_type=0, max_n_iterations=3000, gmres_restart=400,
+                         residual_tolerance=tol)
+    x, it, hist, st = O.solve(sp, [space], [val], rhs, x0)
+    assert st == 0
+    return x
+
+
+@pytest.mark.parametrize("geo,order", [("UnitSquare", 1), ("UnitSquare", 2), ("TwoTriangles", 1)])
+def test_cd2d_golden_errors(golden, geo, order):
+    ref = golden["cd2d_test"][geo][str(order)]
+    tol = golden["cd2d_test"]["tolerance"]
+    sp, info = O.build_space(GEO[geo], 2, 2, order, cond2d=lambda c, t: 1 if c == 1 else 0)
+    p = O.cd_params(example=EX["sine_laplace_2d"], sine_laplace_bc=True)
+    val, rhs = O.assemble(sp, p)
+    x = solve_direct_like(sp, val, rhs)
+    err = O.errors(sp, p, x)
+    assert np.all(np.abs(err - np.array(ref)) < tol), (err, ref)
+
+
+def test_cd2d_golden_detects_planted_error(golden):
+    """A dropped Neumann term (planted in the oracle inputs) breaks the golden check."""
+    ref = golden["cd2d_test"]["UnitSquare"]["1"]
+    sp, info = O.build_space(0, 2, 2, 1, cond2d=lambda c, t: 1 if c == 1 else 0)
+    p = O.cd_params(example=EX["sine_laplace_2d"], sine_laplace_bc=False)   # Neumann integral dropped
+    val, rhs = O.assemble(sp, p)
+    x = solve_direct_like(sp, val, rhs)
+    err = O.errors(sp, p, x)
+    assert abs(err[0] - ref[0]) > 1e-6
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_cd3d_zero_error_examples(golden, idx):
+    name, order, eps, b, c = golden["cd3d_solvers_test"]["examples"][idx]
+    tol = golden["cd3d_solvers_test"]["tolerance_direct"]
+    sp, info = O.build_space(2, 2, 2, order)
+    p = O.cd_params(example=EX[name], eps=eps, b=b, c=c)
+    val, rhs = O.assemble(sp, p)
+    x = solve_direct_like(sp, val, rhs, tol=1e-15)
+    err = O.errors(sp, p, x)
+    # reference tolerance 1e-13 for the direct solver; our Krylov stand-in
+    # reaches the same level of round-off
+    assert err[0] < 10 * tol and err[1] < 1e3 * tol, err
+
+
+def test_matrix_test_spmv_golden(golden):
+    g = golden["matrix_test"]
+
+    def dense(m):
+        a = np.zeros((m["n_rows"], m["n_cols"]))
+        for r in range(m["n_rows"]):
+            for k in range(m["row_ptr"][r], m["row_ptr"][r + 1]):
+                a[r, m["col"][k]] = m["val"][k]
+        return a
+
+    CB = dense(g["C"]) @ dense(g["B"])
+    rp = [0]
+    col, val = [], []
+    for r in range(4):
+        nz = np.nonzero(CB[r])[0]
+        col += list(nz)
+        val += list(CB[r, nz])
+        rp.append(len(col))
+    y = np.zeros(4)
+    O.apply_scaled_add(4, 4, rp, col, val, np.array(g["x"]), y, -0.5)
+    ref = np.array(g["y_minus_half_CB_x"])
+    assert np.all(np.abs((y - ref) / ref) < 1e-14), (y, ref)
+
+
+@pytest.mark.parametrize("method,prec", [(3, 0), (2, 1), (1, 1), (0, 1), (0, 0)])
+def test_iterative_solver_7x7(golden, method, prec):
+    g = golden["iterative_solver_test"]
+    M, rp, col, val = iterative_7x7(g)
+    exact = np.arange(7, dtype=float)
+    b = M @ exact
+    sp = O.csr_general(7, 7, rp, col)
+    params = O.solver_params(iterative_solver_type=method, preconditioner=prec,
+                             residual_tolerance=g["residual_tolerance"],
+                             max_n_iterations=g["max_n_iterations"])
+    x, it, hist, st = O.solve(params, [sp], [val], b, np.zeros(7))
+    assert st == 0
+    assert np.linalg.norm(x - exact) < 100 * g["residual_tolerance"]
+
+
+def one_cell_space(order):
+    """single hexahedron [0,1]^3, every row active (no boundary joints)"""
+    shape, levels = O.mesh(2, 0)
+    nd = O.ndof_cell(shape, order)
+    g = np.arange(nd, dtype=np.int32).reshape(1, nd)
+    rp, col = O.structure(g, nd, nd)
+    return O.OracleSpace(shape, order, levels[0]["verts"], g, nd, nd, rp, col,
+                         np.full((1, 6), -1, np.int32))
+
+
+def to_dense(sp, val):
+    A = np.zeros((sp.n_dof, sp.n_dof))
+    for r in range(sp.n_dof):
+        A[r, sp.col[sp.rp[r]:sp.rp[r + 1]]] = val[sp.rp[r]:sp.rp[r + 1]]
+    return A
+
+
+def test_q2_brick_laplacian_is_kronecker_sum():
+    """On [0,1]^3 the Q2 Laplace matrix is Kx(M)M + M(x)K(x)M + M(x)M(x)K with the
+    1D P2 stiffness/mass matrices (dof index = ix + 3 iy + 9 iz)."""
+    sp = one_cell_space(2)
+    val, rhs = O.assemble(sp, O.cd_params(example=0, eps=1.0, c=0.0))
+    A = to_dense(sp, val)
+    K = np.array([[7, -8, 1], [-8, 16, -8], [1, -8, 7]]) / 3.0
+    M = np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) / 30.0
+    ref = np.kron(np.kron(M, M), K) + np.kron(np.kron(M, K), M) + np.kron(np.kron(K, M), M)
+    assert np.max(np.abs(A - ref)) < 1e-13
+
+
+def test_mass_matrix_sums_to_volume_and_convection_rows_vanish():
+    sp = one_cell_space(2)
+    val, _ = O.assemble(sp, O.cd_params(example=0, eps=0.0, c=1.0))
+    assert abs(val.sum() - 1.0) < 1e-13
+    val, _ = O.assemble(sp, O.cd_params(example=0, eps=0.0, b=(0.3, -1.1, 2.0)))
+    A = to_dense(sp, val)
+    assert np.max(np.abs(A.sum(axis=1))) < 1e-13   # b.grad(1) = 0
+
+
+def test_prolongation_reproduces_polynomials_and_restriction_is_adjoint():
+    coarse, _ = O.build_space(2, 2, 1, 2)
+    fine, _ = O.build_space(2, 2, 2, 2)
+    sh, levels = O.mesh(2, 2)
+
+    def nodal(space, level, f):
+        u = np.zeros(space.n_dof)
+        xi = O.ref_nodes(sh, 2)
+        for c in range(space.n_cells):
+            for k in range(space.ndof_cell):
+                x = O.node_phys(sh, levels[level]["verts"][c], xi[k])
+                u[space.gnum[c, k]] = f(x)
+        return u
+
+    f = lambda x: 1 + x[0] - 2 * x[1] * x[2] + 3 * x[0] ** 2   # noqa: E731 (in Q2)
+    uc = nodal(coarse, 1, f)
+    uf = O.prolongate(coarse, fine, uc)
+    assert np.max(np.abs(uf - nodal(fine, 2, f))) < 1e-13
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal(coarse.n_dof)
+    d = rng.standard_normal(fine.n_dof)
+    lhs = O.prolongate(coarse, fine, u) @ d
+    rhs = u @ O.restrict(coarse, fine, d)
+    assert abs(lhs - rhs) < 1e-12 * max(1.0, abs(lhs))
+
+
+def test_multigrid_fgmres_solves_quadratic_example():
+    """cd3d_solvers_test 'multigrid' setup: FGMRES + MG(W, 2 levels) on example -3."""
+    sc, _ = O.build_space(2, 2, 1, 2)
+    sf, _ = O.build_space(2, 2, 2, 2)
+    p = O.cd_params(example=EX["p2_3d"], eps=1.0, b=(4, 3, 2), c=1.0)
+    vc, rc = O.assemble(sc, p)
+    vf, rf = O.assemble(sf, p)
+    x0 = np.zeros(sf.n_dof)
+    x0[sf.active:] = rf[sf.active:]
+    params = O.solver_params(iterative_solver_type=0, preconditioner=2, max_n_iterations=100,
+                             residual_tolerance=1e-15, mg_cycle=1, mg_smoother_coarse=1,
+                             mg_correction_damp=0.8, mg_n_pre_smooth=3, mg_n_post_smooth=3)
+    x, it, hist, st = O.solve(params, [sc, sf], [vc, vf], rf, x0)
+    err = O.errors(sf, p, x)
+    assert err[0] < 1e-12 and err[1] < 1e-12
+    assert it < 60
