@@ -1,0 +1,246 @@
+"""Benchmark of the CD3D solve path on one MI355X.
+
+Workload (the reference metric's configuration): unit cube, 64^3 Q2 hexahedra
+(2,146,689 DOFs, 130,422,149 nnz), eps = 1e-3, b = (1,2,3)/sqrt(14), c = 0,
+SUPG, Gauss 3x3x3, Dirichlet data and right-hand side of
+u = exp(-x) sin(pi y) sin(pi z); FGMRES(30) preconditioned by a 4-level
+geometric multigrid V(2,2) with damped Jacobi (omega = 0.8), stopping at a
+relative residual of 1e-8 (residual_reduction 1e-8, residual_tolerance 0); the
+coarsest level (8^3 cells, 4,913 DOFs) is solved exactly (the reference's
+default multigrid_smoother_coarse = direct_solve).
+
+One step = one complete solve from the Dirichlet-initialised guess with all
+inputs resident in HBM.  `value` is the time to solution of the mixed path
+(FP32 V-cycle on bf16 row-scaled matrices inside FP64 FGMRES); the plain FP64
+path is timed in the same run.  Every result is checked in FP64 on the GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+B = (1 / math.sqrt(14), 2 / math.sqrt(14), 3 / math.sqrt(14))
+EPS = 1e-3
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--refine", type=int, default=6, help="cube refinements (6 -> 64^3)")
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-refine", type=int, default=4, help="refinements of the CPU sample")
+    return ap.parse_args()
+
+
+def solver_params(pm, precision):
+    return pm.solver_params(
+        iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
+        residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=500,
+        multigrid_cycle_type="V", multigrid_smoother="jacobi", multigrid_smoother_coarse="direct_solve",
+        multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2, jacobi_damp=0.8,
+        multigrid_correction_damp_factor=1.0, multigrid_coarse_residual=0.1,
+        multigrid_coarse_max_n_iterations=10, precision=precision)
+
+
+def cpu_baseline(nref, nlev, max_it=60):
+    """The oracle (sequential C restatement of the reference CPU path) on a
+    bounded sample of the same problem, one host core."""
+    import numpy as np
+    from oracle import oracle as O
+    spaces = []
+    for l in range(nref - nlev + 1, nref + 1):
+        sp, _ = O.build_space(2, nref, l, 2)
+        spaces.append(sp)
+    p = O.cd_params(example=5, disc=1, eps=EPS, b=B, hexa_rule=2)
+    t0 = time.perf_counter()
+    vals, rhss = [], []
+    for sp in spaces:
+        v, r = O.assemble(sp, p)
+        vals.append(v)
+        rhss.append(r)
+    t_asm_all = time.perf_counter() - t0
+    fs = spaces[-1]
+    t0 = time.perf_counter()
+    vf, rf = O.assemble(fs, p)
+    y = np.zeros(fs.n_dof)
+    O.apply_scaled_add(fs.n_dof, fs.active, fs.rp, fs.col, vf, rf, y, 1.0)
+    t_asm_spmv = time.perf_counter() - t0
+    x0 = np.zeros(fs.n_dof)
+    x0[fs.active:] = rf[fs.active:]
+    op = O.solver_params(iterative_solver_type=0, preconditioner=2, gmres_restart=30,
+                         residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=max_it,
+                         mg_cycle=0, mg_smoother_coarse=1, mg_n_pre_smooth=2, mg_n_post_smooth=2,
+                         jacobi_damp=0.8, mg_correction_damp=1.0, mg_coarse_residual=0.1,
+                         mg_coarse_max_n_iterations=10)
+    t0 = time.perf_counter()
+    x, it, hist, st = O.solve(op, spaces, vals, rf, x0)
+    t_solve = time.perf_counter() - t0
+    return dict(n_dof=fs.n_dof, t_solve=t_solve, iterations=it, t_asm_spmv=t_asm_spmv,
+                t_asm_all=t_asm_all, mdofs=fs.n_dof / t_asm_spmv / 1e6)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box of this build has a single MI355X and the "
+              "solve path has no multi-GPU leg (domain decomposition is described, not built).",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from mymoonmd_amd import parmoon as pm
+
+    nref, nlev = args.refine, args.levels
+    ctx = pm.Context(0)
+    dom = pm.TDomain("Default_UnitCube_Hexa", nref)
+    levels = list(range(nref - nlev + 1, nref + 1))
+    t0 = time.perf_counter()
+    spaces = [pm.TFESpace(dom, l, 2, ctx) for l in levels]
+    t_setup_spaces = time.perf_counter() - t0
+    params = pm.cd_params("exp_sin_3d", disc="supg", eps=EPS, b=B, hexa_quad_rule=2)
+    mats, rhss = [], []
+    for s in spaces:
+        A = pm.FEMatrix(ctx, s)
+        f = torch.zeros(s.n_dof, dtype=torch.float64, device="cuda")
+        pm.assemble_cd(ctx, s, params, A, f)
+        mats.append(A)
+        rhss.append(f)
+    fs, A, b = spaces[-1], mats[-1], rhss[-1]
+    ab = fs.active_bound
+
+    # assembly + one SpMV (throughput in MDoF/s), inputs resident
+    x1 = torch.ones(fs.n_dof, dtype=torch.float64, device="cuda")
+    y1 = torch.empty_like(x1)
+    f_tmp = torch.zeros_like(b)
+    t_as = []
+    for _ in range(3):
+        A.reset()
+        f_tmp.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pm.assemble_cd(ctx, fs, params, A, f_tmp)
+        A.apply(x1, y1)
+        t_as.append(time.perf_counter() - t0)
+    t_asm_spmv = min(t_as)
+    del f_tmp, x1, y1
+
+    solvers = {}
+    t_update = {}
+    for prec in ("mixed", "fp64"):
+        t0 = time.perf_counter()
+        sol = pm.Solver(ctx, solver_params(pm, prec))
+        sol.update_matrix(mats, spaces)
+        t_update[prec] = time.perf_counter() - t0
+        solvers[prec] = sol
+
+    x0 = torch.zeros_like(b)
+    x0[ab:] = b[ab:]
+    r0 = b.clone()
+    A.apply_scaled_add(x0, r0, -1.0)
+    r0n = torch.linalg.norm(r0).item()
+
+    def run(prec):
+        x = x0.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rep = solvers[prec].solve(b, x)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        r = b.clone()
+        A.apply_scaled_add(x, r, -1.0)
+        rel = torch.linalg.norm(r).item() / r0n
+        return dt, rep.iterations, rel, x
+
+    for _ in range(args.warmup):
+        run("mixed")
+        run("fp64")
+    times = {"mixed": [], "fp64": []}
+    its, rels = {}, {}
+    xs = {}
+    torch.cuda.synchronize()
+    t_all0 = time.perf_counter()
+    for _ in range(args.steps):
+        for prec in ("mixed", "fp64"):
+            dt, it, rel, x = run(prec)
+            times[prec].append(dt)
+            its[prec], rels[prec], xs[prec] = it, rel, x
+    torch.cuda.synchronize()
+    t_all = time.perf_counter() - t_all0
+    tts = {k: sum(v) / len(v) for k, v in times.items()}
+    sol_diff = (torch.max(torch.abs(xs["mixed"] - xs["fp64"])) /
+                torch.max(torch.abs(xs["fp64"]))).item()
+    errs = {k: pm.compute_errors(ctx, fs, params, xs[k]) for k in xs}
+
+    # roofline of the dominant kernel (finest-level operator kernels)
+    nnz, n = A.nnz, A.n_rows
+    kt = {k: A.time_kernel(k, reps=20) for k in ("apply_fp64", "jacobi_fp64", "apply_bf16", "jacobi_bf16")}
+    bytes_alg = {"apply_fp64": 12 * nnz + 20 * n, "jacobi_fp64": 12 * nnz + 20 * n + 24 * n,
+                 "apply_bf16": 6 * nnz + 16 * n, "jacobi_bf16": 6 * nnz + 16 * n + 12 * n}
+    dom_kernel = "jacobi_bf16"
+    ach = bytes_alg[dom_kernel] / (kt[dom_kernel] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": "k_fe_spmv<64,MODE_JACOBI,bf16,f32>", "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": None, "bytes_per_launch": bytes_alg[dom_kernel],
+            "avg_launch_ms": round(kt[dom_kernel], 4)}
+    spmv_fp64 = bytes_alg["apply_fp64"] / (kt["apply_fp64"] * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        nl_cpu = max(2, args.cpu_refine - 1)
+        c = cpu_baseline(args.cpu_refine, nl_cpu)
+        scale = fs.n_dof / c["n_dof"]
+        per_it = c["t_solve"] / max(1, c["iterations"])
+        est = per_it * scale * its["fp64"]
+        cpu = {"value": round(est, 2), "unit": "s", "cores": 1, "kind": "port",
+               "sample": "oracle (sequential C restatement) FGMRES(30)+MG V(2,2) on the same "
+                         "problem with %d DOFs (%d^3 cells, %d levels, exact coarse solve), first "
+                         "iterations only: %.2f s for %d iterations; value = time per iteration x "
+                         "DOF ratio %.1f x the %d "
+                         "iterations of the 64^3 FP64 solve. Oracle assembly+SpMV: %.3f MDoF/s "
+                         "(GPU/CPU %.0fx)" % (
+                             c["n_dof"], 2 ** args.cpu_refine, nl_cpu, c["t_solve"], c["iterations"],
+                             scale, its["fp64"], c["mdofs"], (n / t_asm_spmv / 1e6) / c["mdofs"])}
+
+    out = {
+        "metric": "time-to-solution @1e-8 rel. residual, mixed vs FP64 GPU; assembly+SpMV MDoF/s",
+        "value": round(tts["mixed"], 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000 * tts["mixed"], 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64 outer / f32+bf16 V-cycle",
+        "data": "synthetic (analytic exp-sin solution, seeded nothing random)",
+        "config": {"workload": "CD3D SUPG Q2 hexa %d^3 (%d DOF, %d nnz), FGMRES(30)+MG V(2,2) "
+                               "Jacobi 0.8, %d levels, 1e-8 rel. residual" % (2 ** nref, n, nnz, nlev)},
+        "time_to_solution_s": {"mixed": round(tts["mixed"], 4), "fp64": round(tts["fp64"], 4)},
+        "speedup_mixed_vs_fp64": round(tts["fp64"] / tts["mixed"], 3),
+        "iterations": its,
+        "fp64_verified_rel_residual": rels,
+        "mixed_vs_fp64_solution_rel_diff": sol_diff,
+        "l2_h1_errors": {k: [errs[k][0], errs[k][1]] for k in errs},
+        "assembly_spmv_mdofs": round(n / t_asm_spmv / 1e6, 2),
+        "assembly_spmv_ms": round(1000 * t_asm_spmv, 3),
+        "solver_update_s": {k: round(v, 3) for k, v in t_update.items()},
+        "kernel_ms": {k: round(v, 4) for k, v in kt.items()},
+        "spmv_fp64_gbs": round(spmv_fp64, 1),
+        "timed_region_s": round(t_all, 3),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -129,6 +129,18 @@ int pmx_matrix_apply_scaled_add(pmx_ctx* ctx, const pmx_matrix* A, const double*
  * identity on non-active rows. */
 int pmx_matrix_apply(pmx_ctx* ctx, const pmx_matrix* A, const double* dev_x, double* dev_y);
 
+/* Times `reps` back-to-back launches of one operator kernel on the context's
+ * stream with HIP events; *avg_ms = mean duration per launch.
+ * kernel: PMX_KERNEL_APPLY_FP64 (y = A x), PMX_KERNEL_JACOBI_FP64 (fused damped
+ * Jacobi step), PMX_KERNEL_APPLY_BF16 / PMX_KERNEL_JACOBI_BF16 (bf16 values with
+ * FP32 row scale and FP32 vectors; the bf16 copy is made internally).
+ * Input vectors are internal buffers filled with ones. */
+#define PMX_KERNEL_APPLY_FP64 0
+#define PMX_KERNEL_JACOBI_FP64 1
+#define PMX_KERNEL_APPLY_BF16 2
+#define PMX_KERNEL_JACOBI_BF16 3
+int pmx_matrix_time_kernel(pmx_ctx* ctx, const pmx_matrix* A, int kernel, int reps, double* avg_ms);
+
 /* ---------------------------------------------------------------- assembly */
 #define PMX_DISC_GALERKIN 0
 #define PMX_DISC_SUPG 1

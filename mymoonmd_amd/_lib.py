@@ -84,6 +84,7 @@ SIGNATURES = {
     "pmx_matrix_set_spmv_lanes": [c_void_p, c_int],
     "pmx_matrix_apply_scaled_add": [c_void_p, c_void_p, c_void_p, c_void_p, c_double],
     "pmx_matrix_apply": [c_void_p, c_void_p, c_void_p, c_void_p],
+    "pmx_matrix_time_kernel": [c_void_p, c_void_p, c_int, c_int, P(c_double)],
     "pmx_assembly_n_quad": [c_void_p, c_int, P(c_int)],
     "pmx_assembly_quad_points": [c_void_p, c_int, P(c_double)],
     "pmx_assemble_cd": [c_void_p, c_void_p, P(CDParams), c_void_p, c_void_p],

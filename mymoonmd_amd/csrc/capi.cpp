@@ -449,6 +449,54 @@ int pmx_matrix_apply(pmx_ctx* ctx, const pmx_matrix* A, const double* x, double*
   });
 }
 
+int pmx_matrix_time_kernel(pmx_ctx* ctx, const pmx_matrix* A, int kernel, int reps, double* avg_ms) {
+  return guarded([&] {
+    require(ctx && A && avg_ms && reps >= 1, "invalid argument");
+    require(kernel >= 0 && kernel <= 3, "unknown kernel");
+    hipStream_t st = ctx->stream;
+    const int n = A->A.n;
+    pmx::DevCSR M = A->A;
+    double *x = nullptr, *y = nullptr, *dinv = nullptr;
+    float *x32 = nullptr, *y32 = nullptr, *d32 = nullptr, *rs = nullptr;
+    uint16_t* vb = nullptr;
+    x = dmalloc<double>(n); y = dmalloc<double>(n); dinv = dmalloc<double>(n);
+    pmx::vec_fill(n, 1.0, x, st);
+    pmx::vec_fill(n, 1.0, dinv, st);
+    if (kernel >= 2) {
+      x32 = dmalloc<float>(n); y32 = dmalloc<float>(n); d32 = dmalloc<float>(n); rs = dmalloc<float>(n);
+      vb = dmalloc<uint16_t>(M.nnz);
+      pmx::csr_to_bf16(M, vb, rs, st);
+      M.val_bf16 = vb;
+      M.rscale = rs;
+      pmx::vec_fill_f32(n, 1.f, x32, st);
+      pmx::vec_fill_f32(n, 1.f, d32, st);
+    }
+    auto launch = [&]() {
+      switch (kernel) {
+        case 0: pmx::fe_apply(M, x, y, st); break;
+        case 1: pmx::fe_jacobi(M, x, x, dinv, y, 0.8, st); break;
+        case 2: pmx::fe_apply_scaled_add_bf16(M, x32, y32, 1.0, st); break;
+        case 3: pmx::fe_jacobi_bf16(M, x32, x32, d32, y32, 0.8, st); break;
+      }
+    };
+    launch();
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) launch();
+    HIP_CHECK(hipEventRecord(e1, st));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipGetLastError());
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(x); dfree(y); dfree(dinv); dfree(x32); dfree(y32); dfree(d32); dfree(rs); dfree(vb);
+  });
+}
+
 int pmx_assembly_n_quad(const pmx_space* s, int hexa_quad_rule, int* n_quad) {
   return guarded([&] {
     require(s && n_quad, "NULL argument");

@@ -372,3 +372,121 @@ void csr_to_bf16(const DevCSR& A, uint16_t* vb, float* rscale, hipStream_t st) {
 }
 
 }  // namespace pmx
+
+// ------------------------------------------------------------------
+// Dense in-place Gauss-Jordan inversion with partial pivoting (set-up of the
+// exact coarse-grid solve).  Step k: k_gj_pivot (one block) selects the pivot
+// row, swaps rows, scales the pivot row and saves the column factors;
+// k_gj_eliminate updates every other row.  k_gj_unpermute undoes the column
+// order implied by the row swaps.  A is n x n row-major.
+namespace pmx {
+namespace {
+
+__global__ __launch_bounds__(1024) void k_gj_pivot(int n, int k, double* __restrict__ A,
+                                                    double* __restrict__ f, int* __restrict__ perm,
+                                                    int* __restrict__ singular) {
+  __shared__ double sv[1024];
+  __shared__ int si[1024];
+  double best = -1.0;
+  int bi = k;
+  for (int i = k + threadIdx.x; i < n; i += blockDim.x) {
+    const double v = fabs(A[(size_t)i * n + k]);
+    if (v > best) { best = v; bi = i; }
+  }
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double o = sv[threadIdx.x + s];
+      const int oi = si[threadIdx.x + s];
+      if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+        sv[threadIdx.x] = o;
+        si[threadIdx.x] = oi;
+      }
+    }
+    __syncthreads();
+  }
+  const int p = si[0];
+  if (threadIdx.x == 0) {
+    perm[k] = p;
+    if (sv[0] == 0.0) singular[0] = 1;
+  }
+  if (p != k)
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      const double t = A[(size_t)k * n + j];
+      A[(size_t)k * n + j] = A[(size_t)p * n + j];
+      A[(size_t)p * n + j] = t;
+    }
+  __syncthreads();
+  const double piv = A[(size_t)k * n + k];
+  const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double v = (j == k) ? 1.0 : A[(size_t)k * n + j];
+    A[(size_t)k * n + j] = v * ip;
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) f[i] = (i == k) ? 0.0 : A[(size_t)i * n + k];
+}
+
+__global__ __launch_bounds__(256) void k_gj_eliminate(int n, int k, double* __restrict__ A,
+                                                      const double* __restrict__ f) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n * n) return;
+  const int i = (int)(idx / n), j = (int)(idx % n);
+  if (i == k) return;
+  const double fi = f[i];
+  if (fi == 0.0) return;
+  const double base = (j == k) ? 0.0 : A[idx];
+  A[idx] = base - fi * A[(size_t)k * n + j];
+}
+
+// columns: apply the swaps (k, perm[k]) in reverse order to every row
+__global__ void k_gj_unpermute(int n, double* __restrict__ A, const int* __restrict__ perm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double* row = A + (size_t)i * n;
+  for (int k = n - 1; k >= 0; --k) {
+    const int p = perm[k];
+    if (p != k) { const double t = row[k]; row[k] = row[p]; row[p] = t; }
+  }
+}
+
+}  // namespace
+
+void dense_invert(int n, double* A, double* f, int* perm, int* singular, hipStream_t st) {
+  const size_t nn = (size_t)n * n;
+  const int grid = (int)((nn + 255) / 256);
+  for (int k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, st, n, k, A, f, perm, singular);
+    hipLaunchKernelGGL(k_gj_eliminate, dim3(grid), dim3(256), 0, st, n, k, A, f);
+  }
+  hipLaunchKernelGGL(k_gj_unpermute, dim3((n + 255) / 256), dim3(256), 0, st, n, A, perm);
+}
+
+}  // namespace pmx
+
+// dense row-major y = A x (one 64-lane wavefront per row), x/y FP64 or FP32
+namespace pmx {
+namespace {
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_mv(int n, const double* __restrict__ A,
+                                                  const T* __restrict__ x, T* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  if (row >= n) return;
+  const double* a = A + (size_t)row * n;
+  double s = 0;
+  for (int j = lane; j < n; j += 64) s += a[j] * (double)x[j];
+  s = group_sum<64>(s);
+  if (lane == 0) y[row] = (T)s;
+}
+}  // namespace
+
+void dense_mv(int n, const double* A, const double* x, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_dense_mv<double>, dim3((n + 3) / 4), dim3(256), 0, st, n, A, x, y);
+}
+void dense_mv_f32(int n, const double* A, const float* x, float* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_dense_mv<float>, dim3((n + 3) / 4), dim3(256), 0, st, n, A, x, y);
+}
+}  // namespace pmx

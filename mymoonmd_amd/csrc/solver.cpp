@@ -72,13 +72,13 @@ struct Level {
   float* dinv32 = nullptr;
   pmx::DevCSR P;             // prolongation to the next finer level (rows: finer DOFs)
   pmx::DevCSR R;             // restriction from the next finer level (rows: this level)
-  pmx::DevCSR Dinv;          // dense inverse (direct coarse solve), stored as full CSR rows
+  double* dinv_dense = nullptr;   // dense inverse (direct coarse solve), row-major n x n
   double *rhs = nullptr, *sol = nullptr, *def = nullptr, *tmp = nullptr;
   float *rhs32 = nullptr, *sol32 = nullptr, *def32 = nullptr, *tmp32 = nullptr;
   double* partial = nullptr;
   void release() {
     dfree(dinv); dfree(dinv32);
-    free_csr(P); free_csr(R); free_csr(Dinv);
+    free_csr(P); free_csr(R); dfree(dinv_dense);
     dfree(rhs); dfree(sol); dfree(def); dfree(tmp);
     dfree(rhs32); dfree(sol32); dfree(def32); dfree(tmp32);
     dfree(partial);
@@ -148,8 +148,8 @@ bool converged(pmx_solver* s, double res, int it) {
 void mg_smooth(pmx_solver* s, int l, bool mixed) {
   Level& L = s->lv[l];
   if (l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) {
-    if (mixed) pmx::csr_spmv_f32(L.Dinv, L.rhs32, L.sol32, 1.0, 0.0, L.n, 0, S(s));
-    else pmx::csr_spmv(L.Dinv, L.rhs, L.sol, 1.0, 0.0, L.n, 0, S(s));
+    if (mixed) pmx::dense_mv_f32(L.n, L.dinv_dense, L.rhs32, L.sol32, S(s));
+    else pmx::dense_mv(L.n, L.dinv_dense, L.rhs, L.sol, S(s));
     return;
   }
   if (mixed) {
@@ -187,10 +187,17 @@ int mg_cycle_step(pmx_solver* s, int step, int level, bool mixed) {
   if (level == 0) {
     // the initial defect only feeds verbose output in the reference
     double res = 1e10;
-    for (int i = 0; i < s->p.mg_coarse_max_n_iterations && res > s->p.mg_coarse_residual; ++i) {
+    if (s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) {
+      // an exact solve does not depend on the start value, so repeating it
+      // until the defect drops below multigrid_coarse_residual (the
+      // reference's loop) yields the same solution as one application
       mg_smooth(s, 0, mixed);
-      mg_defect(s, 0, mixed, s->d_scal);
-      res = read_scalar(s, s->d_scal);
+    } else {
+      for (int i = 0; i < s->p.mg_coarse_max_n_iterations && res > s->p.mg_coarse_residual; ++i) {
+        mg_smooth(s, 0, mixed);
+        mg_defect(s, 0, mixed, s->d_scal);
+        res = read_scalar(s, s->d_scal);
+      }
     }
     if (nl > 1) mg_update_solution_in_finer_grid(s, 0, mixed);
     return 1;
@@ -249,54 +256,31 @@ void mg_apply(pmx_solver* s, const double* v, double* z) {
 }
 
 // Dense inverse of the coarsest matrix by Gauss-Jordan elimination with
-// partial pivoting (set-up only; the coarse solve is then one dense matvec).
+// partial pivoting on the GPU (set-up only; the coarse solve is then one dense
+// matrix-vector product).
 void build_coarse_inverse(pmx_solver* s, const pmx_matrix* M) {
   Level& L = s->lv[0];
   const int n = L.n;
-  if (n > 6000) throw PmxError(PMX_E_ARG, "direct coarse solve supports at most 6000 DOFs");
+  if (n > 20000) throw PmxError(PMX_E_ARG, "direct coarse solve supports at most 20000 DOFs");
   std::vector<double> vals(M->A.nnz);
   HIP_CHECK(hipMemcpyAsync(vals.data(), M->A.val, vals.size() * sizeof(double), hipMemcpyDeviceToHost, S(s)));
   HIP_CHECK(hipStreamSynchronize(S(s)));
   const auto& pat = *M->pattern;
-  std::vector<double> a((size_t)n * n, 0.0), inv((size_t)n * n, 0.0);
-  for (int r = 0; r < n; ++r) {
-    for (int k = pat.row_ptr[r]; k < pat.row_ptr[r + 1]; ++k) a[(size_t)r * n + pat.col[k]] = vals[k];
-    inv[(size_t)r * n + r] = 1.0;
-  }
-  for (int k = 0; k < n; ++k) {
-    int piv = k;
-    double best = std::fabs(a[(size_t)k * n + k]);
-    for (int i = k + 1; i < n; ++i)
-      if (std::fabs(a[(size_t)i * n + k]) > best) { best = std::fabs(a[(size_t)i * n + k]); piv = i; }
-    if (best == 0.0) throw PmxError(PMX_E_ZERO_DIAG, "singular coarse matrix in direct coarse solve");
-    if (piv != k)
-      for (int j = 0; j < n; ++j) {
-        std::swap(a[(size_t)k * n + j], a[(size_t)piv * n + j]);
-        std::swap(inv[(size_t)k * n + j], inv[(size_t)piv * n + j]);
-      }
-    const double d = 1.0 / a[(size_t)k * n + k];
-    for (int j = 0; j < n; ++j) { a[(size_t)k * n + j] *= d; inv[(size_t)k * n + j] *= d; }
-#pragma omp parallel for schedule(static)
-    for (int i = 0; i < n; ++i) {
-      if (i == k) continue;
-      const double f = a[(size_t)i * n + k];
-      if (f == 0.0) continue;
-      double* ai = &a[(size_t)i * n];
-      double* ii = &inv[(size_t)i * n];
-      const double* ak = &a[(size_t)k * n];
-      const double* ik = &inv[(size_t)k * n];
-      for (int j = 0; j < n; ++j) { ai[j] -= f * ak[j]; ii[j] -= f * ik[j]; }
-    }
-  }
-  pmx::CSRPattern dp;
-  dp.n_rows = dp.n_cols = dp.active_bound = n;
-  dp.row_ptr.resize(n + 1);
-  dp.col.resize((size_t)n * n);
-  for (int r = 0; r <= n; ++r) dp.row_ptr[r] = r * n;
+  std::vector<double> a((size_t)n * n, 0.0);
   for (int r = 0; r < n; ++r)
-    for (int c = 0; c < n; ++c) dp.col[(size_t)r * n + c] = c;
-  L.Dinv = upload_csr(dp, inv, S(s));
-  L.Dinv.lpr = 64;
+    for (int k = pat.row_ptr[r]; k < pat.row_ptr[r + 1]; ++k) a[(size_t)r * n + pat.col[k]] = vals[k];
+  L.dinv_dense = upload(a.data(), a.size(), S(s));
+  double* f = dmalloc<double>(n);
+  int* perm = dmalloc<int>(n);
+  int* sing = dmalloc<int>(1);
+  HIP_CHECK(hipMemsetAsync(sing, 0, sizeof(int), S(s)));
+  pmx::dense_invert(n, L.dinv_dense, f, perm, sing, S(s));
+  HIP_CHECK(hipGetLastError());
+  int hs = 0;
+  HIP_CHECK(hipMemcpyAsync(&hs, sing, sizeof(int), hipMemcpyDeviceToHost, S(s)));
+  HIP_CHECK(hipStreamSynchronize(S(s)));
+  dfree(f); dfree(perm); dfree(sing);
+  if (hs) throw PmxError(PMX_E_ZERO_DIAG, "singular coarse matrix in the direct coarse solve");
 }
 
 template <typename T>

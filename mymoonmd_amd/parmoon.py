@@ -176,6 +176,13 @@ class FEMatrix:
              col.ctypes.data_as(ctypes.POINTER(c_int)))
         return rp, col
 
+    def time_kernel(self, kernel, reps=20):
+        """mean duration (ms) of one launch of an operator kernel (HIP events)"""
+        out = c_double()
+        call("pmx_matrix_time_kernel", self.ctx.h, self.h, {"apply_fp64": 0, "jacobi_fp64": 1,
+             "apply_bf16": 2, "jacobi_bf16": 3}[kernel], reps, ctypes.byref(out))
+        return out.value
+
     def set_spmv_lanes(self, lanes):
         call("pmx_matrix_set_spmv_lanes", self.h, lanes)
 

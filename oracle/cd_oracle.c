@@ -665,6 +665,8 @@ typedef struct {
   double **rhs, **sol, **def, **dinv, **tmp;
   int* cc;
   int ncc;
+  double* lu;   /* coarse LU factors (row-major, unit lower part below the diagonal) */
+  int* piv;
 } mg_state;
 
 static void fill_rec(int* rec, int level, int nl, int cycle, int* cc, int* n) {
@@ -677,37 +679,56 @@ static void fill_rec(int* rec, int level, int nl, int cycle, int* cc, int* n) {
   cc[(*n)++] = (level == nl - 1) ? 3 : 2;
 }
 
+/* LU factorisation with partial pivoting of the dense coarsest matrix (once per
+ * solve), then forward/back substitution per application: the exact solve of
+ * DirectSmoother (src/Multigrid/DirectSmoother.C:25-42) */
+static void coarse_factor(mg_state* m) {
+  const or_level* L = &m->lv[0];
+  const int n = L->s.n_dof;
+  double* a = calloc((size_t)n * n, sizeof(double));
+  int* piv = malloc(sizeof(int) * n);
+  for (int r = 0; r < n; ++r)
+    for (int k = L->s.rp[r]; k < L->s.rp[r + 1]; ++k) a[(size_t)r * n + L->s.col[k]] = L->val[k];
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(a[(size_t)i * n + k]) > fabs(a[(size_t)p * n + k])) p = i;
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; ++j) { double t = a[(size_t)k * n + j]; a[(size_t)k * n + j] = a[(size_t)p * n + j]; a[(size_t)p * n + j] = t; }
+    const double d = a[(size_t)k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      const double f = a[(size_t)i * n + k] / d;
+      a[(size_t)i * n + k] = f;
+      if (f == 0) continue;
+      for (int j = k + 1; j < n; ++j) a[(size_t)i * n + j] -= f * a[(size_t)k * n + j];
+    }
+  }
+  m->lu = a;
+  m->piv = piv;
+}
+
 static void mg_smooth(mg_state* m, int l) {
   const or_level* L = &m->lv[l];
   const int n = L->s.n_dof;
   if (l == 0 && m->p->mg_smoother_coarse == 1) {
-    /* direct solve: Gaussian elimination with partial pivoting on the dense matrix */
-    double* a = calloc((size_t)n * n, sizeof(double));
-    double* bb = malloc(sizeof(double) * n);
-    for (int r = 0; r < n; ++r)
-      for (int k = L->s.rp[r]; k < L->s.rp[r + 1]; ++k) a[(size_t)r * n + L->s.col[k]] = L->val[k];
-    memcpy(bb, m->rhs[l], sizeof(double) * n);
+    if (!m->lu) coarse_factor(m);
+    double* y = m->tmp[0];
+    memcpy(y, m->rhs[0], sizeof(double) * n);
     for (int k = 0; k < n; ++k) {
-      int piv = k;
-      for (int i = k + 1; i < n; ++i)
-        if (fabs(a[(size_t)i * n + k]) > fabs(a[(size_t)piv * n + k])) piv = i;
-      if (piv != k) {
-        for (int j = 0; j < n; ++j) { double t = a[(size_t)k * n + j]; a[(size_t)k * n + j] = a[(size_t)piv * n + j]; a[(size_t)piv * n + j] = t; }
-        double t = bb[k]; bb[k] = bb[piv]; bb[piv] = t;
-      }
-      for (int i = k + 1; i < n; ++i) {
-        const double f = a[(size_t)i * n + k] / a[(size_t)k * n + k];
-        if (f == 0) continue;
-        for (int j = k; j < n; ++j) a[(size_t)i * n + j] -= f * a[(size_t)k * n + j];
-        bb[i] -= f * bb[k];
-      }
+      const int p = m->piv[k];
+      if (p != k) { double t = y[k]; y[k] = y[p]; y[p] = t; }
+    }
+    for (int i = 0; i < n; ++i) {
+      double s = y[i];
+      for (int j = 0; j < i; ++j) s -= m->lu[(size_t)i * n + j] * y[j];
+      y[i] = s;
     }
     for (int i = n - 1; i >= 0; --i) {
-      double sum = bb[i];
-      for (int j = i + 1; j < n; ++j) sum -= a[(size_t)i * n + j] * m->sol[l][j];
-      m->sol[l][i] = sum / a[(size_t)i * n + i];
+      double s = y[i];
+      for (int j = i + 1; j < n; ++j) s -= m->lu[(size_t)i * n + j] * m->sol[0][j];
+      m->sol[0][i] = s / m->lu[(size_t)i * n + i];
     }
-    free(a); free(bb);
     return;
   }
   /* JacobiSmoother::smooth: defect = rhs - A sol; defect /= diag; sol += damp*defect */
@@ -951,6 +972,7 @@ int or_solve(const or_solver_params* p, int nl, const or_level* lv, const double
   if (n_hist) *n_hist = cst.n_hist;
   if (cst.diverged) status = 4;
   free(r);
+  free(m.lu); free(m.piv);
   for (int l = 0; l < nl; ++l) { free(m.rhs[l]); free(m.sol[l]); free(m.def[l]); free(m.dinv[l]); free(m.tmp[l]); }
   free(m.rhs); free(m.sol); free(m.def); free(m.dinv); free(m.tmp);
   return status;
@@ -979,6 +1001,7 @@ void or_mg_apply(const or_solver_params* p, int nl, const or_level* lv, const do
   fill_rec(rec, nl - 1, nl, p->mg_cycle, cc, &ncc);
   m.cc = cc; m.ncc = ncc;
   mg_apply(&m, v, z);
+  free(m.lu); free(m.piv);
   for (int l = 0; l < nl; ++l) { free(m.rhs[l]); free(m.sol[l]); free(m.def[l]); free(m.dinv[l]); free(m.tmp[l]); }
   free(m.rhs); free(m.sol); free(m.def); free(m.dinv); free(m.tmp);
 }
