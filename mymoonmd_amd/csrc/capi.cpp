@@ -20,6 +20,33 @@
 using namespace pmx_internal;
 thread_local std::string pmx_internal::g_last_error = "";
 
+namespace pmx_internal {
+
+constexpr int kSellSigma = 4096;
+
+void attach_sell(pmx_matrix* M, hipStream_t st) {
+  const pmx::SellPattern sp = pmx::build_sell(*M->pattern, kSellSigma);
+  pmx::DevSELL& S = M->A.sell;
+  S.n_slices = sp.n_slices;
+  S.total = sp.col.size();
+  S.slice_ptr = upload(sp.slice_ptr.data(), sp.slice_ptr.size(), st);
+  S.row = upload(sp.row.data(), sp.row.size(), st);
+  S.col = upload(sp.col.data(), sp.col.size(), st);
+  S.src = upload(sp.src.data(), sp.src.size(), st);
+  S.val = dmalloc<double>(S.total);
+  M->A.lpr = 0;
+  M->sell_valid = false;
+}
+
+void ensure_sell_values(pmx_matrix* M, hipStream_t st) {
+  if (M->sell_valid || M->A.sell.total == 0) return;
+  pmx::sell_fill_f64(M->A, M->A.sell.val, st);
+  HIP_CHECK(hipGetLastError());
+  M->sell_valid = true;
+}
+
+}  // namespace pmx_internal
+
 namespace {
 
 int auto_lanes(int nnz, int n) {
@@ -329,6 +356,7 @@ int pmx_matrix_create(pmx_ctx* ctx, const pmx_space* s, pmx_matrix** out) {
     M->A.col = upload(p.col.data(), p.col.size(), st);
     M->A.val = dmalloc<double>(p.col.size());
     HIP_CHECK(hipMemsetAsync(M->A.val, 0, p.col.size() * sizeof(double), st));
+    attach_sell(M.get(), st);
     HIP_CHECK(hipStreamSynchronize(st));
     *out = M.release();
   });
@@ -364,6 +392,8 @@ int pmx_matrix_create_csr(pmx_ctx* ctx, int n, int nnz, const int* rp, const int
     M->A.row_ptr = upload(pat->row_ptr.data(), pat->row_ptr.size(), st);
     M->A.col = upload(pat->col.data(), pat->col.size(), st);
     M->A.val = upload(val, (size_t)nnz, st);
+    attach_sell(M.get(), st);
+    HIP_CHECK(hipStreamSynchronize(st));
     *out = M.release();
   });
 }
@@ -404,6 +434,7 @@ int pmx_matrix_set_entries(pmx_ctx* ctx, pmx_matrix* A, const double* host_value
     HIP_CHECK(hipMemcpyAsync(A->A.val, host_values, (size_t)A->A.nnz * sizeof(double),
                              hipMemcpyHostToDevice, ctx->stream));
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    A->sell_valid = false;
   });
 }
 
@@ -411,6 +442,7 @@ int pmx_matrix_entries_device(pmx_matrix* A, double** dev_values) {
   return guarded([&] {
     require(A && dev_values, "NULL argument");
     *dev_values = A->A.val;
+    A->sell_valid = false;   // the caller may write the values
   });
 }
 
@@ -418,6 +450,7 @@ int pmx_matrix_reset(pmx_ctx* ctx, pmx_matrix* A) {
   return guarded([&] {
     require(ctx && A, "NULL argument");
     pmx::vec_fill(A->A.nnz, 0.0, A->A.val, ctx->stream);
+    A->sell_valid = false;
     HIP_CHECK(hipGetLastError());
   });
 }
@@ -426,8 +459,8 @@ int pmx_matrix_set_spmv_lanes(pmx_matrix* A, int lanes) {
   return guarded([&] {
     require(A != nullptr, "NULL matrix");
     require(lanes == 0 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64,
-            "lanes must be 0, 4, 8, 16, 32 or 64");
-    A->A.lpr = lanes ? lanes : auto_lanes(A->A.nnz, A->A.n);
+            "lanes must be 0 (SELL-64 kernel), 4, 8, 16, 32 or 64");
+    A->A.lpr = lanes;
   });
 }
 
@@ -435,6 +468,7 @@ int pmx_matrix_apply_scaled_add(pmx_ctx* ctx, const pmx_matrix* A, const double*
                                 double a) {
   return guarded([&] {
     require(ctx && A && x && y, "NULL argument");
+    ensure_sell_values(const_cast<pmx_matrix*>(A), ctx->stream);
     pmx::fe_apply_scaled_add(A->A, x, y, a, ctx->stream);
     HIP_CHECK(hipGetLastError());
   });
@@ -444,6 +478,7 @@ int pmx_matrix_apply(pmx_ctx* ctx, const pmx_matrix* A, const double* x, double*
   return guarded([&] {
     require(ctx && A && x && y, "NULL argument");
     require(x != y, "x and y must not alias");
+    ensure_sell_values(const_cast<pmx_matrix*>(A), ctx->stream);
     pmx::fe_apply(A->A, x, y, ctx->stream);
     HIP_CHECK(hipGetLastError());
   });
@@ -455,7 +490,9 @@ int pmx_matrix_time_kernel(pmx_ctx* ctx, const pmx_matrix* A, int kernel, int re
     require(kernel >= 0 && kernel <= 3, "unknown kernel");
     hipStream_t st = ctx->stream;
     const int n = A->A.n;
+    ensure_sell_values(const_cast<pmx_matrix*>(A), st);
     pmx::DevCSR M = A->A;
+    uint16_t* sb = nullptr;
     double *x = nullptr, *y = nullptr, *dinv = nullptr;
     float *x32 = nullptr, *y32 = nullptr, *d32 = nullptr, *rs = nullptr;
     uint16_t* vb = nullptr;
@@ -468,6 +505,11 @@ int pmx_matrix_time_kernel(pmx_ctx* ctx, const pmx_matrix* A, int kernel, int re
       pmx::csr_to_bf16(M, vb, rs, st);
       M.val_bf16 = vb;
       M.rscale = rs;
+      if (M.sell.total) {
+        sb = dmalloc<uint16_t>(M.sell.total);
+        pmx::sell_fill_bf16(M, sb, st);
+        M.sell.val_bf16 = sb;
+      }
       pmx::vec_fill_f32(n, 1.f, x32, st);
       pmx::vec_fill_f32(n, 1.f, d32, st);
     }
@@ -493,7 +535,7 @@ int pmx_matrix_time_kernel(pmx_ctx* ctx, const pmx_matrix* A, int kernel, int re
     *avg_ms = ms / reps;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    dfree(x); dfree(y); dfree(dinv); dfree(x32); dfree(y32); dfree(d32); dfree(rs); dfree(vb);
+    dfree(x); dfree(y); dfree(dinv); dfree(x32); dfree(y32); dfree(d32); dfree(rs); dfree(vb); dfree(sb);
   });
 }
 
@@ -530,6 +572,7 @@ int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p, p
     require(p->example >= 0 && p->example <= 6, "unknown example");
     pmx_space* s = const_cast<pmx_space*>(s_);
     hipStream_t st = ctx->stream;
+    A->sell_valid = false;
     const RefTable& t = ref_table(s, p->hexa_quad_rule);
     const pmx::AssemblyData ad = assembly_data(s, t);
     const pmx::CDCoeffs co = cd_coeffs(p);

@@ -81,9 +81,149 @@ void launch_fe_spmv(const DevCSR& A, const VT* va, const float* rscale, const XT
                      A.row_ptr, A.col, va, rscale, x, b, dinv, y, a, partial);
 }
 
+
+// SELL-64-sigma FE-matrix kernel: one wavefront per slice of 64 rows, lane l
+// owns row row[slice*64+l] and sums its entries sequentially in column order
+// (the summation order of TMatrix::multiply).  Eight entries are loaded per
+// step before their x values are gathered, to keep enough bytes in flight.
+// Blocks beyond the active slices handle the non-active (identity) rows.
+template <int MODE, typename VT, typename XT>
+__global__ __launch_bounds__(256) void k_sell_spmv(
+    int n_slices, int sell_blocks, int n, int active, const int* __restrict__ sptr,
+    const int* __restrict__ rowmap, const int* __restrict__ ci, const VT* __restrict__ va,
+    const float* __restrict__ rscale, const XT* __restrict__ x, const XT* __restrict__ b,
+    const XT* __restrict__ dinv, XT* __restrict__ y, double a, double* __restrict__ partial) {
+  __shared__ double red[16];
+  using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
+  double d2 = 0;
+  if ((int)blockIdx.x < sell_blocks) {
+    const int lane = threadIdx.x & 63;
+    const int slice = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (slice < n_slices) {
+      const int r = rowmap[slice * 64 + lane];
+      const int s0 = sptr[slice];
+      const int width = (sptr[slice + 1] - s0) >> 6;
+      const int* cp = ci + s0 + lane;
+      const VT* vp = va + s0 + lane;
+      AT acc = 0;
+      int j = 0;
+      for (; j + 8 <= width; j += 8) {
+        int c[8];
+        AT v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = __builtin_nontemporal_load(cp + (j + u) * 64);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if constexpr (sizeof(VT) == 2) v[u] = (AT)bf16_to_float(__builtin_nontemporal_load(vp + (j + u) * 64));
+          else v[u] = (AT)__builtin_nontemporal_load(vp + (j + u) * 64);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u] * (AT)x[c[u]];
+      }
+      for (; j < width; ++j) {
+        const int c = __builtin_nontemporal_load(cp + j * 64);
+        AT v;
+        if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(__builtin_nontemporal_load(vp + j * 64));
+        else v = (AT)__builtin_nontemporal_load(vp + j * 64);
+        acc += v * (AT)x[c];
+      }
+      if (r >= 0) {
+        if (rscale) acc *= (AT)rscale[r];
+        if (MODE == MODE_SCALED_ADD) {
+          y[r] += (XT)(a * (double)acc);
+        } else if (MODE == MODE_APPLY) {
+          y[r] = (XT)acc;
+        } else if (MODE == MODE_DEFECT) {
+          const XT d = (XT)((AT)b[r] - acc);
+          y[r] = d;
+          d2 = (double)d * (double)d;
+        } else {
+          const AT res = (AT)b[r] - acc;
+          y[r] = (XT)((AT)x[r] + (AT)a * (AT)dinv[r] * res);
+        }
+      }
+    }
+  } else {
+    const int i = active + (int)(blockIdx.x - sell_blocks) * blockDim.x + threadIdx.x;
+    if (i < n) {
+      if (MODE == MODE_SCALED_ADD) {
+        y[i] += (XT)(a * (double)x[i]);
+      } else if (MODE == MODE_APPLY) {
+        y[i] = x[i];
+      } else if (MODE == MODE_DEFECT) {
+        const XT d = (XT)(b[i] - x[i]);
+        y[i] = d;
+        d2 = (double)d * (double)d;
+      } else {
+        const AT res = (AT)(b[i] - x[i]);
+        y[i] = (XT)((AT)x[i] + (AT)a * (AT)dinv[i] * res);
+      }
+    }
+  }
+  if (MODE == MODE_DEFECT) {
+    const double s = block_sum(d2, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+int sell_grid(const DevCSR& A, int* sell_blocks) {
+  *sell_blocks = (A.sell.n_slices + 3) / 4;
+  return *sell_blocks + (A.n - A.active + 255) / 256;
+}
+
+template <int MODE, typename VT, typename XT>
+void launch_sell(const DevCSR& A, const VT* va, const float* rscale, const XT* x, const XT* b,
+                 const XT* dinv, XT* y, double a, double* partial, hipStream_t st) {
+  int sb = 0;
+  const int grid = sell_grid(A, &sb);
+  hipLaunchKernelGGL((k_sell_spmv<MODE, VT, XT>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb,
+                     A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, va, rscale, x, b, dinv, y,
+                     a, partial);
+}
+
+__global__ void k_sell_fill_f64(size_t total, const int* __restrict__ src, const double* __restrict__ v,
+                                double* __restrict__ out) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const int s = src[k];
+    out[k] = (s >= 0) ? v[s] : 0.0;
+  }
+}
+
+// one wavefront per slice: lane l converts the entries of its row
+__global__ __launch_bounds__(256) void k_sell_fill_bf16(int n_slices, const int* __restrict__ sptr,
+                                                        const int* __restrict__ rowmap,
+                                                        const int* __restrict__ src,
+                                                        const double* __restrict__ v,
+                                                        const float* __restrict__ rscale,
+                                                        uint16_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int slice = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (slice >= n_slices) return;
+  const int r = rowmap[slice * 64 + lane];
+  const double inv = (r >= 0) ? 1.0 / (double)rscale[r] : 0.0;
+  const int s0 = sptr[slice], width = (sptr[slice + 1] - s0) >> 6;
+  for (int j = 0; j < width; ++j) {
+    const int k = s0 + j * 64 + lane;
+    const int s = src[k];
+    const float f = (s >= 0) ? (float)(v[s] * inv) : 0.0f;
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    out[k] = (uint16_t)(u >> 16);
+  }
+}
+
 template <int MODE, typename VT, typename XT>
 void dispatch_fe_spmv(const DevCSR& A, const VT* va, const float* rscale, const XT* x, const XT* b,
                       const XT* dinv, XT* y, double a, double* partial, hipStream_t st) {
+  const bool have_sell = (sizeof(VT) == 2) ? (A.sell.val_bf16 != nullptr) : (A.sell.val != nullptr);
+  if (have_sell && A.lpr == 0) {
+    if constexpr (sizeof(VT) == 2)
+      launch_sell<MODE, VT, XT>(A, (const VT*)A.sell.val_bf16, rscale, x, b, dinv, y, a, partial, st);
+    else
+      launch_sell<MODE, VT, XT>(A, (const VT*)A.sell.val, rscale, x, b, dinv, y, a, partial, st);
+    return;
+  }
   switch (A.lpr) {
     case 4: launch_fe_spmv<4, MODE>(A, va, rscale, x, b, dinv, y, a, partial, st); break;
     case 8: launch_fe_spmv<8, MODE>(A, va, rscale, x, b, dinv, y, a, partial, st); break;
@@ -258,10 +398,15 @@ void fe_defect(const DevCSR& A, const double* x, const double* b, double* d, dou
                double* norm_out, hipStream_t st) {
   dispatch_fe_spmv<MODE_DEFECT, double, double>(A, A.val, nullptr, x, b, nullptr, d, 1.0, partial,
                                                 st);
-  const int nb = (A.n + (256 / A.lpr) - 1) / (256 / A.lpr);
+  const int nb = fe_defect_partials(A);
   if (norm_out) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, nb, partial, norm_out, 1);
 }
-int fe_defect_partials(const DevCSR& A) { return (A.n + (256 / A.lpr) - 1) / (256 / A.lpr); }
+int fe_defect_partials(const DevCSR& A) {
+  const int csr = (A.n + (256 / (A.lpr ? A.lpr : 64)) - 1) / (256 / (A.lpr ? A.lpr : 64));
+  int sb = 0;
+  const int sell = A.sell.n_slices ? sell_grid(A, &sb) : 0;
+  return csr > sell ? csr : sell;
+}
 void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* dinv, double* xn,
                double omega, hipStream_t st) {
   dispatch_fe_spmv<MODE_JACOBI, double, double>(A, A.val, nullptr, x, b, dinv, xn, omega, nullptr,
@@ -365,6 +510,15 @@ void vec_f32_to_f64(int n, const float* x, double* y, hipStream_t st) {
 void csr_inv_diag(const DevCSR& A, double* dinv, int* zero_flag, hipStream_t st) {
   hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(A.n)), dim3(256), 0, st, A.n, A.row_ptr, A.col, A.val,
                      dinv, zero_flag);
+}
+void sell_fill_f64(const DevCSR& A, double* sell_val, hipStream_t st) {
+  if (A.sell.total)
+    hipLaunchKernelGGL(k_sell_fill_f64, dim3(4096), dim3(256), 0, st, A.sell.total, A.sell.src, A.val, sell_val);
+}
+void sell_fill_bf16(const DevCSR& A, uint16_t* sell_bf16, hipStream_t st) {
+  if (A.sell.n_slices)
+    hipLaunchKernelGGL(k_sell_fill_bf16, dim3((A.sell.n_slices + 3) / 4), dim3(256), 0, st, A.sell.n_slices,
+                       A.sell.slice_ptr, A.sell.row, A.sell.src, A.val, A.rscale, sell_bf16);
 }
 void csr_to_bf16(const DevCSR& A, uint16_t* vb, float* rscale, hipStream_t st) {
   hipLaunchKernelGGL(k_to_bf16_rows, dim3(grid_for(A.n)), dim3(256), 0, st, A.n, A.row_ptr, A.val, vb,

@@ -7,6 +7,19 @@
 
 namespace pmx {
 
+// Sliced-ELLPACK (SELL-64-sigma) copy of the active rows of a DevCSR: one
+// 64-wide wavefront per slice of 64 rows, entries column-major per slice.
+struct DevSELL {
+  int n_slices = 0;
+  size_t total = 0;               // entries incl. padding
+  int* slice_ptr = nullptr;       // [n_slices+1]
+  int* row = nullptr;             // [n_slices*64] original row (-1: empty slot)
+  int* col = nullptr;             // [total]
+  int* src = nullptr;             // [total] CSR position (-1: padding)
+  double* val = nullptr;          // FP64 values (nullptr: not built)
+  uint16_t* val_bf16 = nullptr;   // bf16 values divided by the row scale
+};
+
 // Device CSR matrix (0-based int32 row pointer / columns, sorted columns).
 struct DevCSR {
   int n = 0;            // rows
@@ -19,7 +32,12 @@ struct DevCSR {
   double* val = nullptr;
   uint16_t* val_bf16 = nullptr;   // mixed precision copy (bf16, row-scaled)
   float* rscale = nullptr;        // per-row scale of the bf16 copy
+  DevSELL sell;                   // optional SELL copy used by the FE-matrix kernels
 };
+
+// SELL values from the CSR values (FP64) / bf16 with the row scales A.rscale
+void sell_fill_f64(const DevCSR& A, double* sell_val, hipStream_t st);
+void sell_fill_bf16(const DevCSR& A, uint16_t* sell_bf16, hipStream_t st);
 
 // ---- FE-matrix operators (FP64)
 void fe_apply_scaled_add(const DevCSR& A, const double* x, double* y, double a, hipStream_t st);

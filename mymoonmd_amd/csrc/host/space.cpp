@@ -301,3 +301,55 @@ void color_cells(const FESpace& s, std::vector<int>& order, std::vector<int>& co
 }
 
 }  // namespace pmx
+
+namespace pmx {
+
+SellPattern build_sell(const CSRPattern& p, int sigma) {
+  constexpr int C = 64;
+  SellPattern s;
+  const int na = p.active_bound;
+  s.n_active = na;
+  s.n_slices = (na + C - 1) / C;
+  std::vector<int> order(na);
+  for (int r = 0; r < na; ++r) order[r] = r;
+  auto len = [&](int r) { return p.row_ptr[r + 1] - p.row_ptr[r]; };
+  for (int w = 0; w < na; w += sigma) {
+    const int e = std::min(na, w + sigma);
+    std::stable_sort(order.begin() + w, order.begin() + e, [&](int a, int b) { return len(a) > len(b); });
+  }
+  s.row.assign((size_t)s.n_slices * C, -1);
+  s.slice_ptr.assign(s.n_slices + 1, 0);
+  for (int sl = 0; sl < s.n_slices; ++sl) {
+    int width = 0;
+    for (int l = 0; l < C; ++l) {
+      const int k = sl * C + l;
+      if (k < na) {
+        s.row[k] = order[k];
+        width = std::max(width, len(order[k]));
+      }
+    }
+    s.slice_ptr[sl + 1] = s.slice_ptr[sl] + width * C;
+  }
+  const size_t total = s.slice_ptr.back();
+  s.col.assign(total, 0);
+  s.src.assign(total, -1);
+#pragma omp parallel for schedule(static)
+  for (int sl = 0; sl < s.n_slices; ++sl) {
+    const int width = (s.slice_ptr[sl + 1] - s.slice_ptr[sl]) / C;
+    for (int l = 0; l < C; ++l) {
+      const int r = s.row[(size_t)sl * C + l];
+      for (int j = 0; j < width; ++j) {
+        const size_t pos = (size_t)s.slice_ptr[sl] + (size_t)j * C + l;
+        if (r >= 0 && j < len(r)) {
+          s.col[pos] = p.col[p.row_ptr[r] + j];
+          s.src[pos] = p.row_ptr[r] + j;
+        } else {
+          s.col[pos] = (r >= 0) ? r : 0;
+        }
+      }
+    }
+  }
+  return s;
+}
+
+}  // namespace pmx

@@ -83,3 +83,22 @@ void transpose_csr(const CSRPattern& A, const std::vector<double>& av, CSRPatter
 void color_cells(const FESpace& s, std::vector<int>& order, std::vector<int>& color_ptr);
 
 }  // namespace pmx
+
+namespace pmx {
+
+// Sliced-ELLPACK copy of the active rows of a CSR pattern (SELL-C-sigma with
+// C = 64 = wavefront width).  Inside windows of `sigma` rows the rows are
+// ordered by decreasing length, so the 64 rows of a slice have (nearly) equal
+// length; entries are stored column-major per slice (entry j of the 64 rows is
+// contiguous), padding entries point at the row itself with value 0.
+struct SellPattern {
+  int n_active = 0;
+  int n_slices = 0;
+  std::vector<int> slice_ptr;   // [n_slices+1] offsets in units of entries
+  std::vector<int> row;         // [n_slices*64] original row of each slot (-1: empty)
+  std::vector<int> col;         // [slice_ptr.back()]
+  std::vector<int> src;         // [slice_ptr.back()] CSR position of the entry (-1: padding)
+};
+SellPattern build_sell(const CSRPattern& p, int sigma);
+
+}  // namespace pmx

@@ -119,8 +119,19 @@ struct pmx_matrix {
   const pmx_space* space = nullptr;
   std::shared_ptr<pmx::CSRPattern> pattern;
   pmx::DevCSR A;
+  bool sell_valid = false;   // SELL values in sync with the CSR values
   ~pmx_matrix() {
-    pmx_internal::dfree(A.row_ptr); pmx_internal::dfree(A.col); pmx_internal::dfree(A.val); pmx_internal::dfree(A.val_bf16); pmx_internal::dfree(A.rscale);
+    using pmx_internal::dfree;
+    dfree(A.row_ptr); dfree(A.col); dfree(A.val); dfree(A.val_bf16); dfree(A.rscale);
+    dfree(A.sell.slice_ptr); dfree(A.sell.row); dfree(A.sell.col); dfree(A.sell.src);
+    dfree(A.sell.val); dfree(A.sell.val_bf16);
   }
 };
+
+namespace pmx_internal {
+// upload the SELL layout of a matrix pattern (once) and mark its values stale
+void attach_sell(pmx_matrix* M, hipStream_t st);
+// refresh the SELL FP64 values from the CSR values when they are stale
+void ensure_sell_values(pmx_matrix* M, hipStream_t st);
+}  // namespace pmx_internal
 

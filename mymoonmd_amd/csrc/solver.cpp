@@ -82,7 +82,7 @@ struct Level {
     dfree(rhs); dfree(sol); dfree(def); dfree(tmp);
     dfree(rhs32); dfree(sol32); dfree(def32); dfree(tmp32);
     dfree(partial);
-    dfree(A.val_bf16); dfree(A.rscale);
+    dfree(A.val_bf16); dfree(A.rscale); dfree(A.sell.val_bf16);
   }
 };
 
@@ -92,6 +92,7 @@ struct pmx_solver {
   pmx_ctx* ctx = nullptr;
   pmx_solver_params p{};
   std::vector<Level> lv;     // coarsest first
+  std::vector<pmx_matrix*> mats;   // level matrices (their SELL values are refreshed per solve)
   std::vector<Dir> cc;
   pmx::DevCSR A;             // finest operator (view)
   int n = 0;
@@ -526,6 +527,9 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
     hipStream_t st = S(s);
     const pmx_matrix* fine = levels[n_levels - 1];
     require(fine != nullptr, "NULL matrix");
+    s->mats.assign(levels, levels + n_levels);
+    for (int l = 0; l < n_levels; ++l)
+      if (levels[l]) ensure_sell_values(levels[l], st);
     s->A = fine->A;
     s->n = fine->A.n;
     const int n = s->n;
@@ -555,6 +559,7 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
         L.A = M->A;
         L.A.val_bf16 = nullptr;
         L.A.rscale = nullptr;
+        L.A.sell.val_bf16 = nullptr;
         L.n = M->A.n;
         L.active = M->A.active;
         L.partial = dmalloc<double>(std::max(4096, pmx::fe_defect_partials(L.A)));
@@ -566,6 +571,10 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
           L.A.val_bf16 = dmalloc<uint16_t>(L.A.nnz);
           L.A.rscale = dmalloc<float>(L.n);
           pmx::csr_to_bf16(L.A, L.A.val_bf16, L.A.rscale, st);
+          if (L.A.sell.total) {
+            L.A.sell.val_bf16 = dmalloc<uint16_t>(L.A.sell.total);
+            pmx::sell_fill_bf16(L.A, L.A.sell.val_bf16, st);
+          }
           L.dinv32 = dmalloc<float>(L.n);
           pmx::vec_f64_to_f32(L.n, L.dinv, L.dinv32, st);
         } else {
@@ -599,6 +608,9 @@ int pmx_solver_solve(pmx_solver* s, const double* b, double* x, pmx_solve_report
     if (!s->updated) throw PmxError(PMX_E_STATE, "pmx_solver_update must be called before solve");
     s->history.clear();
     *rep = pmx_solve_report{};
+    for (pmx_matrix* M : s->mats)
+      if (M && !M->sell_valid)
+        throw PmxError(PMX_E_STATE, "matrix values changed since pmx_solver_update; update the solver first");
     switch (s->p.iterative_solver_type) {
       case PMX_IT_FGMRES: solve_fgmres(s, b, x, rep); break;
       case PMX_IT_CG: solve_cg(s, b, x, rep); break;

@@ -83,7 +83,7 @@ def test_spmv_matches_oracle(ctx):
     rng = np.random.default_rng(3)
     x = rng.standard_normal(s.n_dof)
     y0 = rng.standard_normal(s.n_dof)
-    for lanes in (8, 16, 32, 64):
+    for lanes in (0, 8, 16, 32, 64):   # 0: SELL-64 kernel
         A.set_spmv_lanes(lanes)
         yg = dev(y0)
         A.apply_scaled_add(dev(x), yg, -0.7)
