@@ -394,18 +394,33 @@ void fe_apply(const DevCSR& A, const double* x, double* y, hipStream_t st) {
   dispatch_fe_spmv<MODE_APPLY, double, double>(A, A.val, nullptr, x, nullptr, nullptr, y, 1.0,
                                                nullptr, st);
 }
+// number of blocks (= partial sums) the FE-matrix kernel launches for A
+static int spmv_grid(const DevCSR& A, bool bf16) {
+  const bool have_sell = bf16 ? (A.sell.val_bf16 != nullptr) : (A.sell.val != nullptr);
+  if (have_sell && A.lpr == 0) {
+    int sb = 0;
+    return sell_grid(A, &sb);
+  }
+  const int lpr = A.lpr ? A.lpr : 64;
+  return (A.n + (256 / lpr) - 1) / (256 / lpr);
+}
+
 void fe_defect(const DevCSR& A, const double* x, const double* b, double* d, double* partial,
                double* norm_out, hipStream_t st) {
   dispatch_fe_spmv<MODE_DEFECT, double, double>(A, A.val, nullptr, x, b, nullptr, d, 1.0, partial,
                                                 st);
-  const int nb = fe_defect_partials(A);
+  const int nb = spmv_grid(A, false);
   if (norm_out) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, nb, partial, norm_out, 1);
 }
+// upper bound of the partial sums any FE-matrix kernel writes for A
 int fe_defect_partials(const DevCSR& A) {
-  const int csr = (A.n + (256 / (A.lpr ? A.lpr : 64)) - 1) / (256 / (A.lpr ? A.lpr : 64));
+  const int lpr = A.lpr ? A.lpr : 64;
+  const int csr = (A.n + (256 / lpr) - 1) / (256 / lpr);
+  const int csr64 = (A.n + 3) / 4;
   int sb = 0;
   const int sell = A.sell.n_slices ? sell_grid(A, &sb) : 0;
-  return csr > sell ? csr : sell;
+  int m = csr > sell ? csr : sell;
+  return m > csr64 ? m : csr64;
 }
 void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* dinv, double* xn,
                double omega, hipStream_t st) {
@@ -422,7 +437,7 @@ void fe_defect_bf16(const DevCSR& A, const float* x, const float* b, float* d, d
                     double* norm_out, hipStream_t st) {
   dispatch_fe_spmv<MODE_DEFECT, uint16_t, float>(A, A.val_bf16, A.rscale, x, b, nullptr, d, 1.0,
                                                  partial, st);
-  const int nb = fe_defect_partials(A);
+  const int nb = spmv_grid(A, true);
   if (norm_out) hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, nb, partial, norm_out, 1);
 }
 void fe_jacobi_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv, float* xn,

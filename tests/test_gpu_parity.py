@@ -254,6 +254,9 @@ def test_multigrid_cycle_matches_oracle(ctx):
 
 
 def test_fgmres_multigrid_solution_matches_oracle(ctx):
+    """FGMRES(30) + MG V(2,2) to 1e-8.  Damping 0.5 keeps the Jacobi smoother
+    contractive for this Q2 SUPG operator (lambda_max(D^-1 A) ~ 3), so the
+    iteration count is insensitive to round-off and comparable across paths."""
     nref = 3
     d, spaces, mats, rhs = three_level_cd3d(ctx, nref, MG_KW)
     osp, vals = oracle_levels(spaces, mats, nref)
@@ -264,13 +267,13 @@ def test_fgmres_multigrid_solution_matches_oracle(ctx):
     kw = dict(iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
               residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=200,
               multigrid_cycle_type="V", multigrid_smoother="jacobi", multigrid_smoother_coarse="jacobi",
-              multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2, jacobi_damp=0.8)
+              multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2, jacobi_damp=0.5)
     sol = pm.Solver(ctx, pm.solver_params(**kw))
     sol.update_matrix(mats, spaces)
     rep = sol.solve(b, x)
     op = O.solver_params(iterative_solver_type=0, preconditioner=2, gmres_restart=30, residual_tolerance=0.0,
                          residual_reduction=1e-8, max_n_iterations=200, mg_cycle=0, mg_smoother_coarse=0,
-                         mg_n_pre_smooth=2, mg_n_post_smooth=2, jacobi_damp=0.8)
+                         mg_n_pre_smooth=2, mg_n_post_smooth=2, jacobi_damp=0.5)
     bo = b.cpu().numpy()
     x0 = np.zeros_like(bo)
     x0[fs.active_bound:] = bo[fs.active_bound:]
@@ -302,7 +305,7 @@ def test_mixed_precision_reaches_fp64_residual(ctx):
             iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
             residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=300,
             multigrid_cycle_type="V", multigrid_smoother="jacobi", multigrid_smoother_coarse="jacobi",
-            multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2, jacobi_damp=0.8, precision=prec))
+            multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2, jacobi_damp=0.5, precision=prec))
         sol.update_matrix(mats, spaces)
         rep = sol.solve(b, x)
         assert rep.converged
