@@ -97,8 +97,13 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
   using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
   double d2 = 0;
   if ((int)blockIdx.x < sell_blocks) {
+    // workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of slices so that its L2 sees neighbouring rows (and
+    // the x entries they share).  sell_blocks is a multiple of 8.
+    const int per = sell_blocks >> 3;
+    const int vb = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     const int lane = threadIdx.x & 63;
-    const int slice = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    const int slice = vb * 4 + (int)(threadIdx.x >> 6);
     if (slice < n_slices) {
       const int r = rowmap[slice * 64 + lane];
       const int s0 = sptr[slice];
@@ -167,7 +172,7 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
 }
 
 int sell_grid(const DevCSR& A, int* sell_blocks) {
-  *sell_blocks = (A.sell.n_slices + 3) / 4;
+  *sell_blocks = (((A.sell.n_slices + 3) / 4 + 7) / 8) * 8;
   return *sell_blocks + (A.n - A.active + 255) / 256;
 }
 
