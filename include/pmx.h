@@ -245,6 +245,9 @@ typedef struct {
   int iterations;
   double initial_residual;
   double final_residual;
+  /* 1 when the stop met residual_tolerance or residual_reduction; 0 when the
+   * loop ended at max_n_iterations or on stagnation (the reference's
+   * converged() returns true in all these cases and only its log differs) */
   int converged;
 } pmx_solve_report;
 /* Solver::solve(rhs, solution) (src/Solver/Solver.C:414-488): iterates from

@@ -103,6 +103,7 @@ struct pmx_solver {
   double* partial = nullptr;
   std::vector<double> history;
   double initial_res = 0, previous_res = 0;
+  bool reached = false;   // the last stop met residual_tolerance or residual_reduction
   bool updated = false;
   void release() {
     for (auto& l : lv) l.release();
@@ -139,6 +140,7 @@ bool converged(pmx_solver* s, double res, int it) {
   const bool stagnation = res > 1e10 * s->previous_res;
   const bool diverged = res > p.divergence_factor * s->initial_res;
   bool conv = (small || reduced || stagnation || diverged) && enough;
+  s->reached = (small || reduced) && enough;
   if (it == p.max_n_iterations) conv = true;
   s->previous_res = res;
   if (diverged) throw PmxError(PMX_E_DIVERGED, "Execution of the code stopped because of solver divergence.");
@@ -339,7 +341,7 @@ void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
   double beta = read_scalar(s, s->d_scal);
   double resid = beta;
   rep->initial_residual = beta;
-  if (converged(s, resid, 0)) { rep->iterations = 0; rep->final_residual = resid; rep->converged = 1; return; }
+  if (converged(s, resid, 0)) { rep->iterations = 0; rep->final_residual = resid; rep->converged = s->reached ? 1 : 0; return; }
   int j = 1;
   while (j <= s->p.max_n_iterations) {
     pmx::vec_scale(n, 1.0 / beta, nullptr, 0, s->r, s->V[0], S(s));
@@ -369,7 +371,7 @@ void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
       if (converged(s, resid, j)) {
         gmres_update(s, x, i, H, ld, sv);
         HIP_CHECK(hipStreamSynchronize(S(s)));
-        rep->iterations = j; rep->final_residual = resid; rep->converged = 1;
+        rep->iterations = j; rep->final_residual = resid; rep->converged = s->reached ? 1 : 0;
         return;
       }
     }
@@ -377,7 +379,7 @@ void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
     pmx::fe_defect(s->A, x, b, s->r, s->partial, s->d_scal, S(s));
     beta = read_scalar(s, s->d_scal);
     if (converged(s, resid, j)) {
-      rep->iterations = j; rep->final_residual = resid; rep->converged = 1;
+      rep->iterations = j; rep->final_residual = resid; rep->converged = s->reached ? 1 : 0;
       return;
     }
   }
@@ -399,7 +401,7 @@ void solve_cg(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep) 
   if (normb == 0.0) normb = 1;
   double resid = rn / normb;
   rep->initial_residual = resid;
-  if (converged(s, resid, 0)) { rep->iterations = 0; rep->final_residual = resid; rep->converged = 1; return; }
+  if (converged(s, resid, 0)) { rep->iterations = 0; rep->final_residual = resid; rep->converged = s->reached ? 1 : 0; return; }
   double rho_1 = 0;
   for (int i = 1; i <= s->p.max_n_iterations; ++i) {
     precondition(s, r, z);
@@ -418,7 +420,7 @@ void solve_cg(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep) 
     pmx::vec_axpy(n, -alpha, nullptr, 1.0, q, r, S(s));
     pmx::vec_norm(n, r, s->partial, s->d_scal, S(s));
     resid = read_scalar(s, s->d_scal) / normb;
-    if (converged(s, resid, i)) { rep->iterations = i; rep->final_residual = resid; rep->converged = 1; return; }
+    if (converged(s, resid, i)) { rep->iterations = i; rep->final_residual = resid; rep->converged = s->reached ? 1 : 0; return; }
     rho_1 = rho;
   }
   rep->iterations = s->p.max_n_iterations; rep->final_residual = resid; rep->converged = 0;
@@ -435,7 +437,7 @@ void solve_richardson(pmx_solver* s, const double* b, double* x, pmx_solve_repor
   pmx::fe_defect(s->A, x, b, r, s->partial, s->d_scal, S(s));
   double resid = read_scalar(s, s->d_scal) / normb;
   rep->initial_residual = resid;
-  if (converged(s, resid, 0)) { rep->iterations = 0; rep->final_residual = resid; rep->converged = 1; return; }
+  if (converged(s, resid, 0)) { rep->iterations = 0; rep->final_residual = resid; rep->converged = s->reached ? 1 : 0; return; }
   const double damp = s->p.damping;
   for (int i = 1; i <= s->p.max_n_iterations; ++i) {
     if (jacobi) pmx::vec_mul(n, s->dinv, r, z, S(s));
@@ -443,7 +445,7 @@ void solve_richardson(pmx_solver* s, const double* b, double* x, pmx_solve_repor
     pmx::vec_axpy(n, damp, nullptr, 1.0, z, x, S(s));
     pmx::fe_defect(s->A, x, b, r, s->partial, s->d_scal, S(s));
     resid = read_scalar(s, s->d_scal) / normb;
-    if (converged(s, resid, i)) { rep->iterations = i; rep->final_residual = resid; rep->converged = 1; return; }
+    if (converged(s, resid, i)) { rep->iterations = i; rep->final_residual = resid; rep->converged = s->reached ? 1 : 0; return; }
   }
   rep->iterations = s->p.max_n_iterations; rep->final_residual = resid; rep->converged = 0;
 }
