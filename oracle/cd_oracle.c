@@ -14,6 +14,11 @@
  *                    Dirichlet diagonal set to 1 (:393-405), Dirichlet rhs set to
  *                    the boundary value at the nodal points (:593-667), Neumann
  *                    line integrals in 2D (Assemble2D.C:7529-7570)
+ *   or_assemble_tcd  the same cell loop with the TCD forms TCDStiff, TCDMass,
+ *                    TCDRhs and their 2D SUPG variants (CD_local_assembling_
+ *                    routines.C:6-330), matrices selected as in
+ *                    LocalAssembling<d>::set_parameters_for_tcd (LocalAssembling.C:842-928)
+ *   or_interpolate   TFEFunction2D::Interpolate for Lagrange elements
  *   or_errors        conv_diff_l2_h1_linf_error<d> (ConvDiff.C:2134-2185)
  *   or_apply_scaled_add  FEMatrix::multiplyActive + addScaledNonActive
  *                    (FEMatrix.C:186-199, BlockVector.C:212-228)
@@ -261,9 +266,24 @@ static double cell_diameter(int shape, const double* v) {
 }
 
 /* ------------------------------------------------------------ examples */
-static void exact(int ex, double x, double y, double z, double* v) {
-  v[0] = v[1] = v[2] = v[3] = v[4] = 0;
+/* v[0] = u, v[1..3] = grad u, v[4] = laplace u, v[5] = du/dt at time t */
+static void exact(int ex, double x, double y, double z, double t, double* v) {
+  v[0] = v[1] = v[2] = v[3] = v[4] = v[5] = 0;
   switch (ex) {
+    case OR_EX_TCD_EXP: {
+      /* include/Examples/TCD_2D/exp.h: u = e^t sin(2 pi x) sin(2 pi y) */
+      const double e = exp(t), sx = sin(2 * PI * x), sy = sin(2 * PI * y);
+      v[0] = e * (sx * sy);
+      v[1] = e * 2 * PI * cos(2 * PI * x) * sy;
+      v[2] = e * 2 * PI * sx * cos(2 * PI * y);
+      v[4] = -8 * PI * PI * v[0];
+      v[5] = v[0];
+      break;
+    }
+    case OR_EX_TCD_LINEAR:
+      /* include/Examples/TCD_2D/linear_space_time.h: u = 1 + 2x + 3ty */
+      v[0] = 1 + 2 * x + 3 * t * y; v[1] = 2; v[2] = 3 * t; v[5] = 3 * y;
+      break;
     case OR_EX_SINE_LAPLACE_2D:
     case OR_EX_SINE_2D:
       v[0] = sin(PI * x) * sin(PI * y);
@@ -287,19 +307,21 @@ static void exact(int ex, double x, double y, double z, double* v) {
 
 /* coefficients [eps, b1, b2, b3, c, f] of the example's BilinearCoeffs */
 static void coeffs(const or_cd* p, double x, double y, double z, double* c) {
-  double v[5];
+  double v[6];
   c[0] = p->eps; c[1] = p->b[0]; c[2] = p->b[1]; c[3] = p->b[2]; c[4] = p->c;
   if (p->example == OR_EX_CONST) { c[5] = p->f0; return; }
-  exact(p->example, x, y, z, v);
-  c[5] = -c[0] * v[4];
+  exact(p->example, x, y, z, p->time, v);
+  c[5] = v[5] - c[0] * v[4];
   c[5] += c[1] * v[1] + c[2] * v[2] + c[3] * v[3];
   c[5] += c[4] * v[0];
 }
 
 static double boundary_value(const or_cd* p, double x, double y, double z) {
-  double v[5];
-  if (p->example == OR_EX_SINE_LAPLACE_2D || p->example == OR_EX_CONST) return 0.0;
-  exact(p->example, x, y, z, v);
+  double v[6];
+  if (p->example == OR_EX_SINE_LAPLACE_2D || p->example == OR_EX_CONST ||
+      p->example == OR_EX_TCD_EXP)
+    return 0.0;
+  exact(p->example, x, y, z, p->time, v);
   return v[0];
 }
 
@@ -478,8 +500,8 @@ void or_assemble_cd(const or_space* s, const or_cd* p, double* val, double* rhs)
         const double hE = sqrt((b[0] - a[0]) * (b[0] - a[0]) + (b[1] - a[1]) * (b[1] - a[1])) / 2;
         for (int l = 0; l < npts; ++l) {
           const double t = t0 + 0.5 * (t1 - t0) * (zx[l] + 1);
-          double ev[5];
-          exact(p->example, 1.0, t, 0.0, ev);
+          double ev[6];
+          exact(p->example, 1.0, t, 0.0, p->time, ev);
           double g = p->eps * ev[1];
           g *= hE * zw[l];
           double x[2] = {a[0] + 0.5 * (zx[l] + 1) * (b[0] - a[0]), a[1] + 0.5 * (zx[l] + 1) * (b[1] - a[1])};
@@ -522,8 +544,8 @@ void or_errors(const or_space* s, const or_cd* p, const double* sol, double* err
           g[a] += sol[dof[k]] * t;
         }
       }
-      double ev[5], cf[6];
-      exact(p->example, x[0], x[1], dim == 3 ? x[2] : 0.0, ev);
+      double ev[6], cf[6];
+      exact(p->example, x[0], x[1], dim == 3 ? x[2] : 0.0, p->time, ev);
       coeffs(p, x[0], x[1], dim == 3 ? x[2] : 0.0, cf);
       const double bb[3] = {cf[1], cf[2], dim == 3 ? cf[3] : 0.0};
       const double delta = sdfem_delta(hK, cf[0], bb, dim, p);
@@ -543,6 +565,184 @@ void or_errors(const or_space* s, const or_cd* p, const double* sol, double* err
   }
   err[0] = sqrt(e0s); err[1] = sqrt(e1s); err[2] = sqrt(sds); err[3] = linf;
   free(u); free(gr);
+}
+
+/* Physical basis data at one reference point of an affine cell: values u,
+ * gradients ux[3][nd] (third row zero in 2D), Laplacians lap (if non-NULL),
+ * the physical point x and |det J|. */
+static double phys_basis(int shape, int order, const double* v, const double* xi, double* u,
+                         double* gr, double* hr, double* ux, double* lap, double* x) {
+  const int dim = (shape == OR_HEXA) ? 3 : 2, nd = or_ndof(shape, order);
+  double J[9], Ji[9];
+  x[0] = x[1] = x[2] = 0;
+  ref_map(shape, v, xi, x, J);
+  const double det = inv_jac(dim, J, Ji);
+  or_basis(shape, order, xi, u, gr, hr);
+  for (int k = 0; k < nd; ++k) {
+    for (int a = 0; a < dim; ++a) {
+      double t = 0;
+      for (int r = 0; r < dim; ++r) t += gr[r * nd + k] * Ji[r * dim + a];
+      ux[a * nd + k] = t;
+    }
+    if (dim == 2) ux[2 * nd + k] = 0;
+    if (lap) {
+      double l = 0;
+      for (int a = 0; a < dim; ++a)
+        for (int r = 0; r < dim; ++r)
+          for (int t = 0; t < dim; ++t) {
+            static const int idx3[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+            static const int idx2[2][2] = {{0, 1}, {1, 2}};
+            const int comp = (dim == 3) ? idx3[r][t] : idx2[r][t];
+            l += hr[comp * nd + k] * Ji[r * dim + a] * Ji[t * dim + a];
+          }
+      lap[k] = l;
+    }
+  }
+  return fabs(det);
+}
+
+/* scatter of one local matrix: active rows add (linear search of the row),
+ * Dirichlet rows get their diagonal set to 1 */
+static void scatter_matrix(const or_space* s, const int* dof, const double* A, double* val) {
+  const int nd = s->ndof_cell;
+  for (int m = 0; m < nd; ++m) {
+    const int l = dof[m];
+    if (l < s->active) {
+      for (int k = 0; k < nd; ++k)
+        for (int n = s->rp[l]; n < s->rp[l + 1]; ++n)
+          if (s->col[n] == dof[k]) { val[n] += A[m * nd + k]; break; }
+    } else {
+      for (int n = s->rp[l]; n < s->rp[l + 1]; ++n)
+        if (s->col[n] == l) { val[n] = 1.0; break; }
+    }
+  }
+}
+
+/* Time-dependent convection-diffusion forms, per quadrature point in the
+ * order LocalAssembling<d> (TCDStiffMassRhs) calls them
+ * (src/AssembleRoutines/LocalAssembling.C:862-895):
+ *   TCDStiff, TCDMass, TCDRhs (CD_local_assembling_routines.C:6-40, 83-93, 118-128)
+ *   and for SUPG in 2D TCDStiffSUPG, TCDMassSUPG, TCDRhsSUPG (:130-330), whose
+ *   coefficients are eps*theta1*tau, b*theta1*tau, 1 + c*theta1*tau, with the
+ *   mass-term delta multiplied by theta1*tau.
+ * Scatter and Dirichlet handling as Assemble2D/3D (every square matrix gets
+ * diagonal 1 in Dirichlet rows; rhs gets the boundary value). */
+void or_assemble_tcd(const or_space* s, const or_cd* p, double* aval, double* mval, double* rhs) {
+  const int shape = s->shape, dim = (shape == OR_HEXA) ? 3 : 2, nd = s->ndof_cell;
+  const int nc = (dim == 3) ? 6 : 3;
+  const int supg = (p->disc == OR_SUPG) && dim == 2;
+  double qxi[64 * 3], qw[64];
+  const int nq = or_quad_rule(shape, s->order, p->hexa_rule, qxi, qw);
+  double* u = malloc(sizeof(double) * nd);
+  double* gr = malloc(sizeof(double) * dim * nd);
+  double* hr = malloc(sizeof(double) * nc * nd);
+  double* ux = malloc(sizeof(double) * 3 * nd);
+  double* lap = malloc(sizeof(double) * nd);
+  double* A = malloc(sizeof(double) * nd * nd);
+  double* M = malloc(sizeof(double) * nd * nd);
+  double* F = malloc(sizeof(double) * nd);
+  for (int cell = 0; cell < s->n_cells; ++cell) {
+    const double* v = s->verts + (size_t)cell * s->nv * dim;
+    const int* dof = s->gnum + (size_t)cell * nd;
+    const double hK = cell_diameter(shape, v);
+    memset(A, 0, sizeof(double) * nd * nd);
+    memset(M, 0, sizeof(double) * nd * nd);
+    memset(F, 0, sizeof(double) * nd);
+    for (int q = 0; q < nq; ++q) {
+      double x[3];
+      const double det = phys_basis(shape, s->order, v, qxi + q * dim, u, gr, hr, ux, lap, x);
+      double cf[6];
+      coeffs(p, x[0], x[1], dim == 3 ? x[2] : 0.0, cf);
+      const double Mult = qw[q] * det;
+      const double eps = cf[0], b1 = cf[1], b2 = cf[2], b3 = (dim == 3) ? cf[3] : 0.0;
+      const double c = cf[4], f = cf[5];
+      /* TCDStiff */
+      for (int i = 0; i < nd; ++i) {
+        const double t0 = u[i], tx = ux[i], ty = ux[nd + i], tz = ux[2 * nd + i];
+        for (int j = 0; j < nd; ++j) {
+          double val = eps * (tx * ux[j] + ty * ux[nd + j] + tz * ux[2 * nd + j]);
+          val += (b1 * ux[j] + b2 * ux[nd + j] + b3 * ux[2 * nd + j]) * t0;
+          val += c * t0 * u[j];
+          A[i * nd + j] += Mult * val;
+        }
+      }
+      /* TCDMass */
+      for (int i = 0; i < nd; ++i)
+        for (int j = 0; j < nd; ++j) M[i * nd + j] += Mult * u[i] * u[j];
+      /* TCDRhs */
+      for (int i = 0; i < nd; ++i) F[i] += Mult * u[i] * f;
+      if (supg) {
+        const double t1 = p->theta1, tau = p->tau;
+        const double se = eps * t1 * tau, sb1 = b1 * t1 * tau, sb2 = b2 * t1 * tau;
+        const double sc = 1. + c * t1 * tau;
+        const double sbv[3] = {sb1, sb2, 0.0};
+        const double delta = sdfem_delta(hK, se, sbv, 2, p);
+        /* TCDStiffSUPG */
+        for (int i = 0; i < nd; ++i) {
+          const double bgradv = delta * (sb1 * ux[i] + sb2 * ux[nd + i]);
+          for (int j = 0; j < nd; ++j) {
+            double val = (sb1 * ux[j] + sb2 * ux[nd + j]) * bgradv;
+            val += sc * bgradv * u[j];
+            val -= se * lap[j] * bgradv;
+            A[i * nd + j] += Mult * val;
+          }
+        }
+        /* TCDMassSUPG */
+        const double dm = delta * t1 * tau;
+        for (int i = 0; i < nd; ++i) {
+          const double bgradv = dm * (sb1 * ux[i] + sb2 * ux[nd + i]);
+          for (int j = 0; j < nd; ++j) M[i * nd + j] += Mult * u[j] * bgradv;
+        }
+        /* TCDRhsSUPG */
+        for (int i = 0; i < nd; ++i) {
+          const double bgradv = delta * (sb1 * ux[i] + sb2 * ux[nd + i]);
+          F[i] += Mult * f * bgradv;
+        }
+      }
+    }
+    if (aval) scatter_matrix(s, dof, A, aval);
+    if (mval) scatter_matrix(s, dof, M, mval);
+    if (rhs) {
+      for (int m = 0; m < nd; ++m)
+        if (dof[m] < s->active) rhs[dof[m]] += F[m];
+      for (int j = 0; j < nfaces(shape); ++j) {
+        if (s->bd_comp[cell * nfaces(shape) + j] < 0) continue;
+        for (int k = 0; k < nd; ++k) {
+          if (!dof_on_joint(shape, s->order, k, j)) continue;
+          double xi[3], x[3] = {0, 0, 0}, J[9];
+          node_ref(shape, s->order, k, xi);
+          ref_map(shape, v, xi, x, J);
+          rhs[dof[k]] = boundary_value(p, x[0], x[1], dim == 3 ? x[2] : 0.0);
+        }
+      }
+    }
+  }
+  free(u); free(gr); free(hr); free(ux); free(lap); free(A); free(M); free(F);
+}
+
+/* TFEFunction2D/3D::Interpolate of the exact solution (Lagrange nodal points) */
+void or_interpolate(const or_space* s, const or_cd* p, double* out) {
+  const int shape = s->shape, dim = (shape == OR_HEXA) ? 3 : 2, nd = s->ndof_cell;
+  for (int cell = 0; cell < s->n_cells; ++cell) {
+    const double* v = s->verts + (size_t)cell * s->nv * dim;
+    const int* dof = s->gnum + (size_t)cell * nd;
+    for (int k = 0; k < nd; ++k) {
+      double xi[3], x[3] = {0, 0, 0}, J[9], ev[6];
+      node_ref(shape, s->order, k, xi);
+      ref_map(shape, v, xi, x, J);
+      exact(p->example, x[0], x[1], dim == 3 ? x[2] : 0.0, p->time, ev);
+      out[dof[k]] = ev[0];
+    }
+  }
+}
+
+void or_multiply_active(int active, const int* rp, const int* col, const double* val,
+                        const double* x, double* y, double a) {
+  for (int i = 0; i < active; ++i) {
+    double s = 0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) s += val[k] * x[col[k]];
+    y[i] += a * s;
+  }
 }
 
 /* ------------------------------------------------------ linear algebra */

@@ -16,6 +16,8 @@
 #define OR_EX_P2_3D 4
 #define OR_EX_EXP_SIN_3D 5
 #define OR_EX_SINE_2D 6
+#define OR_EX_TCD_EXP 7      /* TCD_2D/exp.h (reference example 0) */
+#define OR_EX_TCD_LINEAR 8   /* TCD_2D/linear_space_time.h (reference example -1) */
 
 typedef struct {
   int shape, order, nv, n_cells, ndof_cell, n_dof, active;
@@ -29,6 +31,9 @@ typedef struct {
 typedef struct {
   int disc, example, hexa_rule, bc_sine_laplace;
   double eps, b[3], c, f0, delta0, poly_degree;
+  double time;     /* TimeDB->CURRENTTIME (time-dependent examples) */
+  double theta1;   /* TimeDB->THETA1 (TCD SUPG) */
+  double tau;      /* TimeDB->CURRENTTIMESTEPLENGTH (TCD SUPG) */
 } or_cd;
 
 typedef struct {
@@ -58,6 +63,13 @@ void or_basis(int shape, int order, const double* xi, double* v, double* g, doub
 int or_quad_rule(int shape, int order, int hexa_rule, double* xi, double* w);
 void or_assemble_cd(const or_space* s, const or_cd* p, double* val, double* rhs);
 void or_errors(const or_space* s, const or_cd* p, const double* sol, double* err4);
+/* TCD forms; aval, mval, rhs may each be NULL */
+void or_assemble_tcd(const or_space* s, const or_cd* p, double* aval, double* mval, double* rhs);
+/* y[i] += a * (A x)[i] for active rows only (FEMatrix::multiplyActive) */
+void or_multiply_active(int active, const int* rp, const int* col, const double* val,
+                        const double* x, double* y, double a);
+/* interpolation of the example's exact solution at time p->time at the nodal points */
+void or_interpolate(const or_space* s, const or_cd* p, double* u);
 void or_apply_scaled_add(int n, int active, const int* rp, const int* col, const double* val,
                          const double* x, double* y, double a);
 void or_prolongate(const or_space* cs, const or_space* fs, const int* parent, const double* cu, double* fu);

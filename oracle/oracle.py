@@ -371,7 +371,8 @@ class CD(ctypes.Structure):
     _fields_ = [("disc", ctypes.c_int), ("example", ctypes.c_int), ("hexa_rule", ctypes.c_int),
                 ("bc_sine_laplace", ctypes.c_int), ("eps", ctypes.c_double),
                 ("b", ctypes.c_double * 3), ("c", ctypes.c_double), ("f0", ctypes.c_double),
-                ("delta0", ctypes.c_double), ("poly_degree", ctypes.c_double)]
+                ("delta0", ctypes.c_double), ("poly_degree", ctypes.c_double),
+                ("time", ctypes.c_double), ("theta1", ctypes.c_double), ("tau", ctypes.c_double)]
 
 
 class SolverParams(ctypes.Structure):
@@ -401,6 +402,11 @@ def lib():
         L = ctypes.CDLL(LIB)
         L.or_assemble_cd.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p, ctypes.c_void_p]
         L.or_errors.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p, ctypes.c_void_p]
+        L.or_assemble_tcd.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p]
+        L.or_interpolate.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p]
+        L.or_multiply_active.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
         L.or_apply_scaled_add.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
         L.or_prolongate.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(Space), ctypes.c_void_p,
@@ -457,10 +463,11 @@ def build_space(geometry, n_refine, level, order, cond2d=None, cond3d=None):
 
 
 def cd_params(example=0, disc=0, eps=1.0, b=(0, 0, 0), c=0.0, f0=0.0, hexa_rule=0,
-              sine_laplace_bc=False, delta0=1.0, poly_degree=1.0):
+              sine_laplace_bc=False, delta0=1.0, poly_degree=1.0, time=0.0, theta1=1.0, tau=0.0):
     p = CD()
     p.disc, p.example, p.hexa_rule, p.bc_sine_laplace = disc, example, hexa_rule, int(sine_laplace_bc)
     p.eps, p.c, p.f0, p.delta0, p.poly_degree = eps, c, f0, delta0, poly_degree
+    p.time, p.theta1, p.tau = time, theta1, tau
     for k in range(3):
         p.b[k] = b[k] if k < len(b) else 0.0
     return p
@@ -472,6 +479,115 @@ def assemble(space, params):
     s = space.cstruct()
     lib().or_assemble_cd(ctypes.byref(s), ctypes.byref(params), _p(val), _p(rhs))
     return val, rhs
+
+
+def assemble_tcd(space, params, stiffness=True, mass=True, rhs=True):
+    """TCD stiffness, mass and rhs (each None when not requested)."""
+    a = np.zeros(space.nnz) if stiffness else None
+    m = np.zeros(space.nnz) if mass else None
+    f = np.zeros(space.n_dof) if rhs else None
+    s = space.cstruct()
+    lib().or_assemble_tcd(ctypes.byref(s), ctypes.byref(params), None if a is None else _p(a),
+                          None if m is None else _p(m), None if f is None else _p(f))
+    return a, m, f
+
+
+def interpolate(space, params):
+    out = np.zeros(space.n_dof)
+    s = space.cstruct()
+    lib().or_interpolate(ctypes.byref(s), ctypes.byref(params), _p(out))
+    return out
+
+
+def multiply_active(active, rp, col, val, x, y, a):
+    rp = np.ascontiguousarray(rp, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    assert y.dtype == np.float64 and y.flags["C_CONTIGUOUS"]
+    lib().or_multiply_active(active, _p(rp), _p(col), _p(val), _p(x), _p(y), a)
+    return y
+
+
+BACKWARD_EULER, CRANK_NICOLSON, BDF2 = 0, 1, 2
+
+
+def tcd_time_integration(space, example, disc, scheme, tau, n_steps, eps=1.0, b=(0.0, 0.0), c=0.0,
+                         t0=0.0, solver=None, on_step=None):
+    """Restatement of TimeConvectionDiffusion<2> (src/System/TimeConvectionDiffusion.C:
+    294-386 assemble/solve, 222-290 assemble_initial_time) driven like
+    test/tcd2d_test.c++:103-132, with the TimeDiscretization steps
+    prepare_rhs_from_time_disc / prepare_system_matrix
+    (src/TimeDiscRoutines/TimeDiscretizations.C:106-210).  THETA1 follows
+    TimeDiscretization::set_time_disc_parameters (:341-381).  `solver(S, rhs, x0)`
+    returns the solution of the system (default: FGMRES + Jacobi to 1e-14).
+    Returns the (L2, H1) errors after the initial interpolation and after every step."""
+    ab, rp, col = space.active, space.rp, space.col
+    na = int(rp[ab])
+    supg = disc == 1
+
+    def theta(step):
+        if scheme == CRANK_NICOLSON:
+            return 0.5
+        if scheme == BDF2 and step >= 2:
+            return 2.0 / 3.0
+        return 1.0
+
+    def params(t, step):
+        return cd_params(example=example, disc=disc, eps=eps, b=tuple(b) + (0.0,), c=c, time=t,
+                         theta1=theta(step), tau=tau)
+
+    if solver is None:
+        sp = solver_params(iterative_solver_type=0, preconditioner=1, gmres_restart=50,
+                           max_n_iterations=2000, residual_tolerance=1e-14, residual_reduction=0.0)
+
+        def solver(S, rhs, x0):
+            x, it, _, _ = solve(sp, [space], [S], rhs, x0)
+            return x
+
+    t = t0
+    p = params(t, 0)
+    u = interpolate(space, p)
+    A, M, f = assemble_tcd(space, p)
+    old_rhs = f.copy()
+    u_m1, u_m2 = u.copy(), u.copy()
+    errs = [errors(space, p, u)[:2]]
+    for step in range(1, n_steps + 1):
+        t += tau
+        p = params(t, step)
+        if supg:
+            A, M, f = assemble_tcd(space, p)
+        else:
+            A, _, f = assemble_tcd(space, p, mass=False)
+        rhs_td = f.copy()
+        bdf2_full = scheme == BDF2 and step >= 2
+        if scheme == BACKWARD_EULER or (scheme == BDF2 and not bdf2_full):
+            rhs_td[:ab] *= tau
+            multiply_active(ab, rp, col, M, u_m1, rhs_td, 1.0)
+            factor = tau
+        elif scheme == CRANK_NICOLSON:
+            rhs_td[:ab] *= 0.5 * tau
+            rhs_td[:ab] += 0.5 * tau * old_rhs[:ab]
+            multiply_active(ab, rp, col, M, u_m1, rhs_td, 1.0)
+            multiply_active(ab, rp, col, A, u_m1, rhs_td, -0.5 * tau)
+            factor = 0.5 * tau
+        else:
+            rhs_td[:ab] *= (2.0 / 3.0) * tau
+            multiply_active(ab, rp, col, M, u_m1, rhs_td, 4.0 / 3.0)
+            multiply_active(ab, rp, col, M, u_m2, rhs_td, -1.0 / 3.0)
+            factor = (2.0 / 3.0) * tau
+        old_rhs = f.copy()
+        rhs_td[ab:] = f[ab:]
+        S = A.copy()
+        S[:na] *= factor
+        S[:na] += M[:na]
+        u[ab:] = f[ab:]
+        u = solver(S, rhs_td, u)
+        u_m2, u_m1 = u_m1, u.copy()
+        errs.append(errors(space, p, u)[:2])
+        if on_step is not None:
+            on_step(step, t, u)
+    return np.array(errs), u
 
 
 def errors(space, params, u):

@@ -178,3 +178,37 @@ def test_multigrid_fgmres_solves_quadratic_example():
     err = O.errors(sf, p, x)
     assert err[0] < 1e-12 and err[1] < 1e-12
     assert it < 60
+
+
+@pytest.mark.parametrize("disc", ["galerkin", "supg"])
+def test_tcd2d_golden_errors(golden, disc):
+    """test/tcd2d_test.c++: Crank-Nicolson, exp example, Q1 on 32x32 quads;
+    L2/H1 errors after the listed time steps."""
+    g = golden["tcd2d_test"]
+    sp, info = O.build_space(0, 5, 5, 1)
+    errs, _ = O.tcd_time_integration(sp, 7, 1 if disc == "supg" else 0, O.CRANK_NICOLSON,
+                                     g["time_step_length"], g["n_steps"], eps=1.0, b=(1.0, -1.0), c=1.0)
+    for k, m in enumerate(g["steps"]):
+        assert abs(errs[m][0] - g[disc]["L2"][k]) < g["tolerance"], (m, errs[m][0], g[disc]["L2"][k])
+        assert abs(errs[m][1] - g[disc]["H1"][k]) < g["tolerance"], (m, errs[m][1], g[disc]["H1"][k])
+
+
+def test_tcd2d_golden_detects_planted_error(golden):
+    """Backward Euler in place of Crank-Nicolson (planted in the oracle run)
+    misses the golden errors."""
+    g = golden["tcd2d_test"]
+    sp, info = O.build_space(0, 5, 5, 1)
+    errs, _ = O.tcd_time_integration(sp, 7, 0, O.BACKWARD_EULER, g["time_step_length"], 3,
+                                     eps=1.0, b=(1.0, -1.0), c=1.0)
+    assert abs(errs[3][0] - g["galerkin"]["L2"][3]) > 1e-6
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2])
+@pytest.mark.parametrize("geo,order", [("UnitSquare", 1), ("TwoTriangles", 1), ("UnitSquare", 2)])
+def test_tcd_linear_space_time_is_exact(scheme, geo, order):
+    """TCD_2D/linear_space_time.h (u = 1 + 2x + 3ty): the Galerkin space
+    contains u(t) and every scheme integrates u_t = 3y exactly, so the
+    discrete solution equals the interpolant at every step."""
+    sp, info = O.build_space(GEO[geo], 3, 3, order)
+    errs, _ = O.tcd_time_integration(sp, 8, 0, scheme, 0.1, 5, eps=1.0, b=(1.0, 2.0), c=1.0)
+    assert np.max(errs) < 1e-10, errs
