@@ -152,6 +152,10 @@ int pmx_matrix_time_kernel(pmx_ctx* ctx, const pmx_matrix* A, int kernel, int re
 #define PMX_EX_P2_3D 4             /* include/Examples/CD_3D/test_p2.h */
 #define PMX_EX_EXP_SIN_3D 5        /* u = exp(-x) sin(pi y) sin(pi z) */
 #define PMX_EX_SINE_2D 6           /* u = sin(pi x) sin(pi y), Dirichlet u */
+#define PMX_EX_TCD_EXP 7           /* include/Examples/TCD_2D/exp.h (Example_TimeCD2D 0):
+                                      u = e^t sin(2 pi x) sin(2 pi y), Dirichlet 0 */
+#define PMX_EX_TCD_LINEAR 8        /* include/Examples/TCD_2D/linear_space_time.h (example -1):
+                                      u = 1 + 2x + 3ty */
 typedef struct {
   int disc;            /* space_discretization_type: galerkin / supg */
   int example;         /* PMX_EX_* : right-hand side from the exact solution */
@@ -165,6 +169,9 @@ typedef struct {
   const double* dev_coeff_table; /* optional [n_cells][n_quad][8] coefficients (eps,b1,b2,b3,c,f,-,-)
                                     replacing the CoeffFct callback; NULL: constants + example */
   const double* host_dirichlet;  /* optional Dirichlet values [n_dirichlet]; NULL: example values */
+  double time;         /* TDatabase::TimeDB->CURRENTTIME for time-dependent examples */
+  double theta1;       /* TimeDB->THETA1 (TCD SUPG forms) */
+  double tau;          /* TimeDB->CURRENTTIMESTEPLENGTH (TCD SUPG forms) */
 } pmx_cd_params;
 /* number of quadrature points per cell the assembly uses for this space */
 int pmx_assembly_n_quad(const pmx_space* s, int hexa_quad_rule, int* n_quad);
@@ -180,6 +187,22 @@ int pmx_assembly_quad_points(const pmx_space* s, int hexa_quad_rule, double* hos
 int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s, const pmx_cd_params* p, pmx_matrix* A,
                     double* dev_rhs);
 
+/* Assemble2D / Assemble3D with LocalAssembling<d> of type TCDStiffMassRhs /
+ * TCDStiffRhs (src/AssembleRoutines/LocalAssembling.C:842-928) as
+ * TimeConvectionDiffusion<d>::call_assembling_routine calls it
+ * (src/System/TimeConvectionDiffusion.C:465-517): ADDS the stiffness matrix
+ * (TCDStiff) into A, the mass matrix (TCDMass) into M and the load vector
+ * (TCDRhs, f at p->time) into dev_rhs; with SUPG (2D only) also TCDStiffSUPG,
+ * TCDMassSUPG, TCDRhsSUPG with THETA1 = p->theta1 and time step p->tau
+ * (CD_local_assembling_routines.C:130-330).  Any of A, M, dev_rhs may be NULL.
+ * Dirichlet rows of A and M get diagonal 1, Dirichlet rhs entries the boundary
+ * value at p->time.  The caller zeroes A, M, rhs first. */
+int pmx_assemble_tcd(pmx_ctx* ctx, const pmx_space* s, const pmx_cd_params* p, pmx_matrix* A,
+                     pmx_matrix* M, double* dev_rhs);
+/* TFEFunction2D/3D::Interpolate of the example's exact solution at p->time
+ * (Lagrange nodal points of every DOF) */
+int pmx_interpolate(pmx_ctx* ctx, const pmx_space* s, const pmx_cd_params* p, double* dev_u);
+
 /* TFEFunction::GetErrors with conv_diff_l2_h1_linf_error
  * (src/AssembleRoutines/ConvDiff.C:2134-2185): errors[0..3] = L2, H1-semi,
  * SD, L_inf (at quadrature points) against the example's exact solution. */
@@ -192,6 +215,46 @@ int pmx_vector_dot(pmx_ctx* ctx, int n, const double* dev_x, const double* dev_y
 int pmx_vector_norm(pmx_ctx* ctx, int n, const double* dev_x, double* host_out);
 /* BlockVector::add_scaled: y += a*x */
 int pmx_vector_add_scaled(pmx_ctx* ctx, int n, double a, const double* dev_x, double* dev_y);
+/* BlockVector::scale / scaleActive (n = active bound): x *= a (src/Matrix/BlockVector.C:138-181) */
+int pmx_vector_scale(pmx_ctx* ctx, int n, double a, double* dev_x);
+/* BlockVector::copy / copy_nonactive (with offsets): y = x (src/Matrix/BlockVector.C:244-268) */
+int pmx_vector_copy(pmx_ctx* ctx, int n, const double* dev_x, double* dev_y);
+
+/* ------------------------------------------- matrix operations on active rows */
+/* FEMatrix::scaleActive (src/Matrix/FEMatrix.C:139-150): entries of the active
+ * rows *= factor */
+int pmx_matrix_scale_active(pmx_ctx* ctx, pmx_matrix* A, double factor);
+/* BlockFEMatrix::add_blockfe_matrix / add_matrix_actives
+ * (src/Matrix/BlockFEMatrix.C:842-852, 1829-1862): active entries of A +=
+ * factor * active entries of B (same sparsity pattern) */
+int pmx_matrix_add_scaled_active(pmx_ctx* ctx, pmx_matrix* A, const pmx_matrix* B, double factor);
+/* FEMatrix::multiplyActive (src/Matrix/FEMatrix.C:186-199): y[i] += a (A x)[i]
+ * for the active rows, the other entries of y untouched */
+int pmx_matrix_multiply_active(pmx_ctx* ctx, const pmx_matrix* A, const double* dev_x, double* dev_y,
+                               double a);
+
+/* ------------------------------------------------------ time discretisation */
+#define PMX_TIME_BACKWARD_EULER 0
+#define PMX_TIME_CRANK_NICOLSON 1
+#define PMX_TIME_BDF2 2   /* step 1 is done with backward Euler (pre-stage) */
+/* TimeDiscretization::prepare_rhs_from_time_disc
+ * (src/TimeDiscRoutines/TimeDiscretizations.C:106-168): on entry dev_rhs is the
+ * load vector of the new time level; on return the active part holds the
+ * right-hand side of the time-discrete system (BE: tau f + M u_m1; CN:
+ * tau/2 (f + f_old) + M u_m1 - tau/2 A u_m1; BDF2: 2tau/3 f + 4/3 M u_m1 -
+ * 1/3 M u_m2).  A is the stiffness matrix before prepare_system. `step` counts
+ * from 1. */
+int pmx_time_prepare_rhs(pmx_ctx* ctx, int scheme, int step, double tau, const pmx_matrix* A,
+                         const pmx_matrix* M, double* dev_rhs, const double* dev_old_rhs,
+                         const double* dev_u_m1, const double* dev_u_m2);
+/* TimeDiscretization::prepare_system_matrix (TimeDiscretizations.C:172-210):
+ * active entries of A = factor * A + M (factor tau, tau/2 or 2tau/3) */
+int pmx_time_prepare_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_matrix* A,
+                            const pmx_matrix* M);
+/* TimeDiscretization::reset_linear_matrices (TimeDiscretizations.C:194-236):
+ * active entries of A = (A - M) / factor */
+int pmx_time_reset_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_matrix* A,
+                          const pmx_matrix* M);
 
 /* ------------------------------------------------------------------ solver */
 #define PMX_IT_FGMRES 0

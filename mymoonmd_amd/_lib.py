@@ -32,7 +32,8 @@ class CDParams(ctypes.Structure):
     _fields_ = [("disc", c_int), ("example", c_int), ("hexa_quad_rule", c_int),
                 ("eps", c_double), ("b", c_double * 3), ("c", c_double), ("f0", c_double),
                 ("delta0", c_double), ("poly_degree", c_double),
-                ("dev_coeff_table", c_void_p), ("host_dirichlet", c_void_p)]
+                ("dev_coeff_table", c_void_p), ("host_dirichlet", c_void_p),
+                ("time", c_double), ("theta1", c_double), ("tau", c_double)]
 
 
 class SolverParams(ctypes.Structure):
@@ -92,6 +93,17 @@ SIGNATURES = {
     "pmx_vector_dot": [c_void_p, c_int, c_void_p, c_void_p, P(c_double)],
     "pmx_vector_norm": [c_void_p, c_int, c_void_p, P(c_double)],
     "pmx_vector_add_scaled": [c_void_p, c_int, c_double, c_void_p, c_void_p],
+    "pmx_vector_scale": [c_void_p, c_int, c_double, c_void_p],
+    "pmx_vector_copy": [c_void_p, c_int, c_void_p, c_void_p],
+    "pmx_assemble_tcd": [c_void_p, c_void_p, P(CDParams), c_void_p, c_void_p, c_void_p],
+    "pmx_interpolate": [c_void_p, c_void_p, P(CDParams), c_void_p],
+    "pmx_matrix_scale_active": [c_void_p, c_void_p, c_double],
+    "pmx_matrix_add_scaled_active": [c_void_p, c_void_p, c_void_p, c_double],
+    "pmx_matrix_multiply_active": [c_void_p, c_void_p, c_void_p, c_void_p, c_double],
+    "pmx_time_prepare_rhs": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p],
+    "pmx_time_prepare_system": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p],
+    "pmx_time_reset_system": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p],
     "pmx_solver_default_params": [P(SolverParams)],
     "pmx_solver_create": [c_void_p, P(SolverParams), P(c_void_p)],
     "pmx_solver_destroy": [c_void_p],

@@ -107,6 +107,9 @@ pmx::CDCoeffs cd_coeffs(const pmx_cd_params* p) {
   co.delta0 = p->delta0;
   co.poly_degree = p->poly_degree;
   co.table = p->dev_coeff_table;
+  co.time = p->time;
+  co.theta1 = p->theta1;
+  co.tau = p->tau;
   return co;
 }
 
@@ -140,7 +143,7 @@ void neumann_2d(const pmx_space* s, const pmx_cd_params* p, std::vector<double>&
         const double t = t0 + 0.5 * (t1 - t0) * (zx[l] + 1);
         double x[2] = {a[0] + 0.5 * (zx[l] + 1) * (b[0] - a[0]), a[1] + 0.5 * (zx[l] + 1) * (b[1] - a[1])};
         // SineLaplace BoundValue on component 1: eps * du/dx (1, t)
-        double ex[5];
+        double ex[6];
         pmx::example_exact(p->example, 1.0, t, 0.0, ex);
         double g = (comp == 1) ? p->eps * ex[1] : 0.0;
         g *= hE * zw[l];
@@ -274,7 +277,9 @@ int pmx_space_create(pmx_ctx* ctx, const pmx_domain* d, int level, int order, pm
     pmx::color_cells(fs, order_cells, s->color_ptr);
     s->d_cells = upload(order_cells.data(), order_cells.size(), st);
     const int ndir = fs.n_dof - fs.active_bound;
-    s->d_dir_xyz = upload(fs.dof_xyz.data() + (size_t)fs.active_bound * 3, (size_t)ndir * 3, st);
+    s->d_dof_xyz = upload(fs.dof_xyz.data(), (size_t)fs.n_dof * 3, st);
+    s->d_dir_xyz = s->d_dof_xyz + (size_t)fs.active_bound * 3;
+    (void)ndir;
     *out = s.release();
   });
 }
@@ -569,7 +574,7 @@ int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p, p
     require(A->space == s_, "matrix was not created on this space");
     require(s_->ctx == ctx, "space has no device data for this context");
     require(p->disc == PMX_DISC_GALERKIN || p->disc == PMX_DISC_SUPG, "unknown discretisation");
-    require(p->example >= 0 && p->example <= 6, "unknown example");
+    require(p->example >= 0 && p->example <= PMX_EX_TCD_LINEAR, "unknown example");
     pmx_space* s = const_cast<pmx_space*>(s_);
     hipStream_t st = ctx->stream;
     A->sell_valid = false;
@@ -599,6 +604,190 @@ int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p, p
       HIP_CHECK(hipStreamSynchronize(st));
       dfree(d);
     }
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_assemble_tcd(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p, pmx_matrix* A,
+                     pmx_matrix* M, double* dev_rhs) {
+  return guarded([&] {
+    require(ctx && s_ && p, "NULL argument");
+    require(A || M || dev_rhs, "nothing to assemble");
+    require(!A || A->space == s_, "stiffness matrix was not created on this space");
+    require(!M || M->space == s_, "mass matrix was not created on this space");
+    require(s_->ctx == ctx, "space has no device data for this context");
+    require(p->disc == PMX_DISC_GALERKIN || p->disc == PMX_DISC_SUPG, "unknown discretisation");
+    require(p->disc == PMX_DISC_GALERKIN || s_->fs.el.dim == 2,
+            "SUPG for time-dependent problems is implemented in 2D only (as TCDStiffSUPG<2>)");
+    require(p->example >= 0 && p->example <= PMX_EX_TCD_LINEAR, "unknown example");
+    require(p->disc == PMX_DISC_GALERKIN || p->tau > 0, "SUPG needs the time step length tau");
+    pmx_space* s = const_cast<pmx_space*>(s_);
+    hipStream_t st = ctx->stream;
+    const RefTable& t = ref_table(s, p->hexa_quad_rule);
+    const pmx::AssemblyData ad = assembly_data(s, t);
+    const pmx::CDCoeffs co = cd_coeffs(p);
+    pmx::DevCSR none{};
+    const pmx::DevCSR& dA = A ? A->A : none;
+    const pmx::DevCSR& dM = M ? M->A : none;
+    if (A) A->sell_valid = false;
+    if (M) M->sell_valid = false;
+    for (size_t k = 0; k + 1 < s->color_ptr.size(); ++k) {
+      const int n = s->color_ptr[k + 1] - s->color_ptr[k];
+      pmx::assemble_tcd(ad, co, s->d_cells + s->color_ptr[k], n, s->fs.active_bound, dA, dM, dev_rhs, st);
+    }
+    HIP_CHECK(hipGetLastError());
+    const pmx::FESpace& fs = s->fs;
+    const int ndir = fs.n_dof - fs.active_bound;
+    if (dev_rhs && ndir > 0) {
+      if (p->host_dirichlet) {
+        HIP_CHECK(hipMemcpyAsync(dev_rhs + fs.active_bound, p->host_dirichlet, ndir * sizeof(double),
+                                 hipMemcpyHostToDevice, st));
+      } else {
+        pmx::dirichlet_rhs_example(p->example, ndir, s->d_dir_xyz, dev_rhs + fs.active_bound, co, st);
+      }
+    }
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_interpolate(pmx_ctx* ctx, const pmx_space* s, const pmx_cd_params* p, double* dev_u) {
+  return guarded([&] {
+    require(ctx && s && p && dev_u, "NULL argument");
+    require(s->ctx == ctx, "space has no device data for this context");
+    require(p->example >= 0 && p->example <= PMX_EX_TCD_LINEAR, "unknown example");
+    pmx::interpolate_example(p->example, p->time, s->fs.n_dof, s->d_dof_xyz, dev_u, ctx->stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+namespace {
+int active_entries(const pmx_matrix* A) { return A->pattern->row_ptr[A->pattern->active_bound]; }
+void require_same_pattern(const pmx_matrix* A, const pmx_matrix* B) {
+  require(A->pattern == B->pattern ||
+              (A->pattern->row_ptr == B->pattern->row_ptr && A->pattern->col == B->pattern->col &&
+               A->pattern->active_bound == B->pattern->active_bound),
+          "matrices do not share one sparsity pattern");
+}
+}  // namespace
+
+int pmx_matrix_scale_active(pmx_ctx* ctx, pmx_matrix* A, double factor) {
+  return guarded([&] {
+    require(ctx && A, "NULL argument");
+    if (factor == 1.0) return;
+    const int na = active_entries(A);
+    if (factor == 0.0) pmx::vec_fill(na, 0.0, A->A.val, ctx->stream);
+    else pmx::vec_scale(na, factor, nullptr, 0, A->A.val, A->A.val, ctx->stream);
+    A->sell_valid = false;
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_matrix_add_scaled_active(pmx_ctx* ctx, pmx_matrix* A, const pmx_matrix* B, double factor) {
+  return guarded([&] {
+    require(ctx && A && B, "NULL argument");
+    require_same_pattern(A, B);
+    pmx::vec_axpy(active_entries(A), factor, nullptr, 1.0, B->A.val, A->A.val, ctx->stream);
+    A->sell_valid = false;
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_matrix_multiply_active(pmx_ctx* ctx, const pmx_matrix* A, const double* x, double* y, double a) {
+  return guarded([&] {
+    require(ctx && A && x && y, "NULL argument");
+    require(x != y, "x and y must not alias");
+    ensure_sell_values(const_cast<pmx_matrix*>(A), ctx->stream);
+    pmx::fe_multiply_active(A->A, x, y, a, ctx->stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_vector_scale(pmx_ctx* ctx, int n, double a, double* x) {
+  return guarded([&] {
+    require(ctx && x && n >= 0, "invalid argument");
+    pmx::vec_scale(n, a, nullptr, 0, x, x, ctx->stream);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_vector_copy(pmx_ctx* ctx, int n, const double* x, double* y) {
+  return guarded([&] {
+    require(ctx && x && y && n >= 0, "invalid argument");
+    if (n > 0 && x != y)
+      HIP_CHECK(hipMemcpyAsync(y, x, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  });
+}
+
+// ------------------------------------------------------- time discretisation
+namespace {
+// factor of the stiffness matrix in the system matrix
+// (TimeDiscretization::prepare_system_matrix, TimeDiscretizations.C:172-210)
+double system_factor(int scheme, int step, double tau) {
+  if (scheme == PMX_TIME_CRANK_NICOLSON) return 0.5 * tau;
+  if (scheme == PMX_TIME_BDF2 && step >= 2) return tau * (2.0 / 3.0);
+  return tau;
+}
+void require_scheme(int scheme, double tau) {
+  require(scheme == PMX_TIME_BACKWARD_EULER || scheme == PMX_TIME_CRANK_NICOLSON || scheme == PMX_TIME_BDF2,
+          "time stepping scheme is not supported");
+  require(tau > 0, "time step length must be positive");
+}
+}  // namespace
+
+int pmx_time_prepare_rhs(pmx_ctx* ctx, int scheme, int step, double tau, const pmx_matrix* A,
+                         const pmx_matrix* M, double* rhs, const double* old_rhs, const double* u_m1,
+                         const double* u_m2) {
+  return guarded([&] {
+    require(ctx && A && M && rhs && u_m1, "NULL argument");
+    require_scheme(scheme, tau);
+    require_same_pattern(A, M);
+    hipStream_t st = ctx->stream;
+    const int ab = A->pattern->active_bound;
+    ensure_sell_values(const_cast<pmx_matrix*>(M), st);
+    if (scheme == PMX_TIME_BDF2 && step >= 2) {
+      require(u_m2, "BDF2 needs the solution two steps back");
+      pmx::vec_scale(ab, (2.0 / 3.0) * tau, nullptr, 0, rhs, rhs, st);
+      pmx::fe_multiply_active(M->A, u_m1, rhs, 4.0 / 3.0, st);
+      pmx::fe_multiply_active(M->A, u_m2, rhs, -1.0 / 3.0, st);
+    } else if (scheme == PMX_TIME_CRANK_NICOLSON) {
+      require(old_rhs, "Crank-Nicolson needs the right-hand side of the previous step");
+      ensure_sell_values(const_cast<pmx_matrix*>(A), st);
+      pmx::vec_scale(ab, 0.5 * tau, nullptr, 0, rhs, rhs, st);
+      pmx::vec_axpy(ab, 0.5 * tau, nullptr, 1.0, old_rhs, rhs, st);
+      pmx::fe_multiply_active(M->A, u_m1, rhs, 1.0, st);
+      pmx::fe_multiply_active(A->A, u_m1, rhs, -0.5 * tau, st);
+    } else {
+      pmx::vec_scale(ab, tau, nullptr, 0, rhs, rhs, st);
+      pmx::fe_multiply_active(M->A, u_m1, rhs, 1.0, st);
+    }
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_time_prepare_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_matrix* A,
+                            const pmx_matrix* M) {
+  return guarded([&] {
+    require(ctx && A && M, "NULL argument");
+    require_scheme(scheme, tau);
+    require_same_pattern(A, M);
+    const int na = active_entries(A);
+    pmx::vec_scale(na, system_factor(scheme, step, tau), nullptr, 0, A->A.val, A->A.val, ctx->stream);
+    pmx::vec_axpy(na, 1.0, nullptr, 1.0, M->A.val, A->A.val, ctx->stream);
+    A->sell_valid = false;
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_time_reset_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_matrix* A,
+                          const pmx_matrix* M) {
+  return guarded([&] {
+    require(ctx && A && M, "NULL argument");
+    require_scheme(scheme, tau);
+    require_same_pattern(A, M);
+    const int na = active_entries(A);
+    pmx::vec_axpy(na, -1.0, nullptr, 1.0, M->A.val, A->A.val, ctx->stream);
+    pmx::vec_scale(na, 1.0 / system_factor(scheme, step, tau), nullptr, 0, A->A.val, A->A.val, ctx->stream);
+    A->sell_valid = false;
     HIP_CHECK(hipGetLastError());
   });
 }

@@ -7,6 +7,12 @@
 //                       test_p2.h (solutions 1, 1+2x+3y+4z, x^2+yz)
 //   EX_EXP_SIN_3D       u = exp(-x) sin(pi y) sin(pi z) (CD3D benchmark problem)
 //   EX_SINE_2D          u = sin(pi x) sin(pi y) with Dirichlet value u everywhere
+//   EX_TCD_EXP          include/Examples/TCD_2D/exp.h: u = e^t sin(2 pi x) sin(2 pi y),
+//                       Dirichlet value 0 (the reference's example 0 of Example_TimeCD2D)
+//   EX_TCD_LINEAR       include/Examples/TCD_2D/linear_space_time.h: u = 1 + 2x + 3ty
+//                       (example -1)
+// Time-dependent examples take the right-hand side f = du/dt - eps*lap(u) +
+// b.grad(u) + c*u at the given time.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -22,14 +28,29 @@ enum Example : int {
   EX_P2_3D = 4,
   EX_EXP_SIN_3D = 5,
   EX_SINE_2D = 6,
+  EX_TCD_EXP = 7,
+  EX_TCD_LINEAR = 8,
 };
 
 constexpr double kPi = 3.14159265358979323846;
 
-// v[0] = u, v[1..3] = grad u, v[4] = laplace u
-__host__ __device__ inline void example_exact(int ex, double x, double y, double z, double* v) {
-  v[0] = v[1] = v[2] = v[3] = v[4] = 0.0;
+// v[0] = u, v[1..3] = grad u, v[4] = laplace u, v[5] = du/dt at time t
+__host__ __device__ inline void example_exact(int ex, double x, double y, double z, double* v,
+                                              double t = 0.0) {
+  v[0] = v[1] = v[2] = v[3] = v[4] = v[5] = 0.0;
   switch (ex) {
+    case EX_TCD_EXP: {
+      const double e = exp(t), sx = sin(2 * kPi * x), sy = sin(2 * kPi * y);
+      v[0] = e * (sx * sy);
+      v[1] = e * 2 * kPi * cos(2 * kPi * x) * sy;
+      v[2] = e * 2 * kPi * sx * cos(2 * kPi * y);
+      v[4] = -8 * kPi * kPi * v[0];
+      v[5] = v[0];
+      break;
+    }
+    case EX_TCD_LINEAR:
+      v[0] = 1 + 2 * x + 3 * t * y; v[1] = 2; v[2] = 3 * t; v[5] = 3 * y;
+      break;
     case EX_SINE_LAPLACE_2D:
     case EX_SINE_2D: {
       const double sx = sin(kPi * x), sy = sin(kPi * y);
@@ -56,21 +77,22 @@ __host__ __device__ inline void example_exact(int ex, double x, double y, double
 }
 
 __host__ __device__ inline double example_rhs(int ex, double x, double y, double z, double eps,
-                                              const double* b, double c, double f0) {
+                                              const double* b, double c, double f0, double t = 0.0) {
   if (ex == EX_CONST) return f0;
-  double v[5];
-  example_exact(ex, x, y, z, v);
-  double f = -eps * v[4];
+  double v[6];
+  example_exact(ex, x, y, z, v, t);
+  double f = v[5] - eps * v[4];
   f += b[0] * v[1] + b[1] * v[2] + b[2] * v[3];
   f += c * v[0];
   return f;
 }
 
 // Dirichlet boundary value at a boundary point
-__host__ __device__ inline double example_dirichlet(int ex, double x, double y, double z) {
-  if (ex == EX_SINE_LAPLACE_2D || ex == EX_CONST) return 0.0;
-  double v[5];
-  example_exact(ex, x, y, z, v);
+__host__ __device__ inline double example_dirichlet(int ex, double x, double y, double z,
+                                                    double t = 0.0) {
+  if (ex == EX_SINE_LAPLACE_2D || ex == EX_CONST || ex == EX_TCD_EXP) return 0.0;
+  double v[6];
+  example_exact(ex, x, y, z, v, t);
   return v[0];
 }
 

@@ -54,7 +54,7 @@ __device__ void coeffs_at(const CDCoeffs& co, const double* geo, const double* x
     const double* t = co.table + ((size_t)cell * nq + q) * 8;
     eps = t[0]; b[0] = t[1]; b[1] = t[2]; b[2] = (DIM == 3) ? t[3] : 0.0; c = t[4]; f = t[5];
   } else {
-    f = example_rhs(co.example, x[0], x[1], x[2], eps, b, c, co.f0);
+    f = example_rhs(co.example, x[0], x[1], x[2], eps, b, c, co.f0, co.time);
   }
   const double detJ = geo[21], hK = geo[22];
   out[0] = eps; out[1] = b[0]; out[2] = b[1]; out[3] = b[2]; out[4] = c; out[5] = f;
@@ -179,10 +179,173 @@ void launch_asm(const AssemblyData& ad, const CDCoeffs& co, const int* cells, in
                        A, rhs);
 }
 
-__global__ void k_dirichlet_example(int ex, int n, const double* __restrict__ xyz,
+__global__ void k_dirichlet_example(int ex, double t, int n, const double* __restrict__ xyz,
                                     double* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) out[k] = example_dirichlet(ex, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]);
+  if (k < n) out[k] = example_dirichlet(ex, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], t);
+}
+
+__global__ void k_interpolate_example(int ex, double t, int n, const double* __restrict__ xyz,
+                                      double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    double v[6];
+    example_exact(ex, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], v, t);
+    out[k] = v[0];
+  }
+}
+
+// Time-dependent CD forms (src/AssembleRoutines/CD_local_assembling_routines.C):
+//   TCDStiff   A_ij += w|J| (eps grad phi_j . grad phi_i + (b.grad phi_j) phi_i + c phi_j phi_i)
+//   TCDMass    M_ij += w|J| phi_i phi_j
+//   TCDRhs     f_i  += w|J| f phi_i
+// and in 2D with SUPG, with e = eps t1 tau, B = b t1 tau, C = 1 + c t1 tau,
+// delta = Compute_SDFEM_delta(hK, e, B):
+//   TCDStiffSUPG  A_ij += w|J| delta (B.grad phi_i) (B.grad phi_j + C phi_j - e lap phi_j)
+//   TCDMassSUPG   M_ij += w|J| delta t1 tau (B.grad phi_i) phi_j
+//   TCDRhsSUPG    f_i  += w|J| delta (B.grad phi_i) f
+// Same lane layout as k_assemble_cd; each lane keeps its stiffness row, mass
+// row and load entry.
+template <int ND, int DIM, int SLOT>
+__global__ __launch_bounds__(256) void k_assemble_tcd(AssemblyData ad, CDCoeffs co,
+                                                      const int* __restrict__ cells, int n,
+                                                      int active, DevCSR A, DevCSR M,
+                                                      double* __restrict__ rhs) {
+  constexpr int NC = Dims<DIM>::NC;
+  constexpr int CPB = 256 / SLOT;
+  constexpr int RS = 1 + DIM + ND * (1 + DIM + NC);
+  __shared__ double s_u[CPB][ND];
+  __shared__ double s_g[CPB][DIM][ND];
+  __shared__ double s_l[CPB][ND];
+  __shared__ double s_c[CPB][12];
+  const bool supg = co.supg && DIM == 2;
+  const int slot = threadIdx.x / SLOT;
+  const int i = threadIdx.x % SLOT;
+  const int ci = blockIdx.x * CPB + slot;
+  const bool valid = ci < n;
+  const int cell = valid ? cells[ci] : 0;
+  const double* geo = ad.geo + (size_t)cell * kGeoStride;
+  double Jinv[DIM * DIM];
+#pragma unroll
+  for (int k = 0; k < DIM * DIM; ++k) Jinv[k] = geo[12 + k];
+  double G[DIM * DIM];
+#pragma unroll
+  for (int k = 0; k < DIM; ++k)
+#pragma unroll
+    for (int l = 0; l < DIM; ++l) {
+      double s = 0;
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) s += Jinv[k * DIM + a] * Jinv[l * DIM + a];
+      G[k * DIM + l] = s;
+    }
+  double arow[ND], mrow[ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j) { arow[j] = 0.0; mrow[j] = 0.0; }
+  double frow = 0.0;
+
+  for (int q = 0; q < ad.nq; ++q) {
+    const double* rq = ad.ref + (size_t)q * RS;
+    if (valid && i < ND) {
+      const double* dphi = rq + 1 + DIM + ND;
+      s_u[slot][i] = rq[1 + DIM + i];
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < DIM; ++k) s += dphi[k * ND + i] * Jinv[k * DIM + a];
+        s_g[slot][a][i] = s;
+      }
+      if (supg) {
+        const double* d2 = rq + 1 + DIM + ND * (1 + DIM);
+        s_l[slot][i] = d2[0 * ND + i] * G[0] + d2[2 * ND + i] * G[DIM * DIM - 1] +
+                       2.0 * d2[1 * ND + i] * G[1];
+      }
+    }
+    if (valid && i == 0) {
+      double cf[8];
+      coeffs_at<DIM>(co, geo, rq + 1, rq[0], cell, q, ad.nq, cf);
+      double* sc = s_c[slot];
+      for (int k = 0; k < 6; ++k) sc[k] = cf[k];
+      sc[6] = cf[7];   // w |det J|
+      if (supg) {
+        const double t1 = co.theta1, tau = co.tau;
+        const double se = cf[0] * t1 * tau;
+        const double sb[3] = {cf[1] * t1 * tau, cf[2] * t1 * tau, 0.0};
+        const double scc = 1. + cf[4] * t1 * tau;
+        const double hK = geo[22];
+        const double delta = sdfem_delta(hK, se, sb, 2, scc, co.delta0, co.poly_degree);
+        sc[7] = se; sc[8] = sb[0]; sc[9] = sb[1]; sc[10] = scc; sc[11] = delta;
+      }
+    }
+    __syncthreads();
+    if (valid && i < ND) {
+      const double* sc = s_c[slot];
+      const double eps = sc[0], b1 = sc[1], b2 = sc[2];
+      const double b3 = (DIM == 3) ? sc[3] : 0.0;
+      const double c = sc[4], f = sc[5], mult = sc[6];
+      const double ti = s_u[slot][i];
+      const double tx = s_g[slot][0][i], ty = s_g[slot][1][i];
+      const double tz = (DIM == 3) ? s_g[slot][DIM - 1][i] : 0.0;
+      frow += mult * ti * f;
+#pragma unroll
+      for (int j = 0; j < ND; ++j) {
+        const double ax = s_g[slot][0][j], ay = s_g[slot][1][j];
+        const double az = (DIM == 3) ? s_g[slot][DIM - 1][j] : 0.0;
+        const double aj = s_u[slot][j];
+        double val = eps * (tx * ax + ty * ay + tz * az);
+        val += (b1 * ax + b2 * ay + b3 * az) * ti;
+        val += c * ti * aj;
+        arow[j] += mult * val;
+        mrow[j] += mult * ti * aj;
+      }
+      if (supg) {
+        const double se = sc[7], sb1 = sc[8], sb2 = sc[9], scc = sc[10], delta = sc[11];
+        const double bgradv = delta * (sb1 * tx + sb2 * ty);
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+          const double ax = s_g[slot][0][j], ay = s_g[slot][1][j];
+          double val = (sb1 * ax + sb2 * ay) * bgradv;
+          val += scc * bgradv * s_u[slot][j];
+          val -= se * s_l[slot][j] * bgradv;
+          arow[j] += mult * val;
+        }
+        const double bgm = (delta * co.theta1 * co.tau) * (sb1 * tx + sb2 * ty);
+#pragma unroll
+        for (int j = 0; j < ND; ++j) mrow[j] += mult * s_u[slot][j] * bgm;
+        frow += mult * f * bgradv;
+      }
+    }
+    __syncthreads();
+  }
+  if (!valid || i >= ND) return;
+  const int* dofs = ad.dofs + (size_t)cell * ND;
+  const int r = dofs[i];
+  if (r < active) {
+    if (A.val) {
+      const int rs = A.row_ptr[r], re = A.row_ptr[r + 1];
+#pragma unroll
+      for (int j = 0; j < ND; ++j) A.val[find_col(A.col, rs, re, dofs[j])] += arow[j];
+    }
+    if (M.val) {
+      const int rs = M.row_ptr[r], re = M.row_ptr[r + 1];
+#pragma unroll
+      for (int j = 0; j < ND; ++j) M.val[find_col(M.col, rs, re, dofs[j])] += mrow[j];
+    }
+    if (rhs) rhs[r] += frow;
+  } else {
+    if (A.val) A.val[find_col(A.col, A.row_ptr[r], A.row_ptr[r + 1], r)] = 1.0;
+    if (M.val) M.val[find_col(M.col, M.row_ptr[r], M.row_ptr[r + 1], r)] = 1.0;
+  }
+}
+
+template <int ND, int DIM, int SLOT>
+void launch_tcd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n, int active,
+                const DevCSR& A, const DevCSR& M, double* rhs, hipStream_t st) {
+  constexpr int CPB = 256 / SLOT;
+  const int grid = (n + CPB - 1) / CPB;
+  if (grid > 0)
+    hipLaunchKernelGGL((k_assemble_tcd<ND, DIM, SLOT>), dim3(grid), dim3(256), 0, st, ad, co, cells, n,
+                       active, A, M, rhs);
 }
 
 // one thread per cell; partial sums per block
@@ -219,8 +382,8 @@ __global__ __launch_bounds__(256) void k_cd_errors(AssemblyData ad, CDCoeffs co,
         for (int c = 0; c < DIM; ++c) s += geo[3 + r * DIM + c] * rq[1 + c];
         x[r] = s;
       }
-      double ex[5];
-      example_exact(co.example, x[0], x[1], x[2], ex);
+      double ex[6];
+      example_exact(co.example, x[0], x[1], x[2], ex, co.time);
       const double w = cf[7];
       const double e0 = val - ex[0], e1 = g[0] - ex[1], e2 = g[1] - ex[2];
       const double e3 = (DIM == 3) ? g[2] - ex[3] : 0.0;
@@ -282,10 +445,25 @@ void assemble_cd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, i
 }
 
 void dirichlet_rhs_example(int example, int n_dir, const double* dir_xyz, double* rhs_dir,
-                           const CDCoeffs&, hipStream_t st) {
+                           const CDCoeffs& co, hipStream_t st) {
   if (n_dir > 0)
-    hipLaunchKernelGGL(k_dirichlet_example, dim3((n_dir + 255) / 256), dim3(256), 0, st, example, n_dir,
-                       dir_xyz, rhs_dir);
+    hipLaunchKernelGGL(k_dirichlet_example, dim3((n_dir + 255) / 256), dim3(256), 0, st, example,
+                       co.time, n_dir, dir_xyz, rhs_dir);
+}
+
+void interpolate_example(int example, double time, int n, const double* xyz, double* u, hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_interpolate_example, dim3((n + 255) / 256), dim3(256), 0, st, example, time, n,
+                       xyz, u);
+}
+
+void assemble_tcd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n, int active,
+                  const DevCSR& A, const DevCSR& M, double* rhs, hipStream_t st) {
+  if (ad.dim == 3 && ad.ndof == 27) launch_tcd<27, 3, 32>(ad, co, cells, n, active, A, M, rhs, st);
+  else if (ad.dim == 3 && ad.ndof == 8) launch_tcd<8, 3, 8>(ad, co, cells, n, active, A, M, rhs, st);
+  else if (ad.dim == 2 && ad.ndof == 4) launch_tcd<4, 2, 4>(ad, co, cells, n, active, A, M, rhs, st);
+  else if (ad.dim == 2 && ad.ndof == 9) launch_tcd<9, 2, 16>(ad, co, cells, n, active, A, M, rhs, st);
+  else if (ad.dim == 2 && ad.ndof == 3) launch_tcd<3, 2, 4>(ad, co, cells, n, active, A, M, rhs, st);
 }
 
 void cd_errors(const AssemblyData& ad, const CDCoeffs& co, const double* u, double* partial,

@@ -395,6 +395,12 @@ void fe_apply_scaled_add(const DevCSR& A, const double* x, double* y, double a, 
   dispatch_fe_spmv<MODE_SCALED_ADD, double, double>(A, A.val, nullptr, x, nullptr, nullptr, y, a,
                                                     nullptr, st);
 }
+void fe_multiply_active(const DevCSR& A, const double* x, double* y, double a, hipStream_t st) {
+  DevCSR Aa = A;
+  Aa.n = A.active;   // no identity rows
+  dispatch_fe_spmv<MODE_SCALED_ADD, double, double>(Aa, A.val, nullptr, x, nullptr, nullptr, y, a,
+                                                    nullptr, st);
+}
 void fe_apply(const DevCSR& A, const double* x, double* y, hipStream_t st) {
   dispatch_fe_spmv<MODE_APPLY, double, double>(A, A.val, nullptr, x, nullptr, nullptr, y, 1.0,
                                                nullptr, st);

@@ -108,11 +108,20 @@ constexpr int kGeoStride = 24;
 
 struct CDCoeffs {
   int example = 0;        // built-in right-hand side / exact solution
-  int supg = 0;           // add BilinearAssemble_SD
+  int supg = 0;           // add BilinearAssemble_SD (CD) / the TCD SUPG forms
   double eps = 1, b[3] = {0, 0, 0}, c = 0, f0 = 0;
   double delta0 = 1, poly_degree = 1;
+  double time = 0;        // TimeDB->CURRENTTIME of time-dependent examples
+  double theta1 = 1, tau = 0;      // TimeDB->THETA1, CURRENTTIMESTEPLENGTH (TCD SUPG)
   const double* table = nullptr;   // optional per-(cell,q) coefficients [8]
 };
+
+// TCD forms of the cells cells[0..n) (one colour): stiffness into A, mass into
+// M, load vector into rhs; any of A.val, M.val, rhs may be null.
+void assemble_tcd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n, int active,
+                  const DevCSR& A, const DevCSR& M, double* rhs, hipStream_t st);
+// u[i] = exact solution at the nodal point xyz[i] (Lagrange interpolation)
+void interpolate_example(int example, double time, int n, const double* xyz, double* u, hipStream_t st);
 
 // Adds the cell contributions of the cells cells[0..n) (one colour) into A and rhs.
 void assemble_cd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n,
@@ -120,6 +129,8 @@ void assemble_cd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, i
 // rhs[i] = g(x_i) for Dirichlet rows from the built-in example (i >= active)
 void dirichlet_rhs_example(int example, int n_dir, const double* dir_xyz, double* rhs_dir,
                            const CDCoeffs& co, hipStream_t st);
+// y[i] += a (A x)[i] on the active rows only (FEMatrix::multiplyActive)
+void fe_multiply_active(const DevCSR& A, const double* x, double* y, double a, hipStream_t st);
 // L2, H1-semi, SD errors (squared partials) and Linf over quadrature points
 void cd_errors(const AssemblyData& ad, const CDCoeffs& co, const double* u, double* partial4,
                double* out4, hipStream_t st);

@@ -106,10 +106,11 @@ struct pmx_space {
   double* d_geo = nullptr;
   int* d_cells = nullptr;          // cells ordered by colour
   std::vector<int> color_ptr;
-  double* d_dir_xyz = nullptr;     // nodal points of the Dirichlet DOFs
+  double* d_dof_xyz = nullptr;     // nodal points of all DOFs [n_dof][3]
+  double* d_dir_xyz = nullptr;     // the Dirichlet part of d_dof_xyz (not owned)
   std::map<int, RefTable> ref;     // keyed by hexa quad rule
   ~pmx_space() {
-    pmx_internal::dfree(d_dofs); pmx_internal::dfree(d_geo); pmx_internal::dfree(d_cells); pmx_internal::dfree(d_dir_xyz);
+    pmx_internal::dfree(d_dofs); pmx_internal::dfree(d_geo); pmx_internal::dfree(d_cells); pmx_internal::dfree(d_dof_xyz);
     for (auto& kv : ref) pmx_internal::dfree(kv.second.d_ref);
   }
 };

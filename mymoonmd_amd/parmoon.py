@@ -18,7 +18,8 @@ from ._lib import (BC2D_FN, BC3D_FN, CDParams, SolveReport, SolverParams, SpaceI
 
 GEOMETRY = {"UnitSquare": 0, "TwoTriangles": 1, "Default_UnitCube_Hexa": 2}
 EXAMPLES = {"const": 0, "sine_laplace_2d": 1, "p0_3d": 2, "p1_3d": 3, "p2_3d": 4,
-            "exp_sin_3d": 5, "sine_2d": 6}
+            "exp_sin_3d": 5, "sine_2d": 6, "tcd_exp": 7, "tcd_linear": 8}
+TIME_SCHEME = {"backward_euler": 0, "crank_nicolson": 1, "bdf_two": 2}
 IT_SOLVER = {"fgmres": 0, "cg": 1, "richardson": 2, "jacobi": 3}
 PRECOND = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2}
 SMOOTHER = {"jacobi": 0, "direct_solve": 1}
@@ -200,6 +201,21 @@ class FEMatrix:
         call("pmx_matrix_apply_scaled_add", self.ctx.h, self.h, _ptr(x), _ptr(y), a)
         self.ctx.synchronize()
 
+    def multiply_active(self, x, y, a=1.0):
+        """FEMatrix::multiplyActive: y[i] += a (A x)[i] on the active rows."""
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_matrix_multiply_active", self.ctx.h, self.h, _ptr(x), _ptr(y), a)
+        self.ctx.synchronize()
+
+    def scale_active(self, factor):
+        """FEMatrix::scaleActive"""
+        call("pmx_matrix_scale_active", self.ctx.h, self.h, factor)
+
+    def add_scaled_active(self, other, factor=1.0):
+        """BlockFEMatrix::add_blockfe_matrix (active entries += factor * other)"""
+        call("pmx_matrix_add_scaled_active", self.ctx.h, self.h, other.h, factor)
+
     def __del__(self, _d=_destroy):
         if getattr(self, "h", None) and self.h.value:
             _d("pmx_matrix_destroy", self.h)
@@ -207,7 +223,7 @@ class FEMatrix:
 
 
 def cd_params(example="const", disc="galerkin", eps=1.0, b=(0.0, 0.0, 0.0), c=0.0, f0=0.0,
-              hexa_quad_rule=0, delta0=1.0, poly_degree=1.0):
+              hexa_quad_rule=0, delta0=1.0, poly_degree=1.0, time=0.0, theta1=1.0, tau=0.0):
     p = CDParams()
     p.disc = 1 if disc == "supg" else 0
     p.example = EXAMPLES[example]
@@ -219,6 +235,7 @@ def cd_params(example="const", disc="galerkin", eps=1.0, b=(0.0, 0.0, 0.0), c=0.
     p.f0 = f0
     p.delta0 = delta0
     p.poly_degree = poly_degree
+    p.time, p.theta1, p.tau = time, theta1, tau
     return p
 
 
@@ -228,6 +245,34 @@ def assemble_cd(ctx, space, params, matrix, rhs):
     torch.cuda.synchronize()
     call("pmx_assemble_cd", ctx.h, space.h, ctypes.byref(params), matrix.h, _ptr(rhs))
     ctx.synchronize()
+
+
+def assemble_tcd(ctx, space, params, stiffness=None, mass=None, rhs=None):
+    """Assemble2D/3D + LocalAssembling(TCDStiffMassRhs / TCDStiffRhs): adds into
+    the given stiffness matrix, mass matrix and rhs (each optional)."""
+    import torch
+    torch.cuda.synchronize()
+    call("pmx_assemble_tcd", ctx.h, space.h, ctypes.byref(params), stiffness.h if stiffness else None,
+         mass.h if mass else None, _ptr(rhs) if rhs is not None else None)
+    ctx.synchronize()
+
+
+def interpolate(ctx, space, params, u):
+    """TFEFunction::Interpolate of the example's exact solution at params.time."""
+    import torch
+    torch.cuda.synchronize()
+    call("pmx_interpolate", ctx.h, space.h, ctypes.byref(params), _ptr(u))
+    ctx.synchronize()
+
+
+def vector_copy(ctx, x, y):
+    """y = x for two equally long (sub)vectors (BlockVector::copy / copy_nonactive)."""
+    assert x.numel() == y.numel()
+    call("pmx_vector_copy", ctx.h, x.numel(), _ptr(x), _ptr(y))
+
+
+def vector_scale(ctx, x, a):
+    call("pmx_vector_scale", ctx.h, x.numel(), a, _ptr(x))
 
 
 def compute_errors(ctx, space, params, u):
@@ -345,7 +390,8 @@ class ConvectionDiffusion:
             assemble_cd(self.ctx, s, self.params, A, f)
         # solution.copy_nonactive(rhs)
         ab = self.spaces[-1].active_bound
-        self.solution[ab:] = self.rhs[-1][ab:]
+        vector_copy(self.ctx, self.rhs[-1][ab:], self.solution[ab:])
+        self.ctx.synchronize()
 
     def solve(self):
         if self.solver.params.preconditioner == PRECOND["multigrid"]:
@@ -370,3 +416,125 @@ class ConvectionDiffusion:
 
     def get_L_inf_error(self):
         return self.errors[3]
+
+
+class TimeDiscretization:
+    """TimeDiscretization (src/TimeDiscRoutines/TimeDiscretizations.C): the
+    scheme, the time and the step counter, with the rhs / system-matrix steps
+    delegated to the C ABI."""
+
+    def __init__(self, ctx, time_discretization="backward_euler", time_step_length=0.05,
+                 time_start=0.0, time_end=1.0):
+        self.ctx = ctx
+        self.scheme = TIME_SCHEME[time_discretization]
+        self.tau = time_step_length
+        self.start_time, self.end_time = time_start, time_end
+        self.current_time_ = time_start
+        self.current_step_ = 0
+
+    def n_old_solutions(self):
+        return 2 if self.scheme == TIME_SCHEME["bdf_two"] else 1
+
+    def theta1(self):
+        """TDatabase::TimeDB->THETA1 as set_time_disc_parameters sets it (:341-381)."""
+        if self.scheme == TIME_SCHEME["crank_nicolson"]:
+            return 0.5
+        if self.scheme == TIME_SCHEME["bdf_two"] and self.current_step_ >= 2:
+            return 2.0 / 3.0
+        return 1.0
+
+    def reached_final_time_step(self):
+        return self.current_time_ >= self.end_time - 1e-10
+
+    def prepare_rhs_from_time_disc(self, stiffness, mass, rhs, old_rhs, u_m1, u_m2=None):
+        call("pmx_time_prepare_rhs", self.ctx.h, self.scheme, self.current_step_, self.tau,
+             stiffness.h, mass.h, _ptr(rhs), _ptr(old_rhs) if old_rhs is not None else None,
+             _ptr(u_m1), _ptr(u_m2) if u_m2 is not None else None)
+
+    def prepare_system_matrix(self, stiffness, mass):
+        call("pmx_time_prepare_system", self.ctx.h, self.scheme, self.current_step_, self.tau,
+             stiffness.h, mass.h)
+
+    def reset_linear_matrices(self, stiffness, mass):
+        call("pmx_time_reset_system", self.ctx.h, self.scheme, self.current_step_, self.tau,
+             stiffness.h, mass.h)
+
+
+class TimeConvectionDiffusion:
+    """TimeConvectionDiffusion<d> (src/System/TimeConvectionDiffusion.C):
+    systems per multigrid level with stiffness, mass, rhs;
+    assemble_initial_time / assemble / solve / output as in the reference."""
+
+    def __init__(self, ctx, domain, order, example, time_disc, solver, disc="galerkin", eps=1.0,
+                 b=(0.0, 0.0, 0.0), c=0.0, n_mg_levels=1, bc="all_dirichlet"):
+        import torch
+        self.ctx = ctx
+        self.example, self.disc, self.eps, self.b, self.c = example, disc, eps, b, c
+        self.tss = time_disc
+        nref = domain.n_refine
+        levels = list(range(nref - n_mg_levels + 1, nref + 1))
+        self.spaces = [TFESpace(domain, l, order, ctx, bc=bc) for l in levels]
+        self.stiffness = [FEMatrix(ctx, s) for s in self.spaces]
+        self.mass = [FEMatrix(ctx, s) for s in self.spaces]
+        dev = torch.device("cuda", ctx.device)
+        z = lambda s: torch.zeros(s.n_dof, dtype=torch.float64, device=dev)  # noqa: E731
+        self.rhs = [z(s) for s in self.spaces]
+        fs = self.spaces[-1]
+        self.solution, self.solution_m1, self.solution_m2 = z(fs), z(fs), z(fs)
+        self.old_rhs, self.rhs_from_time_disc = z(fs), z(fs)
+        self.solver = Solver(ctx, solver)
+        self.multigrid = solver.preconditioner == PRECOND["multigrid"]
+        self.report = None
+        # initial condition: Interpolate(example.get_initial_cond(0)) at time_start
+        interpolate(ctx, fs, self.params(), self.solution)
+
+    def params(self):
+        return cd_params(self.example, disc=self.disc, eps=self.eps, b=self.b, c=self.c,
+                         time=self.tss.current_time_, theta1=self.tss.theta1(), tau=self.tss.tau)
+
+    def _assemble(self, with_mass):
+        p = self.params()
+        for s, A, M, f in zip(self.spaces, self.stiffness, self.mass, self.rhs):
+            A.reset()
+            f.zero_()
+            if with_mass:
+                M.reset()
+            assemble_tcd(self.ctx, s, p, A, M if with_mass else None, f)
+
+    def assemble_initial_time(self):
+        self._assemble(True)
+        vector_copy(self.ctx, self.rhs[-1], self.old_rhs)
+        vector_copy(self.ctx, self.solution, self.solution_m1)
+        vector_copy(self.ctx, self.solution_m1, self.solution_m2)
+        self.ctx.synchronize()
+
+    def assemble(self):
+        self._assemble(self.disc == "supg")
+        ab = self.spaces[-1].active_bound
+        f, rtd = self.rhs[-1], self.rhs_from_time_disc
+        vector_copy(self.ctx, f, rtd)
+        self.tss.prepare_rhs_from_time_disc(self.stiffness[-1], self.mass[-1], rtd, self.old_rhs,
+                                            self.solution_m1, self.solution_m2)
+        vector_copy(self.ctx, f, self.old_rhs)
+        vector_copy(self.ctx, f[ab:], rtd[ab:])
+        for A, M in zip(self.stiffness, self.mass):
+            self.tss.prepare_system_matrix(A, M)
+        vector_copy(self.ctx, f[ab:], self.solution[ab:])
+        self.ctx.synchronize()
+
+    def solve(self):
+        if self.multigrid:
+            self.solver.update_matrix(self.stiffness, self.spaces)
+        else:
+            self.solver.update_matrix(self.stiffness[-1])
+        self.report = self.solver.solve(self.rhs_from_time_disc, self.solution)
+        for A, M in zip(self.stiffness, self.mass):
+            self.tss.reset_linear_matrices(A, M)
+        vector_copy(self.ctx, self.solution_m1, self.solution_m2)
+        vector_copy(self.ctx, self.solution, self.solution_m1)
+        self.ctx.synchronize()
+        return self.report
+
+    def output(self):
+        """L2, H1-semi, SD, L_inf errors at the current time."""
+        return compute_errors(self.ctx, self.spaces[-1], self.params(), self.solution)
