@@ -56,6 +56,10 @@ int pmx_ctx_synchronize(pmx_ctx* ctx);
 #define PMX_GEO_UNIT_CUBE_HEXA 2     /* "Default_UnitCube_Hexa" */
 typedef struct pmx_domain pmx_domain;
 int pmx_domain_create(int geometry, int n_refine, pmx_domain** out);
+/* Single-level n x n triangulation of the unit square (the mesh of the
+ * batched solves for n != 2^k; no reference counterpart): squares row by row,
+ * each cut along its (0,0)-(1,1) diagonal. */
+int pmx_domain_create_structured_tri(int n_side, pmx_domain** out);
 int pmx_domain_destroy(pmx_domain* d);
 /* number of cells of a level; vertex coordinates [n_cells][nv][dim] */
 int pmx_domain_level_info(const pmx_domain* d, int level, int* n_cells, int* n_vertices_per_cell,
@@ -255,6 +259,75 @@ int pmx_time_prepare_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_
  * active entries of A = (A - M) / factor */
 int pmx_time_reset_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_matrix* A,
                           const pmx_matrix* M);
+
+/* ------------------------------------------- batched time-dependent solves */
+/* Many independent TimeConvectionDiffusion<2> problems (P1 on an n x n
+ * unit-square triangulation each, own coefficients) advanced together by
+ * backward Euler.  No reference counterpart: the reference runs such parameter
+ * sweeps one problem after another; per problem the result is that of
+ * TimeConvectionDiffusion<2>::assemble/solve with the Galerkin TCD forms and
+ * TimeDiscretization (src/System/TimeConvectionDiffusion.C:222-386,
+ * src/TimeDiscRoutines/TimeDiscretizations.C:106-210).  For n = 2^k the mesh is
+ * TwoTriangles refined k times (the reference's cell order and DOF numbering);
+ * otherwise an n x n triangulation with the same diagonal direction, cells row
+ * by row.  The linear solve of each step is FP64-residual iterative refinement
+ * around FP32 GMRES(m) + Jacobi, run for all problems in one persistent launch
+ * (one workgroup per problem); each step is replayed from a HIP graph. */
+typedef struct pmx_batch pmx_batch;
+typedef struct {
+  int n_side;        /* squares per side (2 n^2 triangles) */
+  double eps;        /* diffusion */
+  double b[2];       /* convection */
+  double c;          /* reaction */
+} pmx_batch_problem;
+typedef struct {
+  int example;               /* PMX_EX_TCD_EXP / PMX_EX_TCD_LINEAR: f and boundary data at time t */
+  int scheme;                /* PMX_TIME_BACKWARD_EULER */
+  double tau;                /* time step length */
+  double t0;                 /* start time */
+  int gmres_restart;         /* inner FP32 GMRES(m), m <= 64 */
+  int max_outer;             /* FP64 refinement steps per time step */
+  double inner_reduction;    /* inner residual reduction per refinement step */
+  double residual_reduction; /* FP64 residual reduction per time step */
+  double residual_tolerance; /* FP64 absolute residual per time step */
+  int block;                 /* threads per problem: 256, 512, 1024, or 0 = tune on the first run */
+} pmx_batch_params;
+typedef struct {
+  int n_problems, total_dof, total_active, total_cells;
+  long long total_nnz;
+  int block;
+  double time;
+} pmx_batch_info;
+typedef struct {
+  double seconds;                 /* wall time of the replayed steps */
+  long long total_inner_iterations, total_outer_iterations;
+  int max_inner_iterations;       /* of one problem, summed over the steps run so far */
+  int failed_solves;              /* (problem, step) pairs that missed the tolerance */
+  int block;
+} pmx_batch_report;
+int pmx_batch_default_params(pmx_batch_params* p);
+int pmx_batch_create(pmx_ctx* ctx, int n_problems, const pmx_batch_problem* problems,
+                     const pmx_batch_params* p, pmx_batch** out);
+int pmx_batch_destroy(pmx_batch* b);
+int pmx_batch_get_info(const pmx_batch* b, pmx_batch_info* out);
+int pmx_batch_problem_size(const pmx_batch* b, int problem, int* n_dof, int* active_bound, int* nnz);
+/* time = t0, u = interpolant of the initial condition, assemble stiffness and
+ * mass matrices and the system matrix tau A + M of every problem */
+int pmx_batch_initialize(pmx_batch* b);
+/* advance every problem by n_steps backward-Euler steps (blocks until done) */
+int pmx_batch_run(pmx_batch* b, int n_steps, pmx_batch_report* rep);
+/* solution of one problem in its own DOF numbering */
+int pmx_batch_get_solution(const pmx_batch* b, int problem, double* host_u);
+/* inner / outer iteration counts and failed solves per problem (summed over steps) */
+int pmx_batch_get_iterations(const pmx_batch* b, int* host_inner, int* host_outer, int* host_failed);
+/* CSR pattern (TStructure) and cell DOF map of one problem */
+int pmx_batch_get_structure(const pmx_batch* b, int problem, int* host_row_ptr, int* host_col,
+                            int* host_global_numbers);
+/* stiffness, mass and system matrix values of one problem (any may be NULL) */
+int pmx_batch_get_matrices(const pmx_batch* b, int problem, double* host_stiffness, double* host_mass,
+                           double* host_system);
+/* L2, H1-semi, SD, L_inf errors of one problem at the current time */
+int pmx_batch_errors(pmx_batch* b, int problem, double* host_errors4);
 
 /* ------------------------------------------------------------------ solver */
 #define PMX_IT_FGMRES 0

@@ -36,6 +36,28 @@ class CDParams(ctypes.Structure):
                 ("time", c_double), ("theta1", c_double), ("tau", c_double)]
 
 
+class BatchProblem(ctypes.Structure):
+    _fields_ = [("n_side", c_int), ("eps", c_double), ("b", c_double * 2), ("c", c_double)]
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [("example", c_int), ("scheme", c_int), ("tau", c_double), ("t0", c_double),
+                ("gmres_restart", c_int), ("max_outer", c_int), ("inner_reduction", c_double),
+                ("residual_reduction", c_double), ("residual_tolerance", c_double), ("block", c_int)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("n_problems", c_int), ("total_dof", c_int), ("total_active", c_int),
+                ("total_cells", c_int), ("total_nnz", ctypes.c_longlong), ("block", c_int),
+                ("time", c_double)]
+
+
+class BatchReport(ctypes.Structure):
+    _fields_ = [("seconds", c_double), ("total_inner_iterations", ctypes.c_longlong),
+                ("total_outer_iterations", ctypes.c_longlong), ("max_inner_iterations", c_int),
+                ("failed_solves", c_int), ("block", c_int)]
+
+
 class SolverParams(ctypes.Structure):
     _fields_ = [("iterative_solver_type", c_int), ("preconditioner", c_int),
                 ("max_n_iterations", c_int), ("min_n_iterations", c_int),
@@ -61,6 +83,7 @@ SIGNATURES = {
     "pmx_ctx_destroy": [c_void_p],
     "pmx_ctx_synchronize": [c_void_p],
     "pmx_domain_create": [c_int, c_int, P(c_void_p)],
+    "pmx_domain_create_structured_tri": [c_int, P(c_void_p)],
     "pmx_domain_destroy": [c_void_p],
     "pmx_domain_level_info": [c_void_p, c_int, P(c_int), P(c_int), P(c_int)],
     "pmx_domain_get_vertices": [c_void_p, c_int, P(c_double)],
@@ -104,6 +127,18 @@ SIGNATURES = {
                              c_void_p, c_void_p],
     "pmx_time_prepare_system": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p],
     "pmx_time_reset_system": [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p],
+    "pmx_batch_default_params": [P(BatchParams)],
+    "pmx_batch_create": [c_void_p, c_int, P(BatchProblem), P(BatchParams), P(c_void_p)],
+    "pmx_batch_destroy": [c_void_p],
+    "pmx_batch_get_info": [c_void_p, P(BatchInfo)],
+    "pmx_batch_problem_size": [c_void_p, c_int, P(c_int), P(c_int), P(c_int)],
+    "pmx_batch_initialize": [c_void_p],
+    "pmx_batch_run": [c_void_p, c_int, P(BatchReport)],
+    "pmx_batch_get_solution": [c_void_p, c_int, P(c_double)],
+    "pmx_batch_get_iterations": [c_void_p, P(c_int), P(c_int), P(c_int)],
+    "pmx_batch_get_structure": [c_void_p, c_int, P(c_int), P(c_int), P(c_int)],
+    "pmx_batch_get_matrices": [c_void_p, c_int, P(c_double), P(c_double), P(c_double)],
+    "pmx_batch_errors": [c_void_p, c_int, P(c_double)],
     "pmx_solver_default_params": [P(SolverParams)],
     "pmx_solver_create": [c_void_p, P(SolverParams), P(c_void_p)],
     "pmx_solver_destroy": [c_void_p],

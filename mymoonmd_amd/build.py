@@ -21,8 +21,10 @@ SOURCES = [
     "host/space.cpp",
     "capi.cpp",
     "solver.cpp",
+    "batch.cpp",
     "hip/kernels_la.hip",
     "hip/kernels_asm.hip",
+    "hip/kernels_batch.hip",
 ]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")

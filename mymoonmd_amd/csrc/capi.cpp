@@ -212,6 +212,16 @@ int pmx_domain_create(int geometry, int n_refine, pmx_domain** out) {
   });
 }
 
+int pmx_domain_create_structured_tri(int n_side, pmx_domain** out) {
+  return guarded([&] {
+    require(out != nullptr, "out is NULL");
+    require(n_side >= 1 && n_side <= 4096, "n_side must be in [1,4096]");
+    auto d = std::make_unique<pmx_domain>();
+    d->mesh = pmx::build_structured_tri_mesh(n_side);
+    *out = d.release();
+  });
+}
+
 int pmx_domain_destroy(pmx_domain* d) {
   return guarded([&] { delete d; });
 }

@@ -50,11 +50,16 @@ __device__ void coeffs_at(const CDCoeffs& co, const double* geo, const double* x
     x[r] = s;
   }
   double eps = co.eps, b[3] = {co.b[0], co.b[1], co.b[2]}, c = co.c, f;
+  const double t = co.time_dev ? *co.time_dev : co.time;
+  if (co.prob_coef) {
+    const double* pc = co.prob_coef + 4 * co.cell_prob[cell];
+    eps = pc[0]; b[0] = pc[1]; b[1] = pc[2]; b[2] = 0.0; c = pc[3];
+  }
   if (co.table) {
     const double* t = co.table + ((size_t)cell * nq + q) * 8;
     eps = t[0]; b[0] = t[1]; b[1] = t[2]; b[2] = (DIM == 3) ? t[3] : 0.0; c = t[4]; f = t[5];
   } else {
-    f = example_rhs(co.example, x[0], x[1], x[2], eps, b, c, co.f0, co.time);
+    f = example_rhs(co.example, x[0], x[1], x[2], eps, b, c, co.f0, t);
   }
   const double detJ = geo[21], hK = geo[22];
   out[0] = eps; out[1] = b[0]; out[2] = b[1]; out[3] = b[2]; out[4] = c; out[5] = f;
@@ -320,7 +325,8 @@ __global__ __launch_bounds__(256) void k_assemble_tcd(AssemblyData ad, CDCoeffs 
   if (!valid || i >= ND) return;
   const int* dofs = ad.dofs + (size_t)cell * ND;
   const int r = dofs[i];
-  if (r < active) {
+  const int act_end = co.prob_active_end ? co.prob_active_end[co.cell_prob[cell]] : active;
+  if (r < act_end) {
     if (A.val) {
       const int rs = A.row_ptr[r], re = A.row_ptr[r + 1];
 #pragma unroll

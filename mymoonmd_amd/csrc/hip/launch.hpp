@@ -114,7 +114,55 @@ struct CDCoeffs {
   double time = 0;        // TimeDB->CURRENTTIME of time-dependent examples
   double theta1 = 1, tau = 0;      // TimeDB->THETA1, CURRENTTIMESTEPLENGTH (TCD SUPG)
   const double* table = nullptr;   // optional per-(cell,q) coefficients [8]
+  // batched assembly: coefficients per problem and per-problem active bounds
+  const double* prob_coef = nullptr;      // [n_prob][4] eps, b1, b2, c
+  const int* cell_prob = nullptr;         // problem of each cell
+  const int* prob_active_end = nullptr;   // first Dirichlet row (global) of each problem
+  const double* time_dev = nullptr;       // time read from device memory (graph replays)
 };
+
+// ---- batched time-dependent solve (kernels_batch.hip)
+struct BatchDev {
+  int n_prob = 0;
+  const int* order = nullptr;       // workgroup -> problem (largest first)
+  const int* dof_off = nullptr;     // [n_prob+1]
+  const int* n_active = nullptr;    // [n_prob]
+  const int* slice_off = nullptr;   // [n_prob+1]
+  const long long* ws_off = nullptr;   // FP32 Krylov workspace offset per problem
+  const int* sptr = nullptr;        // slice -> first slot
+  const int* srow = nullptr;        // slot row (local, -1 empty) [slices*64]
+  const int* scol = nullptr;        // slot column (local)
+  const double* S64 = nullptr;      // system matrix tau A + M, FP64 slots
+  const float* S32 = nullptr;       // the same in FP32
+  const float* dinv32 = nullptr;    // 1/diag(S) per DOF
+  double* x = nullptr;
+  const double* rhs = nullptr;
+  double* r = nullptr;
+  float* w = nullptr;
+  float* V = nullptr;
+  int* iters = nullptr;
+  int* outer = nullptr;
+  double* res = nullptr;
+  int* fail = nullptr;
+};
+struct BatchSolveParams {
+  int m = 30;              // GMRES restart (<= 64)
+  int max_outer = 50;      // FP64 refinement steps per time step
+  double inner_red = 1e-4; // FP32 GMRES residual reduction per refinement step
+  double red = 1e-10;      // FP64 residual reduction per time step
+  double tol = 0.0;        // FP64 absolute residual
+};
+void batch_solve(const BatchDev& B, const BatchSolveParams& P, int block, hipStream_t st);
+void batch_rhs(int n_slices, const int* slice_prob, const int* dof_off, const int* sptr, const int* srow,
+               const int* scol, const double* M64, double tau, const double* f, const double* u,
+               double* rhs, hipStream_t st);
+void batch_dirichlet_copy(int n, const int* idx, const double* f, double* rhs, double* u, hipStream_t st);
+void batch_dirichlet_values(int example, const double* t_dev, int n, const int* idx, const double* xyz,
+                            double* out, hipStream_t st);
+void batch_system(int n_rows, const int* rp, const uint8_t* row_active, double tau, const double* A,
+                  const double* M, double* S, hipStream_t st);
+void add_scalar(double* t, double dt, hipStream_t st);
+void sell_gather(size_t total, const int* src, const double* v, double* out64, float* out32, hipStream_t st);
 
 // TCD forms of the cells cells[0..n) (one colour): stiffness into A, mass into
 // M, load vector into rhs; any of A.val, M.val, rhs may be null.

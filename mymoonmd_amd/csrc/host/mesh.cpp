@@ -181,4 +181,32 @@ Mesh build_mesh(int geometry, int n_refine) {
   return m;
 }
 
+Mesh build_structured_tri_mesh(int n) {
+  if (n < 1 || n > 4096) throw std::runtime_error("structured triangle mesh: n out of range [1,4096]");
+  Mesh m;
+  m.geometry = GEO_UNIT_SQUARE_TRI;
+  m.dim = 2; m.shape = SHAPE_TRI; m.nv = 3; m.n_faces = 3;
+  m.base_n = n;
+  MeshLevel l0;
+  l0.n_cells = 2 * n * n;
+  l0.xyz.resize((size_t)l0.n_cells * 6);
+  const double h = 1.0 / n;
+  size_t k = 0;
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i < n; ++i) {
+      const double x0 = i * h, y0 = j * h;
+      const double x1 = (i + 1 == n) ? 1.0 : (i + 1) * h, y1 = (j + 1 == n) ? 1.0 : (j + 1) * h;
+      const double t0[6] = {x0, y0, x1, y0, x1, y1};
+      const double t1[6] = {x0, y0, x1, y1, x0, y1};
+      for (int a = 0; a < 6; ++a) l0.xyz[k + a] = t0[a];
+      for (int a = 0; a < 6; ++a) l0.xyz[k + 6 + a] = t1[a];
+      k += 12;
+    }
+  }
+  l0.parent.assign(l0.n_cells, -1);
+  l0.child_no.assign(l0.n_cells, 0);
+  m.levels.push_back(std::move(l0));
+  return m;
+}
+
 }  // namespace pmx

@@ -43,6 +43,7 @@ struct Mesh {
   Shape shape = SHAPE_QUAD;
   int nv = 0;                  // vertices per cell
   int n_faces = 0;             // joints per cell (edges in 2D, faces in 3D)
+  int base_n = 1;              // level-0 vertices lie on the grid (1/base_n) Z^d
   std::vector<MeshLevel> levels;   // levels[0] = macro mesh
 
   const double* vertex(int level, int cell, int v) const {
@@ -52,6 +53,13 @@ struct Mesh {
 
 // Build the macro mesh and refine it n_refine times uniformly.
 Mesh build_mesh(int geometry, int n_refine);
+
+// Single-level triangulation of the unit square with n x n squares, each cut
+// along its (0,0)-(1,1) diagonal into (v00, v10, v11) and (v00, v11, v01) as
+// the two macro triangles of TwoTriangles; squares row by row from y = 0.
+// For n = 2^k the mesh covers the same triangles as TwoTriangles refined k
+// times, but in another cell order (so another DOF numbering).
+Mesh build_structured_tri_mesh(int n);
 
 // Local vertices of joint j of a cell (ParMooN shape descriptors).  Returns
 // the number of vertices of that joint.

@@ -9,9 +9,9 @@ namespace pmx {
 namespace {
 
 // Lattice resolution: nodal points of order-k elements on a mesh refined L
-// times from the unit square/cube lie on the grid (1/(k 2^L)) Z^d.
+// times from a macro mesh on the grid (1/n) Z^d lie on (1/(k n 2^L)) Z^d.
 int64_t lattice_size(const Mesh& m, int level, int order) {
-  return (int64_t)order << level;
+  return ((int64_t)order * m.base_n) << level;
 }
 
 int64_t lattice_key(const double* x, int dim, int64_t S) {

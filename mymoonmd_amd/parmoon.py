@@ -13,8 +13,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (BC2D_FN, BC3D_FN, CDParams, SolveReport, SolverParams, SpaceInfo, call,
-                   c_double, c_int, c_void_p)
+from ._lib import (BC2D_FN, BC3D_FN, BatchInfo, BatchParams, BatchProblem, BatchReport, CDParams,
+                   SolveReport, SolverParams, SpaceInfo, call, c_double, c_int, c_void_p)
 
 GEOMETRY = {"UnitSquare": 0, "TwoTriangles": 1, "Default_UnitCube_Hexa": 2}
 EXAMPLES = {"const": 0, "sine_laplace_2d": 1, "p0_3d": 2, "p1_3d": 3, "p2_3d": 4,
@@ -58,10 +58,15 @@ class Context:
 class TDomain:
     """Default geometry refined `n_refine` times; level k = collection after k refinements."""
 
-    def __init__(self, geo_file="UnitSquare", n_refine=0):
+    def __init__(self, geo_file="UnitSquare", n_refine=0, structured_n=None):
         self.h = c_void_p()
-        call("pmx_domain_create", GEOMETRY[geo_file], n_refine, ctypes.byref(self.h))
-        self.n_refine = n_refine
+        if structured_n is not None:
+            # n x n triangulation in one level (mesh of the batched solves for n != 2^k)
+            call("pmx_domain_create_structured_tri", structured_n, ctypes.byref(self.h))
+            self.n_refine = 0
+        else:
+            call("pmx_domain_create", GEOMETRY[geo_file], n_refine, ctypes.byref(self.h))
+            self.n_refine = n_refine
 
     def level_info(self, level):
         n, nv, dim = c_int(), c_int(), c_int()
@@ -538,3 +543,93 @@ class TimeConvectionDiffusion:
     def output(self):
         """L2, H1-semi, SD, L_inf errors at the current time."""
         return compute_errors(self.ctx, self.spaces[-1], self.params(), self.solution)
+
+
+class BatchTCD:
+    """Many independent TCD problems (P1 on n x n unit-square triangulations,
+    own eps, b, c) advanced together by backward Euler (pmx_batch_*).
+    `problems` = list of (n_side, eps, (b1, b2), c)."""
+
+    def __init__(self, ctx, problems, example="tcd_exp", tau=0.01, t0=0.0, gmres_restart=30,
+                 max_outer=50, inner_reduction=1e-4, residual_reduction=1e-10, residual_tolerance=0.0,
+                 block=0):
+        self.ctx = ctx
+        prm = BatchParams()
+        call("pmx_batch_default_params", ctypes.byref(prm))
+        prm.example = EXAMPLES[example]
+        prm.scheme = TIME_SCHEME["backward_euler"]
+        prm.tau, prm.t0 = tau, t0
+        prm.gmres_restart, prm.max_outer = gmres_restart, max_outer
+        prm.inner_reduction, prm.residual_reduction = inner_reduction, residual_reduction
+        prm.residual_tolerance, prm.block = residual_tolerance, block
+        arr = (BatchProblem * len(problems))()
+        for k, (n, eps, b, c) in enumerate(problems):
+            arr[k].n_side, arr[k].eps, arr[k].c = n, eps, c
+            arr[k].b[0], arr[k].b[1] = b[0], b[1]
+        self.params = prm
+        self.h = c_void_p()
+        call("pmx_batch_create", ctx.h, len(problems), arr, ctypes.byref(prm), ctypes.byref(self.h))
+        self.n_problems = len(problems)
+
+    def info(self):
+        inf = BatchInfo()
+        call("pmx_batch_get_info", self.h, ctypes.byref(inf))
+        return inf
+
+    def problem_size(self, p):
+        n, ab, nnz = c_int(), c_int(), c_int()
+        call("pmx_batch_problem_size", self.h, p, ctypes.byref(n), ctypes.byref(ab), ctypes.byref(nnz))
+        return n.value, ab.value, nnz.value
+
+    def initialize(self):
+        call("pmx_batch_initialize", self.h)
+
+    def run(self, n_steps):
+        rep = BatchReport()
+        call("pmx_batch_run", self.h, n_steps, ctypes.byref(rep))
+        return rep
+
+    def solution(self, p):
+        n, _, _ = self.problem_size(p)
+        out = np.zeros(n)
+        call("pmx_batch_get_solution", self.h, p, out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out
+
+    def iterations(self):
+        inner = np.zeros(self.n_problems, np.int32)
+        outer = np.zeros(self.n_problems, np.int32)
+        failed = np.zeros(self.n_problems, np.int32)
+        P = ctypes.POINTER(c_int)
+        call("pmx_batch_get_iterations", self.h, inner.ctypes.data_as(P), outer.ctypes.data_as(P),
+             failed.ctypes.data_as(P))
+        return inner, outer, failed
+
+    def structure(self, p):
+        n, ab, nnz = self.problem_size(p)
+        rp = np.zeros(n + 1, np.int32)
+        col = np.zeros(nnz, np.int32)
+        P = ctypes.POINTER(c_int)
+        call("pmx_batch_get_structure", self.h, p, rp.ctypes.data_as(P), col.ctypes.data_as(P), None)
+        return rp, col
+
+    def global_numbers(self, p, n_cells):
+        g = np.zeros(n_cells * 3, np.int32)
+        call("pmx_batch_get_structure", self.h, p, None, None, g.ctypes.data_as(ctypes.POINTER(c_int)))
+        return g.reshape(n_cells, 3)
+
+    def matrices(self, p):
+        _, _, nnz = self.problem_size(p)
+        a, m, s = np.zeros(nnz), np.zeros(nnz), np.zeros(nnz)
+        P = ctypes.POINTER(c_double)
+        call("pmx_batch_get_matrices", self.h, p, a.ctypes.data_as(P), m.ctypes.data_as(P), s.ctypes.data_as(P))
+        return a, m, s
+
+    def errors(self, p):
+        out = (c_double * 4)()
+        call("pmx_batch_errors", self.h, p, out)
+        return list(out)
+
+    def __del__(self, _d=_destroy):
+        if getattr(self, "h", None) and self.h.value:
+            _d("pmx_batch_destroy", self.h)
+            self.h = None

@@ -1,0 +1,131 @@
+"""Batched TCD solves (pmx_batch_*): every problem of a batch against the
+single-problem HIP path (TimeConvectionDiffusion mirror) and, on the
+reference meshes (n = 2^k), against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from mymoonmd_amd import parmoon as pm
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# (n_side, eps, b, c): powers of two use TwoTriangles refinement, others the
+# structured generator
+PROBLEMS = [(8, 1.0, (1.0, 0.0), 0.0), (12, 1e-2, (0.6, 0.8), 0.0), (16, 1e-4, (-1.0, 0.0), 0.0),
+            (20, 0.3, (0.0, -1.0), 0.5), (33, 1e-3, (0.28, -0.96), 0.0)]
+TAU, STEPS = 0.01, 5
+
+
+def single_domain(n):
+    k = int(round(np.log2(n)))
+    if 2 ** k == n:
+        return pm.TDomain("TwoTriangles", k)
+    return pm.TDomain(structured_n=n)
+
+
+def single_problem(ctx, n, eps, b, c, example="tcd_exp", steps=STEPS):
+    d = single_domain(n)
+    tss = pm.TimeDiscretization(ctx, "backward_euler", TAU, 0.0, TAU * steps)
+    solver = pm.solver_params(iterative_solver_type="fgmres", preconditioner="jacobi", gmres_restart=50,
+                              max_n_iterations=5000, residual_tolerance=1e-13, residual_reduction=0.0)
+    tcd = pm.TimeConvectionDiffusion(ctx, d, 1, example, tss, solver, eps=eps, b=(b[0], b[1], 0.0), c=c)
+    tss.current_step_ = 0
+    tcd.assemble_initial_time()
+    while not tss.reached_final_time_step():
+        tss.current_step_ += 1
+        tss.current_time_ += TAU
+        tcd.assemble()
+        tcd.solve()
+    return tcd
+
+
+@pytest.fixture(scope="module")
+def batch(ctx):
+    bt = pm.BatchTCD(ctx, PROBLEMS, tau=TAU, block=256)
+    bt.initialize()
+    rep = bt.run(STEPS)
+    print("batch report", rep.seconds, rep.total_inner_iterations, rep.failed_solves, flush=True)
+    assert rep.failed_solves == 0
+    return bt
+
+
+@pytest.mark.parametrize("k", range(len(PROBLEMS)))
+def test_batch_problem_matches_single_problem_path(ctx, batch, k):
+    n, eps, b, c = PROBLEMS[k]
+    print("problem", k, n, eps, b, c, flush=True)
+    tcd = single_problem(ctx, n, eps, b, c)
+    s = tcd.spaces[-1]
+    rp, col = s.structure()
+    brp, bcol = batch.structure(k)
+    assert np.array_equal(rp, brp) and np.array_equal(col, bcol)
+    # matrices: the batch keeps A, M and S = tau A + M
+    a, m, sy = batch.matrices(k)
+    A1, M1 = pm.FEMatrix(ctx, s), pm.FEMatrix(ctx, s)
+    p = pm.cd_params("tcd_exp", eps=eps, b=(b[0], b[1], 0.0), c=c)
+    pm.assemble_tcd(ctx, s, p, A1, M1, None)
+    ga, gm = A1.get_entries(), M1.get_entries()
+    assert np.max(np.abs(a - ga)) <= 1e-12 * np.max(np.abs(ga))
+    assert np.max(np.abs(m - gm)) <= 1e-12 * np.max(np.abs(gm))
+    na = rp[s.active_bound]
+    assert np.max(np.abs(sy[:na] - (TAU * ga[:na] + gm[:na]))) <= 1e-12 * np.max(np.abs(sy))
+    # solution after the steps
+    u1 = tcd.solution.cpu().numpy()
+    ub = batch.solution(k)
+    assert np.max(np.abs(ub - u1)) <= 1e-8 * np.max(np.abs(u1)), np.max(np.abs(ub - u1))
+    e1 = tcd.output()
+    eb = batch.errors(k)
+    assert abs(eb[0] - e1[0]) <= 1e-8 * e1[0] + 1e-14
+
+
+@pytest.mark.parametrize("k", [0, 2])
+def test_batch_problem_matches_oracle(batch, k):
+    """n = 8, 16: TwoTriangles meshes, the oracle's restatement of
+    TimeConvectionDiffusion with backward Euler."""
+    n, eps, b, c = PROBLEMS[k]
+    lev = int(round(np.log2(n)))
+    sp, info = O.build_space(1, lev, lev, 1)
+    errs, u = O.tcd_time_integration(sp, 7, 0, O.BACKWARD_EULER, TAU, STEPS, eps=eps, b=b, c=c)
+    ub = batch.solution(k)
+    assert np.max(np.abs(ub - u)) <= 1e-8 * np.max(np.abs(u))
+
+
+def test_batch_linear_space_time_exact(ctx):
+    probs = [(4, 1.0, (1.0, 2.0), 1.0), (7, 0.01, (1.0, 2.0), 1.0), (16, 1e-4, (1.0, 2.0), 1.0)]
+    bt = pm.BatchTCD(ctx, probs, example="tcd_linear", tau=0.1, block=512)
+    bt.initialize()
+    rep = bt.run(4)
+    assert rep.failed_solves == 0
+    for k in range(len(probs)):
+        e = bt.errors(k)
+        assert e[0] < 1e-9 and e[1] < 1e-8, (k, e)
+
+
+def test_batch_graph_replays_and_block_sizes_agree(ctx):
+    """5 replays in one run == 5 runs of one replay (eager host work between
+    the replays); workgroup sizes 256 and 1024 agree to the solver tolerance."""
+    probs = PROBLEMS[:3]
+    sols = {}
+    for blk, split in ((256, False), (256, True), (1024, False)):
+        bt = pm.BatchTCD(ctx, probs, tau=TAU, block=blk)
+        bt.initialize()
+        if split:
+            for _ in range(STEPS):
+                rep = bt.run(1)
+                _ = bt.solution(0)
+        else:
+            rep = bt.run(STEPS)
+        assert rep.failed_solves == 0
+        sols[(blk, split)] = [bt.solution(k) for k in range(len(probs))]
+    for k in range(len(probs)):
+        assert np.array_equal(sols[(256, False)][k], sols[(256, True)][k])
+        ref = sols[(256, False)][k]
+        assert np.max(np.abs(sols[(1024, False)][k] - ref)) <= 1e-8 * np.max(np.abs(ref))
+
+
+def test_batch_autotune_picks_a_block(ctx):
+    bt = pm.BatchTCD(ctx, PROBLEMS[:2], tau=TAU, block=0)
+    bt.initialize()
+    rep = bt.run(2)
+    assert rep.block in (256, 512, 1024) and rep.failed_solves == 0
+    assert abs(bt.info().time - 2 * TAU) < 1e-15

@@ -130,3 +130,29 @@ def test_prolongation_matches_oracle_transfer(geo, nref, order):
     for r in range(sf.n_dof):
         np.add.at(PTd, col[rp[r]:rp[r + 1]], val[rp[r]:rp[r + 1]] * dvec[r])
     assert np.max(np.abs(PTd - O.restrict(oc, of, dvec))) < 1e-12
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 13, 32])
+def test_structured_triangulation_pattern_closed_form(n):
+    """n x n triangulation (mesh of the batched solves): P1 has (n+1)^2 DOFs,
+    (n-1)^2 inner ones with 7 entries per row, one entry per Dirichlet row."""
+    d = pm.TDomain(structured_n=n)
+    s = pm.TFESpace(d, 0, 1, None)
+    rp, col = s.structure()
+    assert s.n_dof == (n + 1) ** 2 and s.active_bound == (n - 1) ** 2
+    assert len(col) == 7 * (n - 1) ** 2 + 4 * n
+    for r in range(s.n_dof):
+        row = col[rp[r]:rp[r + 1]]
+        assert np.all(np.diff(row) > 0)
+        if r >= s.active_bound:
+            assert list(row) == [r]
+
+
+def test_structured_triangulation_matches_refined_two_triangles():
+    """For n = 2^k the structured mesh has the same triangles as TwoTriangles
+    refined k times (other cell order): equal DOF coordinates as sets."""
+    a = pm.TFESpace(pm.TDomain(structured_n=8), 0, 1, None).dof_coords()
+    b = pm.TFESpace(pm.TDomain("TwoTriangles", 3), 3, 1, None).dof_coords()
+    ka = sorted(map(tuple, np.round(a, 12)))
+    kb = sorted(map(tuple, np.round(b, 12)))
+    assert ka == kb
