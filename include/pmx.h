@@ -271,9 +271,11 @@ int pmx_time_reset_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_ma
  * TwoTriangles refined k times (the reference's cell order and DOF numbering);
  * otherwise an n x n triangulation with the same diagonal direction, cells row
  * by row.  The linear solve of each step is FP64-residual iterative refinement
- * around FP32 GMRES(m) + Jacobi, run for all problems in one persistent launch
+ * around FP32 GMRES(m) or BiCGStab + Jacobi, run for all problems in one persistent launch
  * (one workgroup per problem); each step is replayed from a HIP graph. */
 typedef struct pmx_batch pmx_batch;
+#define PMX_BATCH_INNER_GMRES 0     /* restarted GMRES(m) (Iteration_gmres, right preconditioned) */
+#define PMX_BATCH_INNER_BICGSTAB 1  /* BiCGStab (Iteration_bicgstab, right preconditioned) */
 typedef struct {
   int n_side;        /* squares per side (2 n^2 triangles) */
   double eps;        /* diffusion */
@@ -285,7 +287,9 @@ typedef struct {
   int scheme;                /* PMX_TIME_BACKWARD_EULER */
   double tau;                /* time step length */
   double t0;                 /* start time */
-  int gmres_restart;         /* inner FP32 GMRES(m), m <= 64 */
+  int inner_solver;          /* PMX_BATCH_INNER_GMRES or PMX_BATCH_INNER_BICGSTAB (FP32) */
+  int gmres_restart;         /* inner GMRES(m), m <= 64 */
+  int max_inner;             /* inner BiCGStab iterations per refinement step */
   int max_outer;             /* FP64 refinement steps per time step */
   double inner_reduction;    /* inner residual reduction per refinement step */
   double residual_reduction; /* FP64 residual reduction per time step */

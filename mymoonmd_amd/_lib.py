@@ -42,7 +42,8 @@ class BatchProblem(ctypes.Structure):
 
 class BatchParams(ctypes.Structure):
     _fields_ = [("example", c_int), ("scheme", c_int), ("tau", c_double), ("t0", c_double),
-                ("gmres_restart", c_int), ("max_outer", c_int), ("inner_reduction", c_double),
+                ("inner_solver", c_int), ("gmres_restart", c_int), ("max_inner", c_int),
+                ("max_outer", c_int), ("inner_reduction", c_double),
                 ("residual_reduction", c_double), ("residual_tolerance", c_double), ("block", c_int)]
 
 

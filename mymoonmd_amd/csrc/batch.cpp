@@ -149,6 +149,8 @@ struct pmx_batch {
 
   pmx::BatchSolveParams solve_params() const {
     pmx::BatchSolveParams P;
+    P.method = prm.inner_solver;
+    P.max_inner = prm.max_inner;
     P.m = prm.gmres_restart;
     P.max_outer = prm.max_outer;
     P.inner_red = prm.inner_reduction;
@@ -220,7 +222,9 @@ int pmx_batch_default_params(pmx_batch_params* p) {
     p->scheme = PMX_TIME_BACKWARD_EULER;
     p->tau = 0.01;
     p->t0 = 0.0;
+    p->inner_solver = PMX_BATCH_INNER_BICGSTAB;
     p->gmres_restart = 30;
+    p->max_inner = 2000;
     p->max_outer = 50;
     p->inner_reduction = 1e-4;
     p->residual_reduction = 1e-10;
@@ -239,6 +243,9 @@ int pmx_batch_create(pmx_ctx* ctx, int n_problems, const pmx_batch_problem* prob
     require(prm->tau > 0, "time step length must be positive");
     require(prm->gmres_restart >= 1 && prm->gmres_restart <= 64, "gmres_restart must be in [1,64]");
     require(prm->max_outer >= 1, "max_outer must be positive");
+    require(prm->inner_solver == PMX_BATCH_INNER_GMRES || prm->inner_solver == PMX_BATCH_INNER_BICGSTAB,
+            "unknown inner solver");
+    require(prm->max_inner >= 1, "max_inner must be positive");
     require(prm->block == 0 || batch_block(prm->block), "block must be 0 (tune), 256, 512 or 1024");
     for (int p = 0; p < n_problems; ++p) {
       require(probs[p].n_side >= 2 && probs[p].n_side <= 2048, "n_side must be in [2,2048]");
@@ -259,7 +266,7 @@ int pmx_batch_create(pmx_ctx* ctx, int n_problems, const pmx_batch_problem* prob
     B->nnz_off.assign(np + 1, 0);
     B->ws_off.assign(np + 1, 0);
     B->n_active.assign(np, 0);
-    const int m1 = prm->gmres_restart + 1;
+    const int m1 = (prm->inner_solver == PMX_BATCH_INNER_GMRES) ? prm->gmres_restart + 1 : 7;
     for (int p = 0; p < np; ++p) {
       const pmx::FESpace& fs = B->spaces[p];
       B->n_active[p] = fs.active_bound;

@@ -172,6 +172,199 @@ __global__ __launch_bounds__(BS) void k_batch_solve(BatchDev B, BatchSolveParams
   }
 }
 
+// Slice loops of one wave over its share of a problem's slices, with U
+// slices in flight (independent accumulators) to hide memory latency: a
+// single workgroup has to stream the whole problem.
+template <int U, typename VT, typename AT, typename Z, typename F>
+__device__ __forceinline__ void wave_spmv(const BatchDev& B, const VT* __restrict__ val, int s0, int s1, int wv,
+                                          int nw, int lane, Z z, F f) {
+  for (int sb = s0 + wv; sb < s1; sb += U * nw) {
+    int lr[U], a[U], wd[U];
+    AT acc[U];
+    int wmax = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = sb + u * nw;
+      const bool ok = s < s1;
+      lr[u] = ok ? B.srow[(size_t)s * kWave + lane] : -1;
+      a[u] = ok ? B.sptr[s] : 0;
+      wd[u] = ok ? (B.sptr[s + 1] - a[u]) / kWave : 0;
+      acc[u] = 0;
+      wmax = wd[u] > wmax ? wd[u] : wmax;
+    }
+    for (int j = 0; j < wmax; ++j) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (j < wd[u]) {
+          const int k = a[u] + j * kWave + lane;
+          acc[u] += (AT)val[k] * z(B.scol[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (lr[u] >= 0) f(lr[u], acc[u]);
+  }
+}
+
+template <int U, typename F>
+__device__ __forceinline__ void wave_rows(const BatchDev& B, int s0, int s1, int wv, int nw, int lane, F f) {
+  for (int sb = s0 + wv; sb < s1; sb += U * nw) {
+    int lr[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int s = sb + u * nw;
+      lr[u] = (s < s1) ? B.srow[(size_t)s * kWave + lane] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (lr[u] >= 0) f(lr[u]);
+  }
+}
+
+// Same refinement loop with BiCGStab (Iteration_bicgstab's method,
+// src/Solver/Iteration_bicgstab.C) as the FP32 inner solver, right-
+// preconditioned by Jacobi.  Every lane owns the rows of its slices for all
+// vector updates, so the only cross-thread data flow is the gather of the
+// SpMVs; dots are fused into the passes that produce their operands.
+// Workspace per problem: ri, rh, p, v, s, t, d (7 x n_active floats).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_batch_solve_bicgstab(BatchDev B, BatchSolveParams P) {
+  constexpr int NW = BS / kWave;
+  constexpr int U = 4;
+  __shared__ double red[NW];
+  __shared__ double red2[NW];
+  const int p = B.order[blockIdx.x];
+  const int off = B.dof_off[p], na = B.n_active[p];
+  const int s0 = B.slice_off[p], s1 = B.slice_off[p + 1];
+  double* x = B.x + off;
+  const double* rhs = B.rhs + off;
+  const float* dinv = B.dinv32 + off;
+  float* ri = B.V + B.ws_off[p];
+  float* rh = ri + na;
+  float* pv = rh + na;
+  float* v = pv + na;
+  float* sv = v + na;
+  float* tv = sv + na;
+  float* dv = tv + na;
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  // two block sums at once (fixed order)
+  auto sum2 = [&](double a, double b, double& ra, double& rb) {
+    a = group_sum<kWave>(a);
+    b = group_sum<kWave>(b);
+    if (lane == 0) { red[wv] = a; red2[wv] = b; }
+    __syncthreads();
+    double sa = 0, sb = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) { sa += red[i]; sb += red2[i]; }
+    __syncthreads();
+    ra = sa;
+    rb = sb;
+  };
+  double rn0 = 0, rn = 0;
+  int outer = 0, inner_total = 0;
+  for (;;) {
+    // FP64 residual; the inner right-hand side ri = rh = (float) r, d = 0
+    double part = 0;
+    wave_spmv<U, double, double>(B, B.S64, s0, s1, wv, NW, lane, [&](int c) { return x[c]; },
+                                 [&](int lr, double acc) {
+                                   const double d = rhs[lr] - acc;
+                                   part += d * d;
+                                   ri[lr] = (float)d;
+                                   rh[lr] = (float)d;
+                                   pv[lr] = 0.0f;
+                                   v[lr] = 0.0f;
+                                   dv[lr] = 0.0f;
+                                 });
+    double rr, dummy;
+    sum2(part, 0.0, rr, dummy);
+    rn = sqrt(rr);
+    if (outer == 0) rn0 = rn;
+    if (rn <= P.red * rn0 || rn <= P.tol || outer >= P.max_outer) break;
+    ++outer;
+    // rho = (rh, ri) of the FP32 copy
+    double rho1 = 0;
+    {
+      double a = 0;
+      wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) { const double q = ri[lr]; a += q * q; });
+      sum2(a, 0.0, rho1, dummy);
+    }
+    const double stop = P.inner_red * sqrt(rho1);
+    double rho = 1, alpha = 1, omega = 1;
+    int it = 0;
+    for (; it < P.max_inner; ++it) {
+      if (!(fabs(rho1) > 0)) break;
+      const float beta = (float)((rho1 / rho) * (alpha / omega));
+      const float om = (float)omega;
+      // p = ri + beta (p - omega v)
+      wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) { pv[lr] = ri[lr] + beta * (pv[lr] - om * v[lr]); });
+      __syncthreads();
+      // v = S D^-1 p ; (rh, v)
+      double a = 0;
+      wave_spmv<U, float, float>(B, B.S32, s0, s1, wv, NW, lane,
+                                 [&](int c) { return (c < na) ? dinv[c] * pv[c] : 0.0f; },
+                                 [&](int lr, float acc) { v[lr] = acc; a += (double)rh[lr] * (double)acc; });
+      double rhv;
+      sum2(a, 0.0, rhv, dummy);
+      if (!(fabs(rhv) > 0)) break;
+      alpha = rho1 / rhv;
+      const float al = (float)alpha;
+      // s = ri - alpha v ; ||s||
+      a = 0;
+      wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) {
+        const float q = ri[lr] - al * v[lr];
+        sv[lr] = q;
+        a += (double)q * (double)q;
+      });
+      double ss;
+      sum2(a, 0.0, ss, dummy);
+      if (sqrt(ss) <= stop) {
+        wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) { dv[lr] += al * dinv[lr] * pv[lr]; });
+        ++it;
+        break;
+      }
+      // t = S D^-1 s ; (t, s), (t, t)
+      a = 0;
+      double bb = 0;
+      wave_spmv<U, float, float>(B, B.S32, s0, s1, wv, NW, lane,
+                                 [&](int c) { return (c < na) ? dinv[c] * sv[c] : 0.0f; },
+                                 [&](int lr, float acc) {
+                                   tv[lr] = acc;
+                                   a += (double)acc * (double)sv[lr];
+                                   bb += (double)acc * (double)acc;
+                                 });
+      double ts, tt;
+      sum2(a, bb, ts, tt);
+      omega = (tt > 0) ? ts / tt : 0.0;
+      const float om2 = (float)omega;
+      // d += alpha D^-1 p + omega D^-1 s ; ri = s - omega t ; (rh, ri), ||ri||
+      a = 0;
+      bb = 0;
+      wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) {
+        dv[lr] += dinv[lr] * (al * pv[lr] + om2 * sv[lr]);
+        const float q = sv[lr] - om2 * tv[lr];
+        ri[lr] = q;
+        a += (double)rh[lr] * (double)q;
+        bb += (double)q * (double)q;
+      });
+      rho = rho1;
+      double rr2;
+      sum2(a, bb, rho1, rr2);
+      if (sqrt(rr2) <= stop || !(fabs(omega) > 0)) { ++it; break; }
+    }
+    inner_total += it;
+    // x += d (FP64)
+    wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) { x[lr] += (double)dv[lr]; });
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    B.iters[p] += inner_total;
+    B.outer[p] += outer;
+    B.res[p] = (rn0 > 0) ? rn / rn0 : 0.0;
+    if (!(rn <= P.red * rn0 || rn <= P.tol)) B.fail[p] += 1;
+  }
+}
+
 // rhs_td = tau f + M u_old on the active rows of every problem (SELL slices)
 __global__ __launch_bounds__(256) void k_batch_rhs(int n_slices, const int* __restrict__ slice_prob,
                                                    const int* __restrict__ dof_off,
@@ -257,6 +450,14 @@ __global__ void k_sell_gather(size_t total, const int* __restrict__ src, const d
 }  // namespace
 
 void batch_solve(const BatchDev& B, const BatchSolveParams& P, int block, hipStream_t st) {
+  if (P.method == 1) {
+    switch (block) {
+      case 256: hipLaunchKernelGGL((k_batch_solve_bicgstab<256>), dim3(B.n_prob), dim3(256), 0, st, B, P); break;
+      case 512: hipLaunchKernelGGL((k_batch_solve_bicgstab<512>), dim3(B.n_prob), dim3(512), 0, st, B, P); break;
+      default: hipLaunchKernelGGL((k_batch_solve_bicgstab<1024>), dim3(B.n_prob), dim3(1024), 0, st, B, P); break;
+    }
+    return;
+  }
   switch (block) {
     case 256: hipLaunchKernelGGL((k_batch_solve<256>), dim3(B.n_prob), dim3(256), 0, st, B, P); break;
     case 512: hipLaunchKernelGGL((k_batch_solve<512>), dim3(B.n_prob), dim3(512), 0, st, B, P); break;

@@ -146,7 +146,9 @@ struct BatchDev {
   int* fail = nullptr;
 };
 struct BatchSolveParams {
+  int method = 0;          // inner solver: 0 GMRES(m), 1 BiCGStab
   int m = 30;              // GMRES restart (<= 64)
+  int max_inner = 1000;    // BiCGStab iterations per refinement step
   int max_outer = 50;      // FP64 refinement steps per time step
   double inner_red = 1e-4; // FP32 GMRES residual reduction per refinement step
   double red = 1e-10;      // FP64 residual reduction per time step

@@ -550,16 +550,17 @@ class BatchTCD:
     own eps, b, c) advanced together by backward Euler (pmx_batch_*).
     `problems` = list of (n_side, eps, (b1, b2), c)."""
 
-    def __init__(self, ctx, problems, example="tcd_exp", tau=0.01, t0=0.0, gmres_restart=30,
-                 max_outer=50, inner_reduction=1e-4, residual_reduction=1e-10, residual_tolerance=0.0,
-                 block=0):
+    def __init__(self, ctx, problems, example="tcd_exp", tau=0.01, t0=0.0, inner_solver="bi_cgstab",
+                 gmres_restart=30, max_inner=2000, max_outer=50, inner_reduction=1e-4,
+                 residual_reduction=1e-10, residual_tolerance=0.0, block=0):
         self.ctx = ctx
         prm = BatchParams()
         call("pmx_batch_default_params", ctypes.byref(prm))
         prm.example = EXAMPLES[example]
         prm.scheme = TIME_SCHEME["backward_euler"]
         prm.tau, prm.t0 = tau, t0
-        prm.gmres_restart, prm.max_outer = gmres_restart, max_outer
+        prm.inner_solver = {"gmres": 0, "bi_cgstab": 1}[inner_solver]
+        prm.gmres_restart, prm.max_inner, prm.max_outer = gmres_restart, max_inner, max_outer
         prm.inner_reduction, prm.residual_reduction = inner_reduction, residual_reduction
         prm.residual_tolerance, prm.block = residual_tolerance, block
         arr = (BatchProblem * len(problems))()

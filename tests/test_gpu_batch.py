@@ -40,9 +40,9 @@ def single_problem(ctx, n, eps, b, c, example="tcd_exp", steps=STEPS):
     return tcd
 
 
-@pytest.fixture(scope="module")
-def batch(ctx):
-    bt = pm.BatchTCD(ctx, PROBLEMS, tau=TAU, block=256)
+@pytest.fixture(scope="module", params=["bi_cgstab", "gmres"])
+def batch(ctx, request):
+    bt = pm.BatchTCD(ctx, PROBLEMS, tau=TAU, block=256, inner_solver=request.param)
     bt.initialize()
     rep = bt.run(STEPS)
     print("batch report", rep.seconds, rep.total_inner_iterations, rep.failed_solves, flush=True)
