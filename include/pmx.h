@@ -260,6 +260,66 @@ int pmx_time_prepare_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_
 int pmx_time_reset_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_matrix* A,
                           const pmx_matrix* M);
 
+/* ------------------------------------------ Oseen / Navier-Stokes (3D, Q2/Q1) */
+/* The saddle-point system of one mesh level with the NSTYPE-1 block layout of
+ * BlockFEMatrix::NSE3D_Type1 (src/Matrix/BlockFEMatrix.C:471-720):
+ * ( A 0 0 B1^T ; 0 A 0 B2^T ; 0 0 A B3^T ; B1 B2 B3 0 ), vectors [u1|u2|u3|p].
+ * Velocity Q2 with Dirichlet conditions on the whole boundary, pressure Q1
+ * (continuous, all DOFs active: an enclosed flow with pressure projection). */
+typedef struct pmx_nse pmx_nse;
+#define PMX_EX_NSE_U2P1 9   /* include/Examples/NSE_3D/test_u_2_p_1.h (Example_NSE3D -3) */
+#define PMX_NSE_BLOCK_A 0
+#define PMX_NSE_BLOCK_B 1   /* structure of B1, B2, B3 */
+#define PMX_NSE_BLOCK_B1 1
+#define PMX_NSE_BLOCK_B2 2
+#define PMX_NSE_BLOCK_B3 3
+#define PMX_NSE_COMBINED 4
+typedef struct {
+  int n_cells, n_u, n_u_active, n_p;
+  long long nnz_a, nnz_b;
+  int n_combined;
+  long long nnz_combined;
+} pmx_nse_info;
+typedef struct {
+  int example;   /* PMX_EX_NSE_U2P1: right-hand side f = -nu lap u + (u.grad) u + grad p, g = 0 */
+  double nu;     /* viscosity 1/Re (coeff[0]) */
+} pmx_nse_params;
+/* NavierStokes<3> spaces and matrices on one level: TFESpace3D velocity (Q2) and
+ * pressure (Q1), TStructure of A (src/Matrix/Structure.C:829-1277) and of B
+ * (rectangular, :1709-1924), the combined matrix pattern. */
+int pmx_nse_create(pmx_ctx* ctx, const pmx_domain* d, int level, pmx_nse** out);
+int pmx_nse_destroy(pmx_nse* s);
+int pmx_nse_get_info(const pmx_nse* s, pmx_nse_info* out);
+/* NavierStokes<3>::assemble_linear_terms + assemble_nonlinear_term
+ * (src/System/NavierStokes.C:407-700) with the Galerkin NSTYPE-1 forms
+ * NSLaplaceGradGradSingle, NSNonlinearTerm_convective_Single (convection field
+ * dev_w = [w1|w2|w3] in the velocity space; NULL: Stokes), NSDivergenceBlocks,
+ * NSRightHandSide (NSE_local_assembling_routines.C); Gauss4 quadrature.
+ * Resets and fills A, B1..B3 and dev_rhs (NULL allowed); Dirichlet velocity
+ * rows of A get diagonal 1 and their rhs the example's boundary values. */
+int pmx_nse_assemble(pmx_ctx* ctx, pmx_nse* s, const pmx_nse_params* p, const double* dev_w,
+                     double* dev_rhs);
+/* BlockFEMatrix::apply / apply_scaled_add (src/Matrix/BlockFEMatrix.C:855-924):
+ * B^T blocks act on the active velocity rows (FEMatrix::multiplyTransposedActive,
+ * src/Matrix/FEMatrix.C:201-218), Dirichlet velocity rows are identity rows. */
+int pmx_nse_apply(pmx_ctx* ctx, const pmx_nse* s, const double* dev_x, double* dev_y);
+int pmx_nse_apply_scaled_add(pmx_ctx* ctx, const pmx_nse* s, const double* dev_x, double* dev_y,
+                             double a);
+/* structures and values (A, B, B1..B3, combined matrix of
+ * BlockFEMatrix::get_combined_matrix, src/Matrix/BlockFEMatrix.C:1168-1345, with
+ * unit Dirichlet rows and the first pressure row replaced by a unit row;
+ * zero entries are kept) */
+int pmx_nse_get_structure(const pmx_nse* s, int which, int* nnz, int* host_row_ptr, int* host_col);
+int pmx_nse_get_values(pmx_ctx* ctx, const pmx_nse* s, int which, double* host_out);
+int pmx_nse_get_global_numbers(const pmx_nse* s, int* host_velocity, int* host_pressure);
+/* exact solution of the example at the nodal points of every DOF */
+int pmx_nse_interpolate(pmx_ctx* ctx, const pmx_nse* s, int example, double* dev_x);
+/* TFEFunction3D::project_into_L20 of the pressure block (FEFunction3D.C:1678-1725) */
+int pmx_nse_project_pressure(pmx_ctx* ctx, pmx_nse* s, double* dev_x);
+/* NavierStokes<3>::get_errors layout: [0] L2 u, [1] H1-semi u, [2] unused,
+ * [3] L2 p, [4] H1-semi p */
+int pmx_nse_errors(pmx_ctx* ctx, pmx_nse* s, int example, const double* dev_x, double* host_err5);
+
 /* ------------------------------------------- batched time-dependent solves */
 /* Many independent TimeConvectionDiffusion<2> problems (P1 on an n x n
  * unit-square triangulation each, own coefficients) advanced together by

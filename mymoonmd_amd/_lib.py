@@ -59,6 +59,16 @@ class BatchReport(ctypes.Structure):
                 ("failed_solves", c_int), ("block", c_int)]
 
 
+class NseInfo(ctypes.Structure):
+    _fields_ = [("n_cells", c_int), ("n_u", c_int), ("n_u_active", c_int), ("n_p", c_int),
+                ("nnz_a", ctypes.c_longlong), ("nnz_b", ctypes.c_longlong), ("n_combined", c_int),
+                ("nnz_combined", ctypes.c_longlong)]
+
+
+class NseParams(ctypes.Structure):
+    _fields_ = [("example", c_int), ("nu", c_double)]
+
+
 class SolverParams(ctypes.Structure):
     _fields_ = [("iterative_solver_type", c_int), ("preconditioner", c_int),
                 ("max_n_iterations", c_int), ("min_n_iterations", c_int),
@@ -140,6 +150,18 @@ SIGNATURES = {
     "pmx_batch_get_structure": [c_void_p, c_int, P(c_int), P(c_int), P(c_int)],
     "pmx_batch_get_matrices": [c_void_p, c_int, P(c_double), P(c_double), P(c_double)],
     "pmx_batch_errors": [c_void_p, c_int, P(c_double)],
+    "pmx_nse_create": [c_void_p, c_void_p, c_int, P(c_void_p)],
+    "pmx_nse_destroy": [c_void_p],
+    "pmx_nse_get_info": [c_void_p, P(NseInfo)],
+    "pmx_nse_assemble": [c_void_p, c_void_p, P(NseParams), c_void_p, c_void_p],
+    "pmx_nse_apply": [c_void_p, c_void_p, c_void_p, c_void_p],
+    "pmx_nse_apply_scaled_add": [c_void_p, c_void_p, c_void_p, c_void_p, c_double],
+    "pmx_nse_get_structure": [c_void_p, c_int, P(c_int), P(c_int), P(c_int)],
+    "pmx_nse_get_values": [c_void_p, c_void_p, c_int, P(c_double)],
+    "pmx_nse_get_global_numbers": [c_void_p, P(c_int), P(c_int)],
+    "pmx_nse_interpolate": [c_void_p, c_void_p, c_int, c_void_p],
+    "pmx_nse_project_pressure": [c_void_p, c_void_p, c_void_p],
+    "pmx_nse_errors": [c_void_p, c_void_p, c_int, c_void_p, P(c_double)],
     "pmx_solver_default_params": [P(SolverParams)],
     "pmx_solver_create": [c_void_p, P(SolverParams), P(c_void_p)],
     "pmx_solver_destroy": [c_void_p],

@@ -22,9 +22,11 @@ SOURCES = [
     "capi.cpp",
     "solver.cpp",
     "batch.cpp",
+    "nse.cpp",
     "hip/kernels_la.hip",
     "hip/kernels_asm.hip",
     "hip/kernels_batch.hip",
+    "hip/kernels_nse.hip",
 ]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")

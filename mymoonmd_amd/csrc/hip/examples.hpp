@@ -13,6 +13,9 @@
 //                       (example -1)
 // Time-dependent examples take the right-hand side f = du/dt - eps*lap(u) +
 // b.grad(u) + c*u at the given time.
+//   EX_NSE_U2P1_*       include/Examples/NSE_3D/test_u_2_p_1.h (NSE3D example -3):
+//                       u = (y^2, xz, -y^2), p = x + y + z - 1.5; one id per
+//                       component (U1, U2, U3, P) for values and errors
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,6 +33,10 @@ enum Example : int {
   EX_SINE_2D = 6,
   EX_TCD_EXP = 7,
   EX_TCD_LINEAR = 8,
+  EX_NSE_U2P1_U1 = 9,
+  EX_NSE_U2P1_U2 = 10,
+  EX_NSE_U2P1_U3 = 11,
+  EX_NSE_U2P1_P = 12,
 };
 
 constexpr double kPi = 3.14159265358979323846;
@@ -51,6 +58,10 @@ __host__ __device__ inline void example_exact(int ex, double x, double y, double
     case EX_TCD_LINEAR:
       v[0] = 1 + 2 * x + 3 * t * y; v[1] = 2; v[2] = 3 * t; v[5] = 3 * y;
       break;
+    case EX_NSE_U2P1_U1: v[0] = y * y; v[2] = 2 * y; v[4] = 2; break;
+    case EX_NSE_U2P1_U2: v[0] = x * z; v[1] = z; v[3] = x; break;
+    case EX_NSE_U2P1_U3: v[0] = -y * y; v[2] = -2 * y; v[4] = -2; break;
+    case EX_NSE_U2P1_P: v[0] = x + y + z - 1.5; v[1] = 1; v[2] = 1; v[3] = 1; break;
     case EX_SINE_LAPLACE_2D:
     case EX_SINE_2D: {
       const double sx = sin(kPi * x), sy = sin(kPi * y);
@@ -94,6 +105,19 @@ __host__ __device__ inline double example_dirichlet(int ex, double x, double y, 
   double v[6];
   example_exact(ex, x, y, z, v, t);
   return v[0];
+}
+
+// Navier-Stokes examples: f = -nu lap(u) + (u.grad) u + grad p, g = div u = 0
+// (LinCoeffs of test_u_2_p_1.h); out = f1, f2, f3, g.  `ex` is the first
+// component id (EX_NSE_U2P1_U1).
+__host__ __device__ inline void nse_example_rhs(int ex, double x, double y, double z, double nu,
+                                                double* out) {
+  double u[3][6], p[6];
+  for (int k = 0; k < 3; ++k) example_exact(ex + k, x, y, z, u[k]);
+  example_exact(ex + 3, x, y, z, p);
+  for (int k = 0; k < 3; ++k)
+    out[k] = -nu * u[k][4] + (u[0][0] * u[k][1] + u[1][0] * u[k][2] + u[2][0] * u[k][3]) + p[1 + k];
+  out[3] = 0.0;
 }
 
 // SUPG parameter of Compute_SDFEM_delta with SDFEM_TYPE 2

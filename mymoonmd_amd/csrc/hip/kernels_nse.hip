@@ -1,0 +1,250 @@
+// Oseen / Navier-Stokes saddle-point kernels (gfx950): Q2/Q1 assembly of the
+// NSTYPE-1 blocks, the combined matrix, the pressure mean, and the batched
+// cell-Vanka smoother.
+//
+// Assembly restates, per hexahedron and quadrature point, the routines that
+// LocalAssembling<3>::set_parameters_for_nse selects for a Galerkin NSTYPE 1
+// problem with the convective form (src/AssembleRoutines/LocalAssembling.C:
+// 931-1185, src/AssembleRoutinesSaddle/NSE_local_assembling_routines.C):
+//   NSLaplaceGradGradSingle            A_ij += w|J| nu grad phi_j . grad phi_i
+//   NSNonlinearTerm_convective_Single  A_ij += w|J| (w.grad phi_j) phi_i,
+//                                      w = the given velocity field at the point
+//   NSDivergenceBlocks (sign 1)        Bk_ij -= w|J| psi_i d_k phi_j
+//   NSRightHandSide (sign -1)          f_k,i += w|J| f_k phi_i;  g_i += w|J| psi_i g
+// with the scatter of Assemble3D (Dirichlet velocity rows of A get diagonal 1;
+// B holds every velocity column).  Lanes: 32 per cell, lane i < 27 owns
+// velocity test function i (its A row, its three load entries and the column
+// i of B1..B3 for the 8 pressure rows); cells of one colour share no vertex, so
+// neither velocity nor pressure rows are written twice.
+#include "common.hpp"
+#include "examples.hpp"
+#include "launch.hpp"
+
+namespace pmx {
+
+namespace {
+
+__device__ __forceinline__ int find_col_n(const int* __restrict__ ci, int s, int e, int c) {
+  int lo = s, hi = e - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (ci[mid] < c) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+constexpr int NU = 27, NP = 8, SLOT = 32, CPB = 256 / SLOT;
+// reference table per quadrature point: w, xi[3], phi[27], dphi[3][27], psi[8]
+constexpr int RS = 1 + 3 + NU + 3 * NU + NP;
+
+__global__ __launch_bounds__(256) void k_assemble_nse(NseAsm d, const int* __restrict__ cells, int n) {
+  __shared__ double s_u[CPB][NU];
+  __shared__ double s_g[CPB][3][NU];
+  __shared__ double s_w[CPB][3][NU];
+  __shared__ double s_c[CPB][8];
+  const int slot = threadIdx.x / SLOT;
+  const int i = threadIdx.x % SLOT;
+  const int ci = blockIdx.x * CPB + slot;
+  const bool valid = ci < n;
+  const int cell = valid ? cells[ci] : 0;
+  const double* geo = d.geo + (size_t)cell * kGeoStride;
+  const int* vd = d.vdofs + (size_t)cell * NU;
+  const int* pd = d.pdofs + (size_t)cell * NP;
+  double Jinv[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Jinv[k] = geo[12 + k];
+  if (valid && i < NU) {
+    for (int k = 0; k < 3; ++k) s_w[slot][k][i] = d.w ? d.w[(size_t)k * d.n_u + vd[i]] : 0.0;
+  }
+  double arow[NU];
+#pragma unroll
+  for (int j = 0; j < NU; ++j) arow[j] = 0.0;
+  double bcol[3][NP];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int q = 0; q < NP; ++q) bcol[k][q] = 0.0;
+  double frow[3] = {0, 0, 0}, grow = 0;
+  __syncthreads();
+  for (int q = 0; q < d.nq; ++q) {
+    const double* rq = d.ref + (size_t)q * RS;
+    if (valid && i < NU) {
+      const double* dphi = rq + 4 + NU;
+      s_u[slot][i] = rq[4 + i];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s += dphi[k * NU + i] * Jinv[k * 3 + a];
+        s_g[slot][a][i] = s;
+      }
+    }
+    __syncthreads();
+    if (valid && i == 0) {
+      double x[3];
+      for (int r = 0; r < 3; ++r) {
+        double s = geo[r];
+        for (int c = 0; c < 3; ++c) s += geo[3 + r * 3 + c] * rq[1 + c];
+        x[r] = s;
+      }
+      double f[4];
+      nse_example_rhs(d.example, x[0], x[1], x[2], d.nu, f);
+      double* sc = s_c[slot];
+      sc[0] = f[0]; sc[1] = f[1]; sc[2] = f[2]; sc[3] = f[3];
+      sc[4] = rq[0] * fabs(geo[21]);
+      for (int k = 0; k < 3; ++k) {
+        double s = 0;
+        for (int j = 0; j < NU; ++j) s += s_w[slot][k][j] * s_u[slot][j];
+        sc[5 + k] = s;
+      }
+    }
+    __syncthreads();
+    if (valid && i < NU) {
+      const double* sc = s_c[slot];
+      const double mult = sc[4], w1 = sc[5], w2 = sc[6], w3 = sc[7];
+      const double ti = s_u[slot][i];
+      const double tx = s_g[slot][0][i], ty = s_g[slot][1][i], tz = s_g[slot][2][i];
+      const double nu = d.nu;
+#pragma unroll
+      for (int j = 0; j < NU; ++j) {
+        const double ax = s_g[slot][0][j], ay = s_g[slot][1][j], az = s_g[slot][2][j];
+        arow[j] += mult * nu * (tx * ax + ty * ay + tz * az);
+        if (d.convection) arow[j] += mult * (w1 * ax + w2 * ay + w3 * az) * ti;
+      }
+      frow[0] += mult * ti * sc[0];
+      frow[1] += mult * ti * sc[1];
+      frow[2] += mult * ti * sc[2];
+      const double* psi = rq + 4 + NU + 3 * NU;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        bcol[0][p] -= mult * psi[p] * tx;
+        bcol[1][p] -= mult * psi[p] * ty;
+        bcol[2][p] -= mult * psi[p] * tz;
+      }
+      if (i < NP) grow += mult * psi[i] * sc[3];
+    }
+    __syncthreads();
+  }
+  if (!valid || i >= NU) return;
+  const int r = vd[i];
+  if (r < d.n_u_active) {
+    const int rs = d.A.row_ptr[r], re = d.A.row_ptr[r + 1];
+#pragma unroll
+    for (int j = 0; j < NU; ++j) d.A.val[find_col_n(d.A.col, rs, re, vd[j])] += arow[j];
+    if (d.rhs)
+      for (int k = 0; k < 3; ++k) d.rhs[(size_t)k * d.n_u + r] += frow[k];
+  } else {
+    d.A.val[find_col_n(d.A.col, d.A.row_ptr[r], d.A.row_ptr[r + 1], r)] = 1.0;
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int pr = pd[p];
+    const int pos = find_col_n(d.Bcol, d.Brp[pr], d.Brp[pr + 1], r);
+    d.B1[pos] += bcol[0][p];
+    d.B2[pos] += bcol[1][p];
+    d.B3[pos] += bcol[2][p];
+  }
+  if (d.rhs && i < NP) d.rhs[(size_t)3 * d.n_u + pd[i]] += grow;
+}
+
+// velocity Dirichlet rows of the rhs: the example's boundary values
+__global__ void k_nse_dirichlet(int ex, int n_u, int active, const double* __restrict__ xyz,
+                                double* __restrict__ rhs) {
+  const int k = active + blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n_u) {
+    for (int c = 0; c < 3; ++c) {
+      double v[6];
+      example_exact(ex + c, xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], v);
+      rhs[(size_t)c * n_u + k] = v[0];
+    }
+  }
+}
+
+// combined matrix values: kind 0 -> 0, 1 -> 1, 2 -> A[pos], 3..5 -> B1..B3[pos]
+__global__ void k_combined_values(long long nnz, const uint8_t* __restrict__ kind, const int* __restrict__ pos,
+                                  const double* __restrict__ A, const double* __restrict__ B1,
+                                  const double* __restrict__ B2, const double* __restrict__ B3,
+                                  double* __restrict__ out) {
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+       k += (long long)gridDim.x * blockDim.x) {
+    const int t = kind[k], p = pos[k];
+    double v = 0.0;
+    if (t == 1) v = 1.0;
+    else if (t == 2) v = A[p];
+    else if (t == 3) v = B1[p];
+    else if (t == 4) v = B2[p];
+    else if (t == 5) v = B3[p];
+    out[k] = v;
+  }
+}
+
+// integral and measure of a Q1 pressure over the cells (per-block partials)
+__global__ __launch_bounds__(256) void k_pressure_integral(int n_cells, int nq, const double* __restrict__ ref,
+                                                           const double* __restrict__ geo,
+                                                           const int* __restrict__ pdofs,
+                                                           const double* __restrict__ p,
+                                                           double* __restrict__ partial) {
+  __shared__ double red[16];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double I = 0, m = 0;
+  if (c < n_cells) {
+    const double det = fabs(geo[(size_t)c * kGeoStride + 21]);
+    double pl[NP];
+    for (int j = 0; j < NP; ++j) pl[j] = p[pdofs[(size_t)c * NP + j]];
+    for (int q = 0; q < nq; ++q) {
+      const double* rq = ref + (size_t)q * RS;
+      const double* psi = rq + 4 + NU + 3 * NU;
+      double v = 0;
+      for (int j = 0; j < NP; ++j) v += pl[j] * psi[j];
+      I += rq[0] * det * v;
+      m += rq[0] * det;
+    }
+  }
+  const double si = block_sum(I, red);
+  const double sm = block_sum(m, red);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = si;
+    partial[2 * blockIdx.x + 1] = sm;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_pressure_shift(int nb, const double* __restrict__ partial, int n_p,
+                                                         double* __restrict__ p) {
+  __shared__ double red[16];
+  __shared__ double mean;
+  double si = 0, sm = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) { si += partial[2 * b]; sm += partial[2 * b + 1]; }
+  const double ti = block_sum(si, red);
+  const double tm = block_sum(sm, red);
+  if (threadIdx.x == 0) mean = (tm > 0) ? ti / tm : 0.0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < n_p; k += blockDim.x) p[k] -= mean;
+}
+
+}  // namespace
+
+void assemble_nse(const NseAsm& d, const int* cells, int n, hipStream_t st) {
+  const int grid = (n + CPB - 1) / CPB;
+  if (grid > 0) hipLaunchKernelGGL(k_assemble_nse, dim3(grid), dim3(256), 0, st, d, cells, n);
+}
+
+void nse_dirichlet_rhs(int example, int n_u, int active, const double* xyz, double* rhs, hipStream_t st) {
+  const int nd = n_u - active;
+  if (nd > 0)
+    hipLaunchKernelGGL(k_nse_dirichlet, dim3((nd + 255) / 256), dim3(256), 0, st, example, n_u, active, xyz, rhs);
+}
+
+void combined_values(long long nnz, const uint8_t* kind, const int* pos, const double* A, const double* B1,
+                     const double* B2, const double* B3, double* out, hipStream_t st) {
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_combined_values, dim3(4096), dim3(256), 0, st, nnz, kind, pos, A, B1, B2, B3, out);
+}
+
+void pressure_project_l20(int n_cells, int nq, const double* ref, const double* geo, const int* pdofs, int n_p,
+                          double* p, double* partial, hipStream_t st) {
+  const int nb = (n_cells + 255) / 256;
+  hipLaunchKernelGGL(k_pressure_integral, dim3(nb), dim3(256), 0, st, n_cells, nq, ref, geo, pdofs, p, partial);
+  hipLaunchKernelGGL(k_pressure_shift, dim3(1), dim3(1024), 0, st, nb, partial, n_p, p);
+}
+
+}  // namespace pmx

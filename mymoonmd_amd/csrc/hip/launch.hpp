@@ -121,6 +121,30 @@ struct CDCoeffs {
   const double* time_dev = nullptr;       // time read from device memory (graph replays)
 };
 
+// ---- Oseen / Navier-Stokes (kernels_nse.hip)
+struct NseAsm {
+  int nq = 0, n_u = 0, n_u_active = 0, example = 0, convection = 0;
+  double nu = 1;
+  const double* ref = nullptr;   // per q: w, xi[3], phi[27], dphi[3][27], psi[8]
+  const double* geo = nullptr;   // per cell (kGeoStride)
+  const int* vdofs = nullptr;    // [cells][27]
+  const int* pdofs = nullptr;    // [cells][8]
+  const double* w = nullptr;     // convection field [3][n_u] (null: Stokes)
+  DevCSR A{};                    // velocity block (NSTYPE 1: shared by the 3 components)
+  const int* Brp = nullptr;
+  const int* Bcol = nullptr;
+  double* B1 = nullptr;
+  double* B2 = nullptr;
+  double* B3 = nullptr;
+  double* rhs = nullptr;         // [3 n_u + n_p] or null
+};
+void assemble_nse(const NseAsm& d, const int* cells, int n, hipStream_t st);
+void nse_dirichlet_rhs(int example, int n_u, int active, const double* xyz, double* rhs, hipStream_t st);
+void combined_values(long long nnz, const uint8_t* kind, const int* pos, const double* A, const double* B1,
+                     const double* B2, const double* B3, double* out, hipStream_t st);
+void pressure_project_l20(int n_cells, int nq, const double* ref, const double* geo, const int* pdofs, int n_p,
+                          double* p, double* partial, hipStream_t st);
+
 // ---- batched time-dependent solve (kernels_batch.hip)
 struct BatchDev {
   int n_prob = 0;

@@ -143,6 +143,40 @@ FESpace build_space(const Mesh& m, int level, int order, const BoundaryCondition
   return s;
 }
 
+CSRPattern build_rect_structure(const FESpace& t, const FESpace& a) {
+  if (t.n_cells != a.n_cells) throw std::runtime_error("spaces are not on the same cells");
+  const int nt = t.ndof_cell, na = a.ndof_cell;
+  CSRPattern p;
+  p.n_rows = t.n_dof;
+  p.n_cols = a.n_dof;
+  p.active_bound = t.active_bound;
+  std::vector<int> inc_ptr(t.n_dof + 1, 0);
+  for (size_t i = 0; i < t.global_numbers.size(); ++i) inc_ptr[t.global_numbers[i] + 1]++;
+  for (int r = 0; r < t.n_dof; ++r) inc_ptr[r + 1] += inc_ptr[r];
+  std::vector<int> inc(inc_ptr[t.n_dof]);
+  {
+    std::vector<int> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+    for (int c = 0; c < t.n_cells; ++c)
+      for (int j = 0; j < nt; ++j) inc[fill[t.global_numbers[(size_t)c * nt + j]]++] = c;
+  }
+  std::vector<std::vector<int>> rows(t.n_dof);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int r = 0; r < t.n_dof; ++r) {
+    std::vector<int>& cols = rows[r];
+    for (int q = inc_ptr[r]; q < inc_ptr[r + 1]; ++q) {
+      const int* dof = &a.global_numbers[(size_t)inc[q] * na];
+      cols.insert(cols.end(), dof, dof + na);
+    }
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+  }
+  p.row_ptr.assign(t.n_dof + 1, 0);
+  for (int r = 0; r < t.n_dof; ++r) p.row_ptr[r + 1] = p.row_ptr[r] + (int)rows[r].size();
+  p.col.resize(p.row_ptr[t.n_dof]);
+  for (int r = 0; r < t.n_dof; ++r) std::copy(rows[r].begin(), rows[r].end(), p.col.begin() + p.row_ptr[r]);
+  return p;
+}
+
 CSRPattern build_structure(const FESpace& s) {
   const int nd = s.ndof_cell;
   CSRPattern p;

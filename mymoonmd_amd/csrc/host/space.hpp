@@ -68,6 +68,12 @@ struct CSRPattern {
 // Square FE matrix pattern of a space.
 CSRPattern build_structure(const FESpace& s);
 
+// Rectangular pattern TStructure(testspace, ansatzspace)
+// (src/Matrix/Structure.C:1709-1924): every row of a test DOF holds the
+// ansatz DOFs of all cells containing it, ascending (both spaces on the same
+// cells); active_bound is the test space's.
+CSRPattern build_rect_structure(const FESpace& test, const FESpace& ansatz);
+
 // Interpolation matrix from the coarse space (mesh level l) to the fine space
 // (level l+1): P[f][c] = phi_c(x_f).  Prolongation of GridTransfer::Prolongate
 // (src/Multigrid/GridTransfer.C:665-941) equals P*u (the reference averages

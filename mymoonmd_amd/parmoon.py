@@ -14,7 +14,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (BC2D_FN, BC3D_FN, BatchInfo, BatchParams, BatchProblem, BatchReport, CDParams,
-                   SolveReport, SolverParams, SpaceInfo, call, c_double, c_int, c_void_p)
+                   NseInfo, NseParams, SolveReport, SolverParams, SpaceInfo, call, c_double, c_int,
+                   c_void_p)
 
 GEOMETRY = {"UnitSquare": 0, "TwoTriangles": 1, "Default_UnitCube_Hexa": 2}
 EXAMPLES = {"const": 0, "sine_laplace_2d": 1, "p0_3d": 2, "p1_3d": 3, "p2_3d": 4,
@@ -633,4 +634,99 @@ class BatchTCD:
     def __del__(self, _d=_destroy):
         if getattr(self, "h", None) and self.h.value:
             _d("pmx_batch_destroy", self.h)
+            self.h = None
+
+
+NSE_EXAMPLES = {"u2_p1": 9}
+NSE_BLOCK = {"A": 0, "B": 1, "B1": 1, "B2": 2, "B3": 3, "combined": 4}
+
+
+class NSESystem:
+    """One level of NavierStokes<3> with Q2/Q1 on hexahedra and the NSTYPE-1
+    BlockFEMatrix (pmx_nse_*): assemble, apply, structures and values,
+    interpolation of the example, errors, pressure projection."""
+
+    def __init__(self, ctx, domain, level):
+        self.ctx = ctx
+        self.domain = domain
+        self.h = c_void_p()
+        call("pmx_nse_create", ctx.h, domain.h, level, ctypes.byref(self.h))
+        inf = NseInfo()
+        call("pmx_nse_get_info", self.h, ctypes.byref(inf))
+        self.info = inf
+        self.n_u, self.n_u_active, self.n_p = inf.n_u, inf.n_u_active, inf.n_p
+        self.n = inf.n_combined
+        self.n_cells = inf.n_cells
+
+    def zeros(self):
+        import torch
+        return torch.zeros(self.n, dtype=torch.float64, device=torch.device("cuda", self.ctx.device))
+
+    def assemble(self, example="u2_p1", nu=1.0, w=None, rhs=None):
+        import torch
+        torch.cuda.synchronize()
+        p = NseParams(NSE_EXAMPLES[example], nu)
+        call("pmx_nse_assemble", self.ctx.h, self.h, ctypes.byref(p), _ptr(w) if w is not None else None,
+             _ptr(rhs) if rhs is not None else None)
+        self.ctx.synchronize()
+
+    def apply(self, x, y):
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_nse_apply", self.ctx.h, self.h, _ptr(x), _ptr(y))
+        self.ctx.synchronize()
+
+    def apply_scaled_add(self, x, y, a=1.0):
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_nse_apply_scaled_add", self.ctx.h, self.h, _ptr(x), _ptr(y), a)
+        self.ctx.synchronize()
+
+    def structure(self, which):
+        nnz = c_int()
+        call("pmx_nse_get_structure", self.h, NSE_BLOCK[which], ctypes.byref(nnz), None, None)
+        n_rows = {"A": self.n_u, "B": self.n_p, "combined": self.n}[which]
+        rp = np.zeros(n_rows + 1, np.int32)
+        col = np.zeros(nnz.value, np.int32)
+        P = ctypes.POINTER(c_int)
+        call("pmx_nse_get_structure", self.h, NSE_BLOCK[which], ctypes.byref(nnz), rp.ctypes.data_as(P),
+             col.ctypes.data_as(P))
+        return rp, col
+
+    def values(self, which):
+        base = "B" if which in ("B1", "B2", "B3") else which
+        rp, col = self.structure(base)
+        out = np.zeros(len(col))
+        call("pmx_nse_get_values", self.ctx.h, self.h, NSE_BLOCK[which], out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out
+
+    def global_numbers(self):
+        gv = np.zeros(self.n_cells * 27, np.int32)
+        gp = np.zeros(self.n_cells * 8, np.int32)
+        P = ctypes.POINTER(c_int)
+        call("pmx_nse_get_global_numbers", self.h, gv.ctypes.data_as(P), gp.ctypes.data_as(P))
+        return gv.reshape(-1, 27), gp.reshape(-1, 8)
+
+    def interpolate(self, x, example="u2_p1"):
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_nse_interpolate", self.ctx.h, self.h, NSE_EXAMPLES[example], _ptr(x))
+        self.ctx.synchronize()
+
+    def project_pressure(self, x):
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_nse_project_pressure", self.ctx.h, self.h, _ptr(x))
+        self.ctx.synchronize()
+
+    def errors(self, x, example="u2_p1"):
+        import torch
+        torch.cuda.synchronize()
+        out = (c_double * 5)()
+        call("pmx_nse_errors", self.ctx.h, self.h, NSE_EXAMPLES[example], _ptr(x), out)
+        return list(out)
+
+    def __del__(self, _d=_destroy):
+        if getattr(self, "h", None) and self.h.value:
+            _d("pmx_nse_destroy", self.h)
             self.h = None

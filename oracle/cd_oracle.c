@@ -745,6 +745,103 @@ void or_multiply_active(int active, const int* rp, const int* col, const double*
   }
 }
 
+/* test_u_2_p_1.h (Example_NSE3D -3): u = (y^2, xz, -y^2), p = x + y + z - 1.5;
+ * ue[k] = u_k, grad u_k (1..3), lap u_k (4); pe = p, grad p */
+static void nse_exact(double x, double y, double z, double ue[3][5], double pe[4]) {
+  for (int k = 0; k < 3; ++k) for (int a = 0; a < 5; ++a) ue[k][a] = 0;
+  ue[0][0] = y * y; ue[0][2] = 2 * y; ue[0][4] = 2;
+  ue[1][0] = x * z; ue[1][1] = z; ue[1][3] = x;
+  ue[2][0] = -y * y; ue[2][2] = -2 * y; ue[2][4] = -2;
+  pe[0] = x + y + z - 1.5; pe[1] = 1; pe[2] = 1; pe[3] = 1;
+}
+
+static void add_entry(const int* rp, const int* col, int row, int c, double* val, double v) {
+  for (int n = rp[row]; n < rp[row + 1]; ++n)
+    if (col[n] == c) { val[n] += v; return; }
+}
+
+/* Assemble3D with LocalAssembling<3> NavierStokesAll + NavierStokesNL for NSTYPE 1
+ * (LocalAssembling.C:931-1185): per quadrature point NSLaplaceGradGradSingle,
+ * NSNonlinearTerm_convective_Single, NSDivergenceBlocks (sign 1),
+ * NSRightHandSide (sign -1) (NSE_local_assembling_routines.C:52-283); the
+ * coefficient function is LinCoeffs of test_u_2_p_1.h. */
+void or_assemble_nse(const or_space* vs, const or_space* ps, double nu, const double* w, double* A,
+                     const int* brp, const int* bcol, double* B1, double* B2, double* B3, double* rhs) {
+  const int nu_d = vs->n_dof;
+  double qxi[64 * 3], qw[64];
+  const int nq = or_quad_rule(OR_HEXA, 2, 0, qxi, qw);
+  double u[27], gr[81], hr[162], ux[81], pv[8], pg[24];
+  double Al[27 * 27], Bl[3][8 * 27], Fl[3][27], Gl[8];
+  for (int cell = 0; cell < vs->n_cells; ++cell) {
+    const double* v = vs->verts + (size_t)cell * 8 * 3;
+    const int* vd = vs->gnum + (size_t)cell * 27;
+    const int* pd = ps->gnum + (size_t)cell * 8;
+    memset(Al, 0, sizeof Al); memset(Bl, 0, sizeof Bl); memset(Fl, 0, sizeof Fl); memset(Gl, 0, sizeof Gl);
+    for (int q = 0; q < nq; ++q) {
+      double x[3];
+      const double det = phys_basis(OR_HEXA, 2, v, qxi + 3 * q, u, gr, hr, ux, NULL, x);
+      or_basis(OR_HEXA, 1, qxi + 3 * q, pv, pg, NULL);
+      const double Mult = qw[q] * det;
+      double ue[3][5], pe[4], f[3];
+      nse_exact(x[0], x[1], x[2], ue, pe);
+      for (int k = 0; k < 3; ++k)
+        f[k] = -nu * ue[k][4] + (ue[0][0] * ue[k][1] + ue[1][0] * ue[k][2] + ue[2][0] * ue[k][3]) + pe[1 + k];
+      const double g = 0.0;
+      double wq[3] = {0, 0, 0};
+      if (w)
+        for (int k = 0; k < 3; ++k)
+          for (int j = 0; j < 27; ++j) wq[k] += w[(size_t)k * nu_d + vd[j]] * u[j];
+      /* NSLaplaceGradGradSingle */
+      for (int i = 0; i < 27; ++i)
+        for (int j = 0; j < 27; ++j)
+          Al[i * 27 + j] += Mult * nu * (ux[i] * ux[j] + ux[27 + i] * ux[27 + j] + ux[54 + i] * ux[54 + j]);
+      /* NSNonlinearTerm_convective_Single */
+      if (w)
+        for (int i = 0; i < 27; ++i)
+          for (int j = 0; j < 27; ++j)
+            Al[i * 27 + j] += Mult * (wq[0] * ux[j] + wq[1] * ux[27 + j] + wq[2] * ux[54 + j]) * u[i];
+      /* NSDivergenceBlocks, sign 1 */
+      for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 27; ++j) {
+          Bl[0][i * 27 + j] -= 1 * Mult * pv[i] * ux[j];
+          Bl[1][i * 27 + j] -= 1 * Mult * pv[i] * ux[27 + j];
+          Bl[2][i * 27 + j] -= Mult * 1 * pv[i] * ux[54 + j];
+        }
+      /* NSRightHandSide, sign -1 */
+      for (int i = 0; i < 27; ++i)
+        for (int k = 0; k < 3; ++k) Fl[k][i] += Mult * u[i] * f[k];
+      for (int i = 0; i < 8; ++i) Gl[i] -= Mult * (-1) * pv[i] * g;
+    }
+    for (int m = 0; m < 27; ++m) {
+      const int r = vd[m];
+      if (r < vs->active) {
+        for (int k = 0; k < 27; ++k) add_entry(vs->rp, vs->col, r, vd[k], A, Al[m * 27 + k]);
+        for (int c = 0; c < 3; ++c) rhs[(size_t)c * nu_d + r] += Fl[c][m];
+      } else {
+        for (int n = vs->rp[r]; n < vs->rp[r + 1]; ++n)
+          if (vs->col[n] == r) { A[n] = 1.0; break; }
+      }
+    }
+    for (int i = 0; i < 8; ++i) {
+      for (int j = 0; j < 27; ++j) {
+        add_entry(brp, bcol, pd[i], vd[j], B1, Bl[0][i * 27 + j]);
+        add_entry(brp, bcol, pd[i], vd[j], B2, Bl[1][i * 27 + j]);
+        add_entry(brp, bcol, pd[i], vd[j], B3, Bl[2][i * 27 + j]);
+      }
+      rhs[(size_t)3 * nu_d + pd[i]] += Gl[i];
+    }
+    /* Dirichlet velocity rhs: boundary values at the nodal points */
+    for (int k = 0; k < 27; ++k) {
+      if (vd[k] < vs->active) continue;
+      double xi[3], xx[3] = {0, 0, 0}, J[9], ue[3][5], pe[4];
+      node_ref(OR_HEXA, 2, k, xi);
+      ref_map(OR_HEXA, v, xi, xx, J);
+      nse_exact(xx[0], xx[1], xx[2], ue, pe);
+      for (int c = 0; c < 3; ++c) rhs[(size_t)c * nu_d + vd[k]] = ue[c][0];
+    }
+  }
+}
+
 /* ------------------------------------------------------ linear algebra */
 void or_apply_scaled_add(int n, int active, const int* rp, const int* col, const double* val,
                          const double* x, double* y, double a) {

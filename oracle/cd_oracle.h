@@ -70,6 +70,12 @@ void or_multiply_active(int active, const int* rp, const int* col, const double*
                         const double* x, double* y, double a);
 /* interpolation of the example's exact solution at time p->time at the nodal points */
 void or_interpolate(const or_space* s, const or_cd* p, double* u);
+/* Navier-Stokes NSTYPE 1 on hexahedra (velocity Q2 space vs, pressure Q1
+ * space ps on the same cells): A (pattern of vs), B1..B3 (pattern brp/bcol:
+ * pressure rows, velocity columns), rhs [3 n_u + n_p]; w = convection field
+ * [3][n_u] or NULL; example test_u_2_p_1 with viscosity nu */
+void or_assemble_nse(const or_space* vs, const or_space* ps, double nu, const double* w, double* A,
+                     const int* brp, const int* bcol, double* B1, double* B2, double* B3, double* rhs);
 void or_apply_scaled_add(int n, int active, const int* rp, const int* col, const double* val,
                          const double* x, double* y, double a);
 void or_prolongate(const or_space* cs, const or_space* fs, const int* parent, const double* cu, double* fu);

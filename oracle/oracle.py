@@ -407,6 +407,9 @@ def lib():
         L.or_interpolate.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p]
         L.or_multiply_active.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
+        L.or_assemble_nse.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(Space), ctypes.c_double,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.or_apply_scaled_add.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
         L.or_prolongate.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(Space), ctypes.c_void_p,
@@ -673,3 +676,112 @@ def csr_general(n, active, rp, col):
     """OracleSpace-like holder for a plain CSR matrix (no mesh)."""
     return OracleSpace(HEXA, 1, np.zeros((1, 8, 3)), np.zeros((1, 8), np.int32), n, active, rp, col,
                        np.full((1, 6), -1, np.int32))
+
+
+# --------------------------------------------------- Navier-Stokes (NSTYPE 1)
+def rect_structure(gtest, gansatz, n_test):
+    """TStructure(testspace, ansatzspace) (src/Matrix/Structure.C:1709-1924):
+    row t = sorted union of the ansatz DOFs of every cell containing t."""
+    rows = [set() for _ in range(n_test)]
+    for ct, ca in zip(gtest, gansatz):
+        for t in ct:
+            rows[int(t)].update(int(a) for a in ca)
+    rp, col = [0], []
+    for r in range(n_test):
+        col.extend(sorted(rows[r]))
+        rp.append(len(col))
+    return np.array(rp, np.int32), np.array(col, np.int32)
+
+
+class NseOracle:
+    """Velocity Q2 (Dirichlet everywhere) and pressure Q1 (Neumann) spaces of
+    one hexahedral level with the patterns of A and B."""
+
+    def __init__(self, n_refine, level):
+        shape, levels = mesh(2, n_refine)
+        v = levels[level]["verts"]
+        gv, iv, bdv = number_dofs(shape, v, 2, None, lambda x, y, z: 0)
+        gp, ip, bdp = number_dofs(shape, v, 1, None, lambda x, y, z: 1)
+        rp, col = structure(gv, iv["n_dof"], iv["active_bound"])
+        self.vs = OracleSpace(shape, 2, v, gv, iv["n_dof"], iv["active_bound"], rp, col, bdv,
+                              levels[level]["parent"])
+        prp, pcol = structure(gp, ip["n_dof"], ip["n_dof"])
+        self.ps = OracleSpace(shape, 1, v, gp, ip["n_dof"], ip["n_dof"], prp, pcol, bdp, levels[level]["parent"])
+        self.brp, self.bcol = rect_structure(gp, gv, ip["n_dof"])
+        self.n_u, self.n_u_active, self.n_p = iv["n_dof"], iv["active_bound"], ip["n_dof"]
+        self.n = 3 * self.n_u + self.n_p
+
+    def assemble(self, nu, w=None):
+        """NSTYPE-1 blocks A, (B1, B2, B3) and rhs of test_u_2_p_1 (or_assemble_nse)."""
+        A = np.zeros(self.vs.nnz)
+        B = [np.zeros(len(self.bcol)) for _ in range(3)]
+        rhs = np.zeros(self.n)
+        vs, ps = self.vs.cstruct(), self.ps.cstruct()
+        wv = None if w is None else _p(np.ascontiguousarray(w, np.float64))
+        keep = w
+        lib().or_assemble_nse(ctypes.byref(vs), ctypes.byref(ps), nu, wv, _p(A), _p(self.brp), _p(self.bcol),
+                              _p(B[0]), _p(B[1]), _p(B[2]), _p(rhs))
+        del keep
+        return A, B, rhs
+
+    def apply(self, A, B, x):
+        """BlockFEMatrix::apply (src/Matrix/BlockFEMatrix.C:855-924) for NSTYPE 1:
+        multiplyActive of A, multiplyTransposedActive of Bk, identity on
+        Dirichlet velocity rows."""
+        import scipy.sparse as sp
+        nu, na, npr = self.n_u, self.n_u_active, self.n_p
+        Am = sp.csr_matrix((A, self.vs.col, self.vs.rp), shape=(nu, nu))
+        Bm = [sp.csr_matrix((b, self.bcol, self.brp), shape=(npr, nu)) for b in B]
+        y = np.zeros(self.n)
+        xp = x[3 * nu:]
+        for k in range(3):
+            xk = x[k * nu:(k + 1) * nu]
+            yk = np.zeros(nu)
+            yk[:na] = (Am @ xk)[:na] + (Bm[k].T @ xp)[:na]
+            yk[na:] = xk[na:]
+            y[k * nu:(k + 1) * nu] = yk
+        y[3 * nu:] = sum(Bm[k] @ x[k * nu:(k + 1) * nu] for k in range(3))
+        return y
+
+    def combined(self, A, B):
+        """BlockFEMatrix::get_combined_matrix (src/Matrix/BlockFEMatrix.C:1168-1345):
+        unit Dirichlet rows, first pressure row a unit row."""
+        import scipy.sparse as sp
+        nu, na, npr = self.n_u, self.n_u_active, self.n_p
+        Am = sp.csr_matrix((A, self.vs.col, self.vs.rp), shape=(nu, nu)).tolil()
+        Bm = [sp.csr_matrix((b, self.bcol, self.brp), shape=(npr, nu)) for b in B]
+        Z = sp.csr_matrix((nu, nu))
+        blocks = [[Am if i == j else Z for j in range(3)] + [Bm[i].T] for i in range(3)]
+        blocks.append([Bm[0], Bm[1], Bm[2], sp.csr_matrix((npr, npr))])
+        K = sp.bmat(blocks, format="lil")
+        for k in range(3):
+            for r in range(na, nu):
+                row = k * nu + r
+                K.rows[row] = [row]
+                K.data[row] = [1.0]
+        row = 3 * nu
+        K.rows[row] = [row]
+        K.data[row] = [1.0]
+        return K.tocsr()
+
+
+def nse_exact(x):
+    """test_u_2_p_1.h: u = (y^2, xz, -y^2), p = x + y + z - 1.5 at points x [n, 3]."""
+    X, Y, Z = x[:, 0], x[:, 1], x[:, 2]
+    return np.stack([Y * Y, X * Z, -Y * Y]), X + Y + Z - 1.5
+
+
+def nodal_coords(space):
+    """Nodal point of every DOF of an OracleSpace."""
+    xi = ref_nodes(space.shape, space.order)
+    out = np.zeros((space.n_dof, 3))
+    for c in range(space.n_cells):
+        for k in range(space.ndof_cell):
+            out[space.gnum[c, k]] = node_phys(space.shape, space.verts[c], xi[k])
+    return out
+
+
+def nse_interpolate(ns):
+    u, _ = nse_exact(nodal_coords(ns.vs))
+    _, p = nse_exact(nodal_coords(ns.ps))
+    return np.concatenate([u[0], u[1], u[2], p])

@@ -212,3 +212,30 @@ def test_tcd_linear_space_time_is_exact(scheme, geo, order):
     sp, info = O.build_space(GEO[geo], 3, 3, order)
     errs, _ = O.tcd_time_integration(sp, 8, 0, scheme, 0.1, 5, eps=1.0, b=(1.0, 2.0), c=1.0)
     assert np.max(errs) < 1e-10, errs
+
+
+@pytest.mark.parametrize("nref,nu", [(1, 1.0), (2, 1e-2)])
+def test_nse_interpolant_solves_oseen_with_its_own_convection(nref, nu):
+    """test_u_2_p_1 (NSE3D example -3): u quadratic, p linear, div u = 0, so the
+    Q2/Q1 interpolant satisfies the discrete Oseen system with w = u exactly
+    (Gauss4 integrates every term exactly): K x_I = rhs to round-off.  The same
+    check without the convection term fails, and so does a flipped B sign."""
+    ns = O.NseOracle(nref, nref)
+    xI = O.nse_interpolate(ns)
+    w = xI[:3 * ns.n_u]
+    A, B, rhs = ns.assemble(nu, w)
+    assert np.max(np.abs(ns.apply(A, B, xI) - rhs)) < 1e-13
+    A0, B0, rhs0 = ns.assemble(nu, None)
+    assert np.max(np.abs(ns.apply(A0, B0, xI) - rhs0)) > 1e-4
+    Bf = [-b for b in B]
+    assert np.max(np.abs(ns.apply(A, Bf, xI) - rhs)) > 1e-4
+
+
+def test_nse_pattern_closed_forms():
+    """B rows (pressure vertices) couple to the Q2 DOFs of the cells around the
+    vertex: an inner vertex of an n^3 mesh touches 8 cells = 5^3 Q2 DOFs."""
+    ns = O.NseOracle(2, 2)
+    n = 4
+    assert ns.n_u == (2 * n + 1) ** 3 and ns.n_u_active == (2 * n - 1) ** 3 and ns.n_p == (n + 1) ** 3
+    lens = np.diff(ns.brp)
+    assert lens.max() == 125 and np.sum(lens == 125) == (n - 1) ** 3
