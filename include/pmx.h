@@ -403,6 +403,7 @@ int pmx_batch_errors(pmx_batch* b, int problem, double* host_errors4);
 #define PMX_PC_MULTIGRID 2
 #define PMX_SMOOTHER_JACOBI 0
 #define PMX_SMOOTHER_DIRECT 1
+#define PMX_SMOOTHER_CELL_VANKA_JACOBI 2   /* VankaSmoother CELL_JACOBI (saddle-point systems) */
 #define PMX_CYCLE_V 0
 #define PMX_CYCLE_W 1
 #define PMX_CYCLE_F 2
@@ -426,7 +427,9 @@ typedef struct {
   double mg_coarse_residual;   /* multigrid_coarse_residual */
   int mg_coarse_max_n_iterations;
   double jacobi_damp;          /* jacobi_damp */
-  int precision;               /* PMX_PRECISION_* (multigrid preconditioner only) */
+  int precision;               /* PMX_PRECISION_* (multigrid preconditioner only; for
+                                  saddle-point systems MIXED stores the Vanka factors in FP32) */
+  double vanka_damp;           /* multigrid_vanka_damp_factor (default 1) */
 } pmx_solver_params;
 /* Defaults of Solver<>::default_solver_database (src/Solver/Solver.C:20-117)
  * merged with Multigrid::default_multigrid_database (src/Multigrid/Multigrid.C:422-514). */
@@ -450,6 +453,13 @@ typedef struct {
    * converged() returns true in all these cases and only its log differs) */
   int converged;
 } pmx_solve_report;
+/* Solver::update_matrix for a BlockFEMatrix saddle-point system + Multigrid
+ * initialize/update (src/Multigrid/Multigrid.C:157-420) with the
+ * cell_vanka_jacobi smoother (src/Multigrid/VankaSmoother.C:44-300): `levels`
+ * are the NSE systems of consecutive mesh levels, COARSEST FIRST (assembled);
+ * the coarsest is solved with the inverse of its combined matrix
+ * (direct_solve) or smoothed.  FGMRES only; without multigrid pass one level. */
+int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels);
 /* Solver::solve(rhs, solution) (src/Solver/Solver.C:414-488): iterates from
  * the given dev_x.  Blocks until done. */
 int pmx_solver_solve(pmx_solver* s, const double* dev_b, double* dev_x, pmx_solve_report* rep);

@@ -78,7 +78,7 @@ class SolverParams(ctypes.Structure):
                 ("mg_smoother_coarse", c_int), ("mg_correction_damp", c_double),
                 ("mg_n_pre_smooth", c_int), ("mg_n_post_smooth", c_int),
                 ("mg_coarse_residual", c_double), ("mg_coarse_max_n_iterations", c_int),
-                ("jacobi_damp", c_double), ("precision", c_int)]
+                ("jacobi_damp", c_double), ("precision", c_int), ("vanka_damp", c_double)]
 
 
 class SolveReport(ctypes.Structure):
@@ -166,6 +166,7 @@ SIGNATURES = {
     "pmx_solver_create": [c_void_p, P(SolverParams), P(c_void_p)],
     "pmx_solver_destroy": [c_void_p],
     "pmx_solver_update": [c_void_p, P(c_void_p), P(c_void_p), c_int],
+    "pmx_solver_update_nse": [c_void_p, P(c_void_p), c_int],
     "pmx_solver_solve": [c_void_p, c_void_p, c_void_p, P(SolveReport)],
     "pmx_solver_get_history": [c_void_p, P(c_double), c_int, P(c_int)],
     "pmx_solver_apply_preconditioner": [c_void_p, c_void_p, c_void_p],

@@ -221,6 +221,156 @@ __global__ __launch_bounds__(1024) void k_pressure_shift(int nb, const double* _
   for (int k = threadIdx.x; k < n_p; k += blockDim.x) p[k] -= mean;
 }
 
+// ------------------------------------------------------------- cell Vanka
+// VankaSmoother with CELL_JACOBI (src/Multigrid/VankaSmoother.C:44-300): one
+// local system per cell on the DOFs [u1 | u2 | u3 | p] of the cell (sorted per
+// block), taken from the combined matrix; LU with partial pivoting (the
+// LAPACK dgetrf of DenseMatrix::decomposeLU, src/Matrix/DenseMatrix.C:136-182);
+// one smoothing step: r = rhs - K x, every cell solves its local system with
+// the local r, the damped local solutions are summed per DOF (in cell order)
+// and divided by the number of cells that touched the DOF.
+constexpr int NL = 3 * NU + NP;   // 89
+constexpr int LDL = NL + 1;       // LDS row stride
+
+// one workgroup (256 threads) per cell: gather, factor in LDS (FP64), store the
+// factors column-major (FP64 and/or FP32) and the pivots
+__global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __restrict__ map,
+                                                      const int* __restrict__ rp, const int* __restrict__ ci,
+                                                      const double* __restrict__ cv, double* __restrict__ lu64,
+                                                      float* __restrict__ lu32, int* __restrict__ piv,
+                                                      int* __restrict__ singular) {
+  __shared__ double M[NL * LDL];
+  __shared__ int dm[NL];
+  __shared__ int sp;
+  const int cell = blockIdx.x;
+  if (cell >= n_cells) return;   // uniform for the workgroup
+  const int t = threadIdx.x;
+  for (int a = t; a < NL; a += blockDim.x) dm[a] = map[(size_t)cell * NL + a];
+  for (int k = t; k < NL * LDL; k += blockDim.x) M[k] = 0.0;
+  __syncthreads();
+  for (int a = t; a < NL; a += blockDim.x) {
+    const int r = dm[a];
+    for (int q = rp[r]; q < rp[r + 1]; ++q) {
+      const int c = ci[q];
+      int lo = 0, hi = NL - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (dm[mid] < c) lo = mid + 1; else hi = mid;
+      }
+      if (dm[lo] == c) M[a * LDL + lo] = cv[q];
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < NL; ++k) {
+    if (t < 64) {
+      // pivot: largest |M[i][k]|, i >= k, first index on ties
+      double best = -1.0;
+      int bi = k;
+      for (int i = k + t; i < NL; i += 64) {
+        const double v = fabs(M[i * LDL + k]);
+        if (v > best) { best = v; bi = i; }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      if (t == 0) sp = bi;
+    }
+    __syncthreads();
+    const int p = sp;
+    if (p != k)
+      for (int j = t; j < NL; j += blockDim.x) {
+        const double tmp = M[k * LDL + j];
+        M[k * LDL + j] = M[p * LDL + j];
+        M[p * LDL + j] = tmp;
+      }
+    __syncthreads();
+    const double d = M[k * LDL + k];
+    if (t == 0) {
+      piv[(size_t)cell * NL + k] = p;
+      if (d == 0.0) atomicOr(singular, 1);
+    }
+    const double inv = (d != 0.0) ? 1.0 / d : 0.0;
+    for (int i = k + 1 + t; i < NL; i += blockDim.x) M[i * LDL + k] *= inv;
+    __syncthreads();
+    const int m = NL - k - 1;
+    for (int e = t; e < m * m; e += blockDim.x) {
+      const int i = k + 1 + e / m, j = k + 1 + e % m;
+      M[i * LDL + j] -= M[i * LDL + k] * M[k * LDL + j];
+    }
+    __syncthreads();
+  }
+  // column-major store: element (i, j) at j * NL + i
+  for (int e = t; e < NL * NL; e += blockDim.x) {
+    const int j = e / NL, i = e % NL;
+    const double v = M[i * LDL + j];
+    if (lu64) lu64[(size_t)cell * NL * NL + e] = v;
+    if (lu32) lu32[(size_t)cell * NL * NL + e] = (float)v;
+  }
+}
+
+// one wavefront per cell: the local rhs lives in registers (rows lane and
+// lane + 64), forward and backward substitution broadcast the pivot row value
+template <typename T>
+__global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const int* __restrict__ map,
+                                                     const T* __restrict__ lu, const int* __restrict__ piv,
+                                                     const double* __restrict__ r, double damp,
+                                                     double* __restrict__ upd) {
+  const int lane = threadIdx.x & 63;
+  const int cell = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  if (cell >= n_cells) return;   // uniform for the wavefront
+  const int* dm = map + (size_t)cell * NL;
+  const T* L = lu + (size_t)cell * NL * NL;
+  const int* pv = piv + (size_t)cell * NL;
+  const int i0 = lane, i1 = lane + 64;
+  const bool h1 = i1 < NL;
+  double b0 = r[dm[i0]];
+  double b1 = h1 ? r[dm[i1]] : 0.0;
+  // row interchanges in factorisation order
+  for (int k = 0; k < NL; ++k) {
+    const int p = pv[k];
+    if (p != k) {
+      const double vk = __shfl(k < 64 ? b0 : b1, k & 63, 64);
+      const double vp = __shfl(p < 64 ? b0 : b1, p & 63, 64);
+      if (i0 == k) b0 = vp; else if (i0 == p) b0 = vk;
+      if (i1 == k) b1 = vp; else if (i1 == p) b1 = vk;
+    }
+  }
+  // L y = b (unit diagonal)
+  for (int k = 0; k < NL - 1; ++k) {
+    const double bk = __shfl(k < 64 ? b0 : b1, k & 63, 64);
+    const T* col = L + (size_t)k * NL;
+    if (i0 > k) b0 -= (double)col[i0] * bk;
+    if (h1 && i1 > k) b1 -= (double)col[i1] * bk;
+  }
+  // U x = y
+  for (int k = NL - 1; k >= 0; --k) {
+    const T* col = L + (size_t)k * NL;
+    const double bk = __shfl(k < 64 ? b0 : b1, k & 63, 64);
+    const double xk = bk / (double)col[k];
+    if (i0 == k) b0 = xk;
+    if (i1 == k) b1 = xk;
+    if (i0 < k) b0 -= (double)col[i0] * xk;
+    if (h1 && i1 < k) b1 -= (double)col[i1] * xk;
+  }
+  upd[(size_t)cell * NL + i0] = damp * b0;
+  if (h1) upd[(size_t)cell * NL + i1] = damp * b1;
+}
+
+// x[i] += (sum of the cell updates of DOF i, in cell order) / their number
+__global__ void k_vanka_accumulate(int n, const int* __restrict__ acc_ptr, const int* __restrict__ acc_idx,
+                                   const double* __restrict__ upd, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = acc_ptr[i], e = acc_ptr[i + 1];
+  if (a == e) return;
+  double s = 0.0;
+  for (int k = a; k < e; ++k) s += upd[acc_idx[k]];
+  x[i] += s / (double)(e - a);
+}
+
 }  // namespace
 
 void assemble_nse(const NseAsm& d, const int* cells, int n, hipStream_t st) {
@@ -238,6 +388,29 @@ void combined_values(long long nnz, const uint8_t* kind, const int* pos, const d
                      const double* B2, const double* B3, double* out, hipStream_t st) {
   if (nnz > 0)
     hipLaunchKernelGGL(k_combined_values, dim3(4096), dim3(256), 0, st, nnz, kind, pos, A, B1, B2, B3, out);
+}
+
+void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, float* lu32, int* piv,
+                  int* singular, hipStream_t st) {
+  if (n_cells > 0)
+    hipLaunchKernelGGL(k_vanka_factor, dim3(n_cells), dim3(256), 0, st, n_cells, map, C.row_ptr, C.col, C.val,
+                       lu64, lu32, piv, singular);
+}
+
+void vanka_solve(int n_cells, const int* map, const double* lu64, const float* lu32, const int* piv,
+                 const double* r, double damp, double* upd, hipStream_t st) {
+  if (n_cells <= 0) return;
+  const int grid = (n_cells + 3) / 4;
+  if (lu32) hipLaunchKernelGGL((k_vanka_solve<float>), dim3(grid), dim3(256), 0, st, n_cells, map, lu32, piv, r,
+                               damp, upd);
+  else hipLaunchKernelGGL((k_vanka_solve<double>), dim3(grid), dim3(256), 0, st, n_cells, map, lu64, piv, r, damp,
+                          upd);
+}
+
+void vanka_accumulate(int n, const int* acc_ptr, const int* acc_idx, const double* upd, double* x,
+                      hipStream_t st) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_vanka_accumulate, dim3((n + 255) / 256), dim3(256), 0, st, n, acc_ptr, acc_idx, upd, x);
 }
 
 void pressure_project_l20(int n_cells, int nq, const double* ref, const double* geo, const int* pdofs, int n_p,

@@ -144,6 +144,14 @@ void combined_values(long long nnz, const uint8_t* kind, const int* pos, const d
                      const double* B2, const double* B3, double* out, hipStream_t st);
 void pressure_project_l20(int n_cells, int nq, const double* ref, const double* geo, const int* pdofs, int n_p,
                           double* p, double* partial, hipStream_t st);
+// cell Vanka (Jacobi variant) on Q2/Q1 hexahedra: 89 local DOFs per cell
+constexpr int kVankaLocal = 89;
+void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, float* lu32, int* piv,
+                  int* singular, hipStream_t st);
+void vanka_solve(int n_cells, const int* map, const double* lu64, const float* lu32, const int* piv,
+                 const double* r, double damp, double* upd, hipStream_t st);
+void vanka_accumulate(int n, const int* acc_ptr, const int* acc_idx, const double* upd, double* x,
+                      hipStream_t st);
 
 // ---- batched time-dependent solve (kernels_batch.hip)
 struct BatchDev {

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "nse_internal.hpp"
 
 using namespace pmx_internal;
 
@@ -65,6 +66,26 @@ pmx::DevCSR upload_csr(const pmx::CSRPattern& p, const std::vector<double>& v, h
   return M;
 }
 
+// one level of the saddle-point multigrid: the system (not owned), the
+// cell-Vanka data and the transfer from this level to the next finer one
+struct NseLevel {
+  pmx_nse* sys = nullptr;
+  int n = 0, n_u = 0, n_ua = 0, n_p = 0, n_cells = 0;
+  int *map = nullptr, *piv = nullptr, *acc_ptr = nullptr, *acc_idx = nullptr;
+  double* lu64 = nullptr;
+  float* lu32 = nullptr;
+  double *upd = nullptr, *rhs = nullptr, *sol = nullptr, *def = nullptr, *res = nullptr;
+  pmx::DevCSR Pu{}, Ru{}, Pp{}, Rp{};   // P: finer rows from this level; R = P^T (rows: this level)
+  double* dinv_dense = nullptr;          // coarsest level: inverse of the combined matrix
+  double* partial = nullptr;
+  void release() {
+    dfree(map); dfree(piv); dfree(acc_ptr); dfree(acc_idx); dfree(lu64); dfree(lu32); dfree(upd);
+    dfree(rhs); dfree(sol); dfree(def); dfree(res);
+    free_csr(Pu); free_csr(Ru); free_csr(Pp); free_csr(Rp);
+    dfree(dinv_dense); dfree(partial);
+  }
+};
+
 struct Level {
   pmx::DevCSR A;             // matrix view: fp64 arrays owned by pmx_matrix, bf16 copy owned here
   int n = 0, active = 0;
@@ -92,6 +113,9 @@ struct pmx_solver {
   pmx_ctx* ctx = nullptr;
   pmx_solver_params p{};
   std::vector<Level> lv;     // coarsest first
+  std::vector<NseLevel> nlv; // saddle-point levels, coarsest first (pmx_solver_update_nse)
+  bool saddle = false;
+  const pmx_nse* nse_fine = nullptr;   // saddle-point operator without multigrid
   std::vector<pmx_matrix*> mats;   // level matrices (their SELL values are refreshed per solve)
   std::vector<Dir> cc;
   pmx::DevCSR A;             // finest operator (view)
@@ -108,6 +132,9 @@ struct pmx_solver {
   void release() {
     for (auto& l : lv) l.release();
     lv.clear();
+    for (auto& l : nlv) l.release();
+    nlv.clear();
+    saddle = false;
     for (auto*& v : V) dfree(v);
     for (auto*& z : Z) dfree(z);
     V.clear(); Z.clear();
@@ -258,6 +285,125 @@ void mg_apply(pmx_solver* s, const double* v, double* z) {
   }
 }
 
+// ------------------------------------------------- saddle-point multigrid
+// Multigrid::cycle / cycle_step (src/Multigrid/Multigrid.C:157-420) on
+// BlockFEMatrix levels: defects with the block operator, cell-Vanka smoothing
+// (CELL_JACOBI) on the combined matrix, velocity and pressure transfers block
+// by block; the coarsest level is solved with the inverse of its combined
+// matrix (DirectSmoother) or smoothed.
+void nse_smooth(pmx_solver* s, int l) {
+  NseLevel& L = s->nlv[l];
+  if (l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) {
+    pmx::dense_mv(L.n, L.dinv_dense, L.rhs, L.sol, S(s));
+    return;
+  }
+  HIP_CHECK(hipMemcpyAsync(L.res, L.rhs, L.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+  pmx::csr_spmv(nse_combined(L.sys), L.sol, L.res, -1.0, 1.0, L.n, 0, S(s));
+  pmx::vanka_solve(L.n_cells, L.map, L.lu64, L.lu32, L.piv, L.res, s->p.vanka_damp, L.upd, S(s));
+  pmx::vanka_accumulate(L.n, L.acc_ptr, L.acc_idx, L.upd, L.sol, S(s));
+}
+
+void nse_defect(pmx_solver* s, int l, double* norm_out) {
+  NseLevel& L = s->nlv[l];
+  HIP_CHECK(hipMemcpyAsync(L.def, L.rhs, L.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+  nse_apply_scaled_add(L.sys, L.sol, L.def, -1.0, S(s));
+  if (norm_out) pmx::vec_norm(L.n, L.def, L.partial, norm_out, S(s));
+}
+
+void nse_update_solution_in_finer_grid(pmx_solver* s, int l) {
+  NseLevel& C = s->nlv[l];
+  NseLevel& F = s->nlv[l + 1];
+  const double damp = s->p.mg_correction_damp;
+  for (int k = 0; k < 3; ++k)
+    pmx::csr_spmv(C.Pu, C.sol + (size_t)k * C.n_u, F.sol + (size_t)k * F.n_u, damp, 1.0, F.n_ua, 0, S(s));
+  pmx::csr_spmv(C.Pp, C.sol + (size_t)3 * C.n_u, F.sol + (size_t)3 * F.n_u, damp, 1.0, F.n_p, 0, S(s));
+}
+
+void nse_update_rhs_in_coarser_grid(pmx_solver* s, int l) {
+  NseLevel& F = s->nlv[l];
+  NseLevel& C = s->nlv[l - 1];
+  for (int k = 0; k < 3; ++k)
+    pmx::csr_spmv(C.Ru, F.def + (size_t)k * F.n_u, C.rhs + (size_t)k * C.n_u, 1.0, 0.0, C.n_ua, 1, S(s));
+  pmx::csr_spmv(C.Rp, F.def + (size_t)3 * F.n_u, C.rhs + (size_t)3 * C.n_u, 1.0, 0.0, C.n_p, 1, S(s));
+}
+
+int nse_cycle_step(pmx_solver* s, int step, int level) {
+  const int nl = (int)s->nlv.size();
+  if (level == 0) {
+    double res = 1e10;
+    if (s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) {
+      nse_smooth(s, 0);   // an exact solve: repeating it gives the same solution
+    } else {
+      for (int i = 0; i < s->p.mg_coarse_max_n_iterations && res > s->p.mg_coarse_residual; ++i) {
+        nse_smooth(s, 0);
+        nse_defect(s, 0, s->d_scal);
+        res = read_scalar(s, s->d_scal);
+      }
+    }
+    if (nl > 1) nse_update_solution_in_finer_grid(s, 0);
+    return 1;
+  }
+  const bool from_below = s->cc[step - 1] == DIR_UP;
+  const bool up = s->cc[step] == DIR_UP;
+  const bool down = s->cc[step] == DIR_DOWN;
+  if (from_below)
+    for (int i = 0; i < s->p.mg_n_post_smooth; ++i) nse_smooth(s, level);
+  if (down) {
+    for (int i = 0; i < s->p.mg_n_pre_smooth; ++i) nse_smooth(s, level);
+    nse_defect(s, level, nullptr);
+    NseLevel& C = s->nlv[level - 1];
+    pmx::vec_fill(C.n, 0.0, C.sol, S(s));
+    nse_update_rhs_in_coarser_grid(s, level);
+    return level - 1;
+  } else if (up) {
+    nse_update_solution_in_finer_grid(s, level);
+    return level + 1;
+  }
+  return -1;
+}
+
+void nse_mg_apply(pmx_solver* s, const double* v, double* z) {
+  NseLevel& F = s->nlv.back();
+  HIP_CHECK(hipMemcpyAsync(F.rhs, v, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+  // finest solution 0, non-actives (Dirichlet velocity entries) from the rhs
+  pmx::vec_fill(F.n, 0.0, F.sol, S(s));
+  for (int k = 0; k < 3; ++k) {
+    const size_t o = (size_t)k * F.n_u + F.n_ua;
+    if (F.n_u > F.n_ua)
+      HIP_CHECK(hipMemcpyAsync(F.sol + o, F.rhs + o, (F.n_u - F.n_ua) * sizeof(double), hipMemcpyDeviceToDevice,
+                               S(s)));
+  }
+  int level = (int)s->nlv.size() - 1;
+  const int n_steps = (int)s->cc.size() - 1;
+  for (int step = 1; step <= n_steps; ++step) level = nse_cycle_step(s, step, level);
+  HIP_CHECK(hipMemcpyAsync(z, F.sol, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+}
+
+// dense inverse of an n x n CSR matrix (device values) on the GPU
+double* dense_inverse_of(pmx_solver* s, int n, const std::vector<int>& rp, const std::vector<int>& col,
+                         const double* dev_vals) {
+  if (n > 20000) throw PmxError(PMX_E_ARG, "direct coarse solve supports at most 20000 DOFs");
+  std::vector<double> vals(col.size());
+  HIP_CHECK(hipMemcpyAsync(vals.data(), dev_vals, vals.size() * sizeof(double), hipMemcpyDeviceToHost, S(s)));
+  HIP_CHECK(hipStreamSynchronize(S(s)));
+  std::vector<double> a((size_t)n * n, 0.0);
+  for (int r = 0; r < n; ++r)
+    for (int k = rp[r]; k < rp[r + 1]; ++k) a[(size_t)r * n + col[k]] = vals[k];
+  double* d = upload(a.data(), a.size(), S(s));
+  double* f = dmalloc<double>(n);
+  int* perm = dmalloc<int>(n);
+  int* sing = dmalloc<int>(1);
+  HIP_CHECK(hipMemsetAsync(sing, 0, sizeof(int), S(s)));
+  pmx::dense_invert(n, d, f, perm, sing, S(s));
+  HIP_CHECK(hipGetLastError());
+  int hs = 0;
+  HIP_CHECK(hipMemcpyAsync(&hs, sing, sizeof(int), hipMemcpyDeviceToHost, S(s)));
+  HIP_CHECK(hipStreamSynchronize(S(s)));
+  dfree(f); dfree(perm); dfree(sing);
+  if (hs) { dfree(d); throw PmxError(PMX_E_ZERO_DIAG, "singular coarse matrix in the direct coarse solve"); }
+  return d;
+}
+
 // Dense inverse of the coarsest matrix by Gauss-Jordan elimination with
 // partial pivoting on the GPU (set-up only; the coarse solve is then one dense
 // matrix-vector product).
@@ -297,8 +443,31 @@ T* dzeros(size_t n, hipStream_t st) {
 
 namespace {
 
+// r = b - K x and ||r|| into d_scal (scalar FE matrix or saddle-point system)
+void op_defect(pmx_solver* s, const double* x, const double* b, double* r) {
+  if (s->saddle) {
+    const pmx_nse* sys = s->nlv.empty() ? s->nse_fine : s->nlv.back().sys;
+    HIP_CHECK(hipMemcpyAsync(r, b, s->n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+    nse_apply_scaled_add(sys, x, r, -1.0, S(s));
+    pmx::vec_norm(s->n, r, s->partial, s->d_scal, S(s));
+  } else {
+    pmx::fe_defect(s->A, x, b, r, s->partial, s->d_scal, S(s));
+  }
+}
+
+void op_apply(pmx_solver* s, const double* x, double* y) {
+  if (s->saddle) {
+    const pmx_nse* sys = s->nlv.empty() ? s->nse_fine : s->nlv.back().sys;
+    pmx::vec_fill(s->n, 0.0, y, S(s));
+    nse_apply_scaled_add(sys, x, y, 1.0, S(s));
+  } else {
+    pmx::fe_apply(s->A, x, y, S(s));
+  }
+}
+
 void precondition(pmx_solver* s, const double* v, double* z) {
-  if (s->p.preconditioner == PMX_PC_MULTIGRID) mg_apply(s, v, z);
+  if (s->p.preconditioner == PMX_PC_MULTIGRID && s->saddle) nse_mg_apply(s, v, z);
+  else if (s->p.preconditioner == PMX_PC_MULTIGRID) mg_apply(s, v, z);
   else if (s->p.preconditioner == PMX_PC_JACOBI) pmx::vec_mul(s->n, s->dinv, v, z, S(s));
   else HIP_CHECK(hipMemcpyAsync(z, v, s->n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
 }
@@ -337,7 +506,7 @@ void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
   const int ld = m + 1;
   std::vector<double> H((size_t)ld * m, 0.0), sv(m + 1), cs(m + 1), sn(m + 1);
   // r = b - A x
-  pmx::fe_defect(s->A, x, b, s->r, s->partial, s->d_scal, S(s));
+  op_defect(s, x, b, s->r);
   double beta = read_scalar(s, s->d_scal);
   double resid = beta;
   rep->initial_residual = beta;
@@ -350,7 +519,7 @@ void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
     int i = 0;
     for (i = 0; i < m && j <= s->p.max_n_iterations; ++i, ++j) {
       precondition(s, s->V[i], s->Z[i]);
-      pmx::fe_apply(s->A, s->Z[i], s->r, S(s));
+      op_apply(s, s->Z[i], s->r);
       double* Hcol = s->d_H + (size_t)i * ld;
       // modified Gram-Schmidt, pipelined: H(k,i) = <r, v_k>; r -= H(k,i) v_k
       pmx::vec_dot(n, s->r, s->V[0], s->partial, Hcol, S(s));
@@ -376,7 +545,7 @@ void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
       }
     }
     gmres_update(s, x, m - 1, H, ld, sv);
-    pmx::fe_defect(s->A, x, b, s->r, s->partial, s->d_scal, S(s));
+    op_defect(s, x, b, s->r);
     beta = read_scalar(s, s->d_scal);
     if (converged(s, resid, j)) {
       rep->iterations = j; rep->final_residual = resid; rep->converged = s->reached ? 1 : 0;
@@ -488,6 +657,7 @@ int pmx_solver_default_params(pmx_solver_params* p) {
     d.mg_coarse_max_n_iterations = 10;
     d.jacobi_damp = 1.0;
     d.precision = PMX_PRECISION_FP64;
+    d.vanka_damp = 1.0;
     *p = d;
   });
 }
@@ -498,8 +668,10 @@ int pmx_solver_create(pmx_ctx* ctx, const pmx_solver_params* p, pmx_solver** out
     require(p->iterative_solver_type >= 0 && p->iterative_solver_type <= 3, "unknown iterative solver");
     require(p->preconditioner >= 0 && p->preconditioner <= 2, "unknown preconditioner");
     require(p->max_n_iterations >= 0 && p->gmres_restart >= 1, "invalid iteration counts");
-    require(p->mg_smoother == PMX_SMOOTHER_JACOBI, "only the jacobi multigrid smoother is available");
-    require(p->mg_smoother_coarse == PMX_SMOOTHER_JACOBI || p->mg_smoother_coarse == PMX_SMOOTHER_DIRECT,
+    require(p->mg_smoother == PMX_SMOOTHER_JACOBI || p->mg_smoother == PMX_SMOOTHER_CELL_VANKA_JACOBI,
+            "unknown multigrid smoother");
+    require(p->mg_smoother_coarse == PMX_SMOOTHER_JACOBI || p->mg_smoother_coarse == PMX_SMOOTHER_DIRECT ||
+                p->mg_smoother_coarse == PMX_SMOOTHER_CELL_VANKA_JACOBI,
             "unknown coarse smoother");
     require(p->precision == PMX_PRECISION_FP64 || p->precision == PMX_PRECISION_MIXED, "unknown precision");
     auto s = std::make_unique<pmx_solver>();
@@ -525,6 +697,9 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
     const bool mg = s->p.preconditioner == PMX_PC_MULTIGRID;
     require(!mg || (spaces && n_levels >= 2), "multigrid needs >= 2 levels and their spaces");
     require(mg || n_levels == 1, "without multigrid pass exactly one matrix");
+    require(!mg || (s->p.mg_smoother == PMX_SMOOTHER_JACOBI &&
+                    s->p.mg_smoother_coarse != PMX_SMOOTHER_CELL_VANKA_JACOBI),
+            "scalar problems use the jacobi smoother (cell Vanka is for saddle-point systems)");
     s->release();
     hipStream_t st = S(s);
     const pmx_matrix* fine = levels[n_levels - 1];
@@ -597,6 +772,118 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
         }
       }
       if (s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) build_coarse_inverse(s, levels[0]);
+      s->cc = cycle_control(s->p.mg_cycle, n_levels);
+    }
+    HIP_CHECK(hipStreamSynchronize(st));
+    s->updated = true;
+  });
+}
+
+int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
+  return guarded([&] {
+    require(s && levels && n_levels >= 1, "invalid argument");
+    const bool mg = s->p.preconditioner == PMX_PC_MULTIGRID;
+    require(s->p.iterative_solver_type == PMX_IT_FGMRES, "saddle-point systems are solved with fgmres");
+    require(s->p.preconditioner != PMX_PC_JACOBI, "use multigrid (cell Vanka) or no preconditioner");
+    require(mg || n_levels == 1, "without multigrid pass exactly one system");
+    require(!mg || n_levels >= 2, "multigrid needs >= 2 levels");
+    require(!mg || s->p.mg_smoother == PMX_SMOOTHER_CELL_VANKA_JACOBI,
+            "saddle-point multigrid uses the cell_vanka_jacobi smoother");
+    require(!mg || s->p.mg_smoother_coarse != PMX_SMOOTHER_JACOBI,
+            "coarse smoother must be direct_solve or cell_vanka_jacobi");
+    s->release();
+    hipStream_t st = S(s);
+    for (int l = 0; l < n_levels; ++l) {
+      require(levels[l] != nullptr, "NULL level");
+      require(levels[l]->values_ready, "assemble the system before updating the solver");
+      ensure_sell_values(levels[l]->A, st);
+    }
+    s->saddle = true;
+    s->nse_fine = levels[n_levels - 1];
+    s->n = s->nse_fine->n_comb;
+    const int n = s->n;
+    s->partial = dmalloc<double>(4096);
+    s->d_scal = dmalloc<double>(64);
+    s->r = dzeros<double>(n, st);
+    const int m = std::min(s->p.gmres_restart, std::max(1, s->p.max_n_iterations));
+    for (int k = 0; k <= m; ++k) s->V.push_back(dzeros<double>(n, st));
+    for (int k = 0; k < m; ++k) s->Z.push_back(dzeros<double>(n, st));
+    s->d_H = dzeros<double>((size_t)(m + 1) * m + 16, st);
+    if (mg) {
+      const bool f32 = s->p.precision == PMX_PRECISION_MIXED;
+      int* sing = dmalloc<int>(1);
+      HIP_CHECK(hipMemsetAsync(sing, 0, sizeof(int), st));
+      s->nlv.resize(n_levels);
+      for (int l = 0; l < n_levels; ++l) {
+        NseLevel& L = s->nlv[l];
+        pmx_nse* sys = levels[l];
+        L.sys = sys;
+        L.n = sys->n_comb;
+        L.n_u = sys->n_u;
+        L.n_ua = sys->n_u_active;
+        L.n_p = sys->n_p;
+        L.n_cells = sys->vs.n_cells;
+        L.rhs = dzeros<double>(L.n, st);
+        L.sol = dzeros<double>(L.n, st);
+        L.def = dzeros<double>(L.n, st);
+        L.res = dzeros<double>(L.n, st);
+        L.partial = dmalloc<double>(4096);
+        const bool vanka = !(l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT);
+        if (vanka) {
+          // local DOF maps: [u1 | u2 | u3 | p] of every cell, sorted per block
+          const int NLc = pmx::kVankaLocal;
+          std::vector<int> map((size_t)L.n_cells * NLc);
+          for (int c = 0; c < L.n_cells; ++c) {
+            int* mc = &map[(size_t)c * NLc];
+            std::vector<int> v(sys->vs.global_numbers.begin() + (size_t)c * 27,
+                               sys->vs.global_numbers.begin() + (size_t)(c + 1) * 27);
+            std::vector<int> p(sys->ps.global_numbers.begin() + (size_t)c * 8,
+                               sys->ps.global_numbers.begin() + (size_t)(c + 1) * 8);
+            std::sort(v.begin(), v.end());
+            std::sort(p.begin(), p.end());
+            for (int k = 0; k < 3; ++k)
+              for (int j = 0; j < 27; ++j) mc[k * 27 + j] = k * L.n_u + v[j];
+            for (int j = 0; j < 8; ++j) mc[81 + j] = 3 * L.n_u + p[j];
+          }
+          std::vector<int> acc_ptr(L.n + 1, 0), acc_idx(map.size());
+          for (int d : map) acc_ptr[d + 1]++;
+          for (int i = 0; i < L.n; ++i) acc_ptr[i + 1] += acc_ptr[i];
+          std::vector<int> fill(acc_ptr.begin(), acc_ptr.end() - 1);
+          for (size_t k = 0; k < map.size(); ++k) acc_idx[fill[map[k]]++] = (int)k;
+          L.map = upload(map.data(), map.size(), st);
+          L.acc_ptr = upload(acc_ptr.data(), acc_ptr.size(), st);
+          L.acc_idx = upload(acc_idx.data(), acc_idx.size(), st);
+          const size_t nlu = (size_t)L.n_cells * NLc * NLc;
+          if (f32) L.lu32 = dmalloc<float>(nlu);
+          else L.lu64 = dmalloc<double>(nlu);
+          L.piv = dmalloc<int>((size_t)L.n_cells * NLc);
+          L.upd = dmalloc<double>((size_t)L.n_cells * NLc);
+          pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.piv, sing, st);
+        } else {
+          L.dinv_dense = dense_inverse_of(s, L.n, sys->Cp.row_ptr, sys->Cp.col, sys->d_Cval);
+        }
+        if (l + 1 < n_levels) {
+          const pmx_nse* fsys = levels[l + 1];
+          require(fsys->dom == sys->dom && fsys->level == sys->level + 1,
+                  "multigrid systems must be consecutive levels of one domain");
+          pmx::CSRPattern P, R;
+          std::vector<double> pv, rv;
+          pmx::build_prolongation(sys->vs, fsys->vs, P, pv);
+          pmx::transpose_csr(P, pv, R, rv);
+          L.Pu = upload_csr(P, pv, st);
+          L.Ru = upload_csr(R, rv, st);
+          pmx::build_prolongation(sys->ps, fsys->ps, P, pv);
+          pmx::transpose_csr(P, pv, R, rv);
+          L.Pp = upload_csr(P, pv, st);
+          L.Rp = upload_csr(R, rv, st);
+        }
+      }
+      HIP_CHECK(hipGetLastError());
+      int hs = 0;
+      HIP_CHECK(hipMemcpyAsync(&hs, sing, sizeof(int), hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      dfree(sing);
+      if (hs) throw PmxError(PMX_E_ZERO_DIAG, "singular local Vanka system");
       s->cc = cycle_control(s->p.mg_cycle, n_levels);
     }
     HIP_CHECK(hipStreamSynchronize(st));

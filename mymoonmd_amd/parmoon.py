@@ -23,7 +23,7 @@ EXAMPLES = {"const": 0, "sine_laplace_2d": 1, "p0_3d": 2, "p1_3d": 3, "p2_3d": 4
 TIME_SCHEME = {"backward_euler": 0, "crank_nicolson": 1, "bdf_two": 2}
 IT_SOLVER = {"fgmres": 0, "cg": 1, "richardson": 2, "jacobi": 3}
 PRECOND = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2}
-SMOOTHER = {"jacobi": 0, "direct_solve": 1}
+SMOOTHER = {"jacobi": 0, "direct_solve": 1, "cell_vanka_jacobi": 2}
 CYCLE = {"V": 0, "W": 1, "F": 2}
 BC_BUILTIN = {"all_dirichlet": 0, "sine_laplace": 1}
 
@@ -305,7 +305,7 @@ def solver_params(**kw):
               "multigrid_n_pre_smooth": "mg_n_pre_smooth", "multigrid_n_post_smooth": "mg_n_post_smooth",
               "multigrid_coarse_residual": "mg_coarse_residual",
               "multigrid_coarse_max_n_iterations": "mg_coarse_max_n_iterations",
-              "jacobi_damp": "jacobi_damp"}
+              "jacobi_damp": "jacobi_damp", "multigrid_vanka_damp_factor": "vanka_damp"}
     for k, v in kw.items():
         if k == "iterative_solver_type":
             p.iterative_solver_type = IT_SOLVER[v]
@@ -342,6 +342,15 @@ class Solver:
         import torch
         torch.cuda.synchronize()
         call("pmx_solver_update", self.h, mats, sps, len(matrices))
+
+    def update_saddle(self, systems):
+        """Solver::update_matrix for NSE systems (NSESystem), coarsest first."""
+        if not isinstance(systems, (list, tuple)):
+            systems = [systems]
+        arr = (c_void_p * len(systems))(*[x.h.value for x in systems])
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_solver_update_nse", self.h, arr, len(systems))
 
     def solve(self, rhs, sol):
         import torch

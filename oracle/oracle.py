@@ -785,3 +785,197 @@ def nse_interpolate(ns):
     u, _ = nse_exact(nodal_coords(ns.vs))
     _, p = nse_exact(nodal_coords(ns.ps))
     return np.concatenate([u[0], u[1], u[2], p])
+
+
+# ------------------------------------------- cell Vanka and saddle-point multigrid
+class NseLevelOracle:
+    """Multigrid level of NavierStokes<3> (NSTYPE 1): system, combined matrix
+    and the cell-Vanka patches of VankaSmoother CELL_JACOBI
+    (src/Multigrid/VankaSmoother.C:44-300)."""
+
+    def __init__(self, ns, A, B):
+        import scipy.sparse  # noqa: F401
+        self.ns, self.A, self.B = ns, A, B
+        self.K = ns.combined(A, B)
+        nu = ns.n_u
+        maps = []
+        for c in range(ns.vs.n_cells):
+            v = np.sort(ns.vs.gnum[c])
+            p = np.sort(ns.ps.gnum[c])
+            maps.append(np.concatenate([v, v + nu, v + 2 * nu, p + 3 * nu]))
+        self.maps = maps
+        self.count = np.zeros(ns.n)
+        for m in maps:
+            self.count[m] += 1
+        self.local = [self.K[m, :][:, m].toarray() for m in maps]
+
+    def vanka(self, rhs, x, damp):
+        """One CELL_JACOBI sweep: residual once, local solves (LAPACK LU with
+        partial pivoting), damped updates averaged over the cells of each DOF."""
+        r = rhs - self.K @ x
+        upd = np.zeros_like(x)
+        for m, L in zip(self.maps, self.local):
+            upd[m] += damp * np.linalg.solve(L, r[m])
+        nz = self.count > 0
+        x = x.copy()
+        x[nz] += upd[nz] / self.count[nz]
+        return x
+
+    def defect(self, rhs, x):
+        return rhs - self.ns.apply(self.A, self.B, x)
+
+
+def nse_mg_apply(levels, v, damp_corr=1.0, vanka_damp=1.0, n_pre=1, n_post=1, coarse="direct_solve",
+                 coarse_maxit=10, coarse_res=0.1, cycle="V"):
+    """Multigrid::cycle (src/Multigrid/Multigrid.C:157-420) for NSE levels
+    (coarsest first): V/W cycle of CycleControl, velocity (Q2) and pressure (Q1)
+    transfers of GridTransfer block by block, restricted defect with the
+    coarse Dirichlet velocity entries reset."""
+    nl = len(levels)
+    cc = _cycle_control(cycle, nl)
+    rhs = [None] * nl
+    sol = [None] * nl
+    F = levels[-1]
+    nu, na = F.ns.n_u, F.ns.n_u_active
+    rhs[-1] = v.copy()
+    x = np.zeros_like(v)
+    for k in range(3):
+        x[k * nu + na:(k + 1) * nu] = v[k * nu + na:(k + 1) * nu]
+    sol[-1] = x
+
+    def smooth(l):
+        L = levels[l]
+        if l == 0 and coarse == "direct_solve":
+            sol[0] = np.linalg.solve(L.K.toarray(), rhs[0])
+        else:
+            sol[l] = L.vanka(rhs[l], sol[l], vanka_damp)
+
+    def up(l):   # coarse l -> fine l + 1
+        C, Fn = levels[l], levels[l + 1]
+        cu, fu = C.ns.n_u, Fn.ns.n_u
+        for k in range(3):
+            pf = prolongate(C.ns.vs, Fn.ns.vs, sol[l][k * cu:(k + 1) * cu])
+            sol[l + 1][k * fu:k * fu + Fn.ns.n_u_active] += damp_corr * pf[:Fn.ns.n_u_active]
+        pp = prolongate(C.ns.ps, Fn.ns.ps, sol[l][3 * cu:])
+        sol[l + 1][3 * fu:] += damp_corr * pp
+
+    def down(l):   # fine l -> coarse l - 1
+        Fn, C = levels[l], levels[l - 1]
+        d = Fn.defect(rhs[l], sol[l])
+        cu, fu = C.ns.n_u, Fn.ns.n_u
+        r = np.zeros(C.ns.n)
+        for k in range(3):
+            rc = restrict(C.ns.vs, Fn.ns.vs, d[k * fu:(k + 1) * fu])
+            rc[C.ns.n_u_active:] = 0.0
+            r[k * cu:(k + 1) * cu] = rc
+        r[3 * cu:] = restrict(C.ns.ps, Fn.ns.ps, d[3 * fu:])
+        rhs[l - 1] = r
+        sol[l - 1] = np.zeros(C.ns.n)
+
+    level = nl - 1
+    for step in range(1, len(cc)):
+        if level == 0:
+            if coarse == "direct_solve":
+                smooth(0)
+            else:
+                res = 1e10
+                i = 0
+                while i < coarse_maxit and res > coarse_res:
+                    smooth(0)
+                    res = np.linalg.norm(levels[0].defect(rhs[0], sol[0]))
+                    i += 1
+            if nl > 1:
+                up(0)
+            level = 1
+            continue
+        if cc[step - 1] == "up":
+            for _ in range(n_post):
+                smooth(level)
+        if cc[step] == "down":
+            for _ in range(n_pre):
+                smooth(level)
+            down(level)
+            level -= 1
+        elif cc[step] == "up":
+            up(level)
+            level += 1
+    return sol[-1]
+
+
+def _cycle_control(cycle, nl):
+    """CycleControl (src/Multigrid/CycleControl.C:17-111): step directions,
+    entry 0 a dummy, the last one 'end'."""
+    rec = [1 if cycle == "V" else 2] * nl
+    rec[nl - 1] = 1
+    cc = ["dummy"]
+
+    def fill(level):
+        if level == 0:
+            cc.append("up")
+            return
+        for _ in range(rec[level]):
+            cc.append("down")
+            fill(level - 1)
+        if cycle == "F":
+            rec[level] = 1
+        cc.append("end" if level == nl - 1 else "up")
+    fill(nl - 1)
+    return cc
+
+
+def fgmres(apply, prec, b, x0, restart=30, max_it=100, tol=1e-8, reduction=0.0):
+    """Iteration_gmres::flexible_gmres (src/Solver/Iteration_gmres.C:374-532) with
+    IterativeMethod::converged: absolute residual |s_{i+1}|, MGS, Givens."""
+    x = x0.copy()
+    r = b - apply(x)
+    beta = np.linalg.norm(r)
+    r0 = beta
+    if beta <= tol or beta <= reduction * r0:
+        return x, 0
+    j = 1
+    while j <= max_it:
+        V = [r / beta]
+        Z = []
+        H = np.zeros((restart + 1, restart))
+        cs, sn, s = np.zeros(restart), np.zeros(restart), np.zeros(restart + 1)
+        s[0] = beta
+        for i in range(restart):
+            Z.append(prec(V[i]))
+            w = apply(Z[i])
+            for k in range(i + 1):
+                H[k, i] = np.dot(w, V[k])
+                w = w - H[k, i] * V[k]
+            H[i + 1, i] = np.linalg.norm(w)
+            V.append(w / H[i + 1, i])
+            for k in range(i):
+                t = cs[k] * H[k, i] + sn[k] * H[k + 1, i]
+                H[k + 1, i] = -sn[k] * H[k, i] + cs[k] * H[k + 1, i]
+                H[k, i] = t
+            dx, dy = H[i, i], H[i + 1, i]
+            if dy == 0:
+                cs[i], sn[i] = 1.0, 0.0
+            elif abs(dy) > abs(dx):
+                t = dx / dy
+                sn[i] = 1 / np.sqrt(1 + t * t)
+                cs[i] = t * sn[i]
+            else:
+                t = dy / dx
+                cs[i] = 1 / np.sqrt(1 + t * t)
+                sn[i] = t * cs[i]
+            H[i, i] = cs[i] * H[i, i] + sn[i] * H[i + 1, i]
+            H[i + 1, i] = 0.0
+            s[i + 1] = -sn[i] * s[i]
+            s[i] = cs[i] * s[i]
+            res = abs(s[i + 1])
+            if res <= tol or res <= reduction * r0 or j >= max_it:
+                y = np.linalg.solve(np.triu(H[:i + 1, :i + 1]), s[:i + 1])
+                for k in range(i + 1):
+                    x = x + y[k] * Z[k]
+                return x, j
+            j += 1
+        y = np.linalg.solve(np.triu(H[:restart, :restart]), s[:restart])
+        for k in range(restart):
+            x = x + y[k] * Z[k]
+        r = b - apply(x)
+        beta = np.linalg.norm(r)
+    return x, max_it

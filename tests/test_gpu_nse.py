@@ -79,3 +79,75 @@ def test_nse_interpolant_residual_errors_and_projection(ctx, pair):
     assert g.errors(x)[3] > 0.3
     g.project_pressure(x)
     assert g.errors(x)[3] < 1e-13
+
+
+MG_KW = dict(iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
+             multigrid_cycle_type="V", multigrid_smoother="cell_vanka_jacobi",
+             multigrid_smoother_coarse="direct_solve", multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2,
+             multigrid_vanka_damp_factor=0.5, multigrid_correction_damp_factor=0.8)
+
+
+def two_levels(ctx, nref, nu=1e-2):
+    """GPU and oracle NSE systems of levels nref-1, nref with w = interpolant of u."""
+    d = pm.TDomain("Default_UnitCube_Hexa", nref)
+    gl, ol, rhs = [], [], None
+    for lev in (nref - 1, nref):
+        g = pm.NSESystem(ctx, d, lev)
+        x = g.zeros()
+        g.interpolate(x)
+        r = g.zeros()
+        g.assemble("u2_p1", nu, x[:3 * g.n_u].clone(), r)
+        o = O.NseOracle(nref, lev)
+        xo = O.nse_interpolate(o)
+        A, B, ro = o.assemble(nu, xo[:3 * o.n_u])
+        gl.append(g)
+        ol.append(O.NseLevelOracle(o, A, B))
+        rhs = (r, ro, x)
+    return d, gl, ol, rhs
+
+
+@pytest.mark.parametrize("coarse", ["direct_solve", "cell_vanka_jacobi"])
+def test_nse_multigrid_vanka_cycle_matches_oracle(ctx, coarse):
+    d, gl, ol, (r, ro, xe) = two_levels(ctx, 2)
+    kw = dict(MG_KW, multigrid_smoother_coarse=coarse, multigrid_coarse_max_n_iterations=3,
+              multigrid_coarse_residual=1e-30)
+    sol = pm.Solver(ctx, pm.solver_params(**kw))
+    sol.update_saddle(gl)
+    v = np.random.default_rng(3).standard_normal(gl[-1].n)
+    z = gl[-1].zeros()
+    sol.apply_preconditioner(dev(v), z)
+    zo = O.nse_mg_apply(ol, v, damp_corr=0.8, vanka_damp=0.5, n_pre=2, n_post=2, coarse=coarse,
+                        coarse_maxit=3, coarse_res=1e-30)
+    assert np.max(np.abs(z.cpu().numpy() - zo)) <= 1e-10 * np.max(np.abs(zo))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_nse_fgmres_multigrid_vanka_solves_example(ctx, precision):
+    """test_u_2_p_1 (NSE3D example -3, Q2/Q1): the Oseen system with w = u is
+    solved exactly (errors of the interpolant ~ 0), in as many FGMRES
+    iterations as the oracle (FP64 Vanka factors) or close to it (FP32)."""
+    d, gl, ol, (r, ro, xe) = two_levels(ctx, 2)
+    g = gl[-1]
+    sol = pm.Solver(ctx, pm.solver_params(residual_tolerance=1e-11, residual_reduction=0.0, max_n_iterations=200,
+                                          precision=precision, **MG_KW))
+    sol.update_saddle(gl)
+    x = g.zeros()
+    nu, na = g.n_u, g.n_u_active
+    for k in range(3):
+        x[k * nu + na:(k + 1) * nu] = r[k * nu + na:(k + 1) * nu]
+    rep = sol.solve(r, x)
+    assert rep.converged
+    g.project_pressure(x)
+    e = g.errors(x)
+    assert e[0] < 1e-9 and e[1] < 1e-8 and e[3] < 1e-8, e
+    F = ol[-1]
+    x0 = np.zeros(F.ns.n)
+    for k in range(3):
+        x0[k * nu + na:(k + 1) * nu] = ro[k * nu + na:(k + 1) * nu]
+    xo, it = O.fgmres(lambda z: F.ns.apply(F.A, F.B, z),
+                      lambda v: O.nse_mg_apply(ol, v, damp_corr=0.8, vanka_damp=0.5, n_pre=2, n_post=2), ro, x0,
+                      restart=30, max_it=200, tol=1e-11)
+    if precision == "fp64":
+        assert abs(rep.iterations - it) <= 1, (rep.iterations, it)
+    else:
+        assert rep.iterations <= it + 5, (rep.iterations, it)
