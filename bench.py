@@ -48,6 +48,7 @@ def parse():
                     help="iteration cap of the omega = 0.8 check (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched TCD2D sweep")
+    ap.add_argument("--no-oseen", action="store_true", help="skip the Oseen Q2/Q1 solve")
     ap.add_argument("--profile-kernel", default=None,
                     choices=["apply_fp64", "jacobi_fp64", "apply_bf16", "jacobi_bf16"],
                     help="only assemble the finest level, launch this operator kernel 5 times and "
@@ -143,6 +144,81 @@ def batched_sweep(pm, ctx, n_problems=256, steps=20, seed=0x5EED):
             "failed_solves": int(rep.failed_solves), "block": int(rep.block),
             "host_setup_s": round(t_setup, 2),
             "method": "per-problem persistent workgroup: FP64 residual refinement around FP32 BiCGStab + Jacobi"}
+
+
+def fourier_field(x, seed=0x5EED, n_modes=8):
+    """Sum of n_modes divergence-free Fourier modes a_k cos(2 pi k.x + phi_k),
+    a_k orthogonal to k, integer wave vectors in [-2, 2]^3 (seeded)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    w = np.zeros((3, len(x)))
+    for _ in range(n_modes):
+        k = rng.integers(-2, 3, 3)
+        while not k.any():
+            k = rng.integers(-2, 3, 3)
+        a = rng.standard_normal(3)
+        a -= (a @ k) / (k @ k) * k
+        w += np.outer(a, np.cos(2 * math.pi * (x @ k) + rng.uniform(0, 2 * math.pi)))
+    return w
+
+
+def oseen_solve(pm, torch, ctx, nref=5, nlev=4, nu=1e-2, vanka_damp=0.2, steps=2):
+    """Oseen on the unit cube, 32^3 hexahedra, Taylor-Hood Q2/Q1 (NSTYPE 1),
+    nu = 1e-2, frozen convection = Q2 interpolant of 8 random divergence-free
+    Fourier modes, rhs and boundary data of test_u_2_p_1; FGMRES(30) + 4-level
+    multigrid V(2,2) with the cell-Vanka (Jacobi) smoother, correction damping
+    0.8, direct coarse solve, relative residual 1e-8."""
+    dom = pm.TDomain("Default_UnitCube_Hexa", nref)
+    t0 = time.perf_counter()
+    systems = []
+    for lev in range(nref - nlev + 1, nref + 1):
+        g = pm.NSESystem(ctx, dom, lev)
+        xyz = pm.TFESpace(dom, lev, 2, None).dof_coords()
+        w = torch.as_tensor(fourier_field(xyz).reshape(-1)).cuda()
+        r = g.zeros()
+        g.assemble("u2_p1", nu, w, r)
+        systems.append((g, r))
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    g, b = systems[-1]
+    nuu, na = g.n_u, g.n_u_active
+
+    def start():
+        x = g.zeros()
+        for k in range(3):
+            x[k * nuu + na:(k + 1) * nuu] = b[k * nuu + na:(k + 1) * nuu]
+        return x
+    r0 = b.clone()
+    g.apply_scaled_add(start(), r0, -1.0)
+    r0n = torch.linalg.norm(r0).item()
+    out = {"n_dof": g.n, "n_u": 3 * g.n_u, "n_p": g.n_p, "levels": nlev, "nnz_combined": int(g.info.nnz_combined),
+           "host_and_assembly_setup_s": round(t_setup, 3), "vanka_damp": vanka_damp}
+    for prec in ("mixed", "fp64"):
+        sol = pm.Solver(ctx, pm.solver_params(
+            iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30, residual_tolerance=0.0,
+            residual_reduction=1e-8, max_n_iterations=400, multigrid_cycle_type="V",
+            multigrid_smoother="cell_vanka_jacobi", multigrid_smoother_coarse="direct_solve",
+            multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2, multigrid_vanka_damp_factor=vanka_damp,
+            multigrid_correction_damp_factor=0.8, precision=prec))
+        t0 = time.perf_counter()
+        sol.update_saddle([s[0] for s in systems])
+        torch.cuda.synchronize()
+        t_up = time.perf_counter() - t0
+        ts = []
+        for _ in range(steps):
+            x = start()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rep = sol.solve(b, x)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        r = b.clone()
+        g.apply_scaled_add(x, r, -1.0)
+        out[prec] = {"time_to_solution_s": round(min(ts), 4), "iterations": rep.iterations,
+                     "fp64_rel_residual": torch.linalg.norm(r).item() / r0n, "solver_update_s": round(t_up, 3)}
+        del sol
+    out["speedup_fp32_vanka_vs_fp64"] = round(out["fp64"]["time_to_solution_s"] / out["mixed"]["time_to_solution_s"], 3)
+    return out
 
 
 def profile_kernel(pm, torch, ctx, dom, nref, kernel):
@@ -286,6 +362,7 @@ def main():
     spmv_fp64 = bytes_alg["apply_fp64"] / (kt["apply_fp64"] * 1e-3) / 1e9
 
     batch = None if args.no_batch else batched_sweep(pm, ctx)
+    oseen = None if args.no_oseen else oseen_solve(pm, torch, ctx)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -334,6 +411,7 @@ def main():
         "timed_region_s": round(t_all, 3),
         "jacobi_damp_0.8_check": damp08,
         "batched_tcd2d": batch,
+        "oseen_q2q1_32": oseen,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
