@@ -430,6 +430,11 @@ typedef struct {
   int precision;               /* PMX_PRECISION_* (multigrid preconditioner only; for
                                   saddle-point systems MIXED stores the Vanka factors in FP32) */
   double vanka_damp;           /* multigrid_vanka_damp_factor (default 1) */
+  int zero_start_shortcut;     /* no reference counterpart.  1 (default): a Jacobi
+                                  smoothing step on a level whose solution is zero on the
+                                  active DOFs (the first step after the cycle resets it)
+                                  multiplies only the Dirichlet-column entries; the result
+                                  is the same bit for bit.  0: always the full sweep */
 } pmx_solver_params;
 /* Defaults of Solver<>::default_solver_database (src/Solver/Solver.C:20-117)
  * merged with Multigrid::default_multigrid_database (src/Multigrid/Multigrid.C:422-514). */

@@ -78,7 +78,8 @@ class SolverParams(ctypes.Structure):
                 ("mg_smoother_coarse", c_int), ("mg_correction_damp", c_double),
                 ("mg_n_pre_smooth", c_int), ("mg_n_post_smooth", c_int),
                 ("mg_coarse_residual", c_double), ("mg_coarse_max_n_iterations", c_int),
-                ("jacobi_damp", c_double), ("precision", c_int), ("vanka_damp", c_double)]
+                ("jacobi_damp", c_double), ("precision", c_int), ("vanka_damp", c_double),
+                ("zero_start_shortcut", c_int)]
 
 
 class SolveReport(ctypes.Structure):

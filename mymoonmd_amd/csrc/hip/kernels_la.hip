@@ -171,6 +171,40 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
   }
 }
 
+// MODE_JACOBI of k_sell_spmv for an x that vanishes on the active rows: the
+// products with those zeros drop out of each row sum exactly (a*0 added to a
+// partial sum leaves it unchanged), so only the row's Dirichlet-column
+// entries are summed, in the same order and with the same values (read from
+// the SELL slots), and the result equals the full sweep bit for bit.  One
+// thread per row; reads b, D^-1 and writes x' (about 12 bytes per row in
+// FP32) instead of streaming the whole matrix.
+template <typename VT, typename XT>
+__global__ __launch_bounds__(256) void k_sell_jacobi_from_zero(
+    int n, int active, const int* __restrict__ bd_ptr, const int* __restrict__ bd_pos,
+    const int* __restrict__ bd_col, const VT* __restrict__ va, const float* __restrict__ rscale,
+    const XT* __restrict__ x, const XT* __restrict__ b, const XT* __restrict__ dinv,
+    XT* __restrict__ y, double a) {
+  using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i < active) {
+    AT acc = 0;
+    const int e = bd_ptr[i + 1];
+    for (int k = bd_ptr[i]; k < e; ++k) {
+      AT v;
+      if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(va[bd_pos[k]]);
+      else v = (AT)va[bd_pos[k]];
+      acc += v * (AT)x[bd_col[k]];
+    }
+    if (rscale) acc *= (AT)rscale[i];
+    const AT res = (AT)b[i] - acc;
+    y[i] = (XT)((AT)a * (AT)dinv[i] * res);
+  } else {
+    const AT res = (AT)(b[i] - x[i]);
+    y[i] = (XT)((AT)x[i] + (AT)a * (AT)dinv[i] * res);
+  }
+}
+
 int sell_grid(const DevCSR& A, int* sell_blocks) {
   *sell_blocks = (((A.sell.n_slices + 3) / 4 + 7) / 8) * 8;
   return *sell_blocks + (A.n - A.active + 255) / 256;
@@ -439,6 +473,25 @@ void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* 
                                                 st);
 }
 
+template <typename VT, typename XT>
+static bool jacobi_from_zero(const DevCSR& A, const VT* sell_vals, const float* rscale, const XT* x,
+                             const XT* b, const XT* dinv, XT* xn, double omega, hipStream_t st) {
+  if (!sell_vals || A.lpr != 0 || !A.sell.bd_ptr) return false;
+  if (A.n == 0) return true;
+  hipLaunchKernelGGL((k_sell_jacobi_from_zero<VT, XT>), dim3((A.n + 255) / 256), dim3(256), 0, st, A.n,
+                     A.active, A.sell.bd_ptr, A.sell.bd_pos, A.sell.bd_col, sell_vals, rscale, x, b,
+                     dinv, xn, omega);
+  return true;
+}
+bool fe_jacobi_from_zero(const DevCSR& A, const double* x, const double* b, const double* dinv,
+                         double* xn, double omega, hipStream_t st) {
+  return jacobi_from_zero<double, double>(A, A.sell.val, nullptr, x, b, dinv, xn, omega, st);
+}
+bool fe_jacobi_from_zero_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv,
+                              float* xn, double omega, hipStream_t st) {
+  return jacobi_from_zero<uint16_t, float>(A, A.sell.val_bf16, A.rscale, x, b, dinv, xn, omega, st);
+}
+
 // mixed precision: bf16 values with FP32 row scale, FP32 vectors
 void fe_apply_scaled_add_bf16(const DevCSR& A, const float* x, float* y, double a, hipStream_t st) {
   dispatch_fe_spmv<MODE_SCALED_ADD, uint16_t, float>(A, A.val_bf16, A.rscale, x, nullptr, nullptr,
@@ -460,10 +513,27 @@ void fe_jacobi_bf16(const DevCSR& A, const float* x, const float* b, const float
 template <typename T>
 void csr_spmv_t(const DevCSR& P, const T* x, T* y, double alpha, double beta, int row_end,
                 int zero_rest, hipStream_t st) {
-  const int lpr = 8;
+  // lanes per row: P.lpr (1, 2, 4 or 8; short interpolation rows use fewer)
+  const int lpr = (P.lpr == 1 || P.lpr == 2 || P.lpr == 4) ? P.lpr : 8;
   const int grid = (P.n + (256 / lpr) - 1) / (256 / lpr);
-  hipLaunchKernelGGL((k_csr_spmv<8, T>), dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, P.val, x,
-                     y, alpha, beta, row_end, zero_rest);
+  if (grid == 0) return;
+  switch (lpr) {
+    case 1:
+      hipLaunchKernelGGL((k_csr_spmv<1, T>), dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, P.val,
+                         x, y, alpha, beta, row_end, zero_rest);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_csr_spmv<2, T>), dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, P.val,
+                         x, y, alpha, beta, row_end, zero_rest);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_csr_spmv<4, T>), dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, P.val,
+                         x, y, alpha, beta, row_end, zero_rest);
+      break;
+    default:
+      hipLaunchKernelGGL((k_csr_spmv<8, T>), dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, P.val,
+                         x, y, alpha, beta, row_end, zero_rest);
+  }
 }
 void csr_spmv(const DevCSR& P, const double* x, double* y, double alpha, double beta, int row_end,
               int zero_rest, hipStream_t st) {

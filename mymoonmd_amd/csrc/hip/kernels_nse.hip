@@ -233,14 +233,19 @@ constexpr int NL = 3 * NU + NP;   // 89
 constexpr int LDL = NL + 1;       // LDS row stride
 
 // one workgroup (256 threads) per cell: gather, factor in LDS (FP64), store the
-// factors column-major (FP64 and/or FP32) and the pivots
+// factors column-major (FP64 and/or FP32) and, in place of the pivots, the
+// global DOF whose defect lands in local row i after all row interchanges
+// (the interchanges of dgetrs applied in factorisation order compose to one
+// permutation, so the solve gathers the permuted defect directly)
 __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __restrict__ map,
                                                       const int* __restrict__ rp, const int* __restrict__ ci,
                                                       const double* __restrict__ cv, double* __restrict__ lu64,
-                                                      float* __restrict__ lu32, int* __restrict__ piv,
+                                                      float* __restrict__ lu32, int* __restrict__ pmap,
                                                       int* __restrict__ singular) {
   __shared__ double M[NL * LDL];
   __shared__ int dm[NL];
+  __shared__ int pk[NL];
+  __shared__ int src[NL];
   __shared__ int sp;
   const int cell = blockIdx.x;
   if (cell >= n_cells) return;   // uniform for the workgroup
@@ -289,7 +294,7 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
     __syncthreads();
     const double d = M[k * LDL + k];
     if (t == 0) {
-      piv[(size_t)cell * NL + k] = p;
+      pk[k] = p;
       if (d == 0.0) atomicOr(singular, 1);
     }
     const double inv = (d != 0.0) ? 1.0 / d : 0.0;
@@ -302,6 +307,19 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
     }
     __syncthreads();
   }
+  if (t == 0) {
+    // compose the interchanges: local row i of the permuted system holds the
+    // defect of local DOF src[i]
+    for (int i = 0; i < NL; ++i) src[i] = i;
+    for (int k = 0; k < NL; ++k) {
+      const int p = pk[k];
+      const int a = src[k];
+      src[k] = src[p];
+      src[p] = a;
+    }
+  }
+  __syncthreads();
+  for (int i = t; i < NL; i += blockDim.x) pmap[(size_t)cell * NL + i] = dm[src[i]];
   // column-major store: element (i, j) at j * NL + i
   for (int e = t; e < NL * NL; e += blockDim.x) {
     const int j = e / NL, i = e % NL;
@@ -312,48 +330,71 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
 }
 
 // one wavefront per cell: the local rhs lives in registers (rows lane and
-// lane + 64), forward and backward substitution broadcast the pivot row value
+// lane + 64), gathered already permuted; forward and backward substitution
+// broadcast the solved component.  The factor columns of the next U steps are
+// loaded before those steps run: their addresses do not depend on the
+// solution, so the loads overlap instead of exposing one latency per step.
 template <typename T>
-__global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const int* __restrict__ map,
-                                                     const T* __restrict__ lu, const int* __restrict__ piv,
+__global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const T* __restrict__ lu,
+                                                     const int* __restrict__ pmap,
                                                      const double* __restrict__ r, double damp,
                                                      double* __restrict__ upd) {
+  constexpr int U = 8;
   const int lane = threadIdx.x & 63;
   const int cell = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
   if (cell >= n_cells) return;   // uniform for the wavefront
-  const int* dm = map + (size_t)cell * NL;
   const T* L = lu + (size_t)cell * NL * NL;
-  const int* pv = piv + (size_t)cell * NL;
+  const int* pm = pmap + (size_t)cell * NL;
   const int i0 = lane, i1 = lane + 64;
   const bool h1 = i1 < NL;
-  double b0 = r[dm[i0]];
-  double b1 = h1 ? r[dm[i1]] : 0.0;
-  // row interchanges in factorisation order
-  for (int k = 0; k < NL; ++k) {
-    const int p = pv[k];
-    if (p != k) {
-      const double vk = __shfl(k < 64 ? b0 : b1, k & 63, 64);
-      const double vp = __shfl(p < 64 ? b0 : b1, p & 63, 64);
-      if (i0 == k) b0 = vp; else if (i0 == p) b0 = vk;
-      if (i1 == k) b1 = vp; else if (i1 == p) b1 = vk;
+  double b0 = r[pm[i0]];
+  double b1 = h1 ? r[pm[i1]] : 0.0;
+  // L y = b (unit diagonal), columns 0 .. NL-2
+  static_assert((NL - 1) % U == 0, "column blocks");
+  for (int k0 = 0; k0 < NL - 1; k0 += U) {
+    T l0[U], l1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T* col = L + (size_t)(k0 + u) * NL;
+      l0[u] = col[i0];
+      l1[u] = h1 ? col[i1] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u;
+      const double bk = __shfl(k < 64 ? b0 : b1, k & 63, 64);
+      if (i0 > k) b0 -= (double)l0[u] * bk;
+      if (h1 && i1 > k) b1 -= (double)l1[u] * bk;
     }
   }
-  // L y = b (unit diagonal)
-  for (int k = 0; k < NL - 1; ++k) {
-    const double bk = __shfl(k < 64 ? b0 : b1, k & 63, 64);
-    const T* col = L + (size_t)k * NL;
-    if (i0 > k) b0 -= (double)col[i0] * bk;
-    if (h1 && i1 > k) b1 -= (double)col[i1] * bk;
-  }
-  // U x = y
-  for (int k = NL - 1; k >= 0; --k) {
-    const T* col = L + (size_t)k * NL;
-    const double bk = __shfl(k < 64 ? b0 : b1, k & 63, 64);
-    const double xk = bk / (double)col[k];
-    if (i0 == k) b0 = xk;
-    if (i1 == k) b1 = xk;
-    if (i0 < k) b0 -= (double)col[i0] * xk;
-    if (h1 && i1 < k) b1 -= (double)col[i1] * xk;
+  // U x = y, columns NL-1 .. 0
+  for (int k0 = NL - 1; k0 >= 0; k0 -= U) {
+    T l0[U], l1[U], dg[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        const T* col = L + (size_t)k * NL;
+        l0[u] = col[i0];
+        l1[u] = h1 ? col[i1] : T(0);
+        dg[u] = col[k];
+      } else {
+        l0[u] = l1[u] = T(0);
+        dg[u] = T(1);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 - u;
+      if (k >= 0) {
+        const double bk = __shfl(k < 64 ? b0 : b1, k & 63, 64);
+        const double xk = bk / (double)dg[u];
+        if (i0 == k) b0 = xk;
+        if (i1 == k) b1 = xk;
+        if (i0 < k) b0 -= (double)l0[u] * xk;
+        if (h1 && i1 < k) b1 -= (double)l1[u] * xk;
+      }
+    }
   }
   upd[(size_t)cell * NL + i0] = damp * b0;
   if (h1) upd[(size_t)cell * NL + i1] = damp * b1;
@@ -390,20 +431,20 @@ void combined_values(long long nnz, const uint8_t* kind, const int* pos, const d
     hipLaunchKernelGGL(k_combined_values, dim3(4096), dim3(256), 0, st, nnz, kind, pos, A, B1, B2, B3, out);
 }
 
-void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, float* lu32, int* piv,
+void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, float* lu32, int* pmap,
                   int* singular, hipStream_t st) {
   if (n_cells > 0)
     hipLaunchKernelGGL(k_vanka_factor, dim3(n_cells), dim3(256), 0, st, n_cells, map, C.row_ptr, C.col, C.val,
-                       lu64, lu32, piv, singular);
+                       lu64, lu32, pmap, singular);
 }
 
-void vanka_solve(int n_cells, const int* map, const double* lu64, const float* lu32, const int* piv,
-                 const double* r, double damp, double* upd, hipStream_t st) {
+void vanka_solve(int n_cells, const double* lu64, const float* lu32, const int* pmap, const double* r,
+                 double damp, double* upd, hipStream_t st) {
   if (n_cells <= 0) return;
   const int grid = (n_cells + 3) / 4;
-  if (lu32) hipLaunchKernelGGL((k_vanka_solve<float>), dim3(grid), dim3(256), 0, st, n_cells, map, lu32, piv, r,
+  if (lu32) hipLaunchKernelGGL((k_vanka_solve<float>), dim3(grid), dim3(256), 0, st, n_cells, lu32, pmap, r,
                                damp, upd);
-  else hipLaunchKernelGGL((k_vanka_solve<double>), dim3(grid), dim3(256), 0, st, n_cells, map, lu64, piv, r, damp,
+  else hipLaunchKernelGGL((k_vanka_solve<double>), dim3(grid), dim3(256), 0, st, n_cells, lu64, pmap, r, damp,
                           upd);
 }
 

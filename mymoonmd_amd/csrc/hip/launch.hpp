@@ -18,6 +18,10 @@ struct DevSELL {
   int* src = nullptr;             // [total] CSR position (-1: padding)
   double* val = nullptr;          // FP64 values (nullptr: not built)
   uint16_t* val_bf16 = nullptr;   // bf16 values divided by the row scale
+  // active-row entries in Dirichlet columns (row order): SELL slot, column
+  int* bd_ptr = nullptr;          // [active+1] (nullptr: not built)
+  int* bd_pos = nullptr;
+  int* bd_col = nullptr;
 };
 
 // Device CSR matrix (0-based int32 row pointer / columns, sorted columns).
@@ -47,12 +51,19 @@ void fe_defect(const DevCSR& A, const double* x, const double* b, double* d, dou
                double* norm_out, hipStream_t st);
 void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* dinv, double* xn,
                double omega, hipStream_t st);
+// the same sweep for an x that is zero on the active rows (only the products
+// with the Dirichlet entries of x remain); bitwise equal to fe_jacobi on such
+// an x.  Returns false (and launches nothing) when A has no SELL copy.
+bool fe_jacobi_from_zero(const DevCSR& A, const double* x, const double* b, const double* dinv,
+                         double* xn, double omega, hipStream_t st);
 // ---- FE-matrix operators (bf16 matrix, FP32 vectors)
 void fe_apply_scaled_add_bf16(const DevCSR& A, const float* x, float* y, double a, hipStream_t st);
 void fe_defect_bf16(const DevCSR& A, const float* x, const float* b, float* d, double* partial,
                     double* norm_out, hipStream_t st);
 void fe_jacobi_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv, float* xn,
                     double omega, hipStream_t st);
+bool fe_jacobi_from_zero_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv,
+                              float* xn, double omega, hipStream_t st);
 // ---- plain CSR (transfer matrices): y = alpha*P x + beta*y on rows < row_end;
 //      rows >= row_end are set to 0 when zero_rest, else left unchanged
 void csr_spmv(const DevCSR& P, const double* x, double* y, double alpha, double beta, int row_end,
@@ -146,10 +157,14 @@ void pressure_project_l20(int n_cells, int nq, const double* ref, const double* 
                           double* p, double* partial, hipStream_t st);
 // cell Vanka (Jacobi variant) on Q2/Q1 hexahedra: 89 local DOFs per cell
 constexpr int kVankaLocal = 89;
-void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, float* lu32, int* piv,
+// factor: LU of every cell's local matrix (gathered from the combined matrix C
+// by the cell's 89 sorted DOFs `map`); pmap[cell][i] = global DOF whose
+// defect enters row i of the row-interchanged local system
+void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, float* lu32, int* pmap,
                   int* singular, hipStream_t st);
-void vanka_solve(int n_cells, const int* map, const double* lu64, const float* lu32, const int* piv,
-                 const double* r, double damp, double* upd, hipStream_t st);
+// upd[cell][i] = damp * (local solution) for local DOF i (the order of map)
+void vanka_solve(int n_cells, const double* lu64, const float* lu32, const int* pmap, const double* r,
+                 double damp, double* upd, hipStream_t st);
 void vanka_accumulate(int n, const int* acc_ptr, const int* acc_idx, const double* upd, double* x,
                       hipStream_t st);
 

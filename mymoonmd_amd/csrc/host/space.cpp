@@ -383,6 +383,28 @@ SellPattern build_sell(const CSRPattern& p, int sigma) {
       }
     }
   }
+  // slot of every (row, j) entry whose column is a Dirichlet DOF
+  std::vector<int> slot_of(na);
+  for (int k = 0; k < na; ++k) slot_of[s.row[k]] = k;
+  s.bd_ptr.assign(na + 1, 0);
+  for (int r = 0; r < na; ++r) {
+    int c = 0;
+    for (int q = p.row_ptr[r]; q < p.row_ptr[r + 1]; ++q) c += p.col[q] >= na;
+    s.bd_ptr[r + 1] = s.bd_ptr[r] + c;
+  }
+  s.bd_pos.resize(s.bd_ptr[na]);
+  s.bd_col.resize(s.bd_ptr[na]);
+  for (int r = 0; r < na; ++r) {
+    const int k = slot_of[r];
+    int o = s.bd_ptr[r];
+    for (int j = 0; j < len(r); ++j) {
+      const int c = p.col[p.row_ptr[r] + j];
+      if (c < na) continue;
+      s.bd_pos[o] = s.slice_ptr[k / C] + j * C + (k % C);
+      s.bd_col[o] = c;
+      ++o;
+    }
+  }
   return s;
 }
 

@@ -104,6 +104,11 @@ struct SellPattern {
   std::vector<int> row;         // [n_slices*64] original row of each slot (-1: empty)
   std::vector<int> col;         // [slice_ptr.back()]
   std::vector<int> src;         // [slice_ptr.back()] CSR position of the entry (-1: padding)
+  // entries of the active rows in Dirichlet columns (col >= n_active), per
+  // row in column order: their slot in the SELL arrays and their column
+  std::vector<int> bd_ptr;      // [n_active+1]
+  std::vector<int> bd_pos;
+  std::vector<int> bd_col;
 };
 SellPattern build_sell(const CSRPattern& p, int sigma);
 

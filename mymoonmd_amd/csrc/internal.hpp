@@ -126,6 +126,7 @@ struct pmx_matrix {
     dfree(A.row_ptr); dfree(A.col); dfree(A.val); dfree(A.val_bf16); dfree(A.rscale);
     dfree(A.sell.slice_ptr); dfree(A.sell.row); dfree(A.sell.col); dfree(A.sell.src);
     dfree(A.sell.val); dfree(A.sell.val_bf16);
+    dfree(A.sell.bd_ptr); dfree(A.sell.bd_pos); dfree(A.sell.bd_col);
   }
 };
 

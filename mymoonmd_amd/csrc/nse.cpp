@@ -37,6 +37,8 @@ pmx_nse::~pmx_nse() {
                                               d_B2, d_B3, d_BTrp, d_BTcol, d_BTsrc, d_BT1, d_BT2, d_BT3, d_Crp,
                                               d_Ccol, d_Cval, d_Ckind, d_Cpos, d_partial, d_pref, d_pgeo})
     dfree(p);
+  for (void* p : std::initializer_list<void*>{Csell.slice_ptr, Csell.row, Csell.col, Csell.src, Csell.val})
+    dfree(p);
   for (auto& kv : err_ref) dfree(kv.second);
 }
 
@@ -73,10 +75,11 @@ pmx::DevCSR nse_combined(const pmx_nse* s) {
   C.n = C.m = s->n_comb;
   C.nnz = (int)s->Cp.col.size();
   C.active = s->n_comb;
-  C.lpr = 64;
+  C.lpr = s->Csell.val ? 0 : 64;   // 0: the FE-matrix kernels use the SELL copy
   C.row_ptr = s->d_Crp;
   C.col = s->d_Ccol;
   C.val = s->d_Cval;
+  C.sell = s->Csell;
   return C;
 }
 
@@ -101,6 +104,7 @@ void nse_refresh(pmx_nse* s, hipStream_t st) {
   pmx::sell_gather(nbt, s->d_BTsrc, s->d_B3, s->d_BT3, nullptr, st);
   pmx::combined_values((long long)s->Cp.col.size(), s->d_Ckind, s->d_Cpos, s->A->A.val, s->d_B1, s->d_B2, s->d_B3,
                        s->d_Cval, st);
+  pmx::sell_fill_f64(nse_combined(s), s->Csell.val, st);
   s->values_ready = true;
 }
 
@@ -280,6 +284,16 @@ int pmx_nse_create(pmx_ctx* ctx, const pmx_domain* d, int level, pmx_nse** out) 
     s->d_Crp = upload(s->Cp.row_ptr.data(), s->Cp.row_ptr.size(), st);
     s->d_Ccol = upload(s->Cp.col.data(), s->Cp.col.size(), st);
     s->d_Cval = dmalloc<double>(s->Cp.col.size());
+    {
+      const pmx::SellPattern sp = pmx::build_sell(s->Cp, 4096);
+      s->Csell.n_slices = sp.n_slices;
+      s->Csell.total = sp.col.size();
+      s->Csell.slice_ptr = upload(sp.slice_ptr.data(), sp.slice_ptr.size(), st);
+      s->Csell.row = upload(sp.row.data(), sp.row.size(), st);
+      s->Csell.col = upload(sp.col.data(), sp.col.size(), st);
+      s->Csell.src = upload(sp.src.data(), sp.src.size(), st);
+      s->Csell.val = dmalloc<double>(s->Csell.total);
+    }
     s->d_Ckind = upload(s->Ckind.data(), s->Ckind.size(), st);
     s->d_Cpos = upload(s->Cpos.data(), s->Cpos.size(), st);
     s->d_partial = dmalloc<double>((size_t)2 * ((nc + 255) / 256) + 8);

@@ -59,7 +59,9 @@ pmx::DevCSR upload_csr(const pmx::CSRPattern& p, const std::vector<double>& v, h
   M.m = p.n_cols;
   M.nnz = (int)p.col.size();
   M.active = p.active_bound;
-  M.lpr = 8;
+  // lanes per row of the transfer kernel from the mean row length
+  const double avg = M.n ? (double)M.nnz / M.n : 0.0;
+  M.lpr = avg <= 2.5 ? 1 : avg <= 5.0 ? 2 : avg <= 10.0 ? 4 : 8;
   M.row_ptr = upload(p.row_ptr.data(), p.row_ptr.size(), st);
   M.col = upload(p.col.data(), p.col.size(), st);
   M.val = upload(v.data(), v.size(), st);
@@ -71,15 +73,16 @@ pmx::DevCSR upload_csr(const pmx::CSRPattern& p, const std::vector<double>& v, h
 struct NseLevel {
   pmx_nse* sys = nullptr;
   int n = 0, n_u = 0, n_ua = 0, n_p = 0, n_cells = 0;
-  int *map = nullptr, *piv = nullptr, *acc_ptr = nullptr, *acc_idx = nullptr;
+  int *map = nullptr, *pmap = nullptr, *acc_ptr = nullptr, *acc_idx = nullptr;
   double* lu64 = nullptr;
   float* lu32 = nullptr;
   double *upd = nullptr, *rhs = nullptr, *sol = nullptr, *def = nullptr, *res = nullptr;
   pmx::DevCSR Pu{}, Ru{}, Pp{}, Rp{};   // P: finer rows from this level; R = P^T (rows: this level)
   double* dinv_dense = nullptr;          // coarsest level: inverse of the combined matrix
   double* partial = nullptr;
+  bool sol_zero = false;                 // sol is zero (set by the cycle's resets)
   void release() {
-    dfree(map); dfree(piv); dfree(acc_ptr); dfree(acc_idx); dfree(lu64); dfree(lu32); dfree(upd);
+    dfree(map); dfree(pmap); dfree(acc_ptr); dfree(acc_idx); dfree(lu64); dfree(lu32); dfree(upd);
     dfree(rhs); dfree(sol); dfree(def); dfree(res);
     free_csr(Pu); free_csr(Ru); free_csr(Pp); free_csr(Rp);
     dfree(dinv_dense); dfree(partial);
@@ -97,6 +100,7 @@ struct Level {
   double *rhs = nullptr, *sol = nullptr, *def = nullptr, *tmp = nullptr;
   float *rhs32 = nullptr, *sol32 = nullptr, *def32 = nullptr, *tmp32 = nullptr;
   double* partial = nullptr;
+  bool sol_zero = false;     // sol vanishes on the active DOFs (set by the cycle's resets)
   void release() {
     dfree(dinv); dfree(dinv32);
     free_csr(P); free_csr(R); dfree(dinv_dense);
@@ -178,15 +182,21 @@ bool converged(pmx_solver* s, double res, int it) {
 void mg_smooth(pmx_solver* s, int l, bool mixed) {
   Level& L = s->lv[l];
   if (l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) {
+    L.sol_zero = false;
     if (mixed) pmx::dense_mv_f32(L.n, L.dinv_dense, L.rhs32, L.sol32, S(s));
     else pmx::dense_mv(L.n, L.dinv_dense, L.rhs, L.sol, S(s));
     return;
   }
+  const bool zero = L.sol_zero && s->p.zero_start_shortcut;
+  L.sol_zero = false;
   if (mixed) {
-    pmx::fe_jacobi_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp, S(s));
+    if (!(zero && pmx::fe_jacobi_from_zero_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32,
+                                                s->p.jacobi_damp, S(s))))
+      pmx::fe_jacobi_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp, S(s));
     std::swap(L.sol32, L.tmp32);
   } else {
-    pmx::fe_jacobi(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s));
+    if (!(zero && pmx::fe_jacobi_from_zero(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s))))
+      pmx::fe_jacobi(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s));
     std::swap(L.sol, L.tmp);
   }
 }
@@ -201,6 +211,7 @@ void mg_update_solution_in_finer_grid(pmx_solver* s, int l, bool mixed) {
   Level& C = s->lv[l];
   Level& F = s->lv[l + 1];
   const double damp = s->p.mg_correction_damp;
+  F.sol_zero = false;
   if (mixed) pmx::csr_spmv_f32(C.P, C.sol32, F.sol32, damp, 1.0, F.active, 0, S(s));
   else pmx::csr_spmv(C.P, C.sol, F.sol, damp, 1.0, F.active, 0, S(s));
 }
@@ -243,6 +254,7 @@ int mg_cycle_step(pmx_solver* s, int step, int level, bool mixed) {
     Level& C = s->lv[level - 1];
     if (mixed) pmx::vec_fill_f32(C.n, 0.f, C.sol32, S(s));
     else pmx::vec_fill(C.n, 0.0, C.sol, S(s));
+    C.sol_zero = true;
     mg_update_rhs_in_coarser_grid(s, level, mixed);
     return level - 1;
   } else if (up) {
@@ -266,6 +278,7 @@ void mg_cycle(pmx_solver* s, bool mixed) {
       HIP_CHECK(hipMemcpyAsync(F.sol + F.active, F.rhs + F.active, (F.n - F.active) * sizeof(double),
                                hipMemcpyDeviceToDevice, S(s)));
   }
+  F.sol_zero = true;
   int level = (int)s->lv.size() - 1;
   const int n_steps = (int)s->cc.size() - 1;
   for (int step = 1; step <= n_steps; ++step) level = mg_cycle_step(s, step, level, mixed);
@@ -294,12 +307,19 @@ void mg_apply(pmx_solver* s, const double* v, double* z) {
 void nse_smooth(pmx_solver* s, int l) {
   NseLevel& L = s->nlv[l];
   if (l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) {
+    L.sol_zero = false;
     pmx::dense_mv(L.n, L.dinv_dense, L.rhs, L.sol, S(s));
     return;
   }
-  HIP_CHECK(hipMemcpyAsync(L.res, L.rhs, L.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
-  pmx::csr_spmv(nse_combined(L.sys), L.sol, L.res, -1.0, 1.0, L.n, 0, S(s));
-  pmx::vanka_solve(L.n_cells, L.map, L.lu64, L.lu32, L.piv, L.res, s->p.vanka_damp, L.upd, S(s));
+  // local defects come from r = rhs - K sol; for sol = 0 that is rhs itself
+  // (K 0 sums signed zeros), so the product is skipped
+  const double* r = L.rhs;
+  if (!(L.sol_zero && s->p.zero_start_shortcut)) {
+    pmx::fe_defect(nse_combined(L.sys), L.sol, L.rhs, L.res, L.partial, nullptr, S(s));
+    r = L.res;
+  }
+  L.sol_zero = false;
+  pmx::vanka_solve(L.n_cells, L.lu64, L.lu32, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
   pmx::vanka_accumulate(L.n, L.acc_ptr, L.acc_idx, L.upd, L.sol, S(s));
 }
 
@@ -314,6 +334,7 @@ void nse_update_solution_in_finer_grid(pmx_solver* s, int l) {
   NseLevel& C = s->nlv[l];
   NseLevel& F = s->nlv[l + 1];
   const double damp = s->p.mg_correction_damp;
+  F.sol_zero = false;
   for (int k = 0; k < 3; ++k)
     pmx::csr_spmv(C.Pu, C.sol + (size_t)k * C.n_u, F.sol + (size_t)k * F.n_u, damp, 1.0, F.n_ua, 0, S(s));
   pmx::csr_spmv(C.Pp, C.sol + (size_t)3 * C.n_u, F.sol + (size_t)3 * F.n_u, damp, 1.0, F.n_p, 0, S(s));
@@ -353,6 +374,7 @@ int nse_cycle_step(pmx_solver* s, int step, int level) {
     nse_defect(s, level, nullptr);
     NseLevel& C = s->nlv[level - 1];
     pmx::vec_fill(C.n, 0.0, C.sol, S(s));
+    C.sol_zero = true;
     nse_update_rhs_in_coarser_grid(s, level);
     return level - 1;
   } else if (up) {
@@ -658,6 +680,7 @@ int pmx_solver_default_params(pmx_solver_params* p) {
     d.jacobi_damp = 1.0;
     d.precision = PMX_PRECISION_FP64;
     d.vanka_damp = 1.0;
+    d.zero_start_shortcut = 1;
     *p = d;
   });
 }
@@ -827,7 +850,7 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
         L.sol = dzeros<double>(L.n, st);
         L.def = dzeros<double>(L.n, st);
         L.res = dzeros<double>(L.n, st);
-        L.partial = dmalloc<double>(4096);
+        L.partial = dmalloc<double>(std::max(4096, pmx::fe_defect_partials(nse_combined(sys))));
         const bool vanka = !(l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT);
         if (vanka) {
           // local DOF maps: [u1 | u2 | u3 | p] of every cell, sorted per block
@@ -856,9 +879,9 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
           const size_t nlu = (size_t)L.n_cells * NLc * NLc;
           if (f32) L.lu32 = dmalloc<float>(nlu);
           else L.lu64 = dmalloc<double>(nlu);
-          L.piv = dmalloc<int>((size_t)L.n_cells * NLc);
+          L.pmap = dmalloc<int>((size_t)L.n_cells * NLc);
           L.upd = dmalloc<double>((size_t)L.n_cells * NLc);
-          pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.piv, sing, st);
+          pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.pmap, sing, st);
         } else {
           L.dinv_dense = dense_inverse_of(s, L.n, sys->Cp.row_ptr, sys->Cp.col, sys->d_Cval);
         }

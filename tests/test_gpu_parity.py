@@ -253,6 +253,39 @@ def test_multigrid_cycle_matches_oracle(ctx):
         assert np.max(np.abs(zg - zo)) <= 1e-10 * np.max(np.abs(zo)), coarse
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("cycle", ["V", "W"])
+def test_zero_start_sweep_is_bitwise_the_full_sweep(ctx, precision, cycle):
+    """The first Jacobi step on a level whose solution was reset multiplies
+    only the Dirichlet-column entries; cycles and solves with and without the
+    shortcut agree bit for bit (the preconditioned vector has non-zero
+    Dirichlet entries, so the Dirichlet couplings are exercised)."""
+    nref = 3
+    d, spaces, mats, rhs = three_level_cd3d(ctx, nref, MG_KW)
+    fs = spaces[-1]
+    out = {}
+    for shortcut in (0, 1):
+        for coarse in ("jacobi", "direct_solve"):
+            prm = pm.solver_params(iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
+                                   residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=100,
+                                   multigrid_cycle_type=cycle, multigrid_smoother="jacobi",
+                                   multigrid_smoother_coarse=coarse, multigrid_n_pre_smooth=2,
+                                   multigrid_n_post_smooth=2, jacobi_damp=0.5, precision=precision,
+                                   zero_start_shortcut=shortcut)
+            sol = pm.Solver(ctx, prm)
+            sol.update_matrix(mats, spaces)
+            z = torch.zeros_like(rhs[-1])
+            sol.apply_preconditioner(rhs[-1].clone(), z)
+            x = torch.zeros_like(rhs[-1])
+            x[fs.active_bound:] = rhs[-1][fs.active_bound:]
+            rep = sol.solve(rhs[-1], x)
+            out[(shortcut, coarse)] = (z.cpu().numpy(), x.cpu().numpy(), rep.iterations)
+    for coarse in ("jacobi", "direct_solve"):
+        a, b = out[(0, coarse)], out[(1, coarse)]
+        assert np.array_equal(a[0], b[0]), coarse
+        assert np.array_equal(a[1], b[1]) and a[2] == b[2], coarse
+
+
 def test_fgmres_multigrid_solution_matches_oracle(ctx):
     """FGMRES(30) + MG V(2,2) to 1e-8.  Damping 0.5 keeps the Jacobi smoother
     contractive for this Q2 SUPG operator (lambda_max(D^-1 A) ~ 3), so the
