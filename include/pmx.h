@@ -316,6 +316,15 @@ int pmx_nse_get_global_numbers(const pmx_nse* s, int* host_velocity, int* host_p
 int pmx_nse_interpolate(pmx_ctx* ctx, const pmx_nse* s, int example, double* dev_x);
 /* TFEFunction3D::project_into_L20 of the pressure block (FEFunction3D.C:1678-1725) */
 int pmx_nse_project_pressure(pmx_ctx* ctx, pmx_nse* s, double* dev_x);
+/* GridTransfer::RestrictFunction (src/Multigrid/GridTransfer.C:1218-1501) of the
+ * three velocity components, as NavierStokes<3>::assemble_nonlinear_term calls
+ * it through RestrictFunctionRepeatedly (src/System/NavierStokes.C:598-612):
+ * dev_coarse_u[0 .. 3 n_u(coarse)) = R dev_fine_u[0 .. 3 n_u(fine)), where per
+ * coarse cell the fine function is projected locally in L2 onto the coarse
+ * element and the cell values of a DOF are averaged.  `fine` is the next finer
+ * level of the same domain; the operator is built on the first call. */
+int pmx_nse_restrict_velocity(pmx_ctx* ctx, pmx_nse* coarse, const pmx_nse* fine, const double* dev_fine_u,
+                              double* dev_coarse_u);
 /* NavierStokes<3>::get_errors layout: [0] L2 u, [1] H1-semi u, [2] unused,
  * [3] L2 p, [4] H1-semi p */
 int pmx_nse_errors(pmx_ctx* ctx, pmx_nse* s, int example, const double* dev_x, double* host_err5);
@@ -435,6 +444,12 @@ typedef struct {
                                   active DOFs (the first step after the cycle resets it)
                                   multiplies only the Dirichlet-column entries; the result
                                   is the same bit for bit.  0: always the full sweep */
+  int renumber_levels;         /* no reference counterpart.  1 (default): the multigrid
+                                  levels above the coarsest run in a solver-internal
+                                  lexicographic DOF order with the entries of each row
+                                  sorted by the new column (better gather locality; the
+                                  row sums of the preconditioner are rounded in another
+                                  order).  0: the reference order throughout */
 } pmx_solver_params;
 /* Defaults of Solver<>::default_solver_database (src/Solver/Solver.C:20-117)
  * merged with Multigrid::default_multigrid_database (src/Multigrid/Multigrid.C:422-514). */
@@ -472,6 +487,13 @@ int pmx_solver_solve(pmx_solver* s, const double* dev_b, double* dev_x, pmx_solv
 int pmx_solver_get_history(const pmx_solver* s, double* host_out, int max, int* n);
 /* One multigrid cycle as preconditioner: dev_z = MG(dev_r) (Iteration_multigrid::apply) */
 int pmx_solver_apply_preconditioner(pmx_solver* s, const double* dev_r, double* dev_z);
+/* No reference counterpart (instrumentation): mean duration (HIP events on the
+ * solver's stream, `reps` launches) of the Jacobi smoothing sweep of multigrid
+ * level `level` (coarsest = 0) exactly as a cycle runs it: the level's own
+ * (possibly renumbered) matrix copy and vectors.  kernel = PMX_KERNEL_JACOBI_FP64
+ * for an FP64 solver, PMX_KERNEL_JACOBI_BF16 for a mixed one.  The level's
+ * scratch vector is overwritten. */
+int pmx_solver_time_level_kernel(pmx_solver* s, int level, int kernel, int reps, double* avg_ms);
 
 #ifdef __cplusplus
 }

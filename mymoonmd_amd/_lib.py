@@ -79,7 +79,7 @@ class SolverParams(ctypes.Structure):
                 ("mg_n_pre_smooth", c_int), ("mg_n_post_smooth", c_int),
                 ("mg_coarse_residual", c_double), ("mg_coarse_max_n_iterations", c_int),
                 ("jacobi_damp", c_double), ("precision", c_int), ("vanka_damp", c_double),
-                ("zero_start_shortcut", c_int)]
+                ("zero_start_shortcut", c_int), ("renumber_levels", c_int)]
 
 
 class SolveReport(ctypes.Structure):
@@ -162,6 +162,7 @@ SIGNATURES = {
     "pmx_nse_get_global_numbers": [c_void_p, P(c_int), P(c_int)],
     "pmx_nse_interpolate": [c_void_p, c_void_p, c_int, c_void_p],
     "pmx_nse_project_pressure": [c_void_p, c_void_p, c_void_p],
+    "pmx_nse_restrict_velocity": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "pmx_nse_errors": [c_void_p, c_void_p, c_int, c_void_p, P(c_double)],
     "pmx_solver_default_params": [P(SolverParams)],
     "pmx_solver_create": [c_void_p, P(SolverParams), P(c_void_p)],
@@ -171,6 +172,7 @@ SIGNATURES = {
     "pmx_solver_solve": [c_void_p, c_void_p, c_void_p, P(SolveReport)],
     "pmx_solver_get_history": [c_void_p, P(c_double), c_int, P(c_int)],
     "pmx_solver_apply_preconditioner": [c_void_p, c_void_p, c_void_p],
+    "pmx_solver_time_level_kernel": [c_void_p, c_int, c_int, c_int, P(c_double)],
 }
 _RESTYPES = {"pmx_last_error": ctypes.c_char_p}
 

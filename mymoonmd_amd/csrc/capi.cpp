@@ -33,6 +33,7 @@ void attach_sell(pmx_matrix* M, hipStream_t st) {
   S.row = upload(sp.row.data(), sp.row.size(), st);
   S.col = upload(sp.col.data(), sp.col.size(), st);
   S.src = upload(sp.src.data(), sp.src.size(), st);
+  S.n_bd = (int)sp.bd_pos.size();
   S.bd_ptr = upload(sp.bd_ptr.data(), sp.bd_ptr.size(), st);
   S.bd_pos = sp.bd_pos.empty() ? nullptr : upload(sp.bd_pos.data(), sp.bd_pos.size(), st);
   S.bd_col = sp.bd_col.empty() ? nullptr : upload(sp.bd_col.data(), sp.bd_col.size(), st);

@@ -174,14 +174,14 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
 // MODE_JACOBI of k_sell_spmv for an x that vanishes on the active rows: the
 // products with those zeros drop out of each row sum exactly (a*0 added to a
 // partial sum leaves it unchanged), so only the row's Dirichlet-column
-// entries are summed, in the same order and with the same values (read from
-// the SELL slots), and the result equals the full sweep bit for bit.  One
-// thread per row; reads b, D^-1 and writes x' (about 12 bytes per row in
-// FP32) instead of streaming the whole matrix.
+// entries are summed, in the same order and with the same values (a compact
+// copy of those SELL slots), and the result equals the full sweep bit for
+// bit.  One thread per row; reads b, D^-1 and writes x' (about 16 bytes per
+// row in FP32) instead of streaming the whole matrix.
 template <typename VT, typename XT>
 __global__ __launch_bounds__(256) void k_sell_jacobi_from_zero(
-    int n, int active, const int* __restrict__ bd_ptr, const int* __restrict__ bd_pos,
-    const int* __restrict__ bd_col, const VT* __restrict__ va, const float* __restrict__ rscale,
+    int n, int active, const int* __restrict__ bd_ptr, const int* __restrict__ bd_col,
+    const VT* __restrict__ bd_val, const float* __restrict__ rscale,
     const XT* __restrict__ x, const XT* __restrict__ b, const XT* __restrict__ dinv,
     XT* __restrict__ y, double a) {
   using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
@@ -192,8 +192,8 @@ __global__ __launch_bounds__(256) void k_sell_jacobi_from_zero(
     const int e = bd_ptr[i + 1];
     for (int k = bd_ptr[i]; k < e; ++k) {
       AT v;
-      if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(va[bd_pos[k]]);
-      else v = (AT)va[bd_pos[k]];
+      if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(bd_val[k]);
+      else v = (AT)bd_val[k];
       acc += v * (AT)x[bd_col[k]];
     }
     if (rscale) acc *= (AT)rscale[i];
@@ -474,23 +474,59 @@ void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* 
 }
 
 template <typename VT, typename XT>
-static bool jacobi_from_zero(const DevCSR& A, const VT* sell_vals, const float* rscale, const XT* x,
+static bool jacobi_from_zero(const DevCSR& A, const VT* bd_val, const float* rscale, const XT* x,
                              const XT* b, const XT* dinv, XT* xn, double omega, hipStream_t st) {
-  if (!sell_vals || A.lpr != 0 || !A.sell.bd_ptr) return false;
+  if (A.lpr != 0 || !A.sell.bd_ptr || (!bd_val && A.sell.bd_pos)) return false;
   if (A.n == 0) return true;
   hipLaunchKernelGGL((k_sell_jacobi_from_zero<VT, XT>), dim3((A.n + 255) / 256), dim3(256), 0, st, A.n,
-                     A.active, A.sell.bd_ptr, A.sell.bd_pos, A.sell.bd_col, sell_vals, rscale, x, b,
-                     dinv, xn, omega);
+                     A.active, A.sell.bd_ptr, A.sell.bd_col, bd_val, rscale, x, b, dinv, xn, omega);
   return true;
 }
-bool fe_jacobi_from_zero(const DevCSR& A, const double* x, const double* b, const double* dinv,
-                         double* xn, double omega, hipStream_t st) {
-  return jacobi_from_zero<double, double>(A, A.sell.val, nullptr, x, b, dinv, xn, omega, st);
+bool fe_jacobi_from_zero(const DevCSR& A, const double* bd_val, const double* x, const double* b,
+                         const double* dinv, double* xn, double omega, hipStream_t st) {
+  return jacobi_from_zero<double, double>(A, bd_val, nullptr, x, b, dinv, xn, omega, st);
 }
-bool fe_jacobi_from_zero_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv,
-                              float* xn, double omega, hipStream_t st) {
-  return jacobi_from_zero<uint16_t, float>(A, A.sell.val_bf16, A.rscale, x, b, dinv, xn, omega, st);
+bool fe_jacobi_from_zero_bf16(const DevCSR& A, const uint16_t* bd_val, const float* x, const float* b,
+                              const float* dinv, float* xn, double omega, hipStream_t st) {
+  return jacobi_from_zero<uint16_t, float>(A, bd_val, A.rscale, x, b, dinv, xn, omega, st);
 }
+template <typename T>
+__global__ void k_gather_positions(int n, const int* __restrict__ pos, const T* __restrict__ src,
+                                   T* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[pos[i]];
+}
+void sell_dirichlet_values(const DevCSR& A, int n_bd, double* out64, uint16_t* out16, hipStream_t st) {
+  if (n_bd <= 0 || !A.sell.bd_pos) return;
+  const dim3 g((n_bd + 255) / 256);
+  if (out64) hipLaunchKernelGGL(k_gather_positions<double>, g, dim3(256), 0, st, n_bd, A.sell.bd_pos, A.sell.val, out64);
+  if (out16)
+    hipLaunchKernelGGL(k_gather_positions<uint16_t>, g, dim3(256), 0, st, n_bd, A.sell.bd_pos, A.sell.val_bf16, out16);
+}
+
+template <typename S, typename D>
+__global__ void k_gather(int n, const int* __restrict__ perm, const S* __restrict__ x, D* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (D)x[perm[i]];
+}
+template <typename S, typename D>
+__global__ void k_scatter(int n, const int* __restrict__ perm, const S* __restrict__ x, D* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[perm[i]] = (D)x[i];
+}
+template <typename S, typename D>
+static void gather_t(int n, const int* perm, const S* x, D* y, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL((k_gather<S, D>), dim3((n + 255) / 256), dim3(256), 0, st, n, perm, x, y);
+}
+template <typename S, typename D>
+static void scatter_t(int n, const int* perm, const S* x, D* y, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL((k_scatter<S, D>), dim3((n + 255) / 256), dim3(256), 0, st, n, perm, x, y);
+}
+void vec_gather(int n, const int* perm, const double* x, double* y, hipStream_t st) { gather_t(n, perm, x, y, st); }
+void vec_gather(int n, const int* perm, const double* x, float* y, hipStream_t st) { gather_t(n, perm, x, y, st); }
+void vec_gather(int n, const int* perm, const float* x, float* y, hipStream_t st) { gather_t(n, perm, x, y, st); }
+void vec_scatter(int n, const int* perm, const double* x, double* y, hipStream_t st) { scatter_t(n, perm, x, y, st); }
+void vec_scatter(int n, const int* perm, const float* x, double* y, hipStream_t st) { scatter_t(n, perm, x, y, st); }
 
 // mixed precision: bf16 values with FP32 row scale, FP32 vectors
 void fe_apply_scaled_add_bf16(const DevCSR& A, const float* x, float* y, double a, hipStream_t st) {

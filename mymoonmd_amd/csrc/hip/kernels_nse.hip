@@ -329,11 +329,21 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
   }
 }
 
+// value of v in lane `lane` (uniform) for every lane of the wavefront
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // one wavefront per cell: the local rhs lives in registers (rows lane and
 // lane + 64), gathered already permuted; forward and backward substitution
-// broadcast the solved component.  The factor columns of the next U steps are
-// loaded before those steps run: their addresses do not depend on the
-// solution, so the loads overlap instead of exposing one latency per step.
+// broadcast the solved component (a lane read, no LDS round trip).  The
+// factor columns of the next U steps are loaded before those steps run: their
+// addresses do not depend on the solution, so the loads overlap instead of
+// exposing one latency per step.  Lanes without a second row (lane + 64 >=
+// 89) load a valid duplicate and ignore it, so no load is predicated.
 template <typename T>
 __global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const T* __restrict__ lu,
                                                      const int* __restrict__ pmap,
@@ -347,53 +357,91 @@ __global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const T* __res
   const int* pm = pmap + (size_t)cell * NL;
   const int i0 = lane, i1 = lane + 64;
   const bool h1 = i1 < NL;
+  const int j1 = h1 ? i1 : i0;   // in-range row for the second load
   double b0 = r[pm[i0]];
-  double b1 = h1 ? r[pm[i1]] : 0.0;
-  // L y = b (unit diagonal), columns 0 .. NL-2
-  static_assert((NL - 1) % U == 0, "column blocks");
-  for (int k0 = 0; k0 < NL - 1; k0 += U) {
-    T l0[U], l1[U];
+  double b1 = r[pm[j1]];
+  if (!h1) b1 = 0.0;
+  // column blocks of U steps; two register buffers so that the loads of the
+  // next block are in flight while the current block is computed
+  static_assert((NL - 1) % U == 0, "column blocks of the forward sweep");
+  // forward-sweep block: columns k0 .. k0+U-1 (all < NL-1)
+  auto load_l = [&](int k0, T(&l0)[U], T(&l1)[U]) {
+    const T* col = L + (size_t)k0 * NL;
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const T* col = L + (size_t)(k0 + u) * NL;
-      l0[u] = col[i0];
-      l1[u] = h1 ? col[i1] : T(0);
+      l0[u] = col[u * NL + i0];
+      l1[u] = col[u * NL + j1];
     }
+  };
+  // backward-sweep block: columns k0, k0-1, ..., k0-U+1; the last block
+  // (k0 < U-1) reads column 0 in place of the negative ones and skips them
+  auto load_u = [&](int k0, T(&l0)[U], T(&l1)[U], T(&dg)[U]) {
+    if (k0 >= U - 1) {
+      const T* col = L + (size_t)k0 * NL;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        l0[u] = col[-u * NL + i0];
+        l1[u] = col[-u * NL + j1];
+        dg[u] = col[-u * NL + (k0 - u)];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 - u >= 0 ? k0 - u : 0;
+        const T* col = L + (size_t)k * NL;
+        l0[u] = col[i0];
+        l1[u] = col[j1];
+        dg[u] = col[k];
+      }
+    }
+  };
+  // L y = b (unit diagonal), columns 0 .. NL-2
+  auto lower = [&](int k0, const T(&l0)[U], const T(&l1)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = k0 + u;
-      const double bk = __shfl(k < 64 ? b0 : b1, k & 63, 64);
-      if (i0 > k) b0 -= (double)l0[u] * bk;
-      if (h1 && i1 > k) b1 -= (double)l1[u] * bk;
-    }
-  }
-  // U x = y, columns NL-1 .. 0
-  for (int k0 = NL - 1; k0 >= 0; k0 -= U) {
-    T l0[U], l1[U], dg[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int k = k0 - u;
-      if (k >= 0) {
-        const T* col = L + (size_t)k * NL;
-        l0[u] = col[i0];
-        l1[u] = h1 ? col[i1] : T(0);
-        dg[u] = col[k];
-      } else {
-        l0[u] = l1[u] = T(0);
-        dg[u] = T(1);
+      if (k < NL - 1) {
+        const double bk = lane_value(k < 64 ? b0 : b1, k & 63);
+        if (i0 > k) b0 -= (double)l0[u] * bk;
+        if (h1 && i1 > k) b1 -= (double)l1[u] * bk;
       }
     }
+  };
+  // U x = y, columns NL-1 .. 0
+  auto upper = [&](int k0, const T(&l0)[U], const T(&l1)[U], const T(&dg)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int k = k0 - u;
       if (k >= 0) {
-        const double bk = __shfl(k < 64 ? b0 : b1, k & 63, 64);
+        const double bk = lane_value(k < 64 ? b0 : b1, k & 63);
         const double xk = bk / (double)dg[u];
         if (i0 == k) b0 = xk;
         if (i1 == k) b1 = xk;
         if (i0 < k) b0 -= (double)l0[u] * xk;
         if (h1 && i1 < k) b1 -= (double)l1[u] * xk;
       }
+    }
+  };
+  {
+    T a0[U], a1[U], c0[U], c1[U];
+    load_l(0, a0, a1);
+#pragma unroll 1
+    for (int k0 = 0; k0 < NL - 1; k0 += 2 * U) {
+      if (k0 + U < NL - 1) load_l(k0 + U, c0, c1);
+      lower(k0, a0, a1);
+      if (k0 + 2 * U < NL - 1) load_l(k0 + 2 * U, a0, a1);
+      if (k0 + U < NL - 1) lower(k0 + U, c0, c1);
+    }
+  }
+  {
+    T a0[U], a1[U], ad[U], c0[U], c1[U], cd[U];
+    load_u(NL - 1, a0, a1, ad);
+#pragma unroll 1
+    for (int k0 = NL - 1; k0 >= 0; k0 -= 2 * U) {
+      if (k0 - U >= 0) load_u(k0 - U, c0, c1, cd);
+      upper(k0, a0, a1, ad);
+      if (k0 - 2 * U >= 0) load_u(k0 - 2 * U, a0, a1, ad);
+      if (k0 - U >= 0) upper(k0 - U, c0, c1, cd);
     }
   }
   upd[(size_t)cell * NL + i0] = damp * b0;

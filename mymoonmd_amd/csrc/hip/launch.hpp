@@ -19,6 +19,7 @@ struct DevSELL {
   double* val = nullptr;          // FP64 values (nullptr: not built)
   uint16_t* val_bf16 = nullptr;   // bf16 values divided by the row scale
   // active-row entries in Dirichlet columns (row order): SELL slot, column
+  int n_bd = 0;                   // number of such entries
   int* bd_ptr = nullptr;          // [active+1] (nullptr: not built)
   int* bd_pos = nullptr;
   int* bd_col = nullptr;
@@ -53,17 +54,21 @@ void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* 
                double omega, hipStream_t st);
 // the same sweep for an x that is zero on the active rows (only the products
 // with the Dirichlet entries of x remain); bitwise equal to fe_jacobi on such
-// an x.  Returns false (and launches nothing) when A has no SELL copy.
-bool fe_jacobi_from_zero(const DevCSR& A, const double* x, const double* b, const double* dinv,
-                         double* xn, double omega, hipStream_t st);
+// an x.  bd_val: the values of A.sell's Dirichlet-column slots in the order of
+// A.sell.bd_pos (sell_dirichlet_values).  Returns false (and launches nothing)
+// when A has no SELL copy.
+bool fe_jacobi_from_zero(const DevCSR& A, const double* bd_val, const double* x, const double* b,
+                         const double* dinv, double* xn, double omega, hipStream_t st);
+// compact copies of the Dirichlet-column slot values of A.sell (FP64 and/or bf16)
+void sell_dirichlet_values(const DevCSR& A, int n_bd, double* out64, uint16_t* out16, hipStream_t st);
 // ---- FE-matrix operators (bf16 matrix, FP32 vectors)
 void fe_apply_scaled_add_bf16(const DevCSR& A, const float* x, float* y, double a, hipStream_t st);
 void fe_defect_bf16(const DevCSR& A, const float* x, const float* b, float* d, double* partial,
                     double* norm_out, hipStream_t st);
 void fe_jacobi_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv, float* xn,
                     double omega, hipStream_t st);
-bool fe_jacobi_from_zero_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv,
-                              float* xn, double omega, hipStream_t st);
+bool fe_jacobi_from_zero_bf16(const DevCSR& A, const uint16_t* bd_val, const float* x, const float* b,
+                              const float* dinv, float* xn, double omega, hipStream_t st);
 // ---- plain CSR (transfer matrices): y = alpha*P x + beta*y on rows < row_end;
 //      rows >= row_end are set to 0 when zero_rest, else left unchanged
 void csr_spmv(const DevCSR& P, const double* x, double* y, double alpha, double beta, int row_end,
@@ -96,6 +101,13 @@ void vec_fill_f32(int n, float v, float* y, hipStream_t st);
 void vec_add_scaled_range(int from, int n, double a, const double* x, double* y, hipStream_t st);
 void vec_add_scaled_range_f32(int from, int n, double a, const float* x, float* y, hipStream_t st);
 void vec_f64_to_f32(int n, const double* x, float* y, hipStream_t st);
+// y[i] = x[perm[i]] (gather into a renumbered vector) / y[perm[i]] = x[i]
+// (scatter back); with conversion between FP64 and FP32 where the types differ
+void vec_gather(int n, const int* perm, const double* x, double* y, hipStream_t st);
+void vec_gather(int n, const int* perm, const double* x, float* y, hipStream_t st);
+void vec_gather(int n, const int* perm, const float* x, float* y, hipStream_t st);
+void vec_scatter(int n, const int* perm, const double* x, double* y, hipStream_t st);
+void vec_scatter(int n, const int* perm, const float* x, double* y, hipStream_t st);
 void vec_f32_to_f64(int n, const float* x, double* y, hipStream_t st);
 
 // ---- dense in-place inverse (n x n row-major) with partial pivoting;

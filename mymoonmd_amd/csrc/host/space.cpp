@@ -1,6 +1,7 @@
 #include "space.hpp"
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <stdexcept>
 
@@ -267,6 +268,116 @@ void build_prolongation(const FESpace& coarse, const FESpace& fine, CSRPattern& 
   }
 }
 
+void build_function_restriction(const FESpace& coarse, const FESpace& fine, CSRPattern& RF,
+                                std::vector<double>& vals) {
+  const Mesh& m = *fine.mesh;
+  if (coarse.mesh != fine.mesh || fine.level != coarse.level + 1)
+    throw std::runtime_error("restriction needs consecutive levels of one mesh");
+  if (m.shape != SHAPE_QUAD && m.shape != SHAPE_HEXA)
+    throw std::runtime_error("function restriction is built for quadrilaterals and hexahedra");
+  const int ndf = fine.ndof_cell, ndc = coarse.ndof_cell, dim = m.dim;
+  const QuadRule q = gauss_tensor(dim, 3);   // exact for the products of Q2 functions
+  const double child_det = 1.0 / (double)(1 << dim);
+  // G = coarse mass matrix on the reference cell and its inverse
+  std::vector<double> phc((size_t)q.n * ndc), phf((size_t)q.n * ndf);
+  for (int k = 0; k < q.n; ++k) {
+    coarse.el.eval(&q.xi[(size_t)k * dim], &phc[(size_t)k * ndc], nullptr, nullptr);
+    fine.el.eval(&q.xi[(size_t)k * dim], &phf[(size_t)k * ndf], nullptr, nullptr);
+  }
+  std::vector<double> G((size_t)ndc * ndc, 0.0), Ginv((size_t)ndc * ndc, 0.0);
+  for (int k = 0; k < q.n; ++k)
+    for (int i = 0; i < ndc; ++i)
+      for (int j = 0; j < ndc; ++j) G[(size_t)i * ndc + j] += q.w[k] * phc[(size_t)k * ndc + i] * phc[(size_t)k * ndc + j];
+  for (int i = 0; i < ndc; ++i) Ginv[(size_t)i * ndc + i] = 1.0;
+  for (int c = 0; c < ndc; ++c) {   // Gauss-Jordan with partial pivoting
+    int p = c;
+    for (int r = c + 1; r < ndc; ++r)
+      if (std::fabs(G[(size_t)r * ndc + c]) > std::fabs(G[(size_t)p * ndc + c])) p = r;
+    for (int j = 0; j < ndc; ++j) {
+      std::swap(G[(size_t)c * ndc + j], G[(size_t)p * ndc + j]);
+      std::swap(Ginv[(size_t)c * ndc + j], Ginv[(size_t)p * ndc + j]);
+    }
+    const double d = 1.0 / G[(size_t)c * ndc + c];
+    for (int j = 0; j < ndc; ++j) { G[(size_t)c * ndc + j] *= d; Ginv[(size_t)c * ndc + j] *= d; }
+    for (int r = 0; r < ndc; ++r) {
+      if (r == c) continue;
+      const double f = G[(size_t)r * ndc + c];
+      if (f == 0.0) continue;
+      for (int j = 0; j < ndc; ++j) {
+        G[(size_t)r * ndc + j] -= f * G[(size_t)c * ndc + j];
+        Ginv[(size_t)r * ndc + j] -= f * Ginv[(size_t)c * ndc + j];
+      }
+    }
+  }
+  // RQ of a child depends only on its position in the parent (uniform
+  // refinement of affine cells): cache by the child centre in parent coordinates
+  std::map<std::vector<long long>, std::vector<double>> cache;
+  const MeshLevel& fl = m.levels[fine.level];
+  std::vector<std::vector<int>> children(coarse.n_cells);
+  for (int fc = 0; fc < fine.n_cells; ++fc) children[fl.parent[fc]].push_back(fc);
+  std::vector<std::vector<std::pair<int, double>>> rows(coarse.n_dof);
+  std::vector<int> count(coarse.n_dof, 0);
+  std::vector<double> v(ndc);
+  for (int pc = 0; pc < coarse.n_cells; ++pc) {
+    const AffineMap ac = affine_map(m, coarse.level, pc);
+    for (int fc : children[pc]) {
+      const AffineMap af = affine_map(m, fine.level, fc);
+      double zero[3] = {0, 0, 0}, x[3], xc[3];
+      map_to_phys(af, zero, x);
+      map_to_ref(ac, x, xc);
+      std::vector<long long> key(dim);
+      for (int d = 0; d < dim; ++d) key[d] = std::llround(xc[d] * 1024.0);
+      auto it = cache.find(key);
+      if (it == cache.end()) {
+        std::vector<double> R((size_t)ndc * ndf, 0.0);
+        for (int k = 0; k < q.n; ++k) {
+          double xp[3], xi[3];
+          map_to_phys(af, &q.xi[(size_t)k * dim], xp);
+          map_to_ref(ac, xp, xi);
+          coarse.el.eval(xi, v.data(), nullptr, nullptr);
+          const double w = q.w[k] * child_det;
+          for (int a = 0; a < ndc; ++a)
+            for (int l = 0; l < ndf; ++l) R[(size_t)a * ndf + l] += w * v[a] * phf[(size_t)k * ndf + l];
+        }
+        std::vector<double> RQ((size_t)ndc * ndf, 0.0);
+        for (int a = 0; a < ndc; ++a)
+          for (int b = 0; b < ndc; ++b)
+            for (int l = 0; l < ndf; ++l) RQ[(size_t)a * ndf + l] += Ginv[(size_t)a * ndc + b] * R[(size_t)b * ndf + l];
+        it = cache.emplace(key, std::move(RQ)).first;
+      }
+      const std::vector<double>& RQ = it->second;
+      for (int a = 0; a < ndc; ++a) {
+        auto& row = rows[coarse.global_numbers[(size_t)pc * ndc + a]];
+        for (int l = 0; l < ndf; ++l) {
+          const double val = RQ[(size_t)a * ndf + l];
+          if (val != 0.0) row.emplace_back(fine.global_numbers[(size_t)fc * ndf + l], val);
+        }
+      }
+    }
+    for (int a = 0; a < ndc; ++a) count[coarse.global_numbers[(size_t)pc * ndc + a]]++;
+  }
+  RF.n_rows = coarse.n_dof;
+  RF.n_cols = fine.n_dof;
+  RF.active_bound = coarse.n_dof;
+  RF.row_ptr.assign(coarse.n_dof + 1, 0);
+  RF.col.clear();
+  vals.clear();
+  for (int r = 0; r < coarse.n_dof; ++r) {
+    auto& row = rows[r];
+    std::sort(row.begin(), row.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+      return a.first < b.first;
+    });
+    for (size_t k = 0; k < row.size();) {
+      const int c = row[k].first;
+      double sum = 0.0;
+      for (; k < row.size() && row[k].first == c; ++k) sum += row[k].second;
+      RF.col.push_back(c);
+      vals.push_back(sum / (double)count[r]);
+    }
+    RF.row_ptr[r + 1] = (int)RF.col.size();
+  }
+}
+
 void transpose_csr(const CSRPattern& A, const std::vector<double>& av, CSRPattern& T,
                    std::vector<double>& tv) {
   T.n_rows = A.n_cols;
@@ -406,6 +517,68 @@ SellPattern build_sell(const CSRPattern& p, int sigma) {
     }
   }
   return s;
+}
+
+}  // namespace pmx
+
+namespace pmx {
+
+std::vector<int> lexicographic_order(const FESpace& s) {
+  const int n = s.n_dof, na = s.active_bound;
+  // coordinates quantised on a grid far finer than any mesh width here, so
+  // that nodes on one plane/line compare equal despite round-off
+  std::vector<long long> key((size_t)3 * n);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i)
+    for (int d = 0; d < 3; ++d) key[(size_t)3 * i + d] = std::llround(s.dof_xyz[(size_t)3 * i + d] * 1073741824.0);
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  auto less = [&](int a, int b) {
+    for (int d = 2; d >= 0; --d) {
+      const long long qa = key[(size_t)3 * a + d], qb = key[(size_t)3 * b + d];
+      if (qa != qb) return qa < qb;
+    }
+    return a < b;
+  };
+  std::sort(perm.begin(), perm.begin() + na, less);
+  std::sort(perm.begin() + na, perm.end(), less);
+  return perm;
+}
+
+void permute_csr(const CSRPattern& A, const std::vector<double>* av, const std::vector<int>& rperm,
+                 const std::vector<int>& cinv, CSRPattern& out, std::vector<double>* ov,
+                 std::vector<int>* opos, bool sort_rows) {
+  const int n = A.n_rows;
+  out.n_rows = A.n_rows;
+  out.n_cols = A.n_cols;
+  out.active_bound = A.active_bound;
+  out.row_ptr.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    const int r = rperm.empty() ? i : rperm[i];
+    out.row_ptr[i + 1] = out.row_ptr[i] + (A.row_ptr[r + 1] - A.row_ptr[r]);
+  }
+  const size_t nnz = A.col.size();
+  out.col.resize(nnz);
+  if (ov) ov->resize(nnz);
+  if (opos) opos->resize(nnz);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const int r = rperm.empty() ? i : rperm[i];
+    const int b = A.row_ptr[r], len = A.row_ptr[r + 1] - b, o = out.row_ptr[i];
+    // order of the row's entries in out: as in A, or by new column
+    int idx[512];
+    std::vector<int> big;
+    int* ord = idx;
+    if (len > 512) { big.resize(len); ord = big.data(); }
+    for (int k = 0; k < len; ++k) ord[k] = k;
+    auto newcol = [&](int k) { return cinv.empty() ? A.col[b + k] : cinv[A.col[b + k]]; };
+    if (sort_rows) std::sort(ord, ord + len, [&](int x, int y) { return newcol(x) < newcol(y); });
+    for (int k = 0; k < len; ++k) {
+      out.col[o + k] = newcol(ord[k]);
+      if (ov) (*ov)[o + k] = (*av)[b + ord[k]];
+      if (opos) (*opos)[o + k] = b + ord[k];
+    }
+  }
 }
 
 }  // namespace pmx

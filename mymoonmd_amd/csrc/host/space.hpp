@@ -81,6 +81,27 @@ CSRPattern build_rect_structure(const FESpace& test, const FESpace& ansatz);
 // (src/Multigrid/GridTransfer.C:943-1230) equals P^T*d.
 void build_prolongation(const FESpace& coarse, const FESpace& fine, CSRPattern& P,
                         std::vector<double>& vals);
+// Lexicographic (z, y, x) order of the DOFs of a space, active DOFs first and
+// Dirichlet DOFs last (each group sorted on its own): perm[new] = old.  A
+// solver-internal numbering whose neighbouring rows touch neighbouring
+// columns, so the columns gathered by one wavefront share cache lines.
+std::vector<int> lexicographic_order(const FESpace& s);
+// Rows and columns renumbered: out row i = row rperm[i] of A (rperm empty:
+// identity), column c -> cinv[c] (cinv empty: identity).  The entries of a
+// row keep their order (and so the row's summation order), or, with
+// sort_rows, are sorted by their new column.  opos (optional) receives the
+// position in A of every entry of out.
+void permute_csr(const CSRPattern& A, const std::vector<double>* av, const std::vector<int>& rperm,
+                 const std::vector<int>& cinv, CSRPattern& out, std::vector<double>* ov,
+                 std::vector<int>* opos, bool sort_rows = false);
+// GridTransfer::RestrictFunction (src/Multigrid/GridTransfer.C:1218-1501) as
+// a matrix: per coarse cell the local L2 projection G^-1 R_j of the fine
+// function on each child j (G the coarse mass matrix, R_j the coarse/fine
+// mixed mass matrix of child j, both on the parent's reference cell), summed
+// over the cells around each coarse DOF and divided by their number.
+// Quadrilaterals and hexahedra.
+void build_function_restriction(const FESpace& coarse, const FESpace& fine, CSRPattern& RF,
+                                std::vector<double>& vals);
 void transpose_csr(const CSRPattern& A, const std::vector<double>& av, CSRPattern& T,
                    std::vector<double>& tv);
 

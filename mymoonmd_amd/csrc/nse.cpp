@@ -37,9 +37,12 @@ pmx_nse::~pmx_nse() {
                                               d_B2, d_B3, d_BTrp, d_BTcol, d_BTsrc, d_BT1, d_BT2, d_BT3, d_Crp,
                                               d_Ccol, d_Cval, d_Ckind, d_Cpos, d_partial, d_pref, d_pgeo})
     dfree(p);
-  for (void* p : std::initializer_list<void*>{Csell.slice_ptr, Csell.row, Csell.col, Csell.src, Csell.val})
-    dfree(p);
   for (auto& kv : err_ref) dfree(kv.second);
+  for (auto& kv : restrict_from) {
+    dfree(kv.second.row_ptr);
+    dfree(kv.second.col);
+    dfree(kv.second.val);
+  }
 }
 
 namespace pmx_internal {
@@ -75,11 +78,12 @@ pmx::DevCSR nse_combined(const pmx_nse* s) {
   C.n = C.m = s->n_comb;
   C.nnz = (int)s->Cp.col.size();
   C.active = s->n_comb;
-  C.lpr = s->Csell.val ? 0 : 64;   // 0: the FE-matrix kernels use the SELL copy
+  // 8 lanes per row: the rows are long (83 entries on average at 32^3) and
+  // few, one row per lane (SELL) leaves too little parallelism per wave
+  C.lpr = 8;
   C.row_ptr = s->d_Crp;
   C.col = s->d_Ccol;
   C.val = s->d_Cval;
-  C.sell = s->Csell;
   return C;
 }
 
@@ -104,7 +108,6 @@ void nse_refresh(pmx_nse* s, hipStream_t st) {
   pmx::sell_gather(nbt, s->d_BTsrc, s->d_B3, s->d_BT3, nullptr, st);
   pmx::combined_values((long long)s->Cp.col.size(), s->d_Ckind, s->d_Cpos, s->A->A.val, s->d_B1, s->d_B2, s->d_B3,
                        s->d_Cval, st);
-  pmx::sell_fill_f64(nse_combined(s), s->Csell.val, st);
   s->values_ready = true;
 }
 
@@ -284,16 +287,6 @@ int pmx_nse_create(pmx_ctx* ctx, const pmx_domain* d, int level, pmx_nse** out) 
     s->d_Crp = upload(s->Cp.row_ptr.data(), s->Cp.row_ptr.size(), st);
     s->d_Ccol = upload(s->Cp.col.data(), s->Cp.col.size(), st);
     s->d_Cval = dmalloc<double>(s->Cp.col.size());
-    {
-      const pmx::SellPattern sp = pmx::build_sell(s->Cp, 4096);
-      s->Csell.n_slices = sp.n_slices;
-      s->Csell.total = sp.col.size();
-      s->Csell.slice_ptr = upload(sp.slice_ptr.data(), sp.slice_ptr.size(), st);
-      s->Csell.row = upload(sp.row.data(), sp.row.size(), st);
-      s->Csell.col = upload(sp.col.data(), sp.col.size(), st);
-      s->Csell.src = upload(sp.src.data(), sp.src.size(), st);
-      s->Csell.val = dmalloc<double>(s->Csell.total);
-    }
     s->d_Ckind = upload(s->Ckind.data(), s->Ckind.size(), st);
     s->d_Cpos = upload(s->Cpos.data(), s->Cpos.size(), st);
     s->d_partial = dmalloc<double>((size_t)2 * ((nc + 255) / 256) + 8);
@@ -430,6 +423,36 @@ int pmx_nse_interpolate(pmx_ctx* ctx, const pmx_nse* s, int example, double* dev
     pmx::interpolate_example(pmx::EX_NSE_U2P1_P, 0.0, s->n_p, pxyz, dev_x + (size_t)3 * s->n_u, st);
     HIP_CHECK(hipStreamSynchronize(st));
     dfree(pxyz);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int pmx_nse_restrict_velocity(pmx_ctx* ctx, pmx_nse* coarse, const pmx_nse* fine, const double* dev_fine_u,
+                              double* dev_coarse_u) {
+  return guarded([&] {
+    require(ctx && coarse && fine && dev_fine_u && dev_coarse_u, "NULL argument");
+    require(coarse->dom == fine->dom && fine->level == coarse->level + 1,
+            "restriction needs consecutive levels of one domain");
+    hipStream_t st = ctx->stream;
+    auto it = coarse->restrict_from.find(fine);
+    if (it == coarse->restrict_from.end()) {
+      pmx::CSRPattern RF;
+      std::vector<double> rv;
+      pmx::build_function_restriction(coarse->vs, fine->vs, RF, rv);
+      pmx::DevCSR D{};
+      D.n = RF.n_rows;
+      D.m = RF.n_cols;
+      D.nnz = (int)RF.col.size();
+      D.active = RF.n_rows;
+      D.lpr = 8;
+      D.row_ptr = upload(RF.row_ptr.data(), RF.row_ptr.size(), st);
+      D.col = upload(RF.col.data(), RF.col.size(), st);
+      D.val = upload(rv.data(), rv.size(), st);
+      it = coarse->restrict_from.emplace(fine, D).first;
+    }
+    for (int k = 0; k < 3; ++k)
+      pmx::csr_spmv(it->second, dev_fine_u + (size_t)k * fine->n_u, dev_coarse_u + (size_t)k * coarse->n_u, 1.0,
+                    0.0, coarse->n_u, 0, st);
     HIP_CHECK(hipGetLastError());
   });
 }

@@ -35,13 +35,13 @@ struct pmx_nse {
   double *d_BT1 = nullptr, *d_BT2 = nullptr, *d_BT3 = nullptr;
   int *d_Crp = nullptr, *d_Ccol = nullptr;
   double* d_Cval = nullptr;
-  pmx::DevSELL Csell;            // SELL copy of the combined matrix (smoothing defects)
   uint8_t* d_Ckind = nullptr;
   int* d_Cpos = nullptr;
   double* d_partial = nullptr;
   double* d_pref = nullptr;
   double* d_pgeo = nullptr;
   std::map<int, double*> err_ref;
+  std::map<const pmx_nse*, pmx::DevCSR> restrict_from;   // function restriction from a finer level
   ~pmx_nse();
 };
 

@@ -29,6 +29,8 @@ using namespace pmx_internal;
 
 namespace {
 
+constexpr int kSellSigmaSolver = 4096;   // SELL sorting window of the renumbered levels
+
 enum Dir { DIR_DUMMY = 0, DIR_DOWN = 1, DIR_UP = 2, DIR_END = 3 };
 
 void fill_recursively(std::vector<int>& rec, int level, int n_levels, int cycle, std::vector<Dir>& cc) {
@@ -101,7 +103,17 @@ struct Level {
   float *rhs32 = nullptr, *sol32 = nullptr, *def32 = nullptr, *tmp32 = nullptr;
   double* partial = nullptr;
   bool sol_zero = false;     // sol vanishes on the active DOFs (set by the cycle's resets)
+  int* perm = nullptr;       // solver-internal numbering: perm[new] = reference DOF (nullptr: identity)
+  double* bd64 = nullptr;    // values of A's Dirichlet-column slots (zero-start sweep), FP64 ...
+  uint16_t* bd16 = nullptr;  // ... or bf16
+  bool own_sell = false;     // A.sell is this level's renumbered copy
   void release() {
+    if (own_sell) {
+      dfree(A.sell.slice_ptr); dfree(A.sell.row); dfree(A.sell.col); dfree(A.sell.src); dfree(A.sell.val);
+      dfree(A.sell.bd_ptr); dfree(A.sell.bd_pos); dfree(A.sell.bd_col);
+      own_sell = false;
+    }
+    dfree(perm); dfree(bd64); dfree(bd16);
     dfree(dinv); dfree(dinv32);
     free_csr(P); free_csr(R); dfree(dinv_dense);
     dfree(rhs); dfree(sol); dfree(def); dfree(tmp);
@@ -190,12 +202,12 @@ void mg_smooth(pmx_solver* s, int l, bool mixed) {
   const bool zero = L.sol_zero && s->p.zero_start_shortcut;
   L.sol_zero = false;
   if (mixed) {
-    if (!(zero && pmx::fe_jacobi_from_zero_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32,
+    if (!(zero && pmx::fe_jacobi_from_zero_bf16(L.A, L.bd16, L.sol32, L.rhs32, L.dinv32, L.tmp32,
                                                 s->p.jacobi_damp, S(s))))
       pmx::fe_jacobi_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp, S(s));
     std::swap(L.sol32, L.tmp32);
   } else {
-    if (!(zero && pmx::fe_jacobi_from_zero(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s))))
+    if (!(zero && pmx::fe_jacobi_from_zero(L.A, L.bd64, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s))))
       pmx::fe_jacobi(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s));
     std::swap(L.sol, L.tmp);
   }
@@ -288,13 +300,17 @@ void mg_cycle(pmx_solver* s, bool mixed) {
 void mg_apply(pmx_solver* s, const double* v, double* z) {
   Level& F = s->lv.back();
   if (s->p.precision == PMX_PRECISION_MIXED) {
-    pmx::vec_f64_to_f32(F.n, v, F.rhs32, S(s));
+    if (F.perm) pmx::vec_gather(F.n, F.perm, v, F.rhs32, S(s));
+    else pmx::vec_f64_to_f32(F.n, v, F.rhs32, S(s));
     mg_cycle(s, true);
-    pmx::vec_f32_to_f64(F.n, F.sol32, z, S(s));
+    if (F.perm) pmx::vec_scatter(F.n, F.perm, F.sol32, z, S(s));
+    else pmx::vec_f32_to_f64(F.n, F.sol32, z, S(s));
   } else {
-    HIP_CHECK(hipMemcpyAsync(F.rhs, v, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+    if (F.perm) pmx::vec_gather(F.n, F.perm, v, F.rhs, S(s));
+    else HIP_CHECK(hipMemcpyAsync(F.rhs, v, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
     mg_cycle(s, false);
-    HIP_CHECK(hipMemcpyAsync(z, F.sol, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+    if (F.perm) pmx::vec_scatter(F.n, F.perm, F.sol, z, S(s));
+    else HIP_CHECK(hipMemcpyAsync(z, F.sol, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
   }
 }
 
@@ -681,6 +697,7 @@ int pmx_solver_default_params(pmx_solver_params* p) {
     d.precision = PMX_PRECISION_FP64;
     d.vanka_damp = 1.0;
     d.zero_start_shortcut = 1;
+    d.renumber_levels = 1;
     *p = d;
   });
 }
@@ -751,48 +768,121 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
     if (mg) {
       const bool mixed = s->p.precision == PMX_PRECISION_MIXED;
       s->lv.resize(n_levels);
+      // solver-internal numbering of the levels above the coarsest (which
+      // keeps the reference order for its dense inverse)
+      std::vector<std::vector<int>> perm(n_levels), pinv(n_levels);
+      for (int l = 1; l < n_levels; ++l)
+        if (s->p.renumber_levels && levels[l] && levels[l]->A.sell.total && spaces[l]) {
+          perm[l] = pmx::lexicographic_order(spaces[l]->fs);
+          pinv[l].resize(perm[l].size());
+          for (size_t i = 0; i < perm[l].size(); ++i) pinv[l][perm[l][i]] = (int)i;
+        }
       for (int l = 0; l < n_levels; ++l) {
         Level& L = s->lv[l];
         const pmx_matrix* M = levels[l];
         require(M != nullptr && spaces[l] != nullptr, "NULL level");
         require(M->space == spaces[l], "matrix/space mismatch");
+        require(perm[l].empty() || (int)perm[l].size() == M->A.n, "space/matrix size mismatch");
         L.A = M->A;
         L.A.val_bf16 = nullptr;
         L.A.rscale = nullptr;
         L.A.sell.val_bf16 = nullptr;
         L.n = M->A.n;
         L.active = M->A.active;
-        L.partial = dmalloc<double>(std::max(4096, pmx::fe_defect_partials(L.A)));
         L.dinv = dmalloc<double>(L.n);
         check_zero_diag(s, L.A, L.dinv);
         if (mixed) {
-          L.rhs32 = dzeros<float>(L.n, st); L.sol32 = dzeros<float>(L.n, st);
-          L.def32 = dzeros<float>(L.n, st); L.tmp32 = dzeros<float>(L.n, st);
           L.A.val_bf16 = dmalloc<uint16_t>(L.A.nnz);
           L.A.rscale = dmalloc<float>(L.n);
           pmx::csr_to_bf16(L.A, L.A.val_bf16, L.A.rscale, st);
-          if (L.A.sell.total) {
-            L.A.sell.val_bf16 = dmalloc<uint16_t>(L.A.sell.total);
-            pmx::sell_fill_bf16(L.A, L.A.sell.val_bf16, st);
+        }
+        if (!perm[l].empty()) {
+          // renumbered SELL copy whose slots read the reference CSR values;
+          // the entries of a row are sorted by their new column, so the
+          // lanes of a slice (rows of one DOF type, neighbours in the new
+          // order) gather neighbouring x entries at every step.  This changes
+          // the summation order within rows (a different rounding of the
+          // preconditioner, not of the FGMRES operator)
+          pmx::CSRPattern pp;
+          std::vector<int> opos;
+          pmx::permute_csr(*M->pattern, nullptr, perm[l], pinv[l], pp, nullptr, &opos, true);
+          pmx::SellPattern sp = pmx::build_sell(pp, kSellSigmaSolver);
+          for (auto& k : sp.src)
+            if (k >= 0) k = opos[k];
+          pmx::DevCSR Ap = L.A;
+          Ap.lpr = 0;
+          Ap.sell = pmx::DevSELL{};
+          Ap.sell.n_slices = sp.n_slices;
+          Ap.sell.total = sp.col.size();
+          Ap.sell.slice_ptr = upload(sp.slice_ptr.data(), sp.slice_ptr.size(), st);
+          Ap.sell.row = upload(sp.row.data(), sp.row.size(), st);
+          Ap.sell.col = upload(sp.col.data(), sp.col.size(), st);
+          Ap.sell.src = upload(sp.src.data(), sp.src.size(), st);
+          Ap.sell.n_bd = (int)sp.bd_pos.size();
+          Ap.sell.bd_ptr = upload(sp.bd_ptr.data(), sp.bd_ptr.size(), st);
+          Ap.sell.bd_pos = sp.bd_pos.empty() ? nullptr : upload(sp.bd_pos.data(), sp.bd_pos.size(), st);
+          Ap.sell.bd_col = sp.bd_col.empty() ? nullptr : upload(sp.bd_col.data(), sp.bd_col.size(), st);
+          L.perm = upload(perm[l].data(), perm[l].size(), st);
+          double* dp = dmalloc<double>(L.n);
+          pmx::vec_gather(L.n, L.perm, L.dinv, dp, st);
+          dfree(L.dinv);
+          L.dinv = dp;
+          if (mixed) {
+            float* rs = dmalloc<float>(L.n);
+            pmx::vec_gather(L.n, L.perm, L.A.rscale, rs, st);
+            Ap.rscale = rs;
+            Ap.val_bf16 = nullptr;
+            Ap.sell.val_bf16 = dmalloc<uint16_t>(Ap.sell.total);
+            pmx::sell_fill_bf16(Ap, Ap.sell.val_bf16, st);
+            HIP_CHECK(hipStreamSynchronize(st));
+            dfree(L.A.val_bf16);
+            dfree(L.A.rscale);
+          } else {
+            Ap.sell.val = dmalloc<double>(Ap.sell.total);
+            pmx::sell_fill_f64(Ap, Ap.sell.val, st);
           }
+          // the CSR arrays of the view are the reference-order ones: never
+          // used for a renumbered level (lpr = 0 selects the SELL kernels)
+          L.A = Ap;
+          L.own_sell = true;
+        } else if (mixed && L.A.sell.total) {
+          L.A.sell.val_bf16 = dmalloc<uint16_t>(L.A.sell.total);
+          pmx::sell_fill_bf16(L.A, L.A.sell.val_bf16, st);
+        }
+        L.partial = dmalloc<double>(std::max(4096, pmx::fe_defect_partials(L.A)));
+        if (L.A.sell.n_bd > 0) {   // compact Dirichlet-slot values for the zero-start sweep
+          if (mixed && L.A.sell.val_bf16) {
+            L.bd16 = dmalloc<uint16_t>(L.A.sell.n_bd);
+            pmx::sell_dirichlet_values(L.A, L.A.sell.n_bd, nullptr, L.bd16, st);
+          } else if (!mixed && L.A.sell.val) {
+            L.bd64 = dmalloc<double>(L.A.sell.n_bd);
+            pmx::sell_dirichlet_values(L.A, L.A.sell.n_bd, L.bd64, nullptr, st);
+          }
+        }
+        if (mixed) {
+          L.rhs32 = dzeros<float>(L.n, st); L.sol32 = dzeros<float>(L.n, st);
+          L.def32 = dzeros<float>(L.n, st); L.tmp32 = dzeros<float>(L.n, st);
           L.dinv32 = dmalloc<float>(L.n);
           pmx::vec_f64_to_f32(L.n, L.dinv, L.dinv32, st);
         } else {
           L.rhs = dzeros<double>(L.n, st); L.sol = dzeros<double>(L.n, st);
           L.def = dzeros<double>(L.n, st); L.tmp = dzeros<double>(L.n, st);
         }
-        if (l + 1 < n_levels) {
-          const pmx_space* cs = spaces[l];
-          const pmx_space* fsp = spaces[l + 1];
-          require(cs->dom == fsp->dom && fsp->fs.level == cs->fs.level + 1,
-                  "multigrid spaces must be consecutive levels of one domain");
-          pmx::CSRPattern P, R;
-          std::vector<double> pv, rv;
-          pmx::build_prolongation(cs->fs, fsp->fs, P, pv);
-          pmx::transpose_csr(P, pv, R, rv);
-          L.P = upload_csr(P, pv, st);
-          L.R = upload_csr(R, rv, st);
-        }
+      }
+      for (int l = 0; l + 1 < n_levels; ++l) {
+        Level& L = s->lv[l];
+        const pmx_space* cs = spaces[l];
+        const pmx_space* fsp = spaces[l + 1];
+        require(cs->dom == fsp->dom && fsp->fs.level == cs->fs.level + 1,
+                "multigrid spaces must be consecutive levels of one domain");
+        pmx::CSRPattern P, R, Pp, Rp;
+        std::vector<double> pv, rv, ppv, rpv;
+        pmx::build_prolongation(cs->fs, fsp->fs, P, pv);
+        pmx::transpose_csr(P, pv, R, rv);
+        pmx::permute_csr(P, &pv, perm[l + 1], pinv[l], Pp, &ppv, nullptr, true);
+        pmx::permute_csr(R, &rv, perm[l], pinv[l + 1], Rp, &rpv, nullptr, true);
+        L.P = upload_csr(Pp, ppv, st);
+        L.R = upload_csr(Rp, rpv, st);
       }
       if (s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) build_coarse_inverse(s, levels[0]);
       s->cc = cycle_control(s->p.mg_cycle, n_levels);
@@ -953,3 +1043,34 @@ int pmx_solver_apply_preconditioner(pmx_solver* s, const double* r, double* z) {
 }
 
 }  // extern "C"
+
+int pmx_solver_time_level_kernel(pmx_solver* s, int level, int kernel, int reps, double* avg_ms) {
+  return guarded([&] {
+    require(s && avg_ms && reps >= 1, "invalid argument");
+    require(s->updated && !s->lv.empty(), "a scalar multigrid solver after pmx_solver_update is needed");
+    require(level >= 0 && level < (int)s->lv.size(), "level out of range");
+    const bool mixed = s->p.precision == PMX_PRECISION_MIXED;
+    require(kernel == (mixed ? PMX_KERNEL_JACOBI_BF16 : PMX_KERNEL_JACOBI_FP64),
+            "kernel must be the solver's smoothing sweep (JACOBI_BF16 mixed, JACOBI_FP64 fp64)");
+    Level& L = s->lv[level];
+    hipStream_t st = S(s);
+    auto launch = [&]() {
+      if (mixed) pmx::fe_jacobi_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp, st);
+      else pmx::fe_jacobi(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, st);
+    };
+    launch();
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    HIP_CHECK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) launch();
+    HIP_CHECK(hipEventRecord(e1, st));
+    HIP_CHECK(hipEventSynchronize(e1));
+    HIP_CHECK(hipGetLastError());
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}

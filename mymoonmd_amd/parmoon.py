@@ -306,7 +306,7 @@ def solver_params(**kw):
               "multigrid_coarse_residual": "mg_coarse_residual",
               "multigrid_coarse_max_n_iterations": "mg_coarse_max_n_iterations",
               "jacobi_damp": "jacobi_damp", "multigrid_vanka_damp_factor": "vanka_damp",
-              "zero_start_shortcut": "zero_start_shortcut"}
+              "zero_start_shortcut": "zero_start_shortcut", "renumber_levels": "renumber_levels"}
     for k, v in kw.items():
         if k == "iterative_solver_type":
             p.iterative_solver_type = IT_SOLVER[v]
@@ -372,6 +372,13 @@ class Solver:
         import torch
         torch.cuda.synchronize()
         call("pmx_solver_apply_preconditioner", self.h, _ptr(r), _ptr(z))
+
+    def time_level_kernel(self, level, kernel, reps=20):
+        """mean duration (ms) of a level's smoothing sweep as the cycle runs it"""
+        out = c_double()
+        call("pmx_solver_time_level_kernel", self.h, level, {"jacobi_fp64": 1, "jacobi_bf16": 3}[kernel], reps,
+             ctypes.byref(out))
+        return out.value
 
     def __del__(self, _d=_destroy):
         if getattr(self, "h", None) and self.h.value:
@@ -736,7 +743,119 @@ class NSESystem:
         call("pmx_nse_errors", self.ctx.h, self.h, NSE_EXAMPLES[example], _ptr(x), out)
         return list(out)
 
+    def restrict_velocity(self, fine, u_fine, u_coarse):
+        """GridTransfer::RestrictFunction of [u1|u2|u3] from the next finer level."""
+        import torch
+        torch.cuda.synchronize()
+        call("pmx_nse_restrict_velocity", self.ctx.h, self.h, fine.h, _ptr(u_fine), _ptr(u_coarse))
+        self.ctx.synchronize()
+
     def __del__(self, _d=_destroy):
         if getattr(self, "h", None) and self.h.value:
             _d("pmx_nse_destroy", self.h)
             self.h = None
+
+
+class NavierStokes:
+    """NavierStokes<3> (src/System/NavierStokes.C) for NSTYPE 1, Q2/Q1 on the
+    unit cube: one NSESystem per multigrid level (finest last), the nonlinear
+    loop of the reference's driver (test/nse3d_solvers_test.c++:107-126):
+
+        ns.assemble_linear_terms(); ns.stop_it(0)
+        for k in 1, 2, ...: ns.solve(); ns.assemble_nonlinear_term(); if ns.stop_it(k): break
+
+    Parameters are the reference's database keys nonlinloop_epsilon,
+    nonlinloop_maxit, nonlinloop_minit, nonlinloop_slowfactor,
+    nonlinloop_damping_factor, nonlinloop_scale_epsilon_with_size,
+    nonlinloop_residual_relative_to_rhs (defaults of
+    src/General/ParameterDatabase.C:1294-1330)."""
+
+    def __init__(self, ctx, domain, levels, solver_params, nu=1.0, example="u2_p1", **db):
+        import torch
+        self.ctx = ctx
+        self.example = example
+        self.nu = nu
+        self.systems = [NSESystem(ctx, domain, l) for l in levels]
+        self.solution = [s.zeros() for s in self.systems]
+        f = self.systems[-1]
+        self.rhs = f.zeros()
+        self.defect = f.zeros()
+        self.solver = Solver(ctx, solver_params)
+        self.db = {"nonlinloop_epsilon": 1e-10, "nonlinloop_maxit": 100, "nonlinloop_minit": 1,
+                   "nonlinloop_slowfactor": 1e10, "nonlinloop_damping_factor": 1.0,
+                   "nonlinloop_scale_epsilon_with_size": False, "nonlinloop_residual_relative_to_rhs": False}
+        for k, v in db.items():
+            if k not in self.db:
+                raise KeyError(k)
+            self.db[k] = v
+        self.residuals = []           # (impuls^2, mass^2, full) per stop_it call
+        self.linear_iterations = []
+        self._torch = torch
+
+    def _assemble_all(self):
+        last = len(self.systems) - 1
+        for k, (s, x) in enumerate(zip(self.systems, self.solution)):
+            s.assemble(self.example, self.nu, x[:3 * s.n_u], self.rhs if k == last else None)
+
+    def assemble_linear_terms(self):
+        """All blocks with the current velocity as convection field (zero at
+        the start); Dirichlet values copied from the rhs into the solution."""
+        self._assemble_all()
+        f, x = self.systems[-1], self.solution[-1]
+        nu, na = f.n_u, f.n_u_active
+        for k in range(3):
+            x[k * nu + na:(k + 1) * nu] = self.rhs[k * nu + na:(k + 1) * nu]
+
+    def assemble_nonlinear_term(self):
+        """Velocity restricted to every coarser level (RestrictFunctionRepeatedly),
+        then the blocks of every level with its own velocity as convection."""
+        for l in range(len(self.systems) - 2, -1, -1):
+            c, fsys = self.systems[l], self.systems[l + 1]
+            c.restrict_velocity(fsys, self.solution[l + 1], self.solution[l])
+        self._assemble_all()
+
+    def compute_residuals(self):
+        f = self.systems[-1]
+        nu = f.n_u
+        self.defect.copy_(self.rhs)
+        f.apply_scaled_add(self.solution[-1], self.defect, -1.0)
+        f.project_pressure(self.defect)
+        imp = float(self._torch.dot(self.defect[:3 * nu], self.defect[:3 * nu]))
+        mass = float(self._torch.dot(self.defect[3 * nu:], self.defect[3 * nu:]))
+        self.residuals.append((imp, mass, (imp + mass) ** 0.5))
+
+    def get_full_residual(self):
+        return self.residuals[-1][2]
+
+    def stop_it(self, k):
+        """NavierStokes<d>::stop_it (src/System/NavierStokes.C:750-818)."""
+        self.compute_residuals()
+        res = self.get_full_residual()
+        if k == 0:
+            self.initial_residual = res
+            self.initial_rhs_norm = float(self._torch.linalg.norm(self.rhs))
+        if k < self.db["nonlinloop_minit"]:
+            return False
+        old = self.residuals[max(0, len(self.residuals) - 10)][2]
+        slow = res >= self.db["nonlinloop_slowfactor"] * old
+        limit = self.db["nonlinloop_epsilon"]
+        if self.db["nonlinloop_scale_epsilon_with_size"]:
+            limit *= self.systems[-1].n ** 0.5
+        if self.db["nonlinloop_residual_relative_to_rhs"]:
+            limit *= self.initial_rhs_norm
+        return res <= limit or k == self.db["nonlinloop_maxit"] or slow
+
+    def solve(self):
+        """NavierStokes<d>::solve (src/System/NavierStokes.C:880-915): update the
+        solver with the current matrices, solve from the current solution,
+        damping, pressure projection."""
+        f, x = self.systems[-1], self.solution[-1]
+        damping = self.db["nonlinloop_damping_factor"]
+        old = x.clone() if damping != 1.0 else None
+        self.solver.update_saddle(self.systems)
+        rep = self.solver.solve(self.rhs, x)
+        self.linear_iterations.append(rep.iterations)
+        if damping != 1.0:
+            x.mul_(damping).add_(old, alpha=1.0 - damping)
+        f.project_pressure(x)
+        return rep

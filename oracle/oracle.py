@@ -979,3 +979,130 @@ def fgmres(apply, prec, b, x0, restart=30, max_it=100, tol=1e-8, reduction=0.0):
         r = b - apply(x)
         beta = np.linalg.norm(r)
     return x, max_it
+
+
+# ------------------------------------------------ nonlinear (Picard) iteration
+def _q2_values(xi):
+    """Q2 Lagrange basis on [-1, 1]^3 at points xi [m, 3] -> [m, 27] (node
+    order of ref_nodes: x fastest)."""
+    def p2(t):
+        return np.stack([0.5 * t * (t - 1.0), 1.0 - t * t, 0.5 * t * (t + 1.0)], -1)
+    bx, by, bz = p2(xi[:, 0]), p2(xi[:, 1]), p2(xi[:, 2])
+    return np.stack([bx[:, k % 3] * by[:, (k // 3) % 3] * bz[:, k // 9] for k in range(27)], -1)
+
+
+def _gauss3_cube():
+    g = np.sqrt(0.6)
+    p1, w1 = np.array([-g, 0.0, g]), np.array([5.0, 8.0, 5.0]) / 9.0
+    X, Y, Z = np.meshgrid(p1, p1, p1, indexing="ij")
+    W = np.einsum("i,j,k->ijk", w1, w1, w1)
+    pts = np.stack([X.ravel(order="F"), Y.ravel(order="F"), Z.ravel(order="F")], -1)
+    return pts, W.ravel(order="F")
+
+
+def restrict_function(cs, fs, uf):
+    """GridTransfer::RestrictFunction (src/Multigrid/GridTransfer.C:1218-1501)
+    for Q2 on axis-parallel hexahedra: fine cells are visited in order; the
+    first child met of a parent triggers the parent's children (in order),
+    Val2 = sum_j G^-1 R_j u_j with G the coarse reference mass matrix and R_j
+    the coarse/fine mixed mass matrix of child j (both from
+    TFEDatabase3D::GetRestrictionMatrix3D, src/FE/FEDatabase3D.C:1771-1990),
+    added to the parent's DOFs; finally every coarse value is divided by the
+    number of coarse cells containing it."""
+    pts, w = _gauss3_cube()
+    phc = _q2_values(pts)
+    G = phc.T @ (w[:, None] * phc)
+    uc = np.zeros(cs.n_dof)
+    aux = np.zeros(cs.n_dof)
+    children = {}
+    for fc in range(fs.n_cells):
+        children.setdefault(int(fs.parent[fc]), []).append(fc)
+    handled = set()
+    def affine(verts):
+        # reference map of the cell (vertex-weighted, affine for parallelepipeds)
+        x0 = node_phys(HEXA, verts, np.zeros(3))
+        J = np.stack([node_phys(HEXA, verts, e) - x0 for e in np.eye(3)], -1)
+        return x0, J
+
+    for fc in range(fs.n_cells):
+        pc = int(fs.parent[fc])
+        if pc in handled:
+            continue
+        handled.add(pc)
+        p0, pJ = affine(cs.verts[pc])
+        val2 = np.zeros(27)
+        for ch in children[pc]:
+            c0, cJ = affine(fs.verts[ch])
+            x = c0 + pts @ cJ.T                                    # child reference -> physical
+            xp = np.linalg.solve(pJ, (x - p0).T).T                 # physical -> parent reference
+            R = _q2_values(xp).T @ ((w / 8.0)[:, None] * phc)      # R[k, l] = (phi_k^H, phi_l^h)
+            RQ = np.linalg.solve(G, R)
+            val2 += RQ @ uf[fs.gnum[ch]]
+        dofs = cs.gnum[pc]
+        uc[dofs] += val2
+        aux[dofs] += 1.0
+    return uc / aux
+
+
+def q1_mean(ps, p):
+    """Mean value of a Q1 function on the union of its (axis-parallel
+    hexahedral) cells: a trilinear function's integral over a brick is the
+    brick volume times the mean of its vertex values."""
+    vol = np.prod(ps.verts.max(1) - ps.verts.min(1), axis=1)
+    return float(np.sum(vol * p[ps.gnum].mean(1)) / np.sum(vol))
+
+
+def nse_picard(n_refine, levels, nu, mg_kw, lin_tol=1e-11, lin_max_it=200, restart=30, eps=1e-10, max_it=10,
+               min_it=1, slowfactor=1e10):
+    """NavierStokes<3> nonlinear loop as test/nse3d_solvers_test.c++:107-126
+    runs it: assemble_linear_terms (convection with the initial, zero,
+    velocity; Dirichlet values copied into the solution), stop_it(0), then
+    repeatedly solve (FGMRES + saddle multigrid from the current solution,
+    pressure projected into L2_0), assemble_nonlinear_term (velocity restricted
+    to the coarser levels, GridTransfer::RestrictFunctionRepeatedly) and
+    stop_it(k) (src/System/NavierStokes.C:750-818: full residual
+    sqrt(|r_u|^2 + |r_p|^2) of rhs - K x with the pressure part of the defect
+    projected into L2_0).  Returns the solution, the residual history and the
+    FGMRES iteration counts."""
+    sys_ = [NseOracle(n_refine, l) for l in levels]
+    sols = [np.zeros(s.n) for s in sys_]
+    F = sys_[-1]
+    nu_, na = F.n_u, F.n_u_active
+
+    def assemble():
+        return [s.assemble(nu, x[:3 * s.n_u]) for s, x in zip(sys_, sols)]
+
+    blocks = assemble()
+    rhs = blocks[-1][2]
+    x = sols[-1]
+    for k in range(3):
+        x[k * nu_ + na:(k + 1) * nu_] = rhs[k * nu_ + na:(k + 1) * nu_]
+
+    def residual(A, B):
+        d = rhs - F.apply(A, B, x)
+        d[3 * nu_:] -= q1_mean(F.ps, d[3 * nu_:])
+        return np.sqrt(np.dot(d[:3 * nu_], d[:3 * nu_]) + np.dot(d[3 * nu_:], d[3 * nu_:]))
+
+    hist = [residual(blocks[-1][0], blocks[-1][1])]
+    lin_its = []
+    k = 0
+    while True:
+        k += 1
+        lv = [NseLevelOracle(s, b[0], b[1]) for s, b in zip(sys_, blocks)]
+        A, B = blocks[-1][0], blocks[-1][1]
+        x, its = fgmres(lambda z: F.apply(A, B, z), lambda v: nse_mg_apply(lv, v, **mg_kw), rhs, x,
+                        restart=restart, max_it=lin_max_it, tol=lin_tol)
+        x[3 * nu_:] -= q1_mean(F.ps, x[3 * nu_:])
+        lin_its.append(its)
+        sols[-1] = x
+        for l in range(len(sys_) - 2, -1, -1):
+            cs, fsv = sys_[l], sys_[l + 1]
+            for c in range(3):
+                sols[l][c * cs.n_u:(c + 1) * cs.n_u] = restrict_function(
+                    cs.vs, fsv.vs, sols[l + 1][c * fsv.n_u:(c + 1) * fsv.n_u])
+        blocks = assemble()
+        hist.append(residual(blocks[-1][0], blocks[-1][1]))
+        old = hist[max(0, len(hist) - 10)]
+        if k >= min_it and (hist[-1] <= eps or k == max_it or hist[-1] >= slowfactor * old):
+            break
+    return x, hist, lin_its

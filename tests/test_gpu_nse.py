@@ -210,3 +210,54 @@ def test_nse_multilevel_vanka_converges(ctx, nref, nlev, precision):
     g.project_pressure(x)
     e = g.errors(x)
     assert e[0] < 1e-8 and e[3] < 1e-7, e
+
+
+def test_nse_restrict_velocity_matches_oracle(ctx):
+    """GridTransfer::RestrictFunction of the three velocity components (4^3 ->
+    2^3 hexahedra) against the oracle, and the restriction of the example's
+    quadratic velocity is its coarse interpolant."""
+    d = pm.TDomain("Default_UnitCube_Hexa", 2)
+    gc, gf = pm.NSESystem(ctx, d, 1), pm.NSESystem(ctx, d, 2)
+    oc, of = O.NseOracle(2, 1), O.NseOracle(2, 2)
+    u = np.random.default_rng(5).standard_normal(3 * gf.n_u)
+    out = torch.zeros(3 * gc.n_u, dtype=torch.float64, device="cuda")
+    gc.restrict_velocity(gf, dev(u), out)
+    got = out.cpu().numpy()
+    for k in range(3):
+        ref = O.restrict_function(oc.vs, of.vs, u[k * gf.n_u:(k + 1) * gf.n_u])
+        assert np.max(np.abs(got[k * gc.n_u:(k + 1) * gc.n_u] - ref)) <= 1e-12 * np.max(np.abs(ref))
+    xf, xc = gf.zeros(), gc.zeros()
+    gf.interpolate(xf)
+    gc.interpolate(xc)
+    gc.restrict_velocity(gf, xf, out)
+    assert torch.max(torch.abs(out - xc[:3 * gc.n_u])).item() < 1e-13
+
+
+def test_nse_picard_matches_oracle(ctx):
+    """The nonlinear loop of NavierStokes<3> (Picard: convection with the last
+    velocity, restricted to the coarse level) on 4^3 hexahedra with two
+    multigrid levels: the residual history follows the oracle's and the loop
+    ends at the example's interpolant (the exact discrete solution)."""
+    d = pm.TDomain("Default_UnitCube_Hexa", 2)
+    prm = pm.solver_params(residual_tolerance=1e-11, residual_reduction=0.0, max_n_iterations=200, **MG_KW)
+    ns = pm.NavierStokes(ctx, d, [1, 2], prm, nu=0.1, nonlinloop_epsilon=1e-10, nonlinloop_maxit=20)
+    ns.assemble_linear_terms()
+    ns.stop_it(0)
+    k = 0
+    while True:
+        k += 1
+        ns.solve()
+        ns.assemble_nonlinear_term()
+        if ns.stop_it(k):
+            break
+    hist = [r[2] for r in ns.residuals]
+    kw = dict(damp_corr=0.8, vanka_damp=0.5, n_pre=2, n_post=2)
+    xo, ho, its = O.nse_picard(2, [1, 2], 0.1, kw, eps=1e-10, max_it=20)
+    print("gpu", hist, ns.linear_iterations, "oracle", ho, its, flush=True)
+    assert len(hist) == len(ho)
+    for a, b in zip(hist, ho):
+        if b > 1e-8:
+            assert abs(a - b) <= 1e-6 * b, (a, b)
+    f = ns.systems[-1]
+    e = f.errors(ns.solution[-1])
+    assert e[0] < 1e-9 and e[3] < 1e-8, e

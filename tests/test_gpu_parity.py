@@ -255,23 +255,27 @@ def test_multigrid_cycle_matches_oracle(ctx):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("cycle", ["V", "W"])
-def test_zero_start_sweep_is_bitwise_the_full_sweep(ctx, precision, cycle):
-    """The first Jacobi step on a level whose solution was reset multiplies
-    only the Dirichlet-column entries; cycles and solves with and without the
-    shortcut agree bit for bit (the preconditioned vector has non-zero
-    Dirichlet entries, so the Dirichlet couplings are exercised)."""
+def test_zero_start_sweep_and_renumbering(ctx, precision, cycle):
+    """Two execution choices of the multigrid.  The first Jacobi step on a
+    level whose solution was reset multiplies only the Dirichlet-column
+    entries: cycles and solves agree bit for bit with the full sweep (the
+    preconditioned vector has non-zero Dirichlet entries, so the Dirichlet
+    couplings are exercised).  The levels above the coarsest run in a
+    lexicographic DOF order with row entries sorted by the new column: only
+    the rounding of the row sums changes (1e-12 FP64, 1e-5 with FP32/bf16),
+    and the solve takes the same number of iterations."""
     nref = 3
     d, spaces, mats, rhs = three_level_cd3d(ctx, nref, MG_KW)
     fs = spaces[-1]
     out = {}
-    for shortcut in (0, 1):
+    for shortcut, renumber in ((0, 0), (1, 0), (0, 1), (1, 1)):
         for coarse in ("jacobi", "direct_solve"):
             prm = pm.solver_params(iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
                                    residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=100,
                                    multigrid_cycle_type=cycle, multigrid_smoother="jacobi",
                                    multigrid_smoother_coarse=coarse, multigrid_n_pre_smooth=2,
                                    multigrid_n_post_smooth=2, jacobi_damp=0.5, precision=precision,
-                                   zero_start_shortcut=shortcut)
+                                   zero_start_shortcut=shortcut, renumber_levels=renumber)
             sol = pm.Solver(ctx, prm)
             sol.update_matrix(mats, spaces)
             z = torch.zeros_like(rhs[-1])
@@ -279,11 +283,16 @@ def test_zero_start_sweep_is_bitwise_the_full_sweep(ctx, precision, cycle):
             x = torch.zeros_like(rhs[-1])
             x[fs.active_bound:] = rhs[-1][fs.active_bound:]
             rep = sol.solve(rhs[-1], x)
-            out[(shortcut, coarse)] = (z.cpu().numpy(), x.cpu().numpy(), rep.iterations)
+            out[(shortcut, renumber, coarse)] = (z.cpu().numpy(), x.cpu().numpy(), rep.iterations)
+    tol = 1e-12 if precision == "fp64" else 1e-5
     for coarse in ("jacobi", "direct_solve"):
-        a, b = out[(0, coarse)], out[(1, coarse)]
-        assert np.array_equal(a[0], b[0]), coarse
-        assert np.array_equal(a[1], b[1]) and a[2] == b[2], coarse
+        for renumber in (0, 1):
+            a, b = out[(0, renumber, coarse)], out[(1, renumber, coarse)]
+            assert np.array_equal(a[0], b[0]), (renumber, coarse)
+            assert np.array_equal(a[1], b[1]) and a[2] == b[2], (renumber, coarse)
+        a, b = out[(1, 0, coarse)], out[(1, 1, coarse)]
+        assert np.max(np.abs(a[0] - b[0])) <= tol * np.max(np.abs(a[0])), coarse
+        assert abs(a[2] - b[2]) <= 1 and np.max(np.abs(a[1] - b[1])) <= 1e-7 * np.max(np.abs(a[1])), coarse
 
 
 def test_fgmres_multigrid_solution_matches_oracle(ctx):

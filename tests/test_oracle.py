@@ -239,3 +239,36 @@ def test_nse_pattern_closed_forms():
     assert ns.n_u == (2 * n + 1) ** 3 and ns.n_u_active == (2 * n - 1) ** 3 and ns.n_p == (n + 1) ** 3
     lens = np.diff(ns.brp)
     assert lens.max() == 125 and np.sum(lens == 125) == (n - 1) ** 3
+
+
+def test_function_restriction_closed_forms():
+    """GridTransfer::RestrictFunction restated: on nested Q2 spaces the local
+    L2 projection returns a prolongated coarse function exactly (RF P = I) and
+    reproduces quadratic interpolants; a wrong local DOF order (planted: two
+    local DOFs of one fine cell exchanged) breaks RF P = I."""
+    cs, fs = O.NseOracle(2, 1).vs, O.NseOracle(2, 2).vs
+    uc = np.random.default_rng(1).standard_normal(cs.n_dof)
+    uf = O.prolongate(cs, fs, uc)
+    assert np.max(np.abs(O.restrict_function(cs, fs, uf) - uc)) < 1e-13
+    xf, xc = O.nodal_coords(fs), O.nodal_coords(cs)
+    f = lambda x: x[:, 0] ** 2 - 2 * x[:, 1] * x[:, 2] + x[:, 2]
+    assert np.max(np.abs(O.restrict_function(cs, fs, f(xf)) - f(xc))) < 1e-13
+    # planted: swap two velocity DOFs of one fine cell's local order
+    bad = fs.gnum.copy()
+    fs2 = O.OracleSpace(fs.shape, fs.order, fs.verts, bad, fs.n_dof, fs.active, fs.rp, fs.col, fs.bd, fs.parent)
+    fs2.gnum[0, [0, 1]] = fs2.gnum[0, [1, 0]]
+    assert np.max(np.abs(O.restrict_function(cs, fs2, uf) - uc)) > 1e-3
+
+
+def test_nse_picard_oracle_converges_to_interpolant():
+    """test_u_2_p_1 is quadratic/linear, so its interpolant is the discrete
+    Navier-Stokes solution (convection included): the Picard iteration
+    converges to it, and the nonlinear residual decreases monotonically."""
+    kw = dict(damp_corr=0.8, vanka_damp=0.5, n_pre=2, n_post=2)
+    x, hist, its = O.nse_picard(2, [1, 2], 0.1, kw, eps=1e-10, max_it=20)
+    ns = O.NseOracle(2, 2)
+    xi = O.nse_interpolate(ns)
+    xi[3 * ns.n_u:] -= O.q1_mean(ns.ps, xi[3 * ns.n_u:])
+    assert hist[-1] <= 1e-10
+    assert all(b < a for a, b in zip(hist, hist[1:]))
+    assert np.max(np.abs(x - xi)) < 1e-8
