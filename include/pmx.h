@@ -449,7 +449,11 @@ typedef struct {
                                   lexicographic DOF order with the entries of each row
                                   sorted by the new column (better gather locality; the
                                   row sums of the preconditioner are rounded in another
-                                  order).  0: the reference order throughout */
+                                  order).  2: as 1, and where every column offset fits in
+                                  16 bits the level matrix stores col - row as int16 (same
+                                  results as 1, bit for bit; fewer bytes but slower on
+                                  MI355X, the Dirichlet columns then need a per-row tail).
+                                  0: the reference order */
 } pmx_solver_params;
 /* Defaults of Solver<>::default_solver_database (src/Solver/Solver.C:20-117)
  * merged with Multigrid::default_multigrid_database (src/Multigrid/Multigrid.C:422-514). */

@@ -24,20 +24,28 @@ namespace pmx_internal {
 
 constexpr int kSellSigma = 4096;
 
-void attach_sell(pmx_matrix* M, hipStream_t st) {
-  const pmx::SellPattern sp = pmx::build_sell(*M->pattern, kSellSigma);
-  pmx::DevSELL& S = M->A.sell;
+pmx::DevSELL upload_sell(const pmx::SellPattern& sp, hipStream_t st) {
+  pmx::DevSELL S;
   S.n_slices = sp.n_slices;
-  S.total = sp.col.size();
+  S.total = sp.src.size();   // slots of either layout (col is empty in the 16-bit one)
+  if ((size_t)sp.slice_ptr.back() != S.total || (sp.off.empty() ? sp.col.size() : sp.off.size()) != S.total)
+    throw PmxError(PMX_E_ARG, "inconsistent SELL layout");
   S.slice_ptr = upload(sp.slice_ptr.data(), sp.slice_ptr.size(), st);
   S.row = upload(sp.row.data(), sp.row.size(), st);
-  S.col = upload(sp.col.data(), sp.col.size(), st);
+  S.col = sp.col.empty() ? nullptr : upload(sp.col.data(), sp.col.size(), st);
+  S.off = sp.off.empty() ? nullptr : upload(sp.off.data(), sp.off.size(), st);
   S.src = upload(sp.src.data(), sp.src.size(), st);
-  S.n_bd = (int)sp.bd_pos.size();
+  S.n_bd = (int)sp.bd_src.size();
   S.bd_ptr = upload(sp.bd_ptr.data(), sp.bd_ptr.size(), st);
-  S.bd_pos = sp.bd_pos.empty() ? nullptr : upload(sp.bd_pos.data(), sp.bd_pos.size(), st);
+  S.bd_src = sp.bd_src.empty() ? nullptr : upload(sp.bd_src.data(), sp.bd_src.size(), st);
   S.bd_col = sp.bd_col.empty() ? nullptr : upload(sp.bd_col.data(), sp.bd_col.size(), st);
-  S.val = dmalloc<double>(S.total);
+  return S;
+}
+
+void attach_sell(pmx_matrix* M, hipStream_t st) {
+  const pmx::SellPattern sp = pmx::build_sell(*M->pattern, kSellSigma);
+  M->A.sell = upload_sell(sp, st);
+  M->A.sell.val = dmalloc<double>(M->A.sell.total);
   M->A.lpr = 0;
   M->sell_valid = false;
 }

@@ -15,6 +15,7 @@
 //
 // Every kernel is HBM-bound: one row is handled by a group of LPR lanes of a
 // 64-wide wavefront so that the row's values/columns are read coalesced.
+#include <stdexcept>
 #include <type_traits>
 
 #include "common.hpp"
@@ -87,12 +88,14 @@ void launch_fe_spmv(const DevCSR& A, const VT* va, const float* rscale, const XT
 // (the summation order of TMatrix::multiply).  Eight entries are loaded per
 // step before their x values are gathered, to keep enough bytes in flight.
 // Blocks beyond the active slices handle the non-active (identity) rows.
-template <int MODE, typename VT, typename XT>
+template <int MODE, typename VT, typename XT, bool REL16>
 __global__ __launch_bounds__(256) void k_sell_spmv(
     int n_slices, int sell_blocks, int n, int active, const int* __restrict__ sptr,
-    const int* __restrict__ rowmap, const int* __restrict__ ci, const VT* __restrict__ va,
-    const float* __restrict__ rscale, const XT* __restrict__ x, const XT* __restrict__ b,
-    const XT* __restrict__ dinv, XT* __restrict__ y, double a, double* __restrict__ partial) {
+    const int* __restrict__ rowmap, const int* __restrict__ ci, const int16_t* __restrict__ off,
+    const int* __restrict__ bd_ptr, const int* __restrict__ bd_col, const VT* __restrict__ bd_val,
+    const VT* __restrict__ va, const float* __restrict__ rscale, const XT* __restrict__ x,
+    const XT* __restrict__ b, const XT* __restrict__ dinv, XT* __restrict__ y, double a,
+    double* __restrict__ partial) {
   __shared__ double red[16];
   using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
   double d2 = 0;
@@ -109,6 +112,8 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
       const int s0 = sptr[slice];
       const int width = (sptr[slice + 1] - s0) >> 6;
       const int* cp = ci + s0 + lane;
+      const int16_t* op = off + s0 + lane;
+      const int rb = r >= 0 ? r : 0;   // 16-bit layout: column = row + offset (empty lanes: offset 0)
       const VT* vp = va + s0 + lane;
       AT acc = 0;
       int j = 0;
@@ -116,7 +121,10 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
         int c[8];
         AT v[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) c[u] = __builtin_nontemporal_load(cp + (j + u) * 64);
+        for (int u = 0; u < 8; ++u) {
+          if constexpr (REL16) c[u] = rb + (int)__builtin_nontemporal_load(op + (j + u) * 64);
+          else c[u] = __builtin_nontemporal_load(cp + (j + u) * 64);
+        }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           if constexpr (sizeof(VT) == 2) v[u] = (AT)bf16_to_float(__builtin_nontemporal_load(vp + (j + u) * 64));
@@ -126,11 +134,25 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
         for (int u = 0; u < 8; ++u) acc += v[u] * (AT)x[c[u]];
       }
       for (; j < width; ++j) {
-        const int c = __builtin_nontemporal_load(cp + j * 64);
+        int c;
+        if constexpr (REL16) c = rb + (int)__builtin_nontemporal_load(op + j * 64);
+        else c = __builtin_nontemporal_load(cp + j * 64);
         AT v;
         if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(__builtin_nontemporal_load(vp + j * 64));
         else v = (AT)__builtin_nontemporal_load(vp + j * 64);
         acc += v * (AT)x[c];
+      }
+      if constexpr (REL16) {
+        // the Dirichlet-column entries, last in the row's column order
+        if (r >= 0) {
+          const int e = bd_ptr[r + 1];
+          for (int k = bd_ptr[r]; k < e; ++k) {
+            AT v;
+            if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(bd_val[k]);
+            else v = (AT)bd_val[k];
+            acc += v * (AT)x[bd_col[k]];
+          }
+        }
       }
       if (r >= 0) {
         if (rscale) acc *= (AT)rscale[r];
@@ -211,13 +233,22 @@ int sell_grid(const DevCSR& A, int* sell_blocks) {
 }
 
 template <int MODE, typename VT, typename XT>
-void launch_sell(const DevCSR& A, const VT* va, const float* rscale, const XT* x, const XT* b,
-                 const XT* dinv, XT* y, double a, double* partial, hipStream_t st) {
+void launch_sell(const DevCSR& A, const VT* va, const VT* bd_val, const float* rscale, const XT* x,
+                 const XT* b, const XT* dinv, XT* y, double a, double* partial, hipStream_t st) {
   int sb = 0;
   const int grid = sell_grid(A, &sb);
-  hipLaunchKernelGGL((k_sell_spmv<MODE, VT, XT>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb,
-                     A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, va, rscale, x, b, dinv, y,
-                     a, partial);
+  if (A.sell.off && A.sell.n_bd > 0 && (!bd_val || !A.sell.bd_ptr || !A.sell.bd_col))
+    throw std::runtime_error("16-bit SELL layout without its Dirichlet-column values");
+  if (A.sell.n_slices > 0 && (A.sell.total == 0 || !va || (!A.sell.off && !A.sell.col)))
+    throw std::runtime_error("SELL copy without slots, values or columns");
+  if (A.sell.off)
+    hipLaunchKernelGGL((k_sell_spmv<MODE, VT, XT, true>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb,
+                       A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, A.sell.off, A.sell.bd_ptr,
+                       A.sell.bd_col, bd_val, va, rscale, x, b, dinv, y, a, partial);
+  else
+    hipLaunchKernelGGL((k_sell_spmv<MODE, VT, XT, false>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb,
+                       A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, A.sell.off, A.sell.bd_ptr,
+                       A.sell.bd_col, bd_val, va, rscale, x, b, dinv, y, a, partial);
 }
 
 __global__ void k_sell_fill_f64(size_t total, const int* __restrict__ src, const double* __restrict__ v,
@@ -258,9 +289,11 @@ void dispatch_fe_spmv(const DevCSR& A, const VT* va, const float* rscale, const 
   const bool have_sell = (sizeof(VT) == 2) ? (A.sell.val_bf16 != nullptr) : (A.sell.val != nullptr);
   if (have_sell && A.lpr == 0) {
     if constexpr (sizeof(VT) == 2)
-      launch_sell<MODE, VT, XT>(A, (const VT*)A.sell.val_bf16, rscale, x, b, dinv, y, a, partial, st);
+      launch_sell<MODE, VT, XT>(A, (const VT*)A.sell.val_bf16, (const VT*)A.sell.bd_val_bf16, rscale, x, b,
+                                dinv, y, a, partial, st);
     else
-      launch_sell<MODE, VT, XT>(A, (const VT*)A.sell.val, rscale, x, b, dinv, y, a, partial, st);
+      launch_sell<MODE, VT, XT>(A, (const VT*)A.sell.val, (const VT*)A.sell.bd_val, rscale, x, b, dinv, y, a,
+                                partial, st);
     return;
   }
   switch (A.lpr) {
@@ -476,32 +509,44 @@ void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* 
 template <typename VT, typename XT>
 static bool jacobi_from_zero(const DevCSR& A, const VT* bd_val, const float* rscale, const XT* x,
                              const XT* b, const XT* dinv, XT* xn, double omega, hipStream_t st) {
-  if (A.lpr != 0 || !A.sell.bd_ptr || (!bd_val && A.sell.bd_pos)) return false;
+  if (A.lpr != 0 || !A.sell.bd_ptr || (!bd_val && A.sell.n_bd > 0)) return false;
   if (A.n == 0) return true;
   hipLaunchKernelGGL((k_sell_jacobi_from_zero<VT, XT>), dim3((A.n + 255) / 256), dim3(256), 0, st, A.n,
                      A.active, A.sell.bd_ptr, A.sell.bd_col, bd_val, rscale, x, b, dinv, xn, omega);
   return true;
 }
-bool fe_jacobi_from_zero(const DevCSR& A, const double* bd_val, const double* x, const double* b,
-                         const double* dinv, double* xn, double omega, hipStream_t st) {
-  return jacobi_from_zero<double, double>(A, bd_val, nullptr, x, b, dinv, xn, omega, st);
+bool fe_jacobi_from_zero(const DevCSR& A, const double* x, const double* b, const double* dinv, double* xn,
+                         double omega, hipStream_t st) {
+  return jacobi_from_zero<double, double>(A, A.sell.bd_val, nullptr, x, b, dinv, xn, omega, st);
 }
-bool fe_jacobi_from_zero_bf16(const DevCSR& A, const uint16_t* bd_val, const float* x, const float* b,
-                              const float* dinv, float* xn, double omega, hipStream_t st) {
-  return jacobi_from_zero<uint16_t, float>(A, bd_val, A.rscale, x, b, dinv, xn, omega, st);
+bool fe_jacobi_from_zero_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv, float* xn,
+                              double omega, hipStream_t st) {
+  return jacobi_from_zero<uint16_t, float>(A, A.sell.bd_val_bf16, A.rscale, x, b, dinv, xn, omega, st);
 }
-template <typename T>
-__global__ void k_gather_positions(int n, const int* __restrict__ pos, const T* __restrict__ src,
-                                   T* __restrict__ dst) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[pos[i]];
+// one thread per active row: its Dirichlet-column values (FP64 copy and/or
+// bf16 of value / row scale, rounded to nearest even as in k_sell_fill_bf16)
+__global__ void k_sell_dirichlet_fill(int na, const int* __restrict__ bd_ptr, const int* __restrict__ bd_src,
+                                      const double* __restrict__ v, const float* __restrict__ rscale,
+                                      double* __restrict__ out64, uint16_t* __restrict__ out16) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= na) return;
+  const int e = bd_ptr[r + 1];
+  const double inv = rscale ? 1.0 / (double)rscale[r] : 0.0;
+  for (int k = bd_ptr[r]; k < e; ++k) {
+    const double a = v[bd_src[k]];
+    if (out64) out64[k] = a;
+    if (out16) {
+      const float f = (float)(a * inv);
+      uint32_t u = __float_as_uint(f);
+      u += 0x7FFFu + ((u >> 16) & 1u);
+      out16[k] = (uint16_t)(u >> 16);
+    }
+  }
 }
-void sell_dirichlet_values(const DevCSR& A, int n_bd, double* out64, uint16_t* out16, hipStream_t st) {
-  if (n_bd <= 0 || !A.sell.bd_pos) return;
-  const dim3 g((n_bd + 255) / 256);
-  if (out64) hipLaunchKernelGGL(k_gather_positions<double>, g, dim3(256), 0, st, n_bd, A.sell.bd_pos, A.sell.val, out64);
-  if (out16)
-    hipLaunchKernelGGL(k_gather_positions<uint16_t>, g, dim3(256), 0, st, n_bd, A.sell.bd_pos, A.sell.val_bf16, out16);
+void sell_dirichlet_fill(const DevCSR& A, double* out64, uint16_t* out16, hipStream_t st) {
+  if (A.sell.n_bd <= 0 || !A.sell.bd_ptr || A.active <= 0) return;
+  hipLaunchKernelGGL(k_sell_dirichlet_fill, dim3((A.active + 255) / 256), dim3(256), 0, st, A.active,
+                     A.sell.bd_ptr, A.sell.bd_src, A.val, out16 ? A.rscale : nullptr, out64, out16);
 }
 
 template <typename S, typename D>

@@ -18,11 +18,17 @@ struct DevSELL {
   int* src = nullptr;             // [total] CSR position (-1: padding)
   double* val = nullptr;          // FP64 values (nullptr: not built)
   uint16_t* val_bf16 = nullptr;   // bf16 values divided by the row scale
-  // active-row entries in Dirichlet columns (row order): SELL slot, column
+  // active-row entries in Dirichlet columns (row order): CSR position, column
+  // and (filled by the owner of the values, see sell_dirichlet_fill) value
   int n_bd = 0;                   // number of such entries
   int* bd_ptr = nullptr;          // [active+1] (nullptr: not built)
-  int* bd_pos = nullptr;
+  int* bd_src = nullptr;
   int* bd_col = nullptr;
+  double* bd_val = nullptr;
+  uint16_t* bd_val_bf16 = nullptr;
+  // 16-bit layout: slots hold col - row of the active-column entries (col and
+  // the Dirichlet-column entries are then only in the bd lists)
+  int16_t* off = nullptr;
 };
 
 // Device CSR matrix (0-based int32 row pointer / columns, sorted columns).
@@ -54,21 +60,22 @@ void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* 
                double omega, hipStream_t st);
 // the same sweep for an x that is zero on the active rows (only the products
 // with the Dirichlet entries of x remain); bitwise equal to fe_jacobi on such
-// an x.  bd_val: the values of A.sell's Dirichlet-column slots in the order of
-// A.sell.bd_pos (sell_dirichlet_values).  Returns false (and launches nothing)
-// when A has no SELL copy.
-bool fe_jacobi_from_zero(const DevCSR& A, const double* bd_val, const double* x, const double* b,
-                         const double* dinv, double* xn, double omega, hipStream_t st);
-// compact copies of the Dirichlet-column slot values of A.sell (FP64 and/or bf16)
-void sell_dirichlet_values(const DevCSR& A, int n_bd, double* out64, uint16_t* out16, hipStream_t st);
+// an x.  Needs A.sell.bd_val (FP64) / bd_val_bf16 (bf16) when A has
+// Dirichlet couplings.  Returns false (and launches nothing) when A has no
+// SELL copy.
+bool fe_jacobi_from_zero(const DevCSR& A, const double* x, const double* b, const double* dinv, double* xn,
+                         double omega, hipStream_t st);
+// values of the Dirichlet-column entries from the CSR values: FP64 copies
+// and/or bf16 with the row scales A.rscale (the rounding of sell_fill_bf16)
+void sell_dirichlet_fill(const DevCSR& A, double* out64, uint16_t* out16, hipStream_t st);
 // ---- FE-matrix operators (bf16 matrix, FP32 vectors)
 void fe_apply_scaled_add_bf16(const DevCSR& A, const float* x, float* y, double a, hipStream_t st);
 void fe_defect_bf16(const DevCSR& A, const float* x, const float* b, float* d, double* partial,
                     double* norm_out, hipStream_t st);
 void fe_jacobi_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv, float* xn,
                     double omega, hipStream_t st);
-bool fe_jacobi_from_zero_bf16(const DevCSR& A, const uint16_t* bd_val, const float* x, const float* b,
-                              const float* dinv, float* xn, double omega, hipStream_t st);
+bool fe_jacobi_from_zero_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv, float* xn,
+                              double omega, hipStream_t st);
 // ---- plain CSR (transfer matrices): y = alpha*P x + beta*y on rows < row_end;
 //      rows >= row_end are set to 0 when zero_rest, else left unchanged
 void csr_spmv(const DevCSR& P, const double* x, double* y, double alpha, double beta, int row_end,

@@ -449,15 +449,33 @@ void color_cells(const FESpace& s, std::vector<int>& order, std::vector<int>& co
 
 namespace pmx {
 
-SellPattern build_sell(const CSRPattern& p, int sigma) {
+SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16) {
   constexpr int C = 64;
   SellPattern s;
   const int na = p.active_bound;
   s.n_active = na;
   s.n_slices = (na + C - 1) / C;
+  // slot entries of a row: all of them, or (rel16) those in active columns,
+  // which precede the Dirichlet columns in a row sorted by column
+  std::vector<int> nact(na);
+  for (int r = 0; r < na; ++r) {
+    int c = 0;
+    for (int q = p.row_ptr[r]; q < p.row_ptr[r + 1]; ++q) c += p.col[q] < na;
+    nact[r] = c;
+  }
+  if (rel16) {
+    bool fits = true;
+    for (int r = 0; r < na && fits; ++r)
+      for (int q = p.row_ptr[r]; q < p.row_ptr[r + 1]; ++q) {
+        const int c = p.col[q];
+        if (c >= na) { fits = fits && (q - p.row_ptr[r]) >= nact[r]; continue; }   // Dirichlet entries last
+        if ((q - p.row_ptr[r]) >= nact[r] || c - r > 32767 || c - r < -32768) { fits = false; break; }
+      }
+    if (!fits) return build_sell(p, sigma, false);
+  }
+  auto len = [&](int r) { return rel16 ? nact[r] : p.row_ptr[r + 1] - p.row_ptr[r]; };
   std::vector<int> order(na);
   for (int r = 0; r < na; ++r) order[r] = r;
-  auto len = [&](int r) { return p.row_ptr[r + 1] - p.row_ptr[r]; };
   for (int w = 0; w < na; w += sigma) {
     const int e = std::min(na, w + sigma);
     std::stable_sort(order.begin() + w, order.begin() + e, [&](int a, int b) { return len(a) > len(b); });
@@ -476,7 +494,8 @@ SellPattern build_sell(const CSRPattern& p, int sigma) {
     s.slice_ptr[sl + 1] = s.slice_ptr[sl] + width * C;
   }
   const size_t total = s.slice_ptr.back();
-  s.col.assign(total, 0);
+  if (rel16) s.off.assign(total, 0);
+  else s.col.assign(total, 0);
   s.src.assign(total, -1);
 #pragma omp parallel for schedule(static)
   for (int sl = 0; sl < s.n_slices; ++sl) {
@@ -486,33 +505,28 @@ SellPattern build_sell(const CSRPattern& p, int sigma) {
       for (int j = 0; j < width; ++j) {
         const size_t pos = (size_t)s.slice_ptr[sl] + (size_t)j * C + l;
         if (r >= 0 && j < len(r)) {
-          s.col[pos] = p.col[p.row_ptr[r] + j];
+          const int c = p.col[p.row_ptr[r] + j];
+          if (rel16) s.off[pos] = (int16_t)(c - r);
+          else s.col[pos] = c;
           s.src[pos] = p.row_ptr[r] + j;
-        } else {
-          s.col[pos] = (r >= 0) ? r : 0;
+        } else if (!rel16) {
+          s.col[pos] = (r >= 0) ? r : 0;   // padding: (row, 0); rel16 padding is offset 0
         }
       }
     }
   }
-  // slot of every (row, j) entry whose column is a Dirichlet DOF
-  std::vector<int> slot_of(na);
-  for (int k = 0; k < na; ++k) slot_of[s.row[k]] = k;
+  // Dirichlet-column entries of every active row, in row order
   s.bd_ptr.assign(na + 1, 0);
-  for (int r = 0; r < na; ++r) {
-    int c = 0;
-    for (int q = p.row_ptr[r]; q < p.row_ptr[r + 1]; ++q) c += p.col[q] >= na;
-    s.bd_ptr[r + 1] = s.bd_ptr[r] + c;
-  }
-  s.bd_pos.resize(s.bd_ptr[na]);
+  for (int r = 0; r < na; ++r) s.bd_ptr[r + 1] = s.bd_ptr[r] + (p.row_ptr[r + 1] - p.row_ptr[r] - nact[r]);
+  s.bd_src.resize(s.bd_ptr[na]);
   s.bd_col.resize(s.bd_ptr[na]);
+#pragma omp parallel for schedule(static)
   for (int r = 0; r < na; ++r) {
-    const int k = slot_of[r];
     int o = s.bd_ptr[r];
-    for (int j = 0; j < len(r); ++j) {
-      const int c = p.col[p.row_ptr[r] + j];
-      if (c < na) continue;
-      s.bd_pos[o] = s.slice_ptr[k / C] + j * C + (k % C);
-      s.bd_col[o] = c;
+    for (int q = p.row_ptr[r]; q < p.row_ptr[r + 1]; ++q) {
+      if (p.col[q] < na) continue;
+      s.bd_src[o] = q;
+      s.bd_col[o] = p.col[q];
       ++o;
     }
   }

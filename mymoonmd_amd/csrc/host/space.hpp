@@ -126,11 +126,17 @@ struct SellPattern {
   std::vector<int> col;         // [slice_ptr.back()]
   std::vector<int> src;         // [slice_ptr.back()] CSR position of the entry (-1: padding)
   // entries of the active rows in Dirichlet columns (col >= n_active), per
-  // row in column order: their slot in the SELL arrays and their column
+  // row in column order: their CSR position and their column
   std::vector<int> bd_ptr;      // [n_active+1]
-  std::vector<int> bd_pos;
+  std::vector<int> bd_src;
   std::vector<int> bd_col;
+  // rel16 layout: the slots hold only the active-column entries, as 16-bit
+  // offsets col - row (col empty); the Dirichlet-column entries (the last ones
+  // of every row) live only in the bd lists
+  std::vector<int16_t> off;
 };
-SellPattern build_sell(const CSRPattern& p, int sigma);
+// rel16: build the 16-bit offset layout; if an offset does not fit, the
+// result has an empty `off` and the 32-bit layout
+SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16 = false);
 
 }  // namespace pmx

@@ -126,11 +126,13 @@ struct pmx_matrix {
     dfree(A.row_ptr); dfree(A.col); dfree(A.val); dfree(A.val_bf16); dfree(A.rscale);
     dfree(A.sell.slice_ptr); dfree(A.sell.row); dfree(A.sell.col); dfree(A.sell.src);
     dfree(A.sell.val); dfree(A.sell.val_bf16);
-    dfree(A.sell.bd_ptr); dfree(A.sell.bd_pos); dfree(A.sell.bd_col);
+    dfree(A.sell.bd_ptr); dfree(A.sell.bd_src); dfree(A.sell.bd_col); dfree(A.sell.off);
   }
 };
 
 namespace pmx_internal {
+// device copy of a SELL layout (values not allocated); sizes checked
+pmx::DevSELL upload_sell(const pmx::SellPattern& sp, hipStream_t st);
 // upload the SELL layout of a matrix pattern (once) and mark its values stale
 void attach_sell(pmx_matrix* M, hipStream_t st);
 // refresh the SELL FP64 values from the CSR values when they are stale

@@ -110,7 +110,7 @@ struct Level {
   void release() {
     if (own_sell) {
       dfree(A.sell.slice_ptr); dfree(A.sell.row); dfree(A.sell.col); dfree(A.sell.src); dfree(A.sell.val);
-      dfree(A.sell.bd_ptr); dfree(A.sell.bd_pos); dfree(A.sell.bd_col);
+      dfree(A.sell.bd_ptr); dfree(A.sell.bd_src); dfree(A.sell.bd_col); dfree(A.sell.off);
       own_sell = false;
     }
     dfree(perm); dfree(bd64); dfree(bd16);
@@ -202,12 +202,12 @@ void mg_smooth(pmx_solver* s, int l, bool mixed) {
   const bool zero = L.sol_zero && s->p.zero_start_shortcut;
   L.sol_zero = false;
   if (mixed) {
-    if (!(zero && pmx::fe_jacobi_from_zero_bf16(L.A, L.bd16, L.sol32, L.rhs32, L.dinv32, L.tmp32,
-                                                s->p.jacobi_damp, S(s))))
+    if (!(zero && pmx::fe_jacobi_from_zero_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp,
+                                                S(s))))
       pmx::fe_jacobi_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp, S(s));
     std::swap(L.sol32, L.tmp32);
   } else {
-    if (!(zero && pmx::fe_jacobi_from_zero(L.A, L.bd64, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s))))
+    if (!(zero && pmx::fe_jacobi_from_zero(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s))))
       pmx::fe_jacobi(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s));
     std::swap(L.sol, L.tmp);
   }
@@ -806,22 +806,13 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
           pmx::CSRPattern pp;
           std::vector<int> opos;
           pmx::permute_csr(*M->pattern, nullptr, perm[l], pinv[l], pp, nullptr, &opos, true);
-          pmx::SellPattern sp = pmx::build_sell(pp, kSellSigmaSolver);
+          pmx::SellPattern sp = pmx::build_sell(pp, kSellSigmaSolver, s->p.renumber_levels >= 2);
           for (auto& k : sp.src)
             if (k >= 0) k = opos[k];
+          for (auto& k : sp.bd_src) k = opos[k];
           pmx::DevCSR Ap = L.A;
           Ap.lpr = 0;
-          Ap.sell = pmx::DevSELL{};
-          Ap.sell.n_slices = sp.n_slices;
-          Ap.sell.total = sp.col.size();
-          Ap.sell.slice_ptr = upload(sp.slice_ptr.data(), sp.slice_ptr.size(), st);
-          Ap.sell.row = upload(sp.row.data(), sp.row.size(), st);
-          Ap.sell.col = upload(sp.col.data(), sp.col.size(), st);
-          Ap.sell.src = upload(sp.src.data(), sp.src.size(), st);
-          Ap.sell.n_bd = (int)sp.bd_pos.size();
-          Ap.sell.bd_ptr = upload(sp.bd_ptr.data(), sp.bd_ptr.size(), st);
-          Ap.sell.bd_pos = sp.bd_pos.empty() ? nullptr : upload(sp.bd_pos.data(), sp.bd_pos.size(), st);
-          Ap.sell.bd_col = sp.bd_col.empty() ? nullptr : upload(sp.bd_col.data(), sp.bd_col.size(), st);
+          Ap.sell = upload_sell(sp, st);
           L.perm = upload(perm[l].data(), perm[l].size(), st);
           double* dp = dmalloc<double>(L.n);
           pmx::vec_gather(L.n, L.perm, L.dinv, dp, st);
@@ -850,13 +841,17 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
           pmx::sell_fill_bf16(L.A, L.A.sell.val_bf16, st);
         }
         L.partial = dmalloc<double>(std::max(4096, pmx::fe_defect_partials(L.A)));
-        if (L.A.sell.n_bd > 0) {   // compact Dirichlet-slot values for the zero-start sweep
-          if (mixed && L.A.sell.val_bf16) {
+        if (L.A.sell.n_bd > 0 && L.A.sell.bd_src) {
+          // values of the Dirichlet-column entries (zero-start sweep; the
+          // 16-bit layout keeps them only there)
+          if (mixed) {
             L.bd16 = dmalloc<uint16_t>(L.A.sell.n_bd);
-            pmx::sell_dirichlet_values(L.A, L.A.sell.n_bd, nullptr, L.bd16, st);
-          } else if (!mixed && L.A.sell.val) {
+            pmx::sell_dirichlet_fill(L.A, nullptr, L.bd16, st);
+            L.A.sell.bd_val_bf16 = L.bd16;
+          } else {
             L.bd64 = dmalloc<double>(L.A.sell.n_bd);
-            pmx::sell_dirichlet_values(L.A, L.A.sell.n_bd, L.bd64, nullptr, st);
+            pmx::sell_dirichlet_fill(L.A, L.bd64, nullptr, st);
+            L.A.sell.bd_val = L.bd64;
           }
         }
         if (mixed) {
