@@ -306,10 +306,13 @@ void dispatch_fe_spmv(const DevCSR& A, const VT* va, const float* rscale, const 
 }
 
 // y = alpha * P x + beta * y over all rows (grid transfer)
-template <int LPR, typename T>
+// VT = uint16_t: the values are bf16 numbers that equal the FP64 ones
+// exactly (nested-space interpolation weights such as 3/8 or -9/256), so the
+// result is the same as with FP64 values, for 2 instead of 8 bytes each
+template <int LPR, typename T, typename VT>
 __global__ __launch_bounds__(256) void k_csr_spmv(int n, const int* __restrict__ rp,
                                                    const int* __restrict__ ci,
-                                                   const double* __restrict__ va,
+                                                   const VT* __restrict__ va,
                                                    const T* __restrict__ x, T* __restrict__ y,
                                                    double alpha, double beta, int row_end,
                                                    int zero_rest) {
@@ -322,7 +325,12 @@ __global__ __launch_bounds__(256) void k_csr_spmv(int n, const int* __restrict__
   }
   double acc = 0;
   const int s = rp[row], e = rp[row + 1];
-  for (int k = s + lane; k < e; k += LPR) acc += va[k] * (double)x[ci[k]];
+  for (int k = s + lane; k < e; k += LPR) {
+    double v;
+    if constexpr (sizeof(VT) == 2) v = (double)bf16_to_float(va[k]);
+    else v = va[k];
+    acc += v * (double)x[ci[k]];
+  }
   acc = group_sum<LPR>(acc);
   if (lane == 0) {
     double r = alpha * acc;
@@ -379,6 +387,38 @@ __global__ __launch_bounds__(256) void k_axpy_dot(int n, const double* __restric
     if (h) { ri -= hk * v[i]; r[i] = ri; }
     s += ri * (w ? w[i] : ri);
   }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// the same step with the previous dot product still in per-block partial
+// sums: every workgroup first sums partial_in (in one fixed order, so all of
+// them get the same h) and workgroup 0 records h; this removes the separate
+// reduction launch between two Gram-Schmidt steps
+__global__ __launch_bounds__(256) void k_axpy_dot_reduced(int n, int m_in, const double* __restrict__ partial_in,
+                                                          double* __restrict__ h_out,
+                                                          const double* __restrict__ v,
+                                                          double* __restrict__ r,
+                                                          const double* __restrict__ w,
+                                                          double* __restrict__ partial) {
+  __shared__ double red[16];
+  __shared__ double hs;
+  double p = 0;
+  for (int i = threadIdx.x; i < m_in; i += blockDim.x) p += partial_in[i];
+  p = block_sum(p, red);
+  if (threadIdx.x == 0) {
+    hs = p;
+    if (blockIdx.x == 0) h_out[0] = p;
+  }
+  __syncthreads();
+  const double hk = hs;
+  double s = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double ri = r[i] - hk * v[i];
+    r[i] = ri;
+    s += ri * (w ? w[i] : ri);
+  }
+  __syncthreads();
   s = block_sum(s, red);
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
@@ -598,22 +638,17 @@ void csr_spmv_t(const DevCSR& P, const T* x, T* y, double alpha, double beta, in
   const int lpr = (P.lpr == 1 || P.lpr == 2 || P.lpr == 4) ? P.lpr : 8;
   const int grid = (P.n + (256 / lpr) - 1) / (256 / lpr);
   if (grid == 0) return;
+  auto go = [&](auto kern, const auto* va) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, va, x, y, alpha, beta, row_end,
+                       zero_rest);
+  };
+  // exact bf16 copy of the values (val_bf16 without row scales), else FP64
+  const bool h = P.val_bf16 && !P.rscale;
   switch (lpr) {
-    case 1:
-      hipLaunchKernelGGL((k_csr_spmv<1, T>), dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, P.val,
-                         x, y, alpha, beta, row_end, zero_rest);
-      break;
-    case 2:
-      hipLaunchKernelGGL((k_csr_spmv<2, T>), dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, P.val,
-                         x, y, alpha, beta, row_end, zero_rest);
-      break;
-    case 4:
-      hipLaunchKernelGGL((k_csr_spmv<4, T>), dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, P.val,
-                         x, y, alpha, beta, row_end, zero_rest);
-      break;
-    default:
-      hipLaunchKernelGGL((k_csr_spmv<8, T>), dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, P.val,
-                         x, y, alpha, beta, row_end, zero_rest);
+    case 1: h ? go(k_csr_spmv<1, T, uint16_t>, P.val_bf16) : go(k_csr_spmv<1, T, double>, P.val); break;
+    case 2: h ? go(k_csr_spmv<2, T, uint16_t>, P.val_bf16) : go(k_csr_spmv<2, T, double>, P.val); break;
+    case 4: h ? go(k_csr_spmv<4, T, uint16_t>, P.val_bf16) : go(k_csr_spmv<4, T, double>, P.val); break;
+    default: h ? go(k_csr_spmv<8, T, uint16_t>, P.val_bf16) : go(k_csr_spmv<8, T, double>, P.val);
   }
 }
 void csr_spmv(const DevCSR& P, const double* x, double* y, double alpha, double beta, int row_end,
@@ -642,6 +677,18 @@ void vec_axpy_dot(int n, const double* h, const double* v, double* r, const doub
   hipLaunchKernelGGL(k_axpy_dot, dim3(kRedBlocks), dim3(256), 0, st, n, h, v, r, w, partial);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, kRedBlocks, partial, out,
                      w ? 0 : 1);
+}
+int vec_partials() { return kRedBlocks; }
+void vec_dot_partial(int n, const double* x, const double* y, double* partial, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_partial<double>, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial);
+}
+void vec_axpy_dot_reduced(int n, const double* partial_in, double* h_out, const double* v, double* r,
+                          const double* w, double* partial_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_dot_reduced, dim3(kRedBlocks), dim3(256), 0, st, n, kRedBlocks, partial_in, h_out, v,
+                     r, w, partial_out);
+}
+void vec_reduce(const double* partial, double* out, int sqrt_of_sum, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, kRedBlocks, partial, out, sqrt_of_sum);
 }
 void vec_axpy(int n, double a, const double* a_dev, double sign, const double* x, double* y,
               hipStream_t st) {

@@ -92,6 +92,15 @@ void vec_norm_f32(int n, const float* x, double* partial, double* out, hipStream
 // r -= h[0]*v (skipped when h == nullptr); out = <r, w> (or ||r|| when w == nullptr)
 void vec_axpy_dot(int n, const double* h, const double* v, double* r, const double* w,
                   double* partial, double* out, hipStream_t st);
+// Gram-Schmidt chain without separate reductions: per-block partial sums
+// (vec_partials() of them) of <x, y>; r -= h v with h = sum(partial_in), written
+// to h_out, and the partial sums of <r, w> (or |r|^2 when w == nullptr);
+// out = sum (or sqrt of the sum) of partial sums
+int vec_partials();
+void vec_dot_partial(int n, const double* x, const double* y, double* partial, hipStream_t st);
+void vec_axpy_dot_reduced(int n, const double* partial_in, double* h_out, const double* v, double* r,
+                          const double* w, double* partial_out, hipStream_t st);
+void vec_reduce(const double* partial, double* out, int sqrt_of_sum, hipStream_t st);
 // y += (a_dev ? sign*a_dev[0] : a) * x
 void vec_axpy(int n, double a, const double* a_dev, double sign, const double* x, double* y,
               hipStream_t st);

@@ -67,6 +67,19 @@ pmx::DevCSR upload_csr(const pmx::CSRPattern& p, const std::vector<double>& v, h
   M.row_ptr = upload(p.row_ptr.data(), p.row_ptr.size(), st);
   M.col = upload(p.col.data(), p.col.size(), st);
   M.val = upload(v.data(), v.size(), st);
+  // grid-transfer weights of nested Lagrange spaces are short binary fractions:
+  // when every value is a bf16 number, the transfer kernels read a 2-byte copy
+  // (same products, same result)
+  std::vector<uint16_t> h(v.size());
+  bool exact = !v.empty();
+  for (size_t k = 0; k < v.size() && exact; ++k) {
+    const float f = (float)v[k];
+    uint32_t u;
+    std::memcpy(&u, &f, sizeof(u));
+    exact = (double)f == v[k] && (u & 0xFFFFu) == 0;
+    h[k] = (uint16_t)(u >> 16);
+  }
+  if (exact) M.val_bf16 = upload(h.data(), h.size(), st);
   return M;
 }
 
@@ -559,11 +572,18 @@ void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
       precondition(s, s->V[i], s->Z[i]);
       op_apply(s, s->Z[i], s->r);
       double* Hcol = s->d_H + (size_t)i * ld;
-      // modified Gram-Schmidt, pipelined: H(k,i) = <r, v_k>; r -= H(k,i) v_k
-      pmx::vec_dot(n, s->r, s->V[0], s->partial, Hcol, S(s));
-      for (int k = 0; k < i; ++k)
-        pmx::vec_axpy_dot(n, Hcol + k, s->V[k], s->r, s->V[k + 1], s->partial, Hcol + k + 1, S(s));
-      pmx::vec_axpy_dot(n, Hcol + i, s->V[i], s->r, nullptr, s->partial, Hcol + i + 1, S(s));
+      // modified Gram-Schmidt, pipelined: H(k,i) = <r, v_k>; r -= H(k,i) v_k;
+      // each step finishes the previous dot product itself (two partial-sum
+      // buffers alternate)
+      double* pa = s->partial;
+      double* pb = s->partial + pmx::vec_partials();
+      pmx::vec_dot_partial(n, s->r, s->V[0], pa, S(s));
+      for (int k = 0; k < i; ++k) {
+        pmx::vec_axpy_dot_reduced(n, pa, Hcol + k, s->V[k], s->r, s->V[k + 1], pb, S(s));
+        std::swap(pa, pb);
+      }
+      pmx::vec_axpy_dot_reduced(n, pa, Hcol + i, s->V[i], s->r, nullptr, pb, S(s));
+      pmx::vec_reduce(pb, Hcol + i + 1, 1, S(s));
       HIP_CHECK(hipMemcpyAsync(&H[(size_t)i * ld], Hcol, (i + 2) * sizeof(double), hipMemcpyDeviceToHost, S(s)));
       HIP_CHECK(hipStreamSynchronize(S(s)));
       const double hn = H[(i + 1) + (size_t)i * ld];
