@@ -148,6 +148,7 @@ int pmx_matrix_time_kernel(pmx_ctx* ctx, const pmx_matrix* A, int kernel, int re
 /* ---------------------------------------------------------------- assembly */
 #define PMX_DISC_GALERKIN 0
 #define PMX_DISC_SUPG 1
+#define PMX_DISC_GLS 2   /* BilinearAssemble_GLS (ConvDiff.C:2012-2131), stationary CD only */
 /* built-in examples (coefficients + exact solution + boundary values) */
 #define PMX_EX_CONST 0
 #define PMX_EX_SINE_LAPLACE_2D 1   /* include/Examples/CD_2D/SineLaplace.h */
@@ -161,7 +162,7 @@ int pmx_matrix_time_kernel(pmx_ctx* ctx, const pmx_matrix* A, int kernel, int re
 #define PMX_EX_TCD_LINEAR 8        /* include/Examples/TCD_2D/linear_space_time.h (example -1):
                                       u = 1 + 2x + 3ty */
 typedef struct {
-  int disc;            /* space_discretization_type: galerkin / supg */
+  int disc;            /* space_discretization_type: galerkin / supg / gls */
   int example;         /* PMX_EX_* : right-hand side from the exact solution */
   int hexa_quad_rule;  /* 0: degree table (Q2 hexa -> Gauss4), 2: Gauss3 (INTERNAL_QUAD_RULE=2) */
   double eps;          /* diffusion coefficient (coeff[0]) */

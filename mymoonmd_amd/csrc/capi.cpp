@@ -111,7 +111,7 @@ pmx::AssemblyData assembly_data(pmx_space* s, const RefTable& t) {
 pmx::CDCoeffs cd_coeffs(const pmx_cd_params* p) {
   pmx::CDCoeffs co;
   co.example = p->example;
-  co.supg = (p->disc == PMX_DISC_SUPG);
+  co.supg = (p->disc == PMX_DISC_SUPG) ? 1 : (p->disc == PMX_DISC_GLS) ? 2 : 0;
   co.eps = p->eps;
   for (int d = 0; d < 3; ++d) co.b[d] = p->b[d];
   co.c = p->c;
@@ -595,7 +595,8 @@ int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p, p
     require(ctx && s_ && p && A && dev_rhs, "NULL argument");
     require(A->space == s_, "matrix was not created on this space");
     require(s_->ctx == ctx, "space has no device data for this context");
-    require(p->disc == PMX_DISC_GALERKIN || p->disc == PMX_DISC_SUPG, "unknown discretisation");
+    require(p->disc == PMX_DISC_GALERKIN || p->disc == PMX_DISC_SUPG || p->disc == PMX_DISC_GLS,
+            "unknown discretisation");
     require(p->example >= 0 && p->example <= PMX_EX_TCD_LINEAR, "unknown example");
     pmx_space* s = const_cast<pmx_space*>(s_);
     hipStream_t st = ctx->stream;

@@ -137,8 +137,20 @@ __global__ __launch_bounds__(256) void k_assemble_cd(AssemblyData ad, CDCoeffs c
       const double tx = s_g[slot][0][i], ty = s_g[slot][1][i];
       const double tz = (DIM == 3) ? s_g[slot][DIM - 1][i] : 0.0;
       const double bgradv = b1 * tx + b2 * ty + b3 * tz;
+      // GLS tests with L phi_i = -eps lap phi_i + b.grad phi_i + c phi_i
+      double Lu = 0.0;
+      if (co.supg == 2) {
+        if (DIM == 3) {
+          Lu = -eps * s_l[slot][i];
+          Lu += b1 * tx + b2 * ty + b3 * tz + c * ti;
+        } else {
+          Lu = -eps * s_l[slot][i] + b1 * tx + b2 * ty + c * ti;
+        }
+      }
       frow += mult * ti * f;
-      if (co.supg) frow += mult * (delta * bgradv) * f;
+      if (co.supg == 1) frow += mult * (delta * bgradv) * f;
+      else if (co.supg == 2) frow += mult * delta * Lu * f;
+      const double gls_w = mult * delta * Lu;   // 3D factor of BilinearAssemble_GLS<3>
 #pragma unroll
       for (int j = 0; j < ND; ++j) {
         const double ax = s_g[slot][0][j], ay = s_g[slot][1][j];
@@ -149,10 +161,19 @@ __global__ __launch_bounds__(256) void k_assemble_cd(AssemblyData ad, CDCoeffs c
         val += bgu * ti;
         val += c * aj * ti;
         arow[j] += val * mult;
-        if (co.supg) {
+        if (co.supg == 1) {
           double v2 = -eps * s_l[slot][j];
           v2 += bgu + c * aj;
           arow[j] += v2 * delta * bgradv * mult;
+        } else if (co.supg == 2) {
+          if (DIM == 3) {
+            double v2 = -eps * s_l[slot][j];
+            v2 += bgu + c * aj;
+            arow[j] += v2 * gls_w;
+          } else {
+            const double v2 = delta * (-eps * s_l[slot][j] + b1 * ax + b2 * ay + c * aj) * Lu;
+            arow[j] += v2 * mult;
+          }
         }
       }
     }

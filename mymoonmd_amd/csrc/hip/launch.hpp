@@ -147,7 +147,7 @@ constexpr int kGeoStride = 24;
 
 struct CDCoeffs {
   int example = 0;        // built-in right-hand side / exact solution
-  int supg = 0;           // add BilinearAssemble_SD (CD) / the TCD SUPG forms
+  int supg = 0;           // 1: add BilinearAssemble_SD (CD) / the TCD SUPG forms; 2: BilinearAssemble_GLS
   double eps = 1, b[3] = {0, 0, 0}, c = 0, f0 = 0;
   double delta0 = 1, poly_degree = 1;
   double time = 0;        // TimeDB->CURRENTTIME of time-dependent examples

@@ -231,7 +231,7 @@ class FEMatrix:
 def cd_params(example="const", disc="galerkin", eps=1.0, b=(0.0, 0.0, 0.0), c=0.0, f0=0.0,
               hexa_quad_rule=0, delta0=1.0, poly_degree=1.0, time=0.0, theta1=1.0, tau=0.0):
     p = CDParams()
-    p.disc = 1 if disc == "supg" else 0
+    p.disc = {"galerkin": 0, "supg": 1, "gls": 2}[disc]
     p.example = EXAMPLES[example]
     p.hexa_quad_rule = hexa_quad_rule
     p.eps = eps

@@ -459,6 +459,32 @@ void or_assemble_cd(const or_space* s, const or_cd* p, double* val, double* rhs)
           }
         }
       }
+      /* BilinearAssemble_GLS<d> (ConvDiff.C:2012-2131): test functions
+         L phi_i = -eps lap phi_i + b.grad phi_i + c phi_i, same delta */
+      if (p->disc == OR_GLS) {
+        const double bb[3] = {b1, b2, b3};
+        const double delta = sdfem_delta(hK, eps, bb, dim, p);
+        for (int i = 0; i < nd; ++i) {
+          double Lu;
+          if (dim == 3) {
+            Lu = -eps * lap[i];
+            Lu += b1 * ux[i] + b2 * ux[nd + i] + b3 * ux[2 * nd + i] + c * u[i];
+          } else {
+            Lu = -eps * lap[i] + b1 * ux[i] + b2 * ux[nd + i] + c * u[i];
+          }
+          F[i] += Mult * delta * Lu * f;
+          for (int j = 0; j < nd; ++j) {
+            if (dim == 3) {
+              double vv = -eps * lap[j];
+              vv += b1 * ux[j] + b2 * ux[nd + j] + b3 * ux[2 * nd + j] + c * u[j];
+              A[i * nd + j] += vv * (Mult * delta * Lu);
+            } else {
+              const double vv = delta * (-eps * lap[j] + b1 * ux[j] + b2 * ux[nd + j] + c * u[j]) * Lu;
+              A[i * nd + j] += vv * Mult;
+            }
+          }
+        }
+      }
     }
     /* scatter with a linear search of the row */
     for (int m = 0; m < nd; ++m) {

@@ -7,6 +7,7 @@
 #define OR_HEXA 2
 #define OR_GALERKIN 0
 #define OR_SUPG 1
+#define OR_GLS 2
 #define OR_DIRICHLET 0
 #define OR_NEUMANN 1
 #define OR_EX_CONST 0

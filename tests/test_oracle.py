@@ -272,3 +272,49 @@ def test_nse_picard_oracle_converges_to_interpolant():
     assert hist[-1] <= 1e-10
     assert all(b < a for a, b in zip(hist, hist[1:]))
     assert np.max(np.abs(x - xi)) < 1e-8
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_gls_keeps_the_zero_error_examples(golden, idx):
+    """GLS is consistent (its terms vanish for the exact solution), so the
+    cd3d_solvers_test examples with polynomial solutions in the discrete
+    space stay exact; a sign error in the GLS right-hand side (planted on the
+    quadratic example) is visible."""
+    name, order, eps, b, c = golden["cd3d_solvers_test"]["examples"][idx]
+    tol = golden["cd3d_solvers_test"]["tolerance_direct"]
+    sp, info = O.build_space(2, 2, 2, order)
+    p = O.cd_params(example=EX[name], disc=2, eps=eps, b=b, c=c)
+    val, rhs = O.assemble(sp, p)
+    x = solve_direct_like(sp, val, rhs, tol=1e-15)
+    err = O.errors(sp, p, x)
+    assert err[0] < 10 * tol and err[1] < 1e3 * tol, err
+    if idx == 2:
+        pg = O.cd_params(example=EX[name], disc=0, eps=eps, b=b, c=c)
+        vg, rg = O.assemble(sp, pg)
+        bad = rhs - 2.0 * (rhs - rg)          # GLS rhs term with the wrong sign
+        bad[sp.active:] = rhs[sp.active:]
+        xb = solve_direct_like(sp, val, bad, tol=1e-15)
+        assert O.errors(sp, p, xb)[0] > 1e-6
+
+
+def test_gls_term_is_symmetric_and_vanishes_for_q1_diffusion():
+    """The GLS term delta (L phi_j, L phi_i) is symmetric (unlike the SUPG term
+    delta (L phi_j, b.grad phi_i)), and on Q1 bricks without reaction
+    (lap phi = 0, c = 0, so L phi = b.grad phi) GLS equals SUPG."""
+    sp, info = O.build_space(2, 2, 2, 2)
+    kw = dict(example=EX["exp_sin_3d"], eps=1e-2, b=(1.0, 2.0, 3.0), c=0.5)
+    vg, _ = O.assemble(sp, O.cd_params(disc=0, **kw))
+    vl, _ = O.assemble(sp, O.cd_params(disc=2, **kw))
+    vs, _ = O.assemble(sp, O.cd_params(disc=1, **kw))
+    D = to_dense(sp, vl - vg)[:sp.active, :sp.active]
+    S = to_dense(sp, vs - vg)[:sp.active, :sp.active]
+    assert np.max(np.abs(D - D.T)) < 1e-12 * np.max(np.abs(D))
+    assert np.max(np.abs(S - S.T)) > 1e-3 * np.max(np.abs(S))
+    sp1, _ = O.build_space(2, 2, 2, 1)
+    k1 = dict(example=EX["exp_sin_3d"], eps=1e-2, b=(1.0, 2.0, 3.0), c=0.0)
+    a, fa = O.assemble(sp1, O.cd_params(disc=1, **k1))
+    g, fg = O.assemble(sp1, O.cd_params(disc=2, **k1))
+    ga, _ = O.assemble(sp1, O.cd_params(disc=0, **k1))
+    assert np.max(np.abs(a - ga)) > 1e-6
+    assert np.max(np.abs(a - g)) < 1e-14 * np.max(np.abs(a))
+    assert np.max(np.abs(fa - fg)) < 1e-14 * np.max(np.abs(fa))
