@@ -414,6 +414,8 @@ int pmx_batch_errors(pmx_batch* b, int problem, double* host_errors4);
 #define PMX_SMOOTHER_JACOBI 0
 #define PMX_SMOOTHER_DIRECT 1
 #define PMX_SMOOTHER_CELL_VANKA_JACOBI 2   /* VankaSmoother CELL_JACOBI (saddle-point systems) */
+#define PMX_SMOOTHER_CELL_VANKA 3          /* VankaSmoother CELL (multiplicative), cells in colour-major
+                                              order: colour = lattice index mod 3 per direction */
 #define PMX_CYCLE_V 0
 #define PMX_CYCLE_W 1
 #define PMX_CYCLE_F 2

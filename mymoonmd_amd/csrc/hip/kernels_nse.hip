@@ -337,30 +337,20 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// one wavefront per cell: the local rhs lives in registers (rows lane and
-// lane + 64), gathered already permuted; forward and backward substitution
-// broadcast the solved component (a lane read, no LDS round trip).  The
-// factor columns of the next U steps are loaded before those steps run: their
-// addresses do not depend on the solution, so the loads overlap instead of
-// exposing one latency per step.  Lanes without a second row (lane + 64 >=
-// 89) load a valid duplicate and ignore it, so no load is predicated.
+// forward and backward substitution with the LU factors L of one cell
+// (column-major, unit lower part) for the local rhs held in registers (rows
+// lane and lane + 64 of the row-interchanged system); on return b0/b1 hold the
+// local solution.  The solved component is broadcast by a lane read; the factor
+// columns of the next U steps are loaded before those steps run (two register
+// buffers), since their addresses do not depend on the solution.  Lanes
+// without a second row (lane + 64 >= 89) load a valid duplicate and ignore it,
+// so no load is predicated.
 template <typename T>
-__global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const T* __restrict__ lu,
-                                                     const int* __restrict__ pmap,
-                                                     const double* __restrict__ r, double damp,
-                                                     double* __restrict__ upd) {
+__device__ __forceinline__ void vanka_substitute(const T* __restrict__ L, int lane, double& b0, double& b1) {
   constexpr int U = 8;
-  const int lane = threadIdx.x & 63;
-  const int cell = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
-  if (cell >= n_cells) return;   // uniform for the wavefront
-  const T* L = lu + (size_t)cell * NL * NL;
-  const int* pm = pmap + (size_t)cell * NL;
   const int i0 = lane, i1 = lane + 64;
   const bool h1 = i1 < NL;
-  const int j1 = h1 ? i1 : i0;   // in-range row for the second load
-  double b0 = r[pm[i0]];
-  double b1 = r[pm[j1]];
-  if (!h1) b1 = 0.0;
+  const int j1 = h1 ? i1 : i0;
   // column blocks of U steps; two register buffers so that the loads of the
   // next block are in flight while the current block is computed
   static_assert((NL - 1) % U == 0, "column blocks of the forward sweep");
@@ -444,8 +434,63 @@ __global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const T* __res
       if (k0 - U >= 0) upper(k0 - U, c0, c1, cd);
     }
   }
+}
+
+// one wavefront per cell: gather the permuted local defect, substitute, store
+// the damped local update (Jacobi variant: summed per DOF afterwards)
+template <typename T>
+__global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const T* __restrict__ lu,
+                                                     const int* __restrict__ pmap,
+                                                     const double* __restrict__ r, double damp,
+                                                     double* __restrict__ upd) {
+  const int lane = threadIdx.x & 63;
+  const int cell = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  if (cell >= n_cells) return;   // uniform for the wavefront
+  const int* pm = pmap + (size_t)cell * NL;
+  const int i0 = lane, i1 = lane + 64;
+  const bool h1 = i1 < NL;
+  const int j1 = h1 ? i1 : i0;
+  double b0 = r[pm[i0]];
+  double b1 = r[pm[j1]];
+  if (!h1) b1 = 0.0;
+  vanka_substitute<T>(lu + (size_t)cell * NL * NL, lane, b0, b1);
   upd[(size_t)cell * NL + i0] = damp * b0;
   if (h1) upd[(size_t)cell * NL + i1] = damp * b1;
+}
+
+// VankaSmoother CELL (multiplicative) over the cells of one colour: cells of a
+// colour share no DOF, so each wave computes its local defect rows
+// f_i - sum_k K_ik x_k with the current x (row entries in CSR order, as the
+// reference's loop), substitutes and adds the damped local solution to x.
+// Colours run one after another, each seeing the updates of the previous ones.
+template <typename T>
+__global__ __launch_bounds__(256) void k_vanka_colour(int n, const int* __restrict__ cells,
+                                                      const T* __restrict__ lu, const int* __restrict__ pmap,
+                                                      const int* __restrict__ map, const int* __restrict__ rp,
+                                                      const int* __restrict__ ci, const double* __restrict__ cv,
+                                                      const double* __restrict__ f, double damp,
+                                                      double* __restrict__ x) {
+  const int lane = threadIdx.x & 63;
+  const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  if (w >= n) return;   // uniform for the wavefront
+  const int cell = cells[w];
+  const int* pm = pmap + (size_t)cell * NL;
+  const int i0 = lane, i1 = lane + 64;
+  const bool h1 = i1 < NL;
+  const int j1 = h1 ? i1 : i0;
+  auto defect = [&](int g) {
+    double t = f[g];
+    const int e = rp[g + 1];
+    for (int k = rp[g]; k < e; ++k) t -= cv[k] * x[ci[k]];
+    return t;
+  };
+  double b0 = defect(pm[i0]);
+  double b1 = defect(pm[j1]);
+  if (!h1) b1 = 0.0;
+  vanka_substitute<T>(lu + (size_t)cell * NL * NL, lane, b0, b1);
+  const int* dm = map + (size_t)cell * NL;
+  x[dm[i0]] += damp * b0;
+  if (h1) x[dm[i1]] += damp * b1;
 }
 
 // x[i] += (sum of the cell updates of DOF i, in cell order) / their number
@@ -484,6 +529,18 @@ void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, fl
   if (n_cells > 0)
     hipLaunchKernelGGL(k_vanka_factor, dim3(n_cells), dim3(256), 0, st, n_cells, map, C.row_ptr, C.col, C.val,
                        lu64, lu32, pmap, singular);
+}
+
+void vanka_colour(int n, const int* cells, const double* lu64, const float* lu32, const int* pmap, const int* map,
+                  const DevCSR& C, const double* f, double damp, double* x, hipStream_t st) {
+  if (n <= 0) return;
+  const int grid = (n + 3) / 4;
+  if (lu32)
+    hipLaunchKernelGGL((k_vanka_colour<float>), dim3(grid), dim3(256), 0, st, n, cells, lu32, pmap, map, C.row_ptr,
+                       C.col, C.val, f, damp, x);
+  else
+    hipLaunchKernelGGL((k_vanka_colour<double>), dim3(grid), dim3(256), 0, st, n, cells, lu64, pmap, map, C.row_ptr,
+                       C.col, C.val, f, damp, x);
 }
 
 void vanka_solve(int n_cells, const double* lu64, const float* lu32, const int* pmap, const double* r,

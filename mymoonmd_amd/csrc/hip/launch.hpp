@@ -195,6 +195,11 @@ void vanka_solve(int n_cells, const double* lu64, const float* lu32, const int* 
                  double damp, double* upd, hipStream_t st);
 void vanka_accumulate(int n, const int* acc_ptr, const int* acc_idx, const double* upd, double* x,
                       hipStream_t st);
+// multiplicative variant over the n cells `cells` of one colour (no shared
+// DOFs): local defect f - C x on the cell's rows, substitution, x += damp * local
+// solution; map[cell][i] = global DOF of local DOF i
+void vanka_colour(int n, const int* cells, const double* lu64, const float* lu32, const int* pmap, const int* map,
+                  const DevCSR& C, const double* f, double damp, double* x, hipStream_t st);
 
 // ---- batched time-dependent solve (kernels_batch.hip)
 struct BatchDev {

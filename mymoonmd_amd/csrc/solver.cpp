@@ -96,11 +96,15 @@ struct NseLevel {
   double* dinv_dense = nullptr;          // coarsest level: inverse of the combined matrix
   double* partial = nullptr;
   bool sol_zero = false;                 // sol is zero (set by the cycle's resets)
+  bool multiplicative = false;           // cell_vanka: colours one after another
+  int* colour_cells = nullptr;           // cells in colour-major order (27 colours)
+  std::vector<int> colour_ptr;
   void release() {
     dfree(map); dfree(pmap); dfree(acc_ptr); dfree(acc_idx); dfree(lu64); dfree(lu32); dfree(upd);
     dfree(rhs); dfree(sol); dfree(def); dfree(res);
     free_csr(Pu); free_csr(Ru); free_csr(Pp); free_csr(Rp);
-    dfree(dinv_dense); dfree(partial);
+    dfree(dinv_dense); dfree(partial); dfree(colour_cells);
+    colour_ptr.clear();
   }
 };
 
@@ -338,6 +342,15 @@ void nse_smooth(pmx_solver* s, int l) {
   if (l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) {
     L.sol_zero = false;
     pmx::dense_mv(L.n, L.dinv_dense, L.rhs, L.sol, S(s));
+    return;
+  }
+  if (L.multiplicative) {
+    // VankaSmoother CELL: each colour's cells see the updates of the earlier ones
+    const pmx::DevCSR C = nse_combined(L.sys);
+    for (size_t c = 0; c + 1 < L.colour_ptr.size(); ++c)
+      pmx::vanka_colour(L.colour_ptr[c + 1] - L.colour_ptr[c], L.colour_cells + L.colour_ptr[c], L.lu64, L.lu32,
+                        L.pmap, L.map, C, L.rhs, s->p.vanka_damp, L.sol, S(s));
+    L.sol_zero = false;
     return;
   }
   // local defects come from r = rhs - K sol; for sol = 0 that is rhs itself
@@ -728,10 +741,12 @@ int pmx_solver_create(pmx_ctx* ctx, const pmx_solver_params* p, pmx_solver** out
     require(p->iterative_solver_type >= 0 && p->iterative_solver_type <= 3, "unknown iterative solver");
     require(p->preconditioner >= 0 && p->preconditioner <= 2, "unknown preconditioner");
     require(p->max_n_iterations >= 0 && p->gmres_restart >= 1, "invalid iteration counts");
-    require(p->mg_smoother == PMX_SMOOTHER_JACOBI || p->mg_smoother == PMX_SMOOTHER_CELL_VANKA_JACOBI,
+    require(p->mg_smoother == PMX_SMOOTHER_JACOBI || p->mg_smoother == PMX_SMOOTHER_CELL_VANKA_JACOBI ||
+                p->mg_smoother == PMX_SMOOTHER_CELL_VANKA,
             "unknown multigrid smoother");
     require(p->mg_smoother_coarse == PMX_SMOOTHER_JACOBI || p->mg_smoother_coarse == PMX_SMOOTHER_DIRECT ||
-                p->mg_smoother_coarse == PMX_SMOOTHER_CELL_VANKA_JACOBI,
+                p->mg_smoother_coarse == PMX_SMOOTHER_CELL_VANKA_JACOBI ||
+                p->mg_smoother_coarse == PMX_SMOOTHER_CELL_VANKA,
             "unknown coarse smoother");
     require(p->precision == PMX_PRECISION_FP64 || p->precision == PMX_PRECISION_MIXED, "unknown precision");
     auto s = std::make_unique<pmx_solver>();
@@ -758,7 +773,8 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
     require(!mg || (spaces && n_levels >= 2), "multigrid needs >= 2 levels and their spaces");
     require(mg || n_levels == 1, "without multigrid pass exactly one matrix");
     require(!mg || (s->p.mg_smoother == PMX_SMOOTHER_JACOBI &&
-                    s->p.mg_smoother_coarse != PMX_SMOOTHER_CELL_VANKA_JACOBI),
+                    s->p.mg_smoother_coarse != PMX_SMOOTHER_CELL_VANKA_JACOBI &&
+                    s->p.mg_smoother_coarse != PMX_SMOOTHER_CELL_VANKA),
             "scalar problems use the jacobi smoother (cell Vanka is for saddle-point systems)");
     s->release();
     hipStream_t st = S(s);
@@ -915,10 +931,10 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
     require(s->p.preconditioner != PMX_PC_JACOBI, "use multigrid (cell Vanka) or no preconditioner");
     require(mg || n_levels == 1, "without multigrid pass exactly one system");
     require(!mg || n_levels >= 2, "multigrid needs >= 2 levels");
-    require(!mg || s->p.mg_smoother == PMX_SMOOTHER_CELL_VANKA_JACOBI,
-            "saddle-point multigrid uses the cell_vanka_jacobi smoother");
+    require(!mg || s->p.mg_smoother == PMX_SMOOTHER_CELL_VANKA_JACOBI || s->p.mg_smoother == PMX_SMOOTHER_CELL_VANKA,
+            "saddle-point multigrid uses the cell_vanka_jacobi or cell_vanka smoother");
     require(!mg || s->p.mg_smoother_coarse != PMX_SMOOTHER_JACOBI,
-            "coarse smoother must be direct_solve or cell_vanka_jacobi");
+            "coarse smoother must be direct_solve, cell_vanka_jacobi or cell_vanka");
     s->release();
     hipStream_t st = S(s);
     for (int l = 0; l < n_levels; ++l) {
@@ -987,6 +1003,43 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
           L.pmap = dmalloc<int>((size_t)L.n_cells * NLc);
           L.upd = dmalloc<double>((size_t)L.n_cells * NLc);
           pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.pmap, sing, st);
+          const int kind = (l == 0) ? s->p.mg_smoother_coarse : s->p.mg_smoother;
+          if (kind == PMX_SMOOTHER_CELL_VANKA) {
+            // colour = cell lattice index mod 3 per direction: cells of one
+            // colour are >= 3 cells apart, so none reads a DOF another updates
+            const pmx::Mesh& m = *sys->vs.mesh;
+            const pmx::MeshLevel& ml = m.levels[sys->vs.level];
+            const int nv = m.nv, dim = m.dim;
+            auto centre = [&](int c, int d) {
+              double t = 0;
+              for (int v = 0; v < nv; ++v) t += ml.xyz[((size_t)c * nv + v) * dim + d];
+              return t / nv;
+            };
+            double h[3];
+            for (int d = 0; d < dim; ++d) {
+              double lo = 1e300, hi = -1e300;
+              for (int v = 0; v < nv; ++v) {
+                const double x = ml.xyz[(size_t)v * dim + d];
+                lo = std::min(lo, x);
+                hi = std::max(hi, x);
+              }
+              h[d] = hi - lo;
+            }
+            std::vector<int> colour(L.n_cells);
+            for (int c = 0; c < L.n_cells; ++c) {
+              int col = 0, mul = 1;
+              for (int d = 0; d < dim; ++d, mul *= 3)
+                col += mul * ((int)std::floor(centre(c, d) / h[d]) % 3);
+              colour[c] = col;
+            }
+            L.colour_ptr.assign(28, 0);
+            for (int c = 0; c < L.n_cells; ++c) L.colour_ptr[colour[c] + 1]++;
+            for (int k = 0; k < 27; ++k) L.colour_ptr[k + 1] += L.colour_ptr[k];
+            std::vector<int> fillc(L.colour_ptr.begin(), L.colour_ptr.end() - 1), cells(L.n_cells);
+            for (int c = 0; c < L.n_cells; ++c) cells[fillc[colour[c]]++] = c;
+            L.colour_cells = upload(cells.data(), cells.size(), st);
+            L.multiplicative = true;
+          }
         } else {
           L.dinv_dense = dense_inverse_of(s, L.n, sys->Cp.row_ptr, sys->Cp.col, sys->d_Cval);
         }

@@ -23,7 +23,7 @@ EXAMPLES = {"const": 0, "sine_laplace_2d": 1, "p0_3d": 2, "p1_3d": 3, "p2_3d": 4
 TIME_SCHEME = {"backward_euler": 0, "crank_nicolson": 1, "bdf_two": 2}
 IT_SOLVER = {"fgmres": 0, "cg": 1, "richardson": 2, "jacobi": 3}
 PRECOND = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2}
-SMOOTHER = {"jacobi": 0, "direct_solve": 1, "cell_vanka_jacobi": 2}
+SMOOTHER = {"jacobi": 0, "direct_solve": 1, "cell_vanka_jacobi": 2, "cell_vanka": 3}
 CYCLE = {"V": 0, "W": 1, "F": 2}
 BC_BUILTIN = {"all_dirichlet": 0, "sine_laplace": 1}
 

@@ -808,6 +808,26 @@ class NseLevelOracle:
         for m in maps:
             self.count[m] += 1
         self.local = [self.K[m, :][:, m].toarray() for m in maps]
+        # colour-major cell order of the multiplicative variant: colour =
+        # cell lattice index mod 3 in each direction (27 colours), so that the
+        # cells of one colour neither share DOFs nor read each other's DOFs
+        ctr = ns.vs.verts.mean(1)
+        h = ns.vs.verts[0].max(0) - ns.vs.verts[0].min(0)
+        ijk = np.floor(ctr / h).astype(int)
+        colour = (ijk[:, 0] % 3) + 3 * (ijk[:, 1] % 3) + 9 * (ijk[:, 2] % 3)
+        self.colour_order = [int(c) for c in np.lexsort((np.arange(len(colour)), colour))]
+
+    def vanka_mult(self, rhs, x, damp):
+        """One CELL sweep (multiplicative, VankaSmoother.C:350-420 with another
+        cell order): per cell, in colour-major order, the local rows of the
+        current defect, the local solve, and x += damp * local solution."""
+        x = x.copy()
+        K = self.K
+        for c in self.colour_order:
+            m = self.maps[c]
+            r = rhs[m] - K[m, :] @ x
+            x[m] += damp * np.linalg.solve(self.local[c], r)
+        return x
 
     def vanka(self, rhs, x, damp):
         """One CELL_JACOBI sweep: residual once, local solves (LAPACK LU with
@@ -826,7 +846,7 @@ class NseLevelOracle:
 
 
 def nse_mg_apply(levels, v, damp_corr=1.0, vanka_damp=1.0, n_pre=1, n_post=1, coarse="direct_solve",
-                 coarse_maxit=10, coarse_res=0.1, cycle="V"):
+                 coarse_maxit=10, coarse_res=0.1, cycle="V", smoother="cell_vanka_jacobi"):
     """Multigrid::cycle (src/Multigrid/Multigrid.C:157-420) for NSE levels
     (coarsest first): V/W cycle of CycleControl, velocity (Q2) and pressure (Q1)
     transfers of GridTransfer block by block, restricted defect with the
@@ -845,8 +865,11 @@ def nse_mg_apply(levels, v, damp_corr=1.0, vanka_damp=1.0, n_pre=1, n_post=1, co
 
     def smooth(l):
         L = levels[l]
-        if l == 0 and coarse == "direct_solve":
+        kind = coarse if l == 0 else smoother
+        if kind == "direct_solve":
             sol[0] = np.linalg.solve(L.K.toarray(), rhs[0])
+        elif kind == "cell_vanka":
+            sol[l] = L.vanka_mult(rhs[l], sol[l], vanka_damp)
         else:
             sol[l] = L.vanka(rhs[l], sol[l], vanka_damp)
 

@@ -318,3 +318,33 @@ def test_gls_term_is_symmetric_and_vanishes_for_q1_diffusion():
     assert np.max(np.abs(a - ga)) > 1e-6
     assert np.max(np.abs(a - g)) < 1e-14 * np.max(np.abs(a))
     assert np.max(np.abs(fa - fg)) < 1e-14 * np.max(np.abs(fa))
+
+
+def test_multiplicative_vanka_colours_are_independent():
+    """The colour-major order of the multiplicative cell Vanka: within a colour
+    (cell lattice index mod 3 per direction) no cell reads a DOF another one
+    updates, so reversing the order inside every colour changes nothing (the
+    GPU runs a colour's cells in parallel); a parity (mod 2) colouring does not
+    have this property."""
+    o = O.NseOracle(3, 3)
+    xo = O.nse_interpolate(o)
+    A, B, ro = o.assemble(1e-2, xo[:3 * o.n_u])
+    L = O.NseLevelOracle(o, A, B)
+    rng = np.random.default_rng(4)
+    rhs, x = rng.standard_normal(o.n), rng.standard_normal(o.n)
+    a = L.vanka_mult(rhs, x, 0.7)
+    ctr = o.vs.verts.mean(1)
+    h = o.vs.verts[0].max(0) - o.vs.verts[0].min(0)
+    ijk = np.floor(ctr / h).astype(int)
+    col3 = (ijk % 3) @ np.array([1, 3, 9])
+    col2 = (ijk % 2) @ np.array([1, 2, 4])
+    base = L.colour_order
+    L.colour_order = [c for k in range(27) for c in sorted(np.where(col3 == k)[0], reverse=True)]
+    b = L.vanka_mult(rhs, x, 0.7)
+    assert np.max(np.abs(a - b)) < 1e-14 * np.max(np.abs(a))
+    L.colour_order = [c for k in range(8) for c in sorted(np.where(col2 == k)[0])]
+    p1 = L.vanka_mult(rhs, x, 0.7)
+    L.colour_order = [c for k in range(8) for c in sorted(np.where(col2 == k)[0], reverse=True)]
+    p2 = L.vanka_mult(rhs, x, 0.7)
+    assert np.max(np.abs(p1 - p2)) > 1e-8 * np.max(np.abs(p1))
+    L.colour_order = base
