@@ -462,10 +462,157 @@ __global__ __launch_bounds__(1024) void k_reduce_errors(int nb, const double* __
 
 }  // namespace
 
+// Q2 hexahedra on the matrix cores: with per quadrature point q the staged
+// physical basis data g = grad phi, the Galerkin/SUPG/GLS element matrix is
+//   A_ij = sum_q m_q [ eps g_i.g_j + (phi_i + delta t_i)(b.g_j + c phi_j)
+//                      + delta t_i (-eps lap phi_j) ]
+// (t = b.grad phi for SUPG, L phi = -eps lap phi + b.grad phi + c phi for
+// GLS, delta = 0 for Galerkin): a 27 x K times K x 27 product with K = 5 per
+// quadrature point (three gradient channels, the convection-reaction channel,
+// the streamline-diffusion channel).  One wavefront per cell accumulates it as
+// 2 x 2 tiles of v_mfma_f64_16x16x4f64 (rows/columns 27..31 zero), FP64
+// throughout; the load vector is f_i = sum_q m_q f (phi_i + delta t_i).
+// Quadrature points are staged 4 at a time (5 MFMA k-steps per chunk).
+// Lane layout of the f64 16x16x4 MFMA: A operand lane l = A[l % 16][l / 16],
+// B operand lane l = B[l / 16][l % 16], result element t of lane l =
+// D[4 t + l / 16][l % 16].
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_assemble_cd_q2_mfma(AssemblyData ad, CDCoeffs co,
+                                                             const int* __restrict__ cells, int n, DevCSR A,
+                                                             double* __restrict__ rhs) {
+  constexpr int ND = 27, DIM = 3, NC = 6, QC = 4, NV = 7;
+  constexpr int RS = 1 + DIM + ND * (1 + DIM + NC);
+  __shared__ double s_p[4][QC][ND][NV];   // per wave: g1 g2 g3, phi + delta t, delta t, b.g + c phi, -eps lap
+  __shared__ double s_c[4][64][8];        // per wave: eps b1 b2 b3 c f delta mult of every quadrature point
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ci = blockIdx.x * 4 + wv;
+  const bool valid = ci < n;
+  const int cell = valid ? cells[ci] : cells[0];   // a real cell for the reads; no writes when !valid
+  const double* geo = ad.geo + (size_t)cell * kGeoStride;
+  double Jinv[9], G[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Jinv[k] = geo[12 + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      double t = 0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) t += Jinv[k * 3 + a] * Jinv[l * 3 + a];
+      G[k * 3 + l] = t;
+    }
+  if (lane < ad.nq) {
+    const double* rq = ad.ref + (size_t)lane * RS;
+    coeffs_at<3>(co, geo, rq + 1, rq[0], cell, lane, ad.nq, s_c[wv][lane]);
+  }
+  __syncthreads();
+  v4d_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4d_t{0.0, 0.0, 0.0, 0.0};
+  double frow = 0.0;
+  for (int q0 = 0; q0 < ad.nq; q0 += QC) {
+    for (int e = lane; e < QC * ND; e += 64) {
+      const int qq = e / ND, i = e % ND, q = q0 + qq;
+      double* out = s_p[wv][qq][i];
+      if (q < ad.nq) {
+        const double* rq = ad.ref + (size_t)q * RS;
+        const double* dphi = rq + 1 + DIM + ND;
+        const double* d2 = rq + 1 + DIM + ND * (1 + DIM);
+        const double phi = rq[1 + DIM + i];
+        double g[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          double t = 0;
+#pragma unroll
+          for (int k = 0; k < 3; ++k) t += dphi[k * ND + i] * Jinv[k * 3 + a];
+          g[a] = t;
+        }
+        const double lap = d2[0 * ND + i] * G[0] + d2[3 * ND + i] * G[4] + d2[5 * ND + i] * G[8] +
+                           2.0 * (d2[1 * ND + i] * G[1] + d2[2 * ND + i] * G[2] + d2[4 * ND + i] * G[5]);
+        const double* c = s_c[wv][q];
+        const double eps = c[0], cc = c[4], delta = c[6];
+        const double bg = c[1] * g[0] + c[2] * g[1] + c[3] * g[2];
+        double t = 0.0;
+        if (co.supg == 1) t = bg;
+        else if (co.supg == 2) { t = -eps * lap; t += bg + cc * phi; }
+        out[0] = g[0]; out[1] = g[1]; out[2] = g[2];
+        out[3] = phi + delta * t;
+        out[4] = delta * t;
+        out[5] = bg + cc * phi;
+        out[6] = -eps * lap;
+      } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) out[v] = 0.0;
+      }
+    }
+    __syncthreads();
+    if (lane < ND)
+      for (int qq = 0; qq < QC && q0 + qq < ad.nq; ++qq) {
+        const double* c = s_c[wv][q0 + qq];
+        frow += c[7] * c[5] * s_p[wv][qq][lane][3];
+      }
+    const int r16 = lane & 15;
+#pragma unroll
+    for (int st = 0; st < 5 * QC / 4; ++st) {
+      const int k = 4 * st + (lane >> 4);
+      const int qq = k / 5, ch = k % 5, q = q0 + qq;
+      double m = 0.0, me = 0.0;
+      if (q < ad.nq) {
+        m = s_c[wv][q][7];
+        me = m * s_c[wv][q][0];
+      }
+      const int ta = (ch < 3) ? ch : ch;       // test channel value: g_ch, phi + delta t, delta t
+      const int tb = (ch < 3) ? ch : ch + 2;   // trial channel value: g_ch, b.g + c phi, -eps lap
+      const double sa = (ch < 3) ? me : m;
+      double av[2], bv[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int i = 16 * h + r16;
+        av[h] = (i < ND) ? sa * s_p[wv][qq][i][ta] : 0.0;
+        bv[h] = (i < ND) ? s_p[wv][qq][i][tb] : 0.0;
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  if (!valid) return;
+  const int* dofs = ad.dofs + (size_t)cell * ND;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int i = 16 * a + 4 * t + (lane >> 4), j = 16 * b + (lane & 15);
+        if (i >= ND || j >= ND) continue;
+        const int r = dofs[i];
+        const int rs = A.row_ptr[r], re = A.row_ptr[r + 1];
+        if (r < A.active) A.val[find_col(A.col, rs, re, dofs[j])] += acc[a][b][t];
+        else if (i == j) A.val[find_col(A.col, rs, re, r)] = 1.0;
+      }
+  if (lane < ND) {
+    const int r = dofs[lane];
+    if (r < A.active) rhs[r] += frow;
+  }
+}
+
 void assemble_cd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n,
                  const DevCSR& A, double* rhs, hipStream_t st) {
-  if (ad.dim == 3 && ad.ndof == 27) launch_asm<27, 3, 32>(ad, co, cells, n, A, rhs, st);
-  else if (ad.dim == 3 && ad.ndof == 8) launch_asm<8, 3, 8>(ad, co, cells, n, A, rhs, st);
+  if (ad.dim == 3 && ad.ndof == 27) {
+    if (ad.nq <= 64) {
+      if (n > 0)
+        hipLaunchKernelGGL(k_assemble_cd_q2_mfma, dim3((n + 3) / 4), dim3(256), 0, st, ad, co, cells, n, A, rhs);
+    } else {
+      launch_asm<27, 3, 32>(ad, co, cells, n, A, rhs, st);
+    }
+  } else if (ad.dim == 3 && ad.ndof == 8) launch_asm<8, 3, 8>(ad, co, cells, n, A, rhs, st);
   else if (ad.dim == 2 && ad.ndof == 4) launch_asm<4, 2, 4>(ad, co, cells, n, A, rhs, st);
   else if (ad.dim == 2 && ad.ndof == 9) launch_asm<9, 2, 16>(ad, co, cells, n, A, rhs, st);
   else if (ad.dim == 2 && ad.ndof == 3) launch_asm<3, 2, 4>(ad, co, cells, n, A, rhs, st);
