@@ -602,7 +602,17 @@ int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p, p
     hipStream_t st = ctx->stream;
     A->sell_valid = false;
     const RefTable& t = ref_table(s, p->hexa_quad_rule);
-    const pmx::AssemblyData ad = assembly_data(s, t);
+    pmx::AssemblyData ad = assembly_data(s, t);
+    if (ad.dim == 3 && ad.ndof == 27) {
+      // the matrix-core kernel scatters through a position table of the
+      // space's pattern (built once, 4 KB per cell)
+      if (!s->d_q2_pos) {
+        s->d_q2_pos = dmalloc<int>((size_t)ad.n_cells * pmx::kQ2PosPerCell);
+        pmx::q2_scatter_positions(ad, A->A, s->d_q2_pos, st);
+        HIP_CHECK(hipGetLastError());
+      }
+      ad.pos = s->d_q2_pos;
+    }
     const pmx::CDCoeffs co = cd_coeffs(p);
     for (size_t k = 0; k + 1 < s->color_ptr.size(); ++k) {
       const int n = s->color_ptr[k + 1] - s->color_ptr[k];

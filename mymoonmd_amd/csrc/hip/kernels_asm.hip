@@ -584,23 +584,79 @@ __global__ __launch_bounds__(256) void k_assemble_cd_q2_mfma(AssemblyData ad, CD
   }
   if (!valid) return;
   const int* dofs = ad.dofs + (size_t)cell * ND;
+  if (ad.pos) {
+    // precomputed positions: all loads (16 coalesced position loads, the 8
+    // row DOFs, the 16 old values) are issued before any store, so the
+    // read-modify-writes overlap instead of paying one memory latency each
+    const int* pc = ad.pos + (size_t)cell * kQ2PosPerCell + lane;
+    int p[16];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+    for (int k = 0; k < 16; ++k) p[k] = pc[k * 64];
+    bool act[8];
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const int i = 16 * a + 4 * t + (lane >> 4), j = 16 * b + (lane & 15);
-        if (i >= ND || j >= ND) continue;
-        const int r = dofs[i];
-        const int rs = A.row_ptr[r], re = A.row_ptr[r + 1];
-        if (r < A.active) A.val[find_col(A.col, rs, re, dofs[j])] += acc[a][b][t];
-        else if (i == j) A.val[find_col(A.col, rs, re, r)] = 1.0;
+        const int i = 16 * a + 4 * t + (lane >> 4);
+        act[a * 4 + t] = dofs[i < ND ? i : 0] < A.active;
       }
+    double old[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) old[k] = A.val[p[k] >= 0 ? p[k] : 0];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int k = (a * 2 + b) * 4 + t;
+          // Dirichlet rows keep only their unit diagonal (the one slot stored)
+          if (p[k] >= 0) A.val[p[k]] = act[a * 4 + t] ? old[k] + acc[a][b][t] : 1.0;
+        }
+  } else {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int i = 16 * a + 4 * t + (lane >> 4), j = 16 * b + (lane & 15);
+          if (i >= ND || j >= ND) continue;
+          const int r = dofs[i];
+          const int rs = A.row_ptr[r], re = A.row_ptr[r + 1];
+          if (r < A.active) A.val[find_col(A.col, rs, re, dofs[j])] += acc[a][b][t];
+          else if (i == j) A.val[find_col(A.col, rs, re, r)] = 1.0;
+        }
+  }
   if (lane < ND) {
     const int r = dofs[lane];
     if (r < A.active) rhs[r] += frow;
   }
+}
+
+// position table of the Q2 matrix-core assembly: slot ((a * 2 + b) * 4 + t) * 64
+// + lane of a cell holds the entry (i, j) = (16 a + 4 t + lane / 16,
+// 16 b + lane % 16) of its element matrix
+__global__ void k_q2_scatter_positions(int n_cells, const int* __restrict__ dofs, DevCSR A, int* __restrict__ pos) {
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (size_t)n_cells * kQ2PosPerCell) return;
+  const int cell = (int)(g / kQ2PosPerCell), slot = (int)(g % kQ2PosPerCell);
+  const int lane = slot & 63, tile = slot >> 6;   // tile = (a * 2 + b) * 4 + t
+  const int t = tile & 3, b = (tile >> 2) & 1, a = tile >> 3;
+  const int i = 16 * a + 4 * t + (lane >> 4), j = 16 * b + (lane & 15);
+  int p = -1;
+  if (i < 27 && j < 27) {
+    const int r = dofs[(size_t)cell * 27 + i], c = dofs[(size_t)cell * 27 + j];
+    if (r < A.active || r == c) p = find_col(A.col, A.row_ptr[r], A.row_ptr[r + 1], c);
+  }
+  pos[g] = p;
+}
+
+void q2_scatter_positions(const AssemblyData& ad, const DevCSR& A, int* pos, hipStream_t st) {
+  const size_t total = (size_t)ad.n_cells * kQ2PosPerCell;
+  if (total > 0)
+    hipLaunchKernelGGL(k_q2_scatter_positions, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, ad.n_cells,
+                       ad.dofs, A, pos);
 }
 
 void assemble_cd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n,

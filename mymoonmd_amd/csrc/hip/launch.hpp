@@ -142,8 +142,15 @@ struct AssemblyData {
   const double* ref = nullptr;     // per q: w, phi[nd], dphi[dim][nd], d2phi[ncomb][nd]
   const double* geo = nullptr;     // per cell: x0[3], J[9], Jinv[9], detJ, hK, pad
   const int* dofs = nullptr;       // [n_cells][ndof] global numbers
+  // Q2 hexahedra: CSR position of every element-matrix entry in the lane
+  // order of the matrix-core kernel, [cell][16][64] (-1: padding or a
+  // Dirichlet row off the diagonal); nullptr: positions are searched
+  const int* pos = nullptr;
 };
 constexpr int kGeoStride = 24;
+constexpr int kQ2PosPerCell = 1024;
+// fill the position table of the Q2 matrix-core assembly for all cells
+void q2_scatter_positions(const AssemblyData& ad, const DevCSR& A, int* pos, hipStream_t st);
 
 struct CDCoeffs {
   int example = 0;        // built-in right-hand side / exact solution

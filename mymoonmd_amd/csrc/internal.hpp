@@ -109,8 +109,10 @@ struct pmx_space {
   double* d_dof_xyz = nullptr;     // nodal points of all DOFs [n_dof][3]
   double* d_dir_xyz = nullptr;     // the Dirichlet part of d_dof_xyz (not owned)
   std::map<int, RefTable> ref;     // keyed by hexa quad rule
+  int* d_q2_pos = nullptr;         // CSR positions of the Q2 element entries (matrix-core assembly)
   ~pmx_space() {
     pmx_internal::dfree(d_dofs); pmx_internal::dfree(d_geo); pmx_internal::dfree(d_cells); pmx_internal::dfree(d_dof_xyz);
+    pmx_internal::dfree(d_q2_pos);
     for (auto& kv : ref) pmx_internal::dfree(kv.second.d_ref);
   }
 };
