@@ -854,9 +854,19 @@ __global__ __launch_bounds__(256) void k_dense_mv(int n, const double* __restric
   const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= n) return;
   const double* a = A + (size_t)row * n;
-  double s = 0;
-  for (int j = lane; j < n; j += 64) s += a[j] * (double)x[j];
-  s = group_sum<64>(s);
+  // four independent partial sums keep four row loads in flight per lane
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int j = lane;
+  for (; j + 192 < n; j += 256) {
+    const double a0 = __builtin_nontemporal_load(a + j), a1 = __builtin_nontemporal_load(a + j + 64);
+    const double a2 = __builtin_nontemporal_load(a + j + 128), a3 = __builtin_nontemporal_load(a + j + 192);
+    s0 += a0 * (double)x[j];
+    s1 += a1 * (double)x[j + 64];
+    s2 += a2 * (double)x[j + 128];
+    s3 += a3 * (double)x[j + 192];
+  }
+  for (; j < n; j += 64) s0 += a[j] * (double)x[j];
+  double s = group_sum<64>((s0 + s1) + (s2 + s3));
   if (lane == 0) y[row] = (T)s;
 }
 }  // namespace

@@ -152,6 +152,16 @@ struct pmx_solver {
   std::vector<pmx_matrix*> mats;   // level matrices (their SELL values are refreshed per solve)
   std::vector<Dir> cc;
   pmx::DevCSR A;             // finest operator (view)
+  // FGMRES in the multigrid's numbering of the finest level (renumbered
+  // levels): the operator is that level's SELL copy with FP64 values, b and x
+  // are gathered once per solve, the preconditioner needs no permutation
+  const int* kperm = nullptr;    // finest level's perm (owned by the level)
+  pmx::DevCSR K;                 // view of the renumbered finest operator
+  double* kval = nullptr;        // FP64 SELL values of K (mixed precision only)
+  double* kbd = nullptr;         // FP64 Dirichlet-column values of K (mixed only)
+  double *kb = nullptr, *kx = nullptr;   // b and x in that numbering
+  const pmx::DevCSR* op = nullptr;       // operator of the running solve
+  bool internal = false;                 // the running solve uses that numbering
   int n = 0;
   std::vector<double*> V, Z;
   double *r = nullptr, *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *dinv = nullptr;
@@ -172,6 +182,9 @@ struct pmx_solver {
     for (auto*& z : Z) dfree(z);
     V.clear(); Z.clear();
     dfree(r); dfree(w1); dfree(w2); dfree(w3); dfree(dinv); dfree(d_H); dfree(d_scal); dfree(partial);
+    dfree(kval); dfree(kbd); dfree(kb); dfree(kx);
+    kperm = nullptr;
+    K = pmx::DevCSR{};
     updated = false;
   }
   ~pmx_solver() { release(); }
@@ -328,6 +341,20 @@ void mg_apply(pmx_solver* s, const double* v, double* z) {
     mg_cycle(s, false);
     if (F.perm) pmx::vec_scatter(F.n, F.perm, F.sol, z, S(s));
     else HIP_CHECK(hipMemcpyAsync(z, F.sol, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+  }
+}
+
+// the same on vectors in the numbering of the finest level (no permutation)
+void mg_apply_internal(pmx_solver* s, const double* v, double* z) {
+  Level& F = s->lv.back();
+  if (s->p.precision == PMX_PRECISION_MIXED) {
+    pmx::vec_f64_to_f32(F.n, v, F.rhs32, S(s));
+    mg_cycle(s, true);
+    pmx::vec_f32_to_f64(F.n, F.sol32, z, S(s));
+  } else {
+    HIP_CHECK(hipMemcpyAsync(F.rhs, v, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
+    mg_cycle(s, false);
+    HIP_CHECK(hipMemcpyAsync(z, F.sol, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
   }
 }
 
@@ -515,7 +542,7 @@ void op_defect(pmx_solver* s, const double* x, const double* b, double* r) {
     nse_apply_scaled_add(sys, x, r, -1.0, S(s));
     pmx::vec_norm(s->n, r, s->partial, s->d_scal, S(s));
   } else {
-    pmx::fe_defect(s->A, x, b, r, s->partial, s->d_scal, S(s));
+    pmx::fe_defect(s->op ? *s->op : s->A, x, b, r, s->partial, s->d_scal, S(s));
   }
 }
 
@@ -525,12 +552,13 @@ void op_apply(pmx_solver* s, const double* x, double* y) {
     pmx::vec_fill(s->n, 0.0, y, S(s));
     nse_apply_scaled_add(sys, x, y, 1.0, S(s));
   } else {
-    pmx::fe_apply(s->A, x, y, S(s));
+    pmx::fe_apply(s->op ? *s->op : s->A, x, y, S(s));
   }
 }
 
 void precondition(pmx_solver* s, const double* v, double* z) {
   if (s->p.preconditioner == PMX_PC_MULTIGRID && s->saddle) nse_mg_apply(s, v, z);
+  else if (s->p.preconditioner == PMX_PC_MULTIGRID && s->internal) mg_apply_internal(s, v, z);
   else if (s->p.preconditioner == PMX_PC_MULTIGRID) mg_apply(s, v, z);
   else if (s->p.preconditioner == PMX_PC_JACOBI) pmx::vec_mul(s->n, s->dinv, v, z, S(s));
   else HIP_CHECK(hipMemcpyAsync(z, v, s->n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
@@ -564,7 +592,7 @@ void gmres_update(pmx_solver* s, double* x, int k, const std::vector<double>& H,
   for (int j = 0; j <= k; ++j) pmx::vec_axpy(s->n, y[j], nullptr, 1.0, s->Z[j], x, S(s));
 }
 
-void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep) {
+void solve_fgmres_run(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep) {
   const int n = s->n;
   const int m = std::min(s->p.gmres_restart, s->p.max_n_iterations);
   const int ld = m + 1;
@@ -626,6 +654,34 @@ void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
   rep->iterations = s->p.max_n_iterations;
   rep->final_residual = resid;
   rep->converged = 0;
+}
+
+// Scalar systems with renumbered multigrid levels run FGMRES in the finest
+// level's numbering: b and x are permuted once per solve instead of the
+// preconditioner's input and output on every iteration, and the operator is
+// that level's SELL copy, whose gathers are local.  Only the summation order
+// of row sums and dot products differs from the reference numbering.
+void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep) {
+  if (!s->kperm) {
+    s->op = &s->A;
+    s->internal = false;
+    solve_fgmres_run(s, b, x, rep);
+    return;
+  }
+  pmx::vec_gather(s->n, s->kperm, b, s->kb, S(s));
+  pmx::vec_gather(s->n, s->kperm, x, s->kx, S(s));
+  s->op = &s->K;
+  s->internal = true;
+  try {
+    solve_fgmres_run(s, s->kb, s->kx, rep);
+  } catch (...) {
+    s->op = &s->A;
+    s->internal = false;
+    throw;
+  }
+  s->op = &s->A;
+  s->internal = false;
+  pmx::vec_scatter(s->n, s->kperm, s->kx, x, S(s));
 }
 
 void solve_cg(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep) {
@@ -917,6 +973,34 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
       }
       if (s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) build_coarse_inverse(s, levels[0]);
       s->cc = cycle_control(s->p.mg_cycle, n_levels);
+      Level& F = s->lv.back();
+      if (s->p.iterative_solver_type == PMX_IT_FGMRES && F.perm && F.own_sell) {
+        s->kperm = F.perm;
+        s->K = F.A;
+        s->K.val_bf16 = nullptr;
+        s->K.rscale = nullptr;
+        s->K.sell.val_bf16 = nullptr;
+        s->K.sell.bd_val_bf16 = nullptr;
+        if (mixed) {
+          // the multigrid keeps only bf16 values: an FP64 copy for FGMRES
+          s->kval = dmalloc<double>(s->K.sell.total);
+          pmx::sell_fill_f64(s->K, s->kval, st);
+          s->K.sell.val = s->kval;
+          if (s->K.sell.n_bd > 0) {
+            s->kbd = dmalloc<double>(s->K.sell.n_bd);
+            pmx::sell_dirichlet_fill(s->K, s->kbd, nullptr, st);
+          }
+          s->K.sell.bd_val = s->kbd;
+        }
+        require(s->K.sell.val != nullptr, "renumbered finest level without FP64 values");
+        require(s->K.sell.n_bd == 0 || s->K.sell.bd_val != nullptr, "renumbered finest level without Dirichlet values");
+        s->kb = dmalloc<double>(n);
+        s->kx = dmalloc<double>(n);
+        if (pmx::fe_defect_partials(s->K) > std::max(4096, pmx::fe_defect_partials(s->A))) {
+          dfree(s->partial);
+          s->partial = dmalloc<double>(pmx::fe_defect_partials(s->K));
+        }
+      }
     }
     HIP_CHECK(hipStreamSynchronize(st));
     s->updated = true;
