@@ -457,6 +457,13 @@ typedef struct {
                                   results as 1, bit for bit; fewer bytes but slower on
                                   MI355X, the Dirichlet columns then need a per-row tail).
                                   0: the reference order */
+  int transfer_sell;           /* no reference counterpart.  1 (default): the scalar
+                                  multigrid's prolongation and restriction read a SELL-64
+                                  copy of P or P^T (one lane per row, rows sorted by
+                                  length; each row summed sequentially in column order)
+                                  when the product has at least 131,072 rows, else CSR
+                                  with 1-8 lanes per row (partial sums combined across
+                                  lanes).  2: SELL for every level.  0: CSR throughout */
 } pmx_solver_params;
 /* Defaults of Solver<>::default_solver_database (src/Solver/Solver.C:20-117)
  * merged with Multigrid::default_multigrid_database (src/Multigrid/Multigrid.C:422-514). */

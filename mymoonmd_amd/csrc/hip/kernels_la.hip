@@ -339,6 +339,63 @@ __global__ __launch_bounds__(256) void k_csr_spmv(int n, const int* __restrict__
   }
 }
 
+// Grid transfer y = alpha P x + beta y on rows [0, n_sell) from the SELL copy
+// of P (one lane per row, the row's entries in column order: the sequential
+// CSR sum), rows [n_sell, n) set to zero (zero_rest) or left alone.  The
+// slices are remapped across the XCDs as in k_sell_spmv.  Transfer rows are
+// short and uneven (1 to 125 entries for Q2): sorting by length within
+// windows gives slices of one length, with no lanes idling on short rows.
+template <typename T, typename VT>
+__global__ __launch_bounds__(256) void k_sell_transfer(int n_slices, int sell_blocks, int n, int n_sell,
+                                                       const int* __restrict__ sptr,
+                                                       const int* __restrict__ rowmap,
+                                                       const int* __restrict__ ci, const VT* __restrict__ va,
+                                                       const T* __restrict__ x, T* __restrict__ y,
+                                                       double alpha, double beta, int zero_rest) {
+  using AT = typename std::conditional<sizeof(T) == 8, double, float>::type;
+  if ((int)blockIdx.x < sell_blocks) {
+    const int per = sell_blocks >> 3;
+    const int vb = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    const int lane = threadIdx.x & 63;
+    const int slice = vb * 4 + (int)(threadIdx.x >> 6);
+    if (slice >= n_slices) return;
+    const int r = rowmap[slice * 64 + lane];
+    const int s0 = sptr[slice];
+    const int width = (sptr[slice + 1] - s0) >> 6;
+    const int* cp = ci + s0 + lane;
+    const VT* vp = va + s0 + lane;
+    AT acc = 0;
+    int j = 0;
+    for (; j + 4 <= width; j += 4) {
+      int c[4];
+      AT v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c[u] = __builtin_nontemporal_load(cp + (j + u) * 64);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if constexpr (sizeof(VT) == 2) v[u] = (AT)bf16_to_float(__builtin_nontemporal_load(vp + (j + u) * 64));
+        else v[u] = (AT)__builtin_nontemporal_load(vp + (j + u) * 64);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc += v[u] * (AT)x[c[u]];
+    }
+    for (; j < width; ++j) {
+      AT v;
+      if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(__builtin_nontemporal_load(vp + j * 64));
+      else v = (AT)__builtin_nontemporal_load(vp + j * 64);
+      acc += v * (AT)x[__builtin_nontemporal_load(cp + j * 64)];
+    }
+    if (r >= 0) {
+      AT res = (AT)alpha * acc;
+      if (beta != 0.0) res += (AT)beta * (AT)y[r];
+      y[r] = (T)res;
+    }
+  } else if (zero_rest) {
+    const int i = n_sell + (int)(blockIdx.x - sell_blocks) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (T)0;
+  }
+}
+
 // ---------------------------------------------------------------- BLAS-1
 constexpr int kRedBlocks = 1024;
 
@@ -634,6 +691,20 @@ void fe_jacobi_bf16(const DevCSR& A, const float* x, const float* b, const float
 template <typename T>
 void csr_spmv_t(const DevCSR& P, const T* x, T* y, double alpha, double beta, int row_end,
                 int zero_rest, hipStream_t st) {
+  if (P.sell.n_slices > 0 && P.sell.n_rows == row_end) {
+    const int sb = ((P.sell.n_slices + 3) / 4 + 7) / 8 * 8;
+    const int tail = zero_rest ? (P.n - row_end + 255) / 256 : 0;
+    const bool h = P.sell.val_bf16 != nullptr;
+    if (h)
+      hipLaunchKernelGGL((k_sell_transfer<T, uint16_t>), dim3(sb + tail), dim3(256), 0, st, P.sell.n_slices, sb, P.n,
+                         row_end, P.sell.slice_ptr, P.sell.row, P.sell.col, P.sell.val_bf16, x, y, alpha, beta,
+                         zero_rest);
+    else
+      hipLaunchKernelGGL((k_sell_transfer<T, double>), dim3(sb + tail), dim3(256), 0, st, P.sell.n_slices, sb, P.n,
+                         row_end, P.sell.slice_ptr, P.sell.row, P.sell.col, P.sell.val, x, y, alpha, beta,
+                         zero_rest);
+    return;
+  }
   // lanes per row: P.lpr (1, 2, 4 or 8; short interpolation rows use fewer)
   const int lpr = (P.lpr == 1 || P.lpr == 2 || P.lpr == 4) ? P.lpr : 8;
   const int grid = (P.n + (256 / lpr) - 1) / (256 / lpr);

@@ -29,6 +29,9 @@ struct DevSELL {
   // 16-bit layout: slots hold col - row of the active-column entries (col and
   // the Dirichlet-column entries are then only in the bd lists)
   int16_t* off = nullptr;
+  // grid-transfer matrices: the SELL copy (slice_ptr, row, col, val and, when
+  // the weights are bf16 numbers, val_bf16) covers rows [0, n_rows)
+  int n_rows = 0;
 };
 
 // Device CSR matrix (0-based int32 row pointer / columns, sorted columns).

@@ -53,6 +53,8 @@ std::vector<Dir> cycle_control(int cycle, int n_levels) {
 
 void free_csr(pmx::DevCSR& M) {
   dfree(M.row_ptr); dfree(M.col); dfree(M.val); dfree(M.val_bf16); dfree(M.rscale);
+  dfree(M.sell.slice_ptr); dfree(M.sell.row); dfree(M.sell.col); dfree(M.sell.val); dfree(M.sell.val_bf16);
+  M.sell = pmx::DevSELL{};
 }
 
 pmx::DevCSR upload_csr(const pmx::CSRPattern& p, const std::vector<double>& v, hipStream_t st) {
@@ -81,6 +83,67 @@ pmx::DevCSR upload_csr(const pmx::CSRPattern& p, const std::vector<double>& v, h
   }
   if (exact) M.val_bf16 = upload(h.data(), h.size(), st);
   return M;
+}
+
+// SELL-64 copy of a grid-transfer matrix over its rows [0, row_end), for the
+// transfer kernel with one lane per row: rows sorted by length (descending,
+// stable) within windows of 4096, each row's entries in CSR column order,
+// padding slots (column 0, value 0) up to the slice's longest row; the bf16
+// copy exists when the CSR upload found every weight to be a bf16 number
+void attach_transfer_sell(pmx::DevCSR& M, const pmx::CSRPattern& p, const std::vector<double>& v, int row_end,
+                          hipStream_t st) {
+  constexpr int C = 64, sigma = 4096;
+  require(row_end >= 0 && row_end <= p.n_rows, "transfer rows out of range");
+  if (row_end == 0) return;
+  auto len = [&](int r) { return p.row_ptr[r + 1] - p.row_ptr[r]; };
+  std::vector<int> order(row_end);
+  for (int r = 0; r < row_end; ++r) order[r] = r;
+  for (int w = 0; w < row_end; w += sigma)
+    std::stable_sort(order.begin() + w, order.begin() + std::min(row_end, w + sigma),
+                     [&](int a, int b) { return len(a) > len(b); });
+  const int n_slices = (row_end + C - 1) / C;
+  std::vector<int> rowmap((size_t)n_slices * C, -1), sptr(n_slices + 1, 0);
+  for (int sl = 0; sl < n_slices; ++sl) {
+    int width = 0;
+    for (int l = 0; l < C && sl * C + l < row_end; ++l) {
+      rowmap[(size_t)sl * C + l] = order[sl * C + l];
+      width = std::max(width, len(order[sl * C + l]));
+    }
+    sptr[sl + 1] = sptr[sl] + width * C;
+  }
+  const size_t total = sptr[n_slices];
+  std::vector<int> col(total, 0);
+  std::vector<double> val(total, 0.0);
+  for (int sl = 0; sl < n_slices; ++sl) {
+    const int width = (sptr[sl + 1] - sptr[sl]) / C;
+    for (int l = 0; l < C; ++l) {
+      const int r = rowmap[(size_t)sl * C + l];
+      if (r < 0) continue;
+      for (int j = 0; j < len(r); ++j) {
+        const size_t k = (size_t)sptr[sl] + (size_t)j * C + l;
+        col[k] = p.col[p.row_ptr[r] + j];
+        val[k] = v[p.row_ptr[r] + j];
+      }
+      (void)width;
+    }
+  }
+  M.sell.n_slices = n_slices;
+  M.sell.total = total;
+  M.sell.n_rows = row_end;
+  M.sell.slice_ptr = upload(sptr.data(), sptr.size(), st);
+  M.sell.row = upload(rowmap.data(), rowmap.size(), st);
+  M.sell.col = upload(col.data(), col.size(), st);
+  M.sell.val = upload(val.data(), val.size(), st);
+  if (M.val_bf16) {
+    std::vector<uint16_t> h(total);
+    for (size_t k = 0; k < total; ++k) {
+      const float f = (float)val[k];
+      uint32_t u;
+      std::memcpy(&u, &f, sizeof(u));
+      h[k] = (uint16_t)(u >> 16);
+    }
+    M.sell.val_bf16 = upload(h.data(), h.size(), st);
+  }
 }
 
 // one level of the saddle-point multigrid: the system (not owned), the
@@ -787,6 +850,7 @@ int pmx_solver_default_params(pmx_solver_params* p) {
     d.vanka_damp = 1.0;
     d.zero_start_shortcut = 1;
     d.renumber_levels = 1;
+    d.transfer_sell = 1;
     *p = d;
   });
 }
@@ -970,6 +1034,12 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
         pmx::permute_csr(R, &rv, perm[l], pinv[l + 1], Rp, &rpv, nullptr, true);
         L.P = upload_csr(Pp, ppv, st);
         L.R = upload_csr(Rp, rpv, st);
+        // SELL pays off where there are enough rows for the lane-per-row
+        // layout to fill the GPU; long restriction rows on small levels run
+        // faster with several lanes per row
+        auto use_sell = [&](int rows) { return s->p.transfer_sell >= 2 || (s->p.transfer_sell == 1 && rows >= 131072); };
+        if (use_sell(s->lv[l + 1].active)) attach_transfer_sell(L.P, Pp, ppv, s->lv[l + 1].active, st);
+        if (use_sell(L.active)) attach_transfer_sell(L.R, Rp, rpv, L.active, st);
       }
       if (s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) build_coarse_inverse(s, levels[0]);
       s->cc = cycle_control(s->p.mg_cycle, n_levels);

@@ -311,6 +311,37 @@ def test_zero_start_sweep_and_renumbering(ctx, precision, cycle):
         assert abs(a[2] - b[2]) <= 1 and np.max(np.abs(a[1] - b[1])) <= 1e-7 * np.max(np.abs(a[1])), coarse
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_transfer_sell_matches_csr(ctx, precision):
+    """Prolongation and restriction from the SELL copy of P and P^T (one lane
+    per row, sequential row sums) against the CSR kernels (1-8 lanes per row):
+    one V-cycle agrees to rounding (1e-12 FP64, 1e-5 FP32), and FGMRES+MG
+    takes the same number of iterations to the same solution."""
+    nref = 3
+    d, spaces, mats, rhs = three_level_cd3d(ctx, nref, MG_KW)
+    fs = spaces[-1]
+    out = {}
+    for ts in (0, 2):
+        prm = pm.solver_params(iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
+                               residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=100,
+                               multigrid_cycle_type="V", multigrid_smoother="jacobi",
+                               multigrid_smoother_coarse="direct_solve", multigrid_n_pre_smooth=2,
+                               multigrid_n_post_smooth=2, jacobi_damp=0.5, precision=precision,
+                               transfer_sell=ts)
+        sol = pm.Solver(ctx, prm)
+        sol.update_matrix(mats, spaces)
+        z = torch.zeros_like(rhs[-1])
+        sol.apply_preconditioner(rhs[-1].clone(), z)
+        x = torch.zeros_like(rhs[-1])
+        x[fs.active_bound:] = rhs[-1][fs.active_bound:]
+        rep = sol.solve(rhs[-1], x)
+        out[ts] = (z.cpu().numpy(), x.cpu().numpy(), rep.iterations)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    a, b = out[0], out[2]
+    assert np.max(np.abs(a[0] - b[0])) <= tol * np.max(np.abs(a[0]))
+    assert abs(a[2] - b[2]) <= 1 and np.max(np.abs(a[1] - b[1])) <= 1e-7 * np.max(np.abs(a[1]))
+
+
 def test_fgmres_multigrid_solution_matches_oracle(ctx):
     """FGMRES(30) + MG V(2,2) to 1e-8.  Damping 0.5 keeps the Jacobi smoother
     contractive for this Q2 SUPG operator (lambda_max(D^-1 A) ~ 3), so the
