@@ -403,6 +403,83 @@ int pmx_batch_get_matrices(const pmx_batch* b, int problem, double* host_stiffne
 /* L2, H1-semi, SD, L_inf errors of one problem at the current time */
 int pmx_batch_errors(pmx_batch* b, int problem, double* host_errors4);
 
+/* -------------------------------------------------- snapshots and POD */
+/* SnapshotsCollector (include/ReducedOrderModels/SnapshotsCollector.h:22-80,
+ * src/ReducedOrderModels/SnapshotsCollector.C:36-95).  The snapshots stay in
+ * HBM as a column-major length x max_snaps matrix (snapshot j at dev + j *
+ * length); when `path` is not NULL every stored snapshot is also appended to
+ * that file in the reference's text format (one line per snapshot, each
+ * entry printed with 12 significant digits followed by a space; the file is
+ * truncated on creation).  steps_per_snap: db["steps_per_snap"]. */
+typedef struct pmx_snapshots pmx_snapshots;
+int pmx_snapshots_create(pmx_ctx* ctx, const char* path, int steps_per_snap, int length, int max_snaps,
+                         pmx_snapshots** out);
+int pmx_snapshots_destroy(pmx_snapshots* s);
+/* SnapshotsCollector::write_data(solution, time_step): stores dev_solution
+ * (length entries) when time_step % steps_per_snap == 0; *written (may be
+ * NULL) tells whether it did.  More than max_snaps snapshots: PMX_E_ARG. */
+int pmx_snapshots_write_data(pmx_snapshots* s, const double* dev_solution, int time_step, int* written);
+/* number of stored snapshots and the device matrix (valid until destroy) */
+int pmx_snapshots_get(const pmx_snapshots* s, int* n_snaps, const double** dev_matrix);
+
+/* POD (include/ReducedOrderModels/POD.h:35-283, src/ReducedOrderModels/
+ * POD.C:20-616): method of snapshots.  The mean over the snapshots is removed
+ * (pod_fluctuations_only), the autocorrelation matrix S^T S (euclidean) or
+ * S^T M S (L2, M = the Gramian set by pmx_pod_set_gramian) is formed on the
+ * FP64 matrix cores, its eigenproblem (LAPACK dsyev in the reference) is
+ * solved on the host by cyclic Jacobi rotations, and the basis
+ * phi_k = S v_k / sqrt(lambda_k) of the `rank` largest eigenvalues above
+ * 1e-10 is formed on the device.  Eigenvector signs are fixed so that the
+ * largest-magnitude component of each v_k is positive (dsyev leaves the sign
+ * to the implementation). */
+#define PMX_POD_EUCLIDEAN 0
+#define PMX_POD_L2 1
+typedef struct pmx_pod pmx_pod;
+typedef struct {
+  int pod_rank;                 /* 0: all eigenvalues above the threshold */
+  int pod_fluctuations_only;    /* 1 (default): subtract the mean of the snapshots */
+  int pod_inner_product;        /* PMX_POD_EUCLIDEAN (default) or PMX_POD_L2 */
+} pmx_pod_params;
+/* POD::default_pod_database (src/ReducedOrderModels/POD.C:20-60) */
+int pmx_pod_default_params(pmx_pod_params* p);
+int pmx_pod_create(pmx_ctx* ctx, const pmx_pod_params* p, pmx_pod** out);
+int pmx_pod_destroy(pmx_pod* pod);
+/* POD::set_gramian (POD.C:238-242): the inner-product matrix of the L2 POD,
+ * e.g. the mass matrix (TimeConvectionDiffusionPOD::assemble_gramian) */
+int pmx_pod_set_gramian(pmx_pod* pod, const pmx_matrix* gramian);
+/* POD::read_snapshots (POD.C:92-118): the snapshot file of a collector */
+int pmx_pod_read_snapshots(pmx_pod* pod, const char* path);
+/* snapshots from device memory (copied): column j at dev_snaps + j * ld */
+int pmx_pod_set_snapshots(pmx_pod* pod, const double* dev_snaps, int length, int n_snaps, int ld);
+/* POD::compute_basis (POD.C:121-235) */
+int pmx_pod_compute_basis(pmx_pod* pod);
+/* length, number of snapshots, rank of the basis, eigenvalues above 1e-10 */
+int pmx_pod_get_info(const pmx_pod* pod, int* length, int* n_snaps, int* rank, int* valid_eigs);
+/* the n_snaps eigenvalues, largest first (POD::get_pod_eigs after compute_basis) */
+int pmx_pod_get_eigs(const pmx_pod* pod, double* host_eigs);
+/* basis (length x rank, mode k at dev_out + k * length) and mean (length)
+ * copied into device memory; POD::get_basis / get_snaps_avr */
+int pmx_pod_get_basis(const pmx_pod* pod, double* dev_out);
+int pmx_pod_get_mean(const pmx_pod* pod, double* dev_out);
+/* POD::write_pod(basename) (POD.C:299-330, 424-498): <directory>/<basename>pod
+ * (header line "POD basis: fluctuating field" or "POD basis: full field", then
+ * one mode per line), <basename>mean (with fluctuations) and <basename>eigs
+ * ("i \t lambda_i \t cumulative energy" per eigenvalue >= 1e-10) */
+int pmx_pod_write(const pmx_pod* pod, const char* directory, const char* basename);
+
+/* The same for every problem of a batched sweep (no reference counterpart:
+ * the reference runs one TimeConvectionDiffusionPOD per problem).  Snapshots
+ * of all problems are stored together (one device copy of the current
+ * solutions per snapshot); the Gram matrices of all problems come from one
+ * launch pair, the L2 variant with each problem's mass matrix. */
+int pmx_batch_snapshots_reserve(pmx_batch* b, int max_snaps);
+int pmx_batch_write_snapshot(pmx_batch* b);
+int pmx_batch_compute_pod(pmx_batch* b, const pmx_pod_params* p);
+/* rank, eigenvalues (n_snaps, largest first), basis (n_dof x rank, mode k at
+ * k * n_dof) and mean (n_dof) of one problem; host pointers may be NULL */
+int pmx_batch_get_pod(const pmx_batch* b, int problem, int* rank, double* host_eigs, double* host_basis,
+                      double* host_mean);
+
 /* ------------------------------------------------------------------ solver */
 #define PMX_IT_FGMRES 0
 #define PMX_IT_CG 1

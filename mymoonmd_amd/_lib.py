@@ -83,6 +83,10 @@ class SolverParams(ctypes.Structure):
                 ("transfer_sell", c_int)]
 
 
+class PodParams(ctypes.Structure):
+    _fields_ = [("pod_rank", c_int), ("pod_fluctuations_only", c_int), ("pod_inner_product", c_int)]
+
+
 class SolveReport(ctypes.Structure):
     _fields_ = [("iterations", c_int), ("initial_residual", c_double),
                 ("final_residual", c_double), ("converged", c_int)]
@@ -165,6 +169,26 @@ SIGNATURES = {
     "pmx_nse_project_pressure": [c_void_p, c_void_p, c_void_p],
     "pmx_nse_restrict_velocity": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "pmx_nse_errors": [c_void_p, c_void_p, c_int, c_void_p, P(c_double)],
+    "pmx_snapshots_create": [c_void_p, ctypes.c_char_p, c_int, c_int, c_int, P(c_void_p)],
+    "pmx_snapshots_destroy": [c_void_p],
+    "pmx_snapshots_write_data": [c_void_p, c_void_p, c_int, P(c_int)],
+    "pmx_snapshots_get": [c_void_p, P(c_int), P(c_void_p)],
+    "pmx_pod_default_params": [P(PodParams)],
+    "pmx_pod_create": [c_void_p, P(PodParams), P(c_void_p)],
+    "pmx_pod_destroy": [c_void_p],
+    "pmx_pod_set_gramian": [c_void_p, c_void_p],
+    "pmx_pod_read_snapshots": [c_void_p, ctypes.c_char_p],
+    "pmx_pod_set_snapshots": [c_void_p, c_void_p, c_int, c_int, c_int],
+    "pmx_pod_compute_basis": [c_void_p],
+    "pmx_pod_get_info": [c_void_p, P(c_int), P(c_int), P(c_int), P(c_int)],
+    "pmx_pod_get_eigs": [c_void_p, P(c_double)],
+    "pmx_pod_get_basis": [c_void_p, c_void_p],
+    "pmx_pod_get_mean": [c_void_p, c_void_p],
+    "pmx_pod_write": [c_void_p, ctypes.c_char_p, ctypes.c_char_p],
+    "pmx_batch_snapshots_reserve": [c_void_p, c_int],
+    "pmx_batch_write_snapshot": [c_void_p],
+    "pmx_batch_compute_pod": [c_void_p, P(PodParams)],
+    "pmx_batch_get_pod": [c_void_p, c_int, P(c_int), P(c_double), P(c_double), P(c_double)],
     "pmx_solver_default_params": [P(SolverParams)],
     "pmx_solver_create": [c_void_p, P(SolverParams), P(c_void_p)],
     "pmx_solver_destroy": [c_void_p],

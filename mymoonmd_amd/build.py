@@ -23,10 +23,12 @@ SOURCES = [
     "solver.cpp",
     "batch.cpp",
     "nse.cpp",
+    "pod.cpp",
     "hip/kernels_la.hip",
     "hip/kernels_asm.hip",
     "hip/kernels_batch.hip",
     "hip/kernels_nse.hip",
+    "hip/kernels_pod.hip",
 ]
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")

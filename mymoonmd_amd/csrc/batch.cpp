@@ -74,6 +74,14 @@ struct pmx_batch {
   double* d_time = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // snapshots of all problems (total_dof x snap_max, column-major) and the
+  // POD of every problem (basis total_dof x pod_kmax, mean total_dof)
+  double* d_snaps = nullptr;
+  int snap_max = 0, snap_count = 0;
+  double *d_pod_basis = nullptr, *d_pod_mean = nullptr;
+  int pod_m = 0, pod_kmax = 0;
+  std::vector<int> pod_rank;
+  std::vector<double> pod_eigs;   // n_prob x pod_m, largest first
 
   ~pmx_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -82,7 +90,8 @@ struct pmx_batch {
              d_rp, d_col, d_A, d_M, d_S, d_sptr, d_srow, d_scol, d_ssrc, d_slice_prob, d_S64, d_M64, d_S32,
              d_dinv, d_dinv32, d_zero, d_geo, d_dofs, d_cells, d_cell_prob, d_active_end, d_coef, d_xyz,
              d_dir, d_row_active, d_ref, d_order, d_dof_off, d_n_active, d_slice_off, d_ws_off, d_x, d_f,
-             d_rhs, d_r, d_w, d_V, d_iters, d_outer, d_fail, d_res, d_time})
+             d_rhs, d_r, d_w, d_V, d_iters, d_outer, d_fail, d_res, d_time, d_snaps, d_pod_basis,
+             d_pod_mean})
       dfree(p);
   }
 
@@ -641,6 +650,104 @@ int pmx_batch_errors(pmx_batch* b, int problem, double* host_errors4) {
     HIP_CHECK(hipMemcpyAsync(host_errors4, b->ctx->scal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     dfree(partial);
+  });
+}
+
+// ------------------------------------------------ snapshots and POD per problem
+int pmx_batch_snapshots_reserve(pmx_batch* b, int max_snaps) {
+  return guarded([&] {
+    require(b && max_snaps >= 1, "invalid argument");
+    dfree(b->d_snaps);
+    b->d_snaps = dmalloc<double>((size_t)b->total_dof * max_snaps);
+    b->snap_max = max_snaps;
+    b->snap_count = 0;
+  });
+}
+
+// SnapshotsCollector::write_data for every problem: one device copy of the
+// current solutions into the next snapshot column
+int pmx_batch_write_snapshot(pmx_batch* b) {
+  return guarded([&] {
+    require(b, "NULL argument");
+    if (!b->d_snaps) throw PmxError(PMX_E_STATE, "reserve snapshot storage first");
+    if (b->snap_count >= b->snap_max) throw PmxError(PMX_E_ARG, "snapshot storage is full");
+    HIP_CHECK(hipMemcpyAsync(b->d_snaps + (size_t)b->snap_count * b->total_dof, b->d_x,
+                             sizeof(double) * b->total_dof, hipMemcpyDeviceToDevice, b->ctx->stream));
+    b->snap_count++;
+  });
+}
+
+// POD::compute_basis for every problem: mean and fluctuations of all problems
+// in one launch, the Gram matrices of all problems from one launch pair (the
+// L2 variant with the block-diagonal mass matrix of the batch), the m x m
+// eigenproblems on the host, the bases of all problems in one launch
+int pmx_batch_compute_pod(pmx_batch* b, const pmx_pod_params* p) {
+  return guarded([&] {
+    require(b && p, "NULL argument");
+    require(p->pod_inner_product == PMX_POD_EUCLIDEAN || p->pod_inner_product == PMX_POD_L2,
+            "pod_inner_product must be euclidean or L2");
+    if (b->snap_count < 1) throw PmxError(PMX_E_STATE, "Snapshots are not available");
+    hipStream_t st = b->ctx->stream;
+    const int n = b->total_dof, m = b->snap_count, np = b->n_prob;
+    double* S = dmalloc<double>((size_t)n * m);
+    HIP_CHECK(hipMemcpyAsync(S, b->d_snaps, sizeof(double) * n * m, hipMemcpyDeviceToDevice, st));
+    dfree(b->d_pod_mean);
+    dfree(b->d_pod_basis);
+    b->d_pod_mean = dmalloc<double>(n);
+    if (p->pod_fluctuations_only) pmx::pod_mean(n, m, n, S, b->d_pod_mean, true, st);
+    else pmx::vec_fill(n, 0.0, b->d_pod_mean, st);
+    double* WS = nullptr;
+    if (p->pod_inner_product == PMX_POD_L2) {
+      WS = dmalloc<double>((size_t)n * m);
+      pmx::pod_spmm(n, m, n, b->d_rp, b->d_col, b->d_M, S, WS, st);
+    }
+    std::vector<int> row0(b->dof_off.begin(), b->dof_off.begin() + np), rows(np);
+    for (int q = 0; q < np; ++q) rows[q] = b->dof_off[q + 1] - b->dof_off[q];
+    std::vector<double> G;
+    pod_gram_jobs(st, row0, rows, m, n, S, WS ? WS : S, G);
+    b->pod_m = m;
+    b->pod_rank.assign(np, 0);
+    b->pod_eigs.assign((size_t)np * m, 0.0);
+    std::vector<PodEig> eig(np);
+#pragma omp parallel for schedule(dynamic)
+    for (int q = 0; q < np; ++q) eig[q] = pod_eigen(G.data() + (size_t)q * m * m, m, p->pod_rank);
+    int kmax = 1;
+    for (int q = 0; q < np; ++q) {
+      b->pod_rank[q] = eig[q].rank;
+      std::copy(eig[q].eigs.begin(), eig[q].eigs.end(), b->pod_eigs.begin() + (size_t)q * m);
+      kmax = std::max(kmax, eig[q].rank);
+    }
+    b->pod_kmax = kmax;
+    std::vector<double> C((size_t)np * m * kmax, 0.0);
+    for (int q = 0; q < np; ++q)
+      for (int k = 0; k < eig[q].rank; ++k)
+        for (int j = 0; j < m; ++j) C[(size_t)q * m * kmax + (size_t)k * m + j] = eig[q].C[(size_t)k * m + j];
+    b->d_pod_basis = dmalloc<double>((size_t)n * kmax);
+    double* d_C = upload(C.data(), C.size(), st);
+    int* d_rank = upload(b->pod_rank.data(), b->pod_rank.size(), st);
+    pmx::pod_basis(n, np, b->d_dof_off, d_rank, m, kmax, n, S, d_C, b->d_pod_basis, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+    HIP_CHECK(hipGetLastError());
+    dfree(d_C); dfree(d_rank); dfree(S); dfree(WS);
+  });
+}
+
+int pmx_batch_get_pod(const pmx_batch* b, int problem, int* rank, double* host_eigs, double* host_basis,
+                      double* host_mean) {
+  return guarded([&] {
+    require(b && problem >= 0 && problem < b->n_prob, "invalid argument");
+    if (!b->d_pod_mean) throw PmxError(PMX_E_STATE, "POD not computed");
+    hipStream_t st = b->ctx->stream;
+    const int o = b->dof_off[problem], n = b->dof_off[problem + 1] - o, r = b->pod_rank[problem];
+    if (rank) *rank = r;
+    if (host_eigs) std::copy(b->pod_eigs.begin() + (size_t)problem * b->pod_m,
+                             b->pod_eigs.begin() + (size_t)(problem + 1) * b->pod_m, host_eigs);
+    if (host_basis && r > 0)
+      HIP_CHECK(hipMemcpy2DAsync(host_basis, sizeof(double) * n, b->d_pod_basis + o, sizeof(double) * b->total_dof,
+                                 sizeof(double) * n, r, hipMemcpyDeviceToHost, st));
+    if (host_mean)
+      HIP_CHECK(hipMemcpyAsync(host_mean, b->d_pod_mean + o, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(hipStreamSynchronize(st));
   });
 }
 

@@ -275,4 +275,16 @@ void fe_multiply_active(const DevCSR& A, const double* x, double* y, double a, h
 void cd_errors(const AssemblyData& ad, const CDCoeffs& co, const double* u, double* partial4,
                double* out4, hipStream_t st);
 
+
+// ---- snapshots / POD (kernels_pod.hip); snapshot matrices are column-major
+// over global rows with leading dimension ld
+constexpr int kPodChunk = 4096;   // rows per workgroup of the Gram kernel
+void pod_mean(int rows, int m, long long ld, double* S, double* mean, bool fluct, hipStream_t st);
+void pod_spmm(int rows, int m, long long ld, const int* rp, const int* ci, const double* va, const double* S,
+              double* WS, hipStream_t st);
+void pod_gram(int n_chunks, const int* chunk_row0, const int* chunk_rows, int n_jobs, const int* job_chunk0, int m,
+              long long ld, const double* S, const double* WS, double* partial, double* G, hipStream_t st);
+void pod_basis(int rows, int n_jobs, const int* job_row0, const int* job_rank, int m, int kmax, long long ld,
+               const double* S, const double* C, double* Phi, hipStream_t st);
+
 }  // namespace pmx

@@ -133,6 +133,19 @@ struct pmx_matrix {
 };
 
 namespace pmx_internal {
+// POD helpers (pod.cpp)
+struct PodEig {
+  std::vector<double> eigs;   // all eigenvalues, largest first
+  int rank = 0, valid = 0;
+  bool clamped = false;       // pod_rank was larger than the valid eigenvalues
+  std::vector<double> C;      // m x rank coefficients of the modes (column-major)
+};
+void sym_eig(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& V);
+PodEig pod_eigen(const double* G, int m, int pod_rank);
+void pod_gram_jobs(hipStream_t st, const std::vector<int>& row0, const std::vector<int>& rows, int m, long long ld,
+                   const double* S, const double* WS, std::vector<double>& G);
+void put_value(std::string& out, double v);
+void make_directory(const std::string& dir);
 // device copy of a SELL layout (values not allocated); sizes checked
 pmx::DevSELL upload_sell(const pmx::SellPattern& sp, hipStream_t st);
 // upload the SELL layout of a matrix pattern (once) and mark its values stale

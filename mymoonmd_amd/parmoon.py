@@ -14,8 +14,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (BC2D_FN, BC3D_FN, BatchInfo, BatchParams, BatchProblem, BatchReport, CDParams,
-                   NseInfo, NseParams, SolveReport, SolverParams, SpaceInfo, call, c_double, c_int,
-                   c_void_p)
+                   NseInfo, NseParams, PodParams, SolveReport, SolverParams, SpaceInfo, call, c_double,
+                   c_int, c_void_p)
 
 GEOMETRY = {"UnitSquare": 0, "TwoTriangles": 1, "Default_UnitCube_Hexa": 2}
 EXAMPLES = {"const": 0, "sine_laplace_2d": 1, "p0_3d": 2, "p1_3d": 3, "p2_3d": 4,
@@ -564,6 +564,45 @@ class TimeConvectionDiffusion:
         return compute_errors(self.ctx, self.spaces[-1], self.params(), self.solution)
 
 
+class TimeConvectionDiffusionPOD:
+    """TimeConvectionDiffusionPOD<d> (src/ReducedOrderModels/
+    TimeConvectionDiffusionPOD.C:37-273): the POD of TCD snapshots on a
+    space of the domain; with pod_inner_product "L2" the Gramian is the
+    Galerkin mass matrix (assemble_gramian, TCDMassOnly), and
+    compute_pod_basis writes <pod_basename>.<inner product>.{pod,mean,eigs}."""
+
+    def __init__(self, ctx, domain, order, example="tcd_exp", pod_directory=".", pod_basename="parmoon_pod",
+                 pod_rank=0, pod_fluctuations_only=True, pod_inner_product="euclidean", eps=1.0,
+                 b=(0.0, 0.0, 0.0), c=0.0, bc="all_dirichlet"):
+        import os
+        import torch
+        self.ctx = ctx
+        self.space = TFESpace(domain, domain.n_refine, order, ctx, bc=bc)
+        self.pod_directory, self.pod_basename = pod_directory, pod_basename
+        self.inner_product = pod_inner_product
+        self.pod = POD(ctx, pod_rank, pod_fluctuations_only, pod_inner_product)
+        self.gramian = FEMatrix(ctx, self.space)
+        self._params = cd_params(example, disc="galerkin", eps=eps, b=b, c=c)
+        self._rhs = torch.zeros(self.space.n_dof, dtype=torch.float64, device="cuda:%d" % ctx.device)
+        os.makedirs(pod_directory, exist_ok=True)
+
+    def assemble_gramian(self):
+        self.gramian.reset()
+        assemble_tcd(self.ctx, self.space, self._params, None, self.gramian, None)
+
+    def compute_pod_basis(self, snapshots):
+        """snapshots: a SnapshotsCollector or the path of a snapshot file"""
+        if self.inner_product == "L2":
+            self.assemble_gramian()
+            self.pod.set_gramian(self.gramian)
+        if isinstance(snapshots, str):
+            self.pod.read_snapshots(snapshots)
+        else:
+            self.pod.set_snapshots(snapshots)
+        self.pod.compute_basis()
+        self.pod.write_pod(self.pod_directory, self.pod_basename + "." + self.inner_product + ".")
+
+
 class BatchTCD:
     """Many independent TCD problems (P1 on n x n unit-square triangulations,
     own eps, b, c) advanced together by backward Euler (pmx_batch_*).
@@ -649,9 +688,161 @@ class BatchTCD:
         call("pmx_batch_errors", self.h, p, out)
         return list(out)
 
+    # snapshots and POD of every problem (pmx_batch_snapshots_* / _pod)
+    def reserve_snapshots(self, max_snaps):
+        call("pmx_batch_snapshots_reserve", self.h, max_snaps)
+        self.n_snaps = 0
+
+    def write_snapshot(self):
+        call("pmx_batch_write_snapshot", self.h)
+        self.n_snaps += 1
+
+    def compute_pod(self, pod_rank=0, pod_fluctuations_only=True, pod_inner_product="euclidean"):
+        call("pmx_batch_compute_pod", self.h, ctypes.byref(pod_params(pod_rank, pod_fluctuations_only,
+                                                                       pod_inner_product)))
+
+    def pod(self, p):
+        """(eigenvalues largest first, basis n_dof x rank, mean) of problem p"""
+        n, _, _ = self.problem_size(p)
+        r = c_int()
+        call("pmx_batch_get_pod", self.h, p, ctypes.byref(r), None, None, None)
+        eigs = np.zeros(self.n_snaps)
+        basis = np.zeros((max(r.value, 1), n))
+        mean = np.zeros(n)
+        P = ctypes.POINTER(c_double)
+        call("pmx_batch_get_pod", self.h, p, ctypes.byref(r), eigs.ctypes.data_as(P), basis.ctypes.data_as(P),
+             mean.ctypes.data_as(P))
+        return eigs, basis[:r.value].T.copy(), mean
+
     def __del__(self, _d=_destroy):
         if getattr(self, "h", None) and self.h.value:
             _d("pmx_batch_destroy", self.h)
+            self.h = None
+
+
+POD_INNER_PRODUCT = {"euclidean": 0, "L2": 1}
+
+
+def pod_params(pod_rank=0, pod_fluctuations_only=True, pod_inner_product="euclidean"):
+    """POD::default_pod_database (src/ReducedOrderModels/POD.C:20-60) with overrides"""
+    p = PodParams()
+    call("pmx_pod_default_params", ctypes.byref(p))
+    p.pod_rank = pod_rank
+    p.pod_fluctuations_only = 1 if pod_fluctuations_only else 0
+    p.pod_inner_product = POD_INNER_PRODUCT[pod_inner_product]
+    return p
+
+
+class SnapshotsCollector:
+    """SnapshotsCollector (src/ReducedOrderModels/SnapshotsCollector.C):
+    write_data(solution, time_step) keeps every steps_per_snap-th solution in
+    device memory and, with write_snaps, appends it to
+    snaps_directory/snaps_basename in the reference's text format."""
+
+    def __init__(self, ctx, length, max_snaps, snaps_directory=".", snaps_basename="parmoon_snapshots",
+                 steps_per_snap=5, write_snaps=True):
+        import os
+        self.ctx = ctx
+        self.length = length
+        self.filename = None
+        if write_snaps:
+            os.makedirs(snaps_directory, exist_ok=True)
+            self.filename = os.path.join(snaps_directory, snaps_basename)
+        self.h = c_void_p()
+        call("pmx_snapshots_create", ctx.h, self.filename.encode() if self.filename else None, steps_per_snap,
+             length, max_snaps, ctypes.byref(self.h))
+
+    def write_data(self, solution, time_step=0):
+        import torch
+        assert solution.is_cuda and solution.numel() == self.length
+        torch.cuda.synchronize()   # torch's stream has produced the tensor
+        w = c_int()
+        call("pmx_snapshots_write_data", self.h, _ptr(solution), time_step, ctypes.byref(w))
+        return bool(w.value)
+
+    def count(self):
+        n = c_int()
+        call("pmx_snapshots_get", self.h, ctypes.byref(n), None)
+        return n.value
+
+    def device_matrix(self):
+        """(pointer, n_snaps): column-major length x n_snaps in device memory"""
+        n, d = c_int(), c_void_p()
+        call("pmx_snapshots_get", self.h, ctypes.byref(n), ctypes.byref(d))
+        return d, n.value
+
+    def __del__(self, _d=_destroy):
+        if getattr(self, "h", None) and self.h.value:
+            _d("pmx_snapshots_destroy", self.h)
+            self.h = None
+
+
+class POD:
+    """POD (include/ReducedOrderModels/POD.h): method of snapshots with the
+    Gram matrix on the device.  set_gramian(FEMatrix) for the L2 inner product
+    (TimeConvectionDiffusionPOD::assemble_gramian assembles the mass matrix)."""
+
+    def __init__(self, ctx, pod_rank=0, pod_fluctuations_only=True, pod_inner_product="euclidean"):
+        self.ctx = ctx
+        self.h = c_void_p()
+        call("pmx_pod_create", ctx.h, ctypes.byref(pod_params(pod_rank, pod_fluctuations_only, pod_inner_product)),
+             ctypes.byref(self.h))
+
+    def set_gramian(self, matrix):
+        self._gram = matrix
+        call("pmx_pod_set_gramian", self.h, matrix.h)
+
+    def read_snapshots(self, path):
+        call("pmx_pod_read_snapshots", self.h, path.encode())
+
+    def set_snapshots(self, snaps):
+        """snaps: collector, or a CUDA tensor (n_snaps, length) (row j = snapshot j)"""
+        if isinstance(snaps, SnapshotsCollector):
+            d, m = snaps.device_matrix()
+            call("pmx_pod_set_snapshots", self.h, d, snaps.length, m, snaps.length)
+        else:
+            import torch
+            assert snaps.is_cuda and snaps.is_contiguous()
+            torch.cuda.synchronize()
+            call("pmx_pod_set_snapshots", self.h, _ptr(snaps), snaps.shape[1], snaps.shape[0], snaps.shape[1])
+
+    def compute_basis(self):
+        call("pmx_pod_compute_basis", self.h)
+
+    def info(self):
+        n, m, r, v = c_int(), c_int(), c_int(), c_int()
+        call("pmx_pod_get_info", self.h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(r), ctypes.byref(v))
+        return n.value, m.value, r.value, v.value
+
+    def get_pod_eigs(self):
+        _, m, _, _ = self.info()
+        out = np.zeros(m)
+        call("pmx_pod_get_eigs", self.h, out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out
+
+    def get_basis(self):
+        """CUDA tensor (rank, length): row k = mode k"""
+        import torch
+        n, _, r, _ = self.info()
+        out = torch.zeros((max(r, 1), n), dtype=torch.float64, device="cuda:%d" % self.ctx.device)
+        torch.cuda.synchronize()   # the fill on torch's stream ends before the copy on the context's
+        call("pmx_pod_get_basis", self.h, _ptr(out))
+        return out[:r]
+
+    def get_snaps_avr(self):
+        import torch
+        n, _, _, _ = self.info()
+        out = torch.zeros(n, dtype=torch.float64, device="cuda:%d" % self.ctx.device)
+        torch.cuda.synchronize()
+        call("pmx_pod_get_mean", self.h, _ptr(out))
+        return out
+
+    def write_pod(self, pod_directory, basename):
+        call("pmx_pod_write", self.h, pod_directory.encode(), basename.encode())
+
+    def __del__(self, _d=_destroy):
+        if getattr(self, "h", None) and self.h.value:
+            _d("pmx_pod_destroy", self.h)
             self.h = None
 
 
