@@ -135,11 +135,26 @@ def batched_sweep(pm, ctx, n_problems=256, steps=20, seed=0x5EED):
     bt = pm.BatchTCD(ctx, probs, tau=1e-2)
     t_setup = time.perf_counter() - t0
     bt.initialize()
-    rep = bt.run(steps)
+    # snapshots of every problem (SnapshotsCollector, steps_per_snap 5: the
+    # initial condition and every 5th step), kept in device memory
+    per_snap = 5
+    bt.reserve_snapshots(steps // per_snap + 1)
+    bt.write_snapshot()
+    seconds, reps = 0.0, []
+    for _ in range(steps // per_snap):
+        r = bt.run(per_snap)
+        seconds += r.seconds
+        reps.append(r)
+        bt.write_snapshot()
+    rep = reps[-1]
+    t0 = time.perf_counter()
+    bt.compute_pod(pod_inner_product="L2")
+    t_pod = time.perf_counter() - t0
     inf = bt.info()
     inner, outer, failed = bt.iterations()
     return {"problems": n_problems, "steps": steps, "total_dof": inf.total_dof, "total_nnz": inf.total_nnz,
-            "seconds": round(rep.seconds, 3), "mdof_steps_per_s": round(inf.total_dof * steps / rep.seconds / 1e6, 2),
+            "seconds": round(seconds, 3), "mdof_steps_per_s": round(inf.total_dof * steps / seconds / 1e6, 2),
+            "pod_l2_all_problems_ms": round(t_pod * 1e3, 2), "snapshots_per_problem": bt.n_snaps,
             "inner_iterations_total": int(rep.total_inner_iterations),
             "inner_iterations_max_problem": int(rep.max_inner_iterations),
             "failed_solves": int(rep.failed_solves), "block": int(rep.block),

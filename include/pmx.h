@@ -403,6 +403,16 @@ int pmx_batch_get_matrices(const pmx_batch* b, int problem, double* host_stiffne
 /* L2, H1-semi, SD, L_inf errors of one problem at the current time */
 int pmx_batch_errors(pmx_batch* b, int problem, double* host_errors4);
 
+/* ------------------------------------------------------------- output */
+/* DataWriter<d>::writeVtk (src/FE/DataWriter.C:232-340): legacy ASCII VTK of
+ * the space's mesh level with one SCALARS block per FE function (values at
+ * the mesh vertices, scientific with 12 digits; coordinates with 8
+ * significant digits).  Scalar Lagrange spaces.  The points are numbered in
+ * order of first appearance in the cell order (the reference sorts its vertex
+ * objects by address).  Reads the functions from device memory. */
+int pmx_write_vtk(const pmx_space* s, int n_functions, const double* const* dev_u, const char* const* names,
+                  double time, const char* path);
+
 /* -------------------------------------------------- snapshots and POD */
 /* SnapshotsCollector (include/ReducedOrderModels/SnapshotsCollector.h:22-80,
  * src/ReducedOrderModels/SnapshotsCollector.C:36-95).  The snapshots stay in

@@ -169,6 +169,7 @@ SIGNATURES = {
     "pmx_nse_project_pressure": [c_void_p, c_void_p, c_void_p],
     "pmx_nse_restrict_velocity": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "pmx_nse_errors": [c_void_p, c_void_p, c_int, c_void_p, P(c_double)],
+    "pmx_write_vtk": [c_void_p, c_int, P(c_void_p), P(ctypes.c_char_p), c_double, ctypes.c_char_p],
     "pmx_snapshots_create": [c_void_p, ctypes.c_char_p, c_int, c_int, c_int, P(c_void_p)],
     "pmx_snapshots_destroy": [c_void_p],
     "pmx_snapshots_write_data": [c_void_p, c_void_p, c_int, P(c_int)],

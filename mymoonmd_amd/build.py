@@ -24,6 +24,7 @@ SOURCES = [
     "batch.cpp",
     "nse.cpp",
     "pod.cpp",
+    "io.cpp",
     "hip/kernels_la.hip",
     "hip/kernels_asm.hip",
     "hip/kernels_batch.hip",

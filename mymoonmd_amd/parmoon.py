@@ -720,6 +720,19 @@ class BatchTCD:
             self.h = None
 
 
+def write_vtk(space, functions, path, time=0.0):
+    """DataWriter::writeVtk for scalar FE functions: functions = {name: CUDA
+    tensor of the space's DOF values}"""
+    import torch
+    names = list(functions)
+    ptrs = (c_void_p * max(1, len(names)))(*[functions[k].data_ptr() for k in names])
+    cnames = (ctypes.c_char_p * max(1, len(names)))(*[k.encode() for k in names])
+    for k in names:
+        assert functions[k].is_cuda and functions[k].numel() == space.n_dof
+    torch.cuda.synchronize()
+    call("pmx_write_vtk", space.h, len(names), ptrs, cnames, time, path.encode())
+
+
 POD_INNER_PRODUCT = {"euclidean": 0, "L2": 1}
 
 
