@@ -534,15 +534,20 @@ typedef struct {
                                   active DOFs (the first step after the cycle resets it)
                                   multiplies only the Dirichlet-column entries; the result
                                   is the same bit for bit.  0: always the full sweep */
-  int renumber_levels;         /* no reference counterpart.  1 (default): the multigrid
-                                  levels above the coarsest run in a solver-internal
+  int renumber_levels;         /* no reference counterpart.  1: the multigrid levels above
+                                  the coarsest (and scalar FGMRES) run in a solver-internal
                                   lexicographic DOF order with the entries of each row
-                                  sorted by the new column (better gather locality; the
-                                  row sums of the preconditioner are rounded in another
-                                  order).  2: as 1, and where every column offset fits in
-                                  16 bits the level matrix stores col - row as int16 (same
-                                  results as 1, bit for bit; fewer bytes but slower on
-                                  MI355X, the Dirichlet columns then need a per-row tail).
+                                  sorted by the new column (better gather locality; row
+                                  sums and dot products are rounded in another order).
+                                  2: as 1, and where every active-column offset fits in 16
+                                  bits the level matrix stores col - row as int16, the
+                                  Dirichlet-column entries in a per-row tail (same results
+                                  as 1 bit for bit; slower).  3 (default): as 2 with the
+                                  Dirichlet-column products of each row summed once per
+                                  reset of the level's solution (its Dirichlet part does
+                                  not change between resets) and added first: fewer bytes
+                                  per sweep and no tail; rounding differs from 1.  Levels
+                                  whose offsets do not fit keep 32-bit columns.
                                   0: the reference order */
   int transfer_sell;           /* no reference counterpart.  1 (default): the scalar
                                   multigrid's prolongation and restriction read a SELL-64

@@ -88,14 +88,18 @@ void launch_fe_spmv(const DevCSR& A, const VT* va, const float* rscale, const XT
 // (the summation order of TMatrix::multiply).  Eight entries are loaded per
 // step before their x values are gathered, to keep enough bytes in flight.
 // Blocks beyond the active slices handle the non-active (identity) rows.
-template <int MODE, typename VT, typename XT, bool REL16>
+// FOLD (16-bit layout only): the row's Dirichlet-column products are not
+// summed per launch; gsum[r] holds their sum (in column order, computed when
+// the solution's Dirichlet part was last set) and starts the row sum, so the
+// row is summed Dirichlet columns first, then the active columns in order.
+template <int MODE, typename VT, typename XT, bool REL16, bool FOLD = false>
 __global__ __launch_bounds__(256) void k_sell_spmv(
     int n_slices, int sell_blocks, int n, int active, const int* __restrict__ sptr,
     const int* __restrict__ rowmap, const int* __restrict__ ci, const int16_t* __restrict__ off,
     const int* __restrict__ bd_ptr, const int* __restrict__ bd_col, const VT* __restrict__ bd_val,
     const VT* __restrict__ va, const float* __restrict__ rscale, const XT* __restrict__ x,
     const XT* __restrict__ b, const XT* __restrict__ dinv, XT* __restrict__ y, double a,
-    double* __restrict__ partial) {
+    double* __restrict__ partial, const XT* __restrict__ gsum = nullptr) {
   __shared__ double red[16];
   using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
   double d2 = 0;
@@ -116,6 +120,7 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
       const int rb = r >= 0 ? r : 0;   // 16-bit layout: column = row + offset (empty lanes: offset 0)
       const VT* vp = va + s0 + lane;
       AT acc = 0;
+      if constexpr (FOLD) acc = (AT)gsum[rb];
       int j = 0;
       for (; j + 8 <= width; j += 8) {
         int c[8];
@@ -142,7 +147,7 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
         else v = (AT)__builtin_nontemporal_load(vp + j * 64);
         acc += v * (AT)x[c];
       }
-      if constexpr (REL16) {
+      if constexpr (REL16 && !FOLD) {
         // the Dirichlet-column entries, last in the row's column order
         if (r >= 0) {
           const int e = bd_ptr[r + 1];
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(256) void k_sell_jacobi_from_zero(
     int n, int active, const int* __restrict__ bd_ptr, const int* __restrict__ bd_col,
     const VT* __restrict__ bd_val, const float* __restrict__ rscale,
     const XT* __restrict__ x, const XT* __restrict__ b, const XT* __restrict__ dinv,
-    XT* __restrict__ y, double a) {
+    XT* __restrict__ y, double a, XT* __restrict__ gsum) {
   using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -218,6 +223,7 @@ __global__ __launch_bounds__(256) void k_sell_jacobi_from_zero(
       else v = (AT)bd_val[k];
       acc += v * (AT)x[bd_col[k]];
     }
+    if (gsum) gsum[i] = (XT)acc;   // the Dirichlet-column sum of the folded layout
     if (rscale) acc *= (AT)rscale[i];
     const AT res = (AT)b[i] - acc;
     y[i] = (XT)((AT)a * (AT)dinv[i] * res);
@@ -225,6 +231,27 @@ __global__ __launch_bounds__(256) void k_sell_jacobi_from_zero(
     const AT res = (AT)(b[i] - x[i]);
     y[i] = (XT)((AT)x[i] + (AT)a * (AT)dinv[i] * res);
   }
+}
+
+// gsum[r] = sum of the Dirichlet-column products of active row r, in column
+// order (the folded 16-bit layout; before the row scale)
+template <typename VT, typename XT>
+__global__ __launch_bounds__(256) void k_sell_dirichlet_sums(int active, const int* __restrict__ bd_ptr,
+                                                             const int* __restrict__ bd_col,
+                                                             const VT* __restrict__ bd_val,
+                                                             const XT* __restrict__ x, XT* __restrict__ gsum) {
+  using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= active) return;
+  AT acc = 0;
+  const int e = bd_ptr[i + 1];
+  for (int k = bd_ptr[i]; k < e; ++k) {
+    AT v;
+    if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(bd_val[k]);
+    else v = (AT)bd_val[k];
+    acc += v * (AT)x[bd_col[k]];
+  }
+  gsum[i] = (XT)acc;
 }
 
 int sell_grid(const DevCSR& A, int* sell_blocks) {
@@ -241,7 +268,13 @@ void launch_sell(const DevCSR& A, const VT* va, const VT* bd_val, const float* r
     throw std::runtime_error("16-bit SELL layout without its Dirichlet-column values");
   if (A.sell.n_slices > 0 && (A.sell.total == 0 || !va || (!A.sell.off && !A.sell.col)))
     throw std::runtime_error("SELL copy without slots, values or columns");
-  if (A.sell.off)
+  if (A.sell.off && A.sell.gsum) {
+    if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
+    hipLaunchKernelGGL((k_sell_spmv<MODE, VT, XT, true, true>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb,
+                       A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, A.sell.off, A.sell.bd_ptr,
+                       A.sell.bd_col, bd_val, va, rscale, x, b, dinv, y, a, partial,
+                       (const XT*)A.sell.gsum);
+  } else if (A.sell.off)
     hipLaunchKernelGGL((k_sell_spmv<MODE, VT, XT, true>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb,
                        A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, A.sell.off, A.sell.bd_ptr,
                        A.sell.bd_col, bd_val, va, rscale, x, b, dinv, y, a, partial);
@@ -608,9 +641,29 @@ static bool jacobi_from_zero(const DevCSR& A, const VT* bd_val, const float* rsc
                              const XT* b, const XT* dinv, XT* xn, double omega, hipStream_t st) {
   if (A.lpr != 0 || !A.sell.bd_ptr || (!bd_val && A.sell.n_bd > 0)) return false;
   if (A.n == 0) return true;
+  XT* g = nullptr;
+  if (A.sell.off && A.sell.gsum) {
+    if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
+    g = (XT*)A.sell.gsum;
+  }
   hipLaunchKernelGGL((k_sell_jacobi_from_zero<VT, XT>), dim3((A.n + 255) / 256), dim3(256), 0, st, A.n,
-                     A.active, A.sell.bd_ptr, A.sell.bd_col, bd_val, rscale, x, b, dinv, xn, omega);
+                     A.active, A.sell.bd_ptr, A.sell.bd_col, bd_val, rscale, x, b, dinv, xn, omega, g);
   return true;
+}
+
+template <typename VT, typename XT>
+static void dirichlet_sums(const DevCSR& A, const VT* bd_val, const XT* x, hipStream_t st) {
+  if (!(A.sell.off && A.sell.gsum) || A.active <= 0) return;
+  if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
+  if (A.sell.n_bd > 0 && (!bd_val || !A.sell.bd_ptr)) throw std::runtime_error("folded SELL layout without Dirichlet values");
+  hipLaunchKernelGGL((k_sell_dirichlet_sums<VT, XT>), dim3((A.active + 255) / 256), dim3(256), 0, st, A.active,
+                     A.sell.bd_ptr, A.sell.bd_col, bd_val, x, (XT*)A.sell.gsum);
+}
+void sell_dirichlet_sums(const DevCSR& A, const double* x, hipStream_t st) {
+  dirichlet_sums<double, double>(A, A.sell.bd_val, x, st);
+}
+void sell_dirichlet_sums_bf16(const DevCSR& A, const float* x, hipStream_t st) {
+  dirichlet_sums<uint16_t, float>(A, A.sell.bd_val_bf16, x, st);
 }
 bool fe_jacobi_from_zero(const DevCSR& A, const double* x, const double* b, const double* dinv, double* xn,
                          double omega, hipStream_t st) {

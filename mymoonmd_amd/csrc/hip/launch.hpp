@@ -29,6 +29,12 @@ struct DevSELL {
   // 16-bit layout: slots hold col - row of the active-column entries (col and
   // the Dirichlet-column entries are then only in the bd lists)
   int16_t* off = nullptr;
+  // folded 16-bit layout: per active row the sum of its Dirichlet-column
+  // products (FP64 or FP32 as the vectors, gsum_bytes = 8 or 4), owned by the
+  // user of the layout and refreshed with sell_dirichlet_sums whenever the
+  // Dirichlet part of the vector the kernels read changes
+  void* gsum = nullptr;
+  int gsum_bytes = 0;
   // grid-transfer matrices: the SELL copy (slice_ptr, row, col, val and, when
   // the weights are bf16 numbers, val_bf16) covers rows [0, n_rows)
   int n_rows = 0;
@@ -77,6 +83,9 @@ void fe_defect_bf16(const DevCSR& A, const float* x, const float* b, float* d, d
                     double* norm_out, hipStream_t st);
 void fe_jacobi_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv, float* xn,
                     double omega, hipStream_t st);
+// folded 16-bit layout: refresh the Dirichlet-column sums for vector x
+void sell_dirichlet_sums(const DevCSR& A, const double* x, hipStream_t st);
+void sell_dirichlet_sums_bf16(const DevCSR& A, const float* x, hipStream_t st);
 bool fe_jacobi_from_zero_bf16(const DevCSR& A, const float* x, const float* b, const float* dinv, float* xn,
                               double omega, hipStream_t st);
 // ---- plain CSR (transfer matrices): y = alpha*P x + beta*y on rows < row_end;

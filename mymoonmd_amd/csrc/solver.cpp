@@ -187,13 +187,18 @@ struct Level {
   double* bd64 = nullptr;    // values of A's Dirichlet-column slots (zero-start sweep), FP64 ...
   uint16_t* bd16 = nullptr;  // ... or bf16
   bool own_sell = false;     // A.sell is this level's renumbered copy
+  // folded 16-bit layout: Dirichlet-column sums of the current solution
+  // (valid until the level's solution is reset)
+  double* g64 = nullptr;
+  float* g32 = nullptr;
+  bool g_valid = false;
   void release() {
     if (own_sell) {
       dfree(A.sell.slice_ptr); dfree(A.sell.row); dfree(A.sell.col); dfree(A.sell.src); dfree(A.sell.val);
       dfree(A.sell.bd_ptr); dfree(A.sell.bd_src); dfree(A.sell.bd_col); dfree(A.sell.off);
       own_sell = false;
     }
-    dfree(perm); dfree(bd64); dfree(bd16);
+    dfree(perm); dfree(bd64); dfree(bd16); dfree(g64); dfree(g32);
     dfree(dinv); dfree(dinv32);
     free_csr(P); free_csr(R); dfree(dinv_dense);
     dfree(rhs); dfree(sol); dfree(def); dfree(tmp);
@@ -223,6 +228,7 @@ struct pmx_solver {
   double* kval = nullptr;        // FP64 SELL values of K (mixed precision only)
   double* kbd = nullptr;         // FP64 Dirichlet-column values of K (mixed only)
   double *kb = nullptr, *kx = nullptr;   // b and x in that numbering
+  bool kown = false;             // K.sell is FGMRES's own 32-bit copy (16-bit levels)
   const pmx::DevCSR* op = nullptr;       // operator of the running solve
   bool internal = false;                 // the running solve uses that numbering
   int n = 0;
@@ -246,6 +252,11 @@ struct pmx_solver {
     V.clear(); Z.clear();
     dfree(r); dfree(w1); dfree(w2); dfree(w3); dfree(dinv); dfree(d_H); dfree(d_scal); dfree(partial);
     dfree(kval); dfree(kbd); dfree(kb); dfree(kx);
+    if (kown) {
+      dfree(K.sell.slice_ptr); dfree(K.sell.row); dfree(K.sell.col); dfree(K.sell.src);
+      dfree(K.sell.bd_ptr); dfree(K.sell.bd_src); dfree(K.sell.bd_col);
+      kown = false;
+    }
     kperm = nullptr;
     K = pmx::DevCSR{};
     updated = false;
@@ -284,6 +295,18 @@ bool converged(pmx_solver* s, double res, int it) {
 }
 
 // ---------------------------------------------------------------- multigrid
+// Folded 16-bit levels: the sums of the Dirichlet-column products of the
+// current solution.  The Dirichlet part of a level's solution is set when the
+// cycle resets it and is left unchanged by the smoothing steps (unit rows:
+// x + w (b - x) = b when x = b) and by the coarse-grid correction (active rows
+// only), so the sums are computed once per reset.
+void ensure_dirichlet_sums(pmx_solver* s, Level& L, bool mixed) {
+  if (L.g_valid || !L.A.sell.gsum) return;
+  if (mixed) pmx::sell_dirichlet_sums_bf16(L.A, L.sol32, S(s));
+  else pmx::sell_dirichlet_sums(L.A, L.sol, S(s));
+  L.g_valid = true;
+}
+
 void mg_smooth(pmx_solver* s, int l, bool mixed) {
   Level& L = s->lv[l];
   if (l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) {
@@ -295,19 +318,28 @@ void mg_smooth(pmx_solver* s, int l, bool mixed) {
   const bool zero = L.sol_zero && s->p.zero_start_shortcut;
   L.sol_zero = false;
   if (mixed) {
-    if (!(zero && pmx::fe_jacobi_from_zero_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp,
-                                                S(s))))
+    // the sweep from zero also stores the Dirichlet-column sums of a folded level
+    if (zero && pmx::fe_jacobi_from_zero_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp, S(s))) {
+      L.g_valid = true;
+    } else {
+      ensure_dirichlet_sums(s, L, mixed);
       pmx::fe_jacobi_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp, S(s));
+    }
     std::swap(L.sol32, L.tmp32);
   } else {
-    if (!(zero && pmx::fe_jacobi_from_zero(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s))))
+    if (zero && pmx::fe_jacobi_from_zero(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s))) {
+      L.g_valid = true;
+    } else {
+      ensure_dirichlet_sums(s, L, mixed);
       pmx::fe_jacobi(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, S(s));
+    }
     std::swap(L.sol, L.tmp);
   }
 }
 
 void mg_defect(pmx_solver* s, int l, bool mixed, double* norm_out) {
   Level& L = s->lv[l];
+  ensure_dirichlet_sums(s, L, mixed);
   if (mixed) pmx::fe_defect_bf16(L.A, L.sol32, L.rhs32, L.def32, L.partial, norm_out, S(s));
   else pmx::fe_defect(L.A, L.sol, L.rhs, L.def, L.partial, norm_out, S(s));
 }
@@ -360,6 +392,7 @@ int mg_cycle_step(pmx_solver* s, int step, int level, bool mixed) {
     if (mixed) pmx::vec_fill_f32(C.n, 0.f, C.sol32, S(s));
     else pmx::vec_fill(C.n, 0.0, C.sol, S(s));
     C.sol_zero = true;
+    C.g_valid = false;
     mg_update_rhs_in_coarser_grid(s, level, mixed);
     return level - 1;
   } else if (up) {
@@ -384,6 +417,7 @@ void mg_cycle(pmx_solver* s, bool mixed) {
                                hipMemcpyDeviceToDevice, S(s)));
   }
   F.sol_zero = true;
+  F.g_valid = false;
   int level = (int)s->lv.size() - 1;
   const int n_steps = (int)s->cc.size() - 1;
   for (int step = 1; step <= n_steps; ++step) level = mg_cycle_step(s, step, level, mixed);
@@ -849,7 +883,7 @@ int pmx_solver_default_params(pmx_solver_params* p) {
     d.precision = PMX_PRECISION_FP64;
     d.vanka_damp = 1.0;
     d.zero_start_shortcut = 1;
-    d.renumber_levels = 1;
+    d.renumber_levels = 3;
     d.transfer_sell = 1;
     *p = d;
   });
@@ -927,6 +961,8 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
       // solver-internal numbering of the levels above the coarsest (which
       // keeps the reference order for its dense inverse)
       std::vector<std::vector<int>> perm(n_levels), pinv(n_levels);
+      pmx::SellPattern kpat;
+      bool have_kpat = false;
       for (int l = 1; l < n_levels; ++l)
         if (s->p.renumber_levels && levels[l] && levels[l]->A.sell.total && spaces[l]) {
           perm[l] = pmx::lexicographic_order(spaces[l]->fs);
@@ -966,6 +1002,15 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
           for (auto& k : sp.src)
             if (k >= 0) k = opos[k];
           for (auto& k : sp.bd_src) k = opos[k];
+          if (l == n_levels - 1 && !sp.off.empty() && s->p.iterative_solver_type == PMX_IT_FGMRES) {
+            // FGMRES applies the finest operator to vectors whose Dirichlet
+            // part changes every time: it keeps a 32-bit copy of the layout
+            kpat = pmx::build_sell(pp, kSellSigmaSolver, false);
+            for (auto& k : kpat.src)
+              if (k >= 0) k = opos[k];
+            for (auto& k : kpat.bd_src) k = opos[k];
+            have_kpat = true;
+          }
           pmx::DevCSR Ap = L.A;
           Ap.lpr = 0;
           Ap.sell = upload_sell(sp, st);
@@ -1010,6 +1055,11 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
             L.A.sell.bd_val = L.bd64;
           }
         }
+        if (s->p.renumber_levels >= 3 && L.A.sell.off && L.own_sell) {
+          // folded 16-bit layout: Dirichlet-column sums per reset, no per-row tail
+          if (mixed) { L.g32 = dzeros<float>(std::max(1, L.active), st); L.A.sell.gsum = L.g32; L.A.sell.gsum_bytes = 4; }
+          else { L.g64 = dzeros<double>(std::max(1, L.active), st); L.A.sell.gsum = L.g64; L.A.sell.gsum_bytes = 8; }
+        }
         if (mixed) {
           L.rhs32 = dzeros<float>(L.n, st); L.sol32 = dzeros<float>(L.n, st);
           L.def32 = dzeros<float>(L.n, st); L.tmp32 = dzeros<float>(L.n, st);
@@ -1051,7 +1101,20 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
         s->K.rscale = nullptr;
         s->K.sell.val_bf16 = nullptr;
         s->K.sell.bd_val_bf16 = nullptr;
-        if (mixed) {
+        s->K.sell.gsum = nullptr;
+        s->K.sell.gsum_bytes = 0;
+        if (have_kpat) {
+          s->K.sell = upload_sell(kpat, st);
+          s->kown = true;
+          s->kval = dmalloc<double>(s->K.sell.total);
+          pmx::sell_fill_f64(s->K, s->kval, st);
+          s->K.sell.val = s->kval;
+          if (s->K.sell.n_bd > 0) {
+            s->kbd = dmalloc<double>(s->K.sell.n_bd);
+            pmx::sell_dirichlet_fill(s->K, s->kbd, nullptr, st);
+          }
+          s->K.sell.bd_val = s->kbd;
+        } else if (mixed) {
           // the multigrid keeps only bf16 values: an FP64 copy for FGMRES
           s->kval = dmalloc<double>(s->K.sell.total);
           pmx::sell_fill_f64(s->K, s->kval, st);
@@ -1277,6 +1340,7 @@ int pmx_solver_time_level_kernel(pmx_solver* s, int level, int kernel, int reps,
     Level& L = s->lv[level];
     hipStream_t st = S(s);
     auto launch = [&]() {
+      ensure_dirichlet_sums(s, L, mixed);
       if (mixed) pmx::fe_jacobi_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp, st);
       else pmx::fe_jacobi(L.A, L.sol, L.rhs, L.dinv, L.tmp, s->p.jacobi_damp, st);
     };

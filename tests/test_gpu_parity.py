@@ -277,12 +277,14 @@ def test_zero_start_sweep_and_renumbering(ctx, precision, cycle):
     lexicographic DOF order with row entries sorted by the new column: only
     the rounding of the row sums changes (1e-12 FP64, 1e-5 with FP32/bf16),
     and the solve takes the same number of iterations; storing those levels'
-    columns as 16-bit offsets (renumber_levels 2) changes nothing."""
+    columns as 16-bit offsets (renumber_levels 2) changes nothing; folding
+    the Dirichlet-column products into per-reset sums (renumber_levels 3)
+    sums them first instead of last (rounding only)."""
     nref = 3
     d, spaces, mats, rhs = three_level_cd3d(ctx, nref, MG_KW)
     fs = spaces[-1]
     out = {}
-    for shortcut, renumber in ((0, 0), (1, 0), (0, 1), (1, 1), (0, 2), (1, 2)):
+    for shortcut, renumber in ((0, 0), (1, 0), (0, 1), (1, 1), (0, 2), (1, 2), (0, 3), (1, 3)):
         for coarse in ("jacobi", "direct_solve"):
             prm = pm.solver_params(iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
                                    residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=100,
@@ -300,15 +302,16 @@ def test_zero_start_sweep_and_renumbering(ctx, precision, cycle):
             out[(shortcut, renumber, coarse)] = (z.cpu().numpy(), x.cpu().numpy(), rep.iterations)
     tol = 1e-12 if precision == "fp64" else 1e-5
     for coarse in ("jacobi", "direct_solve"):
-        for renumber in (0, 1, 2):
+        for renumber in (0, 1, 2, 3):
             a, b = out[(0, renumber, coarse)], out[(1, renumber, coarse)]
             assert np.array_equal(a[0], b[0]), (renumber, coarse)
             assert np.array_equal(a[1], b[1]) and a[2] == b[2], (renumber, coarse)
         a, b = out[(1, 1, coarse)], out[(1, 2, coarse)]
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2], coarse
-        a, b = out[(1, 0, coarse)], out[(1, 1, coarse)]
-        assert np.max(np.abs(a[0] - b[0])) <= tol * np.max(np.abs(a[0])), coarse
-        assert abs(a[2] - b[2]) <= 1 and np.max(np.abs(a[1] - b[1])) <= 1e-7 * np.max(np.abs(a[1])), coarse
+        for r0, r1 in ((0, 1), (1, 3)):
+            a, b = out[(1, r0, coarse)], out[(1, r1, coarse)]
+            assert np.max(np.abs(a[0] - b[0])) <= tol * np.max(np.abs(a[0])), (coarse, r0, r1)
+            assert abs(a[2] - b[2]) <= 1 and np.max(np.abs(a[1] - b[1])) <= 1e-7 * np.max(np.abs(a[1])), (coarse, r0, r1)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
