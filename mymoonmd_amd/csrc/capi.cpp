@@ -39,6 +39,8 @@ pmx::DevSELL upload_sell(const pmx::SellPattern& sp, hipStream_t st) {
   S.bd_ptr = upload(sp.bd_ptr.data(), sp.bd_ptr.size(), st);
   S.bd_src = sp.bd_src.empty() ? nullptr : upload(sp.bd_src.data(), sp.bd_src.size(), st);
   S.bd_col = sp.bd_col.empty() ? nullptr : upload(sp.bd_col.data(), sp.bd_col.size(), st);
+  S.pack = sp.pack;
+  if (S.pack != 1 && (S.pack != 4 || !S.off)) throw PmxError(PMX_E_ARG, "packed SELL slots need the 16-bit layout");
   return S;
 }
 

@@ -92,7 +92,12 @@ void launch_fe_spmv(const DevCSR& A, const VT* va, const float* rscale, const XT
 // summed per launch; gsum[r] holds their sum (in column order, computed when
 // the solution's Dirichlet part was last set) and starts the row sum, so the
 // row is summed Dirichlet columns first, then the active columns in order.
-template <int MODE, typename VT, typename XT, bool REL16, bool FOLD = false>
+// PACK 4 (folded layout only): slot j of lane l sits at s0 + (j/4)*256 +
+// 4l + j%4, so a lane reads four offsets (8 bytes) and four values (8 bytes
+// bf16, 32 bytes FP64) of its row per load instruction; the summation order
+// is unchanged.
+typedef double v2d_sell __attribute__((ext_vector_type(2)));
+template <int MODE, typename VT, typename XT, bool REL16, bool FOLD = false, int PACK = 1>
 __global__ __launch_bounds__(256) void k_sell_spmv(
     int n_slices, int sell_blocks, int n, int active, const int* __restrict__ sptr,
     const int* __restrict__ rowmap, const int* __restrict__ ci, const int16_t* __restrict__ off,
@@ -122,7 +127,45 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
       AT acc = 0;
       if constexpr (FOLD) acc = (AT)gsum[rb];
       int j = 0;
-      for (; j + 8 <= width; j += 8) {
+      if constexpr (PACK == 4) {
+        const uint64_t* o4 = (const uint64_t*)(off + s0) + lane;
+        auto unpack = [&](uint64_t o, int* c) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) c[t] = rb + (int)(int16_t)(uint16_t)(o >> (16 * t));
+        };
+        auto values = [&](int q, AT* v) {   // the four values of group q (slots 4q .. 4q+3)
+          if constexpr (sizeof(VT) == 2) {
+            const uint64_t w = __builtin_nontemporal_load((const uint64_t*)(va + s0) + (size_t)q * 64 + lane);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] = (AT)bf16_to_float((uint16_t)(w >> (16 * t)));
+          } else {
+            const v2d_sell* p2 = (const v2d_sell*)(va + s0 + (size_t)q * 256 + 4 * lane);
+            const v2d_sell a0 = __builtin_nontemporal_load(p2), a1 = __builtin_nontemporal_load(p2 + 1);
+            v[0] = (AT)a0.x; v[1] = (AT)a0.y; v[2] = (AT)a1.x; v[3] = (AT)a1.y;
+          }
+        };
+        for (; j + 8 <= width; j += 8) {
+          const int q = j >> 2;
+          int c[8];
+          AT v[8];
+          unpack(__builtin_nontemporal_load(o4 + (size_t)q * 64), c);
+          unpack(__builtin_nontemporal_load(o4 + (size_t)(q + 1) * 64), c + 4);
+          values(q, v);
+          values(q + 1, v + 4);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc += v[u] * (AT)x[c[u]];
+        }
+        for (; j < width; j += 4) {
+          const int q = j >> 2;
+          int c[4];
+          AT v[4];
+          unpack(__builtin_nontemporal_load(o4 + (size_t)q * 64), c);
+          values(q, v);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc += v[u] * (AT)x[c[u]];
+        }
+      }
+      for (; PACK == 1 && j + 8 <= width; j += 8) {
         int c[8];
         AT v[8];
 #pragma unroll
@@ -138,7 +181,7 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc += v[u] * (AT)x[c[u]];
       }
-      for (; j < width; ++j) {
+      for (; PACK == 1 && j < width; ++j) {
         int c;
         if constexpr (REL16) c = rb + (int)__builtin_nontemporal_load(op + j * 64);
         else c = __builtin_nontemporal_load(cp + j * 64);
@@ -268,7 +311,14 @@ void launch_sell(const DevCSR& A, const VT* va, const VT* bd_val, const float* r
     throw std::runtime_error("16-bit SELL layout without its Dirichlet-column values");
   if (A.sell.n_slices > 0 && (A.sell.total == 0 || !va || (!A.sell.off && !A.sell.col)))
     throw std::runtime_error("SELL copy without slots, values or columns");
-  if (A.sell.off && A.sell.gsum) {
+  if (A.sell.pack != 1 && !(A.sell.pack == 4 && A.sell.off && A.sell.gsum))
+    throw std::runtime_error("packed SELL slots need the folded 16-bit layout");
+  if (A.sell.pack == 4) {
+    if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
+    hipLaunchKernelGGL((k_sell_spmv<MODE, VT, XT, true, true, 4>), dim3(grid), dim3(256), 0, st, A.sell.n_slices,
+                       sb, A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, A.sell.off, A.sell.bd_ptr,
+                       A.sell.bd_col, bd_val, va, rscale, x, b, dinv, y, a, partial, (const XT*)A.sell.gsum);
+  } else if (A.sell.off && A.sell.gsum) {
     if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
     hipLaunchKernelGGL((k_sell_spmv<MODE, VT, XT, true, true>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb,
                        A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, A.sell.off, A.sell.bd_ptr,
@@ -299,7 +349,7 @@ __global__ __launch_bounds__(256) void k_sell_fill_bf16(int n_slices, const int*
                                                         const int* __restrict__ src,
                                                         const double* __restrict__ v,
                                                         const float* __restrict__ rscale,
-                                                        uint16_t* __restrict__ out) {
+                                                        uint16_t* __restrict__ out, int pack) {
   const int lane = threadIdx.x & 63;
   const int slice = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (slice >= n_slices) return;
@@ -307,7 +357,7 @@ __global__ __launch_bounds__(256) void k_sell_fill_bf16(int n_slices, const int*
   const double inv = (r >= 0) ? 1.0 / (double)rscale[r] : 0.0;
   const int s0 = sptr[slice], width = (sptr[slice + 1] - s0) >> 6;
   for (int j = 0; j < width; ++j) {
-    const int k = s0 + j * 64 + lane;
+    const int k = s0 + (j / pack) * 64 * pack + lane * pack + j % pack;
     const int s = src[k];
     const float f = (s >= 0) ? (float)(v[s] * inv) : 0.0f;
     uint32_t u = __float_as_uint(f);
@@ -866,7 +916,7 @@ void sell_fill_f64(const DevCSR& A, double* sell_val, hipStream_t st) {
 void sell_fill_bf16(const DevCSR& A, uint16_t* sell_bf16, hipStream_t st) {
   if (A.sell.n_slices)
     hipLaunchKernelGGL(k_sell_fill_bf16, dim3((A.sell.n_slices + 3) / 4), dim3(256), 0, st, A.sell.n_slices,
-                       A.sell.slice_ptr, A.sell.row, A.sell.src, A.val, A.rscale, sell_bf16);
+                       A.sell.slice_ptr, A.sell.row, A.sell.src, A.val, A.rscale, sell_bf16, A.sell.pack);
 }
 void csr_to_bf16(const DevCSR& A, uint16_t* vb, float* rscale, hipStream_t st) {
   hipLaunchKernelGGL(k_to_bf16_rows, dim3(grid_for(A.n)), dim3(256), 0, st, A.n, A.row_ptr, A.val, vb,

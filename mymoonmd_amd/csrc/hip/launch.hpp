@@ -35,6 +35,7 @@ struct DevSELL {
   // Dirichlet part of the vector the kernels read changes
   void* gsum = nullptr;
   int gsum_bytes = 0;
+  int pack = 1;                   // packed slots (see SellPattern::pack)
   // grid-transfer matrices: the SELL copy (slice_ptr, row, col, val and, when
   // the weights are bf16 numbers, val_bf16) covers rows [0, n_rows)
   int n_rows = 0;

@@ -449,9 +449,11 @@ void color_cells(const FESpace& s, std::vector<int>& order, std::vector<int>& co
 
 namespace pmx {
 
-SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16) {
+SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16, int pack) {
   constexpr int C = 64;
   SellPattern s;
+  if (!rel16 || pack != 4) pack = 1;
+  s.pack = pack;
   const int na = p.active_bound;
   s.n_active = na;
   s.n_slices = (na + C - 1) / C;
@@ -471,7 +473,7 @@ SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16) {
         if (c >= na) { fits = fits && (q - p.row_ptr[r]) >= nact[r]; continue; }   // Dirichlet entries last
         if ((q - p.row_ptr[r]) >= nact[r] || c - r > 32767 || c - r < -32768) { fits = false; break; }
       }
-    if (!fits) return build_sell(p, sigma, false);
+    if (!fits) return build_sell(p, sigma, false, 1);
   }
   auto len = [&](int r) { return rel16 ? nact[r] : p.row_ptr[r + 1] - p.row_ptr[r]; };
   std::vector<int> order(na);
@@ -491,6 +493,7 @@ SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16) {
         width = std::max(width, len(order[k]));
       }
     }
+    width = (width + pack - 1) / pack * pack;
     s.slice_ptr[sl + 1] = s.slice_ptr[sl] + width * C;
   }
   const size_t total = s.slice_ptr.back();
@@ -503,7 +506,7 @@ SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16) {
     for (int l = 0; l < C; ++l) {
       const int r = s.row[(size_t)sl * C + l];
       for (int j = 0; j < width; ++j) {
-        const size_t pos = (size_t)s.slice_ptr[sl] + (size_t)j * C + l;
+        const size_t pos = sell_pos((size_t)s.slice_ptr[sl], j, l, pack);
         if (r >= 0 && j < len(r)) {
           const int c = p.col[p.row_ptr[r] + j];
           if (rel16) s.off[pos] = (int16_t)(c - r);

@@ -134,9 +134,18 @@ struct SellPattern {
   // offsets col - row (col empty); the Dirichlet-column entries (the last ones
   // of every row) live only in the bd lists
   std::vector<int16_t> off;
+  // packed slots (rel16 only, pack 1 or 4): slot j of lane l of a slice lies at
+  // slice_ptr + (j / pack) * 64 * pack + l * pack + j % pack, so one lane
+  // loads `pack` consecutive entries of its row with one instruction; the
+  // slice width is a multiple of pack
+  int pack = 1;
 };
+// position of slot j of lane l in a slice starting at s0
+inline size_t sell_pos(size_t s0, int j, int l, int pack) {
+  return s0 + (size_t)(j / pack) * 64 * pack + (size_t)l * pack + (size_t)(j % pack);
+}
 // rel16: build the 16-bit offset layout; if an offset does not fit, the
 // result has an empty `off` and the 32-bit layout
-SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16 = false);
+SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16 = false, int pack = 1);
 
 }  // namespace pmx

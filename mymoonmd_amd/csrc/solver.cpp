@@ -998,7 +998,8 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
           pmx::CSRPattern pp;
           std::vector<int> opos;
           pmx::permute_csr(*M->pattern, nullptr, perm[l], pinv[l], pp, nullptr, &opos, true);
-          pmx::SellPattern sp = pmx::build_sell(pp, kSellSigmaSolver, s->p.renumber_levels >= 2);
+          pmx::SellPattern sp = pmx::build_sell(pp, kSellSigmaSolver, s->p.renumber_levels >= 2,
+                                                s->p.renumber_levels >= 4 ? 4 : 1);
           for (auto& k : sp.src)
             if (k >= 0) k = opos[k];
           for (auto& k : sp.bd_src) k = opos[k];

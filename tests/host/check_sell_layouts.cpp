@@ -1,7 +1,9 @@
 // CPU check of the SELL layouts of the multigrid levels (built by the host code
 // of the library):
-// 32-bit and 16-bit layouts describe the same rows; every index a kernel
-// would form stays inside its array.
+// 32-bit, 16-bit and packed 16-bit layouts describe the same rows; every
+// index a kernel would form stays inside its array, and a packed slice's
+// four-slot groups start on 8-byte (16-bit arrays) / 32-byte (FP64 values)
+// boundaries.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -15,12 +17,16 @@ void check_level(const FESpace& fs) {
   for (size_t i = 0; i < perm.size(); ++i) pinv[perm[i]] = (int)i;
   CSRPattern pp; std::vector<int> opos;
   permute_csr(p, nullptr, perm, pinv, pp, nullptr, &opos, true);
-  SellPattern a = build_sell(pp, 4096, false), b = build_sell(pp, 4096, true);
+  SellPattern a = build_sell(pp, 4096, false), b = build_sell(pp, 4096, true), c4 = build_sell(pp, 4096, true, 4);
   const int na = pp.active_bound, n = pp.n_rows;
   std::printf("n %d active %d nnz %zu slots32 %zu slots16 %zu rel16 %d n_bd %zu\n", n, na, pp.col.size(), a.src.size(),
               b.src.size(), (int)!b.off.empty(), b.bd_src.size());
-  for (const SellPattern* s : {&a, &b}) {
+  CHECK(c4.off.empty() == b.off.empty());
+  for (const SellPattern* s : {&a, &b, &c4}) {
     const bool rel = !s->off.empty();
+    if (s->pack == 4) {
+      for (int sl = 0; sl <= s->n_slices; ++sl) CHECK(s->slice_ptr[sl] % 256 == 0);
+    }
     const size_t total = s->src.size();
     CHECK((size_t)s->slice_ptr.back() == total);
     CHECK(rel ? s->col.empty() && s->off.size() == total : s->col.size() == total);
@@ -33,7 +39,8 @@ void check_level(const FESpace& fs) {
         CHECK(r >= -1 && r < na);
         const int rb = r >= 0 ? r : 0;
         for (int j = 0; j < width; ++j) {
-          const size_t k = (size_t)s0 + (size_t)j * 64 + l;
+          const size_t k = sell_pos((size_t)s0, j, l, s->pack);
+          if (s->pack == 4 && j % 4 == 0) CHECK(k % 4 == 0);
           CHECK(k < total);
           const int c = rel ? rb + s->off[k] : s->col[k];
           CHECK(c >= 0 && c < n);

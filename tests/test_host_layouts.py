@@ -1,6 +1,6 @@
 """Host code of the solver set-up, compiled for the CPU with g++: the SELL
-layouts of the renumbered multigrid levels (32-bit columns, and 16-bit
-offsets + Dirichlet lists) hold exactly the rows of the permuted CSR pattern,
+layouts of the renumbered multigrid levels (32-bit columns, 16-bit
+offsets + Dirichlet lists, and the same with slots packed by four) hold exactly the rows of the permuted CSR pattern,
 and every index a SELL kernel forms from them stays inside its array
 (slot < number of slots, column < rows) on Q1/Q2 quadrilaterals, P1
 triangles and Q2 hexahedra."""
