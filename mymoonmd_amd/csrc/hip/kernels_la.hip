@@ -165,6 +165,18 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
           for (int u = 0; u < 4; ++u) acc += v[u] * (AT)x[c[u]];
         }
       }
+      // the folded bf16 sweep keeps sixteen entries in flight per lane
+      constexpr bool wide = REL16 && FOLD && PACK == 1 && sizeof(VT) == 2;
+      for (; wide && j + 16 <= width; j += 16) {
+        int c[16];
+        AT v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) c[u] = rb + (int)__builtin_nontemporal_load(op + (j + u) * 64);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = (AT)bf16_to_float(__builtin_nontemporal_load(vp + (j + u) * 64));
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc += v[u] * (AT)x[c[u]];
+      }
       for (; PACK == 1 && j + 8 <= width; j += 8) {
         int c[8];
         AT v[8];
