@@ -166,8 +166,10 @@ def batched_sweep(pm, ctx, n_problems=256, steps=20, seed=0x5EED):
             "inner_iterations_total": int(rep.total_inner_iterations),
             "inner_iterations_max_problem": int(rep.max_inner_iterations),
             "failed_solves": int(rep.failed_solves), "block": int(rep.block),
+            "solve_workgroups": int(inf.n_workgroups),
             "host_setup_s": round(t_setup, 2),
-            "method": "per-problem persistent workgroup: FP64 residual refinement around FP32 BiCGStab + Jacobi"}
+            "method": "persistent launch, one workgroup per problem (several for the costliest): FP64 "
+                      "residual refinement around FP32 BiCGStab + Jacobi"}
 
 
 def fourier_field(x, seed=0x5EED, n_modes=8):

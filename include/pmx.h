@@ -365,12 +365,17 @@ typedef struct {
   double residual_reduction; /* FP64 residual reduction per time step */
   double residual_tolerance; /* FP64 absolute residual per time step */
   int block;                 /* threads per problem: 256, 512, 1024, or 0 = tune on the first run */
+  int max_groups;            /* BiCGStab: workgroups one problem may get (1..16; 1 = one per problem).
+                                The first run measures each problem's iterations and gives the
+                                costliest problems more workgroups, as long as the whole grid
+                                fits on the device at once */
 } pmx_batch_params;
 typedef struct {
   int n_problems, total_dof, total_active, total_cells;
   long long total_nnz;
   int block;
   double time;
+  int n_workgroups;   /* of the solve kernel (> n_problems when problems share several) */
 } pmx_batch_info;
 typedef struct {
   double seconds;                 /* wall time of the replayed steps */

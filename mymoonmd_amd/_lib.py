@@ -44,13 +44,14 @@ class BatchParams(ctypes.Structure):
     _fields_ = [("example", c_int), ("scheme", c_int), ("tau", c_double), ("t0", c_double),
                 ("inner_solver", c_int), ("gmres_restart", c_int), ("max_inner", c_int),
                 ("max_outer", c_int), ("inner_reduction", c_double),
-                ("residual_reduction", c_double), ("residual_tolerance", c_double), ("block", c_int)]
+                ("residual_reduction", c_double), ("residual_tolerance", c_double), ("block", c_int),
+                ("max_groups", c_int)]
 
 
 class BatchInfo(ctypes.Structure):
     _fields_ = [("n_problems", c_int), ("total_dof", c_int), ("total_active", c_int),
                 ("total_cells", c_int), ("total_nnz", ctypes.c_longlong), ("block", c_int),
-                ("time", c_double)]
+                ("time", c_double), ("n_workgroups", c_int)]
 
 
 class BatchReport(ctypes.Structure):

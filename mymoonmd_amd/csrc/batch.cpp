@@ -74,6 +74,13 @@ struct pmx_batch {
   double* d_time = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // workgroups per problem (BiCGStab), set on the first run
+  bool assigned = false;
+  int n_wg = 0;
+  std::vector<int> groups;
+  int *d_wg_prob = nullptr, *d_wg_rank = nullptr, *d_groups = nullptr, *d_bar_timeout = nullptr;
+  unsigned int* d_bar = nullptr;
+  double* d_part = nullptr;
   // snapshots of all problems (total_dof x snap_max, column-major) and the
   // POD of every problem (basis total_dof x pod_kmax, mean total_dof)
   double* d_snaps = nullptr;
@@ -91,7 +98,7 @@ struct pmx_batch {
              d_dinv, d_dinv32, d_zero, d_geo, d_dofs, d_cells, d_cell_prob, d_active_end, d_coef, d_xyz,
              d_dir, d_row_active, d_ref, d_order, d_dof_off, d_n_active, d_slice_off, d_ws_off, d_x, d_f,
              d_rhs, d_r, d_w, d_V, d_iters, d_outer, d_fail, d_res, d_time, d_snaps, d_pod_basis,
-             d_pod_mean})
+             d_pod_mean, d_wg_prob, d_wg_rank, d_groups, d_bar_timeout, d_bar, d_part})
       dfree(p);
   }
 
@@ -153,6 +160,15 @@ struct pmx_batch {
     B.outer = d_outer;
     B.res = d_res;
     B.fail = d_fail;
+    if (d_wg_prob) {
+      B.n_wg = n_wg;
+      B.wg_prob = d_wg_prob;
+      B.wg_rank = d_wg_rank;
+      B.groups = d_groups;
+      B.bar = d_bar;
+      B.part = d_part;
+      B.bar_timeout = d_bar_timeout;
+    }
     return B;
   }
 
@@ -239,6 +255,7 @@ int pmx_batch_default_params(pmx_batch_params* p) {
     p->residual_reduction = 1e-10;
     p->residual_tolerance = 0.0;
     p->block = 0;
+    p->max_groups = 8;
   });
 }
 
@@ -462,6 +479,7 @@ int pmx_batch_get_info(const pmx_batch* b, pmx_batch_info* out) {
     out->total_nnz = b->total_nnz;
     out->block = b->block;
     out->time = b->time;
+    out->n_workgroups = b->d_wg_prob ? b->n_wg : b->n_prob;
   });
 }
 
@@ -513,41 +531,137 @@ int pmx_batch_initialize(pmx_batch* b) {
   });
 }
 
+// Workgroups per problem for the BiCGStab kernel from the measured
+// iterations of one step: cost = iterations x slices; one more workgroup goes
+// to the problem of the largest cost per workgroup while that exceeds twice
+// the mean cost per workgroup of the grid, within the number of workgroups
+// that can be resident at once (the problem's workgroups wait for each other)
+// and max_groups per problem.
+static void clear_groups(pmx_batch* b) {
+  dfree(b->d_wg_prob); dfree(b->d_wg_rank); dfree(b->d_groups); dfree(b->d_bar); dfree(b->d_part);
+  dfree(b->d_bar_timeout);
+  b->groups.clear();
+  b->n_wg = 0;
+}
+
+static void assign_groups(pmx_batch* b, const std::vector<int>& iters) {
+  clear_groups(b);
+  const int np = b->n_prob;
+  const int gmax = std::min(std::max(1, b->prm.max_groups), pmx::kBatchMaxGroups);
+  if (b->prm.inner_solver != PMX_BATCH_INNER_BICGSTAB || gmax <= 1) return;
+  const int capacity = pmx::batch_solve_capacity(b->block);
+  if (capacity <= np) return;
+  std::vector<double> cost(np);
+  double total = 0;
+  for (int q = 0; q < np; ++q) {
+    cost[q] = (double)std::max(1, iters[q]) * (double)(b->slice_off[q + 1] - b->slice_off[q]);
+    total += cost[q];
+  }
+  std::vector<int> G(np, 1);
+  int n_wg = np;
+  for (;;) {
+    int top = 0;
+    for (int q = 1; q < np; ++q)
+      if (cost[q] / G[q] > cost[top] / G[top]) top = q;
+    if (n_wg >= capacity || G[top] >= gmax || cost[top] / G[top] <= 2.0 * total / n_wg) break;
+    ++G[top];
+    ++n_wg;
+  }
+  if (n_wg == np) return;
+  // workgroups of the costliest problems first, ranks contiguous
+  std::vector<int> order(np);
+  for (int q = 0; q < np; ++q) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  std::vector<int> wg_prob, wg_rank;
+  for (int q : order)
+    for (int g = 0; g < G[q]; ++g) {
+      wg_prob.push_back(q);
+      wg_rank.push_back(g);
+    }
+  hipStream_t st = b->ctx->stream;
+  b->groups = G;
+  b->n_wg = n_wg;
+  b->d_wg_prob = upload(wg_prob.data(), wg_prob.size(), st);
+  b->d_wg_rank = upload(wg_rank.data(), wg_rank.size(), st);
+  b->d_groups = upload(G.data(), G.size(), st);
+  b->d_bar = dmalloc<unsigned int>(np);
+  b->d_part = dmalloc<double>((size_t)np * 2 * pmx::kBatchMaxGroups * 2);
+  b->d_bar_timeout = dmalloc<int>(1);
+  HIP_CHECK(hipMemsetAsync(b->d_bar_timeout, 0, sizeof(int), st));
+  HIP_CHECK(hipStreamSynchronize(st));
+}
+
 int pmx_batch_run(pmx_batch* b, int n_steps, pmx_batch_report* rep) {
   return guarded([&] {
     require(b && n_steps >= 0, "invalid argument");
     hipStream_t st = b->ctx->stream;
-    if (b->block == 0) {
-      // tune the workgroup size of the persistent solve on this batch: one
-      // step per candidate from the current state, which is restored after
-      double* x0 = dmalloc<double>(b->total_dof);
+    // the state every trial step starts from and is restored to
+    auto save = [&](double* x0, std::vector<int>& it0) {
       HIP_CHECK(hipMemcpyAsync(x0, b->d_x, sizeof(double) * b->total_dof, hipMemcpyDeviceToDevice, st));
+      HIP_CHECK(hipMemcpyAsync(it0.data(), b->d_iters, sizeof(int) * b->n_prob, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+    };
+    auto restore = [&](const double* x0, const std::vector<int>& it0) {
+      HIP_CHECK(hipMemcpyAsync(b->d_time, &b->time, sizeof(double), hipMemcpyHostToDevice, st));
+      HIP_CHECK(hipMemcpyAsync(b->d_x, x0, sizeof(double) * b->total_dof, hipMemcpyDeviceToDevice, st));
+      HIP_CHECK(hipMemcpyAsync(b->d_iters, it0.data(), sizeof(int) * b->n_prob, hipMemcpyHostToDevice, st));
+      HIP_CHECK(hipMemsetAsync(b->d_outer, 0, sizeof(int) * b->n_prob, st));
+      HIP_CHECK(hipMemsetAsync(b->d_fail, 0, sizeof(int) * b->n_prob, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+    };
+    if (!b->assigned) {
+      // one step with one workgroup per problem measures every problem's
+      // iterations; without a given workgroup size the candidates (size,
+      // with and without workgroups shared by the costliest problems) are
+      // timed on one step each from the same state and the fastest is kept
+      const int given = b->block;
+      double* x0 = dmalloc<double>(b->total_dof);
+      std::vector<int> it0(b->n_prob), it1(b->n_prob);
+      save(x0, it0);
+      clear_groups(b);
+      b->block = given ? given : 1024;
+      HIP_CHECK(hipMemcpyAsync(b->d_time, &b->time, sizeof(double), hipMemcpyHostToDevice, st));
+      b->step(b->block, st);
+      HIP_CHECK(hipMemcpyAsync(it1.data(), b->d_iters, sizeof(int) * b->n_prob, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      for (int q = 0; q < b->n_prob; ++q) it1[q] -= it0[q];
+      restore(x0, it0);
       hipEvent_t e0, e1;
       HIP_CHECK(hipEventCreate(&e0));
       HIP_CHECK(hipEventCreate(&e1));
       float best = 1e30f;
-      int best_blk = 1024;
-      for (int blk : {256, 512, 1024}) {
-        HIP_CHECK(hipMemcpyAsync(b->d_time, &b->time, sizeof(double), hipMemcpyHostToDevice, st));
-        HIP_CHECK(hipMemcpyAsync(b->d_x, x0, sizeof(double) * b->total_dof, hipMemcpyDeviceToDevice, st));
-        HIP_CHECK(hipEventRecord(e0, st));
-        b->step(blk, st);
-        HIP_CHECK(hipEventRecord(e1, st));
-        HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0;
-        HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; best_blk = blk; }
-      }
-      HIP_CHECK(hipMemcpyAsync(b->d_time, &b->time, sizeof(double), hipMemcpyHostToDevice, st));
-      HIP_CHECK(hipMemcpyAsync(b->d_x, x0, sizeof(double) * b->total_dof, hipMemcpyDeviceToDevice, st));
-      HIP_CHECK(hipMemsetAsync(b->d_iters, 0, sizeof(int) * b->n_prob, st));
-      HIP_CHECK(hipMemsetAsync(b->d_outer, 0, sizeof(int) * b->n_prob, st));
-      HIP_CHECK(hipMemsetAsync(b->d_fail, 0, sizeof(int) * b->n_prob, st));
-      HIP_CHECK(hipStreamSynchronize(st));
+      int best_blk = b->block;
+      bool best_grouped = true;
+      // a given size is not timed (the choice stays deterministic): the
+      // costliest problems share workgroups whenever the grid fits
+      std::vector<int> blocks = given ? std::vector<int>{} : std::vector<int>{256, 512, 1024};
+      for (int blk : blocks)
+        for (int grouped = 0; grouped < 2; ++grouped) {
+          b->block = blk;
+          if (grouped) {
+            assign_groups(b, it1);
+            if (!b->d_wg_prob) continue;   // nothing to share at this size
+          } else {
+            clear_groups(b);
+          }
+          HIP_CHECK(hipEventRecord(e0, st));
+          b->step(blk, st);
+          HIP_CHECK(hipEventRecord(e1, st));
+          HIP_CHECK(hipEventSynchronize(e1));
+          float ms = 0;
+          HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+          int flag = 0;
+          if (b->d_bar_timeout) HIP_CHECK(hipMemcpy(&flag, b->d_bar_timeout, sizeof(int), hipMemcpyDeviceToHost));
+          if (!flag && ms < best) { best = ms; best_blk = blk; best_grouped = grouped != 0; }
+          restore(x0, it0);
+        }
+      b->block = best_blk;
+      if (best_grouped) assign_groups(b, it1);
+      else clear_groups(b);
+      b->assigned = true;
       HIP_CHECK(hipEventDestroy(e0));
       HIP_CHECK(hipEventDestroy(e1));
       dfree(x0);
-      b->block = best_blk;
     }
     if (!b->exec) {
       HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -560,6 +674,11 @@ int pmx_batch_run(pmx_batch* b, int n_steps, pmx_batch_report* rep) {
     HIP_CHECK(hipStreamSynchronize(st));
     const auto t1 = std::chrono::steady_clock::now();
     HIP_CHECK(hipGetLastError());
+    if (b->d_bar_timeout) {
+      int flag = 0;
+      HIP_CHECK(hipMemcpy(&flag, b->d_bar_timeout, sizeof(int), hipMemcpyDeviceToHost));
+      if (flag) throw PmxError(PMX_E_INTERNAL, "batch: a workgroup barrier gave up waiting (results invalid)");
+    }
     b->time += n_steps * b->prm.tau;
     if (rep) {
       std::vector<int> it(b->n_prob), fail(b->n_prob), outer(b->n_prob);

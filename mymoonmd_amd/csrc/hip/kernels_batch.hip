@@ -18,6 +18,8 @@
 // SELL slices of 64 rows (local column indices), so one workgroup can run the
 // whole solve of one problem with workgroup barriers only: the solve kernel is
 // one persistent launch with one workgroup per problem (largest first).
+#include <stdexcept>
+
 #include "common.hpp"
 #include "examples.hpp"
 #include "launch.hpp"
@@ -228,15 +230,58 @@ __device__ __forceinline__ void wave_rows(const BatchDev& B, int s0, int s1, int
 // vector updates, so the only cross-thread data flow is the gather of the
 // SpMVs; dots are fused into the passes that produce their operands.
 // Workspace per problem: ri, rh, p, v, s, t, d (7 x n_active floats).
+//
+// A problem may get G > 1 workgroups (B.wg_prob): each takes a contiguous
+// share of the problem's slices; the vectors stay the problem's, so the
+// passes that read other rows (the SpMVs, the FP64 residual) are preceded by
+// a barrier of the problem's workgroups, and every dot product adds the G
+// workgroup partials in rank order, so all G see the same value and take
+// the same branches.  The host launches such problems only when every
+// workgroup of the grid can be resident at once; each barrier wait is
+// bounded and sets B.bar_timeout when it gives up, so the kernel always ends.
+constexpr unsigned kBatchSpinLimit = 1u << 22;
+
 template <int BS>
 __global__ __launch_bounds__(BS) void k_batch_solve_bicgstab(BatchDev B, BatchSolveParams P) {
   constexpr int NW = BS / kWave;
   constexpr int U = 4;
   __shared__ double red[NW];
   __shared__ double red2[NW];
-  const int p = B.order[blockIdx.x];
+  int p, g = 0, G = 1;
+  if (B.wg_prob) {
+    p = B.wg_prob[blockIdx.x];
+    g = B.wg_rank[blockIdx.x];
+    G = B.groups[p];
+  } else {
+    p = B.order[blockIdx.x];
+  }
   const int off = B.dof_off[p], na = B.n_active[p];
-  const int s0 = B.slice_off[p], s1 = B.slice_off[p + 1];
+  const int ps0 = B.slice_off[p], ps1 = B.slice_off[p + 1];
+  const int s0 = ps0 + (int)((long long)(ps1 - ps0) * g / G);
+  const int s1 = ps0 + (int)((long long)(ps1 - ps0) * (g + 1) / G);
+  unsigned epoch = 0;
+  int nred = 0;
+  // barrier of the problem's workgroups (a block barrier when G = 1)
+  auto gbar = [&]() {
+    __syncthreads();
+    if (G == 1) return;
+    ++epoch;
+    if (threadIdx.x == 0) {
+      __threadfence();
+      atomicAdd(B.bar + p, 1u);
+      const unsigned target = epoch * (unsigned)G;
+      unsigned spins = 0;
+      while (__hip_atomic_load(B.bar + p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++spins > kBatchSpinLimit) {
+          atomicOr(B.bar_timeout, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __threadfence();
+    }
+    __syncthreads();
+  };
   double* x = B.x + off;
   const double* rhs = B.rhs + off;
   const float* dinv = B.dinv32 + off;
@@ -248,7 +293,7 @@ __global__ __launch_bounds__(BS) void k_batch_solve_bicgstab(BatchDev B, BatchSo
   float* tv = sv + na;
   float* dv = tv + na;
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-  // two block sums at once (fixed order)
+  // two sums at once (fixed order: waves, then the problem's workgroups)
   auto sum2 = [&](double a, double b, double& ra, double& rb) {
     a = group_sum<kWave>(a);
     b = group_sum<kWave>(b);
@@ -258,6 +303,23 @@ __global__ __launch_bounds__(BS) void k_batch_solve_bicgstab(BatchDev B, BatchSo
 #pragma unroll
     for (int i = 0; i < NW; ++i) { sa += red[i]; sb += red2[i]; }
     __syncthreads();
+    if (G > 1) {
+      // two slot buffers alternate: a workgroup can only overwrite a buffer
+      // after the next barrier, which every reader of it has passed
+      const int buf = (nred++) & 1;
+      double* slots = B.part + ((size_t)p * 2 + buf) * kBatchMaxGroups * 2;
+      if (threadIdx.x == 0) {
+        slots[2 * g] = sa;
+        slots[2 * g + 1] = sb;
+      }
+      gbar();
+      sa = 0;
+      sb = 0;
+      for (int q = 0; q < G; ++q) {
+        sa += __hip_atomic_load(slots + 2 * q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        sb += __hip_atomic_load(slots + 2 * q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
     ra = sa;
     rb = sb;
   };
@@ -298,7 +360,7 @@ __global__ __launch_bounds__(BS) void k_batch_solve_bicgstab(BatchDev B, BatchSo
       const float om = (float)omega;
       // p = ri + beta (p - omega v)
       wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) { pv[lr] = ri[lr] + beta * (pv[lr] - om * v[lr]); });
-      __syncthreads();
+      gbar();
       // v = S D^-1 p ; (rh, v)
       double a = 0;
       wave_spmv<U, float, float>(B, B.S32, s0, s1, wv, NW, lane,
@@ -355,9 +417,9 @@ __global__ __launch_bounds__(BS) void k_batch_solve_bicgstab(BatchDev B, BatchSo
     inner_total += it;
     // x += d (FP64)
     wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) { x[lr] += (double)dv[lr]; });
-    __syncthreads();
+    gbar();
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && g == 0) {
     B.iters[p] += inner_total;
     B.outer[p] += outer;
     B.res[p] = (rn0 > 0) ? rn / rn0 : 0.0;
@@ -447,17 +509,44 @@ __global__ void k_sell_gather(size_t total, const int* __restrict__ src, const d
   }
 }
 
+// zero the per-problem barrier counters (a kernel node: the step is replayed
+// from a graph)
+__global__ void k_fill_u32(int n, unsigned int* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0u;
+}
+
 }  // namespace
+
+int batch_solve_capacity(int block) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  hipError_t e;
+  switch (block) {
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve_bicgstab<256>, 256, 0); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve_bicgstab<512>, 512, 0); break;
+    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve_bicgstab<1024>, 1024, 0); break;
+  }
+  return e == hipSuccess ? per * cus : 0;
+}
 
 void batch_solve(const BatchDev& B, const BatchSolveParams& P, int block, hipStream_t st) {
   if (P.method == 1) {
+    const int grid = B.wg_prob ? B.n_wg : B.n_prob;
+    if (B.wg_prob) {
+      if (!B.groups || !B.bar || !B.part || !B.bar_timeout || B.n_wg < B.n_prob)
+        throw std::runtime_error("batch: incomplete workgroup assignment");
+      hipLaunchKernelGGL(k_fill_u32, dim3((B.n_prob + 255) / 256), dim3(256), 0, st, B.n_prob, B.bar);
+    }
     switch (block) {
-      case 256: hipLaunchKernelGGL((k_batch_solve_bicgstab<256>), dim3(B.n_prob), dim3(256), 0, st, B, P); break;
-      case 512: hipLaunchKernelGGL((k_batch_solve_bicgstab<512>), dim3(B.n_prob), dim3(512), 0, st, B, P); break;
-      default: hipLaunchKernelGGL((k_batch_solve_bicgstab<1024>), dim3(B.n_prob), dim3(1024), 0, st, B, P); break;
+      case 256: hipLaunchKernelGGL((k_batch_solve_bicgstab<256>), dim3(grid), dim3(256), 0, st, B, P); break;
+      case 512: hipLaunchKernelGGL((k_batch_solve_bicgstab<512>), dim3(grid), dim3(512), 0, st, B, P); break;
+      default: hipLaunchKernelGGL((k_batch_solve_bicgstab<1024>), dim3(grid), dim3(1024), 0, st, B, P); break;
     }
     return;
   }
+  if (B.wg_prob) throw std::runtime_error("batch: several workgroups per problem need BiCGStab");
   switch (block) {
     case 256: hipLaunchKernelGGL((k_batch_solve<256>), dim3(B.n_prob), dim3(256), 0, st, B, P); break;
     case 512: hipLaunchKernelGGL((k_batch_solve<512>), dim3(B.n_prob), dim3(512), 0, st, B, P); break;

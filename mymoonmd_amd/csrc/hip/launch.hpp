@@ -244,7 +244,20 @@ struct BatchDev {
   int* outer = nullptr;
   double* res = nullptr;
   int* fail = nullptr;
+  // several workgroups per problem (BiCGStab only; nullptr: one each, in
+  // `order`): workgroup -> problem and rank within it, workgroups per
+  // problem, per-problem barrier counters (zeroed before each launch),
+  // partial-sum slots [n_prob][2][kBatchMaxGroups][2], a flag set when a
+  // barrier wait gave up
+  int n_wg = 0;
+  const int* wg_prob = nullptr;
+  const int* wg_rank = nullptr;
+  const int* groups = nullptr;
+  unsigned int* bar = nullptr;
+  double* part = nullptr;
+  int* bar_timeout = nullptr;
 };
+constexpr int kBatchMaxGroups = 16;
 struct BatchSolveParams {
   int method = 0;          // inner solver: 0 GMRES(m), 1 BiCGStab
   int m = 30;              // GMRES restart (<= 64)
@@ -255,6 +268,8 @@ struct BatchSolveParams {
   double tol = 0.0;        // FP64 absolute residual
 };
 void batch_solve(const BatchDev& B, const BatchSolveParams& P, int block, hipStream_t st);
+// workgroups of the BiCGStab batch kernel that can be resident at once
+int batch_solve_capacity(int block);
 void batch_rhs(int n_slices, const int* slice_prob, const int* dof_off, const int* sptr, const int* srow,
                const int* scol, const double* M64, double tau, const double* f, const double* u,
                double* rhs, hipStream_t st);

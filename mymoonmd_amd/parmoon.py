@@ -610,7 +610,7 @@ class BatchTCD:
 
     def __init__(self, ctx, problems, example="tcd_exp", tau=0.01, t0=0.0, inner_solver="bi_cgstab",
                  gmres_restart=30, max_inner=2000, max_outer=50, inner_reduction=1e-4,
-                 residual_reduction=1e-10, residual_tolerance=0.0, block=0):
+                 residual_reduction=1e-10, residual_tolerance=0.0, block=0, max_groups=8):
         self.ctx = ctx
         prm = BatchParams()
         call("pmx_batch_default_params", ctypes.byref(prm))
@@ -621,6 +621,7 @@ class BatchTCD:
         prm.gmres_restart, prm.max_inner, prm.max_outer = gmres_restart, max_inner, max_outer
         prm.inner_reduction, prm.residual_reduction = inner_reduction, residual_reduction
         prm.residual_tolerance, prm.block = residual_tolerance, block
+        prm.max_groups = max_groups
         arr = (BatchProblem * len(problems))()
         for k, (n, eps, b, c) in enumerate(problems):
             arr[k].n_side, arr[k].eps, arr[k].c = n, eps, c
