@@ -255,7 +255,7 @@ int pmx_batch_default_params(pmx_batch_params* p) {
     p->residual_reduction = 1e-10;
     p->residual_tolerance = 0.0;
     p->block = 0;
-    p->max_groups = 8;
+    p->max_groups = 16;
   });
 }
 
@@ -533,8 +533,8 @@ int pmx_batch_initialize(pmx_batch* b) {
 
 // Workgroups per problem for the BiCGStab kernel from the measured
 // iterations of one step: cost = iterations x slices; one more workgroup goes
-// to the problem of the largest cost per workgroup while that exceeds twice
-// the mean cost per workgroup of the grid, within the number of workgroups
+// to the problem of the largest cost per workgroup while that exceeds the
+// mean cost per workgroup of the grid, within the number of workgroups
 // that can be resident at once (the problem's workgroups wait for each other)
 // and max_groups per problem.
 static void clear_groups(pmx_batch* b) {
@@ -563,7 +563,7 @@ static void assign_groups(pmx_batch* b, const std::vector<int>& iters) {
     int top = 0;
     for (int q = 1; q < np; ++q)
       if (cost[q] / G[q] > cost[top] / G[top]) top = q;
-    if (n_wg >= capacity || G[top] >= gmax || cost[top] / G[top] <= 2.0 * total / n_wg) break;
+    if (n_wg >= capacity || G[top] >= gmax || cost[top] / G[top] <= 1.0 * total / n_wg) break;
     ++G[top];
     ++n_wg;
   }

@@ -257,7 +257,7 @@ struct BatchDev {
   double* part = nullptr;
   int* bar_timeout = nullptr;
 };
-constexpr int kBatchMaxGroups = 16;
+constexpr int kBatchMaxGroups = 32;
 struct BatchSolveParams {
   int method = 0;          // inner solver: 0 GMRES(m), 1 BiCGStab
   int m = 30;              // GMRES restart (<= 64)

@@ -131,18 +131,18 @@ def test_batch_workgroups_per_problem(ctx):
     probs = [(64, 1e-4, (1.0, 0.3), 0.0), (8, 1.0, (1.0, 0.0), 0.0), (12, 0.5, (0.0, 1.0), 0.0),
              (16, 0.3, (0.6, 0.8), 0.0)]
     sols, wgs = {}, {}
-    for mg in (1, 8, 8):
+    for mg in (1, 16, 16):
         bt = pm.BatchTCD(ctx, probs, tau=TAU, block=256, max_groups=mg)
         bt.initialize()
         rep = bt.run(STEPS)
         assert rep.failed_solves == 0
         wgs[mg] = bt.info().n_workgroups
         sols.setdefault(mg, []).append([bt.solution(k) for k in range(len(probs))])
-    assert wgs[1] == len(probs) and wgs[8] > len(probs)
+    assert wgs[1] == len(probs) and wgs[16] > len(probs)
     for k in range(len(probs)):
         ref = sols[1][0][k]
-        assert np.array_equal(sols[8][0][k], sols[8][1][k]), k
-        assert np.max(np.abs(sols[8][0][k] - ref)) <= 1e-8 * np.max(np.abs(ref)), k
+        assert np.array_equal(sols[16][0][k], sols[16][1][k]), k
+        assert np.max(np.abs(sols[16][0][k] - ref)) <= 1e-8 * np.max(np.abs(ref)), k
 
 
 def test_batch_autotune_picks_a_block(ctx):
