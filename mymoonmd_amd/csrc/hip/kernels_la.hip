@@ -664,6 +664,16 @@ void fe_apply(const DevCSR& A, const double* x, double* y, hipStream_t st) {
   dispatch_fe_spmv<MODE_APPLY, double, double>(A, A.val, nullptr, x, nullptr, nullptr, y, 1.0,
                                                nullptr, st);
 }
+// d = b - A x with FP32 copies of the values, FP64 vectors and sums (CSR
+// kernel with A.lpr lanes per row; no partial sums kept)
+void fe_defect_f32val(const DevCSR& A, const float* val32, const double* x, const double* b, double* d,
+                      double* partial, hipStream_t st) {
+  switch (A.lpr) {
+    case 4: launch_fe_spmv<4, MODE_DEFECT>(A, val32, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
+    case 16: launch_fe_spmv<16, MODE_DEFECT>(A, val32, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
+    default: launch_fe_spmv<8, MODE_DEFECT>(A, val32, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
+  }
+}
 // number of blocks (= partial sums) the FE-matrix kernel launches for A
 static int spmv_grid(const DevCSR& A, bool bf16) {
   const bool have_sell = bf16 ? (A.sell.val_bf16 != nullptr) : (A.sell.val != nullptr);

@@ -64,6 +64,8 @@ void sell_fill_bf16(const DevCSR& A, uint16_t* sell_bf16, hipStream_t st);
 void fe_apply_scaled_add(const DevCSR& A, const double* x, double* y, double a, hipStream_t st);
 void fe_apply(const DevCSR& A, const double* x, double* y, hipStream_t st);
 int fe_defect_partials(const DevCSR& A);
+void fe_defect_f32val(const DevCSR& A, const float* val32, const double* x, const double* b, double* d,
+                      double* partial, hipStream_t st);
 void fe_defect(const DevCSR& A, const double* x, const double* b, double* d, double* partial,
                double* norm_out, hipStream_t st);
 void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* dinv, double* xn,

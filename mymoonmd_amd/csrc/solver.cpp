@@ -154,6 +154,7 @@ struct NseLevel {
   int *map = nullptr, *pmap = nullptr, *acc_ptr = nullptr, *acc_idx = nullptr;
   double* lu64 = nullptr;
   float* lu32 = nullptr;
+  float* cval32 = nullptr;     // mixed precision: FP32 copy of the combined values (smoother defects)
   double *upd = nullptr, *rhs = nullptr, *sol = nullptr, *def = nullptr, *res = nullptr;
   pmx::DevCSR Pu{}, Ru{}, Pp{}, Rp{};   // P: finer rows from this level; R = P^T (rows: this level)
   double* dinv_dense = nullptr;          // coarsest level: inverse of the combined matrix
@@ -163,7 +164,7 @@ struct NseLevel {
   int* colour_cells = nullptr;           // cells in colour-major order (27 colours)
   std::vector<int> colour_ptr;
   void release() {
-    dfree(map); dfree(pmap); dfree(acc_ptr); dfree(acc_idx); dfree(lu64); dfree(lu32); dfree(upd);
+    dfree(map); dfree(pmap); dfree(acc_ptr); dfree(acc_idx); dfree(lu64); dfree(lu32); dfree(upd); dfree(cval32);
     dfree(rhs); dfree(sol); dfree(def); dfree(res);
     free_csr(Pu); free_csr(Ru); free_csr(Pp); free_csr(Rp);
     dfree(dinv_dense); dfree(partial); dfree(colour_cells);
@@ -481,7 +482,8 @@ void nse_smooth(pmx_solver* s, int l) {
   // (K 0 sums signed zeros), so the product is skipped
   const double* r = L.rhs;
   if (!(L.sol_zero && s->p.zero_start_shortcut)) {
-    pmx::fe_defect(nse_combined(L.sys), L.sol, L.rhs, L.res, L.partial, nullptr, S(s));
+    if (L.cval32) pmx::fe_defect_f32val(nse_combined(L.sys), L.cval32, L.sol, L.rhs, L.res, L.partial, S(s));
+    else pmx::fe_defect(nse_combined(L.sys), L.sol, L.rhs, L.res, L.partial, nullptr, S(s));
     r = L.res;
   }
   L.sol_zero = false;
@@ -1222,6 +1224,12 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
           L.upd = dmalloc<double>((size_t)L.n_cells * NLc);
           pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.pmap, sing, st);
           const int kind = (l == 0) ? s->p.mg_smoother_coarse : s->p.mg_smoother;
+          if (f32 && kind == PMX_SMOOTHER_CELL_VANKA_JACOBI) {
+            // the smoother's defects read FP32 values (FP64 vectors and sums)
+            const pmx::DevCSR C = nse_combined(sys);
+            L.cval32 = dmalloc<float>(std::max(1, C.nnz));
+            pmx::vec_f64_to_f32(C.nnz, C.val, L.cval32, st);
+          }
           if (kind == PMX_SMOOTHER_CELL_VANKA) {
             // colour = cell lattice index mod 3 per direction: cells of one
             // colour are >= 3 cells apart, so none reads a DOF another updates
