@@ -478,13 +478,22 @@ __global__ __launch_bounds__(1024) void k_reduce_errors(int nb, const double* __
 // D[4 t + l / 16][l % 16].
 typedef double v4d_t __attribute__((ext_vector_type(4)));
 
+// the waves of the matrix-core assembly share no LDS: each synchronises only
+// its own lanes (the fences order its LDS stores before the other lanes' loads)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int QMAX>
 __global__ __launch_bounds__(256) void k_assemble_cd_q2_mfma(AssemblyData ad, CDCoeffs co,
                                                              const int* __restrict__ cells, int n, DevCSR A,
                                                              double* __restrict__ rhs) {
   constexpr int ND = 27, DIM = 3, NC = 6, QC = 4, NV = 7;
   constexpr int RS = 1 + DIM + ND * (1 + DIM + NC);
   __shared__ double s_p[4][QC][ND][NV];   // per wave: g1 g2 g3, phi + delta t, delta t, b.g + c phi, -eps lap
-  __shared__ double s_c[4][64][8];        // per wave: eps b1 b2 b3 c f delta mult of every quadrature point
+  __shared__ double s_c[4][QMAX][8];      // per wave: eps b1 b2 b3 c f delta mult of every quadrature point
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int ci = blockIdx.x * 4 + wv;
   const bool valid = ci < n;
@@ -506,7 +515,7 @@ __global__ __launch_bounds__(256) void k_assemble_cd_q2_mfma(AssemblyData ad, CD
     const double* rq = ad.ref + (size_t)lane * RS;
     coeffs_at<3>(co, geo, rq + 1, rq[0], cell, lane, ad.nq, s_c[wv][lane]);
   }
-  __syncthreads();
+  wave_sync();
   v4d_t acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -548,7 +557,7 @@ __global__ __launch_bounds__(256) void k_assemble_cd_q2_mfma(AssemblyData ad, CD
         for (int v = 0; v < NV; ++v) out[v] = 0.0;
       }
     }
-    __syncthreads();
+    wave_sync();
     if (lane < ND)
       for (int qq = 0; qq < QC && q0 + qq < ad.nq; ++qq) {
         const double* c = s_c[wv][q0 + qq];
@@ -580,7 +589,7 @@ __global__ __launch_bounds__(256) void k_assemble_cd_q2_mfma(AssemblyData ad, CD
         for (int b = 0; b < 2; ++b)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
-    __syncthreads();
+    wave_sync();
   }
   if (!valid) return;
   const int* dofs = ad.dofs + (size_t)cell * ND;
@@ -664,7 +673,12 @@ void assemble_cd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, i
   if (ad.dim == 3 && ad.ndof == 27) {
     if (ad.nq <= 64) {
       if (n > 0)
-        hipLaunchKernelGGL(k_assemble_cd_q2_mfma, dim3((n + 3) / 4), dim3(256), 0, st, ad, co, cells, n, A, rhs);
+        // the coefficient table of 32 quadrature points (Gauss3: 27) leaves
+        // LDS for more workgroups per CU than the one of 64
+        if (ad.nq <= 32)
+          hipLaunchKernelGGL(k_assemble_cd_q2_mfma<32>, dim3((n + 3) / 4), dim3(256), 0, st, ad, co, cells, n, A, rhs);
+        else
+          hipLaunchKernelGGL(k_assemble_cd_q2_mfma<64>, dim3((n + 3) / 4), dim3(256), 0, st, ad, co, cells, n, A, rhs);
     } else {
       launch_asm<27, 3, 32>(ad, co, cells, n, A, rhs, st);
     }
