@@ -565,6 +565,12 @@ typedef struct {
                                   when the product has at least 131,072 rows, else CSR
                                   with 1-8 lanes per row (partial sums combined across
                                   lanes).  2: SELL for every level.  0: CSR throughout */
+  int vanka_inverse;           /* no reference counterpart.  1 (default): the cell_vanka_jacobi
+                                  smoother stores the inverse of each cell's local matrix
+                                  (from its LU factors, in place, as dgetri) and applies it
+                                  by one product; 0: it stores the LU factors and substitutes
+                                  (the reference's dgetrs).  Same smoother; the rounding
+                                  differs. */
 } pmx_solver_params;
 /* Defaults of Solver<>::default_solver_database (src/Solver/Solver.C:20-117)
  * merged with Multigrid::default_multigrid_database (src/Multigrid/Multigrid.C:422-514). */

@@ -237,12 +237,18 @@ constexpr int LDL = NL + 1;       // LDS row stride
 // global DOF whose defect lands in local row i after all row interchanges
 // (the interchanges of dgetrs applied in factorisation order compose to one
 // permutation, so the solve gathers the permuted defect directly)
+// With `inverse`, the factors are turned into the inverse of the
+// row-interchanged local matrix in place, as LAPACK dgetri does: U^-1 column
+// by column (upper triangle), then X = U^-1 L^-1 column by column from the
+// right; the stored matrix then maps the permuted local defect to the local
+// solution by one product.
 __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __restrict__ map,
                                                       const int* __restrict__ rp, const int* __restrict__ ci,
                                                       const double* __restrict__ cv, double* __restrict__ lu64,
                                                       float* __restrict__ lu32, int* __restrict__ pmap,
-                                                      int* __restrict__ singular) {
+                                                      int* __restrict__ singular, int inverse) {
   __shared__ double M[NL * LDL];
+  __shared__ double tcol[NL];
   __shared__ int dm[NL];
   __shared__ int pk[NL];
   __shared__ int src[NL];
@@ -306,6 +312,33 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
       M[i * LDL + j] -= M[i * LDL + k] * M[k * LDL + j];
     }
     __syncthreads();
+  }
+  if (inverse) {
+    // U^-1 in the upper triangle: column j from the columns before it
+    for (int j = 0; j < NL; ++j) {
+      for (int k = t; k <= j; k += blockDim.x) tcol[k] = M[k * LDL + j];   // column j of U
+      __syncthreads();
+      const double djj = tcol[j];
+      const double ij = (djj != 0.0) ? 1.0 / djj : 0.0;
+      for (int i = t; i < j; i += blockDim.x) {
+        double acc = 0.0;
+        for (int k = i; k < j; ++k) acc += M[i * LDL + k] * tcol[k];
+        M[i * LDL + j] = -acc * ij;
+      }
+      if (t == 0) M[j * LDL + j] = ij;
+      __syncthreads();
+    }
+    // X = U^-1 L^-1: column j = (U^-1)(:, j) - sum_{k > j} X(:, k) L(k, j)
+    for (int j = NL - 2; j >= 0; --j) {
+      for (int k = j + 1 + t; k < NL; k += blockDim.x) tcol[k] = M[k * LDL + j];   // column j of L
+      __syncthreads();
+      for (int i = t; i < NL; i += blockDim.x) {
+        double acc = (i <= j) ? M[i * LDL + j] : 0.0;
+        for (int k = j + 1; k < NL; ++k) acc -= M[i * LDL + k] * tcol[k];
+        M[i * LDL + j] = acc;
+      }
+      __syncthreads();
+    }
   }
   if (t == 0) {
     // compose the interchanges: local row i of the permuted system holds the
@@ -458,6 +491,52 @@ __global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const T* __res
   if (h1) upd[(size_t)cell * NL + i1] = damp * b1;
 }
 
+// the same with the explicit local inverse: x = Inv b for the permuted local
+// defect b (column-major Inv; the 89 columns are independent products, so the
+// loads of the next columns are in flight while the current ones are summed)
+template <typename T>
+__global__ __launch_bounds__(256) void k_vanka_apply(int n_cells, const T* __restrict__ inv,
+                                                     const int* __restrict__ pmap,
+                                                     const double* __restrict__ r, double damp,
+                                                     double* __restrict__ upd) {
+  constexpr int U = 8;
+  const int lane = threadIdx.x & 63;
+  const int cell = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  if (cell >= n_cells) return;   // uniform for the wavefront
+  const int* pm = pmap + (size_t)cell * NL;
+  const int i0 = lane, i1 = lane + 64;
+  const bool h1 = i1 < NL;
+  const int j1 = h1 ? i1 : i0;
+  double b0 = r[pm[i0]];
+  double b1 = r[pm[j1]];
+  if (!h1) b1 = 0.0;
+  const T* X = inv + (size_t)cell * NL * NL;
+  double x0 = 0.0, x1 = 0.0;
+  int j = 0;
+  for (; j + U <= NL; j += U) {
+    T c0[U], c1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      c0[u] = __builtin_nontemporal_load(X + (size_t)(j + u) * NL + i0);
+      c1[u] = __builtin_nontemporal_load(X + (size_t)(j + u) * NL + j1);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = j + u;
+      const double bk = lane_value(k < 64 ? b0 : b1, k & 63);
+      x0 += (double)c0[u] * bk;
+      x1 += (double)c1[u] * bk;
+    }
+  }
+  for (; j < NL; ++j) {
+    const double bk = lane_value(j < 64 ? b0 : b1, j & 63);
+    x0 += (double)X[(size_t)j * NL + i0] * bk;
+    x1 += (double)X[(size_t)j * NL + j1] * bk;
+  }
+  upd[(size_t)cell * NL + i0] = damp * x0;
+  if (h1) upd[(size_t)cell * NL + i1] = damp * x1;
+}
+
 // VankaSmoother CELL (multiplicative) over the cells of one colour: cells of a
 // colour share no DOF, so each wave computes its local defect rows
 // f_i - sum_k K_ik x_k with the current x (row entries in CSR order, as the
@@ -525,10 +604,20 @@ void combined_values(long long nnz, const uint8_t* kind, const int* pos, const d
 }
 
 void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, float* lu32, int* pmap,
-                  int* singular, hipStream_t st) {
+                  int* singular, hipStream_t st, bool inverse) {
   if (n_cells > 0)
     hipLaunchKernelGGL(k_vanka_factor, dim3(n_cells), dim3(256), 0, st, n_cells, map, C.row_ptr, C.col, C.val,
-                       lu64, lu32, pmap, singular);
+                       lu64, lu32, pmap, singular, inverse ? 1 : 0);
+}
+
+void vanka_apply_inverse(int n_cells, const double* inv64, const float* inv32, const int* pmap, const double* r,
+                         double damp, double* upd, hipStream_t st) {
+  if (n_cells <= 0) return;
+  const int grid = (n_cells + 3) / 4;
+  if (inv32) hipLaunchKernelGGL((k_vanka_apply<float>), dim3(grid), dim3(256), 0, st, n_cells, inv32, pmap, r,
+                                damp, upd);
+  else hipLaunchKernelGGL((k_vanka_apply<double>), dim3(grid), dim3(256), 0, st, n_cells, inv64, pmap, r, damp,
+                          upd);
 }
 
 void vanka_colour(int n, const int* cells, const double* lu64, const float* lu32, const int* pmap, const int* map,

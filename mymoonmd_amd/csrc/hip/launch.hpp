@@ -211,7 +211,10 @@ constexpr int kVankaLocal = 89;
 // by the cell's 89 sorted DOFs `map`); pmap[cell][i] = global DOF whose
 // defect enters row i of the row-interchanged local system
 void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, float* lu32, int* pmap,
-                  int* singular, hipStream_t st);
+                  int* singular, hipStream_t st, bool inverse = false);
+// Jacobi Vanka step with the explicit local inverses (vanka_factor with inverse)
+void vanka_apply_inverse(int n_cells, const double* inv64, const float* inv32, const int* pmap, const double* r,
+                         double damp, double* upd, hipStream_t st);
 // upd[cell][i] = damp * (local solution) for local DOF i (the order of map)
 void vanka_solve(int n_cells, const double* lu64, const float* lu32, const int* pmap, const double* r,
                  double damp, double* upd, hipStream_t st);

@@ -155,6 +155,7 @@ struct NseLevel {
   double* lu64 = nullptr;
   float* lu32 = nullptr;
   float* cval32 = nullptr;     // mixed precision: FP32 copy of the combined values (smoother defects)
+  bool inv = false;            // lu64/lu32 hold the local inverses instead of the LU factors
   double *upd = nullptr, *rhs = nullptr, *sol = nullptr, *def = nullptr, *res = nullptr;
   pmx::DevCSR Pu{}, Ru{}, Pp{}, Rp{};   // P: finer rows from this level; R = P^T (rows: this level)
   double* dinv_dense = nullptr;          // coarsest level: inverse of the combined matrix
@@ -487,7 +488,8 @@ void nse_smooth(pmx_solver* s, int l) {
     r = L.res;
   }
   L.sol_zero = false;
-  pmx::vanka_solve(L.n_cells, L.lu64, L.lu32, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
+  if (L.inv) pmx::vanka_apply_inverse(L.n_cells, L.lu64, L.lu32, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
+  else pmx::vanka_solve(L.n_cells, L.lu64, L.lu32, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
   pmx::vanka_accumulate(L.n, L.acc_ptr, L.acc_idx, L.upd, L.sol, S(s));
 }
 
@@ -887,6 +889,7 @@ int pmx_solver_default_params(pmx_solver_params* p) {
     d.zero_start_shortcut = 1;
     d.renumber_levels = 3;
     d.transfer_sell = 1;
+    d.vanka_inverse = 1;
     *p = d;
   });
 }
@@ -1222,8 +1225,9 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
           else L.lu64 = dmalloc<double>(nlu);
           L.pmap = dmalloc<int>((size_t)L.n_cells * NLc);
           L.upd = dmalloc<double>((size_t)L.n_cells * NLc);
-          pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.pmap, sing, st);
           const int kind = (l == 0) ? s->p.mg_smoother_coarse : s->p.mg_smoother;
+          L.inv = s->p.vanka_inverse != 0 && kind == PMX_SMOOTHER_CELL_VANKA_JACOBI;
+          pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.pmap, sing, st, L.inv);
           if (f32 && kind == PMX_SMOOTHER_CELL_VANKA_JACOBI) {
             // the smoother's defects read FP32 values (FP64 vectors and sums)
             const pmx::DevCSR C = nse_combined(sys);
