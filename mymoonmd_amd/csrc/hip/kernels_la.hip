@@ -42,12 +42,24 @@ __global__ __launch_bounds__(256) void k_fe_spmv(
   if (row < n) {
     if (row < active) {
       const int s = rp[row], e = rp[row + 1];
-      for (int k = s + lane; k < e; k += LPR) {
-        AT v;
-        if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(va[k]);
-        else v = (AT)va[k];
-        acc += v * (AT)x[ci[k]];
+      auto val = [&](int k) -> AT {
+        if constexpr (sizeof(VT) == 2) return (AT)bf16_to_float(va[k]);
+        else return (AT)va[k];
+      };
+      int k = s + lane;
+      // four entries per lane loaded before their products are summed (in
+      // the same order as one at a time)
+      for (; k + 3 * LPR < e; k += 4 * LPR) {
+        int c[4];
+        AT v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = __builtin_nontemporal_load(ci + k + u * LPR);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = val(k + u * LPR);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += v[u] * (AT)x[c[u]];
       }
+      for (; k < e; k += LPR) acc += val(k) * (AT)x[ci[k]];
       acc = group_sum<LPR>(acc);
       if (rscale) acc *= (AT)rscale[row];
     }
