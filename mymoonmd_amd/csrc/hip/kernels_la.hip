@@ -265,6 +265,63 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
   }
 }
 
+// y_k += a A x_k for three vectors at once (x_k = x + k * ld, y_k likewise):
+// the velocity block of an NSTYPE-1 saddle-point operator acts on all three
+// components, so its SELL slots are read once for the three products.  Each
+// row sum runs in the order of k_sell_spmv (bitwise the same results).
+__global__ __launch_bounds__(256) void k_sell_spmv3(int n_slices, int sell_blocks, int n, int active, size_t ld,
+                                                    const int* __restrict__ sptr, const int* __restrict__ rowmap,
+                                                    const int* __restrict__ ci, const double* __restrict__ va,
+                                                    const double* __restrict__ x, double* __restrict__ y, double a) {
+  if ((int)blockIdx.x < sell_blocks) {
+    const int per = sell_blocks >> 3;
+    const int vb = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    const int lane = threadIdx.x & 63;
+    const int slice = vb * 4 + (int)(threadIdx.x >> 6);
+    if (slice >= n_slices) return;
+    const int r = rowmap[slice * 64 + lane];
+    const int s0 = sptr[slice];
+    const int width = (sptr[slice + 1] - s0) >> 6;
+    const int* cp = ci + s0 + lane;
+    const double* vp = va + s0 + lane;
+    double a0 = 0, a1 = 0, a2 = 0;
+    int j = 0;
+    for (; j + 4 <= width; j += 4) {
+      int c[4];
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c[u] = __builtin_nontemporal_load(cp + (j + u) * 64);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(vp + (j + u) * 64);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a0 += v[u] * x[c[u]];
+        a1 += v[u] * x[ld + c[u]];
+        a2 += v[u] * x[2 * ld + c[u]];
+      }
+    }
+    for (; j < width; ++j) {
+      const int c = __builtin_nontemporal_load(cp + j * 64);
+      const double v = __builtin_nontemporal_load(vp + j * 64);
+      a0 += v * x[c];
+      a1 += v * x[ld + c];
+      a2 += v * x[2 * ld + c];
+    }
+    if (r >= 0) {
+      y[r] += a * a0;
+      y[ld + r] += a * a1;
+      y[2 * ld + r] += a * a2;
+    }
+  } else {
+    const int i = active + (int)(blockIdx.x - sell_blocks) * blockDim.x + threadIdx.x;
+    if (i < n) {
+      y[i] += a * x[i];
+      y[ld + i] += a * x[ld + i];
+      y[2 * ld + i] += a * x[2 * ld + i];
+    }
+  }
+}
+
 // MODE_JACOBI of k_sell_spmv for an x that vanishes on the active rows: the
 // products with those zeros drop out of each row sum exactly (a*0 added to a
 // partial sum leaves it unchanged), so only the row's Dirichlet-column
@@ -665,6 +722,16 @@ int grid_for(int n, int block = 256, int cap = 4096) {
 void fe_apply_scaled_add(const DevCSR& A, const double* x, double* y, double a, hipStream_t st) {
   dispatch_fe_spmv<MODE_SCALED_ADD, double, double>(A, A.val, nullptr, x, nullptr, nullptr, y, a,
                                                     nullptr, st);
+}
+void fe_apply_scaled_add3(const DevCSR& A, const double* x, double* y, size_t ld, double a, hipStream_t st) {
+  if (A.sell.n_slices == 0 || !A.sell.val || A.sell.off || A.lpr != 0) {
+    for (int k = 0; k < 3; ++k) fe_apply_scaled_add(A, x + k * ld, y + k * ld, a, st);
+    return;
+  }
+  int sb = 0;
+  const int grid = sell_grid(A, &sb);
+  hipLaunchKernelGGL(k_sell_spmv3, dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb, A.n, A.active, ld,
+                     A.sell.slice_ptr, A.sell.row, A.sell.col, A.sell.val, x, y, a);
 }
 void fe_multiply_active(const DevCSR& A, const double* x, double* y, double a, hipStream_t st) {
   DevCSR Aa = A;

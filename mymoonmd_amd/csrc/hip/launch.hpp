@@ -62,6 +62,8 @@ void sell_fill_bf16(const DevCSR& A, uint16_t* sell_bf16, hipStream_t st);
 
 // ---- FE-matrix operators (FP64)
 void fe_apply_scaled_add(const DevCSR& A, const double* x, double* y, double a, hipStream_t st);
+// the same for x + k ld, y + k ld, k = 0, 1, 2 (one pass over A)
+void fe_apply_scaled_add3(const DevCSR& A, const double* x, double* y, size_t ld, double a, hipStream_t st);
 void fe_apply(const DevCSR& A, const double* x, double* y, hipStream_t st);
 int fe_defect_partials(const DevCSR& A);
 void fe_defect_f32val(const DevCSR& A, const float* val32, const double* x, const double* b, double* d,

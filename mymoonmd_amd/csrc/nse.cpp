@@ -91,8 +91,9 @@ pmx::DevCSR nse_combined(const pmx_nse* s) {
 void nse_apply_scaled_add(const pmx_nse* s, const double* x, double* y, double a, hipStream_t st) {
   const int nu = s->n_u;
   ensure_sell_values(s->A, st);
+  // A on the three velocity components in one pass over A
+  pmx::fe_apply_scaled_add3(s->A->A, x, y, (size_t)nu, a, st);
   for (int k = 0; k < 3; ++k) {
-    pmx::fe_apply_scaled_add(s->A->A, x + (size_t)k * nu, y + (size_t)k * nu, a, st);
     // B_k^T p on the active velocity rows (FEMatrix::multiplyTransposedActive)
     pmx::csr_spmv(nse_bt(s, k), x + (size_t)3 * nu, y + (size_t)k * nu, a, 1.0, s->n_u_active, 0, st);
   }
