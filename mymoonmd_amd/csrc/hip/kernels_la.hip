@@ -178,14 +178,20 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
         }
       }
       // the folded bf16 sweep keeps sixteen entries in flight per lane
-      constexpr bool wide = REL16 && FOLD && PACK == 1 && sizeof(VT) == 2;
+      constexpr bool wide = PACK == 1 && FOLD;
       for (; wide && j + 16 <= width; j += 16) {
         int c[16];
         AT v[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) c[u] = rb + (int)__builtin_nontemporal_load(op + (j + u) * 64);
+        for (int u = 0; u < 16; ++u) {
+          if constexpr (REL16) c[u] = rb + (int)__builtin_nontemporal_load(op + (j + u) * 64);
+          else c[u] = __builtin_nontemporal_load(cp + (j + u) * 64);
+        }
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = (AT)bf16_to_float(__builtin_nontemporal_load(vp + (j + u) * 64));
+        for (int u = 0; u < 16; ++u) {
+          if constexpr (sizeof(VT) == 2) v[u] = (AT)bf16_to_float(__builtin_nontemporal_load(vp + (j + u) * 64));
+          else v[u] = (AT)__builtin_nontemporal_load(vp + (j + u) * 64);
+        }
 #pragma unroll
         for (int u = 0; u < 16; ++u) acc += v[u] * (AT)x[c[u]];
       }
