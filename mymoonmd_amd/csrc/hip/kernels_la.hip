@@ -566,6 +566,52 @@ __global__ __launch_bounds__(256) void k_sell_transfer(int n_slices, int sell_bl
   }
 }
 
+// The same transfer for three vectors (x + k ldx -> y + k ldy, k = 0, 1, 2):
+// the velocity components of a saddle-point level share one interpolation
+// matrix, read once here; each component's row sum as in k_csr_spmv.
+template <int LPR, typename VT>
+__global__ __launch_bounds__(256) void k_csr_spmv3(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                    const VT* __restrict__ va, const double* __restrict__ x,
+                                                    size_t ldx, double* __restrict__ y, size_t ldy, double alpha,
+                                                    double beta, int row_end, int zero_rest) {
+  const int lane = threadIdx.x & (LPR - 1);
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  if (row >= n) return;
+  if (row >= row_end) {
+    if (zero_rest && lane == 0) {
+      y[row] = 0.0;
+      y[ldy + row] = 0.0;
+      y[2 * ldy + row] = 0.0;
+    }
+    return;
+  }
+  double a0 = 0, a1 = 0, a2 = 0;
+  const int s = rp[row], e = rp[row + 1];
+  for (int k = s + lane; k < e; k += LPR) {
+    double v;
+    if constexpr (sizeof(VT) == 2) v = (double)bf16_to_float(va[k]);
+    else v = va[k];
+    const int c = ci[k];
+    a0 += v * x[c];
+    a1 += v * x[ldx + c];
+    a2 += v * x[2 * ldx + c];
+  }
+  a0 = group_sum<LPR>(a0);
+  a1 = group_sum<LPR>(a1);
+  a2 = group_sum<LPR>(a2);
+  if (lane == 0) {
+    double r0 = alpha * a0, r1 = alpha * a1, r2 = alpha * a2;
+    if (beta != 0.0) {
+      r0 += beta * y[row];
+      r1 += beta * y[ldy + row];
+      r2 += beta * y[2 * ldy + row];
+    }
+    y[row] = r0;
+    y[ldy + row] = r1;
+    y[2 * ldy + row] = r2;
+  }
+}
+
 // ---------------------------------------------------------------- BLAS-1
 constexpr int kRedBlocks = 1024;
 
@@ -930,6 +976,27 @@ void csr_spmv_t(const DevCSR& P, const T* x, T* y, double alpha, double beta, in
     case 2: h ? go(k_csr_spmv<2, T, uint16_t>, P.val_bf16) : go(k_csr_spmv<2, T, double>, P.val); break;
     case 4: h ? go(k_csr_spmv<4, T, uint16_t>, P.val_bf16) : go(k_csr_spmv<4, T, double>, P.val); break;
     default: h ? go(k_csr_spmv<8, T, uint16_t>, P.val_bf16) : go(k_csr_spmv<8, T, double>, P.val);
+  }
+}
+void csr_spmv3(const DevCSR& P, const double* x, size_t ldx, double* y, size_t ldy, double alpha, double beta,
+               int row_end, int zero_rest, hipStream_t st) {
+  if (P.sell.n_slices > 0) {   // the SELL transfer copy: one vector at a time
+    for (int k = 0; k < 3; ++k) csr_spmv(P, x + k * ldx, y + k * ldy, alpha, beta, row_end, zero_rest, st);
+    return;
+  }
+  const int lpr = (P.lpr == 1 || P.lpr == 2 || P.lpr == 4) ? P.lpr : 8;
+  const int grid = (P.n + (256 / lpr) - 1) / (256 / lpr);
+  if (grid == 0) return;
+  const bool h = P.val_bf16 && !P.rscale;
+  auto go = [&](auto kern, const auto* va) {
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, st, P.n, P.row_ptr, P.col, va, x, ldx, y, ldy, alpha, beta,
+                       row_end, zero_rest);
+  };
+  switch (lpr) {
+    case 1: h ? go(k_csr_spmv3<1, uint16_t>, P.val_bf16) : go(k_csr_spmv3<1, double>, P.val); break;
+    case 2: h ? go(k_csr_spmv3<2, uint16_t>, P.val_bf16) : go(k_csr_spmv3<2, double>, P.val); break;
+    case 4: h ? go(k_csr_spmv3<4, uint16_t>, P.val_bf16) : go(k_csr_spmv3<4, double>, P.val); break;
+    default: h ? go(k_csr_spmv3<8, uint16_t>, P.val_bf16) : go(k_csr_spmv3<8, double>, P.val);
   }
 }
 void csr_spmv(const DevCSR& P, const double* x, double* y, double alpha, double beta, int row_end,

@@ -95,6 +95,9 @@ bool fe_jacobi_from_zero_bf16(const DevCSR& A, const float* x, const float* b, c
                               double omega, hipStream_t st);
 // ---- plain CSR (transfer matrices): y = alpha*P x + beta*y on rows < row_end;
 //      rows >= row_end are set to 0 when zero_rest, else left unchanged
+// three vectors x + k ldx -> y + k ldy (k = 0, 1, 2) with one pass over P
+void csr_spmv3(const DevCSR& P, const double* x, size_t ldx, double* y, size_t ldy, double alpha, double beta,
+               int row_end, int zero_rest, hipStream_t st);
 void csr_spmv(const DevCSR& P, const double* x, double* y, double alpha, double beta, int row_end,
               int zero_rest, hipStream_t st);
 void csr_spmv_f32(const DevCSR& P, const float* x, float* y, double alpha, double beta,

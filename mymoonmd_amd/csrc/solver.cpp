@@ -505,16 +505,14 @@ void nse_update_solution_in_finer_grid(pmx_solver* s, int l) {
   NseLevel& F = s->nlv[l + 1];
   const double damp = s->p.mg_correction_damp;
   F.sol_zero = false;
-  for (int k = 0; k < 3; ++k)
-    pmx::csr_spmv(C.Pu, C.sol + (size_t)k * C.n_u, F.sol + (size_t)k * F.n_u, damp, 1.0, F.n_ua, 0, S(s));
+  pmx::csr_spmv3(C.Pu, C.sol, (size_t)C.n_u, F.sol, (size_t)F.n_u, damp, 1.0, F.n_ua, 0, S(s));
   pmx::csr_spmv(C.Pp, C.sol + (size_t)3 * C.n_u, F.sol + (size_t)3 * F.n_u, damp, 1.0, F.n_p, 0, S(s));
 }
 
 void nse_update_rhs_in_coarser_grid(pmx_solver* s, int l) {
   NseLevel& F = s->nlv[l];
   NseLevel& C = s->nlv[l - 1];
-  for (int k = 0; k < 3; ++k)
-    pmx::csr_spmv(C.Ru, F.def + (size_t)k * F.n_u, C.rhs + (size_t)k * C.n_u, 1.0, 0.0, C.n_ua, 1, S(s));
+  pmx::csr_spmv3(C.Ru, F.def, (size_t)F.n_u, C.rhs, (size_t)C.n_u, 1.0, 0.0, C.n_ua, 1, S(s));
   pmx::csr_spmv(C.Rp, F.def + (size_t)3 * F.n_u, C.rhs + (size_t)3 * C.n_u, 1.0, 0.0, C.n_p, 1, S(s));
 }
 
