@@ -802,6 +802,7 @@ void fe_defect_f32val(const DevCSR& A, const float* val32, const double* x, cons
   switch (A.lpr) {
     case 4: launch_fe_spmv<4, MODE_DEFECT>(A, val32, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
     case 16: launch_fe_spmv<16, MODE_DEFECT>(A, val32, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
+    case 32: launch_fe_spmv<32, MODE_DEFECT>(A, val32, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
     default: launch_fe_spmv<8, MODE_DEFECT>(A, val32, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
   }
 }

@@ -78,9 +78,10 @@ pmx::DevCSR nse_combined(const pmx_nse* s) {
   C.n = C.m = s->n_comb;
   C.nnz = (int)s->Cp.col.size();
   C.active = s->n_comb;
-  // 8 lanes per row: the rows are long (83 entries on average at 32^3) and
-  // few, one row per lane (SELL) leaves too little parallelism per wave
-  C.lpr = 8;
+  // 16 lanes per row: the rows are long (83 entries on average at 32^3) and
+  // few; one row per lane (SELL) leaves too little parallelism per wave, 8 or
+  // 32 lanes per row measured slower on MI355X
+  C.lpr = 16;
   C.row_ptr = s->d_Crp;
   C.col = s->d_Ccol;
   C.val = s->d_Cval;
