@@ -612,6 +612,64 @@ __global__ __launch_bounds__(256) void k_csr_spmv3(int n, const int* __restrict_
   }
 }
 
+// Divergence blocks of an NSTYPE-1 saddle-point operator, all three at once
+// (one pattern, values v1 v2 v3):
+//   k_csr_b3:  y[r] += a (B1 x1 + B2 x2 + B3 x3)[r]   (x_k = x + k ldx)
+//   k_csr_bt3: y_k[r] += a (B_k^T p)[r]               (y_k = y + k ldy)
+// for rows r < row_end; the pattern and the gathered p are read once, and
+// every product is summed and added to y in the order of three k_csr_spmv
+// calls (bitwise the same).
+template <int LPR>
+__global__ __launch_bounds__(256) void k_csr_b3(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                const double* __restrict__ v1, const double* __restrict__ v2,
+                                                const double* __restrict__ v3, const double* __restrict__ x,
+                                                size_t ldx, double* __restrict__ y, double a, int row_end) {
+  const int lane = threadIdx.x & (LPR - 1);
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  if (row >= n || row >= row_end) return;
+  double a0 = 0, a1 = 0, a2 = 0;
+  const int s = rp[row], e = rp[row + 1];
+  for (int k = s + lane; k < e; k += LPR) {
+    const int c = ci[k];
+    a0 += v1[k] * x[c];
+    a1 += v2[k] * x[ldx + c];
+    a2 += v3[k] * x[2 * ldx + c];
+  }
+  a0 = group_sum<LPR>(a0);
+  a1 = group_sum<LPR>(a1);
+  a2 = group_sum<LPR>(a2);
+  if (lane == 0) {
+    double t = a * a0 + y[row];
+    t = a * a1 + t;
+    y[row] = a * a2 + t;
+  }
+}
+template <int LPR>
+__global__ __launch_bounds__(256) void k_csr_bt3(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                 const double* __restrict__ v1, const double* __restrict__ v2,
+                                                 const double* __restrict__ v3, const double* __restrict__ p,
+                                                 double* __restrict__ y, size_t ldy, double a, int row_end) {
+  const int lane = threadIdx.x & (LPR - 1);
+  const int row = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  if (row >= n || row >= row_end) return;
+  double a0 = 0, a1 = 0, a2 = 0;
+  const int s = rp[row], e = rp[row + 1];
+  for (int k = s + lane; k < e; k += LPR) {
+    const double pc = p[ci[k]];
+    a0 += v1[k] * pc;
+    a1 += v2[k] * pc;
+    a2 += v3[k] * pc;
+  }
+  a0 = group_sum<LPR>(a0);
+  a1 = group_sum<LPR>(a1);
+  a2 = group_sum<LPR>(a2);
+  if (lane == 0) {
+    y[row] = a * a0 + y[row];
+    y[ldy + row] = a * a1 + y[ldy + row];
+    y[2 * ldy + row] = a * a2 + y[2 * ldy + row];
+  }
+}
+
 // ---------------------------------------------------------------- BLAS-1
 constexpr int kRedBlocks = 1024;
 
@@ -978,6 +1036,20 @@ void csr_spmv_t(const DevCSR& P, const T* x, T* y, double alpha, double beta, in
     case 4: h ? go(k_csr_spmv<4, T, uint16_t>, P.val_bf16) : go(k_csr_spmv<4, T, double>, P.val); break;
     default: h ? go(k_csr_spmv<8, T, uint16_t>, P.val_bf16) : go(k_csr_spmv<8, T, double>, P.val);
   }
+}
+void csr_b3(const DevCSR& B, const double* v2, const double* v3, const double* x, size_t ldx, double* y, double a,
+            int row_end, hipStream_t st) {
+  const int grid = (B.n + 31) / 32;   // 8 lanes per row
+  if (grid > 0)
+    hipLaunchKernelGGL(k_csr_b3<8>, dim3(grid), dim3(256), 0, st, B.n, B.row_ptr, B.col, B.val, v2, v3, x, ldx, y, a,
+                       row_end);
+}
+void csr_bt3(const DevCSR& T, const double* v2, const double* v3, const double* p, double* y, size_t ldy, double a,
+             int row_end, hipStream_t st) {
+  const int grid = (T.n + 31) / 32;
+  if (grid > 0)
+    hipLaunchKernelGGL(k_csr_bt3<8>, dim3(grid), dim3(256), 0, st, T.n, T.row_ptr, T.col, T.val, v2, v3, p, y, ldy,
+                       a, row_end);
 }
 void csr_spmv3(const DevCSR& P, const double* x, size_t ldx, double* y, size_t ldy, double alpha, double beta,
                int row_end, int zero_rest, hipStream_t st) {

@@ -95,6 +95,12 @@ bool fe_jacobi_from_zero_bf16(const DevCSR& A, const float* x, const float* b, c
                               double omega, hipStream_t st);
 // ---- plain CSR (transfer matrices): y = alpha*P x + beta*y on rows < row_end;
 //      rows >= row_end are set to 0 when zero_rest, else left unchanged
+// divergence blocks of a saddle-point operator (values B.val, v2, v3 on one
+// pattern): y += a sum_k B_k (x + k ldx) and (y + k ldy) += a B_k^T p
+void csr_b3(const DevCSR& B, const double* v2, const double* v3, const double* x, size_t ldx, double* y, double a,
+            int row_end, hipStream_t st);
+void csr_bt3(const DevCSR& T, const double* v2, const double* v3, const double* p, double* y, size_t ldy, double a,
+             int row_end, hipStream_t st);
 // three vectors x + k ldx -> y + k ldy (k = 0, 1, 2) with one pass over P
 void csr_spmv3(const DevCSR& P, const double* x, size_t ldx, double* y, size_t ldy, double alpha, double beta,
                int row_end, int zero_rest, hipStream_t st);

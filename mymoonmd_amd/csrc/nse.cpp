@@ -94,12 +94,11 @@ void nse_apply_scaled_add(const pmx_nse* s, const double* x, double* y, double a
   ensure_sell_values(s->A, st);
   // A on the three velocity components in one pass over A
   pmx::fe_apply_scaled_add3(s->A->A, x, y, (size_t)nu, a, st);
-  for (int k = 0; k < 3; ++k) {
-    // B_k^T p on the active velocity rows (FEMatrix::multiplyTransposedActive)
-    pmx::csr_spmv(nse_bt(s, k), x + (size_t)3 * nu, y + (size_t)k * nu, a, 1.0, s->n_u_active, 0, st);
-  }
-  for (int k = 0; k < 3; ++k)
-    pmx::csr_spmv(nse_b(s, k), x + (size_t)k * nu, y + (size_t)3 * nu, a, 1.0, s->n_p, 0, st);
+  // B_k^T p on the active velocity rows (FEMatrix::multiplyTransposedActive),
+  // then B_1 u_1 + B_2 u_2 + B_3 u_3 on the pressure rows, each with one
+  // pass over its pattern
+  pmx::csr_bt3(nse_bt(s, 0), s->d_BT2, s->d_BT3, x + (size_t)3 * nu, y, (size_t)nu, a, s->n_u_active, st);
+  pmx::csr_b3(nse_b(s, 0), s->d_B2, s->d_B3, x, (size_t)nu, y + (size_t)3 * nu, a, s->n_p, st);
 }
 
 void nse_refresh(pmx_nse* s, hipStream_t st) {
