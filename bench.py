@@ -243,7 +243,7 @@ def oseen_solve(pm, torch, ctx, nref=5, nlev=4, nu=1e-2, vanka_damp=0.2, steps=2
         out[prec] = {"time_to_solution_s": round(min(ts), 4), "iterations": rep.iterations,
                      "fp64_rel_residual": torch.linalg.norm(r).item() / r0n, "solver_update_s": round(t_up, 3)}
         del sol
-    out["speedup_fp32_vanka_vs_fp64"] = round(out["fp64"]["time_to_solution_s"] / out["mixed"]["time_to_solution_s"], 3)
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["time_to_solution_s"] / out["mixed"]["time_to_solution_s"], 3)
     return out
 
 

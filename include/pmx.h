@@ -532,7 +532,8 @@ typedef struct {
   int mg_coarse_max_n_iterations;
   double jacobi_damp;          /* jacobi_damp */
   int precision;               /* PMX_PRECISION_* (multigrid preconditioner only; for
-                                  saddle-point systems MIXED stores the Vanka factors in FP32) */
+                                  saddle-point systems MIXED stores the Vanka factors in FP32,
+                                  or bf16 with vanka_inverse 2) */
   double vanka_damp;           /* multigrid_vanka_damp_factor (default 1) */
   int zero_start_shortcut;     /* no reference counterpart.  1 (default): a Jacobi
                                   smoothing step on a level whose solution is zero on the
@@ -565,12 +566,15 @@ typedef struct {
                                   when the product has at least 131,072 rows, else CSR
                                   with 1-8 lanes per row (partial sums combined across
                                   lanes).  2: SELL for every level.  0: CSR throughout */
-  int vanka_inverse;           /* no reference counterpart.  1 (default): the cell_vanka_jacobi
+  int vanka_inverse;           /* no reference counterpart.  1: the cell_vanka_jacobi
                                   smoother stores the inverse of each cell's local matrix
                                   (from its LU factors, in place, as dgetri) and applies it
                                   by one product; 0: it stores the LU factors and substitutes
                                   (the reference's dgetrs).  Same smoother; the rounding
-                                  differs. */
+                                  differs.  2 (default): as 1, and with precision mixed the
+                                  local inverses and the values of the smoother's defect
+                                  f - K x are bf16 instead of FP32 (rounded from FP32;
+                                  FP64 vectors and sums). */
 } pmx_solver_params;
 /* Defaults of Solver<>::default_solver_database (src/Solver/Solver.C:20-117)
  * merged with Multigrid::default_multigrid_database (src/Multigrid/Multigrid.C:422-514). */

@@ -864,6 +864,16 @@ void fe_defect_f32val(const DevCSR& A, const float* val32, const double* x, cons
     default: launch_fe_spmv<8, MODE_DEFECT>(A, val32, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
   }
 }
+// the same with bf16 copies of the values
+void fe_defect_bf16val(const DevCSR& A, const uint16_t* val16, const double* x, const double* b, double* d,
+                       double* partial, hipStream_t st) {
+  switch (A.lpr) {
+    case 4: launch_fe_spmv<4, MODE_DEFECT>(A, val16, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
+    case 16: launch_fe_spmv<16, MODE_DEFECT>(A, val16, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
+    case 32: launch_fe_spmv<32, MODE_DEFECT>(A, val16, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
+    default: launch_fe_spmv<8, MODE_DEFECT>(A, val16, nullptr, x, b, (const double*)nullptr, d, 1.0, partial, st); break;
+  }
+}
 // number of blocks (= partial sums) the FE-matrix kernel launches for A
 static int spmv_grid(const DevCSR& A, bool bf16) {
   const bool have_sell = bf16 ? (A.sell.val_bf16 != nullptr) : (A.sell.val != nullptr);

@@ -491,9 +491,14 @@ __global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const T* __res
   if (h1) upd[(size_t)cell * NL + i1] = damp * b1;
 }
 
+__device__ __forceinline__ double vk_entry(double v) { return v; }
+__device__ __forceinline__ double vk_entry(float v) { return (double)v; }
+__device__ __forceinline__ double vk_entry(uint16_t v) { return (double)bf16_to_float(v); }
+
 // the same with the explicit local inverse: x = Inv b for the permuted local
 // defect b (column-major Inv; the 89 columns are independent products, so the
-// loads of the next columns are in flight while the current ones are summed)
+// loads of the next columns are in flight while the current ones are summed).
+// T = uint16_t: bf16 entries
 template <typename T>
 __global__ __launch_bounds__(256) void k_vanka_apply(int n_cells, const T* __restrict__ inv,
                                                      const int* __restrict__ pmap,
@@ -524,17 +529,26 @@ __global__ __launch_bounds__(256) void k_vanka_apply(int n_cells, const T* __res
     for (int u = 0; u < U; ++u) {
       const int k = j + u;
       const double bk = lane_value(k < 64 ? b0 : b1, k & 63);
-      x0 += (double)c0[u] * bk;
-      x1 += (double)c1[u] * bk;
+      x0 += vk_entry(c0[u]) * bk;
+      x1 += vk_entry(c1[u]) * bk;
     }
   }
   for (; j < NL; ++j) {
     const double bk = lane_value(j < 64 ? b0 : b1, j & 63);
-    x0 += (double)X[(size_t)j * NL + i0] * bk;
-    x1 += (double)X[(size_t)j * NL + j1] * bk;
+    x0 += vk_entry(X[(size_t)j * NL + i0]) * bk;
+    x1 += vk_entry(X[(size_t)j * NL + j1]) * bk;
   }
   upd[(size_t)cell * NL + i0] = damp * x0;
   if (h1) upd[(size_t)cell * NL + i1] = damp * x1;
+}
+
+// FP32 values rounded to bf16 (nearest even)
+__global__ void k_f32_to_bf16(long long n, const float* __restrict__ a, uint16_t* __restrict__ b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    uint32_t u = __float_as_uint(a[i]);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    b[i] = (uint16_t)(u >> 16);
+  }
 }
 
 // VankaSmoother CELL (multiplicative) over the cells of one colour: cells of a
@@ -618,6 +632,17 @@ void vanka_apply_inverse(int n_cells, const double* inv64, const float* inv32, c
                                 damp, upd);
   else hipLaunchKernelGGL((k_vanka_apply<double>), dim3(grid), dim3(256), 0, st, n_cells, inv64, pmap, r, damp,
                           upd);
+}
+
+void vanka_apply_inverse_bf16(int n_cells, const uint16_t* inv16, const int* pmap, const double* r, double damp,
+                              double* upd, hipStream_t st) {
+  if (n_cells <= 0) return;
+  hipLaunchKernelGGL((k_vanka_apply<uint16_t>), dim3((n_cells + 3) / 4), dim3(256), 0, st, n_cells, inv16, pmap, r,
+                     damp, upd);
+}
+
+void f32_to_bf16(long long n, const float* a, uint16_t* b, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL(k_f32_to_bf16, dim3(4096), dim3(256), 0, st, n, a, b);
 }
 
 void vanka_colour(int n, const int* cells, const double* lu64, const float* lu32, const int* pmap, const int* map,

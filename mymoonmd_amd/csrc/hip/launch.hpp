@@ -68,6 +68,8 @@ void fe_apply(const DevCSR& A, const double* x, double* y, hipStream_t st);
 int fe_defect_partials(const DevCSR& A);
 void fe_defect_f32val(const DevCSR& A, const float* val32, const double* x, const double* b, double* d,
                       double* partial, hipStream_t st);
+void fe_defect_bf16val(const DevCSR& A, const uint16_t* val16, const double* x, const double* b, double* d,
+                       double* partial, hipStream_t st);
 void fe_defect(const DevCSR& A, const double* x, const double* b, double* d, double* partial,
                double* norm_out, hipStream_t st);
 void fe_jacobi(const DevCSR& A, const double* x, const double* b, const double* dinv, double* xn,
@@ -226,6 +228,11 @@ void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, fl
 // Jacobi Vanka step with the explicit local inverses (vanka_factor with inverse)
 void vanka_apply_inverse(int n_cells, const double* inv64, const float* inv32, const int* pmap, const double* r,
                          double damp, double* upd, hipStream_t st);
+// the same with bf16 local inverses (f32_to_bf16 of the FP32 ones)
+void vanka_apply_inverse_bf16(int n_cells, const uint16_t* inv16, const int* pmap, const double* r, double damp,
+                              double* upd, hipStream_t st);
+// b = a rounded to bf16 (nearest even)
+void f32_to_bf16(long long n, const float* a, uint16_t* b, hipStream_t st);
 // upd[cell][i] = damp * (local solution) for local DOF i (the order of map)
 void vanka_solve(int n_cells, const double* lu64, const float* lu32, const int* pmap, const double* r,
                  double damp, double* upd, hipStream_t st);

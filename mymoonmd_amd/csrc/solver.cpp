@@ -154,7 +154,9 @@ struct NseLevel {
   int *map = nullptr, *pmap = nullptr, *acc_ptr = nullptr, *acc_idx = nullptr;
   double* lu64 = nullptr;
   float* lu32 = nullptr;
+  uint16_t* lu16 = nullptr;    // bf16 local inverses (vanka_inverse 2, mixed precision)
   float* cval32 = nullptr;     // mixed precision: FP32 copy of the combined values (smoother defects)
+  uint16_t* cval16 = nullptr;  // vanka_inverse 2: bf16 copy instead
   bool inv = false;            // lu64/lu32 hold the local inverses instead of the LU factors
   double *upd = nullptr, *rhs = nullptr, *sol = nullptr, *def = nullptr, *res = nullptr;
   pmx::DevCSR Pu{}, Ru{}, Pp{}, Rp{};   // P: finer rows from this level; R = P^T (rows: this level)
@@ -165,7 +167,7 @@ struct NseLevel {
   int* colour_cells = nullptr;           // cells in colour-major order (27 colours)
   std::vector<int> colour_ptr;
   void release() {
-    dfree(map); dfree(pmap); dfree(acc_ptr); dfree(acc_idx); dfree(lu64); dfree(lu32); dfree(upd); dfree(cval32);
+    dfree(map); dfree(pmap); dfree(acc_ptr); dfree(acc_idx); dfree(lu64); dfree(lu32); dfree(lu16); dfree(upd); dfree(cval32); dfree(cval16);
     dfree(rhs); dfree(sol); dfree(def); dfree(res);
     free_csr(Pu); free_csr(Ru); free_csr(Pp); free_csr(Rp);
     dfree(dinv_dense); dfree(partial); dfree(colour_cells);
@@ -483,12 +485,14 @@ void nse_smooth(pmx_solver* s, int l) {
   // (K 0 sums signed zeros), so the product is skipped
   const double* r = L.rhs;
   if (!(L.sol_zero && s->p.zero_start_shortcut)) {
-    if (L.cval32) pmx::fe_defect_f32val(nse_combined(L.sys), L.cval32, L.sol, L.rhs, L.res, L.partial, S(s));
+    if (L.cval16) pmx::fe_defect_bf16val(nse_combined(L.sys), L.cval16, L.sol, L.rhs, L.res, L.partial, S(s));
+    else if (L.cval32) pmx::fe_defect_f32val(nse_combined(L.sys), L.cval32, L.sol, L.rhs, L.res, L.partial, S(s));
     else pmx::fe_defect(nse_combined(L.sys), L.sol, L.rhs, L.res, L.partial, nullptr, S(s));
     r = L.res;
   }
   L.sol_zero = false;
-  if (L.inv) pmx::vanka_apply_inverse(L.n_cells, L.lu64, L.lu32, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
+  if (L.lu16) pmx::vanka_apply_inverse_bf16(L.n_cells, L.lu16, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
+  else if (L.inv) pmx::vanka_apply_inverse(L.n_cells, L.lu64, L.lu32, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
   else pmx::vanka_solve(L.n_cells, L.lu64, L.lu32, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
   pmx::vanka_accumulate(L.n, L.acc_ptr, L.acc_idx, L.upd, L.sol, S(s));
 }
@@ -887,7 +891,7 @@ int pmx_solver_default_params(pmx_solver_params* p) {
     d.zero_start_shortcut = 1;
     d.renumber_levels = 3;
     d.transfer_sell = 1;
-    d.vanka_inverse = 1;
+    d.vanka_inverse = 2;
     *p = d;
   });
 }
@@ -1226,11 +1230,25 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
           const int kind = (l == 0) ? s->p.mg_smoother_coarse : s->p.mg_smoother;
           L.inv = s->p.vanka_inverse != 0 && kind == PMX_SMOOTHER_CELL_VANKA_JACOBI;
           pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.pmap, sing, st, L.inv);
+          if (f32 && L.inv && s->p.vanka_inverse == 2) {
+            L.lu16 = dmalloc<uint16_t>(nlu);
+            pmx::f32_to_bf16((long long)nlu, L.lu32, L.lu16, st);
+            HIP_CHECK(hipStreamSynchronize(st));
+            dfree(L.lu32);
+            L.lu32 = nullptr;
+          }
           if (f32 && kind == PMX_SMOOTHER_CELL_VANKA_JACOBI) {
             // the smoother's defects read FP32 values (FP64 vectors and sums)
             const pmx::DevCSR C = nse_combined(sys);
             L.cval32 = dmalloc<float>(std::max(1, C.nnz));
             pmx::vec_f64_to_f32(C.nnz, C.val, L.cval32, st);
+            if (L.lu16) {
+              L.cval16 = dmalloc<uint16_t>(std::max(1, C.nnz));
+              pmx::f32_to_bf16(C.nnz, L.cval32, L.cval16, st);
+              HIP_CHECK(hipStreamSynchronize(st));
+              dfree(L.cval32);
+              L.cval32 = nullptr;
+            }
           }
           if (kind == PMX_SMOOTHER_CELL_VANKA) {
             // colour = cell lattice index mod 3 per direction: cells of one

@@ -337,3 +337,36 @@ def test_nse_multiplicative_vanka_solves_faster_than_jacobi_variant(ctx):
         its[smoother] = rep.iterations
     print("iterations", its, flush=True)
     assert its["cell_vanka"] < its["cell_vanka_jacobi"], its
+
+
+def test_nse_vanka_bf16_local_inverses(ctx):
+    """Mixed precision with vanka_inverse 2 (bf16 local inverses and bf16
+    values in the smoother's defect) against vanka_inverse 1 (FP32): one cycle
+    differs only by bf16 rounding (relative 2e-2 at most), and FGMRES reaches
+    the same solution (1e-7, pressures of mean zero) in at most one more
+    iteration."""
+    d, gl, r = gpu_levels(ctx, 3, 3)
+    g = gl[-1]
+    kw = dict(MG_KW, multigrid_vanka_damp_factor=0.3)
+    v = dev(np.random.default_rng(7).standard_normal(g.n))
+    out = {}
+    for inv in (1, 2):
+        print("case vanka_inverse", inv, flush=True)
+        sol = pm.Solver(ctx, pm.solver_params(residual_tolerance=0.0, residual_reduction=1e-10, max_n_iterations=200,
+                                              precision="mixed", vanka_inverse=inv, **kw))
+        sol.update_saddle(gl)
+        z = g.zeros()
+        sol.apply_preconditioner(v.clone(), z)
+        x = g.zeros()
+        nu, na = g.n_u, g.n_u_active
+        for k in range(3):
+            x[k * nu + na:(k + 1) * nu] = r[k * nu + na:(k + 1) * nu]
+        rep = sol.solve(r, x)
+        assert rep.converged
+        g.project_pressure(x)   # the pressure is fixed up to a constant
+        out[inv] = (z.cpu().numpy(), x.cpu().numpy(), rep.iterations)
+    a, b = out[1], out[2]
+    dz = np.max(np.abs(a[0] - b[0])) / np.max(np.abs(a[0]))
+    assert 0 < dz <= 2e-2, dz
+    assert b[2] <= a[2] + 1, (a[2], b[2])
+    assert np.max(np.abs(a[1] - b[1])) <= 1e-7 * np.max(np.abs(a[1]))
