@@ -493,12 +493,10 @@ __global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const T* __res
 
 __device__ __forceinline__ double vk_entry(double v) { return v; }
 __device__ __forceinline__ double vk_entry(float v) { return (double)v; }
-__device__ __forceinline__ double vk_entry(uint16_t v) { return (double)bf16_to_float(v); }
 
 // the same with the explicit local inverse: x = Inv b for the permuted local
 // defect b (column-major Inv; the 89 columns are independent products, so the
-// loads of the next columns are in flight while the current ones are summed).
-// T = uint16_t: bf16 entries
+// loads of the next columns are in flight while the current ones are summed)
 template <typename T>
 __global__ __launch_bounds__(256) void k_vanka_apply(int n_cells, const T* __restrict__ inv,
                                                      const int* __restrict__ pmap,
@@ -540,6 +538,69 @@ __global__ __launch_bounds__(256) void k_vanka_apply(int n_cells, const T* __res
   }
   upd[(size_t)cell * NL + i0] = damp * x0;
   if (h1) upd[(size_t)cell * NL + i1] = damp * x1;
+}
+
+// bf16 local inverses with columns padded to NLP = 90 entries (4-byte
+// aligned): lane l < 45 owns local rows 2l and 2l+1 and reads both entries of
+// a column with one 4-byte load (89 loads per lane instead of 178); the
+// padding row 89 is zero and never written back
+constexpr int NLP = NL + 1;
+__global__ __launch_bounds__(256) void k_vanka_apply_bf16p(int n_cells, const uint32_t* __restrict__ inv,
+                                                           const int* __restrict__ pmap,
+                                                           const double* __restrict__ r, double damp,
+                                                           double* __restrict__ upd) {
+  constexpr int U = 8, H = NLP / 2;
+  const int lane = threadIdx.x & 63;
+  const int cell = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  if (cell >= n_cells) return;   // uniform for the wavefront
+  const int* pm = pmap + (size_t)cell * NL;
+  const int i1 = lane + 64;
+  const bool h1 = i1 < NL;
+  double b0 = r[pm[lane]];
+  double b1 = r[pm[h1 ? i1 : lane]];
+  if (!h1) b1 = 0.0;
+  const int q = lane < H ? lane : H - 1;
+  const uint32_t* X = inv + (size_t)cell * (NLP / 2) * NL;   // NL columns of H words
+  double x0 = 0.0, x1 = 0.0;
+  int j = 0;
+  for (; j + U <= NL; j += U) {
+    uint32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(X + (size_t)(j + u) * H + q);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = j + u;
+      const double bk = lane_value(k < 64 ? b0 : b1, k & 63);
+      x0 += (double)__uint_as_float(c[u] << 16) * bk;
+      x1 += (double)__uint_as_float(c[u] & 0xFFFF0000u) * bk;
+    }
+  }
+  for (; j < NL; ++j) {
+    const uint32_t c = X[(size_t)j * H + q];
+    const double bk = lane_value(j < 64 ? b0 : b1, j & 63);
+    x0 += (double)__uint_as_float(c << 16) * bk;
+    x1 += (double)__uint_as_float(c & 0xFFFF0000u) * bk;
+  }
+  if (lane < H) {
+    upd[(size_t)cell * NL + 2 * lane] = damp * x0;
+    if (2 * lane + 1 < NL) upd[(size_t)cell * NL + 2 * lane + 1] = damp * x1;
+  }
+}
+
+// FP32 local inverses (cell-major, columns of NL) to the padded bf16 layout
+__global__ void k_inv_to_bf16p(long long n_out, const float* __restrict__ a, uint16_t* __restrict__ b) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n_out;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long col = t / NLP;   // cell * NL + j
+    const int i = (int)(t - col * NLP);
+    uint16_t h = 0;
+    if (i < NL) {
+      uint32_t u = __float_as_uint(a[col * NL + i]);
+      u += 0x7FFFu + ((u >> 16) & 1u);
+      h = (uint16_t)(u >> 16);
+    }
+    b[t] = h;
+  }
 }
 
 // FP32 values rounded to bf16 (nearest even)
@@ -637,8 +698,15 @@ void vanka_apply_inverse(int n_cells, const double* inv64, const float* inv32, c
 void vanka_apply_inverse_bf16(int n_cells, const uint16_t* inv16, const int* pmap, const double* r, double damp,
                               double* upd, hipStream_t st) {
   if (n_cells <= 0) return;
-  hipLaunchKernelGGL((k_vanka_apply<uint16_t>), dim3((n_cells + 3) / 4), dim3(256), 0, st, n_cells, inv16, pmap, r,
-                     damp, upd);
+  hipLaunchKernelGGL(k_vanka_apply_bf16p, dim3((n_cells + 3) / 4), dim3(256), 0, st, n_cells,
+                     reinterpret_cast<const uint32_t*>(inv16), pmap, r, damp, upd);
+}
+
+size_t vanka_bf16_inverse_elems(int n_cells) { return (size_t)n_cells * NL * NLP; }
+
+void vanka_inverse_to_bf16(int n_cells, const float* inv32, uint16_t* inv16, hipStream_t st) {
+  const long long n = (long long)vanka_bf16_inverse_elems(n_cells);
+  if (n > 0) hipLaunchKernelGGL(k_inv_to_bf16p, dim3(4096), dim3(256), 0, st, n, inv32, inv16);
 }
 
 void f32_to_bf16(long long n, const float* a, uint16_t* b, hipStream_t st) {

@@ -228,9 +228,13 @@ void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, fl
 // Jacobi Vanka step with the explicit local inverses (vanka_factor with inverse)
 void vanka_apply_inverse(int n_cells, const double* inv64, const float* inv32, const int* pmap, const double* r,
                          double damp, double* upd, hipStream_t st);
-// the same with bf16 local inverses (f32_to_bf16 of the FP32 ones)
+// the same with bf16 local inverses in the layout of vanka_inverse_to_bf16
 void vanka_apply_inverse_bf16(int n_cells, const uint16_t* inv16, const int* pmap, const double* r, double damp,
                               double* upd, hipStream_t st);
+// FP32 local inverses rounded to bf16, columns padded to 90 entries
+// (vanka_bf16_inverse_elems(n_cells) elements)
+size_t vanka_bf16_inverse_elems(int n_cells);
+void vanka_inverse_to_bf16(int n_cells, const float* inv32, uint16_t* inv16, hipStream_t st);
 // b = a rounded to bf16 (nearest even)
 void f32_to_bf16(long long n, const float* a, uint16_t* b, hipStream_t st);
 // upd[cell][i] = damp * (local solution) for local DOF i (the order of map)

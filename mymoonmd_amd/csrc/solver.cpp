@@ -1231,8 +1231,8 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
           L.inv = s->p.vanka_inverse != 0 && kind == PMX_SMOOTHER_CELL_VANKA_JACOBI;
           pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.pmap, sing, st, L.inv);
           if (f32 && L.inv && s->p.vanka_inverse == 2) {
-            L.lu16 = dmalloc<uint16_t>(nlu);
-            pmx::f32_to_bf16((long long)nlu, L.lu32, L.lu16, st);
+            L.lu16 = dmalloc<uint16_t>(pmx::vanka_bf16_inverse_elems(L.n_cells));
+            pmx::vanka_inverse_to_bf16(L.n_cells, L.lu32, L.lu16, st);
             HIP_CHECK(hipStreamSynchronize(st));
             dfree(L.lu32);
             L.lu32 = nullptr;
