@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
                                                       const int* __restrict__ rp, const int* __restrict__ ci,
                                                       const double* __restrict__ cv, double* __restrict__ lu64,
                                                       float* __restrict__ lu32, int* __restrict__ pmap,
-                                                      int* __restrict__ singular, int inverse) {
+                                                      int* __restrict__ singular, int inverse, int ld) {
   __shared__ double M[NL * LDL];
   __shared__ double tcol[NL];
   __shared__ int dm[NL];
@@ -353,12 +353,13 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
   }
   __syncthreads();
   for (int i = t; i < NL; i += blockDim.x) pmap[(size_t)cell * NL + i] = dm[src[i]];
-  // column-major store: element (i, j) at j * NL + i
-  for (int e = t; e < NL * NL; e += blockDim.x) {
-    const int j = e / NL, i = e % NL;
-    const double v = M[i * LDL + j];
-    if (lu64) lu64[(size_t)cell * NL * NL + e] = v;
-    if (lu32) lu32[(size_t)cell * NL * NL + e] = (float)v;
+  // column-major store: element (i, j) at j * ld + i (ld = NL, or NL + 1
+  // with a zero in the padding row)
+  for (int e = t; e < NL * ld; e += blockDim.x) {
+    const int j = e / ld, i = e % ld;
+    const double v = i < NL ? M[i * LDL + j] : 0.0;
+    if (lu64) lu64[(size_t)cell * NL * ld + e] = v;
+    if (lu32) lu32[(size_t)cell * NL * ld + e] = (float)v;
   }
 }
 
@@ -587,6 +588,51 @@ __global__ __launch_bounds__(256) void k_vanka_apply_bf16p(int n_cells, const ui
   }
 }
 
+// the same for FP64 / FP32 inverses in the padded layout: one 16- / 8-byte
+// load per lane and column
+template <typename T>
+__global__ __launch_bounds__(256) void k_vanka_apply_p(int n_cells, const T* __restrict__ inv,
+                                                       const int* __restrict__ pmap, const double* __restrict__ r,
+                                                       double damp, double* __restrict__ upd) {
+  typedef T pair_t __attribute__((ext_vector_type(2)));
+  constexpr int U = 8, H = NLP / 2;
+  const int lane = threadIdx.x & 63;
+  const int cell = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  if (cell >= n_cells) return;   // uniform for the wavefront
+  const int* pm = pmap + (size_t)cell * NL;
+  const int i1 = lane + 64;
+  const bool h1 = i1 < NL;
+  double b0 = r[pm[lane]];
+  double b1 = r[pm[h1 ? i1 : lane]];
+  if (!h1) b1 = 0.0;
+  const int q = lane < H ? lane : H - 1;
+  const pair_t* X = reinterpret_cast<const pair_t*>(inv + (size_t)cell * NLP * NL);
+  double x0 = 0.0, x1 = 0.0;
+  int j = 0;
+  for (; j + U <= NL; j += U) {
+    pair_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = __builtin_nontemporal_load(X + (size_t)(j + u) * H + q);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = j + u;
+      const double bk = lane_value(k < 64 ? b0 : b1, k & 63);
+      x0 += (double)c[u].x * bk;
+      x1 += (double)c[u].y * bk;
+    }
+  }
+  for (; j < NL; ++j) {
+    const pair_t c = X[(size_t)j * H + q];
+    const double bk = lane_value(j < 64 ? b0 : b1, j & 63);
+    x0 += (double)c.x * bk;
+    x1 += (double)c.y * bk;
+  }
+  if (lane < H) {
+    upd[(size_t)cell * NL + 2 * lane] = damp * x0;
+    if (2 * lane + 1 < NL) upd[(size_t)cell * NL + 2 * lane + 1] = damp * x1;
+  }
+}
+
 // FP32 local inverses (cell-major, columns of NL) to the padded bf16 layout
 __global__ void k_inv_to_bf16p(long long n_out, const float* __restrict__ a, uint16_t* __restrict__ b) {
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n_out;
@@ -679,10 +725,10 @@ void combined_values(long long nnz, const uint8_t* kind, const int* pos, const d
 }
 
 void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, float* lu32, int* pmap,
-                  int* singular, hipStream_t st, bool inverse) {
+                  int* singular, hipStream_t st, bool inverse, bool padded) {
   if (n_cells > 0)
     hipLaunchKernelGGL(k_vanka_factor, dim3(n_cells), dim3(256), 0, st, n_cells, map, C.row_ptr, C.col, C.val,
-                       lu64, lu32, pmap, singular, inverse ? 1 : 0);
+                       lu64, lu32, pmap, singular, inverse ? 1 : 0, padded ? NLP : NL);
 }
 
 void vanka_apply_inverse(int n_cells, const double* inv64, const float* inv32, const int* pmap, const double* r,
@@ -703,6 +749,17 @@ void vanka_apply_inverse_bf16(int n_cells, const uint16_t* inv16, const int* pma
 }
 
 size_t vanka_bf16_inverse_elems(int n_cells) { return (size_t)n_cells * NL * NLP; }
+size_t vanka_padded_inverse_elems(int n_cells) { return (size_t)n_cells * NL * NLP; }
+
+void vanka_apply_inverse_padded(int n_cells, const double* inv64, const float* inv32, const int* pmap,
+                                const double* r, double damp, double* upd, hipStream_t st) {
+  if (n_cells <= 0) return;
+  const int grid = (n_cells + 3) / 4;
+  if (inv32) hipLaunchKernelGGL(k_vanka_apply_p<float>, dim3(grid), dim3(256), 0, st, n_cells, inv32, pmap, r, damp,
+                                upd);
+  else hipLaunchKernelGGL(k_vanka_apply_p<double>, dim3(grid), dim3(256), 0, st, n_cells, inv64, pmap, r, damp,
+                          upd);
+}
 
 void vanka_inverse_to_bf16(int n_cells, const float* inv32, uint16_t* inv16, hipStream_t st) {
   const long long n = (long long)vanka_bf16_inverse_elems(n_cells);

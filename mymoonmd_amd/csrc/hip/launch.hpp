@@ -224,7 +224,7 @@ constexpr int kVankaLocal = 89;
 // by the cell's 89 sorted DOFs `map`); pmap[cell][i] = global DOF whose
 // defect enters row i of the row-interchanged local system
 void vanka_factor(int n_cells, const int* map, const DevCSR& C, double* lu64, float* lu32, int* pmap,
-                  int* singular, hipStream_t st, bool inverse = false);
+                  int* singular, hipStream_t st, bool inverse = false, bool padded = false);
 // Jacobi Vanka step with the explicit local inverses (vanka_factor with inverse)
 void vanka_apply_inverse(int n_cells, const double* inv64, const float* inv32, const int* pmap, const double* r,
                          double damp, double* upd, hipStream_t st);
@@ -235,6 +235,11 @@ void vanka_apply_inverse_bf16(int n_cells, const uint16_t* inv16, const int* pma
 // (vanka_bf16_inverse_elems(n_cells) elements)
 size_t vanka_bf16_inverse_elems(int n_cells);
 void vanka_inverse_to_bf16(int n_cells, const float* inv32, uint16_t* inv16, hipStream_t st);
+// FP64 / FP32 local inverses in the same padded layout (vanka_factor with
+// inverse and padded; vanka_padded_inverse_elems(n_cells) elements)
+size_t vanka_padded_inverse_elems(int n_cells);
+void vanka_apply_inverse_padded(int n_cells, const double* inv64, const float* inv32, const int* pmap,
+                                const double* r, double damp, double* upd, hipStream_t st);
 // b = a rounded to bf16 (nearest even)
 void f32_to_bf16(long long n, const float* a, uint16_t* b, hipStream_t st);
 // upd[cell][i] = damp * (local solution) for local DOF i (the order of map)

@@ -158,6 +158,7 @@ struct NseLevel {
   float* cval32 = nullptr;     // mixed precision: FP32 copy of the combined values (smoother defects)
   uint16_t* cval16 = nullptr;  // vanka_inverse 2: bf16 copy instead
   bool inv = false;            // lu64/lu32 hold the local inverses instead of the LU factors
+  bool padded = false;         // ... with columns padded to 90 entries
   double *upd = nullptr, *rhs = nullptr, *sol = nullptr, *def = nullptr, *res = nullptr;
   pmx::DevCSR Pu{}, Ru{}, Pp{}, Rp{};   // P: finer rows from this level; R = P^T (rows: this level)
   double* dinv_dense = nullptr;          // coarsest level: inverse of the combined matrix
@@ -492,6 +493,8 @@ void nse_smooth(pmx_solver* s, int l) {
   }
   L.sol_zero = false;
   if (L.lu16) pmx::vanka_apply_inverse_bf16(L.n_cells, L.lu16, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
+  else if (L.padded)
+    pmx::vanka_apply_inverse_padded(L.n_cells, L.lu64, L.lu32, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
   else if (L.inv) pmx::vanka_apply_inverse(L.n_cells, L.lu64, L.lu32, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
   else pmx::vanka_solve(L.n_cells, L.lu64, L.lu32, L.pmap, r, s->p.vanka_damp, L.upd, S(s));
   pmx::vanka_accumulate(L.n, L.acc_ptr, L.acc_idx, L.upd, L.sol, S(s));
@@ -1222,15 +1225,20 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
           L.map = upload(map.data(), map.size(), st);
           L.acc_ptr = upload(acc_ptr.data(), acc_ptr.size(), st);
           L.acc_idx = upload(acc_idx.data(), acc_idx.size(), st);
-          const size_t nlu = (size_t)L.n_cells * NLc * NLc;
+          const int kind = (l == 0) ? s->p.mg_smoother_coarse : s->p.mg_smoother;
+          L.inv = s->p.vanka_inverse != 0 && kind == PMX_SMOOTHER_CELL_VANKA_JACOBI;
+          const bool bf16 = f32 && L.inv && s->p.vanka_inverse == 2;
+          // explicit inverses kept as they are applied: columns padded to 90
+          // entries (two local rows per lane and load); the bf16 copy is
+          // made from unpadded FP32 inverses
+          L.padded = L.inv && !bf16;
+          const size_t nlu = L.padded ? pmx::vanka_padded_inverse_elems(L.n_cells) : (size_t)L.n_cells * NLc * NLc;
           if (f32) L.lu32 = dmalloc<float>(nlu);
           else L.lu64 = dmalloc<double>(nlu);
           L.pmap = dmalloc<int>((size_t)L.n_cells * NLc);
           L.upd = dmalloc<double>((size_t)L.n_cells * NLc);
-          const int kind = (l == 0) ? s->p.mg_smoother_coarse : s->p.mg_smoother;
-          L.inv = s->p.vanka_inverse != 0 && kind == PMX_SMOOTHER_CELL_VANKA_JACOBI;
-          pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.pmap, sing, st, L.inv);
-          if (f32 && L.inv && s->p.vanka_inverse == 2) {
+          pmx::vanka_factor(L.n_cells, L.map, nse_combined(sys), L.lu64, L.lu32, L.pmap, sing, st, L.inv, L.padded);
+          if (bf16) {
             L.lu16 = dmalloc<uint16_t>(pmx::vanka_bf16_inverse_elems(L.n_cells));
             pmx::vanka_inverse_to_bf16(L.n_cells, L.lu32, L.lu16, st);
             HIP_CHECK(hipStreamSynchronize(st));
