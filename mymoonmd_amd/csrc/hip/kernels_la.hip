@@ -838,6 +838,15 @@ void fe_apply_scaled_add3(const DevCSR& A, const double* x, double* y, size_t ld
     for (int k = 0; k < 3; ++k) fe_apply_scaled_add(A, x + k * ld, y + k * ld, a, st);
     return;
   }
+  if (A.n < 131072) {
+    // too few rows for one lane per row (a coarse multigrid level: ~80
+    // entries per row summed one after another take as long as the finest
+    // level's launch); 8 lanes per row of the CSR values instead
+    for (int k = 0; k < 3; ++k)
+      launch_fe_spmv<8, MODE_SCALED_ADD>(A, A.val, nullptr, x + k * ld, (const double*)nullptr,
+                                         (const double*)nullptr, y + k * ld, a, nullptr, st);
+    return;
+  }
   int sb = 0;
   const int grid = sell_grid(A, &sb);
   hipLaunchKernelGGL(k_sell_spmv3, dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb, A.n, A.active, ld,
