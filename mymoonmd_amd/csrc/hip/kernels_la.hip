@@ -618,7 +618,8 @@ __global__ __launch_bounds__(256) void k_csr_spmv3(int n, const int* __restrict_
 //   k_csr_bt3: y_k[r] += a (B_k^T p)[r]               (y_k = y + k ldy)
 // for rows r < row_end; the pattern and the gathered p are read once, and
 // every product is summed and added to y in the order of three k_csr_spmv
-// calls (bitwise the same).
+// calls with the same lanes per row (bitwise the same; csr_b3 runs 16 lanes
+// per pressure row, csr_bt3 8 per velocity row).
 template <int LPR>
 __global__ __launch_bounds__(256) void k_csr_b3(int n, const int* __restrict__ rp, const int* __restrict__ ci,
                                                 const double* __restrict__ v1, const double* __restrict__ v2,
@@ -1058,10 +1059,10 @@ void csr_spmv_t(const DevCSR& P, const T* x, T* y, double alpha, double beta, in
 }
 void csr_b3(const DevCSR& B, const double* v2, const double* v3, const double* x, size_t ldx, double* y, double a,
             int row_end, hipStream_t st) {
-  const int grid = (B.n + 31) / 32;   // 8 lanes per row
+  const int grid = (B.n + 15) / 16;   // 16 lanes per row (rows of ~125 entries)
   if (grid > 0)
-    hipLaunchKernelGGL(k_csr_b3<8>, dim3(grid), dim3(256), 0, st, B.n, B.row_ptr, B.col, B.val, v2, v3, x, ldx, y, a,
-                       row_end);
+    hipLaunchKernelGGL(k_csr_b3<16>, dim3(grid), dim3(256), 0, st, B.n, B.row_ptr, B.col, B.val, v2, v3, x, ldx, y,
+                       a, row_end);
 }
 void csr_bt3(const DevCSR& T, const double* v2, const double* v3, const double* p, double* y, size_t ldy, double a,
              int row_end, hipStream_t st) {
