@@ -291,6 +291,12 @@ typedef struct {
 int pmx_nse_create(pmx_ctx* ctx, const pmx_domain* d, int level, pmx_nse** out);
 int pmx_nse_destroy(pmx_nse* s);
 int pmx_nse_get_info(const pmx_nse* s, pmx_nse_info* out);
+/* No reference counterpart (kernel choice): the velocity block of the block
+ * operator is applied to the three components in one pass over its SELL copy
+ * (one lane per row) when it has at least `min_rows` rows (default 131,072),
+ * else by three CSR launches with 8 lanes per row.  Same products; the row
+ * sums are rounded in another order. */
+int pmx_nse_set_velocity_sell_rows(pmx_nse* s, int min_rows);
 /* NavierStokes<3>::assemble_linear_terms + assemble_nonlinear_term
  * (src/System/NavierStokes.C:407-700) with the Galerkin NSTYPE-1 forms
  * NSLaplaceGradGradSingle, NSNonlinearTerm_convective_Single (convection field

@@ -160,6 +160,7 @@ SIGNATURES = {
     "pmx_nse_create": [c_void_p, c_void_p, c_int, P(c_void_p)],
     "pmx_nse_destroy": [c_void_p],
     "pmx_nse_get_info": [c_void_p, P(NseInfo)],
+    "pmx_nse_set_velocity_sell_rows": [c_void_p, c_int],
     "pmx_nse_assemble": [c_void_p, c_void_p, P(NseParams), c_void_p, c_void_p],
     "pmx_nse_apply": [c_void_p, c_void_p, c_void_p, c_void_p],
     "pmx_nse_apply_scaled_add": [c_void_p, c_void_p, c_void_p, c_void_p, c_double],

@@ -663,21 +663,56 @@ int pmx_batch_run(pmx_batch* b, int n_steps, pmx_batch_report* rep) {
       HIP_CHECK(hipEventDestroy(e1));
       dfree(x0);
     }
-    if (!b->exec) {
+    auto capture = [&]() {
+      if (b->exec) { HIP_CHECK(hipGraphExecDestroy(b->exec)); b->exec = nullptr; }
+      if (b->graph) { HIP_CHECK(hipGraphDestroy(b->graph)); b->graph = nullptr; }
       HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       b->step(b->block, st);
       HIP_CHECK(hipStreamEndCapture(st, &b->graph));
       HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    };
+    if (!b->exec) capture();
+    // Shared workgroups wait for each other at barriers, which assumes that
+    // all workgroups of the grid are resident at once (occupancy x CUs of an
+    // otherwise idle device).  When a barrier gives up (e.g. another process
+    // holds CUs), the replayed steps are redone from the saved state with one
+    // workgroup per problem, which needs no barrier.  The flag is cleared
+    // before every replay, outside graph capture.
+    double* x_save = nullptr;
+    std::vector<int> it_save(b->n_prob), outer_save(b->n_prob), fail_save(b->n_prob);
+    if (b->d_bar_timeout) {
+      HIP_CHECK(hipMemsetAsync(b->d_bar_timeout, 0, sizeof(int), st));
+      x_save = dmalloc<double>(b->total_dof);
+      HIP_CHECK(hipMemcpyAsync(x_save, b->d_x, sizeof(double) * b->total_dof, hipMemcpyDeviceToDevice, st));
+      HIP_CHECK(hipMemcpyAsync(it_save.data(), b->d_iters, sizeof(int) * b->n_prob, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(outer_save.data(), b->d_outer, sizeof(int) * b->n_prob, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipMemcpyAsync(fail_save.data(), b->d_fail, sizeof(int) * b->n_prob, hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
     }
-    const auto t0 = std::chrono::steady_clock::now();
+    auto t0 = std::chrono::steady_clock::now();
     for (int k = 0; k < n_steps; ++k) HIP_CHECK(hipGraphLaunch(b->exec, st));
     HIP_CHECK(hipStreamSynchronize(st));
-    const auto t1 = std::chrono::steady_clock::now();
+    auto t1 = std::chrono::steady_clock::now();
     HIP_CHECK(hipGetLastError());
     if (b->d_bar_timeout) {
       int flag = 0;
       HIP_CHECK(hipMemcpy(&flag, b->d_bar_timeout, sizeof(int), hipMemcpyDeviceToHost));
-      if (flag) throw PmxError(PMX_E_INTERNAL, "batch: a workgroup barrier gave up waiting (results invalid)");
+      if (flag) {
+        HIP_CHECK(hipMemcpyAsync(b->d_time, &b->time, sizeof(double), hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->d_x, x_save, sizeof(double) * b->total_dof, hipMemcpyDeviceToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->d_iters, it_save.data(), sizeof(int) * b->n_prob, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->d_outer, outer_save.data(), sizeof(int) * b->n_prob, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipMemcpyAsync(b->d_fail, fail_save.data(), sizeof(int) * b->n_prob, hipMemcpyHostToDevice, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        clear_groups(b);
+        capture();
+        t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < n_steps; ++k) HIP_CHECK(hipGraphLaunch(b->exec, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+        t1 = std::chrono::steady_clock::now();
+        HIP_CHECK(hipGetLastError());
+      }
+      dfree(x_save);
     }
     b->time += n_steps * b->prm.tau;
     if (rep) {

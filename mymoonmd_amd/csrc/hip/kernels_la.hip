@@ -839,7 +839,7 @@ void fe_apply_scaled_add3(const DevCSR& A, const double* x, double* y, size_t ld
     for (int k = 0; k < 3; ++k) fe_apply_scaled_add(A, x + k * ld, y + k * ld, a, st);
     return;
   }
-  if (A.n < 131072) {
+  if (A.n < A.sell3_min_rows) {
     // too few rows for one lane per row (a coarse multigrid level: ~80
     // entries per row summed one after another take as long as the finest
     // level's launch); 8 lanes per row of the CSR values instead

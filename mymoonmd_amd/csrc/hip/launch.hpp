@@ -54,6 +54,9 @@ struct DevCSR {
   uint16_t* val_bf16 = nullptr;   // mixed precision copy (bf16, row-scaled)
   float* rscale = nullptr;        // per-row scale of the bf16 copy
   DevSELL sell;                   // optional SELL copy used by the FE-matrix kernels
+  // fe_apply_scaled_add3: matrices with fewer rows use three CSR launches
+  // with 8 lanes per row instead of the one-lane-per-row SELL pass
+  int sell3_min_rows = 131072;
 };
 
 // SELL values from the CSR values (FP64) / bf16 with the row scales A.rscale

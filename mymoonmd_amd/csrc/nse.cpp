@@ -300,6 +300,14 @@ int pmx_nse_destroy(pmx_nse* s) {
   return guarded([&] { delete s; });
 }
 
+int pmx_nse_set_velocity_sell_rows(pmx_nse* s, int min_rows) {
+  return guarded([&] {
+    require(s && s->A, "NULL argument");
+    require(min_rows >= 0, "min_rows must be >= 0");
+    s->A->A.sell3_min_rows = min_rows;
+  });
+}
+
 int pmx_nse_get_info(const pmx_nse* s, pmx_nse_info* out) {
   return guarded([&] {
     require(s && out, "NULL argument");

@@ -35,7 +35,25 @@ def _destroy(name, h):
         pass
 
 
-def _ptr(t):
+def _ptr(t, n, device, what="vector"):
+    """Device address of a float64 CUDA tensor that the C ABI will read or
+    write over `n` entries.  Raises ValueError before any GPU call when the
+    tensor is on the host or another device, is not float64, is not
+    contiguous or is shorter than n: the kernels would otherwise dereference a
+    host pointer or run past the end of the buffer."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError("%s must be a torch tensor, got %s" % (what, type(t).__name__))
+    if not t.is_cuda:
+        raise ValueError("%s must be a CUDA (HIP device) tensor, got device %s" % (what, t.device))
+    if t.device.index != device:
+        raise ValueError("%s is on device %s but the context is on device %d" % (what, t.device, device))
+    if t.dtype != torch.float64:
+        raise ValueError("%s must be float64, got %s" % (what, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % what)
+    if t.numel() < n:
+        raise ValueError("%s has %d entries, %d needed" % (what, t.numel(), n))
     return ctypes.c_void_p(t.data_ptr())
 
 
@@ -164,6 +182,9 @@ class FEMatrix:
         call("pmx_matrix_get_info", self.h, ctypes.byref(n), ctypes.byref(nnz), ctypes.byref(ab))
         self.n_rows, self.nnz, self.active_bound = n.value, nnz.value, ab.value
 
+    def _v(self, t, what):
+        return _ptr(t, self.n_rows, self.ctx.device, what)
+
     def reset(self):
         call("pmx_matrix_reset", self.ctx.h, self.h)
 
@@ -197,21 +218,21 @@ class FEMatrix:
         """BlockFEMatrix::apply: y = A x (identity on non-active rows)."""
         import torch
         torch.cuda.synchronize()
-        call("pmx_matrix_apply", self.ctx.h, self.h, _ptr(x), _ptr(y))
+        call("pmx_matrix_apply", self.ctx.h, self.h, self._v(x, "x"), self._v(y, "y"))
         self.ctx.synchronize()
 
     def apply_scaled_add(self, x, y, a=1.0):
         """BlockFEMatrix::apply_scaled_add: y += a A x (y += a x on non-active rows)."""
         import torch
         torch.cuda.synchronize()
-        call("pmx_matrix_apply_scaled_add", self.ctx.h, self.h, _ptr(x), _ptr(y), a)
+        call("pmx_matrix_apply_scaled_add", self.ctx.h, self.h, self._v(x, "x"), self._v(y, "y"), a)
         self.ctx.synchronize()
 
     def multiply_active(self, x, y, a=1.0):
         """FEMatrix::multiplyActive: y[i] += a (A x)[i] on the active rows."""
         import torch
         torch.cuda.synchronize()
-        call("pmx_matrix_multiply_active", self.ctx.h, self.h, _ptr(x), _ptr(y), a)
+        call("pmx_matrix_multiply_active", self.ctx.h, self.h, self._v(x, "x"), self._v(y, "y"), a)
         self.ctx.synchronize()
 
     def scale_active(self, factor):
@@ -249,7 +270,7 @@ def assemble_cd(ctx, space, params, matrix, rhs):
     """Assemble3D/Assemble2D + LocalAssembling(ConvDiff): adds into matrix and rhs."""
     import torch
     torch.cuda.synchronize()
-    call("pmx_assemble_cd", ctx.h, space.h, ctypes.byref(params), matrix.h, _ptr(rhs))
+    call("pmx_assemble_cd", ctx.h, space.h, ctypes.byref(params), matrix.h, _ptr(rhs, space.n_dof, ctx.device, "rhs"))
     ctx.synchronize()
 
 
@@ -259,7 +280,7 @@ def assemble_tcd(ctx, space, params, stiffness=None, mass=None, rhs=None):
     import torch
     torch.cuda.synchronize()
     call("pmx_assemble_tcd", ctx.h, space.h, ctypes.byref(params), stiffness.h if stiffness else None,
-         mass.h if mass else None, _ptr(rhs) if rhs is not None else None)
+         mass.h if mass else None, _ptr(rhs, space.n_dof, ctx.device, "rhs") if rhs is not None else None)
     ctx.synchronize()
 
 
@@ -267,25 +288,56 @@ def interpolate(ctx, space, params, u):
     """TFEFunction::Interpolate of the example's exact solution at params.time."""
     import torch
     torch.cuda.synchronize()
-    call("pmx_interpolate", ctx.h, space.h, ctypes.byref(params), _ptr(u))
+    call("pmx_interpolate", ctx.h, space.h, ctypes.byref(params), _ptr(u, space.n_dof, ctx.device, "u"))
     ctx.synchronize()
 
 
 def vector_copy(ctx, x, y):
     """y = x for two equally long (sub)vectors (BlockVector::copy / copy_nonactive)."""
-    assert x.numel() == y.numel()
-    call("pmx_vector_copy", ctx.h, x.numel(), _ptr(x), _ptr(y))
+    call("pmx_vector_copy", ctx.h, x.numel(), _ptr(x, x.numel(), ctx.device, "x"), _ptr(y, x.numel(), ctx.device, "y"))
+
+
+def vector_dot(ctx, x, y):
+    """BlockVector::dot (src/Matrix/BlockVector.C:491-530) of two equally long (sub)vectors."""
+    import torch
+    if x.numel() != y.numel():
+        raise ValueError("dot of vectors of lengths %d and %d" % (x.numel(), y.numel()))
+    torch.cuda.synchronize()
+    out = c_double()
+    call("pmx_vector_dot", ctx.h, x.numel(), _ptr(x, x.numel(), ctx.device, "x"),
+         _ptr(y, x.numel(), ctx.device, "y"), ctypes.byref(out))
+    return out.value
+
+
+def vector_norm(ctx, x):
+    """BlockVector::norm (src/Matrix/BlockVector.C:293-386): Euclidean norm."""
+    import torch
+    torch.cuda.synchronize()
+    out = c_double()
+    call("pmx_vector_norm", ctx.h, x.numel(), _ptr(x, x.numel(), ctx.device, "x"), ctypes.byref(out))
+    return out.value
+
+
+def vector_add_scaled(ctx, x, y, a):
+    """BlockVector::add_scaled: y += a x."""
+    import torch
+    if x.numel() != y.numel():
+        raise ValueError("add_scaled of vectors of lengths %d and %d" % (x.numel(), y.numel()))
+    torch.cuda.synchronize()
+    call("pmx_vector_add_scaled", ctx.h, x.numel(), a, _ptr(x, x.numel(), ctx.device, "x"),
+         _ptr(y, x.numel(), ctx.device, "y"))
+    ctx.synchronize()
 
 
 def vector_scale(ctx, x, a):
-    call("pmx_vector_scale", ctx.h, x.numel(), a, _ptr(x))
+    call("pmx_vector_scale", ctx.h, x.numel(), a, _ptr(x, x.numel(), ctx.device, "x"))
 
 
 def compute_errors(ctx, space, params, u):
     import torch
     torch.cuda.synchronize()
     out = (c_double * 4)()
-    call("pmx_compute_errors", ctx.h, space.h, ctypes.byref(params), _ptr(u), out)
+    call("pmx_compute_errors", ctx.h, space.h, ctypes.byref(params), _ptr(u, space.n_dof, ctx.device, "u"), out)
     return list(out)
 
 
@@ -333,7 +385,13 @@ class Solver:
         self.ctx = ctx
         self.params = params
         self.h = c_void_p()
+        self.n = None
         call("pmx_solver_create", ctx.h, ctypes.byref(params), ctypes.byref(self.h))
+
+    def _v(self, t, what):
+        if self.n is None:
+            raise ValueError("update the solver with its matrices before passing vectors")
+        return _ptr(t, self.n, self.ctx.device, what)
 
     def update_matrix(self, matrices, spaces=None):
         """matrices/spaces of all multigrid levels, coarsest first (or one matrix)."""
@@ -344,6 +402,7 @@ class Solver:
         import torch
         torch.cuda.synchronize()
         call("pmx_solver_update", self.h, mats, sps, len(matrices))
+        self.n = matrices[-1].n_rows
 
     def update_saddle(self, systems):
         """Solver::update_matrix for NSE systems (NSESystem), coarsest first."""
@@ -353,12 +412,13 @@ class Solver:
         import torch
         torch.cuda.synchronize()
         call("pmx_solver_update_nse", self.h, arr, len(systems))
+        self.n = systems[-1].n
 
     def solve(self, rhs, sol):
         import torch
         torch.cuda.synchronize()
         rep = SolveReport()
-        call("pmx_solver_solve", self.h, _ptr(rhs), _ptr(sol), ctypes.byref(rep))
+        call("pmx_solver_solve", self.h, self._v(rhs, "rhs"), self._v(sol, "sol"), ctypes.byref(rep))
         return rep
 
     def history(self):
@@ -372,7 +432,7 @@ class Solver:
     def apply_preconditioner(self, r, z):
         import torch
         torch.cuda.synchronize()
-        call("pmx_solver_apply_preconditioner", self.h, _ptr(r), _ptr(z))
+        call("pmx_solver_apply_preconditioner", self.h, self._v(r, "r"), self._v(z, "z"))
 
     def time_level_kernel(self, level, kernel, reps=20):
         """mean duration (ms) of a level's smoothing sweep as the cycle runs it"""
@@ -471,9 +531,11 @@ class TimeDiscretization:
         return self.current_time_ >= self.end_time - 1e-10
 
     def prepare_rhs_from_time_disc(self, stiffness, mass, rhs, old_rhs, u_m1, u_m2=None):
+        def v(t, what):
+            return _ptr(t, stiffness.n_rows, self.ctx.device, what)
         call("pmx_time_prepare_rhs", self.ctx.h, self.scheme, self.current_step_, self.tau,
-             stiffness.h, mass.h, _ptr(rhs), _ptr(old_rhs) if old_rhs is not None else None,
-             _ptr(u_m1), _ptr(u_m2) if u_m2 is not None else None)
+             stiffness.h, mass.h, v(rhs, "rhs"), v(old_rhs, "old_rhs") if old_rhs is not None else None,
+             v(u_m1, "u_m1"), v(u_m2, "u_m2") if u_m2 is not None else None)
 
     def prepare_system_matrix(self, stiffness, mass):
         call("pmx_time_prepare_system", self.ctx.h, self.scheme, self.current_step_, self.tau,
@@ -726,10 +788,9 @@ def write_vtk(space, functions, path, time=0.0):
     tensor of the space's DOF values}"""
     import torch
     names = list(functions)
-    ptrs = (c_void_p * max(1, len(names)))(*[functions[k].data_ptr() for k in names])
+    dev = space.ctx.device if space.ctx is not None else 0
+    ptrs = (c_void_p * max(1, len(names)))(*[_ptr(functions[k], space.n_dof, dev, k).value for k in names])
     cnames = (ctypes.c_char_p * max(1, len(names)))(*[k.encode() for k in names])
-    for k in names:
-        assert functions[k].is_cuda and functions[k].numel() == space.n_dof
     torch.cuda.synchronize()
     call("pmx_write_vtk", space.h, len(names), ptrs, cnames, time, path.encode())
 
@@ -768,10 +829,10 @@ class SnapshotsCollector:
 
     def write_data(self, solution, time_step=0):
         import torch
-        assert solution.is_cuda and solution.numel() == self.length
         torch.cuda.synchronize()   # torch's stream has produced the tensor
         w = c_int()
-        call("pmx_snapshots_write_data", self.h, _ptr(solution), time_step, ctypes.byref(w))
+        call("pmx_snapshots_write_data", self.h, _ptr(solution, self.length, self.ctx.device, "solution"), time_step,
+             ctypes.byref(w))
         return bool(w.value)
 
     def count(self):
@@ -816,9 +877,11 @@ class POD:
             call("pmx_pod_set_snapshots", self.h, d, snaps.length, m, snaps.length)
         else:
             import torch
-            assert snaps.is_cuda and snaps.is_contiguous()
+            if snaps.dim() != 2:
+                raise ValueError("snapshots must be a (n_snaps, length) tensor")
             torch.cuda.synchronize()
-            call("pmx_pod_set_snapshots", self.h, _ptr(snaps), snaps.shape[1], snaps.shape[0], snaps.shape[1])
+            call("pmx_pod_set_snapshots", self.h, _ptr(snaps, snaps.numel(), self.ctx.device, "snapshots"), snaps.shape[1],
+                 snaps.shape[0], snaps.shape[1])
 
     def compute_basis(self):
         call("pmx_pod_compute_basis", self.h)
@@ -840,7 +903,7 @@ class POD:
         n, _, r, _ = self.info()
         out = torch.zeros((max(r, 1), n), dtype=torch.float64, device="cuda:%d" % self.ctx.device)
         torch.cuda.synchronize()   # the fill on torch's stream ends before the copy on the context's
-        call("pmx_pod_get_basis", self.h, _ptr(out))
+        call("pmx_pod_get_basis", self.h, _ptr(out, out.numel(), self.ctx.device, "basis"))
         return out[:r]
 
     def get_snaps_avr(self):
@@ -848,7 +911,7 @@ class POD:
         n, _, _, _ = self.info()
         out = torch.zeros(n, dtype=torch.float64, device="cuda:%d" % self.ctx.device)
         torch.cuda.synchronize()
-        call("pmx_pod_get_mean", self.h, _ptr(out))
+        call("pmx_pod_get_mean", self.h, _ptr(out, n, self.ctx.device, "mean"))
         return out
 
     def write_pod(self, pod_directory, basename):
@@ -881,28 +944,36 @@ class NSESystem:
         self.n = inf.n_combined
         self.n_cells = inf.n_cells
 
+    def _v(self, t, what):
+        return _ptr(t, self.n, self.ctx.device, what)
+
     def zeros(self):
         import torch
         return torch.zeros(self.n, dtype=torch.float64, device=torch.device("cuda", self.ctx.device))
+
+    def set_velocity_sell_rows(self, min_rows):
+        """rows from which the velocity block runs on its SELL copy (default 131,072)"""
+        call("pmx_nse_set_velocity_sell_rows", self.h, min_rows)
 
     def assemble(self, example="u2_p1", nu=1.0, w=None, rhs=None):
         import torch
         torch.cuda.synchronize()
         p = NseParams(NSE_EXAMPLES[example], nu)
-        call("pmx_nse_assemble", self.ctx.h, self.h, ctypes.byref(p), _ptr(w) if w is not None else None,
-             _ptr(rhs) if rhs is not None else None)
+        call("pmx_nse_assemble", self.ctx.h, self.h, ctypes.byref(p),
+             _ptr(w, 3 * self.n_u, self.ctx.device, "w") if w is not None else None,
+             self._v(rhs, "rhs") if rhs is not None else None)
         self.ctx.synchronize()
 
     def apply(self, x, y):
         import torch
         torch.cuda.synchronize()
-        call("pmx_nse_apply", self.ctx.h, self.h, _ptr(x), _ptr(y))
+        call("pmx_nse_apply", self.ctx.h, self.h, self._v(x, "x"), self._v(y, "y"))
         self.ctx.synchronize()
 
     def apply_scaled_add(self, x, y, a=1.0):
         import torch
         torch.cuda.synchronize()
-        call("pmx_nse_apply_scaled_add", self.ctx.h, self.h, _ptr(x), _ptr(y), a)
+        call("pmx_nse_apply_scaled_add", self.ctx.h, self.h, self._v(x, "x"), self._v(y, "y"), a)
         self.ctx.synchronize()
 
     def structure(self, which):
@@ -933,27 +1004,28 @@ class NSESystem:
     def interpolate(self, x, example="u2_p1"):
         import torch
         torch.cuda.synchronize()
-        call("pmx_nse_interpolate", self.ctx.h, self.h, NSE_EXAMPLES[example], _ptr(x))
+        call("pmx_nse_interpolate", self.ctx.h, self.h, NSE_EXAMPLES[example], self._v(x, "x"))
         self.ctx.synchronize()
 
     def project_pressure(self, x):
         import torch
         torch.cuda.synchronize()
-        call("pmx_nse_project_pressure", self.ctx.h, self.h, _ptr(x))
+        call("pmx_nse_project_pressure", self.ctx.h, self.h, self._v(x, "x"))
         self.ctx.synchronize()
 
     def errors(self, x, example="u2_p1"):
         import torch
         torch.cuda.synchronize()
         out = (c_double * 5)()
-        call("pmx_nse_errors", self.ctx.h, self.h, NSE_EXAMPLES[example], _ptr(x), out)
+        call("pmx_nse_errors", self.ctx.h, self.h, NSE_EXAMPLES[example], self._v(x, "x"), out)
         return list(out)
 
     def restrict_velocity(self, fine, u_fine, u_coarse):
         """GridTransfer::RestrictFunction of [u1|u2|u3] from the next finer level."""
         import torch
         torch.cuda.synchronize()
-        call("pmx_nse_restrict_velocity", self.ctx.h, self.h, fine.h, _ptr(u_fine), _ptr(u_coarse))
+        call("pmx_nse_restrict_velocity", self.ctx.h, self.h, fine.h, _ptr(u_fine, 3 * fine.n_u, self.ctx.device, "u_fine"),
+             _ptr(u_coarse, 3 * self.n_u, self.ctx.device, "u_coarse"))
         self.ctx.synchronize()
 
     def __del__(self, _d=_destroy):
@@ -1026,8 +1098,8 @@ class NavierStokes:
         self.defect.copy_(self.rhs)
         f.apply_scaled_add(self.solution[-1], self.defect, -1.0)
         f.project_pressure(self.defect)
-        imp = float(self._torch.dot(self.defect[:3 * nu], self.defect[:3 * nu]))
-        mass = float(self._torch.dot(self.defect[3 * nu:], self.defect[3 * nu:]))
+        imp = vector_dot(self.ctx, self.defect[:3 * nu], self.defect[:3 * nu])
+        mass = vector_dot(self.ctx, self.defect[3 * nu:], self.defect[3 * nu:])
         self.residuals.append((imp, mass, (imp + mass) ** 0.5))
 
     def get_full_residual(self):
@@ -1039,7 +1111,7 @@ class NavierStokes:
         res = self.get_full_residual()
         if k == 0:
             self.initial_residual = res
-            self.initial_rhs_norm = float(self._torch.linalg.norm(self.rhs))
+            self.initial_rhs_norm = vector_norm(self.ctx, self.rhs)
         if k < self.db["nonlinloop_minit"]:
             return False
         old = self.residuals[max(0, len(self.residuals) - 10)][2]

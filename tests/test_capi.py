@@ -35,3 +35,14 @@ def test_version_and_error_reporting_without_device():
     assert rc == 1 and b"geometry" in L.pmx_last_error()
     rc = L.pmx_domain_create(2, 99, ctypes.byref(h))     # too many refinements
     assert rc == 1
+
+
+def test_vector_arguments_are_checked_on_the_host():
+    """Host memory and non-tensors are refused before the C ABI sees them."""
+    import numpy as np
+    import pytest
+    import torch
+    from mymoonmd_amd.parmoon import _ptr
+    for bad in (torch.zeros(4, dtype=torch.float64), np.zeros(4), [0.0] * 4):
+        with pytest.raises(ValueError):
+            _ptr(bad, 4, 0, "x")
