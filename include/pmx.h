@@ -213,6 +213,18 @@ int pmx_interpolate(pmx_ctx* ctx, const pmx_space* s, const pmx_cd_params* p, do
  * SD, L_inf (at quadrature points) against the example's exact solution. */
 int pmx_compute_errors(pmx_ctx* ctx, const pmx_space* s, const pmx_cd_params* p,
                        const double* dev_u, double* host_errors4);
+/* No reference counterpart.  pmx_compute_errors for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops): the same kernels and results, with the per-block partial sums and
+ * the 4 results in a fixed scratch array of the library (2 MiB per GPU,
+ * resolved by pmx_ctx_create and held until the process ends) instead of a
+ * hipMalloc per call.  Limits: at most 16,777,216 cells (PMX_E_ARG beyond);
+ * the space's quadrature tables for p->hexa_quad_rule must already be on the
+ * device (any earlier pmx_assemble_cd / pmx_assemble_tcd / pmx_compute_errors
+ * on the space; else PMX_E_STATE).  All calls on one GPU share the scratch:
+ * calls that overlap on one GPU (several host threads or contexts) race. */
+int pmx_compute_errors_noalloc(pmx_ctx* ctx, const pmx_space* s, const pmx_cd_params* p,
+                               const double* dev_u, double* host_errors4);
 
 /* ---------------------------------------------------------- vector (BLAS-1) */
 /* BlockVector::dot / norm (src/Matrix/BlockVector.C:293-386, 491-530) */

@@ -131,6 +131,7 @@ SIGNATURES = {
     "pmx_assembly_quad_points": [c_void_p, c_int, P(c_double)],
     "pmx_assemble_cd": [c_void_p, c_void_p, P(CDParams), c_void_p, c_void_p],
     "pmx_compute_errors": [c_void_p, c_void_p, P(CDParams), c_void_p, P(c_double)],
+    "pmx_compute_errors_noalloc": [c_void_p, c_void_p, P(CDParams), c_void_p, P(c_double)],
     "pmx_vector_dot": [c_void_p, c_int, c_void_p, c_void_p, P(c_double)],
     "pmx_vector_norm": [c_void_p, c_int, c_void_p, P(c_double)],
     "pmx_vector_add_scaled": [c_void_p, c_int, c_double, c_void_p, c_void_p],

@@ -193,6 +193,10 @@ int pmx_ctx_create(int device, pmx_ctx** out) {
     HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->partial = dmalloc<double>(4096);
     c->scal = dmalloc<double>(4096);
+    // the fixed error-norm scratch of this device (pmx_compute_errors_noalloc)
+    const int se = pmx::errors_scratch_init(device);
+    if (se != 0) throw PmxError(PMX_E_HIP, "hipGetSymbolAddress of the error-norm scratch failed (code " +
+                                               std::to_string(se) + ")");
     *out = c.release();
   });
 }
@@ -844,6 +848,32 @@ int pmx_compute_errors(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p
     HIP_CHECK(hipMemcpyAsync(host_errors4, ctx->scal, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_CHECK(hipStreamSynchronize(st));
     dfree(partial);
+  });
+}
+
+int pmx_compute_errors_noalloc(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p,
+                               const double* dev_u, double* host_errors4) {
+  return guarded([&] {
+    require(ctx && s_ && p && dev_u && host_errors4, "NULL argument");
+    require(s_->ctx == ctx, "space has no device data for this context");
+    const int key = (s_->fs.el.shape == pmx::SHAPE_HEXA) ? p->hexa_quad_rule : 0;
+    const auto it = s_->ref.find(key);
+    if (it == s_->ref.end())
+      throw PmxError(PMX_E_STATE, "the space's quadrature tables for this rule are not on the device yet: "
+                                  "call pmx_assemble_cd, pmx_assemble_tcd or pmx_compute_errors on it once first");
+    if (s_->fs.n_cells > 256 * pmx::kErrorsScratchBlocks)
+      throw PmxError(PMX_E_ARG, "pmx_compute_errors_noalloc supports at most " +
+                                    std::to_string(256 * pmx::kErrorsScratchBlocks) + " cells, the space has " +
+                                    std::to_string(s_->fs.n_cells));
+    double* scratch = pmx::errors_scratch(ctx->device);
+    if (!scratch) throw PmxError(PMX_E_STATE, "error-norm scratch not resolved on this device");
+    const pmx::AssemblyData ad = assembly_data(const_cast<pmx_space*>(s_), it->second);
+    const pmx::CDCoeffs co = cd_coeffs(p);
+    const double* out4 = nullptr;
+    require(pmx::cd_errors_scratch(ad, co, dev_u, scratch, &out4, ctx->stream), "space too large for the scratch");
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipMemcpyAsync(host_errors4, out4, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
   });
 }
 

@@ -17,6 +17,8 @@
 // accumulates its 27-entry row in registers.  Cells of one colour share no
 // DOF, so the scatter into the CSR row (binary search of the sorted columns)
 // is race free without atomics.
+#include <mutex>
+
 #include "common.hpp"
 #include "examples.hpp"
 #include "launch.hpp"
@@ -724,6 +726,48 @@ void cd_errors(const AssemblyData& ad, const CDCoeffs& co, const double* u, doub
   else if (ad.dim == 2 && ad.ndof == 3)
     hipLaunchKernelGGL((k_cd_errors<3, 2>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
   hipLaunchKernelGGL(k_reduce_errors, dim3(1), dim3(1024), 0, st, nb, partial, out4);
+}
+
+// ------------------------------------------ error norms without allocation
+// Fixed device scratch of the error-norm reduction for callers that cannot
+// allocate: the per-block partial sums (4 doubles per block of 256 cells, at
+// most kErrorsScratchBlocks blocks) followed by the 4 results.  Every GPU the
+// library runs on holds its own copy until the process ends; its address on a
+// device comes from hipGetSymbolAddress with that device current (the name
+// itself is only a host-side handle and must never reach a kernel).
+static __device__ __align__(256) double scratch[(size_t)kErrorsScratchBlocks * 4 + 8];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+int errors_scratch_init(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return -1;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[device]) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr) return (int)(e != hipSuccess ? e : hipErrorInvalidSymbol);
+  g_scratch_ptr[device] = static_cast<double*>(ptr);
+  return 0;
+}
+
+double* errors_scratch(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  return g_scratch_ptr[device];
+}
+
+bool cd_errors_scratch(const AssemblyData& ad, const CDCoeffs& co, const double* u, double* scratch_ptr,
+                       const double** out4, hipStream_t st) {
+  const int nb = (ad.n_cells + 255) / 256;
+  if (nb > kErrorsScratchBlocks || scratch_ptr == nullptr) return false;
+  double* res = scratch_ptr + (size_t)kErrorsScratchBlocks * 4;
+  cd_errors(ad, co, u, scratch_ptr, res, st);
+  *out4 = res;
+  return true;
 }
 
 }  // namespace pmx

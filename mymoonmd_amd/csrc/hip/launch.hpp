@@ -334,6 +334,17 @@ void fe_multiply_active(const DevCSR& A, const double* x, double* y, double a, h
 // L2, H1-semi, SD errors (squared partials) and Linf over quadrature points
 void cd_errors(const AssemblyData& ad, const CDCoeffs& co, const double* u, double* partial4,
                double* out4, hipStream_t st);
+// The same reduction on the library's fixed per-device scratch (no
+// allocation): errors_scratch_init resolves the scratch address on `device`
+// (that device current; returns 0 or a HIP error code), errors_scratch gives
+// it back (nullptr before the init).  cd_errors_scratch returns false (and
+// launches nothing) when the space has more than 256 * kErrorsScratchBlocks
+// cells; *out4 is the device address of the 4 results inside the scratch.
+constexpr int kErrorsScratchBlocks = 65536;
+int errors_scratch_init(int device);
+double* errors_scratch(int device);
+bool cd_errors_scratch(const AssemblyData& ad, const CDCoeffs& co, const double* u, double* scratch_ptr,
+                       const double** out4, hipStream_t st);
 
 
 // ---- snapshots / POD (kernels_pod.hip); snapshot matrices are column-major
