@@ -55,11 +55,14 @@ def parse():
                     help="only assemble the finest level, launch this operator kernel 5 times and "
                          "copy 1 GiB 3 times (the calibration of HBM byte counters), then exit")
     ap.add_argument("--cpu-refine", type=int, default=4, help="refinements of the CPU sample")
+    ap.add_argument("--renumber-levels", type=int, default=None,
+                    help="solver parameter renumber_levels (default: the solver's default)")
     return ap.parse_args()
 
 
-def solver_params(pm, precision, omega, max_it=500):
-    return pm.solver_params(
+def solver_params(pm, precision, omega, max_it=500, renumber=None):
+    extra = {} if renumber is None else {"renumber_levels": renumber}
+    return pm.solver_params(**extra,
         iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
         residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=max_it,
         multigrid_cycle_type="V", multigrid_smoother="jacobi", multigrid_smoother_coarse="direct_solve",
@@ -329,7 +332,7 @@ def main():
     t_update = {}
     for prec in ("mixed", "fp64"):
         t0 = time.perf_counter()
-        sol = pm.Solver(ctx, solver_params(pm, prec, args.jacobi_damp))
+        sol = pm.Solver(ctx, solver_params(pm, prec, args.jacobi_damp, renumber=args.renumber_levels))
         sol.update_matrix(mats, spaces)
         t_update[prec] = time.perf_counter() - t0
         solvers[prec] = sol

@@ -566,7 +566,7 @@ typedef struct {
                                   2: as 1, and where every active-column offset fits in 16
                                   bits the level matrix stores col - row as int16, the
                                   Dirichlet-column entries in a per-row tail (same results
-                                  as 1 bit for bit; slower).  3 (default): as 2 with the
+                                  as 1 bit for bit; slower).  3: as 2 with the
                                   Dirichlet-column products of each row summed once per
                                   reset of the level's solution (its Dirichlet part does
                                   not change between resets) and added first: fewer bytes
@@ -574,8 +574,12 @@ typedef struct {
                                   4: as 3 with the slots of a row packed by four (one lane
                                   loads four offsets and four values per instruction); same
                                   results as 3 bit for bit, no faster on MI355X (FP64
-                                  slower).  Levels whose offsets do not fit keep 32-bit
-                                  columns.
+                                  slower).  5 (default): as 3 with the rows grouped into slices of
+                                  equal active-column offsets (a lexicographic order gives
+                                  few such patterns on structured meshes): the offsets of
+                                  a slice are read once per slot by the whole wave and only
+                                  the values are streamed; same results as 3 bit for bit.
+                                  Levels whose offsets do not fit keep 32-bit columns.
                                   0: the reference order */
   int transfer_sell;           /* no reference counterpart.  1 (default): the scalar
                                   multigrid's prolongation and restriction read a SELL-64
@@ -593,6 +597,23 @@ typedef struct {
                                   local inverses and the values of the smoother's defect
                                   f - K x are bf16 instead of FP32 (rounded from FP32;
                                   FP64 vectors and sums). */
+  int refinement;              /* no reference counterpart; precision mixed, FGMRES with
+                                  multigrid on a scalar system whose finest level has the
+                                  folded 16-bit layout (renumber_levels >= 3).  1 (default):
+                                  FP64 iterative refinement: the residual b - A x and the
+                                  solution update in FP64 with the FP64 matrix; each
+                                  correction A d = r from FGMRES(gmres_restart) in FP32
+                                  (FP32 basis vectors, dot products and Hessenberg in FP64)
+                                  with the FP32 multigrid as preconditioner and an FP32 (or
+                                  bf16) copy of the finest matrix as operator.  The stopping
+                                  test of the reference (residual_tolerance,
+                                  residual_reduction, max_n_iterations) is applied to the
+                                  FP64 residual; iterations count the inner ones.  0: FGMRES
+                                  in FP64 throughout, only the preconditioner in FP32. */
+  double refinement_inner_reduction; /* reduction of the inner residual estimate per
+                                  refinement step (default 1e-5) */
+  int refinement_inner_bf16;   /* 1: the inner operator reads the bf16 row-scaled values of
+                                  the multigrid's finest level instead of an FP32 copy */
 } pmx_solver_params;
 /* Defaults of Solver<>::default_solver_database (src/Solver/Solver.C:20-117)
  * merged with Multigrid::default_multigrid_database (src/Multigrid/Multigrid.C:422-514). */

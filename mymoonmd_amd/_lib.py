@@ -81,7 +81,8 @@ class SolverParams(ctypes.Structure):
                 ("mg_coarse_residual", c_double), ("mg_coarse_max_n_iterations", c_int),
                 ("jacobi_damp", c_double), ("precision", c_int), ("vanka_damp", c_double),
                 ("zero_start_shortcut", c_int), ("renumber_levels", c_int),
-                ("transfer_sell", c_int), ("vanka_inverse", c_int)]
+                ("transfer_sell", c_int), ("vanka_inverse", c_int), ("refinement", c_int),
+                ("refinement_inner_reduction", c_double), ("refinement_inner_bf16", c_int)]
 
 
 class PodParams(ctypes.Structure):

@@ -41,6 +41,13 @@ pmx::DevSELL upload_sell(const pmx::SellPattern& sp, hipStream_t st) {
   S.bd_col = sp.bd_col.empty() ? nullptr : upload(sp.bd_col.data(), sp.bd_col.size(), st);
   S.pack = sp.pack;
   if (S.pack != 1 && (S.pack != 4 || !S.off)) throw PmxError(PMX_E_ARG, "packed SELL slots need the 16-bit layout");
+  if (!sp.slice_pat.empty()) {
+    if ((int)sp.slice_pat.size() != sp.n_slices || !S.off)
+      throw PmxError(PMX_E_ARG, "inconsistent SELL pattern layout");
+    S.slice_pat = upload(sp.slice_pat.data(), sp.slice_pat.size(), st);
+    S.pat_off = upload(sp.pat_off.data(), sp.pat_off.size(), st);
+    S.n_patterns = sp.n_patterns;
+  }
   return S;
 }
 

@@ -271,6 +271,106 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
   }
 }
 
+// The pattern layout of the folded 16-bit SELL copy: the 64 rows of a slice
+// share their active-column offsets (SellPattern::slice_pat), so a wave reads
+// slot j's offset once (a uniform, scalar load) and streams only the values:
+// 2 bytes per entry in bf16 instead of 4.  Row sums run in the order of
+// k_sell_spmv<.., REL16, FOLD> (Dirichlet-column sum first, then the active
+// columns), bitwise the same.  Empty lanes (the last slice of a pattern) use
+// the slice's first row as base, so every x read stays inside the active
+// rows; their values are 0 and their results are not stored.  GSUM false:
+// the Dirichlet-column sums are taken as 0, valid when x vanishes on the
+// Dirichlet rows (the corrections of an iterative refinement).
+template <int MODE, typename VT, typename XT, bool GSUM>
+__global__ __launch_bounds__(256) void k_sell_pat(
+    int n_slices, int sell_blocks, int n, int active, const int* __restrict__ sptr,
+    const int* __restrict__ rowmap, const int* __restrict__ slice_pat, const int* __restrict__ pat_off,
+    const VT* __restrict__ va, const float* __restrict__ rscale, const XT* __restrict__ x,
+    const XT* __restrict__ b, const XT* __restrict__ dinv, XT* __restrict__ y, double a,
+    double* __restrict__ partial, const XT* __restrict__ gsum) {
+  __shared__ double red[16];
+  using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
+  double d2 = 0;
+  if ((int)blockIdx.x < sell_blocks) {
+    const int per = sell_blocks >> 3;
+    const int vb = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    const int lane = threadIdx.x & 63;
+    const int slice = __builtin_amdgcn_readfirstlane(vb * 4 + (int)(threadIdx.x >> 6));
+    if (slice < n_slices) {
+      const int r = rowmap[slice * 64 + lane];
+      const int r0 = __builtin_amdgcn_readfirstlane(rowmap[slice * 64]);
+      const int s0 = __builtin_amdgcn_readfirstlane(sptr[slice]);
+      const int width = __builtin_amdgcn_readfirstlane((sptr[slice + 1] - s0) >> 6);
+      const int* po = pat_off + __builtin_amdgcn_readfirstlane(slice_pat[slice]);
+      const int rb = r >= 0 ? r : r0;
+      const VT* vp = va + s0 + lane;
+      AT acc = 0;
+      if constexpr (GSUM) acc = (AT)gsum[rb];
+      auto val = [&](int j) -> AT {
+        if constexpr (sizeof(VT) == 2) return (AT)bf16_to_float(__builtin_nontemporal_load(vp + j * 64));
+        else return (AT)__builtin_nontemporal_load(vp + j * 64);
+      };
+      int j = 0;
+      for (; j + 16 <= width; j += 16) {
+        int c[16];
+        AT v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) c[u] = rb + po[j + u];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = val(j + u);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc += v[u] * (AT)x[c[u]];
+      }
+      for (; j + 8 <= width; j += 8) {
+        int c[8];
+        AT v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) c[u] = rb + po[j + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = val(j + u);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u] * (AT)x[c[u]];
+      }
+      for (; j < width; ++j) acc += val(j) * (AT)x[rb + po[j]];
+      if (r >= 0) {
+        if (rscale) acc *= (AT)rscale[r];
+        if (MODE == MODE_SCALED_ADD) {
+          y[r] += (XT)(a * (double)acc);
+        } else if (MODE == MODE_APPLY) {
+          y[r] = (XT)acc;
+        } else if (MODE == MODE_DEFECT) {
+          const XT d = (XT)((AT)b[r] - acc);
+          y[r] = d;
+          d2 = (double)d * (double)d;
+        } else {
+          const AT res = (AT)b[r] - acc;
+          y[r] = (XT)((AT)x[r] + (AT)a * (AT)dinv[r] * res);
+        }
+      }
+    }
+  } else {
+    const int i = active + (int)(blockIdx.x - sell_blocks) * blockDim.x + threadIdx.x;
+    if (i < n) {
+      if (MODE == MODE_SCALED_ADD) {
+        y[i] += (XT)(a * (double)x[i]);
+      } else if (MODE == MODE_APPLY) {
+        y[i] = x[i];
+      } else if (MODE == MODE_DEFECT) {
+        const XT d = (XT)(b[i] - x[i]);
+        y[i] = d;
+        d2 = (double)d * (double)d;
+      } else {
+        const AT res = (AT)(b[i] - x[i]);
+        y[i] = (XT)((AT)x[i] + (AT)a * (AT)dinv[i] * res);
+      }
+    }
+  }
+  if (MODE == MODE_DEFECT) {
+    const double s = block_sum(d2, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
 // y_k += a A x_k for three vectors at once (x_k = x + k * ld, y_k likewise):
 // the velocity block of an NSTYPE-1 saddle-point operator acts on all three
 // components, so its SELL slots are read once for the three products.  Each
@@ -400,7 +500,12 @@ void launch_sell(const DevCSR& A, const VT* va, const VT* bd_val, const float* r
     throw std::runtime_error("SELL copy without slots, values or columns");
   if (A.sell.pack != 1 && !(A.sell.pack == 4 && A.sell.off && A.sell.gsum))
     throw std::runtime_error("packed SELL slots need the folded 16-bit layout");
-  if (A.sell.pack == 4) {
+  if (A.sell.slice_pat && A.sell.off && A.sell.gsum && A.sell.pack == 1) {
+    if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
+    hipLaunchKernelGGL((k_sell_pat<MODE, VT, XT, true>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb, A.n,
+                       A.active, A.sell.slice_ptr, A.sell.row, A.sell.slice_pat, A.sell.pat_off, va, rscale, x, b,
+                       dinv, y, a, partial, (const XT*)A.sell.gsum);
+  } else if (A.sell.pack == 4) {
     if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
     hipLaunchKernelGGL((k_sell_spmv<MODE, VT, XT, true, true, 4>), dim3(grid), dim3(256), 0, st, A.sell.n_slices,
                        sb, A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, A.sell.off, A.sell.bd_ptr,
@@ -727,11 +832,12 @@ __global__ __launch_bounds__(256) void k_axpy_dot(int n, const double* __restric
 // sums: every workgroup first sums partial_in (in one fixed order, so all of
 // them get the same h) and workgroup 0 records h; this removes the separate
 // reduction launch between two Gram-Schmidt steps
+template <typename T>
 __global__ __launch_bounds__(256) void k_axpy_dot_reduced(int n, int m_in, const double* __restrict__ partial_in,
                                                           double* __restrict__ h_out,
-                                                          const double* __restrict__ v,
-                                                          double* __restrict__ r,
-                                                          const double* __restrict__ w,
+                                                          const T* __restrict__ v,
+                                                          T* __restrict__ r,
+                                                          const T* __restrict__ w,
                                                           double* __restrict__ partial) {
   __shared__ double red[16];
   __shared__ double hs;
@@ -746,9 +852,9 @@ __global__ __launch_bounds__(256) void k_axpy_dot_reduced(int n, int m_in, const
   const double hk = hs;
   double s = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const double ri = r[i] - hk * v[i];
+    const T ri = (T)((double)r[i] - hk * (double)v[i]);
     r[i] = ri;
-    s += ri * (w ? w[i] : ri);
+    s += (double)ri * (double)(w ? w[i] : ri);
   }
   __syncthreads();
   s = block_sum(s, red);
@@ -1125,7 +1231,7 @@ void vec_dot_partial(int n, const double* x, const double* y, double* partial, h
 }
 void vec_axpy_dot_reduced(int n, const double* partial_in, double* h_out, const double* v, double* r,
                           const double* w, double* partial_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_axpy_dot_reduced, dim3(kRedBlocks), dim3(256), 0, st, n, kRedBlocks, partial_in, h_out, v,
+  hipLaunchKernelGGL(k_axpy_dot_reduced<double>, dim3(kRedBlocks), dim3(256), 0, st, n, kRedBlocks, partial_in, h_out, v,
                      r, w, partial_out);
 }
 void vec_reduce(const double* partial, double* out, int sqrt_of_sum, hipStream_t st) {
@@ -1188,6 +1294,88 @@ void sell_fill_bf16(const DevCSR& A, uint16_t* sell_bf16, hipStream_t st) {
 void csr_to_bf16(const DevCSR& A, uint16_t* vb, float* rscale, hipStream_t st) {
   hipLaunchKernelGGL(k_to_bf16_rows, dim3(grid_for(A.n)), dim3(256), 0, st, A.n, A.row_ptr, A.val, vb,
                      rscale);
+}
+
+// ------------------------------------------- FP32 Krylov of the refinement
+namespace {
+
+__global__ void k_sell_fill_f32(size_t total, const int* __restrict__ src, const double* __restrict__ v,
+                                float* __restrict__ out) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const int s = src[k];
+    out[k] = (s >= 0) ? (float)v[s] : 0.0f;
+  }
+}
+
+// y = x / a_dev[0], rounded to FP32
+__global__ void k_scale_to_f32(int n, const double* __restrict__ x, const double* __restrict__ a_dev,
+                               float* __restrict__ y) {
+  const double s = 1.0 / a_dev[0];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = (float)(s * x[i]);
+}
+
+struct Coef64 {
+  double c[64];
+};
+// x += sum_j c_j Z_j (Z_j = Z + j ld, FP32), summed in j order in FP64 per entry
+__global__ void k_multi_axpy_f32(int n, int k, Coef64 c, const float* __restrict__ Z, size_t ld,
+                                 double* __restrict__ x) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = 0;
+    for (int j = 0; j < k; ++j) s += c.c[j] * (double)Z[(size_t)j * ld + i];
+    x[i] += s;
+  }
+}
+
+}  // namespace
+
+void sell_fill_f32(const DevCSR& A, float* out, hipStream_t st) {
+  if (A.sell.total)
+    hipLaunchKernelGGL(k_sell_fill_f32, dim3(4096), dim3(256), 0, st, A.sell.total, A.sell.src, A.val, out);
+}
+void vec_dot_partial_f32(int n, const float* x, const float* y, double* partial, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot_partial<float>, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial);
+}
+void vec_axpy_dot_reduced_f32(int n, const double* partial_in, double* h_out, const float* v, float* r,
+                              const float* w, double* partial_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_dot_reduced<float>, dim3(kRedBlocks), dim3(256), 0, st, n, kRedBlocks, partial_in, h_out,
+                     v, r, w, partial_out);
+}
+void vec_scale_to_f32(int n, const double* x, const double* a_dev, float* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_to_f32, dim3(grid_for(n)), dim3(256), 0, st, n, x, a_dev, y);
+}
+void vec_multi_axpy_f32(int n, int k, const double* c, const float* Z, size_t ld, double* x, hipStream_t st) {
+  if (k < 1 || k > 64) throw std::runtime_error("vec_multi_axpy_f32: 1 <= k <= 64");
+  Coef64 cc{};
+  for (int j = 0; j < k; ++j) cc.c[j] = c[j];
+  hipLaunchKernelGGL(k_multi_axpy_f32, dim3(grid_for(n)), dim3(256), 0, st, n, k, cc, Z, ld, x);
+}
+template <typename VT>
+static void apply_active(const DevCSR& A, const VT* vals, const float* rscale, const float* x, float* y,
+                         const float* zero_gsum, hipStream_t st) {
+  if (!A.sell.off || A.sell.pack != 1 || A.sell.n_slices == 0)
+    throw std::runtime_error("the refinement operator needs the folded 16-bit SELL layout");
+  int sb = 0;
+  const int grid = sell_grid(A, &sb);
+  if (A.sell.slice_pat)
+    hipLaunchKernelGGL((k_sell_pat<MODE_APPLY, VT, float, false>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb,
+                       A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.slice_pat, A.sell.pat_off, vals, rscale,
+                       x, (const float*)nullptr, (const float*)nullptr, y, 1.0, (double*)nullptr,
+                       (const float*)nullptr);
+  else
+    hipLaunchKernelGGL((k_sell_spmv<MODE_APPLY, VT, float, true, true>), dim3(grid), dim3(256), 0, st,
+                       A.sell.n_slices, sb, A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, A.sell.off,
+                       A.sell.bd_ptr, A.sell.bd_col, (const VT*)nullptr, vals, rscale, x, (const float*)nullptr,
+                       (const float*)nullptr, y, 1.0, (double*)nullptr, zero_gsum);
+}
+void fe_apply_active_f32(const DevCSR& A, const float* vals, const float* x, float* y, const float* zero_gsum,
+                         hipStream_t st) {
+  apply_active<float>(A, vals, nullptr, x, y, zero_gsum, st);
+}
+void fe_apply_active_bf16(const DevCSR& A, const float* x, float* y, const float* zero_gsum, hipStream_t st) {
+  if (!A.sell.val_bf16 || !A.rscale) throw std::runtime_error("no bf16 SELL values");
+  apply_active<uint16_t>(A, A.sell.val_bf16, A.rscale, x, y, zero_gsum, st);
 }
 
 }  // namespace pmx

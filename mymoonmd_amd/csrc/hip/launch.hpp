@@ -36,6 +36,11 @@ struct DevSELL {
   void* gsum = nullptr;
   int gsum_bytes = 0;
   int pack = 1;                   // packed slots (see SellPattern::pack)
+  // pattern layout (SellPattern::slice_pat / pat_off): the 64 rows of a
+  // slice share their active-column offsets, read once per slot by the wave
+  int* slice_pat = nullptr;       // [n_slices] position of the slice's pattern in pat_off
+  int* pat_off = nullptr;         // offsets col - row of all patterns
+  int n_patterns = 0;
   // grid-transfer matrices: the SELL copy (slice_ptr, row, col, val and, when
   // the weights are bf16 numbers, val_bf16) covers rows [0, n_rows)
   int n_rows = 0;
@@ -132,6 +137,24 @@ void vec_dot_partial(int n, const double* x, const double* y, double* partial, h
 void vec_axpy_dot_reduced(int n, const double* partial_in, double* h_out, const double* v, double* r,
                           const double* w, double* partial_out, hipStream_t st);
 void vec_reduce(const double* partial, double* out, int sqrt_of_sum, hipStream_t st);
+// FP32 vectors with FP64 dot products (the inner Krylov of an iterative
+// refinement): as vec_dot_partial / vec_axpy_dot_reduced, r updated in FP64
+// and rounded to FP32
+void vec_dot_partial_f32(int n, const float* x, const float* y, double* partial, hipStream_t st);
+void vec_axpy_dot_reduced_f32(int n, const double* partial_in, double* h_out, const float* v, float* r,
+                              const float* w, double* partial_out, hipStream_t st);
+// y = x / a_dev[0] rounded to FP32
+void vec_scale_to_f32(int n, const double* x, const double* a_dev, float* y, hipStream_t st);
+// x += sum_{j<k} c_j Z_j with Z_j = Z + j ld (FP32), k <= 64, c on the host
+void vec_multi_axpy_f32(int n, int k, const double* c, const float* Z, size_t ld, double* x, hipStream_t st);
+// y = A x on the active rows (y = x = 0 elsewhere) for an x that vanishes on
+// the Dirichlet rows, from the folded 16-bit SELL copy (pattern layout when
+// present) with FP32 values `vals` (sell_fill_f32) or the bf16 ones;
+// zero_gsum: A.active zeros (the Dirichlet-column sums of such an x)
+void sell_fill_f32(const DevCSR& A, float* out, hipStream_t st);
+void fe_apply_active_f32(const DevCSR& A, const float* vals, const float* x, float* y, const float* zero_gsum,
+                         hipStream_t st);
+void fe_apply_active_bf16(const DevCSR& A, const float* x, float* y, const float* zero_gsum, hipStream_t st);
 // y += (a_dev ? sign*a_dev[0] : a) * x
 void vec_axpy(int n, double a, const double* a_dev, double sign, const double* x, double* y,
               hipStream_t st);

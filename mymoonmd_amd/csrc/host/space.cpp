@@ -1,6 +1,7 @@
 #include "space.hpp"
 
 #include <algorithm>
+#include <unordered_map>
 #include <map>
 #include <cmath>
 #include <stdexcept>
@@ -449,10 +450,11 @@ void color_cells(const FESpace& s, std::vector<int>& order, std::vector<int>& co
 
 namespace pmx {
 
-SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16, int pack) {
+SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16, int pack, bool patterns, int max_patterns) {
   constexpr int C = 64;
   SellPattern s;
   if (!rel16 || pack != 4) pack = 1;
+  if (!rel16 || pack != 1) patterns = false;
   s.pack = pack;
   const int na = p.active_bound;
   s.n_active = na;
@@ -476,25 +478,95 @@ SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16, int pack) {
     if (!fits) return build_sell(p, sigma, false, 1);
   }
   auto len = [&](int r) { return rel16 ? nact[r] : p.row_ptr[r + 1] - p.row_ptr[r]; };
-  std::vector<int> order(na);
-  for (int r = 0; r < na; ++r) order[r] = r;
-  for (int w = 0; w < na; w += sigma) {
-    const int e = std::min(na, w + sigma);
-    std::stable_sort(order.begin() + w, order.begin() + e, [&](int a, int b) { return len(a) > len(b); });
-  }
-  s.row.assign((size_t)s.n_slices * C, -1);
-  s.slice_ptr.assign(s.n_slices + 1, 0);
-  for (int sl = 0; sl < s.n_slices; ++sl) {
-    int width = 0;
-    for (int l = 0; l < C; ++l) {
-      const int k = sl * C + l;
-      if (k < na) {
-        s.row[k] = order[k];
-        width = std::max(width, len(order[k]));
+  if (patterns) {
+    // pattern id of every active row: its active-column offsets, compared
+    // exactly (a 64-bit hash picks the candidate list)
+    std::vector<uint64_t> h(na);
+#pragma omp parallel for schedule(static)
+    for (int r = 0; r < na; ++r) {
+      uint64_t x = 1469598103934665603ull ^ (uint64_t)nact[r];
+      for (int q = p.row_ptr[r]; q < p.row_ptr[r] + nact[r]; ++q) {
+        x ^= (uint64_t)(uint32_t)(p.col[q] - r);
+        x *= 1099511628211ull;
       }
+      h[r] = x;
     }
-    width = (width + pack - 1) / pack * pack;
-    s.slice_ptr[sl + 1] = s.slice_ptr[sl] + width * C;
+    std::unordered_map<uint64_t, std::vector<int>> cand;   // hash -> pattern ids
+    std::vector<int> rep;                                  // representative row of each pattern
+    std::vector<int> pid(na);
+    bool too_many = false;
+    for (int r = 0; r < na && !too_many; ++r) {
+      auto& list = cand[h[r]];
+      int id = -1;
+      for (int c : list) {
+        const int r0 = rep[c];
+        if (nact[r0] != nact[r]) continue;
+        bool same = true;
+        for (int k = 0; k < nact[r] && same; ++k)
+          same = p.col[p.row_ptr[r0] + k] - r0 == p.col[p.row_ptr[r] + k] - r;
+        if (same) { id = c; break; }
+      }
+      if (id < 0) {
+        id = (int)rep.size();
+        rep.push_back(r);
+        list.push_back(id);
+        too_many = (int)rep.size() > max_patterns;
+      }
+      pid[r] = id;
+    }
+    if (too_many) return build_sell(p, sigma, rel16, 1, false);
+    const int np = (int)rep.size();
+    // rows of each pattern in row order, cut into slices of 64; slices
+    // ordered by their first row
+    std::vector<int> cnt(np + 1, 0);
+    for (int r = 0; r < na; ++r) cnt[pid[r] + 1]++;
+    for (int k = 0; k < np; ++k) cnt[k + 1] += cnt[k];
+    std::vector<int> by_pat(na), fill(cnt.begin(), cnt.end() - 1);
+    for (int r = 0; r < na; ++r) by_pat[fill[pid[r]]++] = r;
+    std::vector<std::pair<int, int>> slices;   // (first position in by_pat, pattern)
+    for (int k = 0; k < np; ++k)
+      for (int q = cnt[k]; q < cnt[k + 1]; q += C) slices.push_back({q, k});
+    std::stable_sort(slices.begin(), slices.end(),
+                     [&](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                       return by_pat[a.first] < by_pat[b.first];
+                     });
+    s.n_slices = (int)slices.size();
+    s.n_patterns = np;
+    std::vector<int> pat_ptr(np + 1, 0);
+    for (int k = 0; k < np; ++k) pat_ptr[k + 1] = pat_ptr[k] + nact[rep[k]];
+    s.pat_off.resize(pat_ptr[np]);
+    for (int k = 0; k < np; ++k)
+      for (int j = 0; j < nact[rep[k]]; ++j) s.pat_off[pat_ptr[k] + j] = p.col[p.row_ptr[rep[k]] + j] - rep[k];
+    s.row.assign((size_t)s.n_slices * C, -1);
+    s.slice_ptr.assign(s.n_slices + 1, 0);
+    s.slice_pat.resize(s.n_slices);
+    for (int sl = 0; sl < s.n_slices; ++sl) {
+      const int q0 = slices[sl].first, k = slices[sl].second;
+      for (int l = 0; l < C && q0 + l < cnt[k + 1]; ++l) s.row[(size_t)sl * C + l] = by_pat[q0 + l];
+      s.slice_pat[sl] = pat_ptr[k];
+      s.slice_ptr[sl + 1] = s.slice_ptr[sl] + nact[rep[k]] * C;
+    }
+  } else {
+    std::vector<int> order(na);
+    for (int r = 0; r < na; ++r) order[r] = r;
+    for (int w = 0; w < na; w += sigma) {
+      const int e = std::min(na, w + sigma);
+      std::stable_sort(order.begin() + w, order.begin() + e, [&](int a, int b) { return len(a) > len(b); });
+    }
+    s.row.assign((size_t)s.n_slices * C, -1);
+    s.slice_ptr.assign(s.n_slices + 1, 0);
+    for (int sl = 0; sl < s.n_slices; ++sl) {
+      int width = 0;
+      for (int l = 0; l < C; ++l) {
+        const int k = sl * C + l;
+        if (k < na) {
+          s.row[k] = order[k];
+          width = std::max(width, len(order[k]));
+        }
+      }
+      width = (width + pack - 1) / pack * pack;
+      s.slice_ptr[sl + 1] = s.slice_ptr[sl] + width * C;
+    }
   }
   const size_t total = s.slice_ptr.back();
   if (rel16) s.off.assign(total, 0);

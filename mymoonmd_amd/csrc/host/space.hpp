@@ -139,6 +139,16 @@ struct SellPattern {
   // loads `pack` consecutive entries of its row with one instruction; the
   // slice width is a multiple of pack
   int pack = 1;
+  // pattern layout (rel16 only): the active rows are grouped by their
+  // sequence of active-column offsets (col - row), and the 64 rows of a slice
+  // share one such pattern; slice_pat[slice] is the position of the slice's
+  // pattern in pat_off (width = the slice's slot count / 64 offsets).  The
+  // slots keep the 16-bit layout (off filled as well), so every 16-bit kernel
+  // also runs on it.  Empty lanes (row -1) exist only in the last slice of a
+  // pattern.  Slices are ordered by their first row.
+  std::vector<int> slice_pat;
+  std::vector<int> pat_off;
+  int n_patterns = 0;
 };
 // position of slot j of lane l in a slice starting at s0
 inline size_t sell_pos(size_t s0, int j, int l, int pack) {
@@ -146,6 +156,10 @@ inline size_t sell_pos(size_t s0, int j, int l, int pack) {
 }
 // rel16: build the 16-bit offset layout; if an offset does not fit, the
 // result has an empty `off` and the 32-bit layout
-SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16 = false, int pack = 1);
+// patterns: group rows by offset pattern (rel16 and pack 1 only); when the
+// rows have more than max_patterns distinct patterns the length-sorted
+// layout is built instead
+SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16 = false, int pack = 1, bool patterns = false,
+                       int max_patterns = 65536);
 
 }  // namespace pmx

@@ -129,6 +129,7 @@ struct pmx_matrix {
     dfree(A.sell.slice_ptr); dfree(A.sell.row); dfree(A.sell.col); dfree(A.sell.src);
     dfree(A.sell.val); dfree(A.sell.val_bf16);
     dfree(A.sell.bd_ptr); dfree(A.sell.bd_src); dfree(A.sell.bd_col); dfree(A.sell.off);
+    dfree(A.sell.slice_pat); dfree(A.sell.pat_off);
   }
 };
 

@@ -201,6 +201,7 @@ struct Level {
     if (own_sell) {
       dfree(A.sell.slice_ptr); dfree(A.sell.row); dfree(A.sell.col); dfree(A.sell.src); dfree(A.sell.val);
       dfree(A.sell.bd_ptr); dfree(A.sell.bd_src); dfree(A.sell.bd_col); dfree(A.sell.off);
+      dfree(A.sell.slice_pat); dfree(A.sell.pat_off);
       own_sell = false;
     }
     dfree(perm); dfree(bd64); dfree(bd16); dfree(g64); dfree(g32);
@@ -234,6 +235,13 @@ struct pmx_solver {
   double* kbd = nullptr;         // FP64 Dirichlet-column values of K (mixed only)
   double *kb = nullptr, *kx = nullptr;   // b and x in that numbering
   bool kown = false;             // K.sell is FGMRES's own 32-bit copy (16-bit levels)
+  // iterative refinement (precision mixed, refinement 1): FP32 Krylov basis
+  // blocks (column j at + j n), the FP32 values of the finest level's folded
+  // copy (nullptr: its bf16 values are used) and zeros for its
+  // Dirichlet-column sums
+  bool ir = false;
+  float *V32 = nullptr, *Z32 = nullptr, *w32 = nullptr, *kval32 = nullptr, *zero32 = nullptr;
+  int ir_outer = 0;              // refinement steps of the last solve
   const pmx::DevCSR* op = nullptr;       // operator of the running solve
   bool internal = false;                 // the running solve uses that numbering
   int n = 0;
@@ -257,6 +265,8 @@ struct pmx_solver {
     V.clear(); Z.clear();
     dfree(r); dfree(w1); dfree(w2); dfree(w3); dfree(dinv); dfree(d_H); dfree(d_scal); dfree(partial);
     dfree(kval); dfree(kbd); dfree(kb); dfree(kx);
+    dfree(V32); dfree(Z32); dfree(w32); dfree(kval32); dfree(zero32);
+    ir = false;
     if (kown) {
       dfree(K.sell.slice_ptr); dfree(K.sell.row); dfree(K.sell.col); dfree(K.sell.src);
       dfree(K.sell.bd_ptr); dfree(K.sell.bd_src); dfree(K.sell.bd_col);
@@ -762,6 +772,103 @@ void solve_fgmres_run(pmx_solver* s, const double* b, double* x, pmx_solve_repor
   rep->converged = 0;
 }
 
+// FP64 iterative refinement around FP32 FGMRES (precision mixed, refinement
+// 1), on vectors in the finest level's numbering.  Outer step: r = b - A x
+// with the FP64 matrix, the reference's stopping test on ||r||; inner:
+// FGMRES(m) for A d = r from d = 0 with FP32 basis vectors (dot products,
+// Hessenberg matrix and Givens rotations in FP64), z_i = multigrid cycle in
+// FP32, w = A z_i with the FP32 (or bf16) copy of the finest level; it stops
+// at m iterations or when its residual estimate drops below
+// max(refinement_inner_reduction ||r||, half the final target); then
+// x += sum y_j z_j in FP64.  Every vector of the inner solve vanishes on the
+// Dirichlet rows (r_D = b_D - x_D = 0 once x_D = b_D; the cycle and the unit
+// rows keep zeros there), so the inner operator needs only the
+// active-column entries; x_D is set to b_D first.
+void solve_ir_run(pmx_solver* s, const double* b, double* x, pmx_solve_report* rep) {
+  const int n = s->n;
+  const int m = std::min(s->p.gmres_restart, s->p.max_n_iterations);
+  const int ld = m + 1;
+  Level& F = s->lv.back();
+  const int na = F.active;
+  hipStream_t st = S(s);
+  if (n > na)
+    HIP_CHECK(hipMemcpyAsync(x + na, b + na, (size_t)(n - na) * sizeof(double), hipMemcpyDeviceToDevice, st));
+  std::vector<double> H((size_t)ld * m, 0.0), sv(m + 1), cs(m + 1), sn(m + 1);
+  op_defect(s, x, b, s->r);
+  double beta = read_scalar(s, s->d_scal);
+  rep->initial_residual = beta;
+  s->ir_outer = 0;
+  if (converged(s, beta, 0)) { rep->iterations = 0; rep->final_residual = beta; rep->converged = s->reached ? 1 : 0; return; }
+  const double target = std::max(s->p.residual_tolerance, s->p.residual_reduction * s->initial_res);
+  float* const saved_rhs = F.rhs32;
+  int j = 0;
+  while (true) {
+    ++s->ir_outer;
+    pmx::vec_scale_to_f32(n, s->r, s->d_scal, s->V32, st);
+    std::fill(sv.begin(), sv.end(), 0.0);
+    sv[0] = beta;
+    const double inner_target = std::max(s->p.refinement_inner_reduction * beta, 0.5 * target);
+    int i = 0, used = 0;
+    for (i = 0; i < m && j < s->p.max_n_iterations; ++i) {
+      float* vi = s->V32 + (size_t)i * n;
+      float* zi = s->Z32 + (size_t)i * n;
+      F.rhs32 = vi;   // the cycle only reads its finest right-hand side
+      try {
+        mg_cycle(s, true);
+      } catch (...) {
+        F.rhs32 = saved_rhs;
+        throw;
+      }
+      F.rhs32 = saved_rhs;
+      HIP_CHECK(hipMemcpyAsync(zi, F.sol32, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      if (s->kval32) pmx::fe_apply_active_f32(F.A, s->kval32, zi, s->w32, s->zero32, st);
+      else pmx::fe_apply_active_bf16(F.A, zi, s->w32, s->zero32, st);
+      double* Hcol = s->d_H + (size_t)i * ld;
+      double* pa = s->partial;
+      double* pb = s->partial + pmx::vec_partials();
+      pmx::vec_dot_partial_f32(n, s->w32, s->V32, pa, st);
+      for (int k = 0; k < i; ++k) {
+        pmx::vec_axpy_dot_reduced_f32(n, pa, Hcol + k, s->V32 + (size_t)k * n, s->w32, s->V32 + (size_t)(k + 1) * n,
+                                      pb, st);
+        std::swap(pa, pb);
+      }
+      pmx::vec_axpy_dot_reduced_f32(n, pa, Hcol + i, vi, s->w32, nullptr, pb, st);
+      pmx::vec_reduce(pb, Hcol + i + 1, 1, st);
+      HIP_CHECK(hipMemcpyAsync(&H[(size_t)i * ld], Hcol, (i + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+      const double hn = H[(i + 1) + (size_t)i * ld];
+      if (hn == 0.0) throw PmxError(PMX_E_BREAKDOWN, "Unhappy breakdown in flexible gmres");
+      pmx::vec_scale_f32(n, 0.0, Hcol + i + 1, 1, s->w32, s->V32 + (size_t)(i + 1) * n, st);
+      for (int k = 0; k < i; ++k) apply_rotation(H[k + (size_t)i * ld], H[k + 1 + (size_t)i * ld], cs[k], sn[k]);
+      generate_rotation(H[i + (size_t)i * ld], H[i + 1 + (size_t)i * ld], cs[i], sn[i]);
+      apply_rotation(H[i + (size_t)i * ld], H[i + 1 + (size_t)i * ld], cs[i], sn[i]);
+      apply_rotation(sv[i], sv[i + 1], cs[i], sn[i]);
+      ++j;
+      used = i + 1;
+      const double est = std::fabs(sv[i + 1]);
+      s->history.push_back(est);
+      if (est <= inner_target) break;
+    }
+    if (used == 0) break;   // max_n_iterations reached
+    // back solve and x += sum_k y_k z_k (FP64 accumulation)
+    std::vector<double> y(sv.begin(), sv.begin() + used);
+    for (int q = used - 1; q >= 0; --q) {
+      y[q] /= H[q + (size_t)q * ld];
+      for (int k = q - 1; k >= 0; --k) y[k] -= H[k + (size_t)q * ld] * y[q];
+    }
+    pmx::vec_multi_axpy_f32(n, used, y.data(), s->Z32, (size_t)n, x, st);
+    op_defect(s, x, b, s->r);
+    beta = read_scalar(s, s->d_scal);
+    if (converged(s, beta, j) || j >= s->p.max_n_iterations) {
+      rep->iterations = j; rep->final_residual = beta; rep->converged = s->reached ? 1 : 0;
+      return;
+    }
+  }
+  rep->iterations = j;
+  rep->final_residual = beta;
+  rep->converged = 0;
+}
+
 // Scalar systems with renumbered multigrid levels run FGMRES in the finest
 // level's numbering: b and x are permuted once per solve instead of the
 // preconditioner's input and output on every iteration, and the operator is
@@ -779,7 +886,8 @@ void solve_fgmres(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
   s->op = &s->K;
   s->internal = true;
   try {
-    solve_fgmres_run(s, s->kb, s->kx, rep);
+    if (s->ir) solve_ir_run(s, s->kb, s->kx, rep);
+    else solve_fgmres_run(s, s->kb, s->kx, rep);
   } catch (...) {
     s->op = &s->A;
     s->internal = false;
@@ -892,9 +1000,12 @@ int pmx_solver_default_params(pmx_solver_params* p) {
     d.precision = PMX_PRECISION_FP64;
     d.vanka_damp = 1.0;
     d.zero_start_shortcut = 1;
-    d.renumber_levels = 3;
+    d.renumber_levels = 5;
     d.transfer_sell = 1;
     d.vanka_inverse = 2;
+    d.refinement = 1;
+    d.refinement_inner_reduction = 1e-5;
+    d.refinement_inner_bf16 = 0;
     *p = d;
   });
 }
@@ -1009,7 +1120,7 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
           std::vector<int> opos;
           pmx::permute_csr(*M->pattern, nullptr, perm[l], pinv[l], pp, nullptr, &opos, true);
           pmx::SellPattern sp = pmx::build_sell(pp, kSellSigmaSolver, s->p.renumber_levels >= 2,
-                                                s->p.renumber_levels >= 4 ? 4 : 1);
+                                                s->p.renumber_levels == 4 ? 4 : 1, s->p.renumber_levels == 5);
           for (auto& k : sp.src)
             if (k >= 0) k = opos[k];
           for (auto& k : sp.bd_src) k = opos[k];
@@ -1143,6 +1254,23 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
         if (pmx::fe_defect_partials(s->K) > std::max(4096, pmx::fe_defect_partials(s->A))) {
           dfree(s->partial);
           s->partial = dmalloc<double>(pmx::fe_defect_partials(s->K));
+        }
+        if (mixed && s->p.refinement == 1 && F.A.sell.off && F.A.sell.gsum && F.A.sell.pack == 1) {
+          // iterative refinement: FP32 Krylov blocks and the inner operator
+          const int m = std::min(s->p.gmres_restart, std::max(1, s->p.max_n_iterations));
+          s->V32 = dzeros<float>((size_t)(m + 1) * n, st);
+          s->Z32 = dzeros<float>((size_t)m * n, st);
+          s->w32 = dzeros<float>(n, st);
+          s->zero32 = dzeros<float>(std::max(1, F.active), st);
+          if (!s->p.refinement_inner_bf16) {
+            s->kval32 = dmalloc<float>(F.A.sell.total);
+            pmx::sell_fill_f32(F.A, s->kval32, st);
+          }
+          s->ir = true;
+          for (auto*& v : s->V) dfree(v);
+          for (auto*& z : s->Z) dfree(z);
+          s->V.clear();
+          s->Z.clear();
         }
       }
     }

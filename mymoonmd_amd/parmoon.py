@@ -359,7 +359,9 @@ def solver_params(**kw):
               "multigrid_coarse_max_n_iterations": "mg_coarse_max_n_iterations",
               "jacobi_damp": "jacobi_damp", "multigrid_vanka_damp_factor": "vanka_damp",
               "zero_start_shortcut": "zero_start_shortcut", "renumber_levels": "renumber_levels",
-              "transfer_sell": "transfer_sell", "vanka_inverse": "vanka_inverse"}
+              "transfer_sell": "transfer_sell", "vanka_inverse": "vanka_inverse", "refinement": "refinement",
+              "refinement_inner_reduction": "refinement_inner_reduction",
+              "refinement_inner_bf16": "refinement_inner_bf16"}
     for k, v in kw.items():
         if k == "iterative_solver_type":
             p.iterative_solver_type = IT_SOLVER[v]

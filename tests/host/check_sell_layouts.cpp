@@ -18,11 +18,36 @@ void check_level(const FESpace& fs) {
   CSRPattern pp; std::vector<int> opos;
   permute_csr(p, nullptr, perm, pinv, pp, nullptr, &opos, true);
   SellPattern a = build_sell(pp, 4096, false), b = build_sell(pp, 4096, true), c4 = build_sell(pp, 4096, true, 4);
+  SellPattern pt = build_sell(pp, 4096, true, 1, true);
   const int na = pp.active_bound, n = pp.n_rows;
   std::printf("n %d active %d nnz %zu slots32 %zu slots16 %zu rel16 %d n_bd %zu\n", n, na, pp.col.size(), a.src.size(),
               b.src.size(), (int)!b.off.empty(), b.bd_src.size());
   CHECK(c4.off.empty() == b.off.empty());
-  for (const SellPattern* s : {&a, &b, &c4}) {
+  CHECK(pt.off.empty() == b.off.empty());
+  if (!pt.off.empty()) {
+    // pattern layout: the lanes of a slice read the slice's pattern offsets;
+    // an empty lane's reads (first row of the slice + offset) stay in the
+    // active rows
+    std::printf("  patterns %d slices %d (length-sorted %d)\n", pt.n_patterns, pt.n_slices, b.n_slices);
+    CHECK((int)pt.slice_pat.size() == pt.n_slices);
+    for (int sl = 0; sl < pt.n_slices; ++sl) {
+      const int s0 = pt.slice_ptr[sl], width = (pt.slice_ptr[sl + 1] - s0) / 64;
+      const int r0 = pt.row[(size_t)sl * 64];
+      CHECK(r0 >= 0);
+      CHECK(pt.slice_pat[sl] >= 0 && (size_t)(pt.slice_pat[sl] + width) <= pt.pat_off.size());
+      for (int l = 0; l < 64; ++l) {
+        const int r = pt.row[(size_t)sl * 64 + l];
+        for (int j = 0; j < width; ++j) {
+          const int o = pt.pat_off[pt.slice_pat[sl] + j];
+          if (r >= 0) CHECK(pt.off[(size_t)s0 + (size_t)j * 64 + l] == o);
+          else CHECK(pt.src[(size_t)s0 + (size_t)j * 64 + l] == -1);
+          const int base = r >= 0 ? r : r0;
+          CHECK(base + o >= 0 && base + o < na);
+        }
+      }
+    }
+  }
+  for (const SellPattern* s : {&a, &b, &c4, &pt}) {
     const bool rel = !s->off.empty();
     if (s->pack == 4) {
       for (int sl = 0; sl <= s->n_slices; ++sl) CHECK(s->slice_ptr[sl] % 256 == 0);
