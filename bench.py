@@ -57,11 +57,15 @@ def parse():
     ap.add_argument("--cpu-refine", type=int, default=4, help="refinements of the CPU sample")
     ap.add_argument("--renumber-levels", type=int, default=None,
                     help="solver parameter renumber_levels (default: the solver's default)")
+    ap.add_argument("--inner-reduction", type=float, default=None,
+                    help="solver parameter refinement_inner_reduction of the mixed path (default: the solver's)")
     return ap.parse_args()
 
 
-def solver_params(pm, precision, omega, max_it=500, renumber=None):
+def solver_params(pm, precision, omega, max_it=500, renumber=None, inner=None):
     extra = {} if renumber is None else {"renumber_levels": renumber}
+    if inner is not None:
+        extra["refinement_inner_reduction"] = inner
     return pm.solver_params(**extra,
         iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
         residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=max_it,
@@ -332,7 +336,8 @@ def main():
     t_update = {}
     for prec in ("mixed", "fp64"):
         t0 = time.perf_counter()
-        sol = pm.Solver(ctx, solver_params(pm, prec, args.jacobi_damp, renumber=args.renumber_levels))
+        sol = pm.Solver(ctx, solver_params(pm, prec, args.jacobi_damp, renumber=args.renumber_levels,
+                                           inner=args.inner_reduction))
         sol.update_matrix(mats, spaces)
         t_update[prec] = time.perf_counter() - t0
         solvers[prec] = sol
@@ -403,20 +408,25 @@ def main():
     kt = {k: A.time_kernel(k, reps=20) for k in ("apply_fp64", "jacobi_fp64", "apply_bf16", "jacobi_bf16")}
     bytes_alg = {"apply_fp64": 12 * nnz + 20 * n, "jacobi_fp64": 12 * nnz + 20 * n + 24 * n,
                  "apply_bf16": 6 * nnz + 16 * n, "jacobi_bf16": 6 * nnz + 16 * n + 12 * n}
-    # the solver's sweep reads the folded 16-bit copy: per entry of an active
-    # row in an active column a bf16 value and an int16 offset (4 B); per
-    # active row the row map, x, b, D^-1, row scale, the Dirichlet-column sum
-    # and the written x' (28 B); per Dirichlet row x, b, D^-1, x' (16 B)
+    # the solver's sweep reads the finest level's folded copy with slices of
+    # equal offset patterns: per entry of an active row in an active column
+    # only its value (bf16: 2 B, FP64: 8 B; the offsets are read once per
+    # slice); per active row the row map, x, b, D^-1 and the written x' (bf16:
+    # 4 B each + the FP32 row scale; FP64: 8 B each) and, outside the
+    # refinement, the Dirichlet-column sum; per Dirichlet row x, b, D^-1, x'.
+    # Inside the refinement (mixed) the vectors vanish on the Dirichlet rows
+    # and the sums are not read.
     n_act, nnz_aa = folded_entries(A, fs.active_bound)
-    bytes_solver = {"jacobi_bf16": 4 * nnz_aa + 28 * n_act + 16 * (n - n_act),
-                    "jacobi_fp64": 10 * nnz_aa + 44 * n_act + 32 * (n - n_act)}
+    n_slices = (n_act + 63) // 64
+    bytes_solver = {"jacobi_bf16": 2 * nnz_aa + 24 * n_act + 16 * (n - n_act) + 12 * n_slices,
+                    "jacobi_fp64": 8 * nnz_aa + 44 * n_act + 32 * (n - n_act) + 12 * n_slices}
     dom_kernel = "jacobi_bf16"
     ach = bytes_solver[dom_kernel] / (kt_solver[dom_kernel] * 1e-3) / 1e9
     roof = {"bound": "hbm",
-            "kernel": "k_sell_spmv<MODE_JACOBI, bf16, float, int16 offsets, folded Dirichlet sums> "
-                      "(finest level, solver-internal lexicographic order)",
+            "kernel": "k_sell_pat<MODE_JACOBI, bf16, float, GSUM=false> (finest level, solver-internal "
+                      "lexicographic order, slices of equal offset patterns, as the refinement's cycles run it)",
             "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic("solver_" + dom_kernel),
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic("solver_pat_" + dom_kernel),
             "bytes_per_launch": bytes_solver[dom_kernel], "avg_launch_ms": round(kt_solver[dom_kernel], 4)}
     spmv_fp64 = bytes_alg["apply_fp64"] / (kt["apply_fp64"] * 1e-3) / 1e9
 

@@ -611,9 +611,13 @@ typedef struct {
                                   FP64 residual; iterations count the inner ones.  0: FGMRES
                                   in FP64 throughout, only the preconditioner in FP32. */
   double refinement_inner_reduction; /* reduction of the inner residual estimate per
-                                  refinement step (default 1e-5) */
+                                  refinement step (default 1e-4) */
   int refinement_inner_bf16;   /* 1: the inner operator reads the bf16 row-scaled values of
                                   the multigrid's finest level instead of an FP32 copy */
+  int graph_cycle;             /* no reference counterpart.  1 (default): with a direct
+                                  coarse solve the scalar multigrid cycle is captured once
+                                  as a HIP graph and replayed (the same kernels on the same
+                                  buffers, fewer launch gaps); 0: launched kernel by kernel */
 } pmx_solver_params;
 /* Defaults of Solver<>::default_solver_database (src/Solver/Solver.C:20-117)
  * merged with Multigrid::default_multigrid_database (src/Multigrid/Multigrid.C:422-514). */

@@ -82,7 +82,8 @@ class SolverParams(ctypes.Structure):
                 ("jacobi_damp", c_double), ("precision", c_int), ("vanka_damp", c_double),
                 ("zero_start_shortcut", c_int), ("renumber_levels", c_int),
                 ("transfer_sell", c_int), ("vanka_inverse", c_int), ("refinement", c_int),
-                ("refinement_inner_reduction", c_double), ("refinement_inner_bf16", c_int)]
+                ("refinement_inner_reduction", c_double), ("refinement_inner_bf16", c_int),
+                ("graph_cycle", c_int)]
 
 
 class PodParams(ctypes.Structure):

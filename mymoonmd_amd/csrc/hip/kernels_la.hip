@@ -281,8 +281,9 @@ __global__ __launch_bounds__(256) void k_sell_spmv(
 // rows; their values are 0 and their results are not stored.  GSUM false:
 // the Dirichlet-column sums are taken as 0, valid when x vanishes on the
 // Dirichlet rows (the corrections of an iterative refinement).
-template <int MODE, typename VT, typename XT, bool GSUM>
-__global__ __launch_bounds__(256) void k_sell_pat(
+typedef float v4f_pat __attribute__((ext_vector_type(4)));
+template <int MODE, typename VT, typename XT, bool GSUM, int PACK = 1, int WPB = 4, int CH = 16>
+__global__ __launch_bounds__(64 * WPB) void k_sell_pat(
     int n_slices, int sell_blocks, int n, int active, const int* __restrict__ sptr,
     const int* __restrict__ rowmap, const int* __restrict__ slice_pat, const int* __restrict__ pat_off,
     const VT* __restrict__ va, const float* __restrict__ rscale, const XT* __restrict__ x,
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(256) void k_sell_pat(
     const int per = sell_blocks >> 3;
     const int vb = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     const int lane = threadIdx.x & 63;
-    const int slice = __builtin_amdgcn_readfirstlane(vb * 4 + (int)(threadIdx.x >> 6));
+    const int slice = __builtin_amdgcn_readfirstlane(vb * WPB + (int)(threadIdx.x >> 6));
     if (slice < n_slices) {
       const int r = rowmap[slice * 64 + lane];
       const int r0 = __builtin_amdgcn_readfirstlane(rowmap[slice * 64]);
@@ -306,12 +307,70 @@ __global__ __launch_bounds__(256) void k_sell_pat(
       const VT* vp = va + s0 + lane;
       AT acc = 0;
       if constexpr (GSUM) acc = (AT)gsum[rb];
+      // the row's own data, loaded before the sum so that it overlaps it
+      AT br = 0, dr = 0, xr = 0, sr = 1;
+      if (r >= 0) {
+        if constexpr (MODE == MODE_DEFECT || MODE == MODE_JACOBI) br = (AT)b[r];
+        if constexpr (MODE == MODE_JACOBI) {
+          dr = (AT)dinv[r];
+          xr = (AT)x[r];
+        }
+        if (rscale) sr = (AT)rscale[r];
+      }
       auto val = [&](int j) -> AT {
         if constexpr (sizeof(VT) == 2) return (AT)bf16_to_float(__builtin_nontemporal_load(vp + j * 64));
         else return (AT)__builtin_nontemporal_load(vp + j * 64);
       };
       int j = 0;
-      for (; j + 16 <= width; j += 16) {
+      if constexpr (PACK == 4) {
+        // slot j of this lane at s0 + (j / 4) 256 + 4 lane + j % 4: one load
+        // per four slots; widths are multiples of 4 (padding: offset 0, value 0)
+        auto group = [&](int q, AT* v) {
+          const VT* g = va + s0 + (size_t)q * 256 + 4 * lane;
+          if constexpr (sizeof(VT) == 2) {
+            const uint64_t w = __builtin_nontemporal_load((const uint64_t*)g);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] = (AT)bf16_to_float((uint16_t)(w >> (16 * t)));
+          } else if constexpr (sizeof(VT) == 4) {
+            const v4f_pat w = __builtin_nontemporal_load((const v4f_pat*)g);
+            v[0] = (AT)w.x; v[1] = (AT)w.y; v[2] = (AT)w.z; v[3] = (AT)w.w;
+          } else {
+            const v2d_sell a0 = __builtin_nontemporal_load((const v2d_sell*)g);
+            const v2d_sell a1 = __builtin_nontemporal_load((const v2d_sell*)g + 1);
+            v[0] = (AT)a0.x; v[1] = (AT)a0.y; v[2] = (AT)a1.x; v[3] = (AT)a1.y;
+          }
+        };
+        for (; j + 16 <= width; j += 16) {
+          int c[16];
+          AT v[16];
+#pragma unroll
+          for (int u = 0; u < 16; ++u) c[u] = rb + po[j + u];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) group((j >> 2) + t, v + 4 * t);
+#pragma unroll
+          for (int u = 0; u < 16; ++u) acc += v[u] * (AT)x[c[u]];
+        }
+        for (; j < width; j += 4) {
+          int c[4];
+          AT v[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) c[u] = rb + po[j + u];
+          group(j >> 2, v);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc += v[u] * (AT)x[c[u]];
+        }
+      }
+      for (; PACK == 1 && j + CH <= width; j += CH) {
+        int c[CH];
+        AT v[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) c[u] = rb + po[j + u];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) v[u] = val(j + u);
+#pragma unroll
+        for (int u = 0; u < CH; ++u) acc += v[u] * (AT)x[c[u]];
+      }
+      for (; PACK == 1 && CH > 16 && j + 16 <= width; j += 16) {
         int c[16];
         AT v[16];
 #pragma unroll
@@ -321,7 +380,7 @@ __global__ __launch_bounds__(256) void k_sell_pat(
 #pragma unroll
         for (int u = 0; u < 16; ++u) acc += v[u] * (AT)x[c[u]];
       }
-      for (; j + 8 <= width; j += 8) {
+      for (; PACK == 1 && j + 8 <= width; j += 8) {
         int c[8];
         AT v[8];
 #pragma unroll
@@ -331,20 +390,20 @@ __global__ __launch_bounds__(256) void k_sell_pat(
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc += v[u] * (AT)x[c[u]];
       }
-      for (; j < width; ++j) acc += val(j) * (AT)x[rb + po[j]];
+      for (; PACK == 1 && j < width; ++j) acc += val(j) * (AT)x[rb + po[j]];
       if (r >= 0) {
-        if (rscale) acc *= (AT)rscale[r];
+        if (rscale) acc *= sr;
         if (MODE == MODE_SCALED_ADD) {
           y[r] += (XT)(a * (double)acc);
         } else if (MODE == MODE_APPLY) {
           y[r] = (XT)acc;
         } else if (MODE == MODE_DEFECT) {
-          const XT d = (XT)((AT)b[r] - acc);
+          const XT d = (XT)(br - acc);
           y[r] = d;
           d2 = (double)d * (double)d;
         } else {
-          const AT res = (AT)b[r] - acc;
-          y[r] = (XT)((AT)x[r] + (AT)a * (AT)dinv[r] * res);
+          const AT res = br - acc;
+          y[r] = (XT)(xr + (AT)a * dr * res);
         }
       }
     }
@@ -463,6 +522,25 @@ __global__ __launch_bounds__(256) void k_sell_jacobi_from_zero(
   }
 }
 
+// k_sell_jacobi_from_zero for an x that vanishes on every row (Dirichlet
+// rows too): no Dirichlet-column products, the same expressions otherwise
+template <typename XT>
+__global__ __launch_bounds__(256) void k_jacobi_from_zero_nodir(int n, int active, const XT* __restrict__ x,
+                                                                const XT* __restrict__ b,
+                                                                const XT* __restrict__ dinv, XT* __restrict__ y,
+                                                                double a) {
+  using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i < active) {
+    const AT res = (AT)b[i];
+    y[i] = (XT)((AT)a * (AT)dinv[i] * res);
+  } else {
+    const AT res = (AT)(b[i] - x[i]);
+    y[i] = (XT)((AT)x[i] + (AT)a * (AT)dinv[i] * res);
+  }
+}
+
 // gsum[r] = sum of the Dirichlet-column products of active row r, in column
 // order (the folded 16-bit layout; before the row scale)
 template <typename VT, typename XT>
@@ -484,9 +562,11 @@ __global__ __launch_bounds__(256) void k_sell_dirichlet_sums(int active, const i
   gsum[i] = (XT)acc;
 }
 
+// four slices (waves) per workgroup
 int sell_grid(const DevCSR& A, int* sell_blocks) {
-  *sell_blocks = (((A.sell.n_slices + 3) / 4 + 7) / 8) * 8;
-  return *sell_blocks + (A.n - A.active + 255) / 256;
+  const int w = 4;
+  *sell_blocks = (((A.sell.n_slices + w - 1) / w + 7) / 8) * 8;
+  return *sell_blocks + (A.n - A.active + 64 * w - 1) / (64 * w);
 }
 
 template <int MODE, typename VT, typename XT>
@@ -500,11 +580,19 @@ void launch_sell(const DevCSR& A, const VT* va, const VT* bd_val, const float* r
     throw std::runtime_error("SELL copy without slots, values or columns");
   if (A.sell.pack != 1 && !(A.sell.pack == 4 && A.sell.off && A.sell.gsum))
     throw std::runtime_error("packed SELL slots need the folded 16-bit layout");
-  if (A.sell.slice_pat && A.sell.off && A.sell.gsum && A.sell.pack == 1) {
+  if (A.sell.slice_pat && A.sell.off && A.sell.gsum) {
     if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
-    hipLaunchKernelGGL((k_sell_pat<MODE, VT, XT, true>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb, A.n,
-                       A.active, A.sell.slice_ptr, A.sell.row, A.sell.slice_pat, A.sell.pat_off, va, rscale, x, b,
-                       dinv, y, a, partial, (const XT*)A.sell.gsum);
+    const XT* g = A.sell.dir_zero ? nullptr : (const XT*)A.sell.gsum;
+#define PMX_PAT(GS, PK, C)                                                                                           \
+  hipLaunchKernelGGL((k_sell_pat<MODE, VT, XT, GS, PK, 4, C>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb,  \
+                     A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.slice_pat, A.sell.pat_off, va, rscale, x,  \
+                     b, dinv, y, a, partial, g)
+    if (A.sell.pack == 4) {
+      if (g) PMX_PAT(true, 4, 16); else PMX_PAT(false, 4, 16);
+    } else {
+      if (g) PMX_PAT(true, 1, 16); else PMX_PAT(false, 1, 16);
+    }
+#undef PMX_PAT
   } else if (A.sell.pack == 4) {
     if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
     hipLaunchKernelGGL((k_sell_spmv<MODE, VT, XT, true, true, 4>), dim3(grid), dim3(256), 0, st, A.sell.n_slices,
@@ -1029,6 +1117,12 @@ static bool jacobi_from_zero(const DevCSR& A, const VT* bd_val, const float* rsc
                              const XT* b, const XT* dinv, XT* xn, double omega, hipStream_t st) {
   if (A.lpr != 0 || !A.sell.bd_ptr || (!bd_val && A.sell.n_bd > 0)) return false;
   if (A.n == 0) return true;
+  if (A.sell.dir_zero) {
+    // x vanishes everywhere: x' = w D^-1 b (the row sums are exactly zero)
+    hipLaunchKernelGGL((k_jacobi_from_zero_nodir<XT>), dim3((A.n + 255) / 256), dim3(256), 0, st, A.n, A.active, x,
+                       b, dinv, xn, omega);
+    return true;
+  }
   XT* g = nullptr;
   if (A.sell.off && A.sell.gsum) {
     if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
@@ -1041,7 +1135,7 @@ static bool jacobi_from_zero(const DevCSR& A, const VT* bd_val, const float* rsc
 
 template <typename VT, typename XT>
 static void dirichlet_sums(const DevCSR& A, const VT* bd_val, const XT* x, hipStream_t st) {
-  if (!(A.sell.off && A.sell.gsum) || A.active <= 0) return;
+  if (!(A.sell.off && A.sell.gsum) || A.active <= 0 || A.sell.dir_zero) return;
   if (A.sell.gsum_bytes != (int)sizeof(XT)) throw std::runtime_error("folded SELL layout: sums of another precision");
   if (A.sell.n_bd > 0 && (!bd_val || !A.sell.bd_ptr)) throw std::runtime_error("folded SELL layout without Dirichlet values");
   hipLaunchKernelGGL((k_sell_dirichlet_sums<VT, XT>), dim3((A.active + 255) / 256), dim3(256), 0, st, A.active,
@@ -1354,15 +1448,20 @@ void vec_multi_axpy_f32(int n, int k, const double* c, const float* Z, size_t ld
 template <typename VT>
 static void apply_active(const DevCSR& A, const VT* vals, const float* rscale, const float* x, float* y,
                          const float* zero_gsum, hipStream_t st) {
-  if (!A.sell.off || A.sell.pack != 1 || A.sell.n_slices == 0)
+  if (!A.sell.off || (A.sell.pack != 1 && !A.sell.slice_pat) || A.sell.n_slices == 0)
     throw std::runtime_error("the refinement operator needs the folded 16-bit SELL layout");
   int sb = 0;
   const int grid = sell_grid(A, &sb);
-  if (A.sell.slice_pat)
-    hipLaunchKernelGGL((k_sell_pat<MODE_APPLY, VT, float, false>), dim3(grid), dim3(256), 0, st, A.sell.n_slices, sb,
-                       A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.slice_pat, A.sell.pat_off, vals, rscale,
-                       x, (const float*)nullptr, (const float*)nullptr, y, 1.0, (double*)nullptr,
-                       (const float*)nullptr);
+#define PMX_PATA(PK, C)                                                                                              \
+  hipLaunchKernelGGL((k_sell_pat<MODE_APPLY, VT, float, false, PK, 4, C>), dim3(grid), dim3(256), 0, st,           \
+                     A.sell.n_slices, sb, A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.slice_pat,            \
+                     A.sell.pat_off, vals, rscale, x, (const float*)nullptr, (const float*)nullptr, y, 1.0,          \
+                     (double*)nullptr, (const float*)nullptr)
+  if (A.sell.slice_pat && A.sell.gsum) {
+    if (A.sell.pack == 4) PMX_PATA(4, 16);
+    else PMX_PATA(1, 16);
+  }
+#undef PMX_PATA
   else
     hipLaunchKernelGGL((k_sell_spmv<MODE_APPLY, VT, float, true, true>), dim3(grid), dim3(256), 0, st,
                        A.sell.n_slices, sb, A.n, A.active, A.sell.slice_ptr, A.sell.row, A.sell.col, A.sell.off,

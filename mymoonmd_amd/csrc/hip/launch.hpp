@@ -35,6 +35,12 @@ struct DevSELL {
   // Dirichlet part of the vector the kernels read changes
   void* gsum = nullptr;
   int gsum_bytes = 0;
+  // 1: every vector this copy is applied to vanishes on the Dirichlet rows
+  // (coarse multigrid levels, whose right-hand side is reset there; the
+  // finest level inside an iterative refinement), so the Dirichlet-column
+  // sums are zero: the sweeps from zero are elementwise, gsum (zeros) is never
+  // recomputed and the pattern kernel does not read it
+  int dir_zero = 0;
   int pack = 1;                   // packed slots (see SellPattern::pack)
   // pattern layout (SellPattern::slice_pat / pat_off): the 64 rows of a
   // slice share their active-column offsets, read once per slot by the wave

@@ -454,7 +454,7 @@ SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16, int pack, boo
   constexpr int C = 64;
   SellPattern s;
   if (!rel16 || pack != 4) pack = 1;
-  if (!rel16 || pack != 1) patterns = false;
+  if (!rel16) patterns = false;
   s.pack = pack;
   const int na = p.active_bound;
   s.n_active = na;
@@ -532,9 +532,12 @@ SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16, int pack, boo
                      });
     s.n_slices = (int)slices.size();
     s.n_patterns = np;
+    // pattern widths padded to a multiple of pack (padding offsets 0: the
+    // row itself, value 0)
+    auto pwidth = [&](int k) { return (nact[rep[k]] + pack - 1) / pack * pack; };
     std::vector<int> pat_ptr(np + 1, 0);
-    for (int k = 0; k < np; ++k) pat_ptr[k + 1] = pat_ptr[k] + nact[rep[k]];
-    s.pat_off.resize(pat_ptr[np]);
+    for (int k = 0; k < np; ++k) pat_ptr[k + 1] = pat_ptr[k] + pwidth(k);
+    s.pat_off.assign(pat_ptr[np], 0);
     for (int k = 0; k < np; ++k)
       for (int j = 0; j < nact[rep[k]]; ++j) s.pat_off[pat_ptr[k] + j] = p.col[p.row_ptr[rep[k]] + j] - rep[k];
     s.row.assign((size_t)s.n_slices * C, -1);
@@ -544,7 +547,7 @@ SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16, int pack, boo
       const int q0 = slices[sl].first, k = slices[sl].second;
       for (int l = 0; l < C && q0 + l < cnt[k + 1]; ++l) s.row[(size_t)sl * C + l] = by_pat[q0 + l];
       s.slice_pat[sl] = pat_ptr[k];
-      s.slice_ptr[sl + 1] = s.slice_ptr[sl] + nact[rep[k]] * C;
+      s.slice_ptr[sl + 1] = s.slice_ptr[sl] + pwidth(k) * C;
     }
   } else {
     std::vector<int> order(na);

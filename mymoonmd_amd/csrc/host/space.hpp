@@ -156,7 +156,7 @@ inline size_t sell_pos(size_t s0, int j, int l, int pack) {
 }
 // rel16: build the 16-bit offset layout; if an offset does not fit, the
 // result has an empty `off` and the 32-bit layout
-// patterns: group rows by offset pattern (rel16 and pack 1 only); when the
+// patterns: group rows by offset pattern (rel16 only); when the
 // rows have more than max_patterns distinct patterns the length-sorted
 // layout is built instead
 SellPattern build_sell(const CSRPattern& p, int sigma, bool rel16 = false, int pack = 1, bool patterns = false,

@@ -240,6 +240,10 @@ struct pmx_solver {
   // copy (nullptr: its bf16 values are used) and zeros for its
   // Dirichlet-column sums
   bool ir = false;
+  // the multigrid cycle captured as a HIP graph (direct coarse solve only),
+  // one per state of the finest level's dir_zero flag
+  hipGraph_t cycle_graph[2] = {nullptr, nullptr};
+  hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
   float *V32 = nullptr, *Z32 = nullptr, *w32 = nullptr, *kval32 = nullptr, *zero32 = nullptr;
   int ir_outer = 0;              // refinement steps of the last solve
   const pmx::DevCSR* op = nullptr;       // operator of the running solve
@@ -267,6 +271,12 @@ struct pmx_solver {
     dfree(kval); dfree(kbd); dfree(kb); dfree(kx);
     dfree(V32); dfree(Z32); dfree(w32); dfree(kval32); dfree(zero32);
     ir = false;
+    for (int k = 0; k < 2; ++k) {
+      if (cycle_exec[k]) (void)hipGraphExecDestroy(cycle_exec[k]);
+      if (cycle_graph[k]) (void)hipGraphDestroy(cycle_graph[k]);
+      cycle_exec[k] = nullptr;
+      cycle_graph[k] = nullptr;
+    }
     if (kown) {
       dfree(K.sell.slice_ptr); dfree(K.sell.row); dfree(K.sell.col); dfree(K.sell.src);
       dfree(K.sell.bd_ptr); dfree(K.sell.bd_src); dfree(K.sell.bd_col);
@@ -438,19 +448,48 @@ void mg_cycle(pmx_solver* s, bool mixed) {
   for (int step = 1; step <= n_steps; ++step) level = mg_cycle_step(s, step, level, mixed);
 }
 
+// The cycle is a fixed sequence of kernels and device copies when the
+// coarsest level is solved directly (no residual read back): it is captured
+// once into a HIP graph and replayed.  The graph holds the buffers of the
+// capture: the host-side pointer swaps of the smoothing steps are not redone
+// on a replay, so the level vectors stay where the captured cycle left them,
+// which is where the replays leave them too.
+void mg_cycle_run(pmx_solver* s, bool mixed) {
+  if (!s->p.graph_cycle || s->p.mg_smoother_coarse != PMX_SMOOTHER_DIRECT) {
+    mg_cycle(s, mixed);
+    return;
+  }
+  hipStream_t st = S(s);
+  const int key = s->lv.back().A.sell.dir_zero ? 1 : 0;
+  if (!s->cycle_exec[key]) {
+    HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      mg_cycle(s, mixed);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIP_CHECK(hipStreamEndCapture(st, &s->cycle_graph[key]));
+    HIP_CHECK(hipGraphInstantiate(&s->cycle_exec[key], s->cycle_graph[key], nullptr, nullptr, 0));
+  }
+  HIP_CHECK(hipGraphLaunch(s->cycle_exec[key], st));
+}
+
 // Iteration_multigrid::apply: z = MG(v)
 void mg_apply(pmx_solver* s, const double* v, double* z) {
   Level& F = s->lv.back();
   if (s->p.precision == PMX_PRECISION_MIXED) {
     if (F.perm) pmx::vec_gather(F.n, F.perm, v, F.rhs32, S(s));
     else pmx::vec_f64_to_f32(F.n, v, F.rhs32, S(s));
-    mg_cycle(s, true);
+    mg_cycle_run(s, true);
     if (F.perm) pmx::vec_scatter(F.n, F.perm, F.sol32, z, S(s));
     else pmx::vec_f32_to_f64(F.n, F.sol32, z, S(s));
   } else {
     if (F.perm) pmx::vec_gather(F.n, F.perm, v, F.rhs, S(s));
     else HIP_CHECK(hipMemcpyAsync(F.rhs, v, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
-    mg_cycle(s, false);
+    mg_cycle_run(s, false);
     if (F.perm) pmx::vec_scatter(F.n, F.perm, F.sol, z, S(s));
     else HIP_CHECK(hipMemcpyAsync(z, F.sol, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
   }
@@ -461,11 +500,11 @@ void mg_apply_internal(pmx_solver* s, const double* v, double* z) {
   Level& F = s->lv.back();
   if (s->p.precision == PMX_PRECISION_MIXED) {
     pmx::vec_f64_to_f32(F.n, v, F.rhs32, S(s));
-    mg_cycle(s, true);
+    mg_cycle_run(s, true);
     pmx::vec_f32_to_f64(F.n, F.sol32, z, S(s));
   } else {
     HIP_CHECK(hipMemcpyAsync(F.rhs, v, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
-    mg_cycle(s, false);
+    mg_cycle_run(s, false);
     HIP_CHECK(hipMemcpyAsync(z, F.sol, F.n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
   }
 }
@@ -800,7 +839,16 @@ void solve_ir_run(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
   s->ir_outer = 0;
   if (converged(s, beta, 0)) { rep->iterations = 0; rep->final_residual = beta; rep->converged = s->reached ? 1 : 0; return; }
   const double target = std::max(s->p.residual_tolerance, s->p.residual_reduction * s->initial_res);
-  float* const saved_rhs = F.rhs32;
+  // the cycle's input vanishes on the Dirichlet rows in here (see above)
+  struct DirZero {
+    Level& L;
+    bool on;
+    ~DirZero() { L.A.sell.dir_zero = 0; L.g_valid = false; }
+  } dz{F, s->p.zero_start_shortcut != 0};
+  if (dz.on) {
+    F.A.sell.dir_zero = 1;
+    if (F.g32) pmx::vec_fill_f32(std::max(1, F.active), 0.f, F.g32, st);
+  }
   int j = 0;
   while (true) {
     ++s->ir_outer;
@@ -812,14 +860,8 @@ void solve_ir_run(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
     for (i = 0; i < m && j < s->p.max_n_iterations; ++i) {
       float* vi = s->V32 + (size_t)i * n;
       float* zi = s->Z32 + (size_t)i * n;
-      F.rhs32 = vi;   // the cycle only reads its finest right-hand side
-      try {
-        mg_cycle(s, true);
-      } catch (...) {
-        F.rhs32 = saved_rhs;
-        throw;
-      }
-      F.rhs32 = saved_rhs;
+      HIP_CHECK(hipMemcpyAsync(F.rhs32, vi, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      mg_cycle_run(s, true);
       HIP_CHECK(hipMemcpyAsync(zi, F.sol32, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
       if (s->kval32) pmx::fe_apply_active_f32(F.A, s->kval32, zi, s->w32, s->zero32, st);
       else pmx::fe_apply_active_bf16(F.A, zi, s->w32, s->zero32, st);
@@ -1004,8 +1046,9 @@ int pmx_solver_default_params(pmx_solver_params* p) {
     d.transfer_sell = 1;
     d.vanka_inverse = 2;
     d.refinement = 1;
-    d.refinement_inner_reduction = 1e-5;
+    d.refinement_inner_reduction = 1e-4;
     d.refinement_inner_bf16 = 0;
+    d.graph_cycle = 1;
     *p = d;
   });
 }
@@ -1120,7 +1163,8 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
           std::vector<int> opos;
           pmx::permute_csr(*M->pattern, nullptr, perm[l], pinv[l], pp, nullptr, &opos, true);
           pmx::SellPattern sp = pmx::build_sell(pp, kSellSigmaSolver, s->p.renumber_levels >= 2,
-                                                s->p.renumber_levels == 4 ? 4 : 1, s->p.renumber_levels == 5);
+                                                (s->p.renumber_levels == 4 || s->p.renumber_levels == 6) ? 4 : 1,
+                                                s->p.renumber_levels >= 5);
           for (auto& k : sp.src)
             if (k >= 0) k = opos[k];
           for (auto& k : sp.bd_src) k = opos[k];
@@ -1182,6 +1226,10 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
           if (mixed) { L.g32 = dzeros<float>(std::max(1, L.active), st); L.A.sell.gsum = L.g32; L.A.sell.gsum_bytes = 4; }
           else { L.g64 = dzeros<double>(std::max(1, L.active), st); L.A.sell.gsum = L.g64; L.A.sell.gsum_bytes = 8; }
         }
+        // below the finest level the cycle resets the right-hand side to zero on
+        // the Dirichlet rows and the solution everywhere, so the solution
+        // vanishes on the Dirichlet rows for good: no Dirichlet-column sums
+        if (l + 1 < n_levels && s->p.zero_start_shortcut) L.A.sell.dir_zero = 1;
         if (mixed) {
           L.rhs32 = dzeros<float>(L.n, st); L.sol32 = dzeros<float>(L.n, st);
           L.def32 = dzeros<float>(L.n, st); L.tmp32 = dzeros<float>(L.n, st);
@@ -1255,7 +1303,8 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
           dfree(s->partial);
           s->partial = dmalloc<double>(pmx::fe_defect_partials(s->K));
         }
-        if (mixed && s->p.refinement == 1 && F.A.sell.off && F.A.sell.gsum && F.A.sell.pack == 1) {
+        if (mixed && s->p.refinement == 1 && F.A.sell.off && F.A.sell.gsum &&
+            (F.A.sell.pack == 1 || F.A.sell.slice_pat)) {
           // iterative refinement: FP32 Krylov blocks and the inner operator
           const int m = std::min(s->p.gmres_restart, std::max(1, s->p.max_n_iterations));
           s->V32 = dzeros<float>((size_t)(m + 1) * n, st);
@@ -1504,6 +1553,18 @@ int pmx_solver_time_level_kernel(pmx_solver* s, int level, int kernel, int reps,
             "kernel must be the solver's smoothing sweep (JACOBI_BF16 mixed, JACOBI_FP64 fp64)");
     Level& L = s->lv[level];
     hipStream_t st = S(s);
+    // the finest level's sweep as the refinement's cycles run it: with zero
+    // Dirichlet-column sums
+    const bool ir_sweep = s->ir && level + 1 == (int)s->lv.size() && s->p.zero_start_shortcut;
+    struct Restore {
+      Level& L;
+      bool on;
+      ~Restore() { if (on) { L.A.sell.dir_zero = 0; L.g_valid = false; } }
+    } restore{L, ir_sweep};
+    if (ir_sweep) {
+      L.A.sell.dir_zero = 1;
+      if (L.g32) pmx::vec_fill_f32(std::max(1, L.active), 0.f, L.g32, st);
+    }
     auto launch = [&]() {
       ensure_dirichlet_sums(s, L, mixed);
       if (mixed) pmx::fe_jacobi_bf16(L.A, L.sol32, L.rhs32, L.dinv32, L.tmp32, s->p.jacobi_damp, st);

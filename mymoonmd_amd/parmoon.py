@@ -361,7 +361,7 @@ def solver_params(**kw):
               "zero_start_shortcut": "zero_start_shortcut", "renumber_levels": "renumber_levels",
               "transfer_sell": "transfer_sell", "vanka_inverse": "vanka_inverse", "refinement": "refinement",
               "refinement_inner_reduction": "refinement_inner_reduction",
-              "refinement_inner_bf16": "refinement_inner_bf16"}
+              "refinement_inner_bf16": "refinement_inner_bf16", "graph_cycle": "graph_cycle"}
     for k, v in kw.items():
         if k == "iterative_solver_type":
             p.iterative_solver_type = IT_SOLVER[v]

@@ -5,6 +5,7 @@
 // four-slot groups start on 8-byte (16-bit arrays) / 32-byte (FP64 values)
 // boundaries.
 #include <cstdio>
+#include <functional>
 #include <cstdlib>
 #include <vector>
 #include "space.hpp"
@@ -18,12 +19,13 @@ void check_level(const FESpace& fs) {
   CSRPattern pp; std::vector<int> opos;
   permute_csr(p, nullptr, perm, pinv, pp, nullptr, &opos, true);
   SellPattern a = build_sell(pp, 4096, false), b = build_sell(pp, 4096, true), c4 = build_sell(pp, 4096, true, 4);
-  SellPattern pt = build_sell(pp, 4096, true, 1, true);
+  SellPattern pt = build_sell(pp, 4096, true, 1, true), pt4 = build_sell(pp, 4096, true, 4, true);
   const int na = pp.active_bound, n = pp.n_rows;
   std::printf("n %d active %d nnz %zu slots32 %zu slots16 %zu rel16 %d n_bd %zu\n", n, na, pp.col.size(), a.src.size(),
               b.src.size(), (int)!b.off.empty(), b.bd_src.size());
   CHECK(c4.off.empty() == b.off.empty());
   CHECK(pt.off.empty() == b.off.empty());
+  for (const SellPattern& pt : {std::cref(pt), std::cref(pt4)})
   if (!pt.off.empty()) {
     // pattern layout: the lanes of a slice read the slice's pattern offsets;
     // an empty lane's reads (first row of the slice + offset) stay in the
@@ -39,15 +41,16 @@ void check_level(const FESpace& fs) {
         const int r = pt.row[(size_t)sl * 64 + l];
         for (int j = 0; j < width; ++j) {
           const int o = pt.pat_off[pt.slice_pat[sl] + j];
-          if (r >= 0) CHECK(pt.off[(size_t)s0 + (size_t)j * 64 + l] == o);
-          else CHECK(pt.src[(size_t)s0 + (size_t)j * 64 + l] == -1);
+          const size_t k = sell_pos((size_t)s0, j, l, pt.pack);
+          if (r >= 0) CHECK(pt.off[k] == o);
+          if (r < 0) CHECK(pt.src[k] == -1);
           const int base = r >= 0 ? r : r0;
           CHECK(base + o >= 0 && base + o < na);
         }
       }
     }
   }
-  for (const SellPattern* s : {&a, &b, &c4, &pt}) {
+  for (const SellPattern* s : {&a, &b, &c4, &pt, &pt4}) {
     const bool rel = !s->off.empty();
     if (s->pack == 4) {
       for (int sl = 0; sl <= s->n_slices; ++sl) CHECK(s->slice_ptr[sl] % 256 == 0);

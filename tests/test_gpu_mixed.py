@@ -67,7 +67,7 @@ def test_refinement_reaches_the_fp64_residual(ctx, inner_bf16, reduction):
     # a bf16 inner operator limits each refinement step to a reduction of about
     # rho(A^-1 (A - A_bf16)) (~1e-2 here): its inner solves stop there
     xm, repm, sol = solve(ctx, spaces, mats, b, "mixed", reduction, refinement=1, refinement_inner_bf16=inner_bf16,
-                          refinement_inner_reduction=2e-2 if inner_bf16 else 1e-5)
+                          refinement_inner_reduction=2e-2 if inner_bf16 else 1e-4)
     assert rep64.converged and repm.converged
     rel = fp64_residual(A, b, xm, x0)
     assert rel <= 1.0001 * reduction, rel
@@ -114,3 +114,40 @@ def test_refinement_off_is_the_plain_mixed_fgmres(ctx):
     x0[fs.active_bound:] = b[fs.active_bound:]
     x, rep, _ = solve(ctx, spaces, mats, b, "mixed", 1e-8, refinement=0)
     assert rep.converged and fp64_residual(A, b, x, x0) <= 1.01e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_cycle_replays_equal_the_launched_cycle(ctx, precision):
+    """The cycle captured as a HIP graph: three solves in a row (every
+    iteration replays the graph), with eager work between them (a
+    preconditioner application, a matrix product), give the same bits and
+    iteration counts as the cycle launched kernel by kernel; so does one
+    preconditioner application."""
+    d, spaces, mats, rhs = levels(ctx, 4, 3)
+    fs, A, b = spaces[-1], mats[-1], rhs[-1]
+    out = {}
+    for graph in (0, 1):
+        x0 = torch.zeros_like(b)
+        x0[fs.active_bound:] = b[fs.active_bound:]
+        prm = pm.solver_params(iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
+                               residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=300,
+                               multigrid_cycle_type="V", multigrid_smoother="jacobi",
+                               multigrid_smoother_coarse="direct_solve", multigrid_n_pre_smooth=2,
+                               multigrid_n_post_smooth=2, jacobi_damp=0.6, precision=precision,
+                               graph_cycle=graph)
+        sol = pm.Solver(ctx, prm)
+        sol.update_matrix(mats, spaces)
+        runs = []
+        for rep in range(3):
+            x = x0.clone()
+            r = sol.solve(b, x)
+            z = torch.zeros_like(b)
+            sol.apply_preconditioner(b.clone(), z)
+            y = torch.zeros_like(b)
+            A.apply(x, y)
+            runs.append((x.cpu().numpy(), r.iterations, z.cpu().numpy()))
+        out[graph] = runs
+    for k in range(3):
+        assert np.array_equal(out[0][k][0], out[1][k][0]) and out[0][k][1] == out[1][k][1], k
+        assert np.array_equal(out[0][k][2], out[1][k][2]), k
+        assert np.array_equal(out[1][0][0], out[1][k][0]), k

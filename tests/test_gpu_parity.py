@@ -281,13 +281,14 @@ def test_zero_start_sweep_and_renumbering(ctx, precision, cycle):
     the Dirichlet-column products into per-reset sums (renumber_levels 3)
     sums them first instead of last (rounding only); packing a row's slots by
     four (renumber_levels 4) changes nothing; nor does grouping the rows of a
-    slice by their offset pattern (renumber_levels 5)."""
+    slice by their offset pattern (renumber_levels 5, and 6 with packed
+    slots)."""
     nref = 3
     d, spaces, mats, rhs = three_level_cd3d(ctx, nref, MG_KW)
     fs = spaces[-1]
     out = {}
     for shortcut, renumber in ((0, 0), (1, 0), (0, 1), (1, 1), (0, 2), (1, 2), (0, 3), (1, 3), (0, 4), (1, 4),
-                               (0, 5), (1, 5)):
+                               (0, 5), (1, 5), (0, 6), (1, 6)):
         for coarse in ("jacobi", "direct_solve"):
             prm = pm.solver_params(iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
                                    residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=100,
@@ -305,11 +306,11 @@ def test_zero_start_sweep_and_renumbering(ctx, precision, cycle):
             out[(shortcut, renumber, coarse)] = (z.cpu().numpy(), x.cpu().numpy(), rep.iterations)
     tol = 1e-12 if precision == "fp64" else 1e-5
     for coarse in ("jacobi", "direct_solve"):
-        for renumber in (0, 1, 2, 3, 4, 5):
+        for renumber in (0, 1, 2, 3, 4, 5, 6):
             a, b = out[(0, renumber, coarse)], out[(1, renumber, coarse)]
             assert np.array_equal(a[0], b[0]), (renumber, coarse)
             assert np.array_equal(a[1], b[1]) and a[2] == b[2], (renumber, coarse)
-        for r0, r1 in ((1, 2), (3, 4), (3, 5)):
+        for r0, r1 in ((1, 2), (3, 4), (3, 5), (3, 6)):
             a, b = out[(1, r0, coarse)], out[(1, r1, coarse)]
             assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2], (coarse, r0, r1)
         for r0, r1 in ((0, 1), (1, 3)):
