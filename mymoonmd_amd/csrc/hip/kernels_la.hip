@@ -16,6 +16,7 @@
 // Every kernel is HBM-bound: one row is handled by a group of LPR lanes of a
 // 64-wide wavefront so that the row's values/columns are read coalesced.
 #include <stdexcept>
+#include <vector>
 #include <type_traits>
 
 #include "common.hpp"
@@ -1480,94 +1481,284 @@ void fe_apply_active_bf16(const DevCSR& A, const float* x, float* y, const float
 }  // namespace pmx
 
 // ------------------------------------------------------------------
-// Dense in-place Gauss-Jordan inversion with partial pivoting (set-up of the
-// exact coarse-grid solve).  Step k: k_gj_pivot (one block) selects the pivot
-// row, swaps rows, scales the pivot row and saves the column factors;
-// k_gj_eliminate updates every other row.  k_gj_unpermute undoes the column
-// order implied by the row swaps.  A is n x n row-major.
+// Dense in-place inversion by blocked Gauss-Jordan elimination with partial
+// pivoting (set-up of the exact coarse-grid solve); A is n x n row-major.
+// Per block column K of width b <= 64:
+//  1. k_bgj_panel (one workgroup): LU with partial pivoting of the panel
+//     A[K:, K:K+b] in a scratch copy chooses the b pivot rows;
+//  2. k_bgj_swap: the pivot rows are swapped into rows K..K+b-1;
+//  3. the block sweep with P = A_KK^-1 (k_bgj_block_inverse, Gauss-Jordan
+//     without further pivoting on the b x b block, whose LU needs none after
+//     step 1), C = A[:, K] with its K rows zeroed, R = P A[K, :]:
+//        A[o, o] -= C R   (k_bgj_update, FP64 matrix cores, 64 x 64 tiles)
+//        A[K, o] = R,  A[o, K] = -C P,  A[K, K] = P   (k_bgj_fix_*).
+// The row swaps leave the inverse of the row-permuted matrix; its columns
+// are put back in order at the end (k_bgj_unpermute).  The trailing update
+// is a rank-b GEMM per block instead of b rank-1 sweeps over the matrix.
 namespace pmx {
 namespace {
 
-__global__ __launch_bounds__(1024) void k_gj_pivot(int n, int k, double* __restrict__ A,
-                                                    double* __restrict__ f, int* __restrict__ perm,
-                                                    int* __restrict__ singular) {
+constexpr int kBgjBlock = 64;
+typedef double v4d_bgj __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(1024) void k_bgj_panel(int n, int K, int b, const double* __restrict__ A,
+                                                     double* __restrict__ W, int* __restrict__ piv,
+                                                     int* __restrict__ singular) {
   __shared__ double sv[1024];
   __shared__ int si[1024];
-  double best = -1.0;
-  int bi = k;
-  for (int i = k + threadIdx.x; i < n; i += blockDim.x) {
-    const double v = fabs(A[(size_t)i * n + k]);
-    if (v > best) { best = v; bi = i; }
+  const int m = n - K;   // panel rows
+  for (size_t e = threadIdx.x; e < (size_t)m * b; e += blockDim.x) {
+    const int i = (int)(e / b), jj = (int)(e % b);
+    W[e] = A[(size_t)(K + i) * n + K + jj];
   }
-  sv[threadIdx.x] = best;
-  si[threadIdx.x] = bi;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      const double o = sv[threadIdx.x + s];
-      const int oi = si[threadIdx.x + s];
-      if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
-        sv[threadIdx.x] = o;
-        si[threadIdx.x] = oi;
+  for (int j = 0; j < b; ++j) {
+    double best = -1.0;
+    int bi = j;
+    for (int i = j + threadIdx.x; i < m; i += blockDim.x) {
+      const double v = fabs(W[(size_t)i * b + j]);
+      if (v > best) { best = v; bi = i; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) {
+        const double o = sv[threadIdx.x + st];
+        const int oi = si[threadIdx.x + st];
+        if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+          sv[threadIdx.x] = o;
+          si[threadIdx.x] = oi;
+        }
       }
+      __syncthreads();
+    }
+    const int p = si[0];
+    if (threadIdx.x == 0) {
+      piv[K + j] = K + p;
+      if (sv[0] == 0.0) singular[0] = 1;
+    }
+    if (p != j && threadIdx.x < b) {
+      const double t = W[(size_t)j * b + threadIdx.x];
+      W[(size_t)j * b + threadIdx.x] = W[(size_t)p * b + threadIdx.x];
+      W[(size_t)p * b + threadIdx.x] = t;
+    }
+    __syncthreads();
+    const double d = W[(size_t)j * b + j];
+    const double id = (d != 0.0) ? 1.0 / d : 0.0;
+    for (int i = j + 1 + threadIdx.x; i < m; i += blockDim.x) {
+      double* wi = W + (size_t)i * b;
+      const double l = wi[j] * id;
+      wi[j] = l;
+      for (int c = j + 1; c < b; ++c) wi[c] -= l * W[(size_t)j * b + c];
     }
     __syncthreads();
   }
-  const int p = si[0];
-  if (threadIdx.x == 0) {
-    perm[k] = p;
-    if (sv[0] == 0.0) singular[0] = 1;
-  }
-  if (p != k)
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      const double t = A[(size_t)k * n + j];
-      A[(size_t)k * n + j] = A[(size_t)p * n + j];
-      A[(size_t)p * n + j] = t;
+}
+
+// one thread per column: the swaps (K + j, piv[K + j]), j = 0 .. b-1, in order
+__global__ void k_bgj_swap(int n, int K, int b, double* __restrict__ A, const int* __restrict__ piv) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  for (int j = 0; j < b; ++j) {
+    const int r = piv[K + j];
+    if (r != K + j) {
+      const double t = A[(size_t)(K + j) * n + c];
+      A[(size_t)(K + j) * n + c] = A[(size_t)r * n + c];
+      A[(size_t)r * n + c] = t;
     }
-  __syncthreads();
-  const double piv = A[(size_t)k * n + k];
-  const double ip = (piv != 0.0) ? 1.0 / piv : 0.0;
-  __syncthreads();
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    const double v = (j == k) ? 1.0 : A[(size_t)k * n + j];
-    A[(size_t)k * n + j] = v * ip;
   }
-  for (int i = threadIdx.x; i < n; i += blockDim.x) f[i] = (i == k) ? 0.0 : A[(size_t)i * n + k];
 }
 
-__global__ __launch_bounds__(256) void k_gj_eliminate(int n, int k, double* __restrict__ A,
-                                                      const double* __restrict__ f) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)n * n) return;
-  const int i = (int)(idx / n), j = (int)(idx % n);
-  if (i == k) return;
-  const double fi = f[i];
-  if (fi == 0.0) return;
-  const double base = (j == k) ? 0.0 : A[idx];
-  A[idx] = base - fi * A[(size_t)k * n + j];
+// P = A[K:K+b, K:K+b]^-1 by in-place Gauss-Jordan without pivoting, padded
+// to bp x bp with zeros
+__global__ __launch_bounds__(256) void k_bgj_block_inverse(int n, int K, int b, int bp, const double* __restrict__ A,
+                                                           double* __restrict__ P, int* __restrict__ singular) {
+  __shared__ double B[kBgjBlock][kBgjBlock + 1];
+  for (int e = threadIdx.x; e < b * b; e += blockDim.x) B[e / b][e % b] = A[(size_t)(K + e / b) * n + K + e % b];
+  __syncthreads();
+  for (int k = 0; k < b; ++k) {
+    const double d = B[k][k];
+    if (d == 0.0) {
+      if (threadIdx.x == 0) singular[0] = 1;
+      return;
+    }
+    const double pk = 1.0 / d;
+    __syncthreads();
+    if ((int)threadIdx.x < b) B[k][threadIdx.x] = (threadIdx.x == (unsigned)k) ? pk : B[k][threadIdx.x] * pk;
+    __syncthreads();
+    // rows i != k: row_i -= f row_k, then B[i][k] = -f pk
+    for (int e = threadIdx.x; e < b * b; e += blockDim.x) {
+      const int i = e / b, c = e % b;
+      if (i == k) continue;
+      const double f = B[i][k];
+      if (c != k) B[i][c] -= f * B[k][c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < b; i += blockDim.x)
+      if (i != k) B[i][k] = -B[i][k] * pk;
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < bp * bp; e += blockDim.x) {
+    const int t = e / bp, c = e % bp;
+    P[e] = (t < b && c < b) ? B[t][c] : 0.0;
+  }
 }
 
-// columns: apply the swaps (k, perm[k]) in reverse order to every row
-__global__ void k_gj_unpermute(int n, double* __restrict__ A, const int* __restrict__ perm) {
+// C[i][s] = A[i][K + s] (zero for the rows K..K+b-1 and for s >= b)
+__global__ void k_bgj_make_c(int n, int K, int b, int bp, const double* __restrict__ A, double* __restrict__ C) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * bp) return;
+  const int i = (int)(e / bp), c = (int)(e % bp);
+  C[e] = (c < b && (i < K || i >= K + b)) ? A[(size_t)i * n + K + c] : 0.0;
+}
+
+// R[t][c] = sum_s P[t][s] A[K + s][c] for t < b (zero rows up to bp)
+__global__ void k_bgj_make_r(int n, int K, int b, int bp, const double* __restrict__ A, const double* __restrict__ P,
+                             double* __restrict__ R) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  double a[kBgjBlock];   // register array: every index below is a constant after unrolling
+#pragma unroll
+  for (int s = 0; s < kBgjBlock; ++s) a[s] = (s < b) ? A[(size_t)(K + s) * n + c] : 0.0;
+  for (int t = 0; t < bp; ++t) {
+    double acc = 0.0;
+#pragma unroll
+    for (int s = 0; s < kBgjBlock; ++s)
+      if (s < b) acc += P[t * bp + s] * a[s];
+    R[(size_t)t * n + c] = acc;
+  }
+}
+
+// A -= C R (n x bp times bp x n) on 64 x 64 tiles with v_mfma_f64_16x16x4f64:
+// wave w of a workgroup owns tile rows 16w .. 16w+15 and four 16-column
+// MFMA tiles; operands straight from the (cache-resident) C and R
+__global__ __launch_bounds__(256) void k_bgj_update(int n, int bp, const double* __restrict__ C,
+                                                    const double* __restrict__ R, double* __restrict__ A) {
+  const int ti = blockIdx.y * 64, tj = blockIdx.x * 64;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int arow = ti + 16 * w + (lane & 15);
+  v4d_bgj acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = v4d_bgj{0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < bp; k0 += 4) {
+    const int k = k0 + (lane >> 4);
+    const double av = (arow < n) ? C[(size_t)arow * bp + k] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = tj + 16 * j + (lane & 15);
+      const double bv = (col < n) ? R[(size_t)k * n + col] : 0.0;
+      acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int row = ti + 16 * w + 4 * t + (lane >> 4), col = tj + 16 * j + (lane & 15);
+      if (row < n && col < n) A[(size_t)row * n + col] -= acc[j][t];
+    }
+}
+
+// rows K..K+b-1: R outside the block columns, P inside
+__global__ void k_bgj_fix_rows(int n, int K, int b, int bp, const double* __restrict__ R, const double* __restrict__ P,
+                               double* __restrict__ A) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  for (int t = 0; t < b; ++t)
+    A[(size_t)(K + t) * n + c] = (c >= K && c < K + b) ? P[t * bp + (c - K)] : R[(size_t)t * n + c];
+}
+
+// the other rows' block columns: -C P
+__global__ void k_bgj_fix_cols(int n, int K, int b, int bp, const double* __restrict__ C, const double* __restrict__ P,
+                               double* __restrict__ A) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double* row = A + (size_t)i * n;
-  for (int k = n - 1; k >= 0; --k) {
-    const int p = perm[k];
-    if (p != k) { const double t = row[k]; row[k] = row[p]; row[p] = t; }
+  if (i >= n || (i >= K && i < K + b)) return;
+  double ci[kBgjBlock];   // register array, as in k_bgj_make_r
+#pragma unroll
+  for (int t = 0; t < kBgjBlock; ++t) ci[t] = (t < b) ? C[(size_t)i * bp + t] : 0.0;
+  for (int s = 0; s < b; ++s) {
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < kBgjBlock; ++t)
+      if (t < b) acc += ci[t] * P[t * bp + s];
+    A[(size_t)i * n + K + s] = -acc;
   }
+}
+
+// out[r][c] = A[r][q[c]]
+__global__ void k_bgj_unpermute(int n, const double* __restrict__ A, const int* __restrict__ q, double* __restrict__ out) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * n) return;
+  const int r = (int)(e / n), c = (int)(e % n);
+  out[e] = A[(size_t)r * n + q[c]];
+}
+
+template <typename T>
+T* bgj_alloc(size_t n) {
+  void* p = nullptr;
+  if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) throw std::runtime_error("dense inverse: out of device memory");
+  return static_cast<T*>(p);
 }
 
 }  // namespace
 
-void dense_invert(int n, double* A, double* f, int* perm, int* singular, hipStream_t st) {
-  const size_t nn = (size_t)n * n;
-  const int grid = (int)((nn + 255) / 256);
-  for (int k = 0; k < n; ++k) {
-    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, st, n, k, A, f, perm, singular);
-    hipLaunchKernelGGL(k_gj_eliminate, dim3(grid), dim3(256), 0, st, n, k, A, f);
+bool dense_invert(int n, double* A, hipStream_t st) {
+  if (n <= 0) return true;
+  const int nb = kBgjBlock;
+  double* W = bgj_alloc<double>((size_t)n * nb);
+  double* C = bgj_alloc<double>((size_t)n * nb);
+  double* R = bgj_alloc<double>((size_t)n * nb);
+  double* P = bgj_alloc<double>((size_t)nb * nb);
+  double* T = bgj_alloc<double>((size_t)n * n);
+  int* piv = bgj_alloc<int>(n);
+  int* q = bgj_alloc<int>(n);
+  int* sing = bgj_alloc<int>(1);
+  auto release = [&]() {
+    (void)hipFree(W); (void)hipFree(C); (void)hipFree(R); (void)hipFree(P);
+    (void)hipFree(T); (void)hipFree(piv); (void)hipFree(q); (void)hipFree(sing);
+  };
+  (void)hipMemsetAsync(sing, 0, sizeof(int), st);
+  const int g1 = (n + 255) / 256;
+  const dim3 tiles((n + 63) / 64, (n + 63) / 64);
+  for (int K = 0; K < n; K += nb) {
+    const int b = n - K < nb ? n - K : nb;
+    const int bp = (b + 3) / 4 * 4;
+    hipLaunchKernelGGL(k_bgj_panel, dim3(1), dim3(1024), 0, st, n, K, b, A, W, piv, sing);
+    hipLaunchKernelGGL(k_bgj_swap, dim3(g1), dim3(256), 0, st, n, K, b, A, piv);
+    hipLaunchKernelGGL(k_bgj_block_inverse, dim3(1), dim3(256), 0, st, n, K, b, bp, A, P, sing);
+    hipLaunchKernelGGL(k_bgj_make_c, dim3((int)(((size_t)n * bp + 255) / 256)), dim3(256), 0, st, n, K, b, bp, A, C);
+    hipLaunchKernelGGL(k_bgj_make_r, dim3(g1), dim3(256), 0, st, n, K, b, bp, A, P, R);
+    hipLaunchKernelGGL(k_bgj_update, tiles, dim3(256), 0, st, n, bp, C, R, A);
+    hipLaunchKernelGGL(k_bgj_fix_rows, dim3(g1), dim3(256), 0, st, n, K, b, bp, R, P, A);
+    hipLaunchKernelGGL(k_bgj_fix_cols, dim3(g1), dim3(256), 0, st, n, K, b, bp, C, P, A);
   }
-  hipLaunchKernelGGL(k_gj_unpermute, dim3((n + 255) / 256), dim3(256), 0, st, n, A, perm);
+  // the swaps in order make the row permutation p (row i of the permuted
+  // matrix = row p[i]); A^-1[r][p[k]] = M[r][k], i.e. column c comes from
+  // column q[c] = p^-1[c]
+  std::vector<int> hp(n), hq(n);
+  int hs = 0;
+  if (hipMemcpyAsync(hp.data(), piv, n * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(&hs, sing, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    release();
+    throw std::runtime_error("dense inverse: device error");
+  }
+  if (hs) {
+    release();
+    return false;
+  }
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  for (int j = 0; j < n; ++j) std::swap(perm[j], perm[hp[j]]);
+  for (int k = 0; k < n; ++k) hq[perm[k]] = k;
+  (void)hipMemcpyAsync(q, hq.data(), n * sizeof(int), hipMemcpyHostToDevice, st);
+  hipLaunchKernelGGL(k_bgj_unpermute, dim3((int)(((size_t)n * n + 255) / 256)), dim3(256), 0, st, n, A, q, T);
+  (void)hipMemcpyAsync(A, T, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st);
+  const bool ok = hipStreamSynchronize(st) == hipSuccess;
+  release();
+  if (!ok) throw std::runtime_error("dense inverse: device error");
+  return true;
 }
 
 }  // namespace pmx

@@ -186,9 +186,9 @@ void vec_scatter(int n, const int* perm, const double* x, double* y, hipStream_t
 void vec_scatter(int n, const int* perm, const float* x, double* y, hipStream_t st);
 void vec_f32_to_f64(int n, const float* x, double* y, hipStream_t st);
 
-// ---- dense in-place inverse (n x n row-major) with partial pivoting;
-//      f: n doubles, perm: n ints, singular: 1 int (set to 1 on a zero pivot)
-void dense_invert(int n, double* A, double* f, int* perm, int* singular, hipStream_t st);
+// ---- dense in-place inverse (n x n row-major): blocked Gauss-Jordan with
+//      partial pivoting; synchronises st; false when a pivot is zero
+bool dense_invert(int n, double* A, hipStream_t st);
 // y = A x for a dense row-major n x n matrix
 void dense_mv(int n, const double* A, const double* x, double* y, hipStream_t st);
 void dense_mv_f32(int n, const double* A, const float* x, float* y, hipStream_t st);

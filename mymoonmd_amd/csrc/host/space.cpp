@@ -229,43 +229,60 @@ void build_prolongation(const FESpace& coarse, const FESpace& fine, CSRPattern& 
   if (coarse.mesh != fine.mesh || fine.level != coarse.level + 1)
     throw std::runtime_error("prolongation needs consecutive levels of one mesh");
   const int ndf = fine.ndof_cell, ndc = coarse.ndof_cell, dim = m.dim;
-  std::vector<std::vector<std::pair<int, double>>> rows(fine.n_dof);
-  std::vector<uint8_t> done(fine.n_dof, 0);
-  std::vector<double> v(ndc);
   const MeshLevel& fl = m.levels[fine.level];
-  for (int fc = 0; fc < fine.n_cells; ++fc) {
+  // each fine DOF's row comes from the first fine cell (in cell order) that
+  // holds it: the coarse basis of its parent evaluated at the DOF's node
+  std::vector<int> own((size_t)fine.n_dof, -1);
+  for (int fc = 0; fc < fine.n_cells; ++fc)
+    for (int k = 0; k < ndf; ++k) {
+      const int g = fine.global_numbers[(size_t)fc * ndf + k];
+      if (own[g] < 0) own[g] = fc * ndf + k;
+    }
+  auto row_of = [&](int g, std::vector<std::pair<int, double>>& row) {
+    row.clear();
+    const int fc = own[g] / ndf, k = own[g] % ndf;
     const int pc = fl.parent[fc];
     const AffineMap af = affine_map(m, fine.level, fc);
     const AffineMap ac = affine_map(m, coarse.level, pc);
-    for (int k = 0; k < ndf; ++k) {
-      const int g = fine.global_numbers[(size_t)fc * ndf + k];
-      if (done[g]) continue;
-      done[g] = 1;
-      double x[3], xi[3];
-      map_to_phys(af, &fine.el.ref_nodes[k * dim], x);
-      map_to_ref(ac, x, xi);
-      coarse.el.eval(xi, v.data(), nullptr, nullptr);
-      auto& row = rows[g];
-      for (int l = 0; l < ndc; ++l) {
-        // basis values at refined nodal points are exact binary fractions;
-        // snap round-off of the inverse map
-        double val = std::nearbyint(v[l] * 1024.0) / 1024.0;
-        if (std::fabs(v[l] - val) > 1e-12) val = v[l];
-        if (val != 0.0) row.emplace_back(coarse.global_numbers[(size_t)pc * ndc + l], val);
-      }
-      std::sort(row.begin(), row.end());
+    double x[3], xi[3], v[64];
+    map_to_phys(af, &fine.el.ref_nodes[k * dim], x);
+    map_to_ref(ac, x, xi);
+    coarse.el.eval(xi, v, nullptr, nullptr);
+    for (int l = 0; l < ndc; ++l) {
+      // basis values at refined nodal points are exact binary fractions;
+      // snap round-off of the inverse map
+      double val = std::nearbyint(v[l] * 1024.0) / 1024.0;
+      if (std::fabs(v[l] - val) > 1e-12) val = v[l];
+      if (val != 0.0) row.emplace_back(coarse.global_numbers[(size_t)pc * ndc + l], val);
     }
-  }
+    std::sort(row.begin(), row.end());
+  };
+  if (ndc > 64) throw std::runtime_error("prolongation: too many local DOFs");
   P.n_rows = fine.n_dof;
   P.n_cols = coarse.n_dof;
   P.active_bound = fine.n_dof;
   P.row_ptr.assign(fine.n_dof + 1, 0);
-  for (int r = 0; r < fine.n_dof; ++r) P.row_ptr[r + 1] = P.row_ptr[r] + (int)rows[r].size();
+#pragma omp parallel
+  {
+    std::vector<std::pair<int, double>> row;
+#pragma omp for schedule(static)
+    for (int g = 0; g < fine.n_dof; ++g) {
+      row_of(g, row);
+      P.row_ptr[g + 1] = (int)row.size();
+    }
+  }
+  for (int r = 0; r < fine.n_dof; ++r) P.row_ptr[r + 1] += P.row_ptr[r];
   P.col.resize(P.row_ptr.back());
   vals.resize(P.row_ptr.back());
-  for (int r = 0; r < fine.n_dof; ++r) {
-    int o = P.row_ptr[r];
-    for (auto& e : rows[r]) { P.col[o] = e.first; vals[o] = e.second; ++o; }
+#pragma omp parallel
+  {
+    std::vector<std::pair<int, double>> row;
+#pragma omp for schedule(static)
+    for (int g = 0; g < fine.n_dof; ++g) {
+      row_of(g, row);
+      int o = P.row_ptr[g];
+      for (auto& e : row) { P.col[o] = e.first; vals[o] = e.second; ++o; }
+    }
   }
 }
 

@@ -1,5 +1,6 @@
 // Internal definitions shared by the C-ABI translation units.
 #pragma once
+#include <atomic>
 #include <hip/hip_runtime.h>
 
 #include <map>
@@ -117,7 +118,15 @@ struct pmx_space {
   }
 };
 
+// a process-wide serial number per object (identifies a matrix across
+// solver updates even when an address is reused)
+inline uint64_t pmx_next_uid() {
+  static std::atomic<uint64_t> c{1};
+  return c.fetch_add(1);
+}
+
 struct pmx_matrix {
+  uint64_t uid = pmx_next_uid();
   pmx_ctx* ctx = nullptr;
   const pmx_space* space = nullptr;
   std::shared_ptr<pmx::CSRPattern> pattern;

@@ -224,6 +224,8 @@ struct pmx_solver {
   bool saddle = false;
   const pmx_nse* nse_fine = nullptr;   // saddle-point operator without multigrid
   std::vector<pmx_matrix*> mats;   // level matrices (their SELL values are refreshed per solve)
+  std::vector<uint64_t> mat_uids;        // ... and their serial numbers at the last update
+  std::vector<const pmx_space*> spaces;  // the levels' spaces at the last update
   std::vector<Dir> cc;
   pmx::DevCSR A;             // finest operator (view)
   // FGMRES in the multigrid's numbering of the finest level (renumbered
@@ -240,6 +242,10 @@ struct pmx_solver {
   // copy (nullptr: its bf16 values are used) and zeros for its
   // Dirichlet-column sums
   bool ir = false;
+  // K is the finest level's folded layout: its FP64 Dirichlet-column sums
+  // of the current operand (formed before every product)
+  bool kfold = false;
+  double* kg64 = nullptr;
   // the multigrid cycle captured as a HIP graph (direct coarse solve only),
   // one per state of the finest level's dir_zero flag
   hipGraph_t cycle_graph[2] = {nullptr, nullptr};
@@ -271,6 +277,8 @@ struct pmx_solver {
     dfree(kval); dfree(kbd); dfree(kb); dfree(kx);
     dfree(V32); dfree(Z32); dfree(w32); dfree(kval32); dfree(zero32);
     ir = false;
+    dfree(kg64);
+    kfold = false;
     for (int k = 0; k < 2; ++k) {
       if (cycle_exec[k]) (void)hipGraphExecDestroy(cycle_exec[k]);
       if (cycle_graph[k]) (void)hipGraphDestroy(cycle_graph[k]);
@@ -636,17 +644,9 @@ double* dense_inverse_of(pmx_solver* s, int n, const std::vector<int>& rp, const
   for (int r = 0; r < n; ++r)
     for (int k = rp[r]; k < rp[r + 1]; ++k) a[(size_t)r * n + col[k]] = vals[k];
   double* d = upload(a.data(), a.size(), S(s));
-  double* f = dmalloc<double>(n);
-  int* perm = dmalloc<int>(n);
-  int* sing = dmalloc<int>(1);
-  HIP_CHECK(hipMemsetAsync(sing, 0, sizeof(int), S(s)));
-  pmx::dense_invert(n, d, f, perm, sing, S(s));
+  const bool ok = pmx::dense_invert(n, d, S(s));
   HIP_CHECK(hipGetLastError());
-  int hs = 0;
-  HIP_CHECK(hipMemcpyAsync(&hs, sing, sizeof(int), hipMemcpyDeviceToHost, S(s)));
-  HIP_CHECK(hipStreamSynchronize(S(s)));
-  dfree(f); dfree(perm); dfree(sing);
-  if (hs) { dfree(d); throw PmxError(PMX_E_ZERO_DIAG, "singular coarse matrix in the direct coarse solve"); }
+  if (!ok) { dfree(d); throw PmxError(PMX_E_ZERO_DIAG, "singular coarse matrix in the direct coarse solve"); }
   return d;
 }
 
@@ -664,18 +664,12 @@ void build_coarse_inverse(pmx_solver* s, const pmx_matrix* M) {
   std::vector<double> a((size_t)n * n, 0.0);
   for (int r = 0; r < n; ++r)
     for (int k = pat.row_ptr[r]; k < pat.row_ptr[r + 1]; ++k) a[(size_t)r * n + pat.col[k]] = vals[k];
-  L.dinv_dense = upload(a.data(), a.size(), S(s));
-  double* f = dmalloc<double>(n);
-  int* perm = dmalloc<int>(n);
-  int* sing = dmalloc<int>(1);
-  HIP_CHECK(hipMemsetAsync(sing, 0, sizeof(int), S(s)));
-  pmx::dense_invert(n, L.dinv_dense, f, perm, sing, S(s));
+  // a refresh keeps the buffer (a captured cycle refers to it)
+  if (!L.dinv_dense) L.dinv_dense = dmalloc<double>((size_t)n * n);
+  HIP_CHECK(hipMemcpyAsync(L.dinv_dense, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice, S(s)));
+  const bool ok = pmx::dense_invert(n, L.dinv_dense, S(s));
   HIP_CHECK(hipGetLastError());
-  int hs = 0;
-  HIP_CHECK(hipMemcpyAsync(&hs, sing, sizeof(int), hipMemcpyDeviceToHost, S(s)));
-  HIP_CHECK(hipStreamSynchronize(S(s)));
-  dfree(f); dfree(perm); dfree(sing);
-  if (hs) throw PmxError(PMX_E_ZERO_DIAG, "singular coarse matrix in the direct coarse solve");
+  if (!ok) throw PmxError(PMX_E_ZERO_DIAG, "singular coarse matrix in the direct coarse solve");
 }
 
 template <typename T>
@@ -691,6 +685,7 @@ namespace {
 
 // r = b - K x and ||r|| into d_scal (scalar FE matrix or saddle-point system)
 void op_defect(pmx_solver* s, const double* x, const double* b, double* r) {
+  if (s->kfold && s->op == &s->K) pmx::sell_dirichlet_sums(s->K, x, S(s));
   if (s->saddle) {
     const pmx_nse* sys = s->nlv.empty() ? s->nse_fine : s->nlv.back().sys;
     HIP_CHECK(hipMemcpyAsync(r, b, s->n * sizeof(double), hipMemcpyDeviceToDevice, S(s)));
@@ -702,6 +697,7 @@ void op_defect(pmx_solver* s, const double* x, const double* b, double* r) {
 }
 
 void op_apply(pmx_solver* s, const double* x, double* y) {
+  if (s->kfold && s->op == &s->K) pmx::sell_dirichlet_sums(s->K, x, S(s));
   if (s->saddle) {
     const pmx_nse* sys = s->nlv.empty() ? s->nse_fine : s->nlv.back().sys;
     pmx::vec_fill(s->n, 0.0, y, S(s));
@@ -1002,6 +998,68 @@ void solve_richardson(pmx_solver* s, const double* b, double* x, pmx_solve_repor
   rep->iterations = s->p.max_n_iterations; rep->final_residual = resid; rep->converged = 0;
 }
 
+void check_zero_diag(pmx_solver* s, const pmx::DevCSR& A, double* dinv);
+
+// Solver::update_matrix with the structure of the last update (the same
+// level matrices and spaces): only what depends on the matrix values is
+// recomputed, in the buffers of the last update (the reference's
+// Multigrid::update after Multigrid::initialize).  Pattern-dependent data --
+// numbering, SELL layouts, transfer matrices -- is kept.
+void refresh_values(pmx_solver* s) {
+  hipStream_t st = S(s);
+  const bool mixed = s->p.precision == PMX_PRECISION_MIXED;
+  for (pmx_matrix* M : s->mats) ensure_sell_values(M, st);
+  s->A = s->mats.back()->A;
+  if (s->p.preconditioner == PMX_PC_JACOBI || s->p.iterative_solver_type == PMX_IT_JACOBI)
+    check_zero_diag(s, s->A, s->dinv);
+  const int nl = (int)s->lv.size();
+  for (int l = 0; l < nl; ++l) {
+    Level& L = s->lv[l];
+    const pmx_matrix* M = s->mats[l];
+    const pmx::DevCSR ref = M->A;   // reference-order CSR (values just assembled)
+    L.sol_zero = false;
+    L.g_valid = false;
+    if (L.own_sell) {
+      double* d = dmalloc<double>(L.n);
+      check_zero_diag(s, ref, d);
+      pmx::vec_gather(L.n, L.perm, d, L.dinv, st);
+      if (mixed) {
+        uint16_t* vb = dmalloc<uint16_t>(std::max(1, ref.nnz));
+        float* rs = dmalloc<float>(L.n);
+        pmx::csr_to_bf16(ref, vb, rs, st);
+        pmx::vec_gather(L.n, L.perm, rs, L.A.rscale, st);
+        pmx::sell_fill_bf16(L.A, L.A.sell.val_bf16, st);
+        HIP_CHECK(hipStreamSynchronize(st));
+        dfree(vb);
+        dfree(rs);
+      } else {
+        pmx::sell_fill_f64(L.A, L.A.sell.val, st);
+      }
+      HIP_CHECK(hipStreamSynchronize(st));
+      dfree(d);
+    } else {
+      check_zero_diag(s, ref, L.dinv);
+      if (mixed) {
+        pmx::csr_to_bf16(L.A, L.A.val_bf16, L.A.rscale, st);
+        if (L.A.sell.val_bf16) pmx::sell_fill_bf16(L.A, L.A.sell.val_bf16, st);
+      }
+    }
+    if (L.bd16) pmx::sell_dirichlet_fill(L.A, nullptr, L.bd16, st);
+    if (L.bd64) pmx::sell_dirichlet_fill(L.A, L.bd64, nullptr, st);
+    if (L.g32) pmx::vec_fill_f32(std::max(1, L.active), 0.f, L.g32, st);
+    if (L.g64) pmx::vec_fill(std::max(1, L.active), 0.0, L.g64, st);
+    if (mixed) pmx::vec_f64_to_f32(L.n, L.dinv, L.dinv32, st);
+  }
+  if (nl > 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) build_coarse_inverse(s, s->mats[0]);
+  if (s->kval) {
+    pmx::sell_fill_f64(s->K, s->kval, st);
+    if (s->kbd) pmx::sell_dirichlet_fill(s->K, s->kbd, nullptr, st);
+  }
+  if (s->kval32) pmx::sell_fill_f32(s->lv.back().A, s->kval32, st);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(st));
+}
+
 void check_zero_diag(pmx_solver* s, const pmx::DevCSR& A, double* dinv) {
   int* flag = dmalloc<int>(1);
   HIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), S(s)));
@@ -1094,11 +1152,26 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
                     s->p.mg_smoother_coarse != PMX_SMOOTHER_CELL_VANKA_JACOBI &&
                     s->p.mg_smoother_coarse != PMX_SMOOTHER_CELL_VANKA),
             "scalar problems use the jacobi smoother (cell Vanka is for saddle-point systems)");
+    for (int l = 0; l < n_levels; ++l) require(levels[l] != nullptr, "NULL matrix");
+    // the same hierarchy as the last update: values only
+    bool same = s->updated && !s->saddle && (int)s->mats.size() == n_levels;
+    for (int l = 0; same && l < n_levels; ++l)
+      same = s->mats[l] == levels[l] && s->mat_uids[l] == levels[l]->uid &&
+             s->spaces[l] == (spaces ? spaces[l] : nullptr);
+    if (same) {
+      refresh_values(s);
+      return;
+    }
     s->release();
     hipStream_t st = S(s);
     const pmx_matrix* fine = levels[n_levels - 1];
-    require(fine != nullptr, "NULL matrix");
     s->mats.assign(levels, levels + n_levels);
+    s->mat_uids.resize(n_levels);
+    s->spaces.assign(n_levels, nullptr);
+    for (int l = 0; l < n_levels; ++l) {
+      s->mat_uids[l] = levels[l]->uid;
+      if (spaces) s->spaces[l] = spaces[l];
+    }
     for (int l = 0; l < n_levels; ++l)
       if (levels[l]) ensure_sell_values(levels[l], st);
     s->A = fine->A;
@@ -1168,9 +1241,11 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
           for (auto& k : sp.src)
             if (k >= 0) k = opos[k];
           for (auto& k : sp.bd_src) k = opos[k];
-          if (l == n_levels - 1 && !sp.off.empty() && s->p.iterative_solver_type == PMX_IT_FGMRES) {
+          if (l == n_levels - 1 && !sp.off.empty() && s->p.iterative_solver_type == PMX_IT_FGMRES &&
+              s->p.renumber_levels < 3) {
             // FGMRES applies the finest operator to vectors whose Dirichlet
-            // part changes every time: it keeps a 32-bit copy of the layout
+            // part changes every time: without the folded layout it keeps a
+            // 32-bit copy of the layout
             kpat = pmx::build_sell(pp, kSellSigmaSolver, false);
             for (auto& k : kpat.src)
               if (k >= 0) k = opos[k];
@@ -1273,7 +1348,28 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
         s->K.sell.bd_val_bf16 = nullptr;
         s->K.sell.gsum = nullptr;
         s->K.sell.gsum_bytes = 0;
-        if (have_kpat) {
+        s->K.sell.dir_zero = 0;
+        if (F.A.sell.off && F.A.sell.gsum) {
+          // the folded layout of the finest level with FP64 values and sums
+          // of its own: the Dirichlet-column sums of each operand are formed
+          // before each product (op_apply / op_defect)
+          if (mixed) {
+            s->kval = dmalloc<double>(s->K.sell.total);
+            pmx::sell_fill_f64(s->K, s->kval, st);
+            s->K.sell.val = s->kval;
+            if (s->K.sell.n_bd > 0) {
+              s->kbd = dmalloc<double>(s->K.sell.n_bd);
+              pmx::sell_dirichlet_fill(s->K, s->kbd, nullptr, st);
+            }
+            s->K.sell.bd_val = s->kbd;
+          } else {
+            s->K.sell.bd_val = F.bd64;
+          }
+          s->kg64 = dzeros<double>(std::max(1, F.active), st);
+          s->K.sell.gsum = s->kg64;
+          s->K.sell.gsum_bytes = 8;
+          s->kfold = true;
+        } else if (have_kpat) {
           s->K.sell = upload_sell(kpat, st);
           s->kown = true;
           s->kval = dmalloc<double>(s->K.sell.total);

@@ -151,3 +151,71 @@ def test_graph_cycle_replays_equal_the_launched_cycle(ctx, precision):
         assert np.array_equal(out[0][k][0], out[1][k][0]) and out[0][k][1] == out[1][k][1], k
         assert np.array_equal(out[0][k][2], out[1][k][2]), k
         assert np.array_equal(out[1][0][0], out[1][k][0]), k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_value_update_equals_a_fresh_set_up(ctx, precision):
+    """Solver::update_matrix with the same matrices and spaces as before only
+    recomputes what depends on the values (diagonals, bf16 copies, FP32
+    copies, coarse inverse) in place: after the matrices are reassembled with
+    other coefficients, the refreshed solver and a freshly set-up one give the
+    same bits and iteration counts."""
+    d, spaces, mats, rhs = levels(ctx, 4, 3)
+    prm = pm.solver_params(iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
+                           residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=300,
+                           multigrid_cycle_type="V", multigrid_smoother="jacobi",
+                           multigrid_smoother_coarse="direct_solve", multigrid_n_pre_smooth=2,
+                           multigrid_n_post_smooth=2, jacobi_damp=0.6, precision=precision)
+    sol = pm.Solver(ctx, prm)
+    sol.update_matrix(mats, spaces)
+    fs = spaces[-1]
+    x = torch.zeros_like(rhs[-1])
+    x[fs.active_bound:] = rhs[-1][fs.active_bound:]
+    sol.solve(rhs[-1], x.clone())
+    kw2 = dict(KW, eps=5e-3, b=(0.3, -0.8, 0.5))
+    for s, A, f in zip(spaces, mats, rhs):
+        A.reset()
+        f.zero_()
+        pm.assemble_cd(ctx, s, pm.cd_params(**kw2), A, f)
+    b = rhs[-1]
+    x0 = torch.zeros_like(b)
+    x0[fs.active_bound:] = b[fs.active_bound:]
+    sol.update_matrix(mats, spaces)          # same hierarchy: values only
+    x1 = x0.clone()
+    r1 = sol.solve(b, x1)
+    fresh = pm.Solver(ctx, prm)
+    fresh.update_matrix(mats, spaces)
+    x2 = x0.clone()
+    r2 = fresh.solve(b, x2)
+    assert r1.converged and r1.iterations == r2.iterations
+    assert torch.equal(x1, x2)
+
+
+def test_direct_coarse_solve_with_pivoting_matches_oracle(ctx):
+    """The blocked Gauss-Jordan inverse of the coarsest matrix (729 DOFs:
+    eleven 64-column blocks and a partial one) in a 2-level cycle against the
+    oracle's cycle with its LU solve, at 1e-10."""
+    nref = 3
+    d = pm.TDomain("Default_UnitCube_Hexa", nref)
+    spaces = [pm.TFESpace(d, l, 2, ctx) for l in (nref - 1, nref)]
+    mats = []
+    for s in spaces:
+        A = pm.FEMatrix(ctx, s)
+        f = torch.zeros(s.n_dof, dtype=torch.float64, device="cuda")
+        pm.assemble_cd(ctx, s, pm.cd_params(**KW), A, f)
+        mats.append(A)
+    prm = pm.solver_params(iterative_solver_type="fgmres", preconditioner="multigrid", multigrid_cycle_type="V",
+                           multigrid_smoother="jacobi", multigrid_smoother_coarse="direct_solve",
+                           multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2, jacobi_damp=0.6)
+    sol = pm.Solver(ctx, prm)
+    sol.update_matrix(mats, spaces)
+    rng = np.random.default_rng(2)
+    v = torch.as_tensor(rng.standard_normal(spaces[-1].n_dof)).cuda()
+    z = torch.zeros_like(v)
+    sol.apply_preconditioner(v, z)
+    osp = [O.build_space(2, nref, l, 2)[0] for l in (nref - 1, nref)]
+    op = O.solver_params(preconditioner=2, mg_cycle=0, mg_smoother_coarse=1, mg_n_pre_smooth=2, mg_n_post_smooth=2,
+                         jacobi_damp=0.6, mg_correction_damp=1.0)
+    zo = O.mg_apply(op, osp, [A.get_entries() for A in mats], v.cpu().numpy())
+    assert spaces[0].n_dof == 729
+    assert np.max(np.abs(z.cpu().numpy() - zo)) <= 1e-10 * np.max(np.abs(zo))
