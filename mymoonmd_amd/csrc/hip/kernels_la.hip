@@ -1006,6 +1006,7 @@ __global__ void k_to_bf16_rows(int n, const int* __restrict__ rp, const double* 
     for (int k = rp[r]; k < rp[r + 1]; ++k) m = fmax(m, fabs(va[k]));
     const double sc = (m > 0) ? m : 1.0;
     rscale[r] = (float)sc;
+    if (!vb) continue;   // row scales only
     const double inv = 1.0 / sc;
     for (int k = rp[r]; k < rp[r + 1]; ++k) {
       const float f = (float)(va[k] * inv);
@@ -1484,8 +1485,9 @@ void fe_apply_active_bf16(const DevCSR& A, const float* x, float* y, const float
 // Dense in-place inversion by blocked Gauss-Jordan elimination with partial
 // pivoting (set-up of the exact coarse-grid solve); A is n x n row-major.
 // Per block column K of width b <= 64:
-//  1. k_bgj_panel (one workgroup): LU with partial pivoting of the panel
-//     A[K:, K:K+b] in a scratch copy chooses the b pivot rows;
+//  1. k_bgj_panel_reg (one workgroup; k_bgj_panel when the panel has more
+//     than 5,120 rows): LU with partial pivoting of the panel A[K:, K:K+b]
+//     in a scratch copy chooses the b pivot rows;
 //  2. k_bgj_swap: the pivot rows are swapped into rows K..K+b-1;
 //  3. the block sweep with P = A_KK^-1 (k_bgj_block_inverse, Gauss-Jordan
 //     without further pivoting on the b x b block, whose LU needs none after
@@ -1493,7 +1495,8 @@ void fe_apply_active_bf16(const DevCSR& A, const float* x, float* y, const float
 //        A[o, o] -= C R   (k_bgj_update, FP64 matrix cores, 64 x 64 tiles)
 //        A[K, o] = R,  A[o, K] = -C P,  A[K, K] = P   (k_bgj_fix_*).
 // The row swaps leave the inverse of the row-permuted matrix; its columns
-// are put back in order at the end (k_bgj_unpermute).  The trailing update
+// are put back in order at the end (k_bgj_unpermute, in place, one row per
+// workgroup through LDS).  The trailing update
 // is a rank-b GEMM per block instead of b rank-1 sweeps over the matrix.
 namespace pmx {
 namespace {
@@ -1556,6 +1559,171 @@ __global__ __launch_bounds__(1024) void k_bgj_panel(int n, int K, int b, const d
   }
 }
 
+// The same pivot choice with the panel rows in registers: thread t holds
+// rows t, t + 512, ... (RPT rows, m <= 512 RPT) of six panel columns at a
+// time (eight spill registers).  Per column: the pivot (largest magnitude, the lowest row on ties, as
+// a sequential search) by a wave reduction and one over the 8 waves, the
+// exchange of the two rows through LDS (their later panel columns are swapped
+// in W), the elimination in registers.  After six columns the panel's later
+// columns take one rank-6 update in W: U12 = L11^-1 A12 by one thread per
+// column, then A22 -= L21 U12 by the row owners.  Only the pivots leave the
+// kernel; two barriers per column instead of a pass over the panel in global
+// memory.
+constexpr int kPanelThreads = 512, kPanelRpt = 10, kPanelSub = 6;
+
+template <int RPT>
+__global__ __launch_bounds__(kPanelThreads) void k_bgj_panel_reg(int n, int K, int b, const double* __restrict__ A,
+                                                                 double* __restrict__ W, int* __restrict__ piv,
+                                                                 int* __restrict__ singular) {
+  constexpr int NT = kPanelThreads, SUB = kPanelSub, NW = NT / 64;
+  __shared__ double s_best[NW];
+  __shared__ int s_bi[NW];
+  __shared__ double s_prow[SUB], s_jrow[SUB];
+  __shared__ double s_l[SUB][SUB];
+  __shared__ double s_u[SUB][kBgjBlock];
+  const int m = n - K, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (size_t e = tid; e < (size_t)m * b; e += NT) {
+    const int i = (int)(e / b), jj = (int)(e % b);
+    W[e] = A[(size_t)(K + i) * n + K + jj];
+  }
+  __syncthreads();
+  for (int s0 = 0; s0 < b; s0 += SUB) {
+    const int ws = b - s0 < SUB ? b - s0 : SUB;
+    const int rest = b - s0 - SUB;   // later panel columns (> 0 only when ws == SUB)
+    double v[RPT][SUB];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      const int i = tid + k * NT;
+#pragma unroll
+      for (int c = 0; c < SUB; ++c) v[k][c] = (i < m && c < ws) ? W[(size_t)i * b + s0 + c] : 0.0;
+    }
+#pragma unroll
+    for (int jj = 0; jj < SUB; ++jj) {
+      if (jj >= ws) break;   // uniform
+      const int j = s0 + jj;
+      double best = -1.0;
+      int bi = 0x7fffffff;
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int i = tid + k * NT;
+        if (i >= j && i < m) {
+          const double a = fabs(v[k][jj]);
+          if (a > best) { best = a; bi = i; }
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      if (lane == 0) { s_best[wv] = best; s_bi[wv] = bi; }
+      __syncthreads();
+      best = s_best[0];
+      bi = s_bi[0];
+      for (int w = 1; w < NW; ++w) {
+        const double ob = s_best[w];
+        const int oi = s_bi[w];
+        if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      }
+      // no row with a number (every candidate NaN): keep row j, report singular
+      const bool none = bi == 0x7fffffff;
+      const int p = none ? j : bi;
+      const int tp = p % NT, kp = p / NT, tj = j % NT, kj = j / NT;
+      if (tid == tp) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+          if (k == kp)
+#pragma unroll
+            for (int c = 0; c < SUB; ++c) s_prow[c] = v[k][c];
+      }
+      if (tid == tj) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+          if (k == kj)
+#pragma unroll
+            for (int c = 0; c < SUB; ++c) s_jrow[c] = v[k][c];
+      }
+      if (tid == 0) {
+        piv[K + j] = K + p;
+        if (none || !(best > 0.0)) singular[0] = 1;
+      }
+      if (p != j)
+        for (int t = tid; t < rest; t += NT) {
+          double* wj = W + (size_t)j * b + s0 + SUB + t;
+          double* wp = W + (size_t)p * b + s0 + SUB + t;
+          const double x = *wj;
+          *wj = *wp;
+          *wp = x;
+        }
+      __syncthreads();
+      if (tid == tj) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+          if (k == kj)
+#pragma unroll
+            for (int c = 0; c < SUB; ++c) v[k][c] = s_prow[c];
+      }
+      if (tid == tp && p != j) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+          if (k == kp)
+#pragma unroll
+            for (int c = 0; c < SUB; ++c) v[k][c] = s_jrow[c];
+      }
+      const double d = s_prow[jj];
+      const double id = (d != 0.0) ? 1.0 / d : 0.0;
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int i = tid + k * NT;
+        if (i > j && i < m) {
+          const double l = v[k][jj] * id;
+          v[k][jj] = l;
+#pragma unroll
+          for (int c = jj + 1; c < SUB; ++c) v[k][c] -= l * s_prow[c];
+        }
+      }
+    }
+    if (rest > 0) {
+      // L11 (unit lower) of the rows s0 .. s0+SUB-1
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int i = tid + k * NT;
+        if (i >= s0 && i < s0 + SUB)
+#pragma unroll
+          for (int c = 0; c < SUB; ++c) s_l[i - s0][c] = v[k][c];
+      }
+      __syncthreads();
+      for (int c = tid; c < rest; c += NT) {
+        double u[SUB];
+#pragma unroll
+        for (int t = 0; t < SUB; ++t) {
+          double x = W[(size_t)(s0 + t) * b + s0 + SUB + c];
+#pragma unroll
+          for (int q = 0; q < t; ++q) x -= s_l[t][q] * u[q];
+          u[t] = x;
+          s_u[t][c] = x;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < RPT; ++k) {
+        const int i = tid + k * NT;
+        if (i >= s0 + SUB && i < m) {
+          double* wi = W + (size_t)i * b + s0 + SUB;
+          for (int c = 0; c < rest; ++c) {
+            double x = wi[c];
+#pragma unroll
+            for (int t = 0; t < SUB; ++t) x -= v[k][t] * s_u[t][c];
+            wi[c] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // one thread per column: the swaps (K + j, piv[K + j]), j = 0 .. b-1, in order
 __global__ void k_bgj_swap(int n, int K, int b, double* __restrict__ A, const int* __restrict__ piv) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1571,34 +1739,36 @@ __global__ void k_bgj_swap(int n, int K, int b, double* __restrict__ A, const in
 }
 
 // P = A[K:K+b, K:K+b]^-1 by in-place Gauss-Jordan without pivoting, padded
-// to bp x bp with zeros
-__global__ __launch_bounds__(256) void k_bgj_block_inverse(int n, int K, int b, int bp, const double* __restrict__ A,
+// to bp x bp with zeros.  Per step the pivot row and column are copied first,
+// so every entry is updated in one pass from the old values (two barriers).
+__global__ __launch_bounds__(512) void k_bgj_block_inverse(int n, int K, int b, int bp, const double* __restrict__ A,
                                                            double* __restrict__ P, int* __restrict__ singular) {
   __shared__ double B[kBgjBlock][kBgjBlock + 1];
+  __shared__ double rowk[kBgjBlock], colk[kBgjBlock];
   for (int e = threadIdx.x; e < b * b; e += blockDim.x) B[e / b][e % b] = A[(size_t)(K + e / b) * n + K + e % b];
-  __syncthreads();
   for (int k = 0; k < b; ++k) {
+    __syncthreads();
     const double d = B[k][k];
     if (d == 0.0) {
       if (threadIdx.x == 0) singular[0] = 1;
       return;
     }
-    const double pk = 1.0 / d;
-    __syncthreads();
-    if ((int)threadIdx.x < b) B[k][threadIdx.x] = (threadIdx.x == (unsigned)k) ? pk : B[k][threadIdx.x] * pk;
-    __syncthreads();
-    // rows i != k: row_i -= f row_k, then B[i][k] = -f pk
-    for (int e = threadIdx.x; e < b * b; e += blockDim.x) {
-      const int i = e / b, c = e % b;
-      if (i == k) continue;
-      const double f = B[i][k];
-      if (c != k) B[i][c] -= f * B[k][c];
+    if ((int)threadIdx.x < b) {
+      rowk[threadIdx.x] = B[k][threadIdx.x];
+      colk[threadIdx.x] = B[threadIdx.x][k];
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < b; i += blockDim.x)
-      if (i != k) B[i][k] = -B[i][k] * pk;
-    __syncthreads();
+    const double pk = 1.0 / d;
+    // row k: B[k][c] pk, B[k][k] = pk; rows i != k: B[i][c] -= B[i][k] (B[k][c] pk),
+    // B[i][k] = -B[i][k] pk
+    for (int e = threadIdx.x; e < b * b; e += blockDim.x) {
+      const int i = e / b, c = e % b;
+      if (i == k) B[k][c] = (c == k) ? pk : rowk[c] * pk;
+      else if (c == k) B[i][k] = -colk[i] * pk;
+      else B[i][c] -= colk[i] * (rowk[c] * pk);
+    }
   }
+  __syncthreads();
   for (int e = threadIdx.x; e < bp * bp; e += blockDim.x) {
     const int t = e / bp, c = e % bp;
     P[e] = (t < b && c < b) ? B[t][c] : 0.0;
@@ -1613,15 +1783,18 @@ __global__ void k_bgj_make_c(int n, int K, int b, int bp, const double* __restri
   C[e] = (c < b && (i < K || i >= K + b)) ? A[(size_t)i * n + K + c] : 0.0;
 }
 
-// R[t][c] = sum_s P[t][s] A[K + s][c] for t < b (zero rows up to bp)
-__global__ void k_bgj_make_r(int n, int K, int b, int bp, const double* __restrict__ A, const double* __restrict__ P,
-                             double* __restrict__ R) {
+// R[t][c] = sum_s P[t][s] A[K + s][c] for t < b (zero rows up to bp); one
+// thread per column c and 16 rows t (blockIdx.y), so a launch has enough
+// workgroups to spread over the chip
+__global__ __launch_bounds__(64) void k_bgj_make_r(int n, int K, int b, int bp, const double* __restrict__ A,
+                                                   const double* __restrict__ P, double* __restrict__ R) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+  const int t0 = blockIdx.y * 16;
+  if (c >= n || t0 >= bp) return;
   double a[kBgjBlock];   // register array: every index below is a constant after unrolling
 #pragma unroll
   for (int s = 0; s < kBgjBlock; ++s) a[s] = (s < b) ? A[(size_t)(K + s) * n + c] : 0.0;
-  for (int t = 0; t < bp; ++t) {
+  for (int t = t0; t < t0 + 16 && t < bp; ++t) {
     double acc = 0.0;
 #pragma unroll
     for (int s = 0; s < kBgjBlock; ++s)
@@ -1669,29 +1842,50 @@ __global__ void k_bgj_fix_rows(int n, int K, int b, int bp, const double* __rest
     A[(size_t)(K + t) * n + c] = (c >= K && c < K + b) ? P[t * bp + (c - K)] : R[(size_t)t * n + c];
 }
 
-// the other rows' block columns: -C P
-__global__ void k_bgj_fix_cols(int n, int K, int b, int bp, const double* __restrict__ C, const double* __restrict__ P,
-                               double* __restrict__ A) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || (i >= K && i < K + b)) return;
-  double ci[kBgjBlock];   // register array, as in k_bgj_make_r
+// the other rows' block columns: -C P.  64 rows per workgroup, four
+// threads per row (16 columns each) into LDS, then stored row by row so the
+// writes of a wave cover whole row segments
+__global__ __launch_bounds__(256) void k_bgj_fix_cols(int n, int K, int b, int bp, const double* __restrict__ C,
+                                                      const double* __restrict__ P, double* __restrict__ A) {
+  __shared__ double out[64][kBgjBlock + 1];
+  const int r = threadIdx.x & 63, sq = threadIdx.x >> 6;
+  const int i0 = blockIdx.x * 64, i = i0 + r;
+  if (i < n) {
+    double ci[kBgjBlock];   // register array, as in k_bgj_make_r
 #pragma unroll
-  for (int t = 0; t < kBgjBlock; ++t) ci[t] = (t < b) ? C[(size_t)i * bp + t] : 0.0;
-  for (int s = 0; s < b; ++s) {
-    double acc = 0.0;
+    for (int t = 0; t < kBgjBlock; ++t) ci[t] = (t < b) ? C[(size_t)i * bp + t] : 0.0;
+    for (int s = sq * 16; s < sq * 16 + 16 && s < b; ++s) {
+      double acc = 0.0;
 #pragma unroll
-    for (int t = 0; t < kBgjBlock; ++t)
-      if (t < b) acc += ci[t] * P[t * bp + s];
-    A[(size_t)i * n + K + s] = -acc;
+      for (int t = 0; t < kBgjBlock; ++t)
+        if (t < b) acc += ci[t] * P[t * bp + s];
+      out[r][s] = -acc;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * b; e += blockDim.x) {
+    const int rr = e / b, s = e % b, ii = i0 + rr;
+    if (ii < n && (ii < K || ii >= K + b)) A[(size_t)ii * n + K + s] = out[rr][s];
   }
 }
 
 // out[r][c] = A[r][q[c]]
-__global__ void k_bgj_unpermute(int n, const double* __restrict__ A, const int* __restrict__ q, double* __restrict__ out) {
+__global__ void k_bgj_unpermute_to(int n, const double* __restrict__ A, const int* __restrict__ q,
+                                   double* __restrict__ out) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)n * n) return;
   const int r = (int)(e / n), c = (int)(e % n);
   out[e] = A[(size_t)r * n + q[c]];
+}
+
+// A[r][c] = A[r][q[c]] in place: one workgroup per row, the row staged in
+// LDS (n doubles of dynamic shared memory, n <= 8192)
+__global__ __launch_bounds__(256) void k_bgj_unpermute(int n, double* __restrict__ A, const int* __restrict__ q) {
+  extern __shared__ double row[];
+  double* a = A + (size_t)blockIdx.x * n;
+  for (int c = threadIdx.x; c < n; c += blockDim.x) row[c] = a[c];
+  __syncthreads();
+  for (int c = threadIdx.x; c < n; c += blockDim.x) a[c] = row[q[c]];
 }
 
 template <typename T>
@@ -1703,6 +1897,23 @@ T* bgj_alloc(size_t n) {
 
 }  // namespace
 
+namespace {
+// D[r][col] = val for the entries of row r (D zeroed before)
+__global__ void k_csr_scatter_dense(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                    const double* __restrict__ val, double* __restrict__ D) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int k = rp[r]; k < rp[r + 1]; ++k) D[(size_t)r * n + col[k]] = val[k];
+}
+}  // namespace
+
+void csr_to_dense(const DevCSR& A, double* D, hipStream_t st) {
+  const size_t nn = (size_t)A.n * A.n;
+  if (nn == 0) return;
+  hipLaunchKernelGGL(k_fill<double>, dim3(4096), dim3(256), 0, st, (int)nn, 0.0, D);   // n <= 20000
+  hipLaunchKernelGGL(k_csr_scatter_dense, dim3((A.n + 255) / 256), dim3(256), 0, st, A.n, A.row_ptr, A.col, A.val, D);
+}
+
 bool dense_invert(int n, double* A, hipStream_t st) {
   if (n <= 0) return true;
   const int nb = kBgjBlock;
@@ -1710,13 +1921,17 @@ bool dense_invert(int n, double* A, hipStream_t st) {
   double* C = bgj_alloc<double>((size_t)n * nb);
   double* R = bgj_alloc<double>((size_t)n * nb);
   double* P = bgj_alloc<double>((size_t)nb * nb);
-  double* T = bgj_alloc<double>((size_t)n * n);
+  // rows up to 8,192 doubles are unpermuted in place through LDS; longer ones
+  // through a second matrix
+  const bool in_place = (size_t)n * sizeof(double) <= 65536;
+  double* T = in_place ? nullptr : bgj_alloc<double>((size_t)n * n);
   int* piv = bgj_alloc<int>(n);
   int* q = bgj_alloc<int>(n);
   int* sing = bgj_alloc<int>(1);
   auto release = [&]() {
     (void)hipFree(W); (void)hipFree(C); (void)hipFree(R); (void)hipFree(P);
-    (void)hipFree(T); (void)hipFree(piv); (void)hipFree(q); (void)hipFree(sing);
+    if (T) (void)hipFree(T);
+    (void)hipFree(piv); (void)hipFree(q); (void)hipFree(sing);
   };
   (void)hipMemsetAsync(sing, 0, sizeof(int), st);
   const int g1 = (n + 255) / 256;
@@ -1724,14 +1939,17 @@ bool dense_invert(int n, double* A, hipStream_t st) {
   for (int K = 0; K < n; K += nb) {
     const int b = n - K < nb ? n - K : nb;
     const int bp = (b + 3) / 4 * 4;
-    hipLaunchKernelGGL(k_bgj_panel, dim3(1), dim3(1024), 0, st, n, K, b, A, W, piv, sing);
+    if (n - K <= kPanelThreads * kPanelRpt)
+      hipLaunchKernelGGL(k_bgj_panel_reg<kPanelRpt>, dim3(1), dim3(kPanelThreads), 0, st, n, K, b, A, W, piv, sing);
+    else
+      hipLaunchKernelGGL(k_bgj_panel, dim3(1), dim3(1024), 0, st, n, K, b, A, W, piv, sing);
     hipLaunchKernelGGL(k_bgj_swap, dim3(g1), dim3(256), 0, st, n, K, b, A, piv);
-    hipLaunchKernelGGL(k_bgj_block_inverse, dim3(1), dim3(256), 0, st, n, K, b, bp, A, P, sing);
+    hipLaunchKernelGGL(k_bgj_block_inverse, dim3(1), dim3(512), 0, st, n, K, b, bp, A, P, sing);
     hipLaunchKernelGGL(k_bgj_make_c, dim3((int)(((size_t)n * bp + 255) / 256)), dim3(256), 0, st, n, K, b, bp, A, C);
-    hipLaunchKernelGGL(k_bgj_make_r, dim3(g1), dim3(256), 0, st, n, K, b, bp, A, P, R);
+    hipLaunchKernelGGL(k_bgj_make_r, dim3((n + 63) / 64, (bp + 15) / 16), dim3(64), 0, st, n, K, b, bp, A, P, R);
     hipLaunchKernelGGL(k_bgj_update, tiles, dim3(256), 0, st, n, bp, C, R, A);
     hipLaunchKernelGGL(k_bgj_fix_rows, dim3(g1), dim3(256), 0, st, n, K, b, bp, R, P, A);
-    hipLaunchKernelGGL(k_bgj_fix_cols, dim3(g1), dim3(256), 0, st, n, K, b, bp, C, P, A);
+    hipLaunchKernelGGL(k_bgj_fix_cols, dim3((n + 63) / 64), dim3(256), 0, st, n, K, b, bp, C, P, A);
   }
   // the swaps in order make the row permutation p (row i of the permuted
   // matrix = row p[i]); A^-1[r][p[k]] = M[r][k], i.e. column c comes from
@@ -1753,8 +1971,12 @@ bool dense_invert(int n, double* A, hipStream_t st) {
   for (int j = 0; j < n; ++j) std::swap(perm[j], perm[hp[j]]);
   for (int k = 0; k < n; ++k) hq[perm[k]] = k;
   (void)hipMemcpyAsync(q, hq.data(), n * sizeof(int), hipMemcpyHostToDevice, st);
-  hipLaunchKernelGGL(k_bgj_unpermute, dim3((int)(((size_t)n * n + 255) / 256)), dim3(256), 0, st, n, A, q, T);
-  (void)hipMemcpyAsync(A, T, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st);
+  if (in_place) {
+    hipLaunchKernelGGL(k_bgj_unpermute, dim3(n), dim3(256), (size_t)n * sizeof(double), st, n, A, q);
+  } else {
+    hipLaunchKernelGGL(k_bgj_unpermute_to, dim3((int)(((size_t)n * n + 255) / 256)), dim3(256), 0, st, n, A, q, T);
+    (void)hipMemcpyAsync(A, T, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, st);
+  }
   const bool ok = hipStreamSynchronize(st) == hipSuccess;
   release();
   if (!ok) throw std::runtime_error("dense inverse: device error");

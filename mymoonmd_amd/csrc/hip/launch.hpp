@@ -189,6 +189,8 @@ void vec_f32_to_f64(int n, const float* x, double* y, hipStream_t st);
 // ---- dense in-place inverse (n x n row-major): blocked Gauss-Jordan with
 //      partial pivoting; synchronises st; false when a pivot is zero
 bool dense_invert(int n, double* A, hipStream_t st);
+// D (n x n row-major, n <= 20000) = the CSR matrix A
+void csr_to_dense(const DevCSR& A, double* D, hipStream_t st);
 // y = A x for a dense row-major n x n matrix
 void dense_mv(int n, const double* A, const double* x, double* y, hipStream_t st);
 void dense_mv_f32(int n, const double* A, const float* x, float* y, hipStream_t st);

@@ -657,16 +657,11 @@ void build_coarse_inverse(pmx_solver* s, const pmx_matrix* M) {
   Level& L = s->lv[0];
   const int n = L.n;
   if (n > 20000) throw PmxError(PMX_E_ARG, "direct coarse solve supports at most 20000 DOFs");
-  std::vector<double> vals(M->A.nnz);
-  HIP_CHECK(hipMemcpyAsync(vals.data(), M->A.val, vals.size() * sizeof(double), hipMemcpyDeviceToHost, S(s)));
-  HIP_CHECK(hipStreamSynchronize(S(s)));
-  const auto& pat = *M->pattern;
-  std::vector<double> a((size_t)n * n, 0.0);
-  for (int r = 0; r < n; ++r)
-    for (int k = pat.row_ptr[r]; k < pat.row_ptr[r + 1]; ++k) a[(size_t)r * n + pat.col[k]] = vals[k];
+  require(M->A.n == n, "coarse matrix size mismatch");
   // a refresh keeps the buffer (a captured cycle refers to it)
   if (!L.dinv_dense) L.dinv_dense = dmalloc<double>((size_t)n * n);
-  HIP_CHECK(hipMemcpyAsync(L.dinv_dense, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice, S(s)));
+  pmx::csr_to_dense(M->A, L.dinv_dense, S(s));
+  HIP_CHECK(hipGetLastError());
   const bool ok = pmx::dense_invert(n, L.dinv_dense, S(s));
   HIP_CHECK(hipGetLastError());
   if (!ok) throw PmxError(PMX_E_ZERO_DIAG, "singular coarse matrix in the direct coarse solve");
@@ -1024,13 +1019,11 @@ void refresh_values(pmx_solver* s) {
       check_zero_diag(s, ref, d);
       pmx::vec_gather(L.n, L.perm, d, L.dinv, st);
       if (mixed) {
-        uint16_t* vb = dmalloc<uint16_t>(std::max(1, ref.nnz));
         float* rs = dmalloc<float>(L.n);
-        pmx::csr_to_bf16(ref, vb, rs, st);
+        pmx::csr_to_bf16(ref, nullptr, rs, st);   // row scales only
         pmx::vec_gather(L.n, L.perm, rs, L.A.rscale, st);
         pmx::sell_fill_bf16(L.A, L.A.sell.val_bf16, st);
         HIP_CHECK(hipStreamSynchronize(st));
-        dfree(vb);
         dfree(rs);
       } else {
         pmx::sell_fill_f64(L.A, L.A.sell.val, st);

@@ -129,6 +129,20 @@ class TFESpace:
         call("pmx_space_get_global_numbers", self.h, out.ctypes.data_as(ctypes.POINTER(c_int)))
         return out.reshape(self.n_cells, self.ndof_cell)
 
+    def n_quad(self, hexa_quad_rule=0):
+        """quadrature points per cell of the assembly (pmx_assembly_n_quad)"""
+        n = c_int()
+        call("pmx_assembly_n_quad", self.h, hexa_quad_rule, ctypes.byref(n))
+        return n.value
+
+    def quad_points(self, hexa_quad_rule=0):
+        """physical quadrature points [n_cells, n_quad, 3] at which a
+        coefficient table is sampled (pmx_assembly_quad_points)"""
+        nq = self.n_quad(hexa_quad_rule)
+        out = np.zeros(self.n_cells * nq * 3)
+        call("pmx_assembly_quad_points", self.h, hexa_quad_rule, out.ctypes.data_as(ctypes.POINTER(c_double)))
+        return out.reshape(self.n_cells, nq, 3)
+
     def dof_coords(self):
         out = np.zeros(self.n_dof * 3)
         call("pmx_space_get_dof_coords", self.h, out.ctypes.data_as(ctypes.POINTER(c_double)))
@@ -250,8 +264,22 @@ class FEMatrix:
 
 
 def cd_params(example="const", disc="galerkin", eps=1.0, b=(0.0, 0.0, 0.0), c=0.0, f0=0.0,
-              hexa_quad_rule=0, delta0=1.0, poly_degree=1.0, time=0.0, theta1=1.0, tau=0.0):
+              hexa_quad_rule=0, delta0=1.0, poly_degree=1.0, time=0.0, theta1=1.0, tau=0.0,
+              coeff_table=None, dirichlet=None):
+    """pmx_cd_params.  `coeff_table`: optional float64 device tensor
+    [n_cells, n_quad, 8] with the coefficients (eps, b1, b2, b3, c, f, -, -) at
+    the points of TFESpace.quad_points(hexa_quad_rule) -- what a user
+    CoeffFct3D returns there (LocalAssembling::GetParameters,
+    LocalAssembling.C:766-840); it replaces the constants and the example's
+    right-hand side.  `dirichlet`: optional float64 host array with the
+    boundary values of the Dirichlet DOFs (DOFs active_bound.. in order),
+    replacing the example's.  Both are checked against the space when the
+    parameters are used."""
     p = CDParams()
+    p._table = coeff_table
+    p._dirichlet = None
+    if dirichlet is not None:
+        p._dirichlet = np.ascontiguousarray(dirichlet, dtype=np.float64)
     p.disc = {"galerkin": 0, "supg": 1, "gls": 2}[disc]
     p.example = EXAMPLES[example]
     p.hexa_quad_rule = hexa_quad_rule
@@ -266,11 +294,30 @@ def cd_params(example="const", disc="galerkin", eps=1.0, b=(0.0, 0.0, 0.0), c=0.
     return p
 
 
+def _bind_user_data(ctx, space, params):
+    """Set the coefficient-table and Dirichlet-value pointers of `params` for
+    this space after checking their sizes (ValueError before any GPU call)."""
+    table = getattr(params, "_table", None)
+    params.dev_coeff_table = None
+    params.host_dirichlet = None
+    if table is not None:
+        nq = space.n_quad(params.hexa_quad_rule)
+        params.dev_coeff_table = _ptr(table, space.n_cells * nq * 8, ctx.device, "coeff_table").value
+    d = getattr(params, "_dirichlet", None)
+    if d is not None:
+        n_dir = space.n_dof - space.active_bound
+        if d.ndim != 1 or d.size < n_dir:
+            raise ValueError("dirichlet has %d entries, %d needed" % (d.size, n_dir))
+        params.host_dirichlet = d.ctypes.data
+
+
 def assemble_cd(ctx, space, params, matrix, rhs):
     """Assemble3D/Assemble2D + LocalAssembling(ConvDiff): adds into matrix and rhs."""
     import torch
+    rp = _ptr(rhs, space.n_dof, ctx.device, "rhs")
+    _bind_user_data(ctx, space, params)
     torch.cuda.synchronize()
-    call("pmx_assemble_cd", ctx.h, space.h, ctypes.byref(params), matrix.h, _ptr(rhs, space.n_dof, ctx.device, "rhs"))
+    call("pmx_assemble_cd", ctx.h, space.h, ctypes.byref(params), matrix.h, rp)
     ctx.synchronize()
 
 
@@ -278,9 +325,11 @@ def assemble_tcd(ctx, space, params, stiffness=None, mass=None, rhs=None):
     """Assemble2D/3D + LocalAssembling(TCDStiffMassRhs / TCDStiffRhs): adds into
     the given stiffness matrix, mass matrix and rhs (each optional)."""
     import torch
+    rp = _ptr(rhs, space.n_dof, ctx.device, "rhs") if rhs is not None else None
+    _bind_user_data(ctx, space, params)
     torch.cuda.synchronize()
     call("pmx_assemble_tcd", ctx.h, space.h, ctypes.byref(params), stiffness.h if stiffness else None,
-         mass.h if mass else None, _ptr(rhs, space.n_dof, ctx.device, "rhs") if rhs is not None else None)
+         mass.h if mass else None, rp)
     ctx.synchronize()
 
 
@@ -337,7 +386,9 @@ def compute_errors(ctx, space, params, u):
     import torch
     torch.cuda.synchronize()
     out = (c_double * 4)()
-    call("pmx_compute_errors", ctx.h, space.h, ctypes.byref(params), _ptr(u, space.n_dof, ctx.device, "u"), out)
+    up = _ptr(u, space.n_dof, ctx.device, "u")
+    _bind_user_data(ctx, space, params)
+    call("pmx_compute_errors", ctx.h, space.h, ctypes.byref(params), up, out)
     return list(out)
 
 

@@ -381,6 +381,21 @@ static double bparam(int comp, double x, double y) {
   }
 }
 
+int or_quad_points(const or_space* s, int hexa_rule, double* xyz) {
+  const int shape = s->shape, dim = (shape == OR_HEXA) ? 3 : 2;
+  double qxi[64 * 3], qw[64];
+  const int nq = or_quad_rule(shape, s->order, hexa_rule, qxi, qw);
+  for (int cell = 0; cell < s->n_cells; ++cell) {
+    const double* v = s->verts + (size_t)cell * s->nv * dim;
+    for (int q = 0; q < nq; ++q) {
+      double x[3] = {0, 0, 0}, J[9];
+      ref_map(shape, v, qxi + q * dim, x, J);
+      for (int k = 0; k < 3; ++k) xyz[((size_t)cell * nq + q) * 3 + k] = x[k];
+    }
+  }
+  return nq;
+}
+
 void or_assemble_cd(const or_space* s, const or_cd* p, double* val, double* rhs) {
   const int shape = s->shape, dim = (shape == OR_HEXA) ? 3 : 2, nd = s->ndof_cell;
   const int nc = (dim == 3) ? 6 : 3;
@@ -429,7 +444,13 @@ void or_assemble_cd(const or_space* s, const or_cd* p, double* val, double* rhs)
         lap[k] = l;
       }
       double cf[6];
-      coeffs(p, x[0], x[1], dim == 3 ? x[2] : 0.0, cf);
+      if (p->table) {
+        const double* t = p->table + ((size_t)cell * nq + q) * 8;
+        for (int k = 0; k < 6; ++k) cf[k] = t[k];
+        if (dim == 2) cf[3] = 0.0;
+      } else {
+        coeffs(p, x[0], x[1], dim == 3 ? x[2] : 0.0, cf);
+      }
       const double Mult = qw[q] * fabs(det);
       const double eps = cf[0], b1 = cf[1], b2 = cf[2], b3 = (dim == 3) ? cf[3] : 0.0;
       const double c = cf[4], f = cf[5];

@@ -35,6 +35,11 @@ typedef struct {
   double time;     /* TimeDB->CURRENTTIME (time-dependent examples) */
   double theta1;   /* TimeDB->THETA1 (TCD SUPG) */
   double tau;      /* TimeDB->CURRENTTIMESTEPLENGTH (TCD SUPG) */
+  /* optional [n_cells][n_quad][8] per-point coefficients (eps, b1, b2, b3, c,
+     f, -, -) in the order of or_quad_points: the values a CoeffFct would
+     return there (LocalAssembling::GetParameters, LocalAssembling.C:766-840);
+     NULL: constants + example */
+  const double* table;
 } or_cd;
 
 typedef struct {
@@ -63,6 +68,8 @@ int or_ndof(int shape, int order);
 void or_basis(int shape, int order, const double* xi, double* v, double* g, double* h);
 int or_quad_rule(int shape, int order, int hexa_rule, double* xi, double* w);
 void or_assemble_cd(const or_space* s, const or_cd* p, double* val, double* rhs);
+/* physical quadrature points [n_cells][n_quad][3] of or_assemble_cd's loop */
+int or_quad_points(const or_space* s, int hexa_rule, double* xyz);
 void or_errors(const or_space* s, const or_cd* p, const double* sol, double* err4);
 /* TCD forms; aval, mval, rhs may each be NULL */
 void or_assemble_tcd(const or_space* s, const or_cd* p, double* aval, double* mval, double* rhs);

@@ -372,7 +372,8 @@ class CD(ctypes.Structure):
                 ("bc_sine_laplace", ctypes.c_int), ("eps", ctypes.c_double),
                 ("b", ctypes.c_double * 3), ("c", ctypes.c_double), ("f0", ctypes.c_double),
                 ("delta0", ctypes.c_double), ("poly_degree", ctypes.c_double),
-                ("time", ctypes.c_double), ("theta1", ctypes.c_double), ("tau", ctypes.c_double)]
+                ("time", ctypes.c_double), ("theta1", ctypes.c_double), ("tau", ctypes.c_double),
+                ("table", ctypes.c_void_p)]
 
 
 class SolverParams(ctypes.Structure):
@@ -402,6 +403,8 @@ def lib():
         L = ctypes.CDLL(LIB)
         L.or_assemble_cd.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p, ctypes.c_void_p]
         L.or_errors.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p, ctypes.c_void_p]
+        L.or_quad_points.argtypes = [ctypes.POINTER(Space), ctypes.c_int, ctypes.c_void_p]
+        L.or_quad_points.restype = ctypes.c_int
         L.or_assemble_tcd.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p]
         L.or_interpolate.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(CD), ctypes.c_void_p]
@@ -466,14 +469,29 @@ def build_space(geometry, n_refine, level, order, cond2d=None, cond3d=None):
 
 
 def cd_params(example=0, disc=0, eps=1.0, b=(0, 0, 0), c=0.0, f0=0.0, hexa_rule=0,
-              sine_laplace_bc=False, delta0=1.0, poly_degree=1.0, time=0.0, theta1=1.0, tau=0.0):
+              sine_laplace_bc=False, delta0=1.0, poly_degree=1.0, time=0.0, theta1=1.0, tau=0.0,
+              table=None):
+    """`table`: optional [n_cells, n_quad, 8] per-point coefficients (eps, b1,
+    b2, b3, c, f, -, -) at the points of quad_points(); kept alive on the
+    returned struct."""
     p = CD()
+    if table is not None:
+        p._table = np.ascontiguousarray(table, dtype=np.float64)
+        p.table = p._table.ctypes.data
     p.disc, p.example, p.hexa_rule, p.bc_sine_laplace = disc, example, hexa_rule, int(sine_laplace_bc)
     p.eps, p.c, p.f0, p.delta0, p.poly_degree = eps, c, f0, delta0, poly_degree
     p.time, p.theta1, p.tau = time, theta1, tau
     for k in range(3):
         p.b[k] = b[k] if k < len(b) else 0.0
     return p
+
+
+def quad_points(space, hexa_rule=0):
+    """physical quadrature points [n_cells, n_quad, 3] of the assembly loop"""
+    out = np.zeros((space.n_cells, 64, 3))
+    s = space.cstruct()
+    nq = lib().or_quad_points(ctypes.byref(s), hexa_rule, _p(out))
+    return np.ascontiguousarray(out.reshape(-1)[: space.n_cells * nq * 3].reshape(space.n_cells, nq, 3))
 
 
 def assemble(space, params):

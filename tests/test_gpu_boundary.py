@@ -1,8 +1,10 @@
 """The C-ABI boundary as a caller meets it: argument checks that run before
 any kernel, the error codes the reference turns into exceptions (zero
 diagonal, divergence, unhappy breakdown), the BLAS-1 entry points, the
-F-cycle, the CD2D Poisson configuration (64 x 64 Q1, CG + Jacobi to 1e-10)
-and the three-component SELL pass of the saddle-point operator."""
+F-cycle, the CD2D Poisson configuration (64 x 64 Q1, CG + Jacobi to 1e-10),
+the three-component SELL pass of the saddle-point operator, and the two
+routes for caller data: a coefficient table in place of the CoeffFct
+callback and caller-given Dirichlet values."""
 import math
 
 import numpy as np
@@ -259,3 +261,114 @@ def test_saddle_velocity_block_sell_path(ctx):
         assert np.max(np.abs(out[rows] - yo)) <= 1e-12 * np.max(np.abs(yo)), rows
     assert np.max(np.abs(out[0] - out[131072])) <= 1e-13 * np.max(np.abs(yo))
     g.set_velocity_sell_rows(131072)
+
+
+def _exp_sin_rhs(X, eps, b, c):
+    """-eps lap u + b.grad u + c u of u = exp(-x) sin(pi y) sin(pi z)"""
+    x, y, z = X[..., 0], X[..., 1], X[..., 2]
+    u = np.exp(-x) * np.sin(np.pi * y) * np.sin(np.pi * z)
+    ux = -u
+    uy = np.pi * np.exp(-x) * np.cos(np.pi * y) * np.sin(np.pi * z)
+    uz = np.pi * np.exp(-x) * np.sin(np.pi * y) * np.cos(np.pi * z)
+    lap = (1 - 2 * np.pi ** 2) * u
+    return -eps * lap + b[0] * ux + b[1] * uy + b[2] * uz + c * u
+
+
+def _variable_table(X):
+    """coefficients of a CoeffFct that no built-in example has: variable
+    diffusion, rotating convection, variable reaction and source"""
+    x, y, z = X[..., 0], X[..., 1], X[..., 2]
+    T = np.zeros(X.shape[:2] + (8,))
+    T[..., 0] = 1e-2 * (1 + x * y)
+    T[..., 1] = 1 + y
+    T[..., 2] = 0.25 * x - z
+    T[..., 3] = 0.5 * x
+    T[..., 4] = 1 + x * y * z
+    T[..., 5] = np.sin(3 * x) + y * z + y
+    return T
+
+
+@pytest.mark.parametrize("rule", [0, 2])
+def test_coefficient_table_reproduces_the_example(ctx, rule):
+    """pmx_cd_params.dev_coeff_table, the route of a user CoeffFct3D
+    (LocalAssembling::GetParameters, LocalAssembling.C:766-840): the
+    exp-sin example's coefficients sampled at pmx_assembly_quad_points give
+    the example's SUPG assembly -- the matrix bit for bit (the same
+    constants), the load vector within 1e-13 (f evaluated by numpy instead of
+    the kernel)."""
+    nref = 2
+    d = pm.TDomain("Default_UnitCube_Hexa", nref)
+    s = pm.TFESpace(d, nref, 2, ctx)
+    eps, b, c = 1e-2, (0.3, -0.2, 0.5), 0.4
+    kw = dict(disc="supg", eps=eps, b=b, c=c, hexa_quad_rule=rule)
+    A0 = pm.FEMatrix(ctx, s)
+    f0 = torch.zeros(s.n_dof, dtype=torch.float64, device="cuda")
+    pm.assemble_cd(ctx, s, pm.cd_params("exp_sin_3d", **kw), A0, f0)
+    X = s.quad_points(rule)
+    assert X.shape == (s.n_cells, 64 if rule == 0 else 27, 3)
+    T = np.zeros(X.shape[:2] + (8,))
+    T[..., 0], T[..., 1:4], T[..., 4] = eps, b, c
+    T[..., 5] = _exp_sin_rhs(X, eps, b, c)
+    A1 = pm.FEMatrix(ctx, s)
+    f1 = torch.zeros_like(f0)
+    pm.assemble_cd(ctx, s, pm.cd_params("exp_sin_3d", coeff_table=dev(T), **kw), A1, f1)
+    assert np.array_equal(A1.get_entries(), A0.get_entries())
+    fa, fb = f0.cpu().numpy(), f1.cpu().numpy()
+    assert np.max(np.abs(fa - fb)) <= 1e-13 * np.max(np.abs(fa))
+
+
+@pytest.mark.parametrize("geo,nref,order,disc", [(2, 2, 2, "galerkin"), (2, 2, 2, "supg"), (2, 2, 1, "supg"),
+                                                 (0, 3, 2, "supg"), (0, 3, 1, "galerkin")])
+def test_coefficient_table_variable_coefficients_match_oracle(ctx, geo, nref, order, disc):
+    """A coefficient function no example has (variable eps, b, c, f) through
+    dev_coeff_table against the oracle fed the same function at its own
+    quadrature points (which are the product's, in the same order): entries
+    and load vector within 1e-12; SUPG takes its delta from each point's
+    coefficients, as BilinearAssemble_SD does."""
+    rule = 2
+    d = pm.TDomain({0: "UnitSquare", 2: "Default_UnitCube_Hexa"}[geo], nref)
+    s = pm.TFESpace(d, nref, order, ctx)
+    Xg = s.quad_points(rule)
+    osp, _ = O.build_space(geo, nref, nref, order)
+    Xo = O.quad_points(osp, rule)
+    assert Xg.shape == Xo.shape and np.max(np.abs(Xg - Xo)) <= 1e-15
+    ex = "exp_sin_3d" if geo == 2 else "sine_2d"
+    A = pm.FEMatrix(ctx, s)
+    f = torch.zeros(s.n_dof, dtype=torch.float64, device="cuda")
+    pm.assemble_cd(ctx, s, pm.cd_params(ex, disc=disc, hexa_quad_rule=rule, coeff_table=dev(_variable_table(Xg))),
+                   A, f)
+    val, rhs = O.assemble(osp, O.cd_params(example=5 if geo == 2 else 6, disc=1 if disc == "supg" else 0,
+                                           hexa_rule=rule, table=_variable_table(Xo)))
+    rp, col = A.get_structure()
+    assert np.array_equal(rp, osp.rp) and np.array_equal(col, osp.col)
+    assert np.max(np.abs(A.get_entries() - val)) <= 1e-12 * np.max(np.abs(val))
+    assert np.max(np.abs(f.cpu().numpy() - rhs)) <= 1e-12 * np.max(np.abs(rhs))
+
+
+def test_host_dirichlet_values(ctx):
+    """pmx_cd_params.host_dirichlet: boundary values computed by the caller
+    at pmx_space_get_dof_coords replace the example's (Assemble3D.C:593-667
+    sets the Dirichlet rhs entries to the boundary values): the Dirichlet
+    part of the rhs is the given array, the active part and the matrix are
+    unchanged; a too short array is refused before any kernel."""
+    nref = 2
+    d = pm.TDomain("Default_UnitCube_Hexa", nref)
+    s = pm.TFESpace(d, nref, 2, ctx)
+    ab = s.active_bound
+    xyz = s.dof_coords()[ab:]
+    g = 1.0 + xyz[:, 0] ** 2 - 2.0 * xyz[:, 1] * xyz[:, 2]
+    kw = dict(disc="supg", eps=1e-2, b=(0.3, -0.2, 0.5), hexa_quad_rule=2)
+    A0, A1 = pm.FEMatrix(ctx, s), pm.FEMatrix(ctx, s)
+    f0 = torch.zeros(s.n_dof, dtype=torch.float64, device="cuda")
+    f1 = torch.zeros_like(f0)
+    pm.assemble_cd(ctx, s, pm.cd_params("exp_sin_3d", **kw), A0, f0)
+    pm.assemble_cd(ctx, s, pm.cd_params("exp_sin_3d", dirichlet=g, **kw), A1, f1)
+    assert np.array_equal(A0.get_entries(), A1.get_entries())
+    a, b = f0.cpu().numpy(), f1.cpu().numpy()
+    assert np.array_equal(a[:ab], b[:ab])
+    assert np.array_equal(b[ab:], g)
+    # the example's own boundary values at the same points, as a cross-check
+    u = np.exp(-xyz[:, 0]) * np.sin(np.pi * xyz[:, 1]) * np.sin(np.pi * xyz[:, 2])
+    assert np.max(np.abs(a[ab:] - u)) <= 1e-15
+    with pytest.raises(ValueError):
+        pm.assemble_cd(ctx, s, pm.cd_params("exp_sin_3d", dirichlet=g[:-1], **kw), A1, f1)
