@@ -1485,15 +1485,22 @@ void fe_apply_active_bf16(const DevCSR& A, const float* x, float* y, const float
 // Dense in-place inversion by blocked Gauss-Jordan elimination with partial
 // pivoting (set-up of the exact coarse-grid solve); A is n x n row-major.
 // Per block column K of width b <= 64:
+//  0. speculation: k_bgj_block_inverse (spec) and k_bgj_check_below test
+//     whether partial pivoting on the panel A[K:, K:K+b] would keep the
+//     diagonal pivots (every multiplier at most 1 in magnitude, ties to the
+//     lower row); then steps 1, 2 and the second block inverse do nothing;
 //  1. k_bgj_panel_reg (one workgroup; k_bgj_panel when the panel has more
-//     than 5,120 rows): LU with partial pivoting of the panel A[K:, K:K+b]
+//     than 5,120 rows): LU with partial pivoting of the panel
 //     in a scratch copy chooses the b pivot rows;
 //  2. k_bgj_swap: the pivot rows are swapped into rows K..K+b-1;
 //  3. the block sweep with P = A_KK^-1 (k_bgj_block_inverse, Gauss-Jordan
 //     without further pivoting on the b x b block, whose LU needs none after
-//     step 1), C = A[:, K] with its K rows zeroed, R = P A[K, :]:
-//        A[o, o] -= C R   (k_bgj_update, FP64 matrix cores, 64 x 64 tiles)
-//        A[K, o] = R,  A[o, K] = -C P,  A[K, K] = P   (k_bgj_fix_*).
+//     step 1), in place (o: the rows / columns outside the block):
+//        C = A[o, K] (compact copy, k_bgj_make_c)
+//        R = A[K, o] = P A[K, o],  A[K, K] = P      (k_bgj_rows)
+//        A[o, o] -= C R                              (k_bgj_update)
+//        A[o, K] = -A[o, K] P                        (k_bgj_cols)
+//     the products on FP64 matrix cores.
 // The row swaps leave the inverse of the row-permuted matrix; its columns
 // are put back in order at the end (k_bgj_unpermute, in place, one row per
 // workgroup through LDS).  The trailing update
@@ -1506,9 +1513,10 @@ typedef double v4d_bgj __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(1024) void k_bgj_panel(int n, int K, int b, const double* __restrict__ A,
                                                      double* __restrict__ W, int* __restrict__ piv,
-                                                     int* __restrict__ singular) {
+                                                     int* __restrict__ singular, const int* __restrict__ need) {
   __shared__ double sv[1024];
   __shared__ int si[1024];
+  if (!*need) return;   // the diagonal pivots are the partial-pivoting choice
   const int m = n - K;   // panel rows
   for (size_t e = threadIdx.x; e < (size_t)m * b; e += blockDim.x) {
     const int i = (int)(e / b), jj = (int)(e % b);
@@ -1574,13 +1582,15 @@ constexpr int kPanelThreads = 512, kPanelRpt = 10, kPanelSub = 6;
 template <int RPT>
 __global__ __launch_bounds__(kPanelThreads) void k_bgj_panel_reg(int n, int K, int b, const double* __restrict__ A,
                                                                  double* __restrict__ W, int* __restrict__ piv,
-                                                                 int* __restrict__ singular) {
+                                                                 int* __restrict__ singular,
+                                                                 const int* __restrict__ need) {
   constexpr int NT = kPanelThreads, SUB = kPanelSub, NW = NT / 64;
   __shared__ double s_best[NW];
   __shared__ int s_bi[NW];
   __shared__ double s_prow[SUB], s_jrow[SUB];
   __shared__ double s_l[SUB][SUB];
   __shared__ double s_u[SUB][kBgjBlock];
+  if (!*need) return;   // the diagonal pivots are the partial-pivoting choice
   const int m = n - K, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   for (size_t e = tid; e < (size_t)m * b; e += NT) {
     const int i = (int)(e / b), jj = (int)(e % b);
@@ -1740,75 +1750,173 @@ __global__ void k_bgj_swap(int n, int K, int b, double* __restrict__ A, const in
 
 // P = A[K:K+b, K:K+b]^-1 by in-place Gauss-Jordan without pivoting, padded
 // to bp x bp with zeros.  Per step the pivot row and column are copied first,
-// so every entry is updated in one pass from the old values (two barriers).
-__global__ __launch_bounds__(512) void k_bgj_block_inverse(int n, int K, int b, int bp, const double* __restrict__ A,
-                                                           double* __restrict__ P, int* __restrict__ singular) {
+// so every entry is updated in one pass from the old values (two barriers);
+// thread (c, g) updates column c of the rows g, g + 4, ..., all 16 reads of
+// a step issued together.
+//
+// spec = 1 runs first, on the block as it stands (diagonal pivots): it marks
+// the pivots as the identity, keeps row k of step k (the row of U in the LU
+// of the block) for U, and sets *need when a pivot is zero or an entry below
+// it is larger in magnitude -- then partial pivoting would swap rows and the
+// panel kernel decides the pivots, after which a spec = 0 launch inverts the
+// swapped block.  spec = 0 does nothing unless *need is set.
+__global__ __launch_bounds__(256) void k_bgj_block_inverse(int n, int K, int b, int bp, const double* __restrict__ A,
+                                                           double* __restrict__ P, double* __restrict__ U,
+                                                           int* __restrict__ need, int* __restrict__ piv,
+                                                           int* __restrict__ singular, int spec) {
   __shared__ double B[kBgjBlock][kBgjBlock + 1];
+  __shared__ double Us[kBgjBlock][kBgjBlock];   // U rows, stored once at the end
   __shared__ double rowk[kBgjBlock], colk[kBgjBlock];
-  for (int e = threadIdx.x; e < b * b; e += blockDim.x) B[e / b][e % b] = A[(size_t)(K + e / b) * n + K + e % b];
+  __shared__ int s_need;
+  if (!spec && !*need) return;
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;   // 4 row groups
+  if (threadIdx.x == 0) s_need = 0;
+  if (spec && (int)threadIdx.x < b) piv[K + threadIdx.x] = K + threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < kBgjBlock / 4; ++m) {
+    const int i = g + 4 * m;
+    B[i][c] = (i < b && c < b) ? A[(size_t)(K + i) * n + K + c] : 0.0;
+  }
   for (int k = 0; k < b; ++k) {
     __syncthreads();
     const double d = B[k][k];
     if (d == 0.0) {
-      if (threadIdx.x == 0) singular[0] = 1;
+      if (threadIdx.x == 0) {
+        if (spec) *need = 1;
+        else singular[0] = 1;
+      }
       return;
     }
     if ((int)threadIdx.x < b) {
       rowk[threadIdx.x] = B[k][threadIdx.x];
       colk[threadIdx.x] = B[threadIdx.x][k];
+      if (spec) {
+        Us[k][threadIdx.x] = ((int)threadIdx.x >= k) ? B[k][threadIdx.x] : 0.0;
+        if ((int)threadIdx.x > k && !(fabs(B[threadIdx.x][k]) <= fabs(d))) s_need = 1;
+      }
     }
     __syncthreads();
     const double pk = 1.0 / d;
+    const double rc = rowk[c] * pk;
     // row k: B[k][c] pk, B[k][k] = pk; rows i != k: B[i][c] -= B[i][k] (B[k][c] pk),
     // B[i][k] = -B[i][k] pk
-    for (int e = threadIdx.x; e < b * b; e += blockDim.x) {
-      const int i = e / b, c = e % b;
-      if (i == k) B[k][c] = (c == k) ? pk : rowk[c] * pk;
-      else if (c == k) B[i][k] = -colk[i] * pk;
-      else B[i][c] -= colk[i] * (rowk[c] * pk);
+    double bi[kBgjBlock / 4], ci[kBgjBlock / 4];
+#pragma unroll
+    for (int m = 0; m < kBgjBlock / 4; ++m) {
+      bi[m] = B[g + 4 * m][c];
+      ci[m] = colk[g + 4 * m];
+    }
+#pragma unroll
+    for (int m = 0; m < kBgjBlock / 4; ++m) {
+      const int i = g + 4 * m;
+      if (i < b && c < b) {
+        double x;
+        if (i == k) x = (c == k) ? pk : rc;
+        else if (c == k) x = -ci[m] * pk;
+        else x = bi[m] - ci[m] * rc;
+        B[i][c] = x;
+      }
     }
   }
   __syncthreads();
+  if (spec) {
+    if (threadIdx.x == 0) *need = s_need;
+    for (int e = threadIdx.x; e < b * kBgjBlock; e += blockDim.x) U[e] = Us[e >> 6][e & 63];
+  }
   for (int e = threadIdx.x; e < bp * bp; e += blockDim.x) {
-    const int t = e / bp, c = e % bp;
-    P[e] = (t < b && c < b) ? B[t][c] : 0.0;
+    const int t = e / bp, cc = e % bp;
+    P[e] = (t < b && cc < b) ? B[t][cc] : 0.0;
   }
 }
 
-// C[i][s] = A[i][K + s] (zero for the rows K..K+b-1 and for s >= b)
+// Rows below the block under diagonal pivots: the multipliers of row i are
+// l = a U^-1 (a = A[i][K:K+b], U from the speculative block inverse), and
+// partial pivoting keeps the diagonal as long as every |a_j - sum_t<j l_t
+// U_tj| <= |U_jj| (ties go to the lower row, the diagonal one).  One thread
+// per row; sets *need otherwise.
+__global__ __launch_bounds__(256) void k_bgj_check_below(int n, int K, int b, const double* __restrict__ A,
+                                                         const double* __restrict__ U, int* __restrict__ need) {
+  __shared__ double Us[kBgjBlock][kBgjBlock + 1];
+  for (int e = threadIdx.x; e < kBgjBlock * kBgjBlock; e += blockDim.x) {
+    const int t = e >> 6, j = e & 63;
+    Us[t][j] = (t < b && j < b && j >= t) ? U[e] : 0.0;
+  }
+  __syncthreads();
+  const int i = K + b + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* a = A + (size_t)i * n + K;
+  double l[kBgjBlock];   // register array: constant indices after unrolling
+  bool bad = false;
+#pragma unroll
+  for (int j = 0; j < kBgjBlock; ++j) {
+    if (j < b) {
+      double r = a[j];
+#pragma unroll
+      for (int t = 0; t < j; ++t) r -= l[t] * Us[t][j];
+      bad |= !(fabs(r) <= fabs(Us[j][j]));
+      l[j] = r / Us[j][j];
+    } else {
+      l[j] = 0.0;
+    }
+  }
+  if (bad) *need = 1;
+}
+
+// Rows K..K+b-1 in place: A[K:K+b, c] = P A[K:K+b, c] outside the block
+// columns and P inside them.  One workgroup per 64 columns: the 64 x 64
+// operands in LDS, the product on FP64 matrix cores (wave w: rows
+// 16w..16w+15 of the 64 x 64 result, v_mfma_f64_16x16x4f64 with A operand
+// (m = lane & 15, k = lane >> 4), B operand (k = lane >> 4, n = lane & 15)
+// and accumulator entry t at (m = 4t + (lane >> 4), n = lane & 15)).
+__global__ __launch_bounds__(256) void k_bgj_rows(int n, int K, int b, int bp, double* __restrict__ A,
+                                                  const double* __restrict__ P, double* __restrict__ R) {
+  __shared__ double Ps[kBgjBlock][kBgjBlock + 1];
+  __shared__ double Ak[kBgjBlock][kBgjBlock + 1];
+  const int c0 = blockIdx.x * 64, cc = threadIdx.x & 63, g = threadIdx.x >> 6;
+  for (int r = g; r < kBgjBlock; r += 4) {
+    Ps[r][cc] = (r < b && cc < b) ? P[r * bp + cc] : 0.0;
+    Ak[r][cc] = (r < b && c0 + cc < n) ? A[(size_t)(K + r) * n + c0 + cc] : 0.0;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  v4d_bgj acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = v4d_bgj{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k0 = 0; k0 < kBgjBlock; k0 += 4) {
+    const double av = Ps[16 * w + (lane & 15)][k0 + (lane >> 4)];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Ak[k0 + (lane >> 4)][16 * j + (lane & 15)], acc[j], 0, 0, 0);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int r = 16 * w + 4 * t + (lane >> 4), c = c0 + 16 * j + (lane & 15);
+      if (c < n && r < bp) R[(size_t)r * n + c] = acc[j][t];   // zero rows r >= b (P padded with zeros)
+      if (r < b && c < n) A[(size_t)(K + r) * n + c] = (c >= K && c < K + b) ? Ps[r][c - K] : acc[j][t];
+    }
+}
+
+// C[i][s] = A[i][K + s] for the rows outside the block (zero rows inside it
+// and zero columns s >= b up to bp): a compact copy of the operand of
+// k_bgj_update, taken before k_bgj_cols overwrites the block columns
 __global__ void k_bgj_make_c(int n, int K, int b, int bp, const double* __restrict__ A, double* __restrict__ C) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)n * bp) return;
-  const int i = (int)(e / bp), c = (int)(e % bp);
-  C[e] = (c < b && (i < K || i >= K + b)) ? A[(size_t)i * n + K + c] : 0.0;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), s = threadIdx.x & 63;
+  if (i >= n || s >= bp) return;
+  C[(size_t)i * bp + s] = (s < b && (i < K || i >= K + b)) ? A[(size_t)i * n + K + s] : 0.0;
 }
 
-// R[t][c] = sum_s P[t][s] A[K + s][c] for t < b (zero rows up to bp); one
-// thread per column c and 16 rows t (blockIdx.y), so a launch has enough
-// workgroups to spread over the chip
-__global__ __launch_bounds__(64) void k_bgj_make_r(int n, int K, int b, int bp, const double* __restrict__ A,
-                                                   const double* __restrict__ P, double* __restrict__ R) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int t0 = blockIdx.y * 16;
-  if (c >= n || t0 >= bp) return;
-  double a[kBgjBlock];   // register array: every index below is a constant after unrolling
-#pragma unroll
-  for (int s = 0; s < kBgjBlock; ++s) a[s] = (s < b) ? A[(size_t)(K + s) * n + c] : 0.0;
-  for (int t = t0; t < t0 + 16 && t < bp; ++t) {
-    double acc = 0.0;
-#pragma unroll
-    for (int s = 0; s < kBgjBlock; ++s)
-      if (s < b) acc += P[t * bp + s] * a[s];
-    R[(size_t)t * n + c] = acc;
-  }
-}
-
-// A -= C R (n x bp times bp x n) on 64 x 64 tiles with v_mfma_f64_16x16x4f64:
-// wave w of a workgroup owns tile rows 16w .. 16w+15 and four 16-column
-// MFMA tiles; operands straight from the (cache-resident) C and R
-__global__ __launch_bounds__(256) void k_bgj_update(int n, int bp, const double* __restrict__ C,
+// A[o, o'] -= C R for the rows o and columns o' outside the block (C = the
+// old block columns, R = P A[K, :]; the block rows already hold R, the block
+// columns are rewritten by k_bgj_cols), on 64 x 64 tiles with
+// v_mfma_f64_16x16x4f64: wave w owns tile rows 16w .. 16w+15 and four
+// 16-column MFMA tiles; operands straight from the (cache-resident) C and R.
+__global__ __launch_bounds__(256) void k_bgj_update(int n, int K, int b, int bp, const double* __restrict__ C,
                                                     const double* __restrict__ R, double* __restrict__ A) {
   const int ti = blockIdx.y * 64, tj = blockIdx.x * 64;
+  if ((ti >= K && ti + 64 <= K + b) || (tj >= K && tj + 64 <= K + b)) return;   // block rows / columns only
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int arow = ti + 16 * w + (lane & 15);
   v4d_bgj acc[4];
@@ -1829,44 +1937,42 @@ __global__ __launch_bounds__(256) void k_bgj_update(int n, int bp, const double*
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int row = ti + 16 * w + 4 * t + (lane >> 4), col = tj + 16 * j + (lane & 15);
-      if (row < n && col < n) A[(size_t)row * n + col] -= acc[j][t];
+      if (row < n && col < n && (row < K || row >= K + b) && (col < K || col >= K + b))
+        A[(size_t)row * n + col] -= acc[j][t];
     }
 }
 
-// rows K..K+b-1: R outside the block columns, P inside
-__global__ void k_bgj_fix_rows(int n, int K, int b, int bp, const double* __restrict__ R, const double* __restrict__ P,
-                               double* __restrict__ A) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  for (int t = 0; t < b; ++t)
-    A[(size_t)(K + t) * n + c] = (c >= K && c < K + b) ? P[t * bp + (c - K)] : R[(size_t)t * n + c];
-}
-
-// the other rows' block columns: -C P.  64 rows per workgroup, four
-// threads per row (16 columns each) into LDS, then stored row by row so the
-// writes of a wave cover whole row segments
-__global__ __launch_bounds__(256) void k_bgj_fix_cols(int n, int K, int b, int bp, const double* __restrict__ C,
-                                                      const double* __restrict__ P, double* __restrict__ A) {
-  __shared__ double out[64][kBgjBlock + 1];
-  const int r = threadIdx.x & 63, sq = threadIdx.x >> 6;
-  const int i0 = blockIdx.x * 64, i = i0 + r;
-  if (i < n) {
-    double ci[kBgjBlock];   // register array, as in k_bgj_make_r
-#pragma unroll
-    for (int t = 0; t < kBgjBlock; ++t) ci[t] = (t < b) ? C[(size_t)i * bp + t] : 0.0;
-    for (int s = sq * 16; s < sq * 16 + 16 && s < b; ++s) {
-      double acc = 0.0;
-#pragma unroll
-      for (int t = 0; t < kBgjBlock; ++t)
-        if (t < b) acc += ci[t] * P[t * bp + s];
-      out[r][s] = -acc;
-    }
+// The block columns of the other rows in place: A[o, K:K+b] = -A[o, K:K+b] P,
+// 64 rows per workgroup on FP64 matrix cores (layout as in k_bgj_rows).
+__global__ __launch_bounds__(256) void k_bgj_cols(int n, int K, int b, int bp, double* __restrict__ A,
+                                                  const double* __restrict__ P) {
+  __shared__ double Ps[kBgjBlock][kBgjBlock + 1];
+  __shared__ double Cs[kBgjBlock][kBgjBlock + 1];
+  const int i0 = blockIdx.x * 64, cc = threadIdx.x & 63, g = threadIdx.x >> 6;
+  for (int r = g; r < kBgjBlock; r += 4) {
+    Ps[r][cc] = (r < b && cc < b) ? P[r * bp + cc] : 0.0;
+    const int i = i0 + r;
+    Cs[r][cc] = (i < n && cc < b) ? A[(size_t)i * n + K + cc] : 0.0;
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * b; e += blockDim.x) {
-    const int rr = e / b, s = e % b, ii = i0 + rr;
-    if (ii < n && (ii < K || ii >= K + b)) A[(size_t)ii * n + K + s] = out[rr][s];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  v4d_bgj acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = v4d_bgj{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k0 = 0; k0 < kBgjBlock; k0 += 4) {
+    const double av = Cs[16 * w + (lane & 15)][k0 + (lane >> 4)];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Ps[k0 + (lane >> 4)][16 * j + (lane & 15)], acc[j], 0, 0, 0);
   }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int i = i0 + 16 * w + 4 * t + (lane >> 4), sidx = 16 * j + (lane & 15);
+      if (i < n && (i < K || i >= K + b) && sidx < b) A[(size_t)i * n + K + sidx] = -acc[j][t];
+    }
 }
 
 // out[r][c] = A[r][q[c]]
@@ -1925,31 +2031,38 @@ bool dense_invert(int n, double* A, hipStream_t st) {
   // through a second matrix
   const bool in_place = (size_t)n * sizeof(double) <= 65536;
   double* T = in_place ? nullptr : bgj_alloc<double>((size_t)n * n);
+  double* U = bgj_alloc<double>((size_t)nb * nb);
   int* piv = bgj_alloc<int>(n);
   int* q = bgj_alloc<int>(n);
-  int* sing = bgj_alloc<int>(1);
+  int* sing = bgj_alloc<int>(2);   // [0] singular, [1] pivoting needed for the current panel
+  int* need = sing + 1;
   auto release = [&]() {
-    (void)hipFree(W); (void)hipFree(C); (void)hipFree(R); (void)hipFree(P);
+    (void)hipFree(W); (void)hipFree(C); (void)hipFree(R); (void)hipFree(P); (void)hipFree(U);
     if (T) (void)hipFree(T);
     (void)hipFree(piv); (void)hipFree(q); (void)hipFree(sing);
   };
-  (void)hipMemsetAsync(sing, 0, sizeof(int), st);
+  (void)hipMemsetAsync(sing, 0, 2 * sizeof(int), st);
   const int g1 = (n + 255) / 256;
   const dim3 tiles((n + 63) / 64, (n + 63) / 64);
   for (int K = 0; K < n; K += nb) {
     const int b = n - K < nb ? n - K : nb;
     const int bp = (b + 3) / 4 * 4;
+    // diagonal pivots first; the pivoting panel and a second block inverse
+    // only when partial pivoting would swap rows in this panel
+    hipLaunchKernelGGL(k_bgj_block_inverse, dim3(1), dim3(256), 0, st, n, K, b, bp, A, P, U, need, piv, sing, 1);
+    if (n - K - b > 0)
+      hipLaunchKernelGGL(k_bgj_check_below, dim3((n - K - b + 255) / 256), dim3(256), 0, st, n, K, b, A, U, need);
     if (n - K <= kPanelThreads * kPanelRpt)
-      hipLaunchKernelGGL(k_bgj_panel_reg<kPanelRpt>, dim3(1), dim3(kPanelThreads), 0, st, n, K, b, A, W, piv, sing);
+      hipLaunchKernelGGL(k_bgj_panel_reg<kPanelRpt>, dim3(1), dim3(kPanelThreads), 0, st, n, K, b, A, W, piv, sing,
+                         need);
     else
-      hipLaunchKernelGGL(k_bgj_panel, dim3(1), dim3(1024), 0, st, n, K, b, A, W, piv, sing);
+      hipLaunchKernelGGL(k_bgj_panel, dim3(1), dim3(1024), 0, st, n, K, b, A, W, piv, sing, need);
     hipLaunchKernelGGL(k_bgj_swap, dim3(g1), dim3(256), 0, st, n, K, b, A, piv);
-    hipLaunchKernelGGL(k_bgj_block_inverse, dim3(1), dim3(512), 0, st, n, K, b, bp, A, P, sing);
-    hipLaunchKernelGGL(k_bgj_make_c, dim3((int)(((size_t)n * bp + 255) / 256)), dim3(256), 0, st, n, K, b, bp, A, C);
-    hipLaunchKernelGGL(k_bgj_make_r, dim3((n + 63) / 64, (bp + 15) / 16), dim3(64), 0, st, n, K, b, bp, A, P, R);
-    hipLaunchKernelGGL(k_bgj_update, tiles, dim3(256), 0, st, n, bp, C, R, A);
-    hipLaunchKernelGGL(k_bgj_fix_rows, dim3(g1), dim3(256), 0, st, n, K, b, bp, R, P, A);
-    hipLaunchKernelGGL(k_bgj_fix_cols, dim3((n + 63) / 64), dim3(256), 0, st, n, K, b, bp, C, P, A);
+    hipLaunchKernelGGL(k_bgj_block_inverse, dim3(1), dim3(256), 0, st, n, K, b, bp, A, P, U, need, piv, sing, 0);
+    hipLaunchKernelGGL(k_bgj_make_c, dim3((n + 3) / 4), dim3(256), 0, st, n, K, b, bp, A, C);
+    hipLaunchKernelGGL(k_bgj_rows, dim3((n + 63) / 64), dim3(256), 0, st, n, K, b, bp, A, P, R);
+    hipLaunchKernelGGL(k_bgj_update, tiles, dim3(256), 0, st, n, K, b, bp, C, R, A);
+    hipLaunchKernelGGL(k_bgj_cols, dim3((n + 63) / 64), dim3(256), 0, st, n, K, b, bp, A, P);
   }
   // the swaps in order make the row permutation p (row i of the permuted
   // matrix = row p[i]); A^-1[r][p[k]] = M[r][k], i.e. column c comes from
