@@ -41,7 +41,13 @@ int pmx_version(void);
 
 /* ----------------------------------------------------------------- context */
 typedef struct pmx_ctx pmx_ctx;
-/* Binds HIP device `device` and creates a stream. */
+/* Binds HIP device `device` and creates the context's stream, a
+ * non-blocking one: every call on the context runs there, ordered among
+ * themselves but not with the caller's own HIP work.  Device buffers handed
+ * in must be complete (synchronous copies, or hipDeviceSynchronize after
+ * asynchronous ones); copies between buffers the library uses go through
+ * pmx_vector_copy; results are complete when a call that returns host values
+ * returns, or after pmx_ctx_synchronize. */
 int pmx_ctx_create(int device, pmx_ctx** out);
 int pmx_ctx_destroy(pmx_ctx* ctx);
 int pmx_ctx_synchronize(pmx_ctx* ctx);
@@ -587,7 +593,13 @@ typedef struct {
                                   length; each row summed sequentially in column order)
                                   when the product has at least 131,072 rows, else CSR
                                   with 1-8 lanes per row (partial sums combined across
-                                  lanes).  2: SELL for every level.  0: CSR throughout */
+                                  lanes).  2: SELL for every level.  3 (default): as 1,
+                                  and on renumbered levels the rows that share their
+                                  column offsets and weights (most of them on structured
+                                  hierarchies) are grouped into slices that read those
+                                  once per wave, with the Dirichlet columns left out (the
+                                  transfers' inputs vanish there); same sums as 1 bit for
+                                  bit.  4: as 3 for every level.  0: CSR throughout */
   int vanka_inverse;           /* no reference counterpart.  1: the cell_vanka_jacobi
                                   smoother stores the inverse of each cell's local matrix
                                   (from its LU factors, in place, as dgetri) and applies it

@@ -55,12 +55,33 @@ def _compile(src):
     return obj
 
 
+CALLER_SRC = os.path.join(ROOT, "tests", "host", "pmx_caller.cpp")
+CALLER = os.path.join(ROOT, "tests", "host", "pmx_caller")
+
+
+def build_caller(verbose=False):
+    """The C++ program of tests/host/pmx_caller.cpp, compiled with g++ against
+    include/pmx.h and linked with libpmx.so (found next to this file through
+    its rpath), as a program on the reference side would be."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(ROCM, "include"), CALLER_SRC, "-o", CALLER + ".tmp", "-L" + HERE, "-lpmx",
+           "-L" + os.path.join(ROCM, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/../../mymoonmd_amd:" + os.path.join(ROCM, "lib")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("caller build failed: %s\n%s\n%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(CALLER + ".tmp", CALLER)
+    if verbose:
+        print("built", CALLER)
+    return CALLER
+
+
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-fopenmp",
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-fopenmp", "-Wl,--no-undefined",
            "-Wl,-rpath," + os.path.join(ROCM, "llvm", "lib"), "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -68,6 +89,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_caller(verbose)
     return LIB
 
 

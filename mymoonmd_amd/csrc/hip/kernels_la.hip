@@ -524,22 +524,18 @@ __global__ __launch_bounds__(256) void k_sell_jacobi_from_zero(
 }
 
 // k_sell_jacobi_from_zero for an x that vanishes on every row (Dirichlet
-// rows too): no Dirichlet-column products, the same expressions otherwise
+// rows too): no Dirichlet-column products, and x is not read at all (so the
+// caller need not clear it); y = w D^-1 b on every row, the expressions of
+// the sweep with x = 0 (the Dirichlet rows: x + w D^-1 (b - x) = w D^-1 b)
 template <typename XT>
-__global__ __launch_bounds__(256) void k_jacobi_from_zero_nodir(int n, int active, const XT* __restrict__ x,
-                                                                const XT* __restrict__ b,
+__global__ __launch_bounds__(256) void k_jacobi_from_zero_nodir(int n, const XT* __restrict__ b,
                                                                 const XT* __restrict__ dinv, XT* __restrict__ y,
                                                                 double a) {
   using AT = typename std::conditional<sizeof(XT) == 8, double, float>::type;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (i < active) {
-    const AT res = (AT)b[i];
-    y[i] = (XT)((AT)a * (AT)dinv[i] * res);
-  } else {
-    const AT res = (AT)(b[i] - x[i]);
-    y[i] = (XT)((AT)x[i] + (AT)a * (AT)dinv[i] * res);
-  }
+  const AT res = (AT)b[i];
+  y[i] = (XT)((AT)a * (AT)dinv[i] * res);
 }
 
 // gsum[r] = sum of the Dirichlet-column products of active row r, in column
@@ -756,6 +752,93 @@ __global__ __launch_bounds__(256) void k_sell_transfer(int n_slices, int sell_bl
     }
   } else if (zero_rest) {
     const int i = n_sell + (int)(blockIdx.x - sell_blocks) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (T)0;
+  }
+}
+
+// y = alpha P x + beta y on the rows of the pattern slices of P: one wave per
+// slice, the slice's column offsets and weights read once per wave (uniform
+// scalar loads), each lane gathers x[base + offset]; the row sum runs in
+// column order as in k_sell_transfer (the left-out Dirichlet columns meet
+// zeros there, which leave a sum unchanged).
+template <typename T>
+__device__ __forceinline__ void pat_transfer_block(int block, int n_slices, int sell_blocks,
+                                                   const int* __restrict__ rowmap, const int* __restrict__ base,
+                                                   const int* __restrict__ slice_pat, const int* __restrict__ pat_ptr,
+                                                   const int* __restrict__ pat_col, const double* __restrict__ pat_val,
+                                                   const T* __restrict__ x, T* __restrict__ y, double alpha,
+                                                   double beta) {
+  using AT = typename std::conditional<sizeof(T) == 8, double, float>::type;
+  const int per = sell_blocks >> 3;
+  const int vb = (block & 7) * per + (block >> 3);
+  const int lane = threadIdx.x & 63;
+  const int slice = __builtin_amdgcn_readfirstlane(vb * 4 + (int)(threadIdx.x >> 6));
+  if (slice >= n_slices) return;
+  const int r = rowmap[slice * 64 + lane];
+  const T* xb = x + base[slice * 64 + lane];
+  const int p = __builtin_amdgcn_readfirstlane(slice_pat[slice]);
+  const int j0 = __builtin_amdgcn_readfirstlane(pat_ptr[p]);
+  const int j1 = __builtin_amdgcn_readfirstlane(pat_ptr[p + 1]);
+  AT acc = 0;
+  int j = j0;
+  for (; j + 8 <= j1; j += 8) {
+    AT xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = (AT)xb[pat_col[j + u]];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += (AT)pat_val[j + u] * xv[u];
+  }
+  for (; j < j1; ++j) acc += (AT)pat_val[j] * (AT)xb[pat_col[j]];
+  if (r >= 0) {
+    AT res = (AT)alpha * acc;
+    if (beta != 0.0) res += (AT)beta * (AT)y[r];
+    y[r] = (T)res;
+  }
+}
+
+// One launch for a transfer with pattern slices: workgroups [0, pat_blocks)
+// run the pattern slices, the next ones the SELL copy of the other rows and
+// the zero tail (k_sell_transfer's layout of blocks, shifted).
+template <typename T, typename VT>
+__global__ __launch_bounds__(256) void k_pat_transfer(int pat_slices, int pat_blocks, const int* __restrict__ prow,
+                                                      const int* __restrict__ pbase, const int* __restrict__ slice_pat,
+                                                      const int* __restrict__ pat_ptr, const int* __restrict__ pat_col,
+                                                      const double* __restrict__ pat_val, int n_slices,
+                                                      int sell_blocks, int n, int n_sell,
+                                                      const int* __restrict__ sptr, const int* __restrict__ rowmap,
+                                                      const int* __restrict__ ci, const VT* __restrict__ va,
+                                                      const T* __restrict__ x, T* __restrict__ y, double alpha,
+                                                      double beta, int zero_rest) {
+  if ((int)blockIdx.x < pat_blocks) {
+    pat_transfer_block<T>((int)blockIdx.x, pat_slices, pat_blocks, prow, pbase, slice_pat, pat_ptr, pat_col, pat_val,
+                          x, y, alpha, beta);
+    return;
+  }
+  const int block = (int)blockIdx.x - pat_blocks;
+  using AT = typename std::conditional<sizeof(T) == 8, double, float>::type;
+  if (block < sell_blocks) {
+    const int per = sell_blocks >> 3;
+    const int vb = (block & 7) * per + (block >> 3);
+    const int lane = threadIdx.x & 63;
+    const int slice = vb * 4 + (int)(threadIdx.x >> 6);
+    if (slice >= n_slices) return;
+    const int r = rowmap[slice * 64 + lane];
+    const int s0 = sptr[slice];
+    const int width = (sptr[slice + 1] - s0) >> 6;
+    AT acc = 0;
+    for (int j = 0; j < width; ++j) {
+      AT v;
+      if constexpr (sizeof(VT) == 2) v = (AT)bf16_to_float(va[s0 + j * 64 + lane]);
+      else v = (AT)va[s0 + j * 64 + lane];
+      acc += v * (AT)x[ci[s0 + j * 64 + lane]];
+    }
+    if (r >= 0) {
+      AT res = (AT)alpha * acc;
+      if (beta != 0.0) res += (AT)beta * (AT)y[r];
+      y[r] = (T)res;
+    }
+  } else if (zero_rest) {
+    const int i = n_sell + (block - sell_blocks) * (int)blockDim.x + (int)threadIdx.x;
     if (i < n) y[i] = (T)0;
   }
 }
@@ -1121,8 +1204,8 @@ static bool jacobi_from_zero(const DevCSR& A, const VT* bd_val, const float* rsc
   if (A.n == 0) return true;
   if (A.sell.dir_zero) {
     // x vanishes everywhere: x' = w D^-1 b (the row sums are exactly zero)
-    hipLaunchKernelGGL((k_jacobi_from_zero_nodir<XT>), dim3((A.n + 255) / 256), dim3(256), 0, st, A.n, A.active, x,
-                       b, dinv, xn, omega);
+    hipLaunchKernelGGL((k_jacobi_from_zero_nodir<XT>), dim3((A.n + 255) / 256), dim3(256), 0, st, A.n, b, dinv, xn,
+                       omega);
     return true;
   }
   XT* g = nullptr;
@@ -1228,6 +1311,20 @@ void fe_jacobi_bf16(const DevCSR& A, const float* x, const float* b, const float
 template <typename T>
 void csr_spmv_t(const DevCSR& P, const T* x, T* y, double alpha, double beta, int row_end,
                 int zero_rest, hipStream_t st) {
+  if (P.pt.n_slices > 0 && P.sell.n_rows == row_end) {
+    // pattern slices, the SELL copy of the other rows and the zero tail in one launch
+    const int pb = ((P.pt.n_slices + 3) / 4 + 7) / 8 * 8;
+    const int sb = ((P.sell.n_slices + 3) / 4 + 7) / 8 * 8;
+    const int tail = zero_rest ? (P.n - row_end + 255) / 256 : 0;
+    auto go = [&](auto kern, const auto* va) {
+      hipLaunchKernelGGL(kern, dim3(pb + sb + tail), dim3(256), 0, st, P.pt.n_slices, pb, P.pt.row, P.pt.base,
+                         P.pt.slice_pat, P.pt.pat_ptr, P.pt.pat_col, P.pt.pat_val, P.sell.n_slices, sb, P.n, row_end,
+                         P.sell.slice_ptr, P.sell.row, P.sell.col, va, x, y, alpha, beta, zero_rest);
+    };
+    if (P.sell.val_bf16) go(k_pat_transfer<T, uint16_t>, (const uint16_t*)P.sell.val_bf16);
+    else go(k_pat_transfer<T, double>, (const double*)P.sell.val);
+    return;
+  }
   if (P.sell.n_slices > 0 && P.sell.n_rows == row_end) {
     const int sb = ((P.sell.n_slices + 3) / 4 + 7) / 8 * 8;
     const int tail = zero_rest ? (P.n - row_end + 255) / 256 : 0;
@@ -1438,6 +1535,157 @@ void vec_axpy_dot_reduced_f32(int n, const double* partial_in, double* h_out, co
   hipLaunchKernelGGL(k_axpy_dot_reduced<float>, dim3(kRedBlocks), dim3(256), 0, st, n, kRedBlocks, partial_in, h_out,
                      v, r, w, partial_out);
 }
+// ---------------- classical Gram-Schmidt over an FP32 basis (FP64 sums)
+namespace {
+constexpr int kMaxDots = 32;
+
+// partial[b kMaxDots + k] = sum over block b's entries of w_i V_k,i (k < NV).
+// Four consecutive entries per thread and step (16-byte loads; ld and the
+// vectors 16-byte aligned), the last n % 4 entries by block 0.
+typedef float v4f_cgs __attribute__((ext_vector_type(4)));
+template <int NV>
+__global__ __launch_bounds__(256) void k_multi_dot_f32(int n, const float* __restrict__ w,
+                                                       const float* __restrict__ V, size_t ld,
+                                                       double* __restrict__ partial) {
+  __shared__ double red[4][kMaxDots];
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  const int n4 = n >> 2;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) {
+    const v4f_cgs wi = ((const v4f_cgs*)w)[i];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const v4f_cgs v = ((const v4f_cgs*)(V + (size_t)k * ld))[i];
+      acc[k] += ((double)wi.x * (double)v.x + (double)wi.y * (double)v.y) +
+                ((double)wi.z * (double)v.z + (double)wi.w * (double)v.w);
+    }
+  }
+  if (blockIdx.x == 0 && (int)threadIdx.x < (n & 3)) {
+    const int i = 4 * n4 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] += (double)w[i] * (double)V[(size_t)k * ld + i];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double t = group_sum<64>(acc[k]);
+    if (lane == 0) red[wv][k] = t;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < NV)
+    partial[(size_t)blockIdx.x * kMaxDots + threadIdx.x] =
+        ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+}
+
+// out[k] = sum over the m blocks of partial[b kMaxDots + k], in a fixed order
+__global__ __launch_bounds__(1024) void k_multi_reduce(int m, int nv, const double* __restrict__ partial,
+                                                       double* __restrict__ out) {
+  __shared__ double red[32][kMaxDots + 1];
+  const int k = threadIdx.x & 31, g = threadIdx.x >> 5;
+  double s = 0.0;
+  if (k < nv)
+    for (int b = g; b < m; b += 32) s += partial[(size_t)b * kMaxDots + k];
+  red[g][k] = s;
+  __syncthreads();
+  if ((int)threadIdx.x < nv) {
+    double t = 0.0;
+    for (int q = 0; q < 32; ++q) t += red[q][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
+// w -= sum_k h_k V_k (in FP64, rounded once), partial[b] = block b's part of
+// |w|^2; four entries per thread and step as in k_multi_dot_f32
+template <int NV>
+__global__ __launch_bounds__(256) void k_multi_axpy_norm_f32(int n, const double* __restrict__ h,
+                                                             const float* __restrict__ V, size_t ld,
+                                                             float* __restrict__ w, double* __restrict__ partial) {
+  __shared__ double hs[kMaxDots];
+  __shared__ double red[16];
+  if ((int)threadIdx.x < NV) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  double s2 = 0.0;
+  const int n4 = n >> 2;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) {
+    const v4f_cgs wi = ((const v4f_cgs*)w)[i];
+    double a0 = wi.x, a1 = wi.y, a2 = wi.z, a3 = wi.w;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const v4f_cgs v = ((const v4f_cgs*)(V + (size_t)k * ld))[i];
+      const double hk = hs[k];
+      a0 -= hk * (double)v.x;
+      a1 -= hk * (double)v.y;
+      a2 -= hk * (double)v.z;
+      a3 -= hk * (double)v.w;
+    }
+    v4f_cgs o;
+    o.x = (float)a0; o.y = (float)a1; o.z = (float)a2; o.w = (float)a3;
+    ((v4f_cgs*)w)[i] = o;
+    s2 += ((double)o.x * (double)o.x + (double)o.y * (double)o.y) + ((double)o.z * (double)o.z + (double)o.w * (double)o.w);
+  }
+  if (blockIdx.x == 0 && (int)threadIdx.x < (n & 3)) {
+    const int i = 4 * n4 + threadIdx.x;
+    double a = (double)w[i];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) a -= hs[k] * (double)V[(size_t)k * ld + i];
+    const float wi = (float)a;
+    w[i] = wi;
+    s2 += (double)wi * (double)wi;
+  }
+  s2 = block_sum(s2, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s2;
+}
+}  // namespace
+
+namespace {
+// host_dst[0..cnt) = src, then *host_flag = seq with system-scope release, so
+// a host thread that sees the flag (acquire) sees the values
+__global__ __launch_bounds__(64) void k_publish_to_host(const double* __restrict__ src, int cnt,
+                                                        double* host_dst, unsigned int* host_flag, unsigned int seq) {
+  for (int i = threadIdx.x; i < cnt; i += blockDim.x) host_dst[i] = src[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence_system();
+    __hip_atomic_store(host_flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+}  // namespace
+
+void publish_to_host(const double* src, int cnt, double* host_dst, unsigned int* host_flag, unsigned int seq,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_publish_to_host, dim3(1), dim3(64), 0, st, src, cnt, host_dst, host_flag, seq);
+}
+
+int vec_multi_dot_max() { return kMaxDots; }
+int vec_multi_partials() { return kRedBlocks * kMaxDots; }
+namespace {
+template <int NV>
+void cgs_nv(int n, const float* V, size_t ld, float* w, double* h_out, double* mpartial, double* partial,
+            hipStream_t st) {
+  hipLaunchKernelGGL(k_multi_dot_f32<NV>, dim3(kRedBlocks), dim3(256), 0, st, n, w, V, ld, mpartial);
+  hipLaunchKernelGGL(k_multi_reduce, dim3(1), dim3(1024), 0, st, kRedBlocks, NV, mpartial, h_out);
+  hipLaunchKernelGGL(k_multi_axpy_norm_f32<NV>, dim3(kRedBlocks), dim3(256), 0, st, n, h_out, V, ld, w, partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, st, kRedBlocks, partial, h_out + NV, 1);
+}
+template <int NV>
+void cgs_dispatch(int nv, int n, const float* V, size_t ld, float* w, double* h_out, double* mpartial,
+                  double* partial, hipStream_t st) {
+  if constexpr (NV <= kMaxDots) {
+    if (nv == NV) cgs_nv<NV>(n, V, ld, w, h_out, mpartial, partial, st);
+    else cgs_dispatch<NV + 1>(nv, n, V, ld, w, h_out, mpartial, partial, st);
+  }
+}
+}  // namespace
+
+void vec_cgs_f32(int n, int nv, const float* V, size_t ld, float* w, double* h_out, double* mpartial,
+                 double* partial, hipStream_t st) {
+  if (nv < 1 || nv > kMaxDots) throw std::runtime_error("classical Gram-Schmidt: too many basis vectors");
+  if (ld % 4 != 0 || ((uintptr_t)V | (uintptr_t)w) % 16 != 0)
+    throw std::runtime_error("classical Gram-Schmidt: basis vectors must be 16-byte aligned");
+  cgs_dispatch<1>(nv, n, V, ld, w, h_out, mpartial, partial, st);
+}
+
 void vec_scale_to_f32(int n, const double* x, const double* a_dev, float* y, hipStream_t st) {
   hipLaunchKernelGGL(k_scale_to_f32, dim3(grid_for(n)), dim3(256), 0, st, n, x, a_dev, y);
 }
@@ -2101,25 +2349,25 @@ bool dense_invert(int n, double* A, hipStream_t st) {
 // dense row-major y = A x (one 64-lane wavefront per row), x/y FP64 or FP32
 namespace pmx {
 namespace {
-template <typename T>
-__global__ __launch_bounds__(256) void k_dense_mv(int n, const double* __restrict__ A,
+template <typename T, typename AT = double>
+__global__ __launch_bounds__(256) void k_dense_mv(int n, const AT* __restrict__ A,
                                                   const T* __restrict__ x, T* __restrict__ y) {
   const int lane = threadIdx.x & 63;
   const int row = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (row >= n) return;
-  const double* a = A + (size_t)row * n;
+  const AT* a = A + (size_t)row * n;
   // four independent partial sums keep four row loads in flight per lane
   double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   int j = lane;
   for (; j + 192 < n; j += 256) {
     const double a0 = __builtin_nontemporal_load(a + j), a1 = __builtin_nontemporal_load(a + j + 64);
     const double a2 = __builtin_nontemporal_load(a + j + 128), a3 = __builtin_nontemporal_load(a + j + 192);
-    s0 += a0 * (double)x[j];
-    s1 += a1 * (double)x[j + 64];
-    s2 += a2 * (double)x[j + 128];
-    s3 += a3 * (double)x[j + 192];
+    s0 += (double)a0 * (double)x[j];
+    s1 += (double)a1 * (double)x[j + 64];
+    s2 += (double)a2 * (double)x[j + 128];
+    s3 += (double)a3 * (double)x[j + 192];
   }
-  for (; j < n; j += 64) s0 += a[j] * (double)x[j];
+  for (; j < n; j += 64) s0 += (double)a[j] * (double)x[j];
   double s = group_sum<64>((s0 + s1) + (s2 + s3));
   if (lane == 0) y[row] = (T)s;
 }
@@ -2130,5 +2378,8 @@ void dense_mv(int n, const double* A, const double* x, double* y, hipStream_t st
 }
 void dense_mv_f32(int n, const double* A, const float* x, float* y, hipStream_t st) {
   hipLaunchKernelGGL(k_dense_mv<float>, dim3((n + 3) / 4), dim3(256), 0, st, n, A, x, y);
+}
+void dense_mv_f32a(int n, const float* A, const float* x, float* y, hipStream_t st) {
+  hipLaunchKernelGGL((k_dense_mv<float, float>), dim3((n + 3) / 4), dim3(256), 0, st, n, A, x, y);
 }
 }  // namespace pmx

@@ -53,6 +53,21 @@ struct DevSELL {
 };
 
 // Device CSR matrix (0-based int32 row pointer / columns, sorted columns).
+// Pattern slices of a grid-transfer matrix (the scalar multigrid's P or
+// P^T on lexicographically renumbered levels): the rows of a slice share
+// their column offsets from the row's first column and their weights, so a
+// wave reads each (offset, weight) once and every lane only its first column.
+// Columns of Dirichlet DOFs are left out (the transfers' inputs vanish there).
+struct DevPatTransfer {
+  int n_slices = 0, n_patterns = 0;
+  int* row = nullptr;         // [n_slices * 64] output row, -1 for an empty lane
+  int* base = nullptr;        // [n_slices * 64] first column of the row (a valid one for empty lanes)
+  int* slice_pat = nullptr;   // [n_slices] pattern of the slice
+  int* pat_ptr = nullptr;     // [n_patterns + 1] into pat_col / pat_val
+  int* pat_col = nullptr;     // column offsets from base, ascending
+  double* pat_val = nullptr;  // weights
+};
+
 struct DevCSR {
   int n = 0;            // rows
   int m = 0;            // columns
@@ -65,6 +80,8 @@ struct DevCSR {
   uint16_t* val_bf16 = nullptr;   // mixed precision copy (bf16, row-scaled)
   float* rscale = nullptr;        // per-row scale of the bf16 copy
   DevSELL sell;                   // optional SELL copy used by the FE-matrix kernels
+  DevPatTransfer pt;              // optional pattern slices of a transfer matrix (the rows
+                                  // not in pattern slices are in sell)
   // fe_apply_scaled_add3: matrices with fewer rows use three CSR launches
   // with 8 lanes per row instead of the one-lane-per-row SELL pass
   int sell3_min_rows = 131072;
@@ -189,11 +206,24 @@ void vec_f32_to_f64(int n, const float* x, double* y, hipStream_t st);
 // ---- dense in-place inverse (n x n row-major): blocked Gauss-Jordan with
 //      partial pivoting; synchronises st; false when a pivot is zero
 bool dense_invert(int n, double* A, hipStream_t st);
+// copy cnt device doubles to host-mapped pinned memory, then set *host_flag =
+// seq (the host waits for the flag instead of a stream synchronisation)
+void publish_to_host(const double* src, int cnt, double* host_dst, unsigned int* host_flag, unsigned int seq,
+                     hipStream_t st);
+// classical Gram-Schmidt of w against the nv FP32 basis vectors V_k = V + k ld:
+// h_out[k] = <w, V_k> (FP64 sums, fixed order), w -= sum h_k V_k,
+// h_out[nv] = |w|.  mpartial: vec_multi_partials() doubles, partial:
+// vec_partials() doubles; nv <= vec_multi_dot_max()
+int vec_multi_dot_max();
+int vec_multi_partials();
+void vec_cgs_f32(int n, int nv, const float* V, size_t ld, float* w, double* h_out, double* mpartial,
+                 double* partial, hipStream_t st);
 // D (n x n row-major, n <= 20000) = the CSR matrix A
 void csr_to_dense(const DevCSR& A, double* D, hipStream_t st);
 // y = A x for a dense row-major n x n matrix
 void dense_mv(int n, const double* A, const double* x, double* y, hipStream_t st);
 void dense_mv_f32(int n, const double* A, const float* x, float* y, hipStream_t st);
+void dense_mv_f32a(int n, const float* A, const float* x, float* y, hipStream_t st);
 
 // ---- assembly
 struct AssemblyData {

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -55,6 +56,9 @@ void free_csr(pmx::DevCSR& M) {
   dfree(M.row_ptr); dfree(M.col); dfree(M.val); dfree(M.val_bf16); dfree(M.rscale);
   dfree(M.sell.slice_ptr); dfree(M.sell.row); dfree(M.sell.col); dfree(M.sell.val); dfree(M.sell.val_bf16);
   M.sell = pmx::DevSELL{};
+  dfree(M.pt.row); dfree(M.pt.base); dfree(M.pt.slice_pat); dfree(M.pt.pat_ptr); dfree(M.pt.pat_col);
+  dfree(M.pt.pat_val);
+  M.pt = pmx::DevPatTransfer{};
 }
 
 pmx::DevCSR upload_csr(const pmx::CSRPattern& p, const std::vector<double>& v, hipStream_t st) {
@@ -90,22 +94,27 @@ pmx::DevCSR upload_csr(const pmx::CSRPattern& p, const std::vector<double>& v, h
 // stable) within windows of 4096, each row's entries in CSR column order,
 // padding slots (column 0, value 0) up to the slice's longest row; the bf16
 // copy exists when the CSR upload found every weight to be a bf16 number
+// (only the rows listed in `rows`, when given: the rest of a pattern transfer)
 void attach_transfer_sell(pmx::DevCSR& M, const pmx::CSRPattern& p, const std::vector<double>& v, int row_end,
-                          hipStream_t st) {
+                          hipStream_t st, const std::vector<int>* rows = nullptr) {
   constexpr int C = 64, sigma = 4096;
   require(row_end >= 0 && row_end <= p.n_rows, "transfer rows out of range");
-  if (row_end == 0) return;
+  M.sell.n_rows = row_end;
   auto len = [&](int r) { return p.row_ptr[r + 1] - p.row_ptr[r]; };
-  std::vector<int> order(row_end);
-  for (int r = 0; r < row_end; ++r) order[r] = r;
-  for (int w = 0; w < row_end; w += sigma)
-    std::stable_sort(order.begin() + w, order.begin() + std::min(row_end, w + sigma),
+  std::vector<int> order;
+  if (rows) order = *rows;
+  else
+    for (int r = 0; r < row_end; ++r) order.push_back(r);
+  const int nr = (int)order.size();
+  if (nr == 0) return;
+  for (int w = 0; w < nr; w += sigma)
+    std::stable_sort(order.begin() + w, order.begin() + std::min(nr, w + sigma),
                      [&](int a, int b) { return len(a) > len(b); });
-  const int n_slices = (row_end + C - 1) / C;
+  const int n_slices = (nr + C - 1) / C;
   std::vector<int> rowmap((size_t)n_slices * C, -1), sptr(n_slices + 1, 0);
   for (int sl = 0; sl < n_slices; ++sl) {
     int width = 0;
-    for (int l = 0; l < C && sl * C + l < row_end; ++l) {
+    for (int l = 0; l < C && sl * C + l < nr; ++l) {
       rowmap[(size_t)sl * C + l] = order[sl * C + l];
       width = std::max(width, len(order[sl * C + l]));
     }
@@ -129,7 +138,6 @@ void attach_transfer_sell(pmx::DevCSR& M, const pmx::CSRPattern& p, const std::v
   }
   M.sell.n_slices = n_slices;
   M.sell.total = total;
-  M.sell.n_rows = row_end;
   M.sell.slice_ptr = upload(sptr.data(), sptr.size(), st);
   M.sell.row = upload(rowmap.data(), rowmap.size(), st);
   M.sell.col = upload(col.data(), col.size(), st);
@@ -144,6 +152,88 @@ void attach_transfer_sell(pmx::DevCSR& M, const pmx::CSRPattern& p, const std::v
     }
     M.sell.val_bf16 = upload(h.data(), h.size(), st);
   }
+}
+
+// Pattern slices of a grid-transfer matrix over its rows [0, row_end) with
+// the columns >= col_end (Dirichlet DOFs: the transfers' inputs are zero
+// there) left out.  Rows whose remaining (column - first column, weight)
+// sequences agree share a pattern; patterns of at least 64 rows are cut into
+// slices of 64 rows (ascending), the rows of rarer patterns go to the SELL
+// copy (attach_transfer_sell with all their columns).  On lexicographically
+// numbered levels of a structured hierarchy a few hundred patterns (fine node
+// type x nearness to the boundary) cover nearly all rows.
+void attach_pattern_transfer(pmx::DevCSR& M, const pmx::CSRPattern& p, const std::vector<double>& v, int row_end,
+                             int col_end, hipStream_t st) {
+  constexpr int C = 64;
+  require(row_end >= 0 && row_end <= p.n_rows, "transfer rows out of range");
+  std::map<std::pair<std::vector<int>, std::vector<double>>, int> ids;
+  std::vector<int> pid(row_end), base(row_end, 0), count;
+  std::vector<std::vector<int>> pat_cols;
+  std::vector<std::vector<double>> pat_vals;
+  for (int r = 0; r < row_end; ++r) {
+    std::vector<int> off;
+    std::vector<double> w;
+    int c0 = -1;
+    for (int k = p.row_ptr[r]; k < p.row_ptr[r + 1]; ++k) {
+      const int c = p.col[k];
+      if (c >= col_end) continue;
+      if (c0 < 0) c0 = c;
+      off.push_back(c - c0);
+      w.push_back(v[k]);
+    }
+    base[r] = std::max(c0, 0);
+    auto key = std::make_pair(off, w);
+    auto it = ids.find(key);
+    if (it == ids.end()) {
+      it = ids.emplace(std::move(key), (int)pat_cols.size()).first;
+      pat_cols.push_back(off);
+      pat_vals.push_back(w);
+      count.push_back(0);
+    }
+    pid[r] = it->second;
+    ++count[it->second];
+  }
+  // kept patterns, renumbered in order of first appearance
+  std::vector<int> keep_id(pat_cols.size(), -1);
+  std::vector<int> pat_ptr(1, 0), pcol;
+  std::vector<double> pval;
+  int n_kept = 0;
+  for (size_t q = 0; q < pat_cols.size(); ++q)
+    if (count[q] >= C) {
+      keep_id[q] = n_kept++;
+      pcol.insert(pcol.end(), pat_cols[q].begin(), pat_cols[q].end());
+      pval.insert(pval.end(), pat_vals[q].begin(), pat_vals[q].end());
+      pat_ptr.push_back((int)pcol.size());
+    }
+  std::vector<std::vector<int>> rows_of(n_kept);
+  std::vector<int> rest;
+  for (int r = 0; r < row_end; ++r) {
+    if (keep_id[pid[r]] >= 0) rows_of[keep_id[pid[r]]].push_back(r);
+    else rest.push_back(r);
+  }
+  std::vector<int> rowmap, bmap, spat;
+  for (int q = 0; q < n_kept; ++q)
+    for (size_t i0 = 0; i0 < rows_of[q].size(); i0 += C) {
+      const int first = rows_of[q][i0];
+      for (int l = 0; l < C; ++l) {
+        const size_t i = i0 + l;
+        rowmap.push_back(i < rows_of[q].size() ? rows_of[q][i] : -1);
+        bmap.push_back(i < rows_of[q].size() ? base[rows_of[q][i]] : base[first]);
+      }
+      spat.push_back(q);
+    }
+  if (pcol.empty()) pcol.push_back(0), pval.push_back(0.0);   // keep the arrays non-empty
+  M.pt.n_slices = (int)spat.size();
+  M.pt.n_patterns = n_kept;
+  if (M.pt.n_slices > 0) {
+    M.pt.row = upload(rowmap.data(), rowmap.size(), st);
+    M.pt.base = upload(bmap.data(), bmap.size(), st);
+    M.pt.slice_pat = upload(spat.data(), spat.size(), st);
+    M.pt.pat_ptr = upload(pat_ptr.data(), pat_ptr.size(), st);
+    M.pt.pat_col = upload(pcol.data(), pcol.size(), st);
+    M.pt.pat_val = upload(pval.data(), pval.size(), st);
+  }
+  attach_transfer_sell(M, p, v, row_end, st, &rest);
 }
 
 // one level of the saddle-point multigrid: the system (not owned), the
@@ -184,6 +274,7 @@ struct Level {
   pmx::DevCSR P;             // prolongation to the next finer level (rows: finer DOFs)
   pmx::DevCSR R;             // restriction from the next finer level (rows: this level)
   double* dinv_dense = nullptr;   // dense inverse (direct coarse solve), row-major n x n
+  float* dinv_dense32 = nullptr;  // its FP32 copy (mixed precision cycle)
   double *rhs = nullptr, *sol = nullptr, *def = nullptr, *tmp = nullptr;
   float *rhs32 = nullptr, *sol32 = nullptr, *def32 = nullptr, *tmp32 = nullptr;
   double* partial = nullptr;
@@ -206,7 +297,7 @@ struct Level {
     }
     dfree(perm); dfree(bd64); dfree(bd16); dfree(g64); dfree(g32);
     dfree(dinv); dfree(dinv32);
-    free_csr(P); free_csr(R); dfree(dinv_dense);
+    free_csr(P); free_csr(R); dfree(dinv_dense); dfree(dinv_dense32);
     dfree(rhs); dfree(sol); dfree(def); dfree(tmp);
     dfree(rhs32); dfree(sol32); dfree(def32); dfree(tmp32);
     dfree(partial);
@@ -251,6 +342,19 @@ struct pmx_solver {
   hipGraph_t cycle_graph[2] = {nullptr, nullptr};
   hipGraphExec_t cycle_exec[2] = {nullptr, nullptr};
   float *V32 = nullptr, *Z32 = nullptr, *w32 = nullptr, *kval32 = nullptr, *zero32 = nullptr;
+  size_t ld32 = 0;   // stride of the FP32 basis vectors (n rounded up to 64 entries)
+  double* mdot = nullptr;   // block partials of the classical Gram-Schmidt dots (refinement)
+  // refinement: one captured cycle per Krylov slot i, reading v_i and
+  // writing z_i in place (no copies in and out of the level vectors)
+  std::vector<hipGraph_t> ir_graph;
+  std::vector<hipGraphExec_t> ir_exec;
+  // pinned host memory the device writes each Hessenberg column to, with a
+  // sequence flag the host waits for (instead of a copy + stream sync)
+  double* hpin = nullptr;
+  unsigned int* hflag = nullptr;
+  double* hpin_dev = nullptr;
+  unsigned int* hflag_dev = nullptr;
+  unsigned int hseq = 0;
   int ir_outer = 0;              // refinement steps of the last solve
   const pmx::DevCSR* op = nullptr;       // operator of the running solve
   bool internal = false;                 // the running solve uses that numbering
@@ -275,7 +379,7 @@ struct pmx_solver {
     V.clear(); Z.clear();
     dfree(r); dfree(w1); dfree(w2); dfree(w3); dfree(dinv); dfree(d_H); dfree(d_scal); dfree(partial);
     dfree(kval); dfree(kbd); dfree(kb); dfree(kx);
-    dfree(V32); dfree(Z32); dfree(w32); dfree(kval32); dfree(zero32);
+    dfree(V32); dfree(Z32); dfree(w32); dfree(kval32); dfree(zero32); dfree(mdot);
     ir = false;
     dfree(kg64);
     kfold = false;
@@ -285,6 +389,17 @@ struct pmx_solver {
       cycle_exec[k] = nullptr;
       cycle_graph[k] = nullptr;
     }
+    for (auto& e : ir_exec)
+      if (e) (void)hipGraphExecDestroy(e);
+    for (auto& g : ir_graph)
+      if (g) (void)hipGraphDestroy(g);
+    ir_exec.clear();
+    ir_graph.clear();
+    if (hpin) (void)hipHostFree(hpin);
+    hpin = nullptr;
+    hflag = nullptr;
+    hpin_dev = nullptr;
+    hflag_dev = nullptr;
     if (kown) {
       dfree(K.sell.slice_ptr); dfree(K.sell.row); dfree(K.sell.col); dfree(K.sell.src);
       dfree(K.sell.bd_ptr); dfree(K.sell.bd_src); dfree(K.sell.bd_col);
@@ -344,7 +459,8 @@ void mg_smooth(pmx_solver* s, int l, bool mixed) {
   Level& L = s->lv[l];
   if (l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) {
     L.sol_zero = false;
-    if (mixed) pmx::dense_mv_f32(L.n, L.dinv_dense, L.rhs32, L.sol32, S(s));
+    if (mixed && L.dinv_dense32) pmx::dense_mv_f32a(L.n, L.dinv_dense32, L.rhs32, L.sol32, S(s));
+    else if (mixed) pmx::dense_mv_f32(L.n, L.dinv_dense, L.rhs32, L.sol32, S(s));
     else pmx::dense_mv(L.n, L.dinv_dense, L.rhs, L.sol, S(s));
     return;
   }
@@ -393,6 +509,20 @@ void mg_update_rhs_in_coarser_grid(pmx_solver* s, int l, bool mixed) {
   else pmx::csr_spmv(C.R, F.def, C.rhs, 1.0, 0.0, C.active, 1, S(s));
 }
 
+// The level's solution need not be cleared when the first operation after the
+// reset overwrites all of it without reading it: the direct coarse solve, or
+// a first smoothing step from zero on a level whose Dirichlet part is zero as
+// well (dir_zero; k_jacobi_from_zero_nodir, the conditions of
+// fe_jacobi_from_zero).
+bool reset_overwritten(const pmx_solver* s, int l, bool mixed) {
+  const Level& L = s->lv[l];
+  if (l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) return true;
+  if (!s->p.zero_start_shortcut || s->p.mg_n_pre_smooth < 1 || !L.A.sell.dir_zero) return false;
+  if (L.A.lpr != 0 || !L.A.sell.bd_ptr) return false;
+  const bool vals = mixed ? L.A.sell.bd_val_bf16 != nullptr : L.A.sell.bd_val != nullptr;
+  return vals || L.A.sell.n_bd == 0;
+}
+
 int mg_cycle_step(pmx_solver* s, int step, int level, bool mixed) {
   const int nl = (int)s->lv.size();
   if (level == 0) {
@@ -422,8 +552,10 @@ int mg_cycle_step(pmx_solver* s, int step, int level, bool mixed) {
     for (int i = 0; i < s->p.mg_n_pre_smooth; ++i) mg_smooth(s, level, mixed);
     mg_defect(s, level, mixed, nullptr);
     Level& C = s->lv[level - 1];
-    if (mixed) pmx::vec_fill_f32(C.n, 0.f, C.sol32, S(s));
-    else pmx::vec_fill(C.n, 0.0, C.sol, S(s));
+    if (!reset_overwritten(s, level - 1, mixed)) {
+      if (mixed) pmx::vec_fill_f32(C.n, 0.f, C.sol32, S(s));
+      else pmx::vec_fill(C.n, 0.0, C.sol, S(s));
+    }
     C.sol_zero = true;
     C.g_valid = false;
     mg_update_rhs_in_coarser_grid(s, level, mixed);
@@ -438,7 +570,10 @@ int mg_cycle_step(pmx_solver* s, int step, int level, bool mixed) {
 // Multigrid::cycle: finest solution = 0 with the rhs' non-active entries
 void mg_cycle(pmx_solver* s, bool mixed) {
   Level& F = s->lv.back();
-  if (mixed) {
+  if (s->lv.size() > 1 && reset_overwritten(s, (int)s->lv.size() - 1, mixed)) {
+    // the rhs vanishes on the Dirichlet rows (dir_zero), so the reset
+    // solution is zero everywhere and the first sweep does not read it
+  } else if (mixed) {
     pmx::vec_fill_f32(F.active, 0.f, F.sol32, S(s));
     if (F.n > F.active)
       HIP_CHECK(hipMemcpyAsync(F.sol32 + F.active, F.rhs32 + F.active, (F.n - F.active) * sizeof(float),
@@ -483,6 +618,67 @@ void mg_cycle_run(pmx_solver* s, bool mixed) {
     HIP_CHECK(hipGraphInstantiate(&s->cycle_exec[key], s->cycle_graph[key], nullptr, nullptr, 0));
   }
   HIP_CHECK(hipGraphLaunch(s->cycle_exec[key], st));
+}
+
+// The refinement's cycle for Krylov slot i: z_i = MG(v_i) with the finest
+// level's right-hand side and solution pointed at v_i and z_i while the cycle
+// is captured (the graph keeps those addresses; a final copy puts the result
+// into z_i if the smoothing steps left it in the other buffer).  Without
+// graphs the level vectors are copied in and out.
+void mg_cycle_slot(pmx_solver* s, int i, const float* vi, float* zi) {
+  Level& F = s->lv.back();
+  hipStream_t st = S(s);
+  if (!s->p.graph_cycle || s->p.mg_smoother_coarse != PMX_SMOOTHER_DIRECT) {
+    HIP_CHECK(hipMemcpyAsync(F.rhs32, vi, (size_t)F.n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    mg_cycle_run(s, true);
+    HIP_CHECK(hipMemcpyAsync(zi, F.sol32, (size_t)F.n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    return;
+  }
+  if ((int)s->ir_exec.size() <= i) {
+    s->ir_exec.resize(i + 1, nullptr);
+    s->ir_graph.resize(i + 1, nullptr);
+  }
+  if (!s->ir_exec[i]) {
+    float* const rhs = F.rhs32;
+    float* const sol = F.sol32;
+    float* const tmp = F.tmp32;
+    F.rhs32 = const_cast<float*>(vi);
+    F.sol32 = zi;
+    F.tmp32 = tmp;
+    HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      mg_cycle(s, true);
+      if (F.sol32 != zi)
+        HIP_CHECK(hipMemcpyAsync(zi, F.sol32, (size_t)F.n * sizeof(float), hipMemcpyDeviceToDevice, st));
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(st, &g);
+      if (g) (void)hipGraphDestroy(g);
+      F.rhs32 = rhs; F.sol32 = sol; F.tmp32 = tmp;
+      throw;
+    }
+    F.rhs32 = rhs; F.sol32 = sol; F.tmp32 = tmp;
+    HIP_CHECK(hipStreamEndCapture(st, &s->ir_graph[i]));
+    HIP_CHECK(hipGraphInstantiate(&s->ir_exec[i], s->ir_graph[i], nullptr, nullptr, 0));
+  }
+  HIP_CHECK(hipGraphLaunch(s->ir_exec[i], st));
+}
+
+// Wait until the device has published sequence number seq (bounded: the
+// stream is queried now and then, and a stream that finished or failed
+// without the flag ends the wait with an error).
+void wait_published(pmx_solver* s, unsigned int seq) {
+  for (long it = 1;; ++it) {
+    if (__atomic_load_n(s->hflag, __ATOMIC_ACQUIRE) == seq) return;
+    if ((it & 0xFFFF) == 0) {
+      const hipError_t e = hipStreamQuery(S(s));
+      if (e == hipSuccess) {
+        if (__atomic_load_n(s->hflag, __ATOMIC_ACQUIRE) == seq) return;
+        throw PmxError(PMX_E_INTERNAL, "published Hessenberg column not visible after the stream finished");
+      }
+      if (e != hipErrorNotReady) HIP_CHECK(e);
+    }
+  }
 }
 
 // Iteration_multigrid::apply: z = MG(v)
@@ -665,6 +861,12 @@ void build_coarse_inverse(pmx_solver* s, const pmx_matrix* M) {
   const bool ok = pmx::dense_invert(n, L.dinv_dense, S(s));
   HIP_CHECK(hipGetLastError());
   if (!ok) throw PmxError(PMX_E_ZERO_DIAG, "singular coarse matrix in the direct coarse solve");
+  if (s->p.precision == PMX_PRECISION_MIXED) {
+    // the FP32 cycle reads an FP32 copy: half the bytes of its coarse solve
+    if (!L.dinv_dense32) L.dinv_dense32 = dmalloc<float>((size_t)n * n);
+    pmx::vec_f64_to_f32((int)((size_t)n * n), L.dinv_dense, L.dinv_dense32, S(s));
+    HIP_CHECK(hipGetLastError());
+  }
 }
 
 template <typename T>
@@ -844,34 +1046,50 @@ void solve_ir_run(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
   while (true) {
     ++s->ir_outer;
     pmx::vec_scale_to_f32(n, s->r, s->d_scal, s->V32, st);
+    const size_t ldv = s->ld32;   // stride of the FP32 basis (ld is the Hessenberg's)
     std::fill(sv.begin(), sv.end(), 0.0);
     sv[0] = beta;
     const double inner_target = std::max(s->p.refinement_inner_reduction * beta, 0.5 * target);
     int i = 0, used = 0;
     for (i = 0; i < m && j < s->p.max_n_iterations; ++i) {
-      float* vi = s->V32 + (size_t)i * n;
-      float* zi = s->Z32 + (size_t)i * n;
-      HIP_CHECK(hipMemcpyAsync(F.rhs32, vi, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
-      mg_cycle_run(s, true);
-      HIP_CHECK(hipMemcpyAsync(zi, F.sol32, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
+      float* vi = s->V32 + (size_t)i * ldv;
+      float* zi = s->Z32 + (size_t)i * ldv;
+      mg_cycle_slot(s, i, vi, zi);
       if (s->kval32) pmx::fe_apply_active_f32(F.A, s->kval32, zi, s->w32, s->zero32, st);
       else pmx::fe_apply_active_bf16(F.A, zi, s->w32, s->zero32, st);
       double* Hcol = s->d_H + (size_t)i * ld;
-      double* pa = s->partial;
-      double* pb = s->partial + pmx::vec_partials();
-      pmx::vec_dot_partial_f32(n, s->w32, s->V32, pa, st);
-      for (int k = 0; k < i; ++k) {
-        pmx::vec_axpy_dot_reduced_f32(n, pa, Hcol + k, s->V32 + (size_t)k * n, s->w32, s->V32 + (size_t)(k + 1) * n,
-                                      pb, st);
-        std::swap(pa, pb);
+      if (s->mdot) {
+        // classical Gram-Schmidt: all dots in one pass over the basis, the
+        // update and the norm in a second (the inner solve only has to
+        // reach refinement_inner_reduction; the FP64 outer residual sets the
+        // accuracy)
+        pmx::vec_cgs_f32(n, i + 1, s->V32, ldv, s->w32, Hcol, s->mdot, s->partial, st);
+      } else {
+        double* pa = s->partial;
+        double* pb = s->partial + pmx::vec_partials();
+        pmx::vec_dot_partial_f32(n, s->w32, s->V32, pa, st);
+        for (int k = 0; k < i; ++k) {
+          pmx::vec_axpy_dot_reduced_f32(n, pa, Hcol + k, s->V32 + (size_t)k * ldv, s->w32,
+                                        s->V32 + (size_t)(k + 1) * ldv, pb, st);
+          std::swap(pa, pb);
+        }
+        pmx::vec_axpy_dot_reduced_f32(n, pa, Hcol + i, vi, s->w32, nullptr, pb, st);
+        pmx::vec_reduce(pb, Hcol + i + 1, 1, st);
       }
-      pmx::vec_axpy_dot_reduced_f32(n, pa, Hcol + i, vi, s->w32, nullptr, pb, st);
-      pmx::vec_reduce(pb, Hcol + i + 1, 1, st);
-      HIP_CHECK(hipMemcpyAsync(&H[(size_t)i * ld], Hcol, (i + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
-      HIP_CHECK(hipStreamSynchronize(st));
+      // v_{i+1} = w / h_{i+1,i} needs no host value: queued before the wait
+      pmx::vec_scale_f32(n, 0.0, Hcol + i + 1, 1, s->w32, s->V32 + (size_t)(i + 1) * ldv, st);
+      if (s->hpin) {
+        const unsigned int seq = ++s->hseq;
+        pmx::publish_to_host(Hcol, i + 2, s->hpin_dev, s->hflag_dev, seq, st);
+        HIP_CHECK(hipGetLastError());
+        wait_published(s, seq);
+        std::copy(s->hpin, s->hpin + i + 2, H.begin() + (size_t)i * ld);
+      } else {
+        HIP_CHECK(hipMemcpyAsync(&H[(size_t)i * ld], Hcol, (i + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(hipStreamSynchronize(st));
+      }
       const double hn = H[(i + 1) + (size_t)i * ld];
       if (hn == 0.0) throw PmxError(PMX_E_BREAKDOWN, "Unhappy breakdown in flexible gmres");
-      pmx::vec_scale_f32(n, 0.0, Hcol + i + 1, 1, s->w32, s->V32 + (size_t)(i + 1) * n, st);
       for (int k = 0; k < i; ++k) apply_rotation(H[k + (size_t)i * ld], H[k + 1 + (size_t)i * ld], cs[k], sn[k]);
       generate_rotation(H[i + (size_t)i * ld], H[i + 1 + (size_t)i * ld], cs[i], sn[i]);
       apply_rotation(H[i + (size_t)i * ld], H[i + 1 + (size_t)i * ld], cs[i], sn[i]);
@@ -889,7 +1107,7 @@ void solve_ir_run(pmx_solver* s, const double* b, double* x, pmx_solve_report* r
       y[q] /= H[q + (size_t)q * ld];
       for (int k = q - 1; k >= 0; --k) y[k] -= H[k + (size_t)q * ld] * y[q];
     }
-    pmx::vec_multi_axpy_f32(n, used, y.data(), s->Z32, (size_t)n, x, st);
+    pmx::vec_multi_axpy_f32(n, used, y.data(), s->Z32, ldv, x, st);
     op_defect(s, x, b, s->r);
     beta = read_scalar(s, s->d_scal);
     if (converged(s, beta, j) || j >= s->p.max_n_iterations) {
@@ -1094,7 +1312,7 @@ int pmx_solver_default_params(pmx_solver_params* p) {
     d.vanka_damp = 1.0;
     d.zero_start_shortcut = 1;
     d.renumber_levels = 5;
-    d.transfer_sell = 1;
+    d.transfer_sell = 3;
     d.vanka_inverse = 2;
     d.refinement = 1;
     d.refinement_inner_reduction = 1e-4;
@@ -1325,9 +1543,18 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
         // SELL pays off where there are enough rows for the lane-per-row
         // layout to fill the GPU; long restriction rows on small levels run
         // faster with several lanes per row
-        auto use_sell = [&](int rows) { return s->p.transfer_sell >= 2 || (s->p.transfer_sell == 1 && rows >= 131072); };
-        if (use_sell(s->lv[l + 1].active)) attach_transfer_sell(L.P, Pp, ppv, s->lv[l + 1].active, st);
-        if (use_sell(L.active)) attach_transfer_sell(L.R, Rp, rpv, L.active, st);
+        const int ts = s->p.transfer_sell;
+        auto use_sell = [&](int rows) { return ts == 2 || ts == 4 || ((ts == 1 || ts == 3) && rows >= 131072); };
+        // on renumbered levels most rows share a few patterns (transfer_sell 3, 4)
+        const bool pat = (ts == 3 || ts == 4) && !perm[l + 1].empty();
+        if (use_sell(s->lv[l + 1].active)) {
+          if (pat) attach_pattern_transfer(L.P, Pp, ppv, s->lv[l + 1].active, L.active, st);
+          else attach_transfer_sell(L.P, Pp, ppv, s->lv[l + 1].active, st);
+        }
+        if (use_sell(L.active)) {
+          if (pat && !perm[l].empty()) attach_pattern_transfer(L.R, Rp, rpv, L.active, s->lv[l + 1].active, st);
+          else attach_transfer_sell(L.R, Rp, rpv, L.active, st);
+        }
       }
       if (s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT) build_coarse_inverse(s, levels[0]);
       s->cc = cycle_control(s->p.mg_cycle, n_levels);
@@ -1396,10 +1623,28 @@ int pmx_solver_update(pmx_solver* s, pmx_matrix* const* levels, const pmx_space*
             (F.A.sell.pack == 1 || F.A.sell.slice_pat)) {
           // iterative refinement: FP32 Krylov blocks and the inner operator
           const int m = std::min(s->p.gmres_restart, std::max(1, s->p.max_n_iterations));
-          s->V32 = dzeros<float>((size_t)(m + 1) * n, st);
-          s->Z32 = dzeros<float>((size_t)m * n, st);
+          s->ld32 = ((size_t)n + 63) / 64 * 64;
+          s->V32 = dzeros<float>((size_t)(m + 1) * s->ld32, st);
+          s->Z32 = dzeros<float>((size_t)m * s->ld32, st);
           s->w32 = dzeros<float>(n, st);
           s->zero32 = dzeros<float>(std::max(1, F.active), st);
+          if (m + 1 <= pmx::vec_multi_dot_max()) s->mdot = dmalloc<double>(pmx::vec_multi_partials());
+          {
+            void* hp = nullptr;
+            if (hipHostMalloc(&hp, 64 * sizeof(double) + 64, hipHostMallocMapped | hipHostMallocCoherent) ==
+                hipSuccess) {
+              s->hpin = static_cast<double*>(hp);
+              s->hflag = reinterpret_cast<unsigned int*>(s->hpin + 64);
+              *s->hflag = 0;
+              s->hseq = 0;
+              void* dp = nullptr;
+              HIP_CHECK(hipHostGetDevicePointer(&dp, hp, 0));
+              s->hpin_dev = static_cast<double*>(dp);
+              s->hflag_dev = reinterpret_cast<unsigned int*>(s->hpin_dev + 64);
+            } else {
+              (void)hipGetLastError();   // no pinned memory: copy + synchronise instead
+            }
+          }
           if (!s->p.refinement_inner_bf16) {
             s->kval32 = dmalloc<float>(F.A.sell.total);
             pmx::sell_fill_f32(F.A, s->kval32, st);
@@ -1527,9 +1772,9 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
           if (kind == PMX_SMOOTHER_CELL_VANKA) {
             // colour = cell lattice index mod 3 per direction: cells of one
             // colour are >= 3 cells apart, so none reads a DOF another updates
-            const pmx::Mesh& m = *sys->vs.mesh;
-            const pmx::MeshLevel& ml = m.levels[sys->vs.level];
-            const int nv = m.nv, dim = m.dim;
+            const pmx::Mesh& mesh = *sys->vs.mesh;
+            const pmx::MeshLevel& ml = mesh.levels[sys->vs.level];
+            const int nv = mesh.nv, dim = mesh.dim;
             auto centre = [&](int c, int d) {
               double t = 0;
               for (int v = 0; v < nv; ++v) t += ml.xyz[((size_t)c * nv + v) * dim + d];

@@ -350,6 +350,37 @@ def test_transfer_sell_matches_csr(ctx, precision):
     assert abs(a[2] - b[2]) <= 1 and np.max(np.abs(a[1] - b[1])) <= 1e-7 * np.max(np.abs(a[1]))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pattern_transfers_match_sell(ctx, precision):
+    """Transfers from pattern slices (rows sharing column offsets and weights,
+    Dirichlet columns left out; transfer_sell 4 puts every renumbered level
+    on them) against the SELL copy of P and P^T (transfer_sell 2): the row
+    sums run in the same column order and the left-out columns only meet
+    zeros, so one V-cycle and the whole FGMRES+MG solve are the same bit for
+    bit."""
+    nref = 3
+    d, spaces, mats, rhs = three_level_cd3d(ctx, nref, MG_KW)
+    fs = spaces[-1]
+    out = {}
+    for ts in (2, 4):
+        prm = pm.solver_params(iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30,
+                               residual_tolerance=0.0, residual_reduction=1e-8, max_n_iterations=100,
+                               multigrid_cycle_type="V", multigrid_smoother="jacobi",
+                               multigrid_smoother_coarse="direct_solve", multigrid_n_pre_smooth=2,
+                               multigrid_n_post_smooth=2, jacobi_damp=0.5, precision=precision,
+                               transfer_sell=ts)
+        sol = pm.Solver(ctx, prm)
+        sol.update_matrix(mats, spaces)
+        z = torch.zeros_like(rhs[-1])
+        sol.apply_preconditioner(rhs[-1].clone(), z)
+        x = torch.zeros_like(rhs[-1])
+        x[fs.active_bound:] = rhs[-1][fs.active_bound:]
+        rep = sol.solve(rhs[-1], x)
+        out[ts] = (z.cpu().numpy(), x.cpu().numpy(), rep.iterations)
+    a, b = out[2], out[4]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
 def test_fgmres_multigrid_solution_matches_oracle(ctx):
     """FGMRES(30) + MG V(2,2) to 1e-8.  Damping 0.5 keeps the Jacobi smoother
     contractive for this Q2 SUPG operator (lambda_max(D^-1 A) ~ 3), so the
