@@ -51,8 +51,9 @@ def parse():
     ap.add_argument("--no-oseen", action="store_true", help="skip the Oseen Q2/Q1 solve")
     ap.add_argument("--profile-kernel", default=None,
                     choices=["apply_fp64", "jacobi_fp64", "apply_bf16", "jacobi_bf16", "solver_jacobi_bf16",
-                             "solver_jacobi_fp64"],
-                    help="only assemble the finest level, launch this operator kernel 5 times and "
+                             "solver_jacobi_fp64", "assembly", "vanka", "batch"],
+                    help="only run this workload a few times (an operator kernel 5 times, the multigrid's "
+                         "finest sweep, the 64^3 assembly, the Oseen Vanka solve, the batched sweep) and "
                          "copy 1 GiB 3 times (the calibration of HBM byte counters), then exit")
     ap.add_argument("--cpu-refine", type=int, default=4, help="refinements of the CPU sample")
     ap.add_argument("--renumber-levels", type=int, default=None,
@@ -367,8 +368,65 @@ def oseen_solve(pm, torch, ctx, nref=5, nlev=4, nu=1e-2, vanka_damp=0.2, steps=2
 
 
 def profile_kernel(pm, torch, ctx, dom, nref, nlev, omega, kernel):
+    """A few launches of one workload for rocprofv3 counter runs, then a
+    1 GiB copy (torch copy_) three times as the calibration of FETCH_SIZE /
+    WRITE_SIZE.  kernel: jacobi_bf16 / apply_fp64 / ... (the FE matrix's own
+    SELL kernels), solver_jacobi_bf16 / solver_jacobi_fp64 (the multigrid's
+    finest-level sweep), assembly (the 64^3 Q2 SUPG assembly, 6 times),
+    vanka (Oseen 32^3 Q2/Q1: 6 mixed-precision FGMRES iterations with the
+    cell-Vanka multigrid), batch (the 256-problem TCD2D sweep, 2 steps)."""
     params = pm.cd_params("exp_sin_3d", disc="supg", eps=EPS, b=B, hexa_quad_rule=2)
-    if kernel.startswith("solver_"):
+    n_rows, nnz, ms = 0, 0, 0.0
+    if kernel == "assembly":
+        s = pm.TFESpace(dom, nref, 2, ctx)
+        A = pm.FEMatrix(ctx, s)
+        f = torch.zeros(s.n_dof, dtype=torch.float64, device="cuda")
+        ts = []
+        for _ in range(6):
+            A.reset()
+            f.zero_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pm.assemble_cd(ctx, s, params, A, f)
+            ts.append(time.perf_counter() - t0)
+        n_rows, nnz, ms = A.n_rows, A.nnz, 1e3 * min(ts[1:])
+    elif kernel == "vanka":
+        dom5 = pm.TDomain("Default_UnitCube_Hexa", 5)
+        systems = []
+        for lev in range(2, 6):
+            g = pm.NSESystem(ctx, dom5, lev)
+            xyz = pm.TFESpace(dom5, lev, 2, None).dof_coords()
+            w = torch.as_tensor(fourier_field(xyz).reshape(-1)).cuda()
+            r = g.zeros()
+            g.assemble("u2_p1", 1e-2, w, r)
+            systems.append((g, r))
+        sol = pm.Solver(ctx, pm.solver_params(
+            iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30, residual_tolerance=0.0,
+            residual_reduction=1e-30, max_n_iterations=6, multigrid_cycle_type="V",
+            multigrid_smoother="cell_vanka_jacobi", multigrid_smoother_coarse="direct_solve",
+            multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2, multigrid_vanka_damp_factor=0.2,
+            multigrid_correction_damp_factor=0.8, precision="mixed"))
+        sol.update_saddle([q[0] for q in systems])
+        g, b = systems[-1]
+        x = g.zeros()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol.solve(b, x)
+        torch.cuda.synchronize()
+        n_rows, nnz, ms = g.n, int(g.info.nnz_combined), 1e3 * (time.perf_counter() - t0)
+    elif kernel == "batch":
+        import numpy as np
+        rng = np.random.default_rng(0x5EED)
+        ns = rng.integers(32, 257, 256)
+        eps = 10.0 ** rng.uniform(-4, 0, 256)
+        th = rng.uniform(0, 2 * math.pi, 256)
+        bt = pm.BatchTCD(ctx, [(int(n_), float(e), (math.cos(t), math.sin(t)), 0.0) for n_, e, t in zip(ns, eps, th)],
+                         tau=1e-2)
+        bt.initialize()
+        r = bt.run(2)
+        inf = bt.info()
+        n_rows, nnz, ms = inf.total_dof, inf.total_nnz, 1e3 * r.seconds / 2
+    elif kernel.startswith("solver_"):
         # the finest-level sweep as the multigrid runs it (solver-internal order)
         spaces = [pm.TFESpace(dom, l, 2, ctx) for l in range(nref - nlev + 1, nref + 1)]
         mats = []
@@ -388,12 +446,14 @@ def profile_kernel(pm, torch, ctx, dom, nref, nlev, omega, kernel):
         f = torch.zeros(s.n_dof, dtype=torch.float64, device="cuda")
         pm.assemble_cd(ctx, s, params, A, f)
         ms = A.time_kernel(kernel, reps=5)
+    if kernel.startswith("solver_") or kernel not in ("assembly", "vanka", "batch"):
+        n_rows, nnz = A.n_rows, A.nnz
     src = torch.ones(128 * 1024 * 1024, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"kernel": kernel, "n_rows": A.n_rows, "nnz": A.nnz, "avg_ms": ms,
+    print(json.dumps({"kernel": kernel, "n_rows": n_rows, "nnz": nnz, "ms": ms,
                       "copy_bytes": 2 * src.numel() * 8}))
 
 
