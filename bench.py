@@ -199,6 +199,48 @@ def cpu_baseline_full(pm, nref, spaces, mats, b, x0, omega, iterations=(1, 2)):
     return {"per_iteration_s": (t[k1] - t[k0]) / (k1 - k0), "runs_s": {str(k): round(v, 2) for k, v in t.items()}}
 
 
+def fp32_assembly_run(pm, torch, ctx, spaces, mats, rhss, params_ow, solver, b, x0, x64, r0n, A, fs, params):
+    """The mixed path with FP32 element matrices (fp32_assembly) on all
+    levels: entry error against the FP64 assembly, solve time, residual
+    against the FP32- and the FP64-assembled finest matrix, solution against
+    the FP64 path's, L2/H1 errors.  Restores the FP64 matrices afterwards."""
+    import numpy as np
+    v64 = A.get_entries().copy()
+    rp = A.get_structure()[0]
+    p32 = pm.cd_params("exp_sin_3d", disc="supg", eps=EPS, b=B, hexa_quad_rule=2, overwrite=True,
+                       fp32_assembly=True)
+    for sp_, M_, f_ in zip(spaces, mats, rhss):
+        pm.assemble_cd(ctx, sp_, p32, M_, f_)
+    v32 = A.get_entries()
+    rmax = np.repeat(np.maximum.reduceat(np.abs(v64), rp[:-1]), np.diff(rp))
+    ent = np.abs(v32 - v64) / rmax
+    solver.update_matrix(mats, spaces)
+    x = x0.clone()
+    solver.solve(b, x.clone())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = solver.solve(b, x)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    r = b.clone()
+    A.apply_scaled_add(x, r, -1.0)
+    res32 = torch.linalg.norm(r).item() / r0n
+    A.set_entries(v64)
+    r = b.clone()
+    A.apply_scaled_add(x, r, -1.0)
+    res64 = torch.linalg.norm(r).item() / r0n
+    err = pm.compute_errors(ctx, fs, params, x)
+    out = {"entry_max_rel_error": float(ent.max()), "entry_median_rel_error": float(np.median(ent)),
+           "mixed_time_to_solution_s": round(dt, 4), "iterations": rep.iterations,
+           "rel_residual_own_matrix": res32, "rel_residual_fp64_matrix": res64,
+           "solution_rel_diff_vs_fp64": (torch.max(torch.abs(x - x64)) / torch.max(torch.abs(x64))).item(),
+           "l2_h1_errors": [err[0], err[1]]}
+    for sp_, M_, f_ in zip(spaces, mats, rhss):
+        pm.assemble_cd(ctx, sp_, params_ow, M_, f_)
+    solver.update_matrix(mats, spaces)
+    return out
+
+
 def cd2d_config(pm, torch, ctx, with_oracle):
     """CD2D: -lap u = f on the unit square, 64 x 64 Q1 (4,225 DOFs),
     u = sin(pi x) sin(pi y) with Dirichlet data, Gauss 3x3, CG + Jacobi to a
@@ -372,7 +414,8 @@ def profile_kernel(pm, torch, ctx, dom, nref, nlev, omega, kernel):
     1 GiB copy (torch copy_) three times as the calibration of FETCH_SIZE /
     WRITE_SIZE.  kernel: jacobi_bf16 / apply_fp64 / ... (the FE matrix's own
     SELL kernels), solver_jacobi_bf16 / solver_jacobi_fp64 (the multigrid's
-    finest-level sweep), assembly (the 64^3 Q2 SUPG assembly, 6 times),
+    finest-level sweep), assembly (the 64^3 Q2 SUPG assembly, 6 times per
+    route: element GEMM FP64 and FP32, quadrature kernel),
     vanka (Oseen 32^3 Q2/Q1: 6 mixed-precision FGMRES iterations with the
     cell-Vanka multigrid), batch (the 256-problem TCD2D sweep, 2 steps)."""
     params = pm.cd_params("exp_sin_3d", disc="supg", eps=EPS, b=B, hexa_quad_rule=2)
@@ -381,14 +424,17 @@ def profile_kernel(pm, torch, ctx, dom, nref, nlev, omega, kernel):
         s = pm.TFESpace(dom, nref, 2, ctx)
         A = pm.FEMatrix(ctx, s)
         f = torch.zeros(s.n_dof, dtype=torch.float64, device="cuda")
-        ts = []
-        for _ in range(6):
-            A.reset()
-            f.zero_()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            pm.assemble_cd(ctx, s, params, A, f)
-            ts.append(time.perf_counter() - t0)
+        for kw in (dict(overwrite=True), dict(overwrite=True, fp32_assembly=True), dict(assembly_variant=1)):
+            prm = pm.cd_params("exp_sin_3d", disc="supg", eps=EPS, b=B, hexa_quad_rule=2, **kw)
+            ts = []
+            for _ in range(6):
+                A.reset()
+                f.zero_()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                pm.assemble_cd(ctx, s, prm, A, f)
+                ts.append(time.perf_counter() - t0)
+            print("assembly", kw, "%.3f ms" % (1e3 * min(ts[1:])), flush=True)
         n_rows, nnz, ms = A.n_rows, A.nnz, 1e3 * min(ts[1:])
     elif kernel == "vanka":
         dom5 = pm.TDomain("Default_UnitCube_Hexa", 5)
@@ -488,20 +534,33 @@ def main():
     fs, A, b = spaces[-1], mats[-1], rhss[-1]
     ab = fs.active_bound
 
-    # assembly + one SpMV (throughput in MDoF/s), inputs resident
+    # assembly + one SpMV (throughput in MDoF/s), inputs resident.  The
+    # assembly sets matrix and rhs (overwrite: the reference's reset + add in
+    # one pass); the same for each assembly route alone
+    params_ow = pm.cd_params("exp_sin_3d", disc="supg", eps=EPS, b=B, hexa_quad_rule=2, overwrite=True)
     x1 = torch.ones(fs.n_dof, dtype=torch.float64, device="cuda")
     y1 = torch.empty_like(x1)
     f_tmp = torch.zeros_like(b)
     t_as = []
     for _ in range(3):
-        A.reset()
-        f_tmp.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        pm.assemble_cd(ctx, fs, params, A, f_tmp)
+        pm.assemble_cd(ctx, fs, params_ow, A, f_tmp)
         A.apply(x1, y1)
         t_as.append(time.perf_counter() - t0)
     t_asm_spmv = min(t_as)
+    t_routes = {}
+    for name, kw in (("element_gemm_fp64", {}), ("element_gemm_fp32", dict(fp32_assembly=True)),
+                     ("quadrature_kernel", dict(assembly_variant=1))):
+        prm = pm.cd_params("exp_sin_3d", disc="supg", eps=EPS, b=B, hexa_quad_rule=2, overwrite=True, **kw)
+        ts = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pm.assemble_cd(ctx, fs, prm, A, f_tmp)
+            ts.append(time.perf_counter() - t0)
+        t_routes[name] = round(1000 * min(ts), 3)
+    pm.assemble_cd(ctx, fs, params_ow, A, f_tmp)
     del f_tmp, x1, y1
 
     solvers = {}
@@ -564,7 +623,7 @@ def main():
         solvers[prec].apply_preconditioner(b.clone(), z)
         zs[prec] = z
     stage_errors = {
-        "assembly": "FP64 in both paths (the same kernel)",
+        "assembly": "FP64 in both paths (the same kernel); FP32 element matrices: fp32_assembly",
         "bf16_matrix_max_rel_entry_error": bf16_row_scaled_error(A.get_entries(), A.get_structure()[0]),
         "cycle_rel_diff": (torch.max(torch.abs(zs["mixed"] - zs["fp64"])) / torch.max(torch.abs(zs["fp64"]))).item(),
         "solution_rel_diff": sol_diff,
@@ -587,14 +646,18 @@ def main():
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             for sp_, M_, f_ in zip(spaces, mats, rhss):
-                M_.reset()
-                f_.zero_()
-                pm.assemble_cd(ctx, sp_, params, M_, f_)
+                pm.assemble_cd(ctx, sp_, params_ow, M_, f_)
             solvers[prec].update_matrix(mats, spaces)
             solvers[prec].solve(b, x)
             torch.cuda.synchronize()
             ts.append(time.perf_counter() - t0)
         t_e2e[prec] = min(ts)
+
+    # FP32 element matrices (FP32 matrix cores) on every level, then the
+    # mixed solve: its residual against the FP32-assembled matrix and against
+    # the FP64 one, the solution against the FP64 path's, the errors
+    fp32asm = fp32_assembly_run(pm, torch, ctx, spaces, mats, rhss, params_ow, solvers["mixed"], b, x0,
+                                xs["fp64"], r0n, A, fs, params)
 
     # configuration 3's tolerance: 1e-10 (mixed and FP64)
     tight = {}
@@ -707,6 +770,8 @@ def main():
         "l2_h1_errors": {k: [errs[k][0], errs[k][1]] for k in errs},
         "assembly_spmv_mdofs": round(n / t_asm_spmv / 1e6, 2),
         "assembly_spmv_ms": round(1000 * t_asm_spmv, 3),
+        "assembly_ms_by_route": t_routes,
+        "fp32_assembly": fp32asm,
         "solver_update_s": {k: round(v, 3) for k, v in t_update.items()},
         "solver_value_update_s": {k: round(v, 4) for k, v in t_refresh.items()},
         "assembly_update_solve_s": {k: round(v, 4) for k, v in t_e2e.items()},

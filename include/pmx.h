@@ -183,6 +183,24 @@ typedef struct {
   double time;         /* TDatabase::TimeDB->CURRENTTIME for time-dependent examples */
   double theta1;       /* TimeDB->THETA1 (TCD SUPG forms) */
   double tau;          /* TimeDB->CURRENTTIMESTEPLENGTH (TCD SUPG forms) */
+  /* Q2 hexahedra, pmx_assemble_cd.  0 (default): where the coefficients are
+     constant on each cell (no dev_coeff_table) and disc is Galerkin or SUPG,
+     the element matrices are one GEMM of per-cell coefficients with
+     reference matrices of the quadrature rule on the matrix cores, and each
+     CSR row is assembled once from its cells' element matrices (cell order);
+     elsewhere the quadrature kernel.  1: always the quadrature kernel (one
+     launch per cell colour). */
+  int assembly_variant;
+  /* with assembly_variant 0: element matrices on the FP32 matrix cores (FP32
+     operands and accumulation), summed into the FP64 CSR in FP64; entries
+     carry relative errors of at most about 1e-6 of their row's largest entry.
+     PMX_E_ARG where the GEMM path does not apply. */
+  int fp32_assembly;
+  /* pmx_assemble_cd: 0 adds into A and dev_rhs (the caller zeroes them first,
+     as ConvectionDiffusion::assemble does); 1 sets them: the element-GEMM
+     route writes every entry once (no zeroing pass, no read of the old
+     values), the other routes zero A and dev_rhs themselves first. */
+  int overwrite;
 } pmx_cd_params;
 /* number of quadrature points per cell the assembly uses for this space */
 int pmx_assembly_n_quad(const pmx_space* s, int hexa_quad_rule, int* n_quad);
@@ -194,7 +212,8 @@ int pmx_assembly_quad_points(const pmx_space* s, int hexa_quad_rule, double* hos
  * ADDS the Galerkin (+SUPG) matrix into A and the load vector into dev_rhs,
  * sets Dirichlet diagonals to 1 and Dirichlet rhs entries to the boundary
  * values, and adds Neumann boundary integrals.  As in the reference the
- * caller zeroes A and rhs first (src/System/ConvectionDiffusion.C:329-340). */
+ * caller zeroes A and rhs first (src/System/ConvectionDiffusion.C:329-340),
+ * or sets p->overwrite. */
 int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s, const pmx_cd_params* p, pmx_matrix* A,
                     double* dev_rhs);
 

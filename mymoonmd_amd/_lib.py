@@ -33,7 +33,8 @@ class CDParams(ctypes.Structure):
                 ("eps", c_double), ("b", c_double * 3), ("c", c_double), ("f0", c_double),
                 ("delta0", c_double), ("poly_degree", c_double),
                 ("dev_coeff_table", c_void_p), ("host_dirichlet", c_void_p),
-                ("time", c_double), ("theta1", c_double), ("tau", c_double)]
+                ("time", c_double), ("theta1", c_double), ("tau", c_double),
+                ("assembly_variant", c_int), ("fp32_assembly", c_int), ("overwrite", c_int)]
 
 
 class BatchProblem(ctypes.Structure):

@@ -602,6 +602,87 @@ int pmx_assembly_quad_points(const pmx_space* s, int hexa_quad_rule, double* hos
   });
 }
 
+// Incidence slots of the Q2 row gather, built once per space: for every
+// active row the (cell, local DOF) pairs in increasing cell order, the slot of
+// each pair and the row positions of each such cell's 27 DOFs.  A space whose
+// rows are too long or where too many cells meet at a DOF keeps the
+// quadrature kernel.
+bool q2_gather_ready(pmx_space* s, const pmx_matrix* A, hipStream_t st) {
+  if (s->q2_gather_state) return s->q2_gather_state > 0;
+  s->q2_gather_state = -1;
+  const pmx::FESpace& fs = s->fs;
+  const pmx::CSRPattern& pat = *s->pattern;
+  const int na = fs.active_bound;
+  for (int r = 0; r < na; ++r)
+    if (pat.row_ptr[r + 1] - pat.row_ptr[r] > pmx::kQ2MaxRow) return false;
+  std::vector<int> ptr(na + 1, 0);
+  const std::vector<int>& gn = fs.global_numbers;
+  for (size_t k = 0; k < gn.size(); ++k)
+    if (gn[k] < na) ++ptr[gn[k] + 1];
+  for (int r = 0; r < na; ++r) {
+    if (ptr[r + 1] > pmx::kQ2MaxInc) return false;
+    ptr[r + 1] += ptr[r];
+  }
+  std::vector<int> inc(ptr[na]), slot(gn.size(), -1), fill(ptr.begin(), ptr.end() - 1);
+  for (int e = 0; e < fs.n_cells; ++e)
+    for (int i = 0; i < 27; ++i) {
+      const int r = gn[(size_t)e * 27 + i];
+      if (r >= na) continue;
+      slot[(size_t)e * 27 + i] = fill[r];
+      inc[fill[r]++] = 27 * e + i;
+    }
+  s->d_q2_inc_ptr = upload(ptr.data(), ptr.size(), st);
+  s->d_q2_slot = upload(slot.data(), slot.size(), st);
+  s->d_q2_off = dmalloc<uint8_t>((size_t)inc.size() * 32);
+  pmx::Q2Gather g;
+  g.n_active = na;
+  g.n_inc = (int)inc.size();
+  g.inc_ptr = s->d_q2_inc_ptr;
+  g.slot = s->d_q2_slot;
+  g.off = s->d_q2_off;
+  int* d_inc = upload(inc.data(), inc.size(), st);
+  int* d_bad = dmalloc<int>(1);
+  HIP_CHECK(hipMemsetAsync(d_bad, 0, sizeof(int), st));
+  pmx::q2_gather_offsets(g, d_inc, s->d_dofs, A->A, s->d_q2_off, d_bad, st);
+  HIP_CHECK(hipGetLastError());
+  int bad = 0;
+  HIP_CHECK(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_CHECK(hipStreamSynchronize(st));
+  dfree(d_bad);
+  dfree(d_inc);
+  if (bad) {
+    dfree(s->d_q2_inc_ptr); dfree(s->d_q2_slot); dfree(s->d_q2_off);
+    return false;
+  }
+  s->q2g = g;
+  s->q2_gather_state = 1;
+  return true;
+}
+
+// element GEMM + row gather (Q2 hexahedra, constant coefficients per cell)
+void assemble_cd_q2_gemm(pmx_space* s, RefTable& t, const pmx::AssemblyData& ad, const pmx::CDCoeffs& co,
+                         bool fp32, bool overwrite, pmx_matrix* A, double* dev_rhs, hipStream_t st) {
+  if (!t.d_R64) {
+    std::vector<double> R((size_t)pmx::kQ2Terms * pmx::kQ2N);
+    pmx::q2_reference_matrices(t.h_ref.data(), t.nq, R.data());
+    std::vector<float> R32(R.begin(), R.end());
+    t.d_R64 = upload(R.data(), R.size(), st);
+    t.d_R32 = upload(R32.data(), R32.size(), st);
+  }
+  const pmx::Q2Gather& g = s->q2g;
+  const size_t ne = (size_t)g.n_inc * 32;
+  if (!s->d_q2_fe) s->d_q2_fe = dmalloc<double>(std::max(1, g.n_inc));
+  if (fp32) {
+    if (!s->d_q2_E32) s->d_q2_E32 = dmalloc<float>(std::max<size_t>(1, ne));
+    pmx::q2_element_gemm(ad, co, g, t.d_R32, s->d_q2_E32, s->d_q2_fe, st);
+    pmx::q2_row_gather(g, s->d_q2_E32, s->d_q2_fe, A->A, dev_rhs, overwrite, st);
+  } else {
+    if (!s->d_q2_E64) s->d_q2_E64 = dmalloc<double>(std::max<size_t>(1, ne));
+    pmx::q2_element_gemm(ad, co, g, t.d_R64, s->d_q2_E64, s->d_q2_fe, st);
+    pmx::q2_row_gather(g, s->d_q2_E64, s->d_q2_fe, A->A, dev_rhs, overwrite, st);
+  }
+}
+
 int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p, pmx_matrix* A,
                     double* dev_rhs) {
   return guarded([&] {
@@ -611,12 +692,25 @@ int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p, p
     require(p->disc == PMX_DISC_GALERKIN || p->disc == PMX_DISC_SUPG || p->disc == PMX_DISC_GLS,
             "unknown discretisation");
     require(p->example >= 0 && p->example <= PMX_EX_TCD_LINEAR, "unknown example");
+    require(p->assembly_variant == 0 || p->assembly_variant == 1, "assembly_variant must be 0 or 1");
     pmx_space* s = const_cast<pmx_space*>(s_);
     hipStream_t st = ctx->stream;
     A->sell_valid = false;
-    const RefTable& t = ref_table(s, p->hexa_quad_rule);
+    RefTable& t = const_cast<RefTable&>(ref_table(s, p->hexa_quad_rule));
     pmx::AssemblyData ad = assembly_data(s, t);
-    if (ad.dim == 3 && ad.ndof == 27) {
+    const pmx::CDCoeffs co = cd_coeffs(p);
+    const bool gemm = ad.dim == 3 && ad.ndof == 27 && p->assembly_variant == 0 && !co.table && co.supg != 2 &&
+                      ad.nq <= 64 && q2_gather_ready(s, A, st);
+    require(!p->fp32_assembly || gemm,
+            "fp32_assembly needs the element-GEMM route: Q2 hexahedra, Galerkin or SUPG, no coefficient "
+            "table, at most 64 quadrature points, assembly_variant 0");
+    if (p->overwrite && !gemm) {
+      pmx::vec_fill(A->A.nnz, 0.0, A->A.val, st);
+      pmx::vec_fill(s->fs.n_dof, 0.0, dev_rhs, st);
+    }
+    if (gemm) {
+      assemble_cd_q2_gemm(s, t, ad, co, p->fp32_assembly != 0, p->overwrite != 0, A, dev_rhs, st);
+    } else if (ad.dim == 3 && ad.ndof == 27) {
       // the matrix-core kernel scatters through a position table of the
       // space's pattern (built once, 4 KB per cell)
       if (!s->d_q2_pos) {
@@ -626,11 +720,11 @@ int pmx_assemble_cd(pmx_ctx* ctx, const pmx_space* s_, const pmx_cd_params* p, p
       }
       ad.pos = s->d_q2_pos;
     }
-    const pmx::CDCoeffs co = cd_coeffs(p);
-    for (size_t k = 0; k + 1 < s->color_ptr.size(); ++k) {
-      const int n = s->color_ptr[k + 1] - s->color_ptr[k];
-      pmx::assemble_cd(ad, co, s->d_cells + s->color_ptr[k], n, A->A, dev_rhs, st);
-    }
+    if (!gemm)
+      for (size_t k = 0; k + 1 < s->color_ptr.size(); ++k) {
+        const int n = s->color_ptr[k + 1] - s->color_ptr[k];
+        pmx::assemble_cd(ad, co, s->d_cells + s->color_ptr[k], n, A->A, dev_rhs, st);
+      }
     HIP_CHECK(hipGetLastError());
     const pmx::FESpace& fs = s->fs;
     const int ndir = fs.n_dof - fs.active_bound;

@@ -17,6 +17,7 @@
 // accumulates its 27-entry row in registers.  Cells of one colour share no
 // DOF, so the scatter into the CSR row (binary search of the sorted columns)
 // is race free without atomics.
+#include <algorithm>
 #include <mutex>
 
 #include "common.hpp"
@@ -668,6 +669,401 @@ void q2_scatter_positions(const AssemblyData& ad, const DevCSR& A, int* pos, hip
   if (total > 0)
     hipLaunchKernelGGL(k_q2_scatter_positions, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, ad.n_cells,
                        ad.dofs, A, pos);
+}
+
+// ---- Q2 hexahedra, coefficients constant on each cell: reference matrices
+//
+// On an affine cell with constant eps, b, c (the built-in examples; a
+// per-point coefficient table goes to the quadrature kernel above) the
+// Galerkin/SUPG element matrix of k_assemble_cd_q2_mfma expands, term by
+// term, into
+//   A_e = sum_r alpha_r(e) R_r,   R_r[i][j] = sum_q w_q u_r(q, i) v_r(q, j),
+// with the same quadrature rule on the reference cell.  With beta = J^-1 b
+// (b.grad phi = sum_k beta_k d_k phi), G = J^-1 J^-T, m = |det J|, D_k the
+// reference derivatives and D_s, s = xx xy xz yy yz zz, the second ones:
+//   r 0-2    D_k phi_i D_k phi_j                 m (eps G_kk + delta beta_k^2)
+//   r 3-5    D_k phi_i D_l phi_j + (k <-> l)     m (eps G_kl + delta beta_k beta_l)   kl = 01 02 12
+//   r 6-8    phi_i D_l phi_j                     m beta_l
+//   r 9      phi_i phi_j                         m c
+//   r 10-12  D_k phi_i phi_j                     m delta c beta_k
+//   r 13-30  D_k phi_i D_s phi_j  (13 + 6k + s)  -m delta eps beta_k G'_s
+// (G'_s = G_mm for s = mm, 2 G_mn for s = mn, m != n: -eps lap phi =
+// -eps sum_s G'_s D_s phi; delta = 0 for Galerkin).  So all element matrices
+// are one GEMM E[cells][27 x 32] = alpha[cells][32] R[32][27 x 32] (element
+// rows padded to 32 entries), run on the matrix cores in FP64 or FP32 (FP32
+// accumulation).  Element row i of cell e is written to the slot of the
+// incidence (row dofs[e][i], cell e) in the order the rows read them, and a
+// second kernel assembles every active CSR row once from its contiguous,
+// aligned slots, adding them in increasing cell order (the order of
+// Assemble3D's cell loop, Assemble3D.C:319-409) in LDS and writing the row
+// with coalesced stores -- instead of a scattered read-modify-write per
+// element entry in eight colour passes.
+
+void q2_reference_matrices(const double* ref, int nq, double* R) {
+  constexpr int ND = 27, RS = 1 + 3 + ND * 10;
+  for (size_t k = 0; k < (size_t)kQ2Terms * kQ2N; ++k) R[k] = 0.0;
+  static const int kl[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+  for (int q = 0; q < nq; ++q) {
+    const double* rq = ref + (size_t)q * RS;
+    const double w = rq[0];
+    const double* phi = rq + 4;
+    const double* d1 = rq + 4 + ND;            // d1[k * 27 + i]
+    const double* d2 = rq + 4 + ND * 4;        // d2[s * 27 + i]
+    for (int i = 0; i < ND; ++i)
+      for (int j = 0; j < ND; ++j) {
+        const int n = i * 32 + j;
+        auto add = [&](int r, double v) { R[(size_t)r * kQ2N + n] += w * v; };
+        for (int k = 0; k < 3; ++k) add(k, d1[k * ND + i] * d1[k * ND + j]);
+        for (int t = 0; t < 3; ++t) {
+          const int k = kl[t][0], l = kl[t][1];
+          add(3 + t, d1[k * ND + i] * d1[l * ND + j] + d1[l * ND + i] * d1[k * ND + j]);
+        }
+        for (int l = 0; l < 3; ++l) add(6 + l, phi[i] * d1[l * ND + j]);
+        add(9, phi[i] * phi[j]);
+        for (int k = 0; k < 3; ++k) add(10 + k, d1[k * ND + i] * phi[j]);
+        for (int k = 0; k < 3; ++k)
+          for (int s = 0; s < 6; ++s) add(13 + 6 * k + s, d1[k * ND + i] * d2[s * ND + j]);
+      }
+  }
+}
+
+namespace {
+
+// alpha_r of one cell (the table above) and (m, delta, beta) for its load vector
+__device__ void q2_cell_terms(const CDCoeffs& co, const double* geo, double* al, double* cd) {
+  double Ji[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Ji[k] = geo[12 + k];
+  const double m = fabs(geo[21]), hK = geo[22];
+  const double eps = co.eps, c = co.c;
+  double beta[3], G[9];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) beta[k] = Ji[3 * k] * co.b[0] + Ji[3 * k + 1] * co.b[1] + Ji[3 * k + 2] * co.b[2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int l = 0; l < 3; ++l)
+      G[3 * k + l] = Ji[3 * k] * Ji[3 * l] + Ji[3 * k + 1] * Ji[3 * l + 1] + Ji[3 * k + 2] * Ji[3 * l + 2];
+  const double delta = co.supg ? sdfem_delta(hK, eps, co.b, 3, c, co.delta0, co.poly_degree) : 0.0;
+  al[0] = m * (eps * G[0] + delta * beta[0] * beta[0]);
+  al[1] = m * (eps * G[4] + delta * beta[1] * beta[1]);
+  al[2] = m * (eps * G[8] + delta * beta[2] * beta[2]);
+  al[3] = m * (eps * G[1] + delta * beta[0] * beta[1]);
+  al[4] = m * (eps * G[2] + delta * beta[0] * beta[2]);
+  al[5] = m * (eps * G[5] + delta * beta[1] * beta[2]);
+#pragma unroll
+  for (int l = 0; l < 3; ++l) al[6 + l] = m * beta[l];
+  al[9] = m * c;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) al[10 + k] = m * delta * c * beta[k];
+  const double Gs[6] = {G[0], 2.0 * G[1], 2.0 * G[2], G[4], 2.0 * G[5], G[8]};
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int s = 0; s < 6; ++s) al[13 + 6 * k + s] = -m * delta * eps * beta[k] * Gs[s];
+  al[31] = 0.0;
+  cd[0] = m;
+  cd[1] = delta;
+  cd[2] = beta[0];
+  cd[3] = beta[1];
+  cd[4] = beta[2];
+}
+
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+
+// one MFMA k-step of a 16 x 16 tile and the row of result element t of a lane
+__device__ __forceinline__ v4d_t mfma16(double a, double b, v4d_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f_t mfma16(float a, float b, v4f_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// f64 16x16x4: element t of lane l is D[4 t + l / 16][l % 16];
+// f32 16x16x4: element t of lane l is D[4 (l / 16) + t][l % 16]
+__device__ __forceinline__ int mfma16_row(double, int t, int lane) { return 4 * t + (lane >> 4); }
+__device__ __forceinline__ int mfma16_row(float, int t, int lane) { return 4 * (lane >> 4) + t; }
+template <typename T> struct Acc16;
+template <> struct Acc16<double> { typedef v4d_t type; };
+template <> struct Acc16<float> { typedef v4f_t type; };
+
+constexpr int kQ2GemmCells = 64;   // cells per workgroup of the element GEMM
+constexpr int kQ2MaxQ = 64;        // quadrature points the load-vector stage holds
+
+// 64 cells per 256-thread workgroup: (1) alpha and the load-vector data of
+// each cell, (2) w_q f(x_q) at every quadrature point, (3) the element load
+// vectors, (4) E = alpha R: every wave owns the N tiles nt = wave (mod 4) of
+// all four 16-cell M tiles, with the four tiles' alpha operands in registers;
+// element row i of cell e goes to its incidence slot (lane e of the wave
+// reads the slot of cell e, the lanes holding a result fetch it).
+template <typename T>
+__global__ __launch_bounds__(256) void k_q2_element_gemm(AssemblyData ad, CDCoeffs co, Q2Gather g,
+                                                         const T* __restrict__ R, T* __restrict__ E,
+                                                         double* __restrict__ fe) {
+  constexpr int ND = 27, RS = 1 + 3 + ND * 10, NC = kQ2GemmCells;
+  __shared__ double s_al[NC][kQ2Terms + 1];
+  __shared__ double s_cd[NC][6];
+  // w_q f(x_q) per cell and point; after the load vectors the cells' slots
+  __shared__ double s_wf[NC][kQ2MaxQ + 1];
+  int* s_slot = reinterpret_cast<int*>(&s_wf[0][0]);
+  const int tid = threadIdx.x, c0 = blockIdx.x * NC, nq = ad.nq, n_cells = ad.n_cells;
+  if (tid < NC) {
+    const int e = c0 + tid;
+    if (e < n_cells) {
+      q2_cell_terms(co, ad.geo + (size_t)e * kGeoStride, s_al[tid], s_cd[tid]);
+    } else {
+      for (int r = 0; r < kQ2Terms; ++r) s_al[tid][r] = 0.0;
+      for (int r = 0; r < 6; ++r) s_cd[tid][r] = 0.0;
+    }
+  }
+  const double t = co.time_dev ? *co.time_dev : co.time;
+  for (int k = tid; k < NC * nq; k += 256) {
+    const int lc = k / nq, q = k - lc * nq, e = c0 + lc;
+    double v = 0.0;
+    if (e < n_cells) {
+      const double* geo = ad.geo + (size_t)e * kGeoStride;
+      const double* rq = ad.ref + (size_t)q * RS;
+      double x[3];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) x[r] = geo[r] + geo[3 + 3 * r] * rq[1] + geo[4 + 3 * r] * rq[2] + geo[5 + 3 * r] * rq[3];
+      v = rq[0] * example_rhs(co.example, x[0], x[1], x[2], co.eps, co.b, co.c, co.f0, t);
+    }
+    s_wf[lc][q] = v;
+  }
+  __syncthreads();
+  // fe = m sum_q w_q f_q (phi_i + delta beta.D phi_i) of every active (e, i):
+  // thread t < 243 keeps the basis function i = t % 27 and sums the cells
+  // t / 27 + 9 c, so each reference value it loads serves up to 8 cells
+  if (tid < 9 * ND) {
+    constexpr int CPT = (NC + 8) / 9;
+    const int i = tid % ND, c1 = tid / ND;
+    double s0[CPT], s1[CPT], s2[CPT], s3[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) s0[c] = s1[c] = s2[c] = s3[c] = 0.0;
+    for (int q = 0; q < nq; ++q) {
+      const double* rq = ad.ref + (size_t)q * RS + 4;
+      const double p0 = rq[i], p1 = rq[ND + i], p2 = rq[2 * ND + i], p3 = rq[3 * ND + i];
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        const int lc = c1 + 9 * c;
+        if (lc < NC) {
+          const double wf = s_wf[lc][q];
+          s0[c] += wf * p0;
+          s1[c] += wf * p1;
+          s2[c] += wf * p2;
+          s3[c] += wf * p3;
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int lc = c1 + 9 * c, e = c0 + lc;
+      if (lc >= NC || e >= n_cells) continue;
+      const int sl = g.slot[(size_t)e * ND + i];
+      if (sl < 0) continue;
+      const double* cd = s_cd[lc];
+      fe[sl] = cd[0] * (s0[c] + cd[1] * (cd[2] * s1[c] + cd[3] * s2[c] + cd[4] * s3[c]));
+    }
+  }
+  __syncthreads();
+  for (int k = tid; k < NC * ND; k += 256)
+    s_slot[k] = ((size_t)c0 * ND + k < (size_t)n_cells * ND) ? g.slot[(size_t)c0 * ND + k] : -1;
+  __syncthreads();
+  // E = alpha R on the matrix cores
+  const int wv = tid >> 6, lane = tid & 63, r16 = lane & 15, kq = lane >> 4;
+  T a[4][8];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) a[mt][s] = (T)s_al[16 * mt + r16][4 * s + kq];
+  for (int nt = wv; nt < kQ2N / 16; nt += 4) {
+    const int li = nt >> 1, j = (nt & 1) * 16 + r16;
+    const int my_slot = s_slot[lane * ND + li];
+    T b[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) b[s] = R[(size_t)(4 * s + kq) * kQ2N + li * 32 + j];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      typename Acc16<T>::type acc = {0, 0, 0, 0};
+#pragma unroll
+      for (int s = 0; s < 8; ++s) acc = mfma16(a[mt][s], b[s], acc);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        // the padding entries (j >= 27, zero rows of R) are stored too, so
+        // that every 128-byte line of a slot is written whole
+        const int sl = __shfl(my_slot, 16 * mt + mfma16_row(T(), u, lane));
+        if (sl >= 0) E[(size_t)sl * 32 + j] = acc[u];
+      }
+    }
+  }
+}
+
+// Persistent row gather: every half-wave streams a contiguous chunk of
+// active rows.  For each row the element rows of its slots (lanes 0-26),
+// their positions and (when adding) the row's old values are loaded, the
+// element rows are added into the row's LDS copy in slot (cell) order, and the
+// row is written with coalesced stores.  The next row's pointers and data are
+// loaded before the current row is accumulated, so one memory round trip per
+// row is in flight while the previous one completes.  (FP32, overwrite, 64^3:
+// 0.65 ms; one short-lived workgroup per 8 rows 0.79 ms, two rows per
+// half-wave or an XCD-ordered row mapping slower still.)
+constexpr int kGatherBlocks = 1280;   // 5 per CU: all resident at the kernel's occupancy
+
+template <typename T>
+struct GatherRow {
+  int rs = 0, len = 0, k0 = 0, nk = 0;
+  T v[kQ2MaxInc];
+  int o[kQ2MaxInc];
+  double old[kQ2MaxRow / 32];
+  double fv = 0.0, rold = 0.0;   // load-vector entry of slot l; the row's old rhs (lane 0)
+};
+
+template <typename T, bool OVERWRITE>
+__device__ __forceinline__ void gather_ptrs(const Q2Gather& g, const DevCSR& A, int row, int row_end, GatherRow<T>& w) {
+  w.rs = w.len = w.k0 = w.nk = 0;
+  if (row < row_end) {
+    w.rs = A.row_ptr[row];
+    w.len = A.row_ptr[row + 1] - w.rs;
+    w.k0 = g.inc_ptr[row];
+    w.nk = g.inc_ptr[row + 1] - w.k0;
+  }
+}
+
+template <typename T, bool OVERWRITE>
+__device__ __forceinline__ void gather_data(const Q2Gather& g, const DevCSR& A, const T* __restrict__ E,
+                                            const double* __restrict__ fe, const double* __restrict__ rhs,
+                                            int row, int l, GatherRow<T>& w) {
+#pragma unroll
+  for (int k = 0; k < kQ2MaxInc; ++k) {
+    w.v[k] = 0;
+    w.o[k] = 0;
+    if (k < w.nk && l < 27) {
+      w.v[k] = E[(size_t)(w.k0 + k) * 32 + l];
+      w.o[k] = g.off[(size_t)(w.k0 + k) * 32 + l];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kQ2MaxRow / 32; ++q)
+    w.old[q] = (!OVERWRITE && 32 * q + l < w.len) ? A.val[w.rs + 32 * q + l] : 0.0;
+  w.fv = (l < w.nk) ? fe[w.k0 + l] : 0.0;
+  w.rold = (!OVERWRITE && l == 0 && w.len > 0) ? rhs[row] : 0.0;
+}
+
+template <typename T, bool OVERWRITE>
+__global__ __launch_bounds__(256) void k_q2_row_gather(Q2Gather g, const T* __restrict__ E,
+                                                       const double* __restrict__ fe, DevCSR A,
+                                                       double* __restrict__ rhs) {
+  constexpr int NQ = kQ2MaxRow / 32;
+  __shared__ double s_row[8][kQ2MaxRow];
+  const int h = threadIdx.x >> 5, l = threadIdx.x & 31;
+  const int n_hw = gridDim.x * 8, hw = blockIdx.x * 8 + h;
+  const int chunk = (g.n_active + n_hw - 1) / n_hw;
+  const int r0 = min(hw * chunk, g.n_active), r1 = min(r0 + chunk, g.n_active);
+  double* b = s_row[h];
+  GatherRow<T> cur, nxt;
+  gather_ptrs<T, OVERWRITE>(g, A, r0, r1, cur);
+  gather_data<T, OVERWRITE>(g, A, E, fe, rhs, r0, l, cur);
+  gather_ptrs<T, OVERWRITE>(g, A, r0 + 1, r1, nxt);
+  for (int row = r0; row < r1; ++row) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) b[32 * q + l] = 0.0;
+    wave_sync();
+    // LDS adds of one wave run in issue order: each entry is summed in slot
+    // order without a wait between the slots (the 27 positions of a slot
+    // are distinct)
+#pragma unroll
+    for (int k = 0; k < kQ2MaxInc; ++k)
+      if (k < cur.nk && l < 27) atomicAdd(&b[cur.o[k]], (double)cur.v[k]);
+    // the next row's data and the pointers of the one after it
+    GatherRow<T> nn;
+    gather_data<T, OVERWRITE>(g, A, E, fe, rhs, row + 1, l, nxt);
+    gather_ptrs<T, OVERWRITE>(g, A, row + 2, r1, nn);
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      if (32 * q + l < cur.len) A.val[cur.rs + 32 * q + l] = OVERWRITE ? b[32 * q + l] : cur.old[q] + b[32 * q + l];
+    // the row's load vector: its slots' entries in slot order
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kQ2MaxInc; ++k) {
+      const double t = __shfl(cur.fv, k, 32);
+      if (k < cur.nk) s += t;
+    }
+    if (l == 0) rhs[row] = OVERWRITE ? s : cur.rold + s;
+    wave_sync();
+    cur = nxt;
+    nxt = nn;
+  }
+}
+
+__global__ void k_q2_gather_offsets(Q2Gather g, const int* __restrict__ inc, const int* __restrict__ dofs, DevCSR A,
+                                    uint8_t* __restrict__ off, int* __restrict__ bad) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)g.n_inc * 32) return;
+  const int k = (int)(t >> 5), j = (int)(t & 31);
+  int o = 0;
+  if (j < 27) {
+    const int ic = inc[k], e = ic / 27, i = ic - 27 * e;
+    const int r = dofs[(size_t)e * 27 + i], c = dofs[(size_t)e * 27 + j];
+    const int rs = A.row_ptr[r], re = A.row_ptr[r + 1];
+    const int p = find_col(A.col, rs, re, c);
+    if (p >= re || A.col[p] != c || p - rs >= kQ2MaxRow) *bad = 1;
+    else o = p - rs;
+  }
+  off[t] = (uint8_t)o;
+}
+
+__global__ void k_dirichlet_unit_diagonal(DevCSR A) {
+  const int i = A.active + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < A.n) A.val[A.row_ptr[i]] = 1.0;
+}
+
+template <typename T>
+void q2_element_gemm_t(const AssemblyData& ad, const CDCoeffs& co, const Q2Gather& g, const T* R, T* E,
+                       double* fe, hipStream_t st) {
+  if (ad.n_cells > 0)
+    hipLaunchKernelGGL(k_q2_element_gemm<T>, dim3((ad.n_cells + kQ2GemmCells - 1) / kQ2GemmCells), dim3(256), 0, st,
+                       ad, co, g, R, E, fe);
+}
+
+template <typename T>
+void q2_row_gather_t(const Q2Gather& g, const T* E, const double* fe, const DevCSR& A, double* rhs, bool overwrite,
+                     hipStream_t st) {
+  if (g.n_active > 0) {
+    const int grid = std::min(kGatherBlocks, (g.n_active + 7) / 8);
+    if (overwrite)
+      hipLaunchKernelGGL((k_q2_row_gather<T, true>), dim3(grid), dim3(256), 0, st, g, E, fe, A, rhs);
+    else
+      hipLaunchKernelGGL((k_q2_row_gather<T, false>), dim3(grid), dim3(256), 0, st, g, E, fe, A, rhs);
+  }
+  if (A.n > A.active)
+    hipLaunchKernelGGL(k_dirichlet_unit_diagonal, dim3((A.n - A.active + 255) / 256), dim3(256), 0, st, A);
+}
+
+}  // namespace
+
+void q2_gather_offsets(const Q2Gather& g, const int* inc, const int* dofs, const DevCSR& A, uint8_t* off, int* bad,
+                       hipStream_t st) {
+  const size_t total = (size_t)g.n_inc * 32;
+  if (total > 0)
+    hipLaunchKernelGGL(k_q2_gather_offsets, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, g, inc, dofs,
+                       A, off, bad);
+}
+
+void q2_element_gemm(const AssemblyData& ad, const CDCoeffs& co, const Q2Gather& g, const double* R, double* E,
+                     double* fe, hipStream_t st) {
+  q2_element_gemm_t(ad, co, g, R, E, fe, st);
+}
+void q2_element_gemm(const AssemblyData& ad, const CDCoeffs& co, const Q2Gather& g, const float* R, float* E,
+                     double* fe, hipStream_t st) {
+  q2_element_gemm_t(ad, co, g, R, E, fe, st);
+}
+void q2_row_gather(const Q2Gather& g, const double* E, const double* fe, const DevCSR& A, double* rhs,
+                   bool overwrite, hipStream_t st) {
+  q2_row_gather_t(g, E, fe, A, rhs, overwrite, st);
+}
+void q2_row_gather(const Q2Gather& g, const float* E, const double* fe, const DevCSR& A, double* rhs,
+                   bool overwrite, hipStream_t st) {
+  q2_row_gather_t(g, E, fe, A, rhs, overwrite, st);
 }
 
 void assemble_cd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n,

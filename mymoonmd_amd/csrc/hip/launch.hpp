@@ -387,6 +387,48 @@ void interpolate_example(int example, double time, int n, const double* xyz, dou
 // Adds the cell contributions of the cells cells[0..n) (one colour) into A and rhs.
 void assemble_cd(const AssemblyData& ad, const CDCoeffs& co, const int* cells, int n,
                  const DevCSR& A, double* rhs, hipStream_t st);
+
+// Q2 hexahedra with coefficients constant on each cell: element matrices as
+// E_e = sum_r alpha_r(e) R_r (kQ2Terms reference matrices of the quadrature
+// rule, [kQ2Terms][kQ2N] row-major, entry n = 32 i + j, j < 27), then every
+// active CSR row gathered once from the element matrices of its cells.
+constexpr int kQ2Terms = 32;   // 31 reference matrices and one zero row
+constexpr int kQ2N = 27 * 32;  // element rows padded to 32 entries (54 tiles of 16)
+constexpr int kQ2MaxInc = 8;   // cells per DOF (a vertex of a hexahedral mesh)
+constexpr int kQ2MaxRow = 128; // CSR row length the gather holds in LDS
+// The incidences of the active rows, in row order and within a row in
+// increasing cell order: row r owns the incidence slots inc_ptr[r] ..
+// inc_ptr[r + 1]; slot[27 e + i] is the slot of (cell e, local DOF i), -1 for
+// a Dirichlet row.  Per slot k, off[32 k + j] is the position of the cell's
+// local DOF j in row r relative to row_ptr[r] (j < 27); the element GEMM
+// writes row i of cell e's element matrix to E[32 k ..] and its load-vector
+// entry to fe[k].
+struct Q2Gather {
+  int n_active = 0;
+  int n_inc = 0;
+  const int* inc_ptr = nullptr;
+  const int* slot = nullptr;
+  const uint8_t* off = nullptr;
+};
+// host: the reference matrices of a Q2 hexahedron quadrature table
+// (ref layout of AssemblyData::ref), FP64, [kQ2Terms][kQ2N]
+void q2_reference_matrices(const double* ref, int nq, double* R);
+// off[] of all slots from inc[k] = 27 e + i (the slot's cell and local DOF);
+// *bad = 1 if a position is missing or beyond the LDS row of the gather
+void q2_gather_offsets(const Q2Gather& g, const int* inc, const int* dofs, const DevCSR& A, uint8_t* off,
+                       int* bad, hipStream_t st);
+// E[32 k + j] and fe[k] of all slots k
+void q2_element_gemm(const AssemblyData& ad, const CDCoeffs& co, const Q2Gather& g, const double* R, double* E,
+                     double* fe, hipStream_t st);
+void q2_element_gemm(const AssemblyData& ad, const CDCoeffs& co, const Q2Gather& g, const float* R, float* E,
+                     double* fe, hipStream_t st);
+// A.val[row] += sum over the row's slots (cell order), rhs[row] += the
+// element load-vector entries, for the active rows (= with overwrite);
+// Dirichlet diagonals set to 1
+void q2_row_gather(const Q2Gather& g, const double* E, const double* fe, const DevCSR& A, double* rhs,
+                   bool overwrite, hipStream_t st);
+void q2_row_gather(const Q2Gather& g, const float* E, const double* fe, const DevCSR& A, double* rhs,
+                   bool overwrite, hipStream_t st);
 // rhs[i] = g(x_i) for Dirichlet rows from the built-in example (i >= active)
 void dirichlet_rhs_example(int example, int n_dir, const double* dir_xyz, double* rhs_dir,
                            const CDCoeffs& co, hipStream_t st);

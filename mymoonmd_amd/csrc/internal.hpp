@@ -91,6 +91,9 @@ struct RefTable {
   double* d_ref = nullptr;
   std::vector<double> h_ref;
   pmx::QuadRule rule;
+  // Q2 hexahedra: reference matrices of the element GEMM (FP64, FP32 copy)
+  double* d_R64 = nullptr;
+  float* d_R32 = nullptr;
 };
 
 struct pmx_space {
@@ -111,10 +114,27 @@ struct pmx_space {
   double* d_dir_xyz = nullptr;     // the Dirichlet part of d_dof_xyz (not owned)
   std::map<int, RefTable> ref;     // keyed by hexa quad rule
   int* d_q2_pos = nullptr;         // CSR positions of the Q2 element entries (matrix-core assembly)
+  // Q2 element GEMM + row gather: incidence slots of the active rows (state
+  // 0: not built, 1: built, -1: the pattern does not fit the gather), element
+  // rows (FP64 / FP32, 32 entries per slot) and load-vector entries per slot
+  int q2_gather_state = 0;
+  pmx::Q2Gather q2g;
+  int* d_q2_inc_ptr = nullptr;
+  int* d_q2_slot = nullptr;
+  uint8_t* d_q2_off = nullptr;
+  double* d_q2_E64 = nullptr;
+  float* d_q2_E32 = nullptr;
+  double* d_q2_fe = nullptr;
   ~pmx_space() {
     pmx_internal::dfree(d_dofs); pmx_internal::dfree(d_geo); pmx_internal::dfree(d_cells); pmx_internal::dfree(d_dof_xyz);
     pmx_internal::dfree(d_q2_pos);
-    for (auto& kv : ref) pmx_internal::dfree(kv.second.d_ref);
+    pmx_internal::dfree(d_q2_inc_ptr); pmx_internal::dfree(d_q2_slot); pmx_internal::dfree(d_q2_off);
+    pmx_internal::dfree(d_q2_E64); pmx_internal::dfree(d_q2_E32); pmx_internal::dfree(d_q2_fe);
+    for (auto& kv : ref) {
+      pmx_internal::dfree(kv.second.d_ref);
+      pmx_internal::dfree(kv.second.d_R64);
+      pmx_internal::dfree(kv.second.d_R32);
+    }
   }
 };
 

@@ -265,7 +265,7 @@ class FEMatrix:
 
 def cd_params(example="const", disc="galerkin", eps=1.0, b=(0.0, 0.0, 0.0), c=0.0, f0=0.0,
               hexa_quad_rule=0, delta0=1.0, poly_degree=1.0, time=0.0, theta1=1.0, tau=0.0,
-              coeff_table=None, dirichlet=None):
+              coeff_table=None, dirichlet=None, assembly_variant=0, fp32_assembly=False, overwrite=False):
     """pmx_cd_params.  `coeff_table`: optional float64 device tensor
     [n_cells, n_quad, 8] with the coefficients (eps, b1, b2, b3, c, f, -, -) at
     the points of TFESpace.quad_points(hexa_quad_rule) -- what a user
@@ -274,7 +274,11 @@ def cd_params(example="const", disc="galerkin", eps=1.0, b=(0.0, 0.0, 0.0), c=0.
     right-hand side.  `dirichlet`: optional float64 host array with the
     boundary values of the Dirichlet DOFs (DOFs active_bound.. in order),
     replacing the example's.  Both are checked against the space when the
-    parameters are used."""
+    parameters are used.  `assembly_variant` / `fp32_assembly`: the Q2
+    hexahedron assembly route (0: element GEMM + row gather where it applies,
+    1: the quadrature kernel) and FP32 matrix cores for its element
+    matrices.  `overwrite`: assemble_cd sets the matrix and rhs instead of
+    adding to them."""
     p = CDParams()
     p._table = coeff_table
     p._dirichlet = None
@@ -291,6 +295,9 @@ def cd_params(example="const", disc="galerkin", eps=1.0, b=(0.0, 0.0, 0.0), c=0.
     p.delta0 = delta0
     p.poly_degree = poly_degree
     p.time, p.theta1, p.tau = time, theta1, tau
+    p.assembly_variant = assembly_variant
+    p.fp32_assembly = 1 if fp32_assembly else 0
+    p.overwrite = 1 if overwrite else 0
     return p
 
 
@@ -312,7 +319,8 @@ def _bind_user_data(ctx, space, params):
 
 
 def assemble_cd(ctx, space, params, matrix, rhs):
-    """Assemble3D/Assemble2D + LocalAssembling(ConvDiff): adds into matrix and rhs."""
+    """Assemble3D/Assemble2D + LocalAssembling(ConvDiff): adds into matrix and
+    rhs (sets them with params.overwrite)."""
     import torch
     rp = _ptr(rhs, space.n_dof, ctx.device, "rhs")
     _bind_user_data(ctx, space, params)
@@ -521,10 +529,16 @@ class ConvectionDiffusion:
         self.report = None
 
     def assemble(self):
-        for s, A, f in zip(self.spaces, self.matrices, self.rhs):
-            A.reset()
-            f.zero_()
-            assemble_cd(self.ctx, s, self.params, A, f)
+        # the reference resets matrix and rhs and adds the cell contributions
+        # (ConvectionDiffusion.C:329-340); pmx_assemble_cd with overwrite
+        # gives the same result without the separate reset
+        ow = self.params.overwrite
+        self.params.overwrite = 1
+        try:
+            for s, A, f in zip(self.spaces, self.matrices, self.rhs):
+                assemble_cd(self.ctx, s, self.params, A, f)
+        finally:
+            self.params.overwrite = ow
         # solution.copy_nonactive(rhs)
         ab = self.spaces[-1].active_bound
         vector_copy(self.ctx, self.rhs[-1][ab:], self.solution[ab:])

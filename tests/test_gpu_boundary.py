@@ -293,9 +293,10 @@ def test_coefficient_table_reproduces_the_example(ctx, rule):
     """pmx_cd_params.dev_coeff_table, the route of a user CoeffFct3D
     (LocalAssembling::GetParameters, LocalAssembling.C:766-840): the
     exp-sin example's coefficients sampled at pmx_assembly_quad_points give
-    the example's SUPG assembly -- the matrix bit for bit (the same
-    constants), the load vector within 1e-13 (f evaluated by numpy instead of
-    the kernel)."""
+    the example's SUPG assembly on the same (quadrature-kernel) route -- the
+    matrix bit for bit (the same constants), the load vector within 1e-13 (f
+    evaluated by numpy instead of the kernel) -- and agree with the
+    element-GEMM route the constant coefficients take by default to 1e-13."""
     nref = 2
     d = pm.TDomain("Default_UnitCube_Hexa", nref)
     s = pm.TFESpace(d, nref, 2, ctx)
@@ -303,7 +304,10 @@ def test_coefficient_table_reproduces_the_example(ctx, rule):
     kw = dict(disc="supg", eps=eps, b=b, c=c, hexa_quad_rule=rule)
     A0 = pm.FEMatrix(ctx, s)
     f0 = torch.zeros(s.n_dof, dtype=torch.float64, device="cuda")
-    pm.assemble_cd(ctx, s, pm.cd_params("exp_sin_3d", **kw), A0, f0)
+    pm.assemble_cd(ctx, s, pm.cd_params("exp_sin_3d", assembly_variant=1, **kw), A0, f0)
+    Ag = pm.FEMatrix(ctx, s)
+    fg = torch.zeros_like(f0)
+    pm.assemble_cd(ctx, s, pm.cd_params("exp_sin_3d", **kw), Ag, fg)
     X = s.quad_points(rule)
     assert X.shape == (s.n_cells, 64 if rule == 0 else 27, 3)
     T = np.zeros(X.shape[:2] + (8,))
@@ -315,6 +319,9 @@ def test_coefficient_table_reproduces_the_example(ctx, rule):
     assert np.array_equal(A1.get_entries(), A0.get_entries())
     fa, fb = f0.cpu().numpy(), f1.cpu().numpy()
     assert np.max(np.abs(fa - fb)) <= 1e-13 * np.max(np.abs(fa))
+    ga, gb = A0.get_entries(), Ag.get_entries()
+    assert np.max(np.abs(ga - gb)) <= 1e-13 * np.max(np.abs(ga))
+    assert np.max(np.abs(fa - fg.cpu().numpy())) <= 1e-13 * np.max(np.abs(fa))
 
 
 @pytest.mark.parametrize("geo,nref,order,disc", [(2, 2, 2, "galerkin"), (2, 2, 2, "supg"), (2, 2, 1, "supg"),

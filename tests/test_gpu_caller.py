@@ -30,6 +30,7 @@ def test_cpp_caller_matches_python_route(ctx):
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["failures"] == 0 and out["matrix_equal"], out
     assert out["rhs_rel_diff"] <= 1e-13 and out["fp64_rel_residual"] <= 1.01e-8
+    assert out["gemm_route_rel_diff"] <= 1e-13
     assert out["error_code"] == 1 and "NULL" in out["error_message"]
     # the Python mirror on the built-in example
     d = pm.TDomain("Default_UnitCube_Hexa", nref)
