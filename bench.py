@@ -290,19 +290,25 @@ def cd2d_config(pm, torch, ctx, with_oracle):
     return out, oracle
 
 
-def batched_sweep(pm, ctx, n_problems=256, steps=20, seed=0x5EED):
-    """256 independent TCD2D problems (P1, n ~ U{32..256} squares per side,
-    eps log-uniform in [1e-4, 1], |b| = 1 at a uniform angle), backward Euler
-    tau = 1e-2 for 20 steps, all advanced together; FP64 residual reduction
-    1e-10 per step per problem."""
+def batch_problems(n_problems=256, seed=0x5EED):
+    """(n_side, eps, b, c) of the sweep: n ~ U{32..256}, eps log-uniform in
+    [1e-4, 1], |b| = 1 at a uniform angle (seeded)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     ns = rng.integers(32, 257, n_problems)
     eps = 10.0 ** rng.uniform(-4, 0, n_problems)
     th = rng.uniform(0, 2 * math.pi, n_problems)
-    probs = [(int(n), float(e), (math.cos(t), math.sin(t)), 0.0) for n, e, t in zip(ns, eps, th)]
+    return [(int(n), float(e), (math.cos(t), math.sin(t)), 0.0) for n, e, t in zip(ns, eps, th)]
+
+
+def batched_sweep(pm, ctx, n_problems=256, steps=20, seed=0x5EED, inner_solver="bi_cgstab"):
+    """256 independent TCD2D problems (P1, n ~ U{32..256} squares per side,
+    eps log-uniform in [1e-4, 1], |b| = 1 at a uniform angle), backward Euler
+    tau = 1e-2 for 20 steps, all advanced together; FP64 residual reduction
+    1e-10 per step per problem; FP32 inner BiCGStab (or GMRES(30))."""
+    probs = batch_problems(n_problems, seed)
     t0 = time.perf_counter()
-    bt = pm.BatchTCD(ctx, probs, tau=1e-2)
+    bt = pm.BatchTCD(ctx, probs, tau=1e-2, inner_solver=inner_solver)
     t_setup = time.perf_counter() - t0
     bt.initialize()
     # snapshots of every problem (SnapshotsCollector, steps_per_snap 5: the
@@ -330,8 +336,74 @@ def batched_sweep(pm, ctx, n_problems=256, steps=20, seed=0x5EED):
             "failed_solves": int(rep.failed_solves), "block": int(rep.block),
             "solve_workgroups": int(inf.n_workgroups),
             "host_setup_s": round(t_setup, 2),
-            "method": "persistent launch, one workgroup per problem (several for the costliest): FP64 "
-                      "residual refinement around FP32 BiCGStab + Jacobi"}
+            "method": "persistent launch, one workgroup per problem (several for the costliest with BiCGStab): "
+                      "FP64 residual refinement around FP32 %s + Jacobi" % (
+                          "BiCGStab" if inner_solver == "bi_cgstab" else "GMRES(30)")}
+
+
+def cpu_baseline_batch(batch, n_sample=8, steps=3):
+    """The oracle (one core) on a bounded sample of the sweep: backward Euler
+    for the first problems' (eps, b) on the 64 x 64 TwoTriangles mesh (4,225
+    DOFs, the oracle's meshes are refinements), FGMRES + Jacobi to the same
+    relative residual 1e-10 per step; DOF-steps per second and the sweep's
+    time at that rate."""
+    from threadpoolctl import threadpool_limits
+    from oracle import oracle as O
+    sp, _ = O.build_space(1, 6, 6, 1)
+    prm = O.solver_params(iterative_solver_type=0, preconditioner=1, gmres_restart=30, max_n_iterations=5000,
+                          residual_tolerance=0.0, residual_reduction=1e-10)
+
+    def solver(S, rhs, x0):
+        x, it, _, _ = O.solve(prm, [sp], [S], rhs, x0)
+        return x
+    t = 0.0
+    with threadpool_limits(1):
+        for n, e, bb, c in batch_problems()[:n_sample]:
+            t0 = time.perf_counter()
+            O.tcd_time_integration(sp, 7, 0, O.BACKWARD_EULER, 1e-2, steps, eps=e, b=bb, c=c, solver=solver)
+            t += time.perf_counter() - t0
+    rate = sp.n_dof * steps * n_sample / t
+    total = batch["total_dof"] * batch["steps"] / rate
+    return {"mdof_steps_per_s": round(rate / 1e6, 4), "estimated_sweep_s": round(total, 1), "cores": 1,
+            "kind": "port", "sample": "oracle backward Euler, %d problems x %d steps on the 4,225-DOF mesh "
+            "(%.1f s incl. assembly and the initial interpolation), FGMRES + Jacobi to 1e-10" % (
+                n_sample, steps, t)}
+
+
+def cpu_baseline_oseen(oseen, iterations=5, nref=3, nu=1e-2, vanka_damp=0.2):
+    """The oracle (numpy with one BLAS thread) on a bounded sample of the Oseen
+    leg: 8^3 Q2/Q1 (2 levels), the same Fourier convection field, FGMRES +
+    multigrid V(2,2) with cell Vanka (damping 0.2, correction 0.8, direct
+    coarse solve) for a few iterations; seconds per iteration per DOF times the
+    32^3 DOFs and the GPU's iteration count."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+    from oracle import oracle as O
+    levels = []
+    rhs = None
+    for lev in (nref - 1, nref):
+        o = O.NseOracle(nref, lev)
+        w = fourier_field(O.nodal_coords(o.vs)).reshape(-1)
+        A, B, rhs = o.assemble(nu, w)
+        levels.append(O.NseLevelOracle(o, A, B))
+    F = levels[-1]
+    nuu, na = F.ns.n_u, F.ns.n_u_active
+    x0 = np.zeros(F.ns.n)
+    for k in range(3):
+        x0[k * nuu + na:(k + 1) * nuu] = rhs[k * nuu + na:(k + 1) * nuu]
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        O.fgmres(lambda z: F.ns.apply(F.A, F.B, z),
+                 lambda v: O.nse_mg_apply(levels, v, damp_corr=0.8, vanka_damp=vanka_damp, n_pre=2, n_post=2),
+                 rhs, x0, restart=30, max_it=iterations, tol=0.0)
+        t = time.perf_counter() - t0
+    per_it_dof = t / iterations / F.ns.n
+    it_gpu = oseen["fp64"]["iterations"]
+    est = per_it_dof * oseen["n_dof"] * it_gpu
+    return {"value": round(est, 1), "unit": "s", "cores": 1, "kind": "port",
+            "sample": "oracle FGMRES + cell-Vanka multigrid on 8^3 (%d DOFs, 2 levels), %d iterations in %.2f s: "
+                      "per-iteration cost per DOF x %d DOFs x the %d iterations of the GPU FP64 solve" % (
+                          F.ns.n, iterations, t, oseen["n_dof"], it_gpu)}
 
 
 def fourier_field(x, seed=0x5EED, n_modes=8):
@@ -684,18 +756,20 @@ def main():
     damp08 = None
     if args.damp08_iterations > 0:
         del solvers
-        sol = pm.Solver(ctx, solver_params(pm, "mixed", 0.8, args.damp08_iterations))
-        sol.update_matrix(mats, spaces)
-        x = x0.clone()
-        t0 = time.perf_counter()
-        rep = sol.solve(b, x)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        r = b.clone()
-        A.apply_scaled_add(x, r, -1.0)
-        damp08 = {"precision": "mixed", "iterations": rep.iterations, "converged": bool(rep.converged),
-                  "fp64_rel_residual": torch.linalg.norm(r).item() / r0n, "time_s": round(dt, 3)}
-        del sol
+        damp08 = {}
+        for prec in ("mixed", "fp64"):
+            sol = pm.Solver(ctx, solver_params(pm, prec, 0.8, args.damp08_iterations))
+            sol.update_matrix(mats, spaces)
+            x = x0.clone()
+            t0 = time.perf_counter()
+            rep = sol.solve(b, x)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            r = b.clone()
+            A.apply_scaled_add(x, r, -1.0)
+            damp08[prec] = {"iterations": rep.iterations, "converged": bool(rep.converged),
+                            "fp64_rel_residual": torch.linalg.norm(r).item() / r0n, "time_s": round(dt, 3)}
+            del sol
 
     # roofline of the dominant kernel: the fused bf16 Jacobi sweep of the
     # finest level (3 full sweeps + the defect of the 5 finest-level operator
@@ -729,6 +803,7 @@ def main():
 
     cd2d, cd2d_cpu = cd2d_config(pm, torch, ctx, not args.no_cpu_baseline)
     batch = None if args.no_batch else batched_sweep(pm, ctx)
+    batch_gmres = None if args.no_batch else batched_sweep(pm, ctx, steps=5, inner_solver="gmres")
     oseen = None if args.no_oseen else oseen_solve(pm, torch, ctx)
 
     cpu = None
@@ -745,6 +820,10 @@ def main():
                              n, full["runs_s"], full["per_iteration_s"], its["fp64"], c["mdofs"],
                              (n / t_asm_spmv / 1e6) / c["mdofs"]),
                "cd2d": cd2d_cpu}
+        if batch is not None:
+            cpu["batched_tcd2d"] = cpu_baseline_batch(batch)
+        if oseen is not None:
+            cpu["oseen_q2q1_32"] = cpu_baseline_oseen(oseen)
 
     out = {
         "metric": "time-to-solution @1e-8 rel. residual, mixed vs FP64 GPU; assembly+SpMV MDoF/s",
@@ -784,6 +863,7 @@ def main():
         "jacobi_damp_0.8_check": damp08,
         "cd2d_poisson_64": cd2d,
         "batched_tcd2d": batch,
+        "batched_tcd2d_gmres": batch_gmres,
         "oseen_q2q1_32": oseen,
         "roofline": roof,
         "cpu_baseline": cpu,
