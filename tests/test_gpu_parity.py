@@ -7,6 +7,7 @@ import pytest
 import torch
 
 from mymoonmd_amd import parmoon as pm
+from mymoonmd_amd._lib import PmxError
 from conftest import iterative_7x7
 from oracle import oracle as O
 
@@ -189,10 +190,31 @@ def test_cd2d_golden_errors_fgmres_multigrid(ctx, golden):
 @pytest.mark.parametrize("disc", ["galerkin", "gls"])
 def test_cd3d_zero_error_examples_on_gpu(ctx, golden, idx, solver, disc):
     name, order, eps, b, c = golden["cd3d_solvers_test"]["examples"][idx]
-    if disc == "gls" and solver == "jacobi" and order == 2:
-        pytest.skip("Richardson + Jacobi diverges for the Q2 GLS matrix (spectral radius of the "
-                    "Jacobi iteration > 1); the multigrid variant covers this case")
     prm = pm.cd_params(name, disc=disc, eps=eps, b=b, c=c)
+    if disc == "gls" and solver == "jacobi" and order == 2:
+        # Richardson + Jacobi cannot converge on the Q2 GLS matrix: the
+        # spectral radius of I - D^-1 A on the active block is 1.53 (the
+        # multigrid variant covers this case); the solver must report the
+        # divergence (IterativeMethod.h:105-131) rather than return
+        sp = pm.solver_params(iterative_solver_type="richardson", preconditioner="jacobi",
+                              residual_tolerance=1e-10, residual_reduction=0.0, max_n_iterations=1000,
+                              divergence_factor=1e3)
+        cd = pm.ConvectionDiffusion(ctx, pm.TDomain("Default_UnitCube_Hexa", 2), order, prm, sp)
+        cd.assemble()
+        A, s = cd.matrices[-1], cd.spaces[-1]
+        rp, col = A.get_structure()
+        v = A.get_entries()
+        ab = s.active_bound
+        D = np.zeros((s.n_dof, s.n_dof))
+        for r in range(s.n_dof):
+            D[r, col[rp[r]:rp[r + 1]]] = v[rp[r]:rp[r + 1]]
+        Aa = D[:ab, :ab]
+        rho = np.max(np.abs(np.linalg.eigvals(np.eye(ab) - Aa / np.diag(Aa)[:, None])))
+        assert rho > 1.5, rho
+        with pytest.raises(PmxError) as e:
+            cd.solve()
+        assert e.value.code == 4
+        return
     if solver == "multigrid":
         # GLS adds delta (L phi_j, L phi_i) with -eps lap phi (eps = 1): the
         # undamped Jacobi smoother of this set-up contracts more slowly there
