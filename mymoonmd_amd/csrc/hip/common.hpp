@@ -40,4 +40,13 @@ __device__ __forceinline__ float bf16_to_float(uint16_t h) {
   return __uint_as_float(((uint32_t)h) << 16);
 }
 
+// synchronises the lanes of one wavefront only (for waves that share no LDS
+// with other waves): the fences order its LDS stores before the other lanes'
+// loads, the barrier keeps the compiler from moving accesses across it
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 }  // namespace pmx

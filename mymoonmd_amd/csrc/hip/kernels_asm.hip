@@ -481,13 +481,6 @@ __global__ __launch_bounds__(1024) void k_reduce_errors(int nb, const double* __
 // D[4 t + l / 16][l % 16].
 typedef double v4d_t __attribute__((ext_vector_type(4)));
 
-// the waves of the matrix-core assembly share no LDS: each synchronises only
-// its own lanes (the fences order its LDS stores before the other lanes' loads)
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <int QMAX>
 __global__ __launch_bounds__(256) void k_assemble_cd_q2_mfma(AssemblyData ad, CDCoeffs co,
