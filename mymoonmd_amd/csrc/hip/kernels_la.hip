@@ -611,12 +611,28 @@ void launch_sell(const DevCSR& A, const VT* va, const VT* bd_val, const float* r
                        A.sell.bd_col, bd_val, va, rscale, x, b, dinv, y, a, partial);
 }
 
-__global__ void k_sell_fill_f64(size_t total, const int* __restrict__ src, const double* __restrict__ v,
-                                double* __restrict__ out) {
-  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
-       k += (size_t)gridDim.x * blockDim.x) {
-    const int s = src[k];
-    out[k] = (s >= 0) ? v[s] : 0.0;
+// SELL values from the CSR values through the slot -> position map; each
+// thread keeps 16 slots in flight (index loads, then the gathers, then the
+// stores) instead of one dependent chain at a time
+template <typename T>
+__global__ void k_sell_fill(size_t total, const int* __restrict__ src, const double* __restrict__ v,
+                            T* __restrict__ out) {
+  constexpr int U = 16;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; k + (U - 1) * stride < total; k += U * stride) {
+    int s[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s[u] = src[k + u * stride];
+    double x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) x[u] = (s[u] >= 0) ? v[s[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) out[k + u * stride] = (T)x[u];
+  }
+  for (; k < total; k += stride) {
+    const int s1 = src[k];
+    out[k] = (T)((s1 >= 0) ? v[s1] : 0.0);
   }
 }
 
@@ -633,14 +649,26 @@ __global__ __launch_bounds__(256) void k_sell_fill_bf16(int n_slices, const int*
   const int r = rowmap[slice * 64 + lane];
   const double inv = (r >= 0) ? 1.0 / (double)rscale[r] : 0.0;
   const int s0 = sptr[slice], width = (sptr[slice + 1] - s0) >> 6;
-  for (int j = 0; j < width; ++j) {
-    const int k = s0 + (j / pack) * 64 * pack + lane * pack + j % pack;
-    const int s = src[k];
+  auto slot = [&](int j) { return s0 + (j / pack) * 64 * pack + lane * pack + j % pack; };
+  auto conv = [&](int s) -> uint16_t {
     const float f = (s >= 0) ? (float)(v[s] * inv) : 0.0f;
     uint32_t u = __float_as_uint(f);
     u += 0x7FFFu + ((u >> 16) & 1u);
-    out[k] = (uint16_t)(u >> 16);
+    return (uint16_t)(u >> 16);
+  };
+  // 16 slots in flight: the index loads, then the gathers, then the stores
+  int j = 0;
+  for (; j + 16 <= width; j += 16) {
+    int s[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s[u] = src[slot(j + u)];
+    uint16_t h[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) h[u] = conv(s[u]);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) out[slot(j + u)] = h[u];
   }
+  for (; j < width; ++j) out[slot(j)] = conv(src[slot(j)]);
 }
 
 template <int MODE, typename VT, typename XT>
@@ -1067,31 +1095,44 @@ __global__ void k_add_scaled_range(int from, int n, double a, const TX* __restri
     y[i] = (TY)((double)y[i] + a * (double)x[i]);
 }
 
-// dinv[i] = 1 / A_ii ; flags zero diagonals
-__global__ void k_inv_diag(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                           const double* __restrict__ va, double* __restrict__ dinv,
-                           int* __restrict__ zero_flag) {
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+// dinv[i] = 1 / A_ii ; flags zero diagonals.  16 lanes per row read the
+// row's entries coalesced; the one lane holding the diagonal entry (a sum
+// with zeros from the others, exact) gives d
+__global__ __launch_bounds__(256) void k_inv_diag(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                  const double* __restrict__ va, double* __restrict__ dinv,
+                                                  int* __restrict__ zero_flag) {
+  const int l = threadIdx.x & 15;
+  for (int r = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 4); r < n; r += (int)((gridDim.x * blockDim.x) >> 4)) {
     double d = 0;
-    for (int k = rp[r]; k < rp[r + 1]; ++k)
-      if (ci[k] == r) { d = va[k]; break; }
-    if (d == 0.0) { atomicOr(zero_flag, 1); dinv[r] = 0.0; }
-    else dinv[r] = 1.0 / d;
+    for (int k = rp[r] + l; k < rp[r + 1]; k += 16)
+      if (ci[k] == r) d = va[k];
+#pragma unroll
+    for (int m = 8; m >= 1; m >>= 1) d += __shfl_xor(d, m, 16);
+    if (l == 0) {
+      if (d == 0.0) { atomicOr(zero_flag, 1); dinv[r] = 0.0; }
+      else dinv[r] = 1.0 / d;
+    }
   }
 }
 
 // FP64 values -> bf16 values with an FP32 per-row scale (row max-abs
 // normalised to 1), for the mixed-precision V-cycle
-__global__ void k_to_bf16_rows(int n, const int* __restrict__ rp, const double* __restrict__ va,
-                               uint16_t* __restrict__ vb, float* __restrict__ rscale) {
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+// per-row scale max |a_ij| (16 lanes per row, coalesced; max is exact in
+// any order) and, when vb is given, the row's bf16 values a_ij / scale
+__global__ __launch_bounds__(256) void k_to_bf16_rows(int n, const int* __restrict__ rp, const double* __restrict__ va,
+                                                      uint16_t* __restrict__ vb, float* __restrict__ rscale) {
+  const int l = threadIdx.x & 15;
+  for (int r = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 4); r < n; r += (int)((gridDim.x * blockDim.x) >> 4)) {
+    const int k0 = rp[r], k1 = rp[r + 1];
     double m = 0;
-    for (int k = rp[r]; k < rp[r + 1]; ++k) m = fmax(m, fabs(va[k]));
+    for (int k = k0 + l; k < k1; k += 16) m = fmax(m, fabs(va[k]));
+#pragma unroll
+    for (int s = 8; s >= 1; s >>= 1) m = fmax(m, __shfl_xor(m, s, 16));
     const double sc = (m > 0) ? m : 1.0;
-    rscale[r] = (float)sc;
+    if (l == 0) rscale[r] = (float)sc;
     if (!vb) continue;   // row scales only
     const double inv = 1.0 / sc;
-    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+    for (int k = k0 + l; k < k1; k += 16) {
       const float f = (float)(va[k] * inv);
       uint32_t u = __float_as_uint(f);
       u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
@@ -1472,12 +1513,12 @@ void vec_f32_to_f64(int n, const float* x, double* y, hipStream_t st) {
   hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(256), 0, st, n, x, y);
 }
 void csr_inv_diag(const DevCSR& A, double* dinv, int* zero_flag, hipStream_t st) {
-  hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(A.n)), dim3(256), 0, st, A.n, A.row_ptr, A.col, A.val,
+  hipLaunchKernelGGL(k_inv_diag, dim3(grid_for(A.n, 16)), dim3(256), 0, st, A.n, A.row_ptr, A.col, A.val,
                      dinv, zero_flag);
 }
 void sell_fill_f64(const DevCSR& A, double* sell_val, hipStream_t st) {
   if (A.sell.total)
-    hipLaunchKernelGGL(k_sell_fill_f64, dim3(4096), dim3(256), 0, st, A.sell.total, A.sell.src, A.val, sell_val);
+    hipLaunchKernelGGL(k_sell_fill<double>, dim3(4096), dim3(256), 0, st, A.sell.total, A.sell.src, A.val, sell_val);
 }
 void sell_fill_bf16(const DevCSR& A, uint16_t* sell_bf16, hipStream_t st) {
   if (A.sell.n_slices)
@@ -1485,21 +1526,12 @@ void sell_fill_bf16(const DevCSR& A, uint16_t* sell_bf16, hipStream_t st) {
                        A.sell.slice_ptr, A.sell.row, A.sell.src, A.val, A.rscale, sell_bf16, A.sell.pack);
 }
 void csr_to_bf16(const DevCSR& A, uint16_t* vb, float* rscale, hipStream_t st) {
-  hipLaunchKernelGGL(k_to_bf16_rows, dim3(grid_for(A.n)), dim3(256), 0, st, A.n, A.row_ptr, A.val, vb,
+  hipLaunchKernelGGL(k_to_bf16_rows, dim3(grid_for(A.n, 16)), dim3(256), 0, st, A.n, A.row_ptr, A.val, vb,
                      rscale);
 }
 
 // ------------------------------------------- FP32 Krylov of the refinement
 namespace {
-
-__global__ void k_sell_fill_f32(size_t total, const int* __restrict__ src, const double* __restrict__ v,
-                                float* __restrict__ out) {
-  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
-       k += (size_t)gridDim.x * blockDim.x) {
-    const int s = src[k];
-    out[k] = (s >= 0) ? (float)v[s] : 0.0f;
-  }
-}
 
 // y = x / a_dev[0], rounded to FP32
 __global__ void k_scale_to_f32(int n, const double* __restrict__ x, const double* __restrict__ a_dev,
@@ -1525,7 +1557,7 @@ __global__ void k_multi_axpy_f32(int n, int k, Coef64 c, const float* __restrict
 
 void sell_fill_f32(const DevCSR& A, float* out, hipStream_t st) {
   if (A.sell.total)
-    hipLaunchKernelGGL(k_sell_fill_f32, dim3(4096), dim3(256), 0, st, A.sell.total, A.sell.src, A.val, out);
+    hipLaunchKernelGGL(k_sell_fill<float>, dim3(4096), dim3(256), 0, st, A.sell.total, A.sell.src, A.val, out);
 }
 void vec_dot_partial_f32(int n, const float* x, const float* y, double* partial, hipStream_t st) {
   hipLaunchKernelGGL(k_dot_partial<float>, dim3(kRedBlocks), dim3(256), 0, st, n, x, y, partial);
