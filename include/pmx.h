@@ -404,7 +404,9 @@ typedef struct {
   int gmres_restart;         /* inner GMRES(m), m <= 64 */
   int max_inner;             /* inner BiCGStab iterations per refinement step */
   int max_outer;             /* FP64 refinement steps per time step */
-  double inner_reduction;    /* inner residual reduction per refinement step */
+  double inner_reduction;    /* inner residual reduction per refinement step (default 1e-2; the
+                                256-problem sweep takes 1.65 s for 20 steps with it, 2.11 s with
+                                1e-4; the FP64 residual test decides convergence either way) */
   double residual_reduction; /* FP64 residual reduction per time step */
   double residual_tolerance; /* FP64 absolute residual per time step */
   int block;                 /* threads per problem: 256, 512, 1024, or 0 = tune on the first run */

@@ -251,7 +251,7 @@ int pmx_batch_default_params(pmx_batch_params* p) {
     p->gmres_restart = 30;
     p->max_inner = 2000;
     p->max_outer = 50;
-    p->inner_reduction = 1e-4;
+    p->inner_reduction = 1e-2;
     p->residual_reduction = 1e-10;
     p->residual_tolerance = 0.0;
     p->block = 0;

@@ -738,7 +738,7 @@ class BatchTCD:
     `problems` = list of (n_side, eps, (b1, b2), c)."""
 
     def __init__(self, ctx, problems, example="tcd_exp", tau=0.01, t0=0.0, inner_solver="bi_cgstab",
-                 gmres_restart=30, max_inner=2000, max_outer=50, inner_reduction=1e-4,
+                 gmres_restart=30, max_inner=2000, max_outer=50, inner_reduction=1e-2,
                  residual_reduction=1e-10, residual_tolerance=0.0, block=0, max_groups=16):
         self.ctx = ctx
         prm = BatchParams()
