@@ -306,6 +306,14 @@ int pmx_time_reset_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_ma
  * (continuous, all DOFs active: an enclosed flow with pressure projection). */
 typedef struct pmx_nse pmx_nse;
 #define PMX_EX_NSE_U2P1 9   /* include/Examples/NSE_3D/test_u_2_p_1.h (Example_NSE3D -3) */
+#define PMX_EX_NSE_U0P0 13  /* include/Examples/NSE_3D/test_u_0_p_0.h (Example_NSE3D -1):
+                               u = (1,0,0), p = 0, Dirichlet boundary except the interior
+                               of the face x = 1 (Neumann, data 0) */
+/* pressure spaces (TDatabase PRESSURE_SPACE codes of
+ * NavierStokes.C:get_velocity_pressure_orders): Q1 continuous, or P1
+ * discontinuous (D_P1_3D_H_A, the pair of VELOCITY_SPACE 22) */
+#define PMX_NSE_PRESSURE_Q1 1
+#define PMX_NSE_PRESSURE_P1DISC (-110)
 #define PMX_NSE_BLOCK_A 0
 #define PMX_NSE_BLOCK_B 1   /* structure of B1, B2, B3 */
 #define PMX_NSE_BLOCK_B1 1
@@ -317,15 +325,30 @@ typedef struct {
   long long nnz_a, nnz_b;
   int n_combined;
   long long nnz_combined;
+  int pressure_space;     /* PMX_NSE_PRESSURE_Q1 or PMX_NSE_PRESSURE_P1DISC */
+  int n_p_cell;           /* pressure DOFs per cell: 8 or 4 */
+  int enclosed;           /* 1: no Neumann velocity DOF (pressure row correction and projection) */
 } pmx_nse_info;
 typedef struct {
-  int example;   /* PMX_EX_NSE_U2P1: right-hand side f = -nu lap u + (u.grad) u + grad p, g = 0 */
+  int example;   /* PMX_EX_NSE_U2P1 or PMX_EX_NSE_U0P0 (the example the system was
+                    created for): right-hand side f = -nu lap u + (u.grad) u + grad p, g = 0 */
   double nu;     /* viscosity 1/Re (coeff[0]) */
 } pmx_nse_params;
 /* NavierStokes<3> spaces and matrices on one level: TFESpace3D velocity (Q2) and
  * pressure (Q1), TStructure of A (src/Matrix/Structure.C:829-1277) and of B
- * (rectangular, :1709-1924), the combined matrix pattern. */
+ * (rectangular, :1709-1924), the combined matrix pattern.  The velocity space
+ * has Dirichlet conditions on the whole boundary (example PMX_EX_NSE_U2P1). */
 int pmx_nse_create(pmx_ctx* ctx, const pmx_domain* d, int level, pmx_nse** out);
+/* The same for a chosen pair and example, as the NavierStokes<3> constructor
+ * builds its spaces (src/System/NavierStokes.C:270-360, velocity space 2 or 22;
+ * TFESpace3D with the example's BoundCondition, evaluated at the centre of
+ * every boundary face, Dirichlet winning on shared DOFs).  pressure_space:
+ * PMX_NSE_PRESSURE_Q1 or PMX_NSE_PRESSURE_P1DISC; example: PMX_EX_NSE_U2P1 or
+ * PMX_EX_NSE_U0P0.  With Neumann velocity DOFs the flow is not enclosed
+ * (BlockFEMatrix.C:1869-1920): the combined matrix keeps its first pressure
+ * row, and the pressure is not projected. */
+int pmx_nse_create_pair(pmx_ctx* ctx, const pmx_domain* d, int level, int pressure_space, int example,
+                        pmx_nse** out);
 int pmx_nse_destroy(pmx_nse* s);
 int pmx_nse_get_info(const pmx_nse* s, pmx_nse_info* out);
 /* No reference counterpart (kernel choice): the velocity block of the block

@@ -64,7 +64,8 @@ class BatchReport(ctypes.Structure):
 class NseInfo(ctypes.Structure):
     _fields_ = [("n_cells", c_int), ("n_u", c_int), ("n_u_active", c_int), ("n_p", c_int),
                 ("nnz_a", ctypes.c_longlong), ("nnz_b", ctypes.c_longlong), ("n_combined", c_int),
-                ("nnz_combined", ctypes.c_longlong)]
+                ("nnz_combined", ctypes.c_longlong), ("pressure_space", c_int), ("n_p_cell", c_int),
+                ("enclosed", c_int)]
 
 
 class NseParams(ctypes.Structure):
@@ -162,6 +163,7 @@ SIGNATURES = {
     "pmx_batch_get_matrices": [c_void_p, c_int, P(c_double), P(c_double), P(c_double)],
     "pmx_batch_errors": [c_void_p, c_int, P(c_double)],
     "pmx_nse_create": [c_void_p, c_void_p, c_int, P(c_void_p)],
+    "pmx_nse_create_pair": [c_void_p, c_void_p, c_int, c_int, c_int, P(c_void_p)],
     "pmx_nse_destroy": [c_void_p],
     "pmx_nse_get_info": [c_void_p, P(NseInfo)],
     "pmx_nse_set_velocity_sell_rows": [c_void_p, c_int],

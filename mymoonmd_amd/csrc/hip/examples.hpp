@@ -16,6 +16,9 @@
 //   EX_NSE_U2P1_*       include/Examples/NSE_3D/test_u_2_p_1.h (NSE3D example -3):
 //                       u = (y^2, xz, -y^2), p = x + y + z - 1.5; one id per
 //                       component (U1, U2, U3, P) for values and errors
+//   EX_NSE_U0P0_*       include/Examples/NSE_3D/test_u_0_p_0.h (NSE3D example -1):
+//                       u = (1, 0, 0), p = 0; homogeneous Neumann data
+//                       (nu grad u - p I) n = 0 on the interior of the face x = 1
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -37,6 +40,10 @@ enum Example : int {
   EX_NSE_U2P1_U2 = 10,
   EX_NSE_U2P1_U3 = 11,
   EX_NSE_U2P1_P = 12,
+  EX_NSE_U0P0_U1 = 13,
+  EX_NSE_U0P0_U2 = 14,
+  EX_NSE_U0P0_U3 = 15,
+  EX_NSE_U0P0_P = 16,
 };
 
 constexpr double kPi = 3.14159265358979323846;
@@ -62,6 +69,10 @@ __host__ __device__ inline void example_exact(int ex, double x, double y, double
     case EX_NSE_U2P1_U2: v[0] = x * z; v[1] = z; v[3] = x; break;
     case EX_NSE_U2P1_U3: v[0] = -y * y; v[2] = -2 * y; v[4] = -2; break;
     case EX_NSE_U2P1_P: v[0] = x + y + z - 1.5; v[1] = 1; v[2] = 1; v[3] = 1; break;
+    case EX_NSE_U0P0_U1: v[0] = 1; break;
+    case EX_NSE_U0P0_U2:
+    case EX_NSE_U0P0_U3:
+    case EX_NSE_U0P0_P: break;
     case EX_SINE_LAPLACE_2D:
     case EX_SINE_2D: {
       const double sx = sin(kPi * x), sy = sin(kPi * y);

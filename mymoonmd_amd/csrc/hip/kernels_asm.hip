@@ -19,6 +19,7 @@
 // is race free without atomics.
 #include <algorithm>
 #include <mutex>
+#include <stdexcept>
 
 #include "common.hpp"
 #include "examples.hpp"
@@ -1108,12 +1109,16 @@ void cd_errors(const AssemblyData& ad, const CDCoeffs& co, const double* u, doub
     hipLaunchKernelGGL((k_cd_errors<27, 3>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
   else if (ad.dim == 3 && ad.ndof == 8)
     hipLaunchKernelGGL((k_cd_errors<8, 3>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
+  else if (ad.dim == 3 && ad.ndof == 4)   // discontinuous P1 on hexahedra
+    hipLaunchKernelGGL((k_cd_errors<4, 3>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
   else if (ad.dim == 2 && ad.ndof == 4)
     hipLaunchKernelGGL((k_cd_errors<4, 2>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
   else if (ad.dim == 2 && ad.ndof == 9)
     hipLaunchKernelGGL((k_cd_errors<9, 2>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
   else if (ad.dim == 2 && ad.ndof == 3)
     hipLaunchKernelGGL((k_cd_errors<3, 2>), dim3(nb), dim3(256), 0, st, ad, co, u, partial);
+  else
+    throw std::runtime_error("error norms: no kernel for this element");
   hipLaunchKernelGGL(k_reduce_errors, dim3(1), dim3(1024), 0, st, nb, partial, out4);
 }
 

@@ -16,6 +16,8 @@
 // velocity test function i (its A row, its three load entries and the column
 // i of B1..B3 for the 8 pressure rows); cells of one colour share no vertex, so
 // neither velocity nor pressure rows are written twice.
+#include <stdexcept>
+
 #include "common.hpp"
 #include "examples.hpp"
 #include "launch.hpp"
@@ -37,6 +39,9 @@ constexpr int NU = 27, NP = 8, SLOT = 32, CPB = 256 / SLOT;
 // reference table per quadrature point: w, xi[3], phi[27], dphi[3][27], psi[8]
 constexpr int RS = 1 + 3 + NU + 3 * NU + NP;
 
+// NPC: pressure DOFs per cell (8: Q1, 4: discontinuous P1); the table keeps 8
+// psi slots either way
+template <int NPC>
 __global__ __launch_bounds__(256) void k_assemble_nse(NseAsm d, const int* __restrict__ cells, int n) {
   __shared__ double s_u[CPB][NU];
   __shared__ double s_g[CPB][3][NU];
@@ -49,7 +54,7 @@ __global__ __launch_bounds__(256) void k_assemble_nse(NseAsm d, const int* __res
   const int cell = valid ? cells[ci] : 0;
   const double* geo = d.geo + (size_t)cell * kGeoStride;
   const int* vd = d.vdofs + (size_t)cell * NU;
-  const int* pd = d.pdofs + (size_t)cell * NP;
+  const int* pd = d.pdofs + (size_t)cell * NPC;
   double Jinv[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) Jinv[k] = geo[12 + k];
@@ -59,11 +64,11 @@ __global__ __launch_bounds__(256) void k_assemble_nse(NseAsm d, const int* __res
   double arow[NU];
 #pragma unroll
   for (int j = 0; j < NU; ++j) arow[j] = 0.0;
-  double bcol[3][NP];
+  double bcol[3][NPC];
 #pragma unroll
   for (int k = 0; k < 3; ++k)
 #pragma unroll
-    for (int q = 0; q < NP; ++q) bcol[k][q] = 0.0;
+    for (int q = 0; q < NPC; ++q) bcol[k][q] = 0.0;
   double frow[3] = {0, 0, 0}, grow = 0;
   __syncthreads();
   for (int q = 0; q < d.nq; ++q) {
@@ -116,12 +121,12 @@ __global__ __launch_bounds__(256) void k_assemble_nse(NseAsm d, const int* __res
       frow[2] += mult * ti * sc[2];
       const double* psi = rq + 4 + NU + 3 * NU;
 #pragma unroll
-      for (int p = 0; p < NP; ++p) {
+      for (int p = 0; p < NPC; ++p) {
         bcol[0][p] -= mult * psi[p] * tx;
         bcol[1][p] -= mult * psi[p] * ty;
         bcol[2][p] -= mult * psi[p] * tz;
       }
-      if (i < NP) grow += mult * psi[i] * sc[3];
+      if (i < NPC) grow += mult * psi[i] * sc[3];
     }
     __syncthreads();
   }
@@ -137,14 +142,14 @@ __global__ __launch_bounds__(256) void k_assemble_nse(NseAsm d, const int* __res
     d.A.val[find_col_n(d.A.col, d.A.row_ptr[r], d.A.row_ptr[r + 1], r)] = 1.0;
   }
 #pragma unroll
-  for (int p = 0; p < NP; ++p) {
+  for (int p = 0; p < NPC; ++p) {
     const int pr = pd[p];
     const int pos = find_col_n(d.Bcol, d.Brp[pr], d.Brp[pr + 1], r);
     d.B1[pos] += bcol[0][p];
     d.B2[pos] += bcol[1][p];
     d.B3[pos] += bcol[2][p];
   }
-  if (d.rhs && i < NP) d.rhs[(size_t)3 * d.n_u + pd[i]] += grow;
+  if (d.rhs && i < NPC) d.rhs[(size_t)3 * d.n_u + pd[i]] += grow;
 }
 
 // velocity Dirichlet rows of the rhs: the example's boundary values
@@ -178,7 +183,9 @@ __global__ void k_combined_values(long long nnz, const uint8_t* __restrict__ kin
   }
 }
 
-// integral and measure of a Q1 pressure over the cells (per-block partials)
+// integral and measure of the pressure over the cells (per-block partials);
+// NPC pressure DOFs per cell
+template <int NPC>
 __global__ __launch_bounds__(256) void k_pressure_integral(int n_cells, int nq, const double* __restrict__ ref,
                                                            const double* __restrict__ geo,
                                                            const int* __restrict__ pdofs,
@@ -189,13 +196,13 @@ __global__ __launch_bounds__(256) void k_pressure_integral(int n_cells, int nq, 
   double I = 0, m = 0;
   if (c < n_cells) {
     const double det = fabs(geo[(size_t)c * kGeoStride + 21]);
-    double pl[NP];
-    for (int j = 0; j < NP; ++j) pl[j] = p[pdofs[(size_t)c * NP + j]];
+    double pl[NPC];
+    for (int j = 0; j < NPC; ++j) pl[j] = p[pdofs[(size_t)c * NPC + j]];
     for (int q = 0; q < nq; ++q) {
       const double* rq = ref + (size_t)q * RS;
       const double* psi = rq + 4 + NU + 3 * NU;
       double v = 0;
-      for (int j = 0; j < NP; ++j) v += pl[j] * psi[j];
+      for (int j = 0; j < NPC; ++j) v += pl[j] * psi[j];
       I += rq[0] * det * v;
       m += rq[0] * det;
     }
@@ -229,8 +236,17 @@ __global__ __launch_bounds__(1024) void k_pressure_shift(int nb, const double* _
 // one smoothing step: r = rhs - K x, every cell solves its local system with
 // the local r, the damped local solutions are summed per DOF (in cell order)
 // and divided by the number of cells that touched the DOF.
+// A pressure element with 4 DOFs per cell (discontinuous P1) leaves local
+// rows 85..88 without a global DOF: their map entry is kVankaNone (sorted
+// last), the factorisation gives them a unit row and column, so they neither
+// pivot nor couple and the 85 real rows are factored exactly as an 85 x 85
+// system; their defects read as 0 and their updates are never accumulated.
 constexpr int NL = 3 * NU + NP;   // 89
 constexpr int LDL = NL + 1;       // LDS row stride
+
+__device__ __forceinline__ double vk_defect(const double* __restrict__ r, int g) {
+  return g != kVankaNone ? r[g] : 0.0;
+}
 
 // one workgroup (256 threads) per cell: gather, factor in LDS (FP64), store the
 // factors column-major (FP64 and/or FP32) and, in place of the pivots, the
@@ -261,6 +277,10 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
   __syncthreads();
   for (int a = t; a < NL; a += blockDim.x) {
     const int r = dm[a];
+    if (r == kVankaNone) {
+      M[a * LDL + a] = 1.0;
+      continue;
+    }
     for (int q = rp[r]; q < rp[r + 1]; ++q) {
       const int c = ci[q];
       int lo = 0, hi = NL - 1;
@@ -484,8 +504,8 @@ __global__ __launch_bounds__(256) void k_vanka_solve(int n_cells, const T* __res
   const int i0 = lane, i1 = lane + 64;
   const bool h1 = i1 < NL;
   const int j1 = h1 ? i1 : i0;
-  double b0 = r[pm[i0]];
-  double b1 = r[pm[j1]];
+  double b0 = vk_defect(r, pm[i0]);
+  double b1 = vk_defect(r, pm[j1]);
   if (!h1) b1 = 0.0;
   vanka_substitute<T>(lu + (size_t)cell * NL * NL, lane, b0, b1);
   upd[(size_t)cell * NL + i0] = damp * b0;
@@ -511,8 +531,8 @@ __global__ __launch_bounds__(256) void k_vanka_apply(int n_cells, const T* __res
   const int i0 = lane, i1 = lane + 64;
   const bool h1 = i1 < NL;
   const int j1 = h1 ? i1 : i0;
-  double b0 = r[pm[i0]];
-  double b1 = r[pm[j1]];
+  double b0 = vk_defect(r, pm[i0]);
+  double b1 = vk_defect(r, pm[j1]);
   if (!h1) b1 = 0.0;
   const T* X = inv + (size_t)cell * NL * NL;
   double x0 = 0.0, x1 = 0.0;
@@ -557,8 +577,8 @@ __global__ __launch_bounds__(256) void k_vanka_apply_bf16p(int n_cells, const ui
   const int* pm = pmap + (size_t)cell * NL;
   const int i1 = lane + 64;
   const bool h1 = i1 < NL;
-  double b0 = r[pm[lane]];
-  double b1 = r[pm[h1 ? i1 : lane]];
+  double b0 = vk_defect(r, pm[lane]);
+  double b1 = vk_defect(r, pm[h1 ? i1 : lane]);
   if (!h1) b1 = 0.0;
   const int q = lane < H ? lane : H - 1;
   const uint32_t* X = inv + (size_t)cell * (NLP / 2) * NL;   // NL columns of H words
@@ -602,8 +622,8 @@ __global__ __launch_bounds__(256) void k_vanka_apply_p(int n_cells, const T* __r
   const int* pm = pmap + (size_t)cell * NL;
   const int i1 = lane + 64;
   const bool h1 = i1 < NL;
-  double b0 = r[pm[lane]];
-  double b1 = r[pm[h1 ? i1 : lane]];
+  double b0 = vk_defect(r, pm[lane]);
+  double b1 = vk_defect(r, pm[h1 ? i1 : lane]);
   if (!h1) b1 = 0.0;
   const int q = lane < H ? lane : H - 1;
   const pair_t* X = reinterpret_cast<const pair_t*>(inv + (size_t)cell * NLP * NL);
@@ -679,6 +699,7 @@ __global__ __launch_bounds__(256) void k_vanka_colour(int n, const int* __restri
   const bool h1 = i1 < NL;
   const int j1 = h1 ? i1 : i0;
   auto defect = [&](int g) {
+    if (g == kVankaNone) return 0.0;
     double t = f[g];
     const int e = rp[g + 1];
     for (int k = rp[g]; k < e; ++k) t -= cv[k] * x[ci[k]];
@@ -689,8 +710,8 @@ __global__ __launch_bounds__(256) void k_vanka_colour(int n, const int* __restri
   if (!h1) b1 = 0.0;
   vanka_substitute<T>(lu + (size_t)cell * NL * NL, lane, b0, b1);
   const int* dm = map + (size_t)cell * NL;
-  x[dm[i0]] += damp * b0;
-  if (h1) x[dm[i1]] += damp * b1;
+  if (dm[i0] != kVankaNone) x[dm[i0]] += damp * b0;
+  if (h1 && dm[i1] != kVankaNone) x[dm[i1]] += damp * b1;
 }
 
 // x[i] += (sum of the cell updates of DOF i, in cell order) / their number
@@ -709,7 +730,10 @@ __global__ void k_vanka_accumulate(int n, const int* __restrict__ acc_ptr, const
 
 void assemble_nse(const NseAsm& d, const int* cells, int n, hipStream_t st) {
   const int grid = (n + CPB - 1) / CPB;
-  if (grid > 0) hipLaunchKernelGGL(k_assemble_nse, dim3(grid), dim3(256), 0, st, d, cells, n);
+  if (d.np != 8 && d.np != 4) throw std::runtime_error("pressure element with 8 or 4 DOFs per cell expected");
+  if (grid <= 0) return;
+  if (d.np == 4) hipLaunchKernelGGL(k_assemble_nse<4>, dim3(grid), dim3(256), 0, st, d, cells, n);
+  else hipLaunchKernelGGL(k_assemble_nse<8>, dim3(grid), dim3(256), 0, st, d, cells, n);
 }
 
 void nse_dirichlet_rhs(int example, int n_u, int active, const double* xyz, double* rhs, hipStream_t st) {
@@ -798,10 +822,15 @@ void vanka_accumulate(int n, const int* acc_ptr, const int* acc_idx, const doubl
     hipLaunchKernelGGL(k_vanka_accumulate, dim3((n + 255) / 256), dim3(256), 0, st, n, acc_ptr, acc_idx, upd, x);
 }
 
-void pressure_project_l20(int n_cells, int nq, const double* ref, const double* geo, const int* pdofs, int n_p,
-                          double* p, double* partial, hipStream_t st) {
+void pressure_project_l20(int n_cells, int nq, int np_cell, const double* ref, const double* geo, const int* pdofs,
+                          int n_p, double* p, double* partial, hipStream_t st) {
   const int nb = (n_cells + 255) / 256;
-  hipLaunchKernelGGL(k_pressure_integral, dim3(nb), dim3(256), 0, st, n_cells, nq, ref, geo, pdofs, p, partial);
+  if (np_cell == 4)
+    hipLaunchKernelGGL(k_pressure_integral<4>, dim3(nb), dim3(256), 0, st, n_cells, nq, ref, geo, pdofs, p, partial);
+  else if (np_cell == 8)
+    hipLaunchKernelGGL(k_pressure_integral<8>, dim3(nb), dim3(256), 0, st, n_cells, nq, ref, geo, pdofs, p, partial);
+  else
+    throw std::runtime_error("pressure element with 8 or 4 DOFs per cell expected");
   hipLaunchKernelGGL(k_pressure_shift, dim3(1), dim3(1024), 0, st, nb, partial, n_p, p);
 }
 

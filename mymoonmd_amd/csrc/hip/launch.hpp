@@ -262,11 +262,12 @@ struct CDCoeffs {
 // ---- Oseen / Navier-Stokes (kernels_nse.hip)
 struct NseAsm {
   int nq = 0, n_u = 0, n_u_active = 0, example = 0, convection = 0;
+  int np = 8;                    // pressure DOFs per cell: 8 (Q1) or 4 (discontinuous P1)
   double nu = 1;
   const double* ref = nullptr;   // per q: w, xi[3], phi[27], dphi[3][27], psi[8]
   const double* geo = nullptr;   // per cell (kGeoStride)
   const int* vdofs = nullptr;    // [cells][27]
-  const int* pdofs = nullptr;    // [cells][8]
+  const int* pdofs = nullptr;    // [cells][np]
   const double* w = nullptr;     // convection field [3][n_u] (null: Stokes)
   DevCSR A{};                    // velocity block (NSTYPE 1: shared by the 3 components)
   const int* Brp = nullptr;
@@ -280,10 +281,12 @@ void assemble_nse(const NseAsm& d, const int* cells, int n, hipStream_t st);
 void nse_dirichlet_rhs(int example, int n_u, int active, const double* xyz, double* rhs, hipStream_t st);
 void combined_values(long long nnz, const uint8_t* kind, const int* pos, const double* A, const double* B1,
                      const double* B2, const double* B3, double* out, hipStream_t st);
-void pressure_project_l20(int n_cells, int nq, const double* ref, const double* geo, const int* pdofs, int n_p,
-                          double* p, double* partial, hipStream_t st);
-// cell Vanka (Jacobi variant) on Q2/Q1 hexahedra: 89 local DOFs per cell
+void pressure_project_l20(int n_cells, int nq, int np_cell, const double* ref, const double* geo, const int* pdofs,
+                          int n_p, double* p, double* partial, hipStream_t st);
+// cell Vanka (Jacobi variant) on Q2 hexahedra: 89 local DOFs per cell (Q1
+// pressure); with discontinuous P1 pressure the last 4 are kVankaNone
 constexpr int kVankaLocal = 89;
+constexpr int kVankaNone = 0x7fffffff;
 // factor: LU of every cell's local matrix (gathered from the combined matrix C
 // by the cell's 89 sorted DOFs `map`); pmap[cell][i] = global DOF whose
 // defect enters row i of the row-interchanged local system

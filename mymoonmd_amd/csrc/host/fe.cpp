@@ -48,7 +48,48 @@ Element make_element(Shape shape, int order) {
   return e;
 }
 
+Element make_element_p1disc_hexa() {
+  Element e;
+  e.shape = SHAPE_HEXA;
+  e.order = 1;
+  e.dim = 3;
+  e.ndof = 4;
+  e.disc = true;
+  e.ref_nodes.assign(12, 0.0);   // no nodal points: the cell centre for every DOF
+  return e;
+}
+
+void p1disc_nf_points(double* xi24) {
+  const double a = 0.5773502691896257645091489;
+  for (int i = 0; i < 8; ++i) {
+    xi24[3 * i] = (i & 1) ? a : -a;
+    xi24[3 * i + 1] = (i & 2) ? a : -a;
+    xi24[3 * i + 2] = (i & 4) ? a : -a;
+  }
+}
+
+void p1disc_nf_apply(const double* v, double* out) {
+  double xi[24];
+  p1disc_nf_points(xi);
+  double s = 0;
+  for (int i = 0; i < 8; ++i) s += v[i];
+  out[0] = s * 0.125;
+  for (int d = 0; d < 3; ++d) {
+    double t = 0;
+    for (int i = 0; i < 8; ++i) t += v[i] * xi[3 * i + d];
+    out[1 + d] = t * 0.375;
+  }
+}
+
 void Element::eval(const double* xi, double* v, double* d1, double* d2) const {
+  if (disc) {
+    v[0] = 1; v[1] = xi[0]; v[2] = xi[1]; v[3] = xi[2];
+    if (d1)
+      for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < 4; ++k) d1[a * 4 + k] = (k == a + 1) ? 1.0 : 0.0;
+    if (d2) for (int i = 0; i < 24; ++i) d2[i] = 0;
+    return;
+  }
   if (shape == SHAPE_TRI) {
     v[0] = 1 - xi[0] - xi[1]; v[1] = xi[0]; v[2] = xi[1];
     if (d1) { d1[0] = -1; d1[1] = 1; d1[2] = 0; d1[3] = -1; d1[4] = 0; d1[5] = 1; }
@@ -89,6 +130,7 @@ void Element::eval(const double* xi, double* v, double* d1, double* d2) const {
 
 std::vector<int> Element::joint_dofs(int j) const {
   std::vector<int> out;
+  if (disc) return out;
   for (int k = 0; k < ndof; ++k) {
     const double* p = &ref_nodes[k * dim];
     bool on = false;

@@ -29,6 +29,10 @@ struct Element {
   int dim;
   int ndof;
   std::vector<double> ref_nodes;   // [ndof][dim] nodal points in reference coordinates
+  // discontinuous P1 on hexahedra (D_P1_3D_H_A): basis 1, xi, eta, zeta
+  // (include/FE3D/BF_D_H_P1_3D.h), every DOF inner to its cell, nodal
+  // functionals from the 8 points (+-1/sqrt 3)^3 (include/FE3D/NF_D_H_P1_3D.h)
+  bool disc = false;
 
   // basis values at one reference point:
   //  v[ndof], d1[dim][ndof] (d/dxi, d/deta, d/dzeta),
@@ -40,6 +44,13 @@ struct Element {
 };
 
 Element make_element(Shape shape, int order);
+Element make_element_p1disc_hexa();
+
+// Nodal functionals of the discontinuous P1 hexahedron: F0 = mean of the
+// values at the 8 points (+-1/sqrt 3)^3 (xi fastest), F1..F3 = 0.375 sum of
+// value x the point's xi / eta / zeta (NF_D_H_P1_3D_EvalAll).
+void p1disc_nf_points(double* xi24);
+void p1disc_nf_apply(const double* values8, double* out4);
 
 struct QuadRule {
   int n = 0;

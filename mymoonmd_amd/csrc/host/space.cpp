@@ -145,6 +145,33 @@ FESpace build_space(const Mesh& m, int level, int order, const BoundaryCondition
   return s;
 }
 
+FESpace build_space_p1disc(const Mesh& m, int level) {
+  if (level < 0 || level >= (int)m.levels.size()) throw std::runtime_error("mesh level out of range");
+  if (m.shape != SHAPE_HEXA) throw std::runtime_error("the discontinuous P1 space is built on hexahedra");
+  FESpace s;
+  s.mesh = &m;
+  s.level = level;
+  s.el = make_element_p1disc_hexa();
+  s.n_cells = m.levels[level].n_cells;
+  s.ndof_cell = 4;
+  s.n_dof = 4 * s.n_cells;
+  s.n_inner = s.n_dof;
+  s.active_bound = s.n_dof;
+  s.global_numbers.resize((size_t)s.n_dof);
+  for (int k = 0; k < s.n_dof; ++k) s.global_numbers[k] = k;
+  s.dof_xyz.assign((size_t)s.n_dof * 3, 0.0);
+  s.dof_class.assign(s.n_dof, 0);
+  for (int c = 0; c < s.n_cells; ++c) {
+    const AffineMap a = affine_map(m, level, c);
+    const double xi[3] = {0, 0, 0};
+    double x[3];
+    map_to_phys(a, xi, x);
+    for (int k = 0; k < 4; ++k)
+      for (int d = 0; d < 3; ++d) s.dof_xyz[((size_t)c * 4 + k) * 3 + d] = x[d];
+  }
+  return s;
+}
+
 CSRPattern build_rect_structure(const FESpace& t, const FESpace& a) {
   if (t.n_cells != a.n_cells) throw std::runtime_error("spaces are not on the same cells");
   const int nt = t.ndof_cell, na = a.ndof_cell;
@@ -238,6 +265,7 @@ void build_prolongation(const FESpace& coarse, const FESpace& fine, CSRPattern& 
       const int g = fine.global_numbers[(size_t)fc * ndf + k];
       if (own[g] < 0) own[g] = fc * ndf + k;
     }
+  if (fine.el.disc != coarse.el.disc) throw std::runtime_error("prolongation between different elements");
   auto row_of = [&](int g, std::vector<std::pair<int, double>>& row) {
     row.clear();
     const int fc = own[g] / ndf, k = own[g] % ndf;
@@ -245,9 +273,26 @@ void build_prolongation(const FESpace& coarse, const FESpace& fine, CSRPattern& 
     const AffineMap af = affine_map(m, fine.level, fc);
     const AffineMap ac = affine_map(m, coarse.level, pc);
     double x[3], xi[3], v[64];
-    map_to_phys(af, &fine.el.ref_nodes[k * dim], x);
-    map_to_ref(ac, x, xi);
-    coarse.el.eval(xi, v, nullptr, nullptr);
+    if (fine.el.disc) {
+      // fine nodal functional k of every coarse basis function
+      double pts[24], pv[8][4];
+      p1disc_nf_points(pts);
+      for (int i = 0; i < 8; ++i) {
+        map_to_phys(af, pts + 3 * i, x);
+        map_to_ref(ac, x, xi);
+        coarse.el.eval(xi, pv[i], nullptr, nullptr);
+      }
+      for (int l = 0; l < ndc; ++l) {
+        double vals8[8], f[4];
+        for (int i = 0; i < 8; ++i) vals8[i] = pv[i][l];
+        p1disc_nf_apply(vals8, f);
+        v[l] = f[k];
+      }
+    } else {
+      map_to_phys(af, &fine.el.ref_nodes[k * dim], x);
+      map_to_ref(ac, x, xi);
+      coarse.el.eval(xi, v, nullptr, nullptr);
+    }
     for (int l = 0; l < ndc; ++l) {
       // basis values at refined nodal points are exact binary fractions;
       // snap round-off of the inverse map

@@ -58,6 +58,11 @@ struct FESpace {
 };
 
 FESpace build_space(const Mesh& m, int level, int order, const BoundaryConditions& bc);
+// Discontinuous P1 on the hexahedra of a level (TFESpace3D with D_P1_3D_H_A):
+// every DOF is an inner DOF of one cell, numbered in cell order and local
+// order (FESpace3D.C:1486-1492 counts them per cell; the renumbering pass of
+// :1835-1880 keeps the order of first occurrence): global = 4 cell + local.
+FESpace build_space_p1disc(const Mesh& m, int level);
 
 struct CSRPattern {
   int n_rows = 0, n_cols = 0;
@@ -79,6 +84,11 @@ CSRPattern build_rect_structure(const FESpace& test, const FESpace& ansatz);
 // (src/Multigrid/GridTransfer.C:665-941) equals P*u (the reference averages
 // identical per-child values by multiplicity), and DefectRestriction
 // (src/Multigrid/GridTransfer.C:943-1230) equals P^T*d.
+// For the discontinuous P1 space the row of fine DOF k of a child is the
+// child's nodal functional k applied to the parent's basis functions
+// (TFEDatabase3D::GetProlongationMatrix3D evaluates the fine nodal
+// functionals on the coarse basis); each fine DOF has one cell, so nothing is
+// averaged.
 void build_prolongation(const FESpace& coarse, const FESpace& fine, CSRPattern& P,
                         std::vector<double>& vals);
 // Lexicographic (z, y, x) order of the DOFs of a space, active DOFs first and

@@ -116,10 +116,11 @@ void nse_refresh(pmx_nse* s, hipStream_t st) {
 
 namespace {
 
-// reference table of the Q2/Q1 pair at the quadrature points of the velocity rule
-std::vector<double> nse_ref_table(const pmx::QuadRule& rule) {
+// reference table of the velocity/pressure pair at the quadrature points of
+// the velocity rule: w, xi[3], phi[27], dphi[3][27], psi[8] (a 4-DOF
+// pressure element fills psi[0..3], the rest stays 0)
+std::vector<double> nse_ref_table(const pmx::QuadRule& rule, const pmx::Element& ep) {
   const pmx::Element ev = pmx::make_element(pmx::SHAPE_HEXA, 2);
-  const pmx::Element ep = pmx::make_element(pmx::SHAPE_HEXA, 1);
   const int rs = 1 + 3 + 27 + 81 + 8;
   std::vector<double> t((size_t)rs * rule.n, 0.0);
   std::vector<double> d2(6 * 27), pv(8), pg(24), pd2(48);
@@ -129,14 +130,26 @@ std::vector<double> nse_ref_table(const pmx::QuadRule& rule) {
     for (int a = 0; a < 3; ++a) r[1 + a] = rule.xi[3 * q + a];
     ev.eval(&rule.xi[3 * q], r + 4, r + 4 + 27, d2.data());
     ep.eval(&rule.xi[3 * q], pv.data(), pg.data(), pd2.data());
-    for (int j = 0; j < 8; ++j) r[4 + 27 + 81 + j] = pv[j];
+    for (int j = 0; j < ep.ndof; ++j) r[4 + 27 + 81 + j] = pv[j];
   }
   return t;
 }
 
+// example id of component k (0..2 velocity, 3 pressure)
+int nse_component(int example, int k) {
+  return (example == PMX_EX_NSE_U0P0 ? pmx::EX_NSE_U0P0_U1 : pmx::EX_NSE_U2P1_U1) + k;
+}
+
+// test_u_0_p_0.h on_front_face_interior: the face x = 1 without its boundary
+bool front_face_interior(double x, double y, double z) {
+  const double tol = 1e-10;
+  return std::fabs(1 - x) < tol && !(std::fabs(y - 1) < tol) && !(std::fabs(y) < tol) &&
+         !(std::fabs(z - 1) < tol) && !(std::fabs(z) < tol);
+}
+
 // scalar reference table (w, xi, phi, dphi, d2phi) for the error kernel
 double* scalar_ref(pmx_nse* s, const pmx::FESpace& fs, int* nq) {
-  const int key = fs.el.order;
+  const int key = fs.el.disc ? -fs.el.order : fs.el.order;
   auto it = s->err_ref.find(key);
   const pmx::QuadRule rule = pmx::assembly_rule(fs.el, 0);
   *nq = rule.n;
@@ -159,21 +172,40 @@ double* scalar_ref(pmx_nse* s, const pmx::FESpace& fs, int* nq) {
 extern "C" {
 
 int pmx_nse_create(pmx_ctx* ctx, const pmx_domain* d, int level, pmx_nse** out) {
+  return pmx_nse_create_pair(ctx, d, level, PMX_NSE_PRESSURE_Q1, PMX_EX_NSE_U2P1, out);
+}
+
+int pmx_nse_create_pair(pmx_ctx* ctx, const pmx_domain* d, int level, int pressure_space, int example,
+                        pmx_nse** out) {
   return guarded([&] {
     require(ctx && d && out, "NULL argument");
-    require(d->mesh.shape == pmx::SHAPE_HEXA, "the saddle-point path is implemented for hexahedra (Q2/Q1)");
+    require(d->mesh.shape == pmx::SHAPE_HEXA, "the saddle-point path is implemented for hexahedra");
     require(level >= 0 && level < (int)d->mesh.levels.size(), "level out of range");
+    require(pressure_space == PMX_NSE_PRESSURE_Q1 || pressure_space == PMX_NSE_PRESSURE_P1DISC,
+            "pressure space must be PMX_NSE_PRESSURE_Q1 or PMX_NSE_PRESSURE_P1DISC");
+    require(example == PMX_EX_NSE_U2P1 || example == PMX_EX_NSE_U0P0, "unknown Navier-Stokes example");
     auto s = std::make_unique<pmx_nse>();
     s->ctx = ctx;
     s->dom = d;
     s->level = level;
+    s->pressure_space = pressure_space;
+    s->example = example;
     pmx::BoundaryConditions vbc, pbc;
     vbc.cond2d = [](int, double) { return (int)pmx::BC_DIRICHLET; };
-    vbc.cond3d = [](double, double, double) { return (int)pmx::BC_DIRICHLET; };
+    if (example == PMX_EX_NSE_U0P0)
+      vbc.cond3d = [](double x, double y, double z) {
+        return (int)(front_face_interior(x, y, z) ? pmx::BC_NEUMANN : pmx::BC_DIRICHLET);
+      };
+    else
+      vbc.cond3d = [](double, double, double) { return (int)pmx::BC_DIRICHLET; };
     pbc.cond2d = [](int, double) { return (int)pmx::BC_NEUMANN; };
     pbc.cond3d = [](double, double, double) { return (int)pmx::BC_NEUMANN; };
     s->vs = pmx::build_space(d->mesh, level, 2, vbc);
-    s->ps = pmx::build_space(d->mesh, level, 1, pbc);
+    s->ps = (pressure_space == PMX_NSE_PRESSURE_P1DISC) ? pmx::build_space_p1disc(d->mesh, level)
+                                                        : pmx::build_space(d->mesh, level, 1, pbc);
+    s->np_cell = s->ps.ndof_cell;
+    // BlockFEMatrix.C:1869-1920: an enclosed flow has no Neumann or Robin velocity DOF
+    s->enclosed = s->vs.n_bound[pmx::BC_NEUMANN - 1] == 0 && s->vs.n_bound[pmx::BC_ROBIN - 1] == 0;
     s->n_u = s->vs.n_dof;
     s->n_u_active = s->vs.active_bound;
     s->n_p = s->ps.n_dof;
@@ -190,8 +222,8 @@ int pmx_nse_create(pmx_ctx* ctx, const pmx_domain* d, int level, pmx_nse** out) 
       for (size_t k = 0; k < tpos.size(); ++k) s->BTsrc[k] = (int)tpos[k];
     }
     // combined matrix (BlockFEMatrix::get_combined_matrix): Dirichlet velocity
-    // rows are unit rows, the first pressure row becomes a unit row (pressure
-    // projection of an enclosed flow, BlockFEMatrix.C:1237-1300)
+    // rows are unit rows; for an enclosed flow the first pressure row becomes
+    // a unit row (pressure projection, BlockFEMatrix.C:1237-1300)
     {
       const int nu = s->n_u, na = s->n_u_active, np = s->n_p;
       const pmx::CSRPattern& A = *Ap;
@@ -222,7 +254,7 @@ int pmx_nse_create(pmx_ctx* ctx, const pmx_domain* d, int level, pmx_nse** out) 
       }
       for (int r = 0; r < np; ++r) {
         const int row = 3 * nu + r;
-        if (r == 0) {
+        if (r == 0 && s->enclosed) {
           col.push_back(row);
           s->Ckind.push_back(1);
           s->Cpos.push_back(0);
@@ -272,7 +304,7 @@ int pmx_nse_create(pmx_ctx* ctx, const pmx_domain* d, int level, pmx_nse** out) 
     s->d_vxyz = upload(s->vs.dof_xyz.data(), s->vs.dof_xyz.size(), st);
     const pmx::QuadRule rule = pmx::assembly_rule(s->vs.el, 0);   // Gauss4 (degree 3 * 2)
     s->nq = rule.n;
-    const std::vector<double> ref = nse_ref_table(rule);
+    const std::vector<double> ref = nse_ref_table(rule, s->ps.el);
     s->d_ref = upload(ref.data(), ref.size(), st);
     s->d_Brp = upload(s->Bp.row_ptr.data(), s->Bp.row_ptr.size(), st);
     s->d_Bcol = upload(s->Bp.col.data(), s->Bp.col.size(), st);
@@ -319,6 +351,9 @@ int pmx_nse_get_info(const pmx_nse* s, pmx_nse_info* out) {
     out->nnz_b = (long long)s->Bp.col.size();
     out->n_combined = s->n_comb;
     out->nnz_combined = (long long)s->Cp.col.size();
+    out->pressure_space = s->pressure_space;
+    out->n_p_cell = s->np_cell;
+    out->enclosed = s->enclosed ? 1 : 0;
   });
 }
 
@@ -326,7 +361,7 @@ int pmx_nse_assemble(pmx_ctx* ctx, pmx_nse* s, const pmx_nse_params* p, const do
   return guarded([&] {
     require(ctx && s && p, "NULL argument");
     require(s->ctx == ctx, "system was created on another context");
-    require(p->example == PMX_EX_NSE_U2P1, "unknown Navier-Stokes example");
+    require(p->example == s->example, "the system was created for another example (boundary conditions)");
     hipStream_t st = ctx->stream;
     const size_t na = s->A->pattern->col.size(), nb = s->Bp.col.size();
     pmx::vec_fill((int)na, 0.0, s->A->A.val, st);
@@ -339,7 +374,8 @@ int pmx_nse_assemble(pmx_ctx* ctx, pmx_nse* s, const pmx_nse_params* p, const do
     a.nq = s->nq;
     a.n_u = s->n_u;
     a.n_u_active = s->n_u_active;
-    a.example = pmx::EX_NSE_U2P1_U1;
+    a.example = nse_component(s->example, 0);
+    a.np = s->np_cell;
     a.nu = p->nu;
     a.convection = dev_w != nullptr;
     a.ref = s->d_ref;
@@ -356,7 +392,7 @@ int pmx_nse_assemble(pmx_ctx* ctx, pmx_nse* s, const pmx_nse_params* p, const do
     a.rhs = dev_rhs;
     for (size_t k = 0; k + 1 < s->color_ptr.size(); ++k)
       pmx::assemble_nse(a, s->d_cells + s->color_ptr[k], s->color_ptr[k + 1] - s->color_ptr[k], st);
-    if (dev_rhs) pmx::nse_dirichlet_rhs(pmx::EX_NSE_U2P1_U1, s->n_u, s->n_u_active, s->d_vxyz, dev_rhs, st);
+    if (dev_rhs) pmx::nse_dirichlet_rhs(nse_component(s->example, 0), s->n_u, s->n_u_active, s->d_vxyz, dev_rhs, st);
     nse_refresh(s, st);
     HIP_CHECK(hipGetLastError());
   });
@@ -424,14 +460,36 @@ int pmx_nse_get_global_numbers(const pmx_nse* s, int* host_velocity, int* host_p
 int pmx_nse_interpolate(pmx_ctx* ctx, const pmx_nse* s, int example, double* dev_x) {
   return guarded([&] {
     require(ctx && s && dev_x, "NULL argument");
-    require(example == PMX_EX_NSE_U2P1, "unknown Navier-Stokes example");
+    require(example == PMX_EX_NSE_U2P1 || example == PMX_EX_NSE_U0P0, "unknown Navier-Stokes example");
     hipStream_t st = ctx->stream;
     for (int k = 0; k < 3; ++k)
-      pmx::interpolate_example(pmx::EX_NSE_U2P1_U1 + k, 0.0, s->n_u, s->d_vxyz, dev_x + (size_t)k * s->n_u, st);
-    double* pxyz = upload(s->ps.dof_xyz.data(), s->ps.dof_xyz.size(), st);
-    pmx::interpolate_example(pmx::EX_NSE_U2P1_P, 0.0, s->n_p, pxyz, dev_x + (size_t)3 * s->n_u, st);
-    HIP_CHECK(hipStreamSynchronize(st));
-    dfree(pxyz);
+      pmx::interpolate_example(nse_component(example, k), 0.0, s->n_u, s->d_vxyz, dev_x + (size_t)k * s->n_u, st);
+    if (s->ps.el.disc) {
+      // nodal functionals of every cell (NF_D_H_P1_3D_EvalAll) on the host
+      std::vector<double> p((size_t)s->n_p);
+      double pts[24];
+      pmx::p1disc_nf_points(pts);
+      for (int c = 0; c < s->ps.n_cells; ++c) {
+        const pmx::AffineMap a = pmx::affine_map(s->dom->mesh, s->level, c);
+        double v8[8], f[4];
+        for (int i = 0; i < 8; ++i) {
+          double x[3], e[6];
+          pmx::map_to_phys(a, pts + 3 * i, x);
+          pmx::example_exact(nse_component(example, 3), x[0], x[1], x[2], e);
+          v8[i] = e[0];
+        }
+        pmx::p1disc_nf_apply(v8, f);
+        for (int k = 0; k < 4; ++k) p[s->ps.global_numbers[(size_t)c * 4 + k]] = f[k];
+      }
+      HIP_CHECK(hipMemcpyAsync(dev_x + (size_t)3 * s->n_u, p.data(), p.size() * sizeof(double),
+                               hipMemcpyHostToDevice, st));
+      HIP_CHECK(hipStreamSynchronize(st));
+    } else {
+      double* pxyz = upload(s->ps.dof_xyz.data(), s->ps.dof_xyz.size(), st);
+      pmx::interpolate_example(nse_component(example, 3), 0.0, s->n_p, pxyz, dev_x + (size_t)3 * s->n_u, st);
+      HIP_CHECK(hipStreamSynchronize(st));
+      dfree(pxyz);
+    }
     HIP_CHECK(hipGetLastError());
   });
 }
@@ -469,7 +527,7 @@ int pmx_nse_restrict_velocity(pmx_ctx* ctx, pmx_nse* coarse, const pmx_nse* fine
 int pmx_nse_project_pressure(pmx_ctx* ctx, pmx_nse* s, double* dev_x) {
   return guarded([&] {
     require(ctx && s && dev_x, "NULL argument");
-    pmx::pressure_project_l20(s->vs.n_cells, s->nq, s->d_ref, s->d_geo, s->d_pdofs, s->n_p,
+    pmx::pressure_project_l20(s->vs.n_cells, s->nq, s->np_cell, s->d_ref, s->d_geo, s->d_pdofs, s->n_p,
                               dev_x + (size_t)3 * s->n_u, s->d_partial, ctx->stream);
     HIP_CHECK(hipGetLastError());
   });
@@ -478,7 +536,7 @@ int pmx_nse_project_pressure(pmx_ctx* ctx, pmx_nse* s, double* dev_x) {
 int pmx_nse_errors(pmx_ctx* ctx, pmx_nse* s, int example, const double* dev_x, double* host_err) {
   return guarded([&] {
     require(ctx && s && dev_x && host_err, "NULL argument");
-    require(example == PMX_EX_NSE_U2P1, "unknown Navier-Stokes example");
+    require(example == PMX_EX_NSE_U2P1 || example == PMX_EX_NSE_U0P0, "unknown Navier-Stokes example");
     hipStream_t st = ctx->stream;
     double l2u = 0, h1u = 0;
     for (int k = 0; k < 4; ++k) {
@@ -491,7 +549,7 @@ int pmx_nse_errors(pmx_ctx* ctx, pmx_nse* s, int example, const double* dev_x, d
       ad.geo = s->d_geo;
       ad.dofs = (k < 3) ? s->d_vdofs : s->d_pdofs;
       pmx::CDCoeffs co;
-      co.example = pmx::EX_NSE_U2P1_U1 + k;
+      co.example = nse_component(example, k);
       const int nb = (ad.n_cells + 255) / 256;
       double* partial = dmalloc<double>((size_t)nb * 4);
       pmx::cd_errors(ad, co, dev_x + (size_t)k * s->n_u, partial, ctx->scal, st);

@@ -12,7 +12,11 @@ struct pmx_nse {
   pmx_ctx* ctx = nullptr;
   const pmx_domain* dom = nullptr;
   int level = 0;
-  pmx::FESpace vs, ps;           // velocity Q2 (all Dirichlet), pressure Q1 (all Neumann)
+  pmx::FESpace vs, ps;           // velocity Q2, pressure Q1 (continuous) or discontinuous P1
+  int pressure_space = 1;        // PMX_NSE_PRESSURE_Q1 or PMX_NSE_PRESSURE_P1DISC
+  int np_cell = 8;               // pressure DOFs per cell
+  int example = 0;               // example whose boundary conditions built the velocity space
+  bool enclosed = true;          // no Neumann/Robin velocity DOF: pressure row correction and projection
   int n_u = 0, n_u_active = 0, n_p = 0, n_comb = 0;
   pmx_matrix* A = nullptr;       // velocity block (+ SELL copy)
   pmx::CSRPattern Bp, BTp, Cp;   // B (p x u), B^T (u x p), combined matrix

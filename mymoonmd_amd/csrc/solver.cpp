@@ -1714,25 +1714,31 @@ int pmx_solver_update_nse(pmx_solver* s, pmx_nse* const* levels, int n_levels) {
         const bool vanka = !(l == 0 && s->p.mg_smoother_coarse == PMX_SMOOTHER_DIRECT);
         if (vanka) {
           // local DOF maps: [u1 | u2 | u3 | p] of every cell, sorted per block
-          const int NLc = pmx::kVankaLocal;
-          std::vector<int> map((size_t)L.n_cells * NLc);
+          // (VankaSmoother.C:44-105 collects the cell's DOFs block by block);
+          // a 4-DOF pressure element leaves the last 4 entries kVankaNone
+          const int NLc = pmx::kVankaLocal, npc = sys->np_cell;
+          require(npc == 8 || npc == 4, "cell Vanka needs 8 or 4 pressure DOFs per cell");
+          std::vector<int> map((size_t)L.n_cells * NLc, pmx::kVankaNone);
           for (int c = 0; c < L.n_cells; ++c) {
             int* mc = &map[(size_t)c * NLc];
             std::vector<int> v(sys->vs.global_numbers.begin() + (size_t)c * 27,
                                sys->vs.global_numbers.begin() + (size_t)(c + 1) * 27);
-            std::vector<int> p(sys->ps.global_numbers.begin() + (size_t)c * 8,
-                               sys->ps.global_numbers.begin() + (size_t)(c + 1) * 8);
+            std::vector<int> p(sys->ps.global_numbers.begin() + (size_t)c * npc,
+                               sys->ps.global_numbers.begin() + (size_t)(c + 1) * npc);
             std::sort(v.begin(), v.end());
             std::sort(p.begin(), p.end());
             for (int k = 0; k < 3; ++k)
               for (int j = 0; j < 27; ++j) mc[k * 27 + j] = k * L.n_u + v[j];
-            for (int j = 0; j < 8; ++j) mc[81 + j] = 3 * L.n_u + p[j];
+            for (int j = 0; j < npc; ++j) mc[81 + j] = 3 * L.n_u + p[j];
           }
-          std::vector<int> acc_ptr(L.n + 1, 0), acc_idx(map.size());
-          for (int d : map) acc_ptr[d + 1]++;
+          std::vector<int> acc_ptr(L.n + 1, 0), acc_idx;
+          for (int d : map)
+            if (d != pmx::kVankaNone) acc_ptr[d + 1]++;
           for (int i = 0; i < L.n; ++i) acc_ptr[i + 1] += acc_ptr[i];
+          acc_idx.resize(acc_ptr[L.n]);
           std::vector<int> fill(acc_ptr.begin(), acc_ptr.end() - 1);
-          for (size_t k = 0; k < map.size(); ++k) acc_idx[fill[map[k]]++] = (int)k;
+          for (size_t k = 0; k < map.size(); ++k)
+            if (map[k] != pmx::kVankaNone) acc_idx[fill[map[k]]++] = (int)k;
           L.map = upload(map.data(), map.size(), st);
           L.acc_ptr = upload(acc_ptr.data(), acc_ptr.size(), st);
           L.acc_idx = upload(acc_idx.data(), acc_idx.size(), st);

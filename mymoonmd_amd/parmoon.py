@@ -990,26 +990,36 @@ class POD:
             self.h = None
 
 
-NSE_EXAMPLES = {"u2_p1": 9}
+NSE_EXAMPLES = {"u2_p1": 9, "u0_p0": 13}
+# pressure spaces: Q1 (continuous, velocity space 2) and discontinuous P1
+# (D_P1_3D_H_A, velocity space 22), as NavierStokes.C:get_velocity_pressure_orders pairs them
+NSE_PRESSURE = {"q1": 1, "p1disc": -110}
 NSE_BLOCK = {"A": 0, "B": 1, "B1": 1, "B2": 2, "B3": 3, "combined": 4}
 
 
 class NSESystem:
-    """One level of NavierStokes<3> with Q2/Q1 on hexahedra and the NSTYPE-1
-    BlockFEMatrix (pmx_nse_*): assemble, apply, structures and values,
-    interpolation of the example, errors, pressure projection."""
+    """One level of NavierStokes<3> with Q2 velocity and Q1 or discontinuous P1
+    pressure on hexahedra and the NSTYPE-1 BlockFEMatrix (pmx_nse_*):
+    assemble, apply, structures and values, interpolation of the example,
+    errors, pressure projection.  `example` fixes the velocity boundary
+    conditions ("u2_p1": Dirichlet everywhere; "u0_p0": Neumann on the
+    interior of the face x = 1)."""
 
-    def __init__(self, ctx, domain, level):
+    def __init__(self, ctx, domain, level, pressure="q1", example="u2_p1"):
         self.ctx = ctx
         self.domain = domain
+        self.example = example
         self.h = c_void_p()
-        call("pmx_nse_create", ctx.h, domain.h, level, ctypes.byref(self.h))
+        call("pmx_nse_create_pair", ctx.h, domain.h, level, NSE_PRESSURE[pressure], NSE_EXAMPLES[example],
+             ctypes.byref(self.h))
         inf = NseInfo()
         call("pmx_nse_get_info", self.h, ctypes.byref(inf))
         self.info = inf
         self.n_u, self.n_u_active, self.n_p = inf.n_u, inf.n_u_active, inf.n_p
         self.n = inf.n_combined
         self.n_cells = inf.n_cells
+        self.n_p_cell = inf.n_p_cell
+        self.enclosed = bool(inf.enclosed)
 
     def _v(self, t, what):
         return _ptr(t, self.n, self.ctx.device, what)
@@ -1022,10 +1032,10 @@ class NSESystem:
         """rows from which the velocity block runs on its SELL copy (default 131,072)"""
         call("pmx_nse_set_velocity_sell_rows", self.h, min_rows)
 
-    def assemble(self, example="u2_p1", nu=1.0, w=None, rhs=None):
+    def assemble(self, example=None, nu=1.0, w=None, rhs=None):
         import torch
         torch.cuda.synchronize()
-        p = NseParams(NSE_EXAMPLES[example], nu)
+        p = NseParams(NSE_EXAMPLES[example or self.example], nu)
         call("pmx_nse_assemble", self.ctx.h, self.h, ctypes.byref(p),
              _ptr(w, 3 * self.n_u, self.ctx.device, "w") if w is not None else None,
              self._v(rhs, "rhs") if rhs is not None else None)
@@ -1063,15 +1073,15 @@ class NSESystem:
 
     def global_numbers(self):
         gv = np.zeros(self.n_cells * 27, np.int32)
-        gp = np.zeros(self.n_cells * 8, np.int32)
+        gp = np.zeros(self.n_cells * self.n_p_cell, np.int32)
         P = ctypes.POINTER(c_int)
         call("pmx_nse_get_global_numbers", self.h, gv.ctypes.data_as(P), gp.ctypes.data_as(P))
-        return gv.reshape(-1, 27), gp.reshape(-1, 8)
+        return gv.reshape(-1, 27), gp.reshape(-1, self.n_p_cell)
 
-    def interpolate(self, x, example="u2_p1"):
+    def interpolate(self, x, example=None):
         import torch
         torch.cuda.synchronize()
-        call("pmx_nse_interpolate", self.ctx.h, self.h, NSE_EXAMPLES[example], self._v(x, "x"))
+        call("pmx_nse_interpolate", self.ctx.h, self.h, NSE_EXAMPLES[example or self.example], self._v(x, "x"))
         self.ctx.synchronize()
 
     def project_pressure(self, x):
@@ -1080,11 +1090,11 @@ class NSESystem:
         call("pmx_nse_project_pressure", self.ctx.h, self.h, self._v(x, "x"))
         self.ctx.synchronize()
 
-    def errors(self, x, example="u2_p1"):
+    def errors(self, x, example=None):
         import torch
         torch.cuda.synchronize()
         out = (c_double * 5)()
-        call("pmx_nse_errors", self.ctx.h, self.h, NSE_EXAMPLES[example], self._v(x, "x"), out)
+        call("pmx_nse_errors", self.ctx.h, self.h, NSE_EXAMPLES[example or self.example], self._v(x, "x"), out)
         return list(out)
 
     def restrict_velocity(self, fine, u_fine, u_coarse):
@@ -1102,9 +1112,10 @@ class NSESystem:
 
 
 class NavierStokes:
-    """NavierStokes<3> (src/System/NavierStokes.C) for NSTYPE 1, Q2/Q1 on the
-    unit cube: one NSESystem per multigrid level (finest last), the nonlinear
-    loop of the reference's driver (test/nse3d_solvers_test.c++:107-126):
+    """NavierStokes<3> (src/System/NavierStokes.C) for NSTYPE 1, Q2/Q1 or
+    Q2/P1disc (pressure="p1disc") on the unit cube: one NSESystem per
+    multigrid level (finest last), the nonlinear loop of the reference's
+    driver (test/nse3d_solvers_test.c++:107-126):
 
         ns.assemble_linear_terms(); ns.stop_it(0)
         for k in 1, 2, ...: ns.solve(); ns.assemble_nonlinear_term(); if ns.stop_it(k): break
@@ -1115,12 +1126,12 @@ class NavierStokes:
     nonlinloop_residual_relative_to_rhs (defaults of
     src/General/ParameterDatabase.C:1294-1330)."""
 
-    def __init__(self, ctx, domain, levels, solver_params, nu=1.0, example="u2_p1", **db):
+    def __init__(self, ctx, domain, levels, solver_params, nu=1.0, example="u2_p1", pressure="q1", **db):
         import torch
         self.ctx = ctx
         self.example = example
         self.nu = nu
-        self.systems = [NSESystem(ctx, domain, l) for l in levels]
+        self.systems = [NSESystem(ctx, domain, l, pressure, example) for l in levels]
         self.solution = [s.zeros() for s in self.systems]
         f = self.systems[-1]
         self.rhs = f.zeros()
@@ -1164,7 +1175,8 @@ class NavierStokes:
         nu = f.n_u
         self.defect.copy_(self.rhs)
         f.apply_scaled_add(self.solution[-1], self.defect, -1.0)
-        f.project_pressure(self.defect)
+        if f.enclosed:   # BlockFEMatrix::pressure_projection_enabled
+            f.project_pressure(self.defect)
         imp = vector_dot(self.ctx, self.defect[:3 * nu], self.defect[:3 * nu])
         mass = vector_dot(self.ctx, self.defect[3 * nu:], self.defect[3 * nu:])
         self.residuals.append((imp, mass, (imp + mass) ** 0.5))
@@ -1202,5 +1214,6 @@ class NavierStokes:
         self.linear_iterations.append(rep.iterations)
         if damping != 1.0:
             x.mul_(damping).add_(old, alpha=1.0 - damping)
-        f.project_pressure(x)
+        if f.enclosed:
+            f.project_pressure(x)
         return rep

@@ -802,6 +802,19 @@ static void nse_exact(double x, double y, double z, double ue[3][5], double pe[4
   pe[0] = x + y + z - 1.5; pe[1] = 1; pe[2] = 1; pe[3] = 1;
 }
 
+/* test_u_0_p_0.h (Example_NSE3D -1): u = (1, 0, 0), p = 0 */
+static void nse_exact_u0p0(double x, double y, double z, double ue[3][5], double pe[4]) {
+  (void)x; (void)y; (void)z;
+  for (int k = 0; k < 3; ++k) for (int a = 0; a < 5; ++a) ue[k][a] = 0;
+  ue[0][0] = 1;
+  for (int a = 0; a < 4; ++a) pe[a] = 0;
+}
+
+/* discontinuous P1 on hexahedra (include/FE3D/BF_D_H_P1_3D.h): 1, xi, eta, zeta */
+static void p1disc_basis(const double* xi, double* v) {
+  v[0] = 1; v[1] = xi[0]; v[2] = xi[1]; v[3] = xi[2];
+}
+
 static void add_entry(const int* rp, const int* col, int row, int c, double* val, double v) {
   for (int n = rp[row]; n < rp[row + 1]; ++n)
     if (col[n] == c) { val[n] += v; return; }
@@ -812,9 +825,12 @@ static void add_entry(const int* rp, const int* col, int row, int c, double* val
  * NSNonlinearTerm_convective_Single, NSDivergenceBlocks (sign 1),
  * NSRightHandSide (sign -1) (NSE_local_assembling_routines.C:52-283); the
  * coefficient function is LinCoeffs of test_u_2_p_1.h. */
-void or_assemble_nse(const or_space* vs, const or_space* ps, double nu, const double* w, double* A,
-                     const int* brp, const int* bcol, double* B1, double* B2, double* B3, double* rhs) {
+void or_assemble_nse2(const or_space* vs, const or_space* ps, int np_cell, int example, double nu,
+                      const double* w, double* A, const int* brp, const int* bcol, double* B1, double* B2,
+                      double* B3, double* rhs) {
   const int nu_d = vs->n_dof;
+  const int npc = np_cell;   /* 8: Q1 pressure, 4: discontinuous P1 */
+  void (*exact)(double, double, double, double[3][5], double[4]) = example == 1 ? nse_exact_u0p0 : nse_exact;
   double qxi[64 * 3], qw[64];
   const int nq = or_quad_rule(OR_HEXA, 2, 0, qxi, qw);
   double u[27], gr[81], hr[162], ux[81], pv[8], pg[24];
@@ -822,15 +838,16 @@ void or_assemble_nse(const or_space* vs, const or_space* ps, double nu, const do
   for (int cell = 0; cell < vs->n_cells; ++cell) {
     const double* v = vs->verts + (size_t)cell * 8 * 3;
     const int* vd = vs->gnum + (size_t)cell * 27;
-    const int* pd = ps->gnum + (size_t)cell * 8;
+    const int* pd = ps->gnum + (size_t)cell * npc;
     memset(Al, 0, sizeof Al); memset(Bl, 0, sizeof Bl); memset(Fl, 0, sizeof Fl); memset(Gl, 0, sizeof Gl);
     for (int q = 0; q < nq; ++q) {
       double x[3];
       const double det = phys_basis(OR_HEXA, 2, v, qxi + 3 * q, u, gr, hr, ux, NULL, x);
-      or_basis(OR_HEXA, 1, qxi + 3 * q, pv, pg, NULL);
+      if (npc == 4) p1disc_basis(qxi + 3 * q, pv);
+      else or_basis(OR_HEXA, 1, qxi + 3 * q, pv, pg, NULL);
       const double Mult = qw[q] * det;
       double ue[3][5], pe[4], f[3];
-      nse_exact(x[0], x[1], x[2], ue, pe);
+      exact(x[0], x[1], x[2], ue, pe);
       for (int k = 0; k < 3; ++k)
         f[k] = -nu * ue[k][4] + (ue[0][0] * ue[k][1] + ue[1][0] * ue[k][2] + ue[2][0] * ue[k][3]) + pe[1 + k];
       const double g = 0.0;
@@ -848,7 +865,7 @@ void or_assemble_nse(const or_space* vs, const or_space* ps, double nu, const do
           for (int j = 0; j < 27; ++j)
             Al[i * 27 + j] += Mult * (wq[0] * ux[j] + wq[1] * ux[27 + j] + wq[2] * ux[54 + j]) * u[i];
       /* NSDivergenceBlocks, sign 1 */
-      for (int i = 0; i < 8; ++i)
+      for (int i = 0; i < npc; ++i)
         for (int j = 0; j < 27; ++j) {
           Bl[0][i * 27 + j] -= 1 * Mult * pv[i] * ux[j];
           Bl[1][i * 27 + j] -= 1 * Mult * pv[i] * ux[27 + j];
@@ -857,7 +874,7 @@ void or_assemble_nse(const or_space* vs, const or_space* ps, double nu, const do
       /* NSRightHandSide, sign -1 */
       for (int i = 0; i < 27; ++i)
         for (int k = 0; k < 3; ++k) Fl[k][i] += Mult * u[i] * f[k];
-      for (int i = 0; i < 8; ++i) Gl[i] -= Mult * (-1) * pv[i] * g;
+      for (int i = 0; i < npc; ++i) Gl[i] -= Mult * (-1) * pv[i] * g;
     }
     for (int m = 0; m < 27; ++m) {
       const int r = vd[m];
@@ -869,7 +886,7 @@ void or_assemble_nse(const or_space* vs, const or_space* ps, double nu, const do
           if (vs->col[n] == r) { A[n] = 1.0; break; }
       }
     }
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < npc; ++i) {
       for (int j = 0; j < 27; ++j) {
         add_entry(brp, bcol, pd[i], vd[j], B1, Bl[0][i * 27 + j]);
         add_entry(brp, bcol, pd[i], vd[j], B2, Bl[1][i * 27 + j]);
@@ -883,10 +900,15 @@ void or_assemble_nse(const or_space* vs, const or_space* ps, double nu, const do
       double xi[3], xx[3] = {0, 0, 0}, J[9], ue[3][5], pe[4];
       node_ref(OR_HEXA, 2, k, xi);
       ref_map(OR_HEXA, v, xi, xx, J);
-      nse_exact(xx[0], xx[1], xx[2], ue, pe);
+      exact(xx[0], xx[1], xx[2], ue, pe);
       for (int c = 0; c < 3; ++c) rhs[(size_t)c * nu_d + vd[k]] = ue[c][0];
     }
   }
+}
+
+void or_assemble_nse(const or_space* vs, const or_space* ps, double nu, const double* w, double* A,
+                     const int* brp, const int* bcol, double* B1, double* B2, double* B3, double* rhs) {
+  or_assemble_nse2(vs, ps, 8, 0, nu, w, A, brp, bcol, B1, B2, B3, rhs);
 }
 
 /* ------------------------------------------------------ linear algebra */

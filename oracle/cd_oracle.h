@@ -84,6 +84,12 @@ void or_interpolate(const or_space* s, const or_cd* p, double* u);
  * [3][n_u] or NULL; example test_u_2_p_1 with viscosity nu */
 void or_assemble_nse(const or_space* vs, const or_space* ps, double nu, const double* w, double* A,
                      const int* brp, const int* bcol, double* B1, double* B2, double* B3, double* rhs);
+/* the same for np_cell = 8 (Q1) or 4 (discontinuous P1, basis 1, xi, eta,
+ * zeta, ps->gnum [n_cells][4]) pressure and example 0 (test_u_2_p_1) or 1
+ * (test_u_0_p_0: u = (1,0,0), p = 0, f = 0) */
+void or_assemble_nse2(const or_space* vs, const or_space* ps, int np_cell, int example, double nu,
+                      const double* w, double* A, const int* brp, const int* bcol, double* B1, double* B2,
+                      double* B3, double* rhs);
 void or_apply_scaled_add(int n, int active, const int* rp, const int* col, const double* val,
                          const double* x, double* y, double a);
 void or_prolongate(const or_space* cs, const or_space* fs, const int* parent, const double* cu, double* fu);

@@ -413,6 +413,8 @@ def lib():
         L.or_assemble_nse.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(Space), ctypes.c_double,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.or_assemble_nse2.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(Space), ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_double] + [ctypes.c_void_p] * 8
         L.or_apply_scaled_add.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
         L.or_prolongate.argtypes = [ctypes.POINTER(Space), ctypes.POINTER(Space), ctypes.c_void_p,
@@ -711,36 +713,72 @@ def rect_structure(gtest, gansatz, n_test):
     return np.array(rp, np.int32), np.array(col, np.int32)
 
 
-class NseOracle:
-    """Velocity Q2 (Dirichlet everywhere) and pressure Q1 (Neumann) spaces of
-    one hexahedral level with the patterns of A and B."""
+def front_face_interior(x, y, z):
+    """test_u_0_p_0.h on_front_face_interior: the face x = 1 without its boundary."""
+    tol = 1e-10
+    return abs(1 - x) < tol and not abs(y - 1) < tol and not abs(y) < tol and \
+        not abs(z - 1) < tol and not abs(z) < tol
 
-    def __init__(self, n_refine, level):
+
+class NseOracle:
+    """Velocity Q2 and pressure Q1 (continuous, Neumann) or discontinuous P1
+    (D_P1_3D_H_A: 4 DOFs per cell numbered 4 cell + local) spaces of one
+    hexahedral level with the patterns of A and B.  example "u2_p1"
+    (test_u_2_p_1.h): Dirichlet everywhere; "u0_p0" (test_u_0_p_0.h):
+    Neumann on the interior of the face x = 1, judged at the centre of each
+    boundary face as ConstructSpace does."""
+
+    def __init__(self, n_refine, level, pressure="q1", example="u2_p1"):
         shape, levels = mesh(2, n_refine)
         v = levels[level]["verts"]
-        gv, iv, bdv = number_dofs(shape, v, 2, None, lambda x, y, z: 0)
-        gp, ip, bdp = number_dofs(shape, v, 1, None, lambda x, y, z: 1)
+        self.pressure, self.example = pressure, example
+        if example == "u0_p0":
+            vcond = lambda x, y, z: 1 if front_face_interior(x, y, z) else 0   # noqa: E731
+        else:
+            vcond = lambda x, y, z: 0   # noqa: E731
+        gv, iv, bdv = number_dofs(shape, v, 2, None, vcond)
         rp, col = structure(gv, iv["n_dof"], iv["active_bound"])
         self.vs = OracleSpace(shape, 2, v, gv, iv["n_dof"], iv["active_bound"], rp, col, bdv,
                               levels[level]["parent"])
-        prp, pcol = structure(gp, ip["n_dof"], ip["n_dof"])
-        self.ps = OracleSpace(shape, 1, v, gp, ip["n_dof"], ip["n_dof"], prp, pcol, bdp, levels[level]["parent"])
-        self.brp, self.bcol = rect_structure(gp, gv, ip["n_dof"])
-        self.n_u, self.n_u_active, self.n_p = iv["n_dof"], iv["active_bound"], ip["n_dof"]
+        if pressure == "p1disc":
+            nc = len(v)
+            gp = np.arange(4 * nc, dtype=np.int32).reshape(nc, 4)
+            n_p = 4 * nc
+            self.ps = OracleSpace(shape, 1, v, gp, n_p, n_p, np.zeros(n_p + 1, np.int32), np.zeros(0, np.int32),
+                                  np.zeros((nc, 6), np.int32), levels[level]["parent"])
+        else:
+            gp, ip, bdp = number_dofs(shape, v, 1, None, lambda x, y, z: 1)
+            n_p = ip["n_dof"]
+            prp, pcol = structure(gp, n_p, n_p)
+            self.ps = OracleSpace(shape, 1, v, gp, n_p, n_p, prp, pcol, bdp, levels[level]["parent"])
+        self.brp, self.bcol = rect_structure(gp, gv, n_p)
+        self.n_u, self.n_u_active, self.n_p = iv["n_dof"], iv["active_bound"], n_p
         self.n = 3 * self.n_u + self.n_p
+        # BlockFEMatrix.C:1869-1920: enclosed flow = no Neumann / Robin velocity DOF
+        self.enclosed = iv["n_bound"][0] == 0 and iv["n_bound"][1] == 0
 
     def assemble(self, nu, w=None):
-        """NSTYPE-1 blocks A, (B1, B2, B3) and rhs of test_u_2_p_1 (or_assemble_nse)."""
+        """NSTYPE-1 blocks A, (B1, B2, B3) and rhs of the example (or_assemble_nse2)."""
         A = np.zeros(self.vs.nnz)
         B = [np.zeros(len(self.bcol)) for _ in range(3)]
         rhs = np.zeros(self.n)
         vs, ps = self.vs.cstruct(), self.ps.cstruct()
         wv = None if w is None else _p(np.ascontiguousarray(w, np.float64))
         keep = w
-        lib().or_assemble_nse(ctypes.byref(vs), ctypes.byref(ps), nu, wv, _p(A), _p(self.brp), _p(self.bcol),
-                              _p(B[0]), _p(B[1]), _p(B[2]), _p(rhs))
+        lib().or_assemble_nse2(ctypes.byref(vs), ctypes.byref(ps), self.ps.ndof_cell,
+                               1 if self.example == "u0_p0" else 0, nu, wv, _p(A), _p(self.brp), _p(self.bcol),
+                               _p(B[0]), _p(B[1]), _p(B[2]), _p(rhs))
         del keep
         return A, B, rhs
+
+    def pressure_mean(self, p):
+        """Mean value of a pressure function on the (axis-parallel hexahedral)
+        cells: Q1 the vertex mean per brick, discontinuous P1 its constant
+        coefficient (xi, eta, zeta integrate to 0)."""
+        if self.pressure == "p1disc":
+            vol = np.prod(self.ps.verts.max(1) - self.ps.verts.min(1), axis=1)
+            return float(np.sum(vol * p[self.ps.gnum[:, 0]]) / np.sum(vol))
+        return q1_mean(self.ps, p)
 
     def apply(self, A, B, x):
         """BlockFEMatrix::apply (src/Matrix/BlockFEMatrix.C:855-924) for NSTYPE 1:
@@ -777,10 +815,70 @@ class NseOracle:
                 row = k * nu + r
                 K.rows[row] = [row]
                 K.data[row] = [1.0]
-        row = 3 * nu
-        K.rows[row] = [row]
-        K.data[row] = [1.0]
+        if self.enclosed:
+            row = 3 * nu
+            K.rows[row] = [row]
+            K.data[row] = [1.0]
         return K.tocsr()
+
+
+def p1disc_nf_points():
+    """The 8 points of the discontinuous P1 nodal functionals
+    (NF_D_H_P1_3D.h): (+-1/sqrt 3)^3, xi fastest."""
+    a = 0.5773502691896257645091489
+    return np.array([[a if i & 1 else -a, a if i & 2 else -a, a if i & 4 else -a] for i in range(8)])
+
+
+def p1disc_nf(values8):
+    """NF_D_H_P1_3D_EvalAll: F0 = mean of the 8 values, F1..F3 = 0.375 sum of
+    value x xi / eta / zeta of the point.  values8 [..., 8] -> [..., 4]."""
+    pts = p1disc_nf_points()
+    v = np.asarray(values8)
+    return np.concatenate([v.mean(-1, keepdims=True), 0.375 * (v @ pts)], -1)
+
+
+def _affine(verts):
+    x0 = node_phys(HEXA, verts, np.zeros(3))
+    J = np.stack([node_phys(HEXA, verts, e) - x0 for e in np.eye(3)], -1)
+    return x0, J
+
+
+def p1disc_prolongation(cs, fs):
+    """Prolongation of the discontinuous P1 pressure (GridTransfer::Prolongate
+    with the matrices of TFEDatabase3D::GetProlongationMatrix3D): the row of
+    fine DOF k of child fc holds the child's nodal functional k applied to
+    the parent's basis functions 1, xi, eta, zeta; each fine DOF lies in one
+    cell, so nothing is averaged.  Returns a scipy CSR matrix [n_f, n_c]."""
+    import scipy.sparse as sp
+    pts = p1disc_nf_points()
+    rows, cols, vals = [], [], []
+    for fc in range(fs.n_cells):
+        pc = int(fs.parent[fc])
+        f0, fJ = _affine(fs.verts[fc])
+        c0, cJ = _affine(cs.verts[pc])
+        x = f0 + pts @ fJ.T
+        xi = np.linalg.solve(cJ, (x - c0).T).T                       # [8, 3] parent reference coords
+        basis = np.concatenate([np.ones((8, 1)), xi], 1)             # [8, 4] parent basis at the points
+        F = p1disc_nf(basis.T)                                       # [4 coarse, 4 fine functionals]
+        for k in range(4):
+            for l in range(4):
+                if abs(F[l, k]) > 1e-14:
+                    rows.append(int(fs.gnum[fc, k]))
+                    cols.append(int(cs.gnum[pc, l]))
+                    vals.append(F[l, k])
+    return sp.csr_matrix((vals, (rows, cols)), shape=(fs.n_dof, cs.n_dof))
+
+
+def pressure_prolongate(cns, fns, u):
+    if cns.pressure == "p1disc":
+        return p1disc_prolongation(cns.ps, fns.ps) @ u
+    return prolongate(cns.ps, fns.ps, u)
+
+
+def pressure_restrict(cns, fns, d):
+    if cns.pressure == "p1disc":
+        return p1disc_prolongation(cns.ps, fns.ps).T @ d
+    return restrict(cns.ps, fns.ps, d)
 
 
 def nse_exact(x):
@@ -799,9 +897,27 @@ def nodal_coords(space):
     return out
 
 
+def nse_exact_u0p0(x):
+    """test_u_0_p_0.h: u = (1, 0, 0), p = 0 at points x [n, 3]."""
+    z = np.zeros(len(x))
+    return np.stack([z + 1.0, z, z]), z
+
+
 def nse_interpolate(ns):
-    u, _ = nse_exact(nodal_coords(ns.vs))
-    _, p = nse_exact(nodal_coords(ns.ps))
+    """The example's exact solution in the spaces: velocity at the Q2 nodal
+    points, pressure at the Q1 nodal points or through the discontinuous P1
+    nodal functionals of every cell."""
+    exact = nse_exact_u0p0 if getattr(ns, "example", "u2_p1") == "u0_p0" else nse_exact
+    u, _ = exact(nodal_coords(ns.vs))
+    if getattr(ns, "pressure", "q1") == "p1disc":
+        pts = p1disc_nf_points()
+        p = np.zeros(ns.n_p)
+        for c in range(ns.ps.n_cells):
+            x0, J = _affine(ns.ps.verts[c])
+            _, pv = exact(x0 + pts @ J.T)
+            p[ns.ps.gnum[c]] = p1disc_nf(pv)
+    else:
+        _, p = exact(nodal_coords(ns.ps))
     return np.concatenate([u[0], u[1], u[2], p])
 
 
@@ -897,7 +1013,7 @@ def nse_mg_apply(levels, v, damp_corr=1.0, vanka_damp=1.0, n_pre=1, n_post=1, co
         for k in range(3):
             pf = prolongate(C.ns.vs, Fn.ns.vs, sol[l][k * cu:(k + 1) * cu])
             sol[l + 1][k * fu:k * fu + Fn.ns.n_u_active] += damp_corr * pf[:Fn.ns.n_u_active]
-        pp = prolongate(C.ns.ps, Fn.ns.ps, sol[l][3 * cu:])
+        pp = pressure_prolongate(C.ns, Fn.ns, sol[l][3 * cu:])
         sol[l + 1][3 * fu:] += damp_corr * pp
 
     def down(l):   # fine l -> coarse l - 1
@@ -909,7 +1025,7 @@ def nse_mg_apply(levels, v, damp_corr=1.0, vanka_damp=1.0, n_pre=1, n_post=1, co
             rc = restrict(C.ns.vs, Fn.ns.vs, d[k * fu:(k + 1) * fu])
             rc[C.ns.n_u_active:] = 0.0
             r[k * cu:(k + 1) * cu] = rc
-        r[3 * cu:] = restrict(C.ns.ps, Fn.ns.ps, d[3 * fu:])
+        r[3 * cu:] = pressure_restrict(C.ns, Fn.ns, d[3 * fu:])
         rhs[l - 1] = r
         sol[l - 1] = np.zeros(C.ns.n)
 
@@ -964,14 +1080,16 @@ def _cycle_control(cycle, nl):
     return cc
 
 
-def fgmres(apply, prec, b, x0, restart=30, max_it=100, tol=1e-8, reduction=0.0):
+def fgmres(apply, prec, b, x0, restart=30, max_it=100, tol=1e-8, reduction=0.0, min_it=0):
     """Iteration_gmres::flexible_gmres (src/Solver/Iteration_gmres.C:374-532) with
-    IterativeMethod::converged: absolute residual |s_{i+1}|, MGS, Givens."""
+    IterativeMethod::converged: absolute residual |s_{i+1}|, MGS, Givens;
+    convergence counts only after more than min_it iterations
+    (include/Solver/IterativeMethod.h:97-98)."""
     x = x0.copy()
     r = b - apply(x)
     beta = np.linalg.norm(r)
     r0 = beta
-    if beta <= tol or beta <= reduction * r0:
+    if (beta <= tol or beta <= reduction * r0) and min_it == 0:
         return x, 0
     j = 1
     while j <= max_it:
@@ -1008,7 +1126,7 @@ def fgmres(apply, prec, b, x0, restart=30, max_it=100, tol=1e-8, reduction=0.0):
             s[i + 1] = -sn[i] * s[i]
             s[i] = cs[i] * s[i]
             res = abs(s[i + 1])
-            if res <= tol or res <= reduction * r0 or j >= max_it:
+            if ((res <= tol or res <= reduction * r0) and (j > min_it or min_it == 0)) or j >= max_it:
                 y = np.linalg.solve(np.triu(H[:i + 1, :i + 1]), s[:i + 1])
                 for k in range(i + 1):
                     x = x + y[k] * Z[k]
@@ -1094,7 +1212,7 @@ def q1_mean(ps, p):
 
 
 def nse_picard(n_refine, levels, nu, mg_kw, lin_tol=1e-11, lin_max_it=200, restart=30, eps=1e-10, max_it=10,
-               min_it=1, slowfactor=1e10):
+               min_it=1, slowfactor=1e10, pressure="q1", example="u2_p1", lin_min_it=0):
     """NavierStokes<3> nonlinear loop as test/nse3d_solvers_test.c++:107-126
     runs it: assemble_linear_terms (convection with the initial, zero,
     velocity; Dirichlet values copied into the solution), stop_it(0), then
@@ -1104,8 +1222,9 @@ def nse_picard(n_refine, levels, nu, mg_kw, lin_tol=1e-11, lin_max_it=200, resta
     stop_it(k) (src/System/NavierStokes.C:750-818: full residual
     sqrt(|r_u|^2 + |r_p|^2) of rhs - K x with the pressure part of the defect
     projected into L2_0).  Returns the solution, the residual history and the
-    FGMRES iteration counts."""
-    sys_ = [NseOracle(n_refine, l) for l in levels]
+    FGMRES iteration counts.  The pressure is projected only for an enclosed
+    flow (BlockFEMatrix::pressure_projection_enabled)."""
+    sys_ = [NseOracle(n_refine, l, pressure, example) for l in levels]
     sols = [np.zeros(s.n) for s in sys_]
     F = sys_[-1]
     nu_, na = F.n_u, F.n_u_active
@@ -1121,7 +1240,8 @@ def nse_picard(n_refine, levels, nu, mg_kw, lin_tol=1e-11, lin_max_it=200, resta
 
     def residual(A, B):
         d = rhs - F.apply(A, B, x)
-        d[3 * nu_:] -= q1_mean(F.ps, d[3 * nu_:])
+        if F.enclosed:
+            d[3 * nu_:] -= F.pressure_mean(d[3 * nu_:])
         return np.sqrt(np.dot(d[:3 * nu_], d[:3 * nu_]) + np.dot(d[3 * nu_:], d[3 * nu_:]))
 
     hist = [residual(blocks[-1][0], blocks[-1][1])]
@@ -1132,8 +1252,9 @@ def nse_picard(n_refine, levels, nu, mg_kw, lin_tol=1e-11, lin_max_it=200, resta
         lv = [NseLevelOracle(s, b[0], b[1]) for s, b in zip(sys_, blocks)]
         A, B = blocks[-1][0], blocks[-1][1]
         x, its = fgmres(lambda z: F.apply(A, B, z), lambda v: nse_mg_apply(lv, v, **mg_kw), rhs, x,
-                        restart=restart, max_it=lin_max_it, tol=lin_tol)
-        x[3 * nu_:] -= q1_mean(F.ps, x[3 * nu_:])
+                        restart=restart, max_it=lin_max_it, tol=lin_tol, min_it=lin_min_it)
+        if F.enclosed:
+            x[3 * nu_:] -= F.pressure_mean(x[3 * nu_:])
         lin_its.append(its)
         sols[-1] = x
         for l in range(len(sys_) - 2, -1, -1):

@@ -1,0 +1,179 @@
+"""Q2/P1disc (discontinuous P1 pressure, D_P1_3D_H_A) and test_u_0_p_0
+(Neumann on the interior of the face x = 1) on the HIP path against the
+oracle: numbering, patterns, blocks, combined matrix (no pressure row
+correction when the flow is not enclosed), interpolation and errors, one
+multigrid cycle with cell Vanka on 85 local DOFs, and the reference's own
+cell-Vanka test (test/nse3d_solvers_test.c++:334-341)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from mymoonmd_amd import parmoon as pm
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("p1disc", "u0_p0"), ("p1disc", "u2_p1"), ("q1", "u0_p0")]
+
+
+def dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+@pytest.fixture(scope="module", params=CASES, ids=["-".join(c) for c in CASES])
+def case(ctx, request):
+    pressure, example = request.param
+    d = pm.TDomain("Default_UnitCube_Hexa", 2)
+    g = pm.NSESystem(ctx, d, 2, pressure, example)
+    o = O.NseOracle(2, 2, pressure, example)
+    return pressure, example, d, g, o
+
+
+def test_numbering_patterns_and_enclosure(case):
+    pressure, example, d, g, o = case
+    print("case", pressure, example, flush=True)
+    assert (g.n_u, g.n_u_active, g.n_p) == (o.n_u, o.n_u_active, o.n_p)
+    assert g.enclosed == o.enclosed == (example == "u2_p1")
+    gv, gp = g.global_numbers()
+    assert np.array_equal(gv, o.vs.gnum) and np.array_equal(gp, o.ps.gnum)
+    rp, col = g.structure("A")
+    assert np.array_equal(rp, o.vs.rp) and np.array_equal(col, o.vs.col)
+    rp, col = g.structure("B")
+    assert np.array_equal(rp, o.brp) and np.array_equal(col, o.bcol)
+
+
+def test_blocks_operator_and_combined_match_oracle(ctx, case):
+    pressure, example, d, g, o = case
+    print("case", pressure, example, flush=True)
+    rng = np.random.default_rng(5)
+    w = rng.standard_normal(3 * o.n_u)
+    rhs = g.zeros()
+    g.assemble(example, 1e-2, dev(w), rhs)
+    A, B, r = o.assemble(1e-2, w)
+    assert np.max(np.abs(g.values("A") - A)) <= 1e-12 * np.max(np.abs(A))
+    for k in range(3):
+        assert np.max(np.abs(g.values("B%d" % (k + 1)) - B[k])) <= 1e-12 * np.max(np.abs(B[k]))
+    assert np.max(np.abs(rhs.cpu().numpy() - r)) <= 1e-12 * np.max(np.abs(r))
+    x = rng.standard_normal(o.n)
+    y = g.zeros()
+    g.apply(dev(x), y)
+    yo = o.apply(A, B, x)
+    assert np.max(np.abs(y.cpu().numpy() - yo)) <= 1e-12 * np.max(np.abs(yo))
+    K = o.combined(A, B)
+    rp, col = g.structure("combined")
+    G = sp.csr_matrix((g.values("combined"), col, rp), shape=(o.n, o.n))
+    assert abs(G - K).max() <= 1e-12 * abs(K).max()
+    # the first pressure row is a unit row exactly for an enclosed flow
+    row = G.getrow(3 * o.n_u)
+    unit = row.nnz == 1 and row.indices[0] == 3 * o.n_u and row.data[0] == 1.0
+    assert unit == o.enclosed
+
+
+def test_interpolant_solves_and_has_zero_errors(ctx, case):
+    """The interpolant of the example (pressure through the P1disc nodal
+    functionals) equals the oracle's, solves the Oseen system with w = u and
+    has zero errors (both solutions lie in the spaces)."""
+    pressure, example, d, g, o = case
+    print("case", pressure, example, flush=True)
+    x = g.zeros()
+    g.interpolate(x)
+    assert np.max(np.abs(x.cpu().numpy() - O.nse_interpolate(o))) < 1e-14
+    rhs = g.zeros()
+    g.assemble(example, 1e-2, x[:3 * g.n_u].clone(), rhs)
+    r = rhs.clone()
+    g.apply_scaled_add(x, r, -1.0)
+    assert torch.max(torch.abs(r)).item() < 1e-13
+    e = g.errors(x)
+    assert max(e[0], e[3]) < 1e-13 and max(e[1], e[4]) < 1e-12, e
+    if g.enclosed:
+        x[3 * g.n_u:] += 0.37
+        assert g.errors(x)[3] > 0.3
+        g.project_pressure(x)
+        assert g.errors(x)[3] < 1e-13
+
+
+def p1disc_levels(ctx, example="u0_p0", nu=1.0):
+    d = pm.TDomain("Default_UnitCube_Hexa", 2)
+    gl, ol, rhs = [], [], None
+    for lev in (1, 2):
+        g = pm.NSESystem(ctx, d, lev, "p1disc", example)
+        x = g.zeros()
+        g.interpolate(x)
+        r = g.zeros()
+        g.assemble(example, nu, x[:3 * g.n_u].clone(), r)
+        o = O.NseOracle(2, lev, "p1disc", example)
+        xo = O.nse_interpolate(o)
+        A, B, ro = o.assemble(nu, xo[:3 * o.n_u])
+        gl.append(g)
+        ol.append(O.NseLevelOracle(o, A, B))
+        rhs = (r, ro)
+    return d, gl, ol, rhs
+
+
+@pytest.mark.parametrize("example", ["u0_p0", "u2_p1"])
+def test_p1disc_multigrid_vanka_cycle_matches_oracle(ctx, example):
+    """One V(1,1) cycle with cell_vanka_jacobi on both levels (3 coarse sweeps)
+    on 85-DOF local systems, P1disc pressure transfers."""
+    print("case", example, flush=True)
+    d, gl, ol, _ = p1disc_levels(ctx, example)
+    kw = dict(iterative_solver_type="fgmres", preconditioner="multigrid", multigrid_cycle_type="V",
+              multigrid_smoother="cell_vanka_jacobi", multigrid_smoother_coarse="cell_vanka_jacobi",
+              multigrid_n_pre_smooth=1, multigrid_n_post_smooth=1, multigrid_vanka_damp_factor=0.8,
+              multigrid_correction_damp_factor=0.8, multigrid_coarse_max_n_iterations=3,
+              multigrid_coarse_residual=1e-30)
+    sol = pm.Solver(ctx, pm.solver_params(**kw))
+    sol.update_saddle(gl)
+    v = np.random.default_rng(3).standard_normal(gl[-1].n)
+    z = gl[-1].zeros()
+    sol.apply_preconditioner(dev(v), z)
+    zo = O.nse_mg_apply(ol, v, damp_corr=0.8, vanka_damp=0.8, n_pre=1, n_post=1, coarse="cell_vanka_jacobi",
+                        coarse_maxit=3, coarse_res=1e-30)
+    assert np.max(np.abs(z.cpu().numpy() - zo)) <= 1e-10 * np.max(np.abs(zo))
+
+
+REF_TEST = dict(iterative_solver_type="fgmres", preconditioner="multigrid", residual_tolerance=1e-8,
+                residual_reduction=0.0, max_n_iterations=10, min_n_iterations=3, gmres_restart=20,
+                multigrid_cycle_type="V", multigrid_smoother="cell_vanka_jacobi",
+                multigrid_smoother_coarse="cell_vanka_jacobi", multigrid_correction_damp_factor=0.8,
+                multigrid_n_pre_smooth=1, multigrid_n_post_smooth=1, multigrid_coarse_residual=0.1,
+                multigrid_coarse_max_n_iterations=10, multigrid_vanka_damp_factor=0.8)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_reference_cell_vanka_test(ctx, precision):
+    """test/nse3d_solvers_test.c++ with cell_vanka_jacobi: example -1 on 4^3
+    hexahedra, Q2/P1disc, 2 levels, the solver settings of :171-191, Picard
+    to 1e-12 within 15 steps; expected errors 0 with tolerance 1e-8.  FP64:
+    the residual history and FGMRES counts follow the oracle's."""
+    print("case", precision, flush=True)
+    d = pm.TDomain("Default_UnitCube_Hexa", 2)
+    prm = pm.solver_params(precision=precision, **REF_TEST)
+    ns = pm.NavierStokes(ctx, d, [1, 2], prm, nu=1.0, example="u0_p0", pressure="p1disc",
+                         nonlinloop_epsilon=1e-12, nonlinloop_maxit=15)
+    ns.assemble_linear_terms()
+    ns.stop_it(0)
+    k = 0
+    while True:
+        k += 1
+        ns.solve()
+        ns.assemble_nonlinear_term()
+        if ns.stop_it(k):
+            break
+    hist = [r[2] for r in ns.residuals]
+    e = ns.systems[-1].errors(ns.solution[-1])
+    print("gpu", hist, ns.linear_iterations, e, flush=True)
+    for i in (0, 1, 3, 4):
+        assert abs(e[i]) <= 1e-8, e
+    assert hist[-1] <= 1e-12
+    if precision == "fp64":
+        mg = dict(damp_corr=0.8, vanka_damp=0.8, n_pre=1, n_post=1, coarse="cell_vanka_jacobi", coarse_maxit=10,
+                  coarse_res=0.1, cycle="V", smoother="cell_vanka_jacobi")
+        xo, ho, its = O.nse_picard(2, [1, 2], 1.0, mg, lin_tol=1e-8, lin_max_it=10, restart=20, eps=1e-12,
+                                   max_it=15, pressure="p1disc", example="u0_p0", lin_min_it=3)
+        print("oracle", ho, its, flush=True)
+        assert len(hist) == len(ho) and ns.linear_iterations == its
+        for a, b in zip(hist, ho):
+            if b > 1e-9:
+                assert abs(a - b) <= 1e-6 * b, (a, b)
+        assert np.max(np.abs(ns.solution[-1].cpu().numpy() - xo)) < 1e-9
