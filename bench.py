@@ -370,19 +370,19 @@ def cpu_baseline_batch(batch, n_sample=8, steps=3):
                 n_sample, steps, t)}
 
 
-def cpu_baseline_oseen(oseen, iterations=5, nref=3, nu=1e-2, vanka_damp=0.2):
+def cpu_baseline_oseen(oseen, iterations=5, nref=3, nu=1e-2, vanka_damp=0.2, pressure="q1"):
     """The oracle (numpy with one BLAS thread) on a bounded sample of the Oseen
-    leg: 8^3 Q2/Q1 (2 levels), the same Fourier convection field, FGMRES +
-    multigrid V(2,2) with cell Vanka (damping 0.2, correction 0.8, direct
-    coarse solve) for a few iterations; seconds per iteration per DOF times the
-    32^3 DOFs and the GPU's iteration count."""
+    leg: 8^3 Q2/Q1 or Q2/P1disc (2 levels), the same Fourier convection field,
+    FGMRES + multigrid V(2,2) with cell Vanka (the leg's damping, correction
+    0.8, direct coarse solve) for a few iterations; seconds per iteration per
+    DOF times the 32^3 DOFs and the GPU's iteration count."""
     import numpy as np
     from threadpoolctl import threadpool_limits
     from oracle import oracle as O
     levels = []
     rhs = None
     for lev in (nref - 1, nref):
-        o = O.NseOracle(nref, lev)
+        o = O.NseOracle(nref, lev, pressure)
         w = fourier_field(O.nodal_coords(o.vs)).reshape(-1)
         A, B, rhs = o.assemble(nu, w)
         levels.append(O.NseLevelOracle(o, A, B))
@@ -422,17 +422,18 @@ def fourier_field(x, seed=0x5EED, n_modes=8):
     return w
 
 
-def oseen_solve(pm, torch, ctx, nref=5, nlev=4, nu=1e-2, vanka_damp=0.2, steps=2):
-    """Oseen on the unit cube, 32^3 hexahedra, Taylor-Hood Q2/Q1 (NSTYPE 1),
-    nu = 1e-2, frozen convection = Q2 interpolant of 8 random divergence-free
-    Fourier modes, rhs and boundary data of test_u_2_p_1; FGMRES(30) + 4-level
-    multigrid V(2,2) with the cell-Vanka (Jacobi) smoother, correction damping
-    0.8, direct coarse solve, relative residual 1e-8."""
+def oseen_solve(pm, torch, ctx, nref=5, nlev=4, nu=1e-2, vanka_damp=0.2, steps=2, pressure="q1"):
+    """Oseen on the unit cube, 32^3 hexahedra, Taylor-Hood Q2/Q1 (or Q2/P1disc
+    with pressure="p1disc"; NSTYPE 1), nu = 1e-2, frozen convection = Q2
+    interpolant of 8 random divergence-free Fourier modes, rhs and boundary
+    data of test_u_2_p_1; FGMRES(30) + 4-level multigrid V(2,2) with the
+    cell-Vanka (Jacobi) smoother, correction damping 0.8, direct coarse
+    solve, relative residual 1e-8."""
     dom = pm.TDomain("Default_UnitCube_Hexa", nref)
     t0 = time.perf_counter()
     systems = []
     for lev in range(nref - nlev + 1, nref + 1):
-        g = pm.NSESystem(ctx, dom, lev)
+        g = pm.NSESystem(ctx, dom, lev, pressure)
         xyz = pm.TFESpace(dom, lev, 2, None).dof_coords()
         w = torch.as_tensor(fourier_field(xyz).reshape(-1)).cuda()
         r = g.zeros()
@@ -451,8 +452,9 @@ def oseen_solve(pm, torch, ctx, nref=5, nlev=4, nu=1e-2, vanka_damp=0.2, steps=2
     r0 = b.clone()
     g.apply_scaled_add(start(), r0, -1.0)
     r0n = torch.linalg.norm(r0).item()
-    out = {"n_dof": g.n, "n_u": 3 * g.n_u, "n_p": g.n_p, "levels": nlev, "nnz_combined": int(g.info.nnz_combined),
-           "host_and_assembly_setup_s": round(t_setup, 3), "vanka_damp": vanka_damp}
+    out = {"pressure": pressure, "n_dof": g.n, "n_u": 3 * g.n_u, "n_p": g.n_p, "levels": nlev,
+           "nnz_combined": int(g.info.nnz_combined), "host_and_assembly_setup_s": round(t_setup, 3),
+           "vanka_damp": vanka_damp}
     for prec in ("mixed", "fp64"):
         sol = pm.Solver(ctx, pm.solver_params(
             iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30, residual_tolerance=0.0,
@@ -805,6 +807,9 @@ def main():
     batch = None if args.no_batch else batched_sweep(pm, ctx)
     batch_gmres = None if args.no_batch else batched_sweep(pm, ctx, steps=5, inner_solver="gmres")
     oseen = None if args.no_oseen else oseen_solve(pm, torch, ctx)
+    # the reference's cell-Vanka pair (Q2/P1disc) at the same size, with the
+    # damping of its own Vanka test (0.8)
+    oseen_p1 = None if args.no_oseen else oseen_solve(pm, torch, ctx, vanka_damp=0.8, pressure="p1disc")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -824,6 +829,7 @@ def main():
             cpu["batched_tcd2d"] = cpu_baseline_batch(batch)
         if oseen is not None:
             cpu["oseen_q2q1_32"] = cpu_baseline_oseen(oseen)
+            cpu["oseen_q2p1disc_32"] = cpu_baseline_oseen(oseen_p1, vanka_damp=0.8, pressure="p1disc")
 
     out = {
         "metric": "time-to-solution @1e-8 rel. residual, mixed vs FP64 GPU; assembly+SpMV MDoF/s",
@@ -865,6 +871,7 @@ def main():
         "batched_tcd2d": batch,
         "batched_tcd2d_gmres": batch_gmres,
         "oseen_q2q1_32": oseen,
+        "oseen_q2p1disc_32": oseen_p1,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
