@@ -44,136 +44,6 @@ __device__ __forceinline__ double wg_sum(double v, double* red) {
   return s;
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_batch_solve(BatchDev B, BatchSolveParams P) {
-  constexpr int NW = BS / kWave;
-  constexpr int LD = kMaxRestart + 1;
-  __shared__ double red[NW];
-  __shared__ double H[LD * kMaxRestart];
-  __shared__ double cs[kMaxRestart], sn[kMaxRestart], g[LD], y[kMaxRestart];
-  __shared__ double bc[1];
-  const int p = B.order[blockIdx.x];
-  const int off = B.dof_off[p], na = B.n_active[p];
-  const int s0 = B.slice_off[p], s1 = B.slice_off[p + 1];
-  double* x = B.x + off;
-  const double* rhs = B.rhs + off;
-  double* r = B.r + off;
-  float* w = B.w + off;
-  const float* dinv = B.dinv32 + off;
-  float* V = B.V + B.ws_off[p];
-  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-  const int m = P.m;
-  double rn0 = 0, rn = 0;
-  int outer = 0, inner_total = 0;
-  for (;;) {
-    // FP64 residual r = rhs - S x on the active rows
-    double part = 0;
-    for (int s = s0 + wv; s < s1; s += NW) {
-      const int lr = B.srow[(size_t)s * kWave + lane];
-      const int a = B.sptr[s], width = (B.sptr[s + 1] - a) / kWave;
-      double acc = 0;
-      for (int j = 0; j < width; ++j) {
-        const int k = a + j * kWave + lane;
-        acc += B.S64[k] * x[B.scol[k]];
-      }
-      if (lr >= 0) {
-        const double d = rhs[lr] - acc;
-        r[lr] = d;
-        part += d * d;
-      }
-    }
-    rn = sqrt(wg_sum<BS>(part, red));
-    if (outer == 0) rn0 = rn;
-    if (rn <= P.red * rn0 || rn <= P.tol || outer >= P.max_outer) break;
-    ++outer;
-    // inner GMRES(m) for S d = r, d0 = 0, right Jacobi preconditioning
-    const double inv = 1.0 / rn;
-    for (int i = threadIdx.x; i < na; i += BS) V[i] = (float)(r[i] * inv);
-    if (threadIdx.x == 0) g[0] = rn;
-    __syncthreads();
-    int k = 0;
-    for (int j = 0; j < m; ++j) {
-      const float* vj = V + (size_t)j * na;
-      float* vn = V + (size_t)(j + 1) * na;
-      for (int s = s0 + wv; s < s1; s += NW) {
-        const int lr = B.srow[(size_t)s * kWave + lane];
-        const int a = B.sptr[s], width = (B.sptr[s + 1] - a) / kWave;
-        float acc = 0;
-        for (int jj = 0; jj < width; ++jj) {
-          const int kk = a + jj * kWave + lane;
-          const int c = B.scol[kk];
-          const float z = (c < na) ? dinv[c] * vj[c] : 0.0f;   // correction is 0 on Dirichlet DOFs
-          acc += B.S32[kk] * z;
-        }
-        if (lr >= 0) w[lr] = acc;
-      }
-      __syncthreads();
-      // modified Gram-Schmidt
-      for (int i = 0; i <= j; ++i) {
-        const float* vi = V + (size_t)i * na;
-        double d = 0;
-        for (int t = threadIdx.x; t < na; t += BS) d += (double)w[t] * (double)vi[t];
-        const double h = wg_sum<BS>(d, red);
-        const float hf = (float)h;
-        for (int t = threadIdx.x; t < na; t += BS) w[t] -= hf * vi[t];
-        if (threadIdx.x == 0) H[i + j * LD] = h;
-      }
-      double d = 0;
-      for (int t = threadIdx.x; t < na; t += BS) d += (double)w[t] * (double)w[t];
-      const double hn = sqrt(wg_sum<BS>(d, red));
-      const float ih = (hn > 0) ? (float)(1.0 / hn) : 0.0f;
-      for (int t = threadIdx.x; t < na; t += BS) vn[t] = w[t] * ih;
-      if (threadIdx.x == 0) {
-        H[j + 1 + j * LD] = hn;
-        for (int i = 0; i < j; ++i) {
-          const double t0 = H[i + j * LD], t1 = H[i + 1 + j * LD];
-          H[i + j * LD] = cs[i] * t0 + sn[i] * t1;
-          H[i + 1 + j * LD] = -sn[i] * t0 + cs[i] * t1;
-        }
-        const double a0 = H[j + j * LD], a1 = H[j + 1 + j * LD];
-        double c = 1, sv = 0;
-        if (a1 != 0) {
-          const double rr = sqrt(a0 * a0 + a1 * a1);
-          c = a0 / rr;
-          sv = a1 / rr;
-        }
-        cs[j] = c;
-        sn[j] = sv;
-        H[j + j * LD] = c * a0 + sv * a1;
-        H[j + 1 + j * LD] = 0;
-        g[j + 1] = -sv * g[j];
-        g[j] = c * g[j];
-        bc[0] = (hn > 0) ? fabs(g[j + 1]) : 0.0;
-      }
-      __syncthreads();
-      k = j + 1;
-      if (bc[0] <= P.inner_red * rn) break;
-    }
-    inner_total += k;
-    if (threadIdx.x == 0) {
-      for (int i = k - 1; i >= 0; --i) {
-        double s = g[i];
-        for (int l = i + 1; l < k; ++l) s -= H[i + l * LD] * y[l];
-        const double dd = H[i + i * LD];
-        y[i] = (dd != 0) ? s / dd : 0.0;
-      }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < na; t += BS) {
-      double acc = 0;
-      for (int i = 0; i < k; ++i) acc += y[i] * (double)V[(size_t)i * na + t];
-      x[t] += (double)dinv[t] * acc;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    B.iters[p] += inner_total;
-    B.outer[p] += outer;
-    B.res[p] = (rn0 > 0) ? rn / rn0 : 0.0;
-    if (!(rn <= P.red * rn0 || rn <= P.tol)) B.fail[p] += 1;
-  }
-}
-
 // Slice loops of one wave over its share of a problem's slices, with U
 // slices in flight (independent accumulators) to hide memory latency: a
 // single workgroup has to stream the whole problem.
@@ -221,6 +91,153 @@ __device__ __forceinline__ void wave_rows(const BatchDev& B, int s0, int s1, int
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (lr[u] >= 0) f(lr[u]);
+  }
+}
+
+// Restarted GMRES(m) inside the refinement loop (one workgroup per problem).
+// Orthogonalisation by classical Gram-Schmidt: the j+1 projections of the
+// new vector are summed in one sweep over the basis (eight basis vectors per
+// sweep of w) and reduced together, then subtracted in one pass that also
+// sums |w|^2; a workgroup waits twice per step instead of twice per basis
+// vector, and each pass keeps many loads in flight per thread.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_batch_solve(BatchDev B, BatchSolveParams P) {
+  constexpr int NW = BS / kWave;
+  constexpr int LD = kMaxRestart + 1;
+  constexpr int U = 4;
+  __shared__ double red[NW];
+  __shared__ double dred[kMaxRestart * NW];
+  __shared__ float hf[kMaxRestart];
+  __shared__ double H[LD * kMaxRestart];
+  __shared__ double cs[kMaxRestart], sn[kMaxRestart], g[LD], y[kMaxRestart];
+  __shared__ double bc[1];
+  const int p = B.order[blockIdx.x];
+  const int off = B.dof_off[p], na = B.n_active[p];
+  const int s0 = B.slice_off[p], s1 = B.slice_off[p + 1];
+  double* x = B.x + off;
+  const double* rhs = B.rhs + off;
+  double* r = B.r + off;
+  float* w = B.w + off;
+  const float* dinv = B.dinv32 + off;
+  float* V = B.V + B.ws_off[p];
+  const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+  const int m = P.m;
+  double rn0 = 0, rn = 0;
+  int outer = 0, inner_total = 0;
+  for (;;) {
+    // FP64 residual r = rhs - S x on the active rows
+    double part = 0;
+    wave_spmv<U, double, double>(B, B.S64, s0, s1, wv, NW, lane, [&](int c) { return x[c]; },
+                                 [&](int lr, double acc) {
+                                   const double d = rhs[lr] - acc;
+                                   r[lr] = d;
+                                   part += d * d;
+                                 });
+    rn = sqrt(wg_sum<BS>(part, red));
+    if (outer == 0) rn0 = rn;
+    if (rn <= P.red * rn0 || rn <= P.tol || outer >= P.max_outer) break;
+    ++outer;
+    // inner GMRES(m) for S d = r, d0 = 0, right Jacobi preconditioning
+    const double inv = 1.0 / rn;
+    for (int i = threadIdx.x; i < na; i += BS) V[i] = (float)(r[i] * inv);
+    if (threadIdx.x == 0) g[0] = rn;
+    __syncthreads();
+    int k = 0;
+    for (int j = 0; j < m; ++j) {
+      const float* vj = V + (size_t)j * na;
+      float* vn = V + (size_t)(j + 1) * na;
+      // w = S D^-1 v_j (the correction is 0 on Dirichlet DOFs)
+      wave_spmv<U, float, float>(B, B.S32, s0, s1, wv, NW, lane,
+                                 [&](int c) { return (c < na) ? dinv[c] * vj[c] : 0.0f; },
+                                 [&](int lr, float acc) { w[lr] = acc; });
+      __syncthreads();
+      // h_i = (w, v_i), i <= j
+      for (int i0 = 0; i0 <= j; i0 += 8) {
+        double d[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) d[u] = 0.0;
+        for (int t = threadIdx.x; t < na; t += BS) {
+          const double wt = (double)w[t];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (i0 + u <= j) d[u] += wt * (double)V[(size_t)(i0 + u) * na + t];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const double sd = group_sum<kWave>(d[u]);
+          if (lane == 0 && i0 + u <= j) dred[(i0 + u) * NW + wv] = sd;
+        }
+      }
+      __syncthreads();
+      if ((int)threadIdx.x <= j) {
+        double h = 0;
+#pragma unroll
+        for (int q = 0; q < NW; ++q) h += dred[threadIdx.x * NW + q];
+        H[threadIdx.x + j * LD] = h;
+        hf[threadIdx.x] = (float)h;
+      }
+      __syncthreads();
+      // w -= sum_i h_i v_i, |w|^2
+      double dn = 0;
+      for (int t = threadIdx.x; t < na; t += BS) {
+        float wt = w[t];
+#pragma unroll 8
+        for (int i = 0; i <= j; ++i) wt -= hf[i] * V[(size_t)i * na + t];
+        w[t] = wt;
+        dn += (double)wt * (double)wt;
+      }
+      const double hn = sqrt(wg_sum<BS>(dn, red));
+      const float ih = (hn > 0) ? (float)(1.0 / hn) : 0.0f;
+      for (int t = threadIdx.x; t < na; t += BS) vn[t] = w[t] * ih;
+      if (threadIdx.x == 0) {
+        H[j + 1 + j * LD] = hn;
+        for (int i = 0; i < j; ++i) {
+          const double t0 = H[i + j * LD], t1 = H[i + 1 + j * LD];
+          H[i + j * LD] = cs[i] * t0 + sn[i] * t1;
+          H[i + 1 + j * LD] = -sn[i] * t0 + cs[i] * t1;
+        }
+        const double a0 = H[j + j * LD], a1 = H[j + 1 + j * LD];
+        double c = 1, sv = 0;
+        if (a1 != 0) {
+          const double rr = sqrt(a0 * a0 + a1 * a1);
+          c = a0 / rr;
+          sv = a1 / rr;
+        }
+        cs[j] = c;
+        sn[j] = sv;
+        H[j + j * LD] = c * a0 + sv * a1;
+        H[j + 1 + j * LD] = 0;
+        g[j + 1] = -sv * g[j];
+        g[j] = c * g[j];
+        bc[0] = (hn > 0) ? fabs(g[j + 1]) : 0.0;
+      }
+      __syncthreads();
+      k = j + 1;
+      if (bc[0] <= P.inner_red * rn) break;
+    }
+    inner_total += k;
+    if (threadIdx.x == 0) {
+      for (int i = k - 1; i >= 0; --i) {
+        double sacc = g[i];
+        for (int l = i + 1; l < k; ++l) sacc -= H[i + l * LD] * y[l];
+        const double dd = H[i + i * LD];
+        y[i] = (dd != 0) ? sacc / dd : 0.0;
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < na; t += BS) {
+      double acc = 0;
+#pragma unroll 8
+      for (int i = 0; i < k; ++i) acc += y[i] * (double)V[(size_t)i * na + t];
+      x[t] += (double)dinv[t] * acc;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    B.iters[p] += inner_total;
+    B.outer[p] += outer;
+    B.res[p] = (rn0 > 0) ? rn / rn0 : 0.0;
+    if (!(rn <= P.red * rn0 || rn <= P.tol)) B.fail[p] += 1;
   }
 }
 
