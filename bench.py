@@ -805,7 +805,7 @@ def main():
 
     cd2d, cd2d_cpu = cd2d_config(pm, torch, ctx, not args.no_cpu_baseline)
     batch = None if args.no_batch else batched_sweep(pm, ctx)
-    batch_gmres = None if args.no_batch else batched_sweep(pm, ctx, steps=5, inner_solver="gmres")
+    batch_gmres = None if args.no_batch else batched_sweep(pm, ctx, steps=20, inner_solver="gmres")
     oseen = None if args.no_oseen else oseen_solve(pm, torch, ctx)
     # the reference's cell-Vanka pair (Q2/P1disc) at the same size, with the
     # damping of its own Vanka test (0.8)

@@ -433,7 +433,7 @@ typedef struct {
   double residual_reduction; /* FP64 residual reduction per time step */
   double residual_tolerance; /* FP64 absolute residual per time step */
   int block;                 /* threads per problem: 256, 512, 1024, or 0 = tune on the first run */
-  int max_groups;            /* BiCGStab: workgroups one problem may get (1..32, default 16; 1 = one per problem).
+  int max_groups;            /* workgroups one problem may get (1..32, default 16; 1 = one per problem).
                                 The first run measures each problem's iterations and gives the
                                 costliest problems more workgroups, as long as the whole grid
                                 fits on the device at once */

@@ -531,7 +531,7 @@ int pmx_batch_initialize(pmx_batch* b) {
   });
 }
 
-// Workgroups per problem for the BiCGStab kernel from the measured
+// Workgroups per problem for the inner solver's kernel from the measured
 // iterations of one step: cost = iterations x slices; one more workgroup goes
 // to the problem of the largest cost per workgroup while that exceeds the
 // mean cost per workgroup of the grid, within the number of workgroups
@@ -548,8 +548,9 @@ static void assign_groups(pmx_batch* b, const std::vector<int>& iters) {
   clear_groups(b);
   const int np = b->n_prob;
   const int gmax = std::min(std::max(1, b->prm.max_groups), pmx::kBatchMaxGroups);
-  if (b->prm.inner_solver != PMX_BATCH_INNER_BICGSTAB || gmax <= 1) return;
-  const int capacity = pmx::batch_solve_capacity(b->block);
+  if (gmax <= 1) return;
+  const int capacity =
+      pmx::batch_solve_capacity(b->block, b->prm.inner_solver == PMX_BATCH_INNER_BICGSTAB ? 1 : 0);
   if (capacity <= np) return;
   std::vector<double> cost(np);
   double total = 0;
@@ -585,7 +586,7 @@ static void assign_groups(pmx_batch* b, const std::vector<int>& iters) {
   b->d_wg_rank = upload(wg_rank.data(), wg_rank.size(), st);
   b->d_groups = upload(G.data(), G.size(), st);
   b->d_bar = dmalloc<unsigned int>(np);
-  b->d_part = dmalloc<double>((size_t)np * 2 * pmx::kBatchMaxGroups * 2);
+  b->d_part = dmalloc<double>((size_t)np * 2 * pmx::kBatchMaxGroups * pmx::kBatchPartStride);
   b->d_bar_timeout = dmalloc<int>(1);
   HIP_CHECK(hipMemsetAsync(b->d_bar_timeout, 0, sizeof(int), st));
   HIP_CHECK(hipStreamSynchronize(st));

@@ -29,6 +29,7 @@ namespace pmx {
 namespace {
 
 constexpr int kMaxRestart = 64;
+constexpr unsigned kBatchSpinLimit = 1u << 22;
 
 // sum over the workgroup, returned to every thread (fixed summation order)
 template <int BS>
@@ -94,12 +95,17 @@ __device__ __forceinline__ void wave_rows(const BatchDev& B, int s0, int s1, int
   }
 }
 
-// Restarted GMRES(m) inside the refinement loop (one workgroup per problem).
-// Orthogonalisation by classical Gram-Schmidt: the j+1 projections of the
-// new vector are summed in one sweep over the basis (eight basis vectors per
-// sweep of w) and reduced together, then subtracted in one pass that also
-// sums |w|^2; a workgroup waits twice per step instead of twice per basis
-// vector, and each pass keeps many loads in flight per thread.
+// Restarted GMRES(m) inside the refinement loop.  Orthogonalisation by
+// classical Gram-Schmidt: the j+1 projections of the new vector are summed in
+// one sweep over the basis (eight basis vectors per sweep of w) and reduced
+// together, then subtracted in one pass that also sums |w|^2, so a step needs
+// two reductions instead of two per basis vector.  Every lane owns the rows of
+// its slices for all vector passes.  A problem may get G > 1 workgroups
+// (B.wg_prob), as in the BiCGStab kernel below: each takes a contiguous share
+// of the slices; the SpMVs start behind a barrier of the problem's
+// workgroups, the sums add the workgroup partials in rank order (every
+// workgroup gets the same Hessenberg column and takes the same branches), and
+// thread 0 of every workgroup runs the same Givens and triangular solves.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_batch_solve(BatchDev B, BatchSolveParams P) {
   constexpr int NW = BS / kWave;
@@ -107,13 +113,81 @@ __global__ __launch_bounds__(BS) void k_batch_solve(BatchDev B, BatchSolveParams
   constexpr int U = 4;
   __shared__ double red[NW];
   __shared__ double dred[kMaxRestart * NW];
+  __shared__ double hs[kMaxRestart];
   __shared__ float hf[kMaxRestart];
   __shared__ double H[LD * kMaxRestart];
   __shared__ double cs[kMaxRestart], sn[kMaxRestart], g[LD], y[kMaxRestart];
   __shared__ double bc[1];
-  const int p = B.order[blockIdx.x];
+  int p, rk = 0, G = 1;
+  if (B.wg_prob) {
+    p = B.wg_prob[blockIdx.x];
+    rk = B.wg_rank[blockIdx.x];
+    G = B.groups[p];
+  } else {
+    p = B.order[blockIdx.x];
+  }
   const int off = B.dof_off[p], na = B.n_active[p];
-  const int s0 = B.slice_off[p], s1 = B.slice_off[p + 1];
+  const int ps0 = B.slice_off[p], ps1 = B.slice_off[p + 1];
+  const int s0 = ps0 + (int)((long long)(ps1 - ps0) * rk / G);
+  const int s1 = ps0 + (int)((long long)(ps1 - ps0) * (rk + 1) / G);
+  unsigned epoch = 0;
+  int nred = 0;
+  auto gbar = [&]() {
+    __syncthreads();
+    if (G == 1) return;
+    ++epoch;
+    if (threadIdx.x == 0) {
+      __threadfence();
+      atomicAdd(B.bar + p, 1u);
+      const unsigned target = epoch * (unsigned)G;
+      unsigned spins = 0;
+      while (__hip_atomic_load(B.bar + p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++spins > kBatchSpinLimit) {
+          atomicOr(B.bar_timeout, 1);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      __threadfence();
+    }
+    __syncthreads();
+  };
+  // hs[0..n) = the problem's sums of the workgroup sums in hs[0..n)
+  // (n <= kMaxRestart; two slot buffers alternate as in the BiCGStab kernel)
+  auto sum_groups = [&](int n) {
+    if (G == 1) return;
+    const int buf = (nred++) & 1;
+    double* slots = B.part + ((size_t)p * 2 + buf) * kBatchMaxGroups * kBatchPartStride;
+    // thread 0 publishes all n values: the barrier's release is its own
+    if (threadIdx.x == 0)
+      for (int i = 0; i < n; ++i) slots[rk * kBatchPartStride + i] = hs[i];
+    gbar();
+    if ((int)threadIdx.x < n) {
+      double t = 0;
+      for (int q = 0; q < G; ++q)
+        t += __hip_atomic_load(slots + q * kBatchPartStride + threadIdx.x, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      hs[threadIdx.x] = t;
+    }
+    __syncthreads();
+  };
+  // one sum over the problem
+  auto sum1 = [&](double v) {
+    v = group_sum<kWave>(v);
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0;
+#pragma unroll
+      for (int i = 0; i < NW; ++i) t += red[i];
+      hs[0] = t;
+    }
+    __syncthreads();
+    sum_groups(1);
+    const double r = hs[0];
+    __syncthreads();
+    return r;
+  };
   double* x = B.x + off;
   const double* rhs = B.rhs + off;
   double* r = B.r + off;
@@ -125,7 +199,8 @@ __global__ __launch_bounds__(BS) void k_batch_solve(BatchDev B, BatchSolveParams
   double rn0 = 0, rn = 0;
   int outer = 0, inner_total = 0;
   for (;;) {
-    // FP64 residual r = rhs - S x on the active rows
+    // FP64 residual r = rhs - S x on the active rows (x of all rows)
+    gbar();
     double part = 0;
     wave_spmv<U, double, double>(B, B.S64, s0, s1, wv, NW, lane, [&](int c) { return x[c]; },
                                  [&](int lr, double acc) {
@@ -133,35 +208,34 @@ __global__ __launch_bounds__(BS) void k_batch_solve(BatchDev B, BatchSolveParams
                                    r[lr] = d;
                                    part += d * d;
                                  });
-    rn = sqrt(wg_sum<BS>(part, red));
+    rn = sqrt(sum1(part));
     if (outer == 0) rn0 = rn;
     if (rn <= P.red * rn0 || rn <= P.tol || outer >= P.max_outer) break;
     ++outer;
     // inner GMRES(m) for S d = r, d0 = 0, right Jacobi preconditioning
     const double inv = 1.0 / rn;
-    for (int i = threadIdx.x; i < na; i += BS) V[i] = (float)(r[i] * inv);
+    wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) { V[lr] = (float)(r[lr] * inv); });
     if (threadIdx.x == 0) g[0] = rn;
-    __syncthreads();
     int k = 0;
     for (int j = 0; j < m; ++j) {
       const float* vj = V + (size_t)j * na;
       float* vn = V + (size_t)(j + 1) * na;
-      // w = S D^-1 v_j (the correction is 0 on Dirichlet DOFs)
+      // w = S D^-1 v_j (v_j of all rows; the correction is 0 on Dirichlet DOFs)
+      gbar();
       wave_spmv<U, float, float>(B, B.S32, s0, s1, wv, NW, lane,
                                  [&](int c) { return (c < na) ? dinv[c] * vj[c] : 0.0f; },
                                  [&](int lr, float acc) { w[lr] = acc; });
-      __syncthreads();
-      // h_i = (w, v_i), i <= j
+      // h_i = (w, v_i), i <= j (each lane reads back the rows it wrote)
       for (int i0 = 0; i0 <= j; i0 += 8) {
         double d[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) d[u] = 0.0;
-        for (int t = threadIdx.x; t < na; t += BS) {
-          const double wt = (double)w[t];
+        wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) {
+          const double wt = (double)w[lr];
 #pragma unroll
           for (int u = 0; u < 8; ++u)
-            if (i0 + u <= j) d[u] += wt * (double)V[(size_t)(i0 + u) * na + t];
-        }
+            if (i0 + u <= j) d[u] += wt * (double)V[(size_t)(i0 + u) * na + lr];
+        });
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const double sd = group_sum<kWave>(d[u]);
@@ -173,22 +247,27 @@ __global__ __launch_bounds__(BS) void k_batch_solve(BatchDev B, BatchSolveParams
         double h = 0;
 #pragma unroll
         for (int q = 0; q < NW; ++q) h += dred[threadIdx.x * NW + q];
-        H[threadIdx.x + j * LD] = h;
-        hf[threadIdx.x] = (float)h;
+        hs[threadIdx.x] = h;
+      }
+      __syncthreads();
+      sum_groups(j + 1);
+      if ((int)threadIdx.x <= j) {
+        H[threadIdx.x + j * LD] = hs[threadIdx.x];
+        hf[threadIdx.x] = (float)hs[threadIdx.x];
       }
       __syncthreads();
       // w -= sum_i h_i v_i, |w|^2
       double dn = 0;
-      for (int t = threadIdx.x; t < na; t += BS) {
-        float wt = w[t];
+      wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) {
+        float wt = w[lr];
 #pragma unroll 8
-        for (int i = 0; i <= j; ++i) wt -= hf[i] * V[(size_t)i * na + t];
-        w[t] = wt;
+        for (int i = 0; i <= j; ++i) wt -= hf[i] * V[(size_t)i * na + lr];
+        w[lr] = wt;
         dn += (double)wt * (double)wt;
-      }
-      const double hn = sqrt(wg_sum<BS>(dn, red));
+      });
+      const double hn = sqrt(sum1(dn));
       const float ih = (hn > 0) ? (float)(1.0 / hn) : 0.0f;
-      for (int t = threadIdx.x; t < na; t += BS) vn[t] = w[t] * ih;
+      wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) { vn[lr] = w[lr] * ih; });
       if (threadIdx.x == 0) {
         H[j + 1 + j * LD] = hn;
         for (int i = 0; i < j; ++i) {
@@ -225,15 +304,14 @@ __global__ __launch_bounds__(BS) void k_batch_solve(BatchDev B, BatchSolveParams
       }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < na; t += BS) {
+    wave_rows<U>(B, s0, s1, wv, NW, lane, [&](int lr) {
       double acc = 0;
 #pragma unroll 8
-      for (int i = 0; i < k; ++i) acc += y[i] * (double)V[(size_t)i * na + t];
-      x[t] += (double)dinv[t] * acc;
-    }
-    __syncthreads();
+      for (int i = 0; i < k; ++i) acc += y[i] * (double)V[(size_t)i * na + lr];
+      x[lr] += (double)dinv[lr] * acc;
+    });
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && rk == 0) {
     B.iters[p] += inner_total;
     B.outer[p] += outer;
     B.res[p] = (rn0 > 0) ? rn / rn0 : 0.0;
@@ -256,7 +334,6 @@ __global__ __launch_bounds__(BS) void k_batch_solve(BatchDev B, BatchSolveParams
 // the same branches.  The host launches such problems only when every
 // workgroup of the grid can be resident at once; each barrier wait is
 // bounded and sets B.bar_timeout when it gives up, so the kernel always ends.
-constexpr unsigned kBatchSpinLimit = 1u << 22;
 
 template <int BS>
 __global__ __launch_bounds__(BS) void k_batch_solve_bicgstab(BatchDev B, BatchSolveParams P) {
@@ -535,15 +612,23 @@ __global__ void k_fill_u32(int n, unsigned int* __restrict__ a) {
 
 }  // namespace
 
-int batch_solve_capacity(int block) {
+int batch_solve_capacity(int block, int method) {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   hipError_t e;
-  switch (block) {
-    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve_bicgstab<256>, 256, 0); break;
-    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve_bicgstab<512>, 512, 0); break;
-    default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve_bicgstab<1024>, 1024, 0); break;
+  if (method == 1) {
+    switch (block) {
+      case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve_bicgstab<256>, 256, 0); break;
+      case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve_bicgstab<512>, 512, 0); break;
+      default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve_bicgstab<1024>, 1024, 0); break;
+    }
+  } else {
+    switch (block) {
+      case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve<256>, 256, 0); break;
+      case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve<512>, 512, 0); break;
+      default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_batch_solve<1024>, 1024, 0); break;
+    }
   }
   return e == hipSuccess ? per * cus : 0;
 }
@@ -563,11 +648,17 @@ void batch_solve(const BatchDev& B, const BatchSolveParams& P, int block, hipStr
     }
     return;
   }
-  if (B.wg_prob) throw std::runtime_error("batch: several workgroups per problem need BiCGStab");
+  if (P.m > kMaxRestart) throw std::runtime_error("batch: GMRES restart above 64");
+  const int grid = B.wg_prob ? B.n_wg : B.n_prob;
+  if (B.wg_prob) {
+    if (!B.groups || !B.bar || !B.part || !B.bar_timeout || B.n_wg < B.n_prob)
+      throw std::runtime_error("batch: incomplete workgroup assignment");
+    hipLaunchKernelGGL(k_fill_u32, dim3((B.n_prob + 255) / 256), dim3(256), 0, st, B.n_prob, B.bar);
+  }
   switch (block) {
-    case 256: hipLaunchKernelGGL((k_batch_solve<256>), dim3(B.n_prob), dim3(256), 0, st, B, P); break;
-    case 512: hipLaunchKernelGGL((k_batch_solve<512>), dim3(B.n_prob), dim3(512), 0, st, B, P); break;
-    default: hipLaunchKernelGGL((k_batch_solve<1024>), dim3(B.n_prob), dim3(1024), 0, st, B, P); break;
+    case 256: hipLaunchKernelGGL((k_batch_solve<256>), dim3(grid), dim3(256), 0, st, B, P); break;
+    case 512: hipLaunchKernelGGL((k_batch_solve<512>), dim3(grid), dim3(512), 0, st, B, P); break;
+    default: hipLaunchKernelGGL((k_batch_solve<1024>), dim3(grid), dim3(1024), 0, st, B, P); break;
   }
 }
 

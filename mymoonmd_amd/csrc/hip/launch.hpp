@@ -343,10 +343,10 @@ struct BatchDev {
   int* outer = nullptr;
   double* res = nullptr;
   int* fail = nullptr;
-  // several workgroups per problem (BiCGStab only; nullptr: one each, in
-  // `order`): workgroup -> problem and rank within it, workgroups per
-  // problem, per-problem barrier counters (zeroed before each launch),
-  // partial-sum slots [n_prob][2][kBatchMaxGroups][2], a flag set when a
+  // several workgroups per problem (nullptr: one each, in `order`):
+  // workgroup -> problem and rank within it, workgroups per problem,
+  // per-problem barrier counters (zeroed before each launch), partial-sum
+  // slots [n_prob][2][kBatchMaxGroups][kBatchPartStride], a flag set when a
   // barrier wait gave up
   int n_wg = 0;
   const int* wg_prob = nullptr;
@@ -357,6 +357,7 @@ struct BatchDev {
   int* bar_timeout = nullptr;
 };
 constexpr int kBatchMaxGroups = 32;
+constexpr int kBatchPartStride = 64;   // partial-sum slots per workgroup (GMRES: up to 64 sums at once)
 struct BatchSolveParams {
   int method = 0;          // inner solver: 0 GMRES(m), 1 BiCGStab
   int m = 30;              // GMRES restart (<= 64)
@@ -368,7 +369,7 @@ struct BatchSolveParams {
 };
 void batch_solve(const BatchDev& B, const BatchSolveParams& P, int block, hipStream_t st);
 // workgroups of the BiCGStab batch kernel that can be resident at once
-int batch_solve_capacity(int block);
+int batch_solve_capacity(int block, int method);
 void batch_rhs(int n_slices, const int* slice_prob, const int* dof_off, const int* sptr, const int* srow,
                const int* scol, const double* M64, double tau, const double* f, const double* u,
                double* rhs, hipStream_t st);

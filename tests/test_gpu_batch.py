@@ -123,16 +123,18 @@ def test_batch_graph_replays_and_block_sizes_agree(ctx):
         assert np.max(np.abs(sols[(1024, False)][k] - ref)) <= 1e-8 * np.max(np.abs(ref))
 
 
-def test_batch_workgroups_per_problem(ctx):
-    """BiCGStab with the costliest problems spread over several workgroups
-    (barrier-synchronised, partial sums added in rank order) against one
-    workgroup per problem: same solutions to the solver tolerance, and the
-    graph replays stay bitwise reproducible."""
+@pytest.mark.parametrize("inner", ["bi_cgstab", "gmres"])
+def test_batch_workgroups_per_problem(ctx, inner):
+    """The inner solver (BiCGStab or GMRES) with the costliest problems spread
+    over several workgroups (barrier-synchronised, partial sums added in rank
+    order) against one workgroup per problem: same solutions to the solver
+    tolerance, and the graph replays stay bitwise reproducible."""
+    print("case", inner, flush=True)
     probs = [(64, 1e-4, (1.0, 0.3), 0.0), (8, 1.0, (1.0, 0.0), 0.0), (12, 0.5, (0.0, 1.0), 0.0),
              (16, 0.3, (0.6, 0.8), 0.0)]
     sols, wgs = {}, {}
     for mg in (1, 16, 16):
-        bt = pm.BatchTCD(ctx, probs, tau=TAU, block=256, max_groups=mg)
+        bt = pm.BatchTCD(ctx, probs, tau=TAU, block=256, max_groups=mg, inner_solver=inner)
         bt.initialize()
         rep = bt.run(STEPS)
         assert rep.failed_solves == 0
