@@ -17,7 +17,8 @@
 // Dirichlet DOFs last).  The active rows of every problem are cut into its own
 // SELL slices of 64 rows (local column indices), so one workgroup can run the
 // whole solve of one problem with workgroup barriers only: the solve kernel is
-// one persistent launch with one workgroup per problem (largest first).
+// one persistent launch with one workgroup per problem (largest first), or
+// with several for the costliest problems (each kernel below says how).
 #include <stdexcept>
 
 #include "common.hpp"
