@@ -2193,9 +2193,13 @@ __global__ void k_bgj_make_c(int n, int K, int b, int bp, const double* __restri
 // columns are rewritten by k_bgj_cols), on 64 x 64 tiles with
 // v_mfma_f64_16x16x4f64: wave w owns tile rows 16w .. 16w+15 and four
 // 16-column MFMA tiles; operands straight from the (cache-resident) C and R.
+// Column tiles from tile tj0 on; the tile column starting at skip_col (-1:
+// none) is left out (the look-ahead strip, updated by its own launch).
 __global__ __launch_bounds__(256) void k_bgj_update(int n, int K, int b, int bp, const double* __restrict__ C,
-                                                    const double* __restrict__ R, double* __restrict__ A) {
-  const int ti = blockIdx.y * 64, tj = blockIdx.x * 64;
+                                                    const double* __restrict__ R, double* __restrict__ A, int tj0,
+                                                    int skip_col) {
+  const int ti = blockIdx.y * 64, tj = (blockIdx.x + tj0) * 64;
+  if (tj == skip_col) return;
   if ((ti >= K && ti + 64 <= K + b) || (tj >= K && tj + 64 <= K + b)) return;   // block rows / columns only
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int arow = ti + 16 * w + (lane & 15);
@@ -2306,7 +2310,7 @@ bool dense_invert(int n, double* A, hipStream_t st) {
   double* W = bgj_alloc<double>((size_t)n * nb);
   double* C = bgj_alloc<double>((size_t)n * nb);
   double* R = bgj_alloc<double>((size_t)n * nb);
-  double* P = bgj_alloc<double>((size_t)nb * nb);
+  double* P2 = bgj_alloc<double>((size_t)2 * nb * nb);   // block inverses of even / odd blocks
   // rows up to 8,192 doubles are unpermuted in place through LDS; longer ones
   // through a second matrix
   const bool in_place = (size_t)n * sizeof(double) <= 65536;
@@ -2316,22 +2320,40 @@ bool dense_invert(int n, double* A, hipStream_t st) {
   int* q = bgj_alloc<int>(n);
   int* sing = bgj_alloc<int>(2);   // [0] singular, [1] pivoting needed for the current panel
   int* need = sing + 1;
+  // look-ahead: while the trailing update of block K runs, a second stream
+  // updates the next block's column strip and runs that block's speculative
+  // inverse and check below it (they read nothing else the update writes)
+  hipStream_t s2 = nullptr;
+  hipEvent_t ev_rows = nullptr, ev_next = nullptr;
+  if (hipStreamCreate(&s2) != hipSuccess || hipEventCreateWithFlags(&ev_rows, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ev_next, hipEventDisableTiming) != hipSuccess)
+    throw std::runtime_error("dense inverse: stream or event creation failed");
   auto release = [&]() {
-    (void)hipFree(W); (void)hipFree(C); (void)hipFree(R); (void)hipFree(P); (void)hipFree(U);
+    (void)hipStreamSynchronize(s2);
+    (void)hipEventDestroy(ev_rows); (void)hipEventDestroy(ev_next); (void)hipStreamDestroy(s2);
+    (void)hipFree(W); (void)hipFree(C); (void)hipFree(R); (void)hipFree(P2); (void)hipFree(U);
     if (T) (void)hipFree(T);
     (void)hipFree(piv); (void)hipFree(q); (void)hipFree(sing);
   };
   (void)hipMemsetAsync(sing, 0, 2 * sizeof(int), st);
   const int g1 = (n + 255) / 256;
   const dim3 tiles((n + 63) / 64, (n + 63) / 64);
+  // diagonal pivots first; the pivoting panel and a second block inverse
+  // only when partial pivoting would swap rows in a panel
+  auto speculate = [&](int K, hipStream_t s) {
+    const int b = n - K < nb ? n - K : nb;
+    const int bp = (b + 3) / 4 * 4;
+    double* P = P2 + (size_t)((K / nb) & 1) * nb * nb;
+    hipLaunchKernelGGL(k_bgj_block_inverse, dim3(1), dim3(256), 0, s, n, K, b, bp, A, P, U, need, piv, sing, 1);
+    if (n - K - b > 0)
+      hipLaunchKernelGGL(k_bgj_check_below, dim3((n - K - b + 255) / 256), dim3(256), 0, s, n, K, b, A, U, need);
+  };
+  speculate(0, st);
   for (int K = 0; K < n; K += nb) {
     const int b = n - K < nb ? n - K : nb;
     const int bp = (b + 3) / 4 * 4;
-    // diagonal pivots first; the pivoting panel and a second block inverse
-    // only when partial pivoting would swap rows in this panel
-    hipLaunchKernelGGL(k_bgj_block_inverse, dim3(1), dim3(256), 0, st, n, K, b, bp, A, P, U, need, piv, sing, 1);
-    if (n - K - b > 0)
-      hipLaunchKernelGGL(k_bgj_check_below, dim3((n - K - b + 255) / 256), dim3(256), 0, st, n, K, b, A, U, need);
+    double* P = P2 + (size_t)((K / nb) & 1) * nb * nb;
+    if (K > 0) (void)hipStreamWaitEvent(st, ev_next, 0);   // this block's speculation (second stream)
     if (n - K <= kPanelThreads * kPanelRpt)
       hipLaunchKernelGGL(k_bgj_panel_reg<kPanelRpt>, dim3(1), dim3(kPanelThreads), 0, st, n, K, b, A, W, piv, sing,
                          need);
@@ -2341,7 +2363,16 @@ bool dense_invert(int n, double* A, hipStream_t st) {
     hipLaunchKernelGGL(k_bgj_block_inverse, dim3(1), dim3(256), 0, st, n, K, b, bp, A, P, U, need, piv, sing, 0);
     hipLaunchKernelGGL(k_bgj_make_c, dim3((n + 3) / 4), dim3(256), 0, st, n, K, b, bp, A, C);
     hipLaunchKernelGGL(k_bgj_rows, dim3((n + 63) / 64), dim3(256), 0, st, n, K, b, bp, A, P, R);
-    hipLaunchKernelGGL(k_bgj_update, tiles, dim3(256), 0, st, n, K, b, bp, C, R, A);
+    const int Kn = K + nb;
+    if (Kn < n) {
+      // the next block's column strip, then its speculation, on the second stream
+      (void)hipEventRecord(ev_rows, st);
+      (void)hipStreamWaitEvent(s2, ev_rows, 0);
+      hipLaunchKernelGGL(k_bgj_update, dim3(1, tiles.y), dim3(256), 0, s2, n, K, b, bp, C, R, A, Kn / 64, -1);
+      speculate(Kn, s2);
+      (void)hipEventRecord(ev_next, s2);
+    }
+    hipLaunchKernelGGL(k_bgj_update, tiles, dim3(256), 0, st, n, K, b, bp, C, R, A, 0, Kn < n ? Kn : -1);
     hipLaunchKernelGGL(k_bgj_cols, dim3((n + 63) / 64), dim3(256), 0, st, n, K, b, bp, A, P);
   }
   // the swaps in order make the row permutation p (row i of the permuted
