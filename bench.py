@@ -336,7 +336,7 @@ def batched_sweep(pm, ctx, n_problems=256, steps=20, seed=0x5EED, inner_solver="
             "failed_solves": int(rep.failed_solves), "block": int(rep.block),
             "solve_workgroups": int(inf.n_workgroups),
             "host_setup_s": round(t_setup, 2),
-            "method": "persistent launch, one workgroup per problem (several for the costliest with BiCGStab): "
+            "method": "persistent launch, one workgroup per problem (several for the costliest): "
                       "FP64 residual refinement around FP32 %s + Jacobi" % (
                           "BiCGStab" if inner_solver == "bi_cgstab" else "GMRES(30)")}
 
