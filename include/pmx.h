@@ -320,6 +320,18 @@ typedef struct pmx_nse pmx_nse;
 #define PMX_NSE_BLOCK_B2 2
 #define PMX_NSE_BLOCK_B3 3
 #define PMX_NSE_COMBINED 4
+/* The B^T blocks: structure (PMX_NSE_BLOCK_BT, velocity rows x pressure
+ * columns) and values (B1T..B3T: Bk^T on the active velocity rows, zero on
+ * the Dirichlet rows, which the block operator never applies).  With them
+ * and A an NSTYPE-4 caller (BlockFEMatrix::NSE3D_Type4,
+ * BlockFEMatrix.C:611-660) reads its blocks: for the Galerkin forms with
+ * LAPLACETYPE 0 assembled here A11 = A22 = A33 = A, A12, A13, A21, A23, A31,
+ * A32 = 0 and BkT as above (NSTYPE 4 stores the same values in separate
+ * blocks). */
+#define PMX_NSE_BLOCK_BT 5
+#define PMX_NSE_BLOCK_B1T 5
+#define PMX_NSE_BLOCK_B2T 6
+#define PMX_NSE_BLOCK_B3T 7
 typedef struct {
   int n_cells, n_u, n_u_active, n_p;
   long long nnz_a, nnz_b;

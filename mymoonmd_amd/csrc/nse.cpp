@@ -420,6 +420,7 @@ int pmx_nse_get_structure(const pmx_nse* s, int which, int* nnz, int* host_row_p
     require(s && nnz, "NULL argument");
     const pmx::CSRPattern* P = (which == PMX_NSE_BLOCK_A) ? s->A->pattern.get()
                                : (which == PMX_NSE_BLOCK_B) ? &s->Bp
+                               : (which == PMX_NSE_BLOCK_BT) ? &s->BTp
                                : (which == PMX_NSE_COMBINED) ? &s->Cp : nullptr;
     require(P != nullptr, "unknown block");
     *nnz = (int)P->col.size();
@@ -439,11 +440,20 @@ int pmx_nse_get_values(pmx_ctx* ctx, const pmx_nse* s, int which, double* host_o
       case PMX_NSE_BLOCK_B2: src = s->d_B2; n = s->Bp.col.size(); break;
       case PMX_NSE_BLOCK_B3: src = s->d_B3; n = s->Bp.col.size(); break;
       case PMX_NSE_COMBINED: src = s->d_Cval; n = s->Cp.col.size(); break;
+      case PMX_NSE_BLOCK_B1T: src = s->d_BT1; n = s->BTp.col.size(); break;
+      case PMX_NSE_BLOCK_B2T: src = s->d_BT2; n = s->BTp.col.size(); break;
+      case PMX_NSE_BLOCK_B3T: src = s->d_BT3; n = s->BTp.col.size(); break;
       default: break;
     }
     require(src != nullptr, "unknown block");
     HIP_CHECK(hipMemcpyAsync(host_out, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (which == PMX_NSE_BLOCK_B1T || which == PMX_NSE_BLOCK_B2T || which == PMX_NSE_BLOCK_B3T) {
+      // the B^T blocks act on the active velocity rows only: their Dirichlet
+      // rows are empty, as the rectangular blocks of NSTYPE 4 are assembled
+      for (int r = s->n_u_active; r < s->n_u; ++r)
+        for (int q = s->BTp.row_ptr[r]; q < s->BTp.row_ptr[r + 1]; ++q) host_out[q] = 0.0;
+    }
   });
 }
 

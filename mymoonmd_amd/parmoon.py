@@ -994,7 +994,7 @@ NSE_EXAMPLES = {"u2_p1": 9, "u0_p0": 13}
 # pressure spaces: Q1 (continuous, velocity space 2) and discontinuous P1
 # (D_P1_3D_H_A, velocity space 22), as NavierStokes.C:get_velocity_pressure_orders pairs them
 NSE_PRESSURE = {"q1": 1, "p1disc": -110}
-NSE_BLOCK = {"A": 0, "B": 1, "B1": 1, "B2": 2, "B3": 3, "combined": 4}
+NSE_BLOCK = {"A": 0, "B": 1, "B1": 1, "B2": 2, "B3": 3, "combined": 4, "BT": 5, "B1T": 5, "B2T": 6, "B3T": 7}
 
 
 class NSESystem:
@@ -1056,7 +1056,7 @@ class NSESystem:
     def structure(self, which):
         nnz = c_int()
         call("pmx_nse_get_structure", self.h, NSE_BLOCK[which], ctypes.byref(nnz), None, None)
-        n_rows = {"A": self.n_u, "B": self.n_p, "combined": self.n}[which]
+        n_rows = {"A": self.n_u, "B": self.n_p, "BT": self.n_u, "combined": self.n}[which]
         rp = np.zeros(n_rows + 1, np.int32)
         col = np.zeros(nnz.value, np.int32)
         P = ctypes.POINTER(c_int)
@@ -1065,7 +1065,7 @@ class NSESystem:
         return rp, col
 
     def values(self, which):
-        base = "B" if which in ("B1", "B2", "B3") else which
+        base = "B" if which in ("B1", "B2", "B3") else "BT" if which in ("B1T", "B2T", "B3T") else which
         rp, col = self.structure(base)
         out = np.zeros(len(col))
         call("pmx_nse_get_values", self.ctx.h, self.h, NSE_BLOCK[which], out.ctypes.data_as(ctypes.POINTER(c_double)))

@@ -70,6 +70,28 @@ def test_blocks_operator_and_combined_match_oracle(ctx, case):
     assert unit == o.enclosed
 
 
+def test_nstype4_blocks(ctx, case):
+    """The blocks an NSTYPE-4 caller reads (the reference's Vanka test runs
+    NSTYPE 4): BkT on the transposed pattern holds Bk^T on the active velocity
+    rows and nothing on the Dirichlet rows (NSTYPE 4 keeps them in separate
+    blocks; for these Galerkin forms the values are equal), after assembly
+    with convection."""
+    pressure, example, d, g, o = case
+    print("case", pressure, example, flush=True)
+    rng = np.random.default_rng(9)
+    w = rng.standard_normal(3 * o.n_u)
+    g.assemble(example, 1e-2, dev(w), None)
+    brp, bcol = g.structure("B")
+    trp, tcol = g.structure("BT")
+    for k in range(3):
+        Bk = sp.csr_matrix((g.values("B%d" % (k + 1)), bcol, brp), shape=(o.n_p, o.n_u))
+        BkT = sp.csr_matrix((g.values("B%dT" % (k + 1)), tcol, trp), shape=(o.n_u, o.n_p))
+        want = Bk.T.tocsr().tolil()
+        want[o.n_u_active:, :] = 0.0   # Dirichlet velocity rows: empty
+        assert abs(BkT - want.tocsr()).max() == 0.0
+        assert abs(BkT[:o.n_u_active] - Bk.T.tocsr()[:o.n_u_active]).max() == 0.0
+
+
 def test_interpolant_solves_and_has_zero_errors(ctx, case):
     """The interpolant of the example (pressure through the P1disc nodal
     functionals) equals the oracle's, solves the Oseen system with w = u and
