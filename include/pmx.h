@@ -298,12 +298,14 @@ int pmx_time_prepare_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_
 int pmx_time_reset_system(pmx_ctx* ctx, int scheme, int step, double tau, pmx_matrix* A,
                           const pmx_matrix* M);
 
-/* ------------------------------------------ Oseen / Navier-Stokes (3D, Q2/Q1) */
+/* ------------------- Oseen / Navier-Stokes (3D, Q2/Q1 and Q2/P1disc) */
 /* The saddle-point system of one mesh level with the NSTYPE-1 block layout of
  * BlockFEMatrix::NSE3D_Type1 (src/Matrix/BlockFEMatrix.C:471-720):
  * ( A 0 0 B1^T ; 0 A 0 B2^T ; 0 0 A B3^T ; B1 B2 B3 0 ), vectors [u1|u2|u3|p].
- * Velocity Q2 with Dirichlet conditions on the whole boundary, pressure Q1
- * (continuous, all DOFs active: an enclosed flow with pressure projection). */
+ * Velocity Q2; pressure Q1 (continuous) or discontinuous P1, all pressure
+ * DOFs active.  pmx_nse_create: Dirichlet conditions on the whole boundary
+ * (an enclosed flow with pressure projection); pmx_nse_create_pair chooses
+ * the pressure space and the example whose boundary conditions apply. */
 typedef struct pmx_nse pmx_nse;
 #define PMX_EX_NSE_U2P1 9   /* include/Examples/NSE_3D/test_u_2_p_1.h (Example_NSE3D -3) */
 #define PMX_EX_NSE_U0P0 13  /* include/Examples/NSE_3D/test_u_0_p_0.h (Example_NSE3D -1):
