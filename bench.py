@@ -48,10 +48,10 @@ def parse():
                     help="iteration cap of the omega = 0.8 check (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched TCD2D sweep")
-    ap.add_argument("--no-oseen", action="store_true", help="skip the Oseen Q2/Q1 solve")
+    ap.add_argument("--no-oseen", action="store_true", help="skip the Oseen solves (Q2/Q1 and Q2/P1disc)")
     ap.add_argument("--profile-kernel", default=None,
                     choices=["apply_fp64", "jacobi_fp64", "apply_bf16", "jacobi_bf16", "solver_jacobi_bf16",
-                             "solver_jacobi_fp64", "assembly", "vanka", "batch"],
+                             "solver_jacobi_fp64", "assembly", "vanka", "vanka_p1disc", "batch", "batch_gmres"],
                     help="only run this workload a few times (an operator kernel 5 times, the multigrid's "
                          "finest sweep, the 64^3 assembly, the Oseen Vanka solve, the batched sweep) and "
                          "copy 1 GiB 3 times (the calibration of HBM byte counters), then exit")
@@ -490,8 +490,9 @@ def profile_kernel(pm, torch, ctx, dom, nref, nlev, omega, kernel):
     SELL kernels), solver_jacobi_bf16 / solver_jacobi_fp64 (the multigrid's
     finest-level sweep), assembly (the 64^3 Q2 SUPG assembly, 6 times per
     route: element GEMM FP64 and FP32, quadrature kernel),
-    vanka (Oseen 32^3 Q2/Q1: 6 mixed-precision FGMRES iterations with the
-    cell-Vanka multigrid), batch (the 256-problem TCD2D sweep, 2 steps)."""
+    vanka / vanka_p1disc (Oseen 32^3 Q2/Q1 or Q2/P1disc: 6 mixed-precision
+    FGMRES iterations with the cell-Vanka multigrid), batch / batch_gmres (the
+    256-problem TCD2D sweep with BiCGStab or GMRES, 2 steps)."""
     params = pm.cd_params("exp_sin_3d", disc="supg", eps=EPS, b=B, hexa_quad_rule=2)
     n_rows, nnz, ms = 0, 0, 0.0
     if kernel == "assembly":
@@ -510,11 +511,12 @@ def profile_kernel(pm, torch, ctx, dom, nref, nlev, omega, kernel):
                 ts.append(time.perf_counter() - t0)
             print("assembly", kw, "%.3f ms" % (1e3 * min(ts[1:])), flush=True)
         n_rows, nnz, ms = A.n_rows, A.nnz, 1e3 * min(ts[1:])
-    elif kernel == "vanka":
+    elif kernel in ("vanka", "vanka_p1disc"):
+        p1 = kernel == "vanka_p1disc"
         dom5 = pm.TDomain("Default_UnitCube_Hexa", 5)
         systems = []
         for lev in range(2, 6):
-            g = pm.NSESystem(ctx, dom5, lev)
+            g = pm.NSESystem(ctx, dom5, lev, "p1disc" if p1 else "q1")
             xyz = pm.TFESpace(dom5, lev, 2, None).dof_coords()
             w = torch.as_tensor(fourier_field(xyz).reshape(-1)).cuda()
             r = g.zeros()
@@ -524,7 +526,7 @@ def profile_kernel(pm, torch, ctx, dom, nref, nlev, omega, kernel):
             iterative_solver_type="fgmres", preconditioner="multigrid", gmres_restart=30, residual_tolerance=0.0,
             residual_reduction=1e-30, max_n_iterations=6, multigrid_cycle_type="V",
             multigrid_smoother="cell_vanka_jacobi", multigrid_smoother_coarse="direct_solve",
-            multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2, multigrid_vanka_damp_factor=0.2,
+            multigrid_n_pre_smooth=2, multigrid_n_post_smooth=2, multigrid_vanka_damp_factor=0.8 if p1 else 0.2,
             multigrid_correction_damp_factor=0.8, precision="mixed"))
         sol.update_saddle([q[0] for q in systems])
         g, b = systems[-1]
@@ -534,14 +536,14 @@ def profile_kernel(pm, torch, ctx, dom, nref, nlev, omega, kernel):
         sol.solve(b, x)
         torch.cuda.synchronize()
         n_rows, nnz, ms = g.n, int(g.info.nnz_combined), 1e3 * (time.perf_counter() - t0)
-    elif kernel == "batch":
+    elif kernel in ("batch", "batch_gmres"):
         import numpy as np
         rng = np.random.default_rng(0x5EED)
         ns = rng.integers(32, 257, 256)
         eps = 10.0 ** rng.uniform(-4, 0, 256)
         th = rng.uniform(0, 2 * math.pi, 256)
         bt = pm.BatchTCD(ctx, [(int(n_), float(e), (math.cos(t), math.sin(t)), 0.0) for n_, e, t in zip(ns, eps, th)],
-                         tau=1e-2)
+                         tau=1e-2, inner_solver="gmres" if kernel == "batch_gmres" else "bi_cgstab")
         bt.initialize()
         r = bt.run(2)
         inf = bt.info()
