@@ -568,7 +568,7 @@ def profile_kernel(pm, torch, ctx, dom, nref, nlev, omega, kernel):
         f = torch.zeros(s.n_dof, dtype=torch.float64, device="cuda")
         pm.assemble_cd(ctx, s, params, A, f)
         ms = A.time_kernel(kernel, reps=5)
-    if kernel.startswith("solver_") or kernel not in ("assembly", "vanka", "batch"):
+    if kernel.startswith("solver_") or kernel not in ("assembly", "vanka", "vanka_p1disc", "batch", "batch_gmres"):
         n_rows, nnz = A.n_rows, A.nnz
     src = torch.ones(128 * 1024 * 1024, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
