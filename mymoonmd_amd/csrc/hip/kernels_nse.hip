@@ -275,20 +275,29 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
   for (int a = t; a < NL; a += blockDim.x) dm[a] = map[(size_t)cell * NL + a];
   for (int k = t; k < NL * LDL; k += blockDim.x) M[k] = 0.0;
   __syncthreads();
-  for (int a = t; a < NL; a += blockDim.x) {
-    const int r = dm[a];
-    if (r == kVankaNone) {
-      M[a * LDL + a] = 1.0;
-      continue;
-    }
-    for (int q = rp[r]; q < rp[r + 1]; ++q) {
-      const int c = ci[q];
-      int lo = 0, hi = NL - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (dm[mid] < c) lo = mid + 1; else hi = mid;
+  // gather: each wavefront takes local rows in turn and its 64 lanes stride
+  // the row's CSR entries (64 loads in flight instead of one thread walking a
+  // row of up to a few hundred entries); every entry finds its local column
+  // by binary search in the sorted map
+  {
+    const int lane = t & 63, wv = t >> 6, nwv = blockDim.x >> 6;
+    for (int a = wv; a < NL; a += nwv) {
+      const int r = dm[a];
+      if (r == kVankaNone) {
+        if (lane == 0) M[a * LDL + a] = 1.0;
+        continue;
       }
-      if (dm[lo] == c) M[a * LDL + lo] = cv[q];
+      const int qe = rp[r + 1];
+      for (int q = rp[r] + lane; q < qe; q += 64) {
+        const int c = ci[q];
+        const double v = cv[q];
+        int lo = 0, hi = NL - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (dm[mid] < c) lo = mid + 1; else hi = mid;
+        }
+        if (dm[lo] == c) M[a * LDL + lo] = v;
+      }
     }
   }
   __syncthreads();
