@@ -351,6 +351,7 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
       const double ij = (djj != 0.0) ? 1.0 / djj : 0.0;
       for (int i = t; i < j; i += blockDim.x) {
         double acc = 0.0;
+#pragma unroll 8
         for (int k = i; k < j; ++k) acc += M[i * LDL + k] * tcol[k];
         M[i * LDL + j] = -acc * ij;
       }
@@ -363,6 +364,7 @@ __global__ __launch_bounds__(256) void k_vanka_factor(int n_cells, const int* __
       __syncthreads();
       for (int i = t; i < NL; i += blockDim.x) {
         double acc = (i <= j) ? M[i * LDL + j] : 0.0;
+#pragma unroll 8
         for (int k = j + 1; k < NL; ++k) acc -= M[i * LDL + k] * tcol[k];
         M[i * LDL + j] = acc;
       }
