@@ -34,7 +34,7 @@
 using namespace pmx_internal;
 
 struct pmx_batch {
-  pmx_ctx* ctx = nullptr;
+  CtxRef ctx;
   pmx_batch_params prm{};
   int n_prob = 0;
   std::vector<pmx_batch_problem> probs;

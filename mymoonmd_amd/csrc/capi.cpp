@@ -183,6 +183,16 @@ void neumann_2d(const pmx_space* s, const pmx_cd_params* p, std::vector<double>&
 }  // namespace
 
 // ========================================================================
+void pmx_ctx_release(pmx_ctx* ctx) {
+  if (ctx->refs.fetch_sub(1) != 1) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  dfree(ctx->partial);
+  dfree(ctx->scal);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
 extern "C" {
 
 const char* pmx_last_error(void) { return g_last_error.c_str(); }
@@ -211,12 +221,7 @@ int pmx_ctx_create(int device, pmx_ctx** out) {
 int pmx_ctx_destroy(pmx_ctx* ctx) {
   return guarded([&] {
     if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    dfree(ctx->partial);
-    dfree(ctx->scal);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    pmx_ctx_release(ctx);
   });
 }
 

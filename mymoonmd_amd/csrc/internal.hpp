@@ -80,6 +80,35 @@ struct pmx_ctx {
   hipStream_t stream = nullptr;
   double* partial = nullptr;   // reduction scratch (>= 1024 doubles)
   double* scal = nullptr;      // device scalars
+  // one reference held by the caller (pmx_ctx_destroy drops it) and one by
+  // every object created on the context, so that objects destroyed after
+  // pmx_ctx_destroy (garbage collectors finalise in any order) still find
+  // their stream; the last release frees the context
+  std::atomic<int> refs{1};
+};
+
+// drop one reference; the last one synchronises and frees the context
+void pmx_ctx_release(pmx_ctx* c);
+
+// the context pointer an object keeps: takes a reference on assignment and
+// drops it when the object is destroyed (after the object's own destructor,
+// which may still use the stream)
+struct CtxRef {
+  pmx_ctx* p = nullptr;
+  CtxRef() = default;
+  CtxRef(const CtxRef&) = delete;
+  CtxRef& operator=(const CtxRef&) = delete;
+  CtxRef& operator=(pmx_ctx* c) {
+    if (c) c->refs.fetch_add(1);
+    if (p) pmx_ctx_release(p);
+    p = c;
+    return *this;
+  }
+  ~CtxRef() {
+    if (p) pmx_ctx_release(p);
+  }
+  operator pmx_ctx*() const { return p; }
+  pmx_ctx* operator->() const { return p; }
 };
 
 struct pmx_domain {
@@ -97,7 +126,7 @@ struct RefTable {
 };
 
 struct pmx_space {
-  pmx_ctx* ctx = nullptr;
+  CtxRef ctx;
   const pmx_domain* dom = nullptr;
   pmx::FESpace fs;
   int builtin_bc = 0;
@@ -147,7 +176,7 @@ inline uint64_t pmx_next_uid() {
 
 struct pmx_matrix {
   uint64_t uid = pmx_next_uid();
-  pmx_ctx* ctx = nullptr;
+  CtxRef ctx;
   const pmx_space* space = nullptr;
   std::shared_ptr<pmx::CSRPattern> pattern;
   pmx::DevCSR A;

@@ -9,7 +9,7 @@
 #include "internal.hpp"
 
 struct pmx_nse {
-  pmx_ctx* ctx = nullptr;
+  CtxRef ctx;
   const pmx_domain* dom = nullptr;
   int level = 0;
   pmx::FESpace vs, ps;           // velocity Q2, pressure Q1 (continuous) or discontinuous P1

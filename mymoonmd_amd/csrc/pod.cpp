@@ -166,7 +166,7 @@ void make_directory(const std::string& dir) {
 }  // namespace pmx_internal
 
 struct pmx_snapshots {
-  pmx_ctx* ctx = nullptr;
+  CtxRef ctx;
   FILE* file = nullptr;
   int steps = 5, length = 0, max = 0, count = 0;
   double* d = nullptr;
@@ -178,7 +178,7 @@ struct pmx_snapshots {
 };
 
 struct pmx_pod {
-  pmx_ctx* ctx = nullptr;
+  CtxRef ctx;
   pmx_pod_params p{};
   const pmx_matrix* gram = nullptr;
   int length = 0, m = 0, rank = 0, valid = 0;

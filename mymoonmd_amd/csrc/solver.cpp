@@ -308,7 +308,7 @@ struct Level {
 }  // namespace
 
 struct pmx_solver {
-  pmx_ctx* ctx = nullptr;
+  CtxRef ctx;
   pmx_solver_params p{};
   std::vector<Level> lv;     // coarsest first
   std::vector<NseLevel> nlv; // saddle-point levels, coarsest first (pmx_solver_update_nse)

@@ -379,3 +379,24 @@ def test_host_dirichlet_values(ctx):
     assert np.max(np.abs(a[ab:] - u)) <= 1e-15
     with pytest.raises(ValueError):
         pm.assemble_cd(ctx, s, pm.cd_params("exp_sin_3d", dirichlet=g[:-1], **kw), A1, f1)
+
+
+def test_objects_outlive_their_context(ctx):
+    """pmx_ctx_destroy before the objects created on the context (a garbage
+    collector may finalise them in any order): each object holds a reference
+    to the context, so its destroy still finds the stream; the context is
+    freed with the last of them."""
+    import gc
+    c2 = pm.Context(0)
+    d = pm.TDomain("Default_UnitCube_Hexa", 1)
+    s = pm.TFESpace(d, 1, 2, c2)
+    A = pm.FEMatrix(c2, s)
+    sol = pm.Solver(c2, pm.solver_params(iterative_solver_type="fgmres", preconditioner="jacobi"))
+    g = pm.NSESystem(c2, d, 1)
+    pm._destroy("pmx_ctx_destroy", c2.h)
+    c2.h = None
+    del sol, A, g, s
+    gc.collect()
+    # the library still works on the other context
+    x = torch.ones(8, dtype=torch.float64, device="cuda")
+    assert abs(pm.vector_norm(ctx, x) - 8 ** 0.5) < 1e-14
