@@ -49,6 +49,10 @@ typedef struct pmx_ctx pmx_ctx;
  * pmx_vector_copy; results are complete when a call that returns host values
  * returns, or after pmx_ctx_synchronize. */
 int pmx_ctx_create(int device, pmx_ctx** out);
+/* Drops the caller's reference: every object created on the context holds
+ * one too, so objects may be destroyed before or after the context (in any
+ * order); the context's stream and buffers are freed with the last reference.
+ * The handle must not be used for new calls afterwards. */
 int pmx_ctx_destroy(pmx_ctx* ctx);
 int pmx_ctx_synchronize(pmx_ctx* ctx);
 
